@@ -1,0 +1,2 @@
+"""MI355X-native MPIDForce (multipoles + induced dipoles, PME) evaluation."""
+from .mpidforce import MPIDForce, MPIDForceError  # noqa: F401

@@ -1,0 +1,87 @@
+/*
+ * FP64 CPU oracle for the MPIDForce PME + induced-dipole path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the
+ * checker.  The product path (mpidopenmmplugin_amd/) never links or calls it.
+ *
+ * It restates, in plain double precision, the algorithm of
+ * platforms/reference/src/SimTKReference/MPIDReferenceForce.cpp
+ * (class MPIDReferencePmeForce) together with the parameter flattening of
+ * platforms/reference/src/MPIDReferenceKernels.cpp:84-239 and the validation
+ * of openmmapi/src/MPIDForceImpl.cpp:51-150.  Each function in
+ * mpid_oracle.cpp cites the reference lines it follows.
+ */
+#ifndef MPID_ORACLE_H
+#define MPID_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Input orderings are the MPIDForce API orderings (MPIDForce.h:236-237):
+ * quadrupole XX XY YY XZ YZ ZZ, octopole XXX XXY XYY YYY XXZ XYZ YYZ XZZ YZZ ZZZ.
+ * Covalent lists are CSR: the atoms of list type t (0=1-2, 1=1-3, 2=1-4, 3=1-5)
+ * of atom a are cov_atoms[cov_offsets[4*a+t] .. cov_offsets[4*a+t+1]). */
+typedef struct {
+    int n;
+    const double* charges;      /* [n]    */
+    const double* dipoles;      /* [3n]   */
+    const double* quadrupoles;  /* [6n]   */
+    const double* octopoles;    /* [10n]  */
+    const int* axis_types;      /* [n]    */
+    const int* atom_z;          /* [n]    */
+    const int* atom_x;          /* [n]    */
+    const int* atom_y;          /* [n]    */
+    const double* tholes;       /* [n]    */
+    const double* polarity;     /* [3n]   */
+    const int* cov_offsets;     /* [4n+1] */
+    const int* cov_atoms;
+    int polarization;           /* 0 = Mutual, 1 = Direct */
+    double cutoff;
+    double alpha;
+    int nx, ny, nz;
+    double default_thole;
+    double scale14;
+    int max_iter;
+    double target_eps;
+    double box[9];              /* three box vectors, row-major */
+} oracle_system;
+
+/* Every pointer may be NULL.  Internal orderings: quadrupole QXX QXY QXZ QYY
+ * QYZ QZZ, octopole QXXX QXXY QXXZ QXYY QXYZ QXZZ QYYY QYYZ QYZZ QZZZ,
+ * polarisation tensor XX XY XZ YY YZ ZZ. */
+typedef struct {
+    double* lab_dipoles;     /* [3n]  */
+    double* lab_quads;       /* [6n]  */
+    double* lab_octs;        /* [10n] */
+    double* lab_pol;         /* [6n]  */
+    double* fixed_field;     /* [3n]  permanent field E (before alpha) */
+    double* induced;         /* [3n]  converged induced dipoles */
+    double* phi;             /* [35n] fractional potential derivatives, fixed grid */
+    double* phidp;           /* [35n] fractional potential derivatives, induced grid */
+    double* torques;         /* [3n]  */
+    double* forces;          /* [3n]  overwritten (not accumulated) */
+    double energy_terms[4];  /* real space, recip fixed, recip induced, self */
+    double energy;
+    int iterations;
+    double eps;
+    int converged;
+    char error[256];
+} oracle_out;
+
+/* Solver modes: 0 = DIIS as the reference (MPIDReferenceForce.cpp:1182-1291). */
+int oracle_evaluate(const oracle_system* sys, const double* pos, oracle_out* out);
+
+/* Building blocks, exposed for unit tests. */
+void oracle_bspline(double w, double theta[6][5]);
+void oracle_bspline_moduli(int n, double* moduli);
+int oracle_fft3d(double* data_interleaved, int nx, int ny, int nz, int sign);
+/* Induced field T*mu (real + reciprocal + self) for given lab dipoles, no
+ * solver: the matrix-vector product of the mutual solver. */
+int oracle_induced_field(const oracle_system* sys, const double* pos, const double* mu, double* field_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,169 @@
+"""Writes tests/golden/reference_tests.json: the input systems and golden values
+of the reference's own test suite, transcribed as data.
+
+Source of every number: platforms/reference/tests/TestReferenceMPIDForce.cpp
+  * make_waterbox (lines 109-660), 6-atom dimer coordinates (lines 226-236)
+  * make_methanolbox (lines 662-880), 12-atom coordinates (lines 845-860)
+  * make_charge_square (lines 64-99)
+  * expected energies/forces: testMethanolDimer* (972-1299), testWaterDimer* (1301-1601),
+    test14Scaling* (1603-1700).
+The same values appear in platforms/cuda/tests/TestCudaMPIDForce.cpp.
+Coordinates in the reference are Angstrom and multiplied by NmPerAngstrom (0.1).
+
+Run:  python tests/golden/make_reference_fixtures.py
+"""
+import json
+import os
+
+NM = 0.1
+
+
+def water_dimer():
+    # per-type parameters (TestReferenceMPIDForce.cpp:125-190)
+    types = {
+        "O": dict(charge=-0.51966, dipole=[0.0, 0.0, 0.00755612136146],
+                  quadrupole=[0.000354030721139, 0.0, -0.000390257077096, 0.0, 0.0, 3.62263559571e-05],
+                  octopole=[0, 0, 0, 0, -6.285758282686837e-07, 0, -9.452653225954594e-08, 0, 0, 7.231018665791977e-07],
+                  axis=1, anchors=[1, 2, 0], alphas=[0.000837] * 3, thole=0.39,
+                  cov12=[1, 2], cov13=[]),
+        "H1": dict(charge=0.25983, dipole=[-0.00204209484795, 0.0, -0.00307875299958],
+                   quadrupole=[-3.42848248983e-05, 0.0, -0.000100240875193, -1.89485963908e-06, 0.0, 0.000134525700091],
+                   octopole=[-2.405600937552608e-07, 0, -6.415084018183151e-08, 0, -1.152422607026746e-06, 0,
+                             -2.558537436767218e-06, 3.047102424084479e-07, 0, 3.710960043793964e-06],
+                   axis=0, anchors=[-1, 1, 0], alphas=[0.000496] * 3, thole=0.39,
+                   cov12=[-1], cov13=[1]),
+        "H2": dict(charge=0.25983, dipole=[-0.00204209484795, 0.0, -0.00307875299958],
+                   quadrupole=[-3.42848248983e-05, 0.0, -0.000100240875193, -1.89485963908e-06, 0.0, 0.000134525700091],
+                   octopole=[-2.405600937552608e-07, 0, -6.415084018183151e-08, 0, -1.152422607026746e-06, 0,
+                             -2.558537436767218e-06, 3.047102424084479e-07, 0, 3.710960043793964e-06],
+                   axis=0, anchors=[-2, -1, 0], alphas=[0.000496] * 3, thole=0.39,
+                   cov12=[-2], cov13=[-1]),
+    }
+    coords = [[2.0, 2.0, 2.0], [2.5, 2.0, 3.0], [1.5, 2.0, 3.0],
+              [0.0, 0.0, 0.0], [0.5, 0.0, 1.0], [-0.5, 0.0, 1.0]]
+    order = ["O", "H1", "H2"]
+    atoms = []
+    for a in range(6):
+        t = types[order[a % 3]]
+        z = a + t["anchors"][0]
+        x = a + t["anchors"][1]
+        y = -1 if t["anchors"][2] == 0 else a + t["anchors"][2]
+        atoms.append(dict(charge=t["charge"], dipole=t["dipole"], quadrupole=t["quadrupole"],
+                          octopole=t["octopole"], axisType=t["axis"], atomZ=z, atomX=x, atomY=y,
+                          thole=t["thole"], alphas=t["alphas"],
+                          cov12=[a + c for c in t["cov12"]], cov13=[a + c for c in t["cov13"]], cov14=[]))
+    return dict(atoms=atoms, positions=[[c * NM for c in p] for p in coords])
+
+
+def methanol_dimer():
+    zero3, zero6, zero10 = [0.0] * 3, [0.0] * 6, [0.0] * 10
+    types = {
+        "C1": dict(charge=-0.140, dipole=zero3, quadrupole=zero6, octopole=zero10, axis=4,
+                   anchors=[1, 0, 0], alphas=[0.001] * 3, thole=1.3, cov12=[1, 3, 4, 5], cov13=[2]),
+        "O1": dict(charge=-0.460, dipole=[0.00026405942708641, 0.0, 0.00550661803258754],
+                   quadrupole=[9.383755641232907e-05, 0, -0.0001547997648007625, -1.577493985246555e-06, 0,
+                               6.096220838843343e-05],
+                   octopole=[-3.230426667733178e-08, 0, 3.684859776955582e-08, 0, -2.245492298396793e-07, 0,
+                             7.675967953604524e-07, -4.445541285871346e-09, 0, -5.43047565520773e-07],
+                   axis=0, anchors=[-1, 1, 0], alphas=[0.00100024, 0.00125025, 0.00083350], thole=1.3,
+                   cov12=[-1, 1], cov13=[2, 3, 4]),
+        "HO1": dict(charge=0.360, dipole=zero3, quadrupole=zero6, octopole=zero10, axis=4,
+                    anchors=[-1, 0, 0], alphas=zero3, thole=0.0, cov12=[-1], cov13=[-1]),
+        "H1A": dict(charge=0.080, dipole=zero3, quadrupole=zero6, octopole=zero10, axis=4,
+                    anchors=[-3, 0, 0], alphas=zero3, thole=0.0, cov12=[-3], cov13=[-2, 1, 2]),
+        "H1B": dict(charge=0.080, dipole=zero3, quadrupole=zero6, octopole=zero10, axis=4,
+                    anchors=[-4, 0, 0], alphas=zero3, thole=0.0, cov12=[-4], cov13=[-3, -1, 1]),
+        "H1C": dict(charge=0.080, dipole=zero3, quadrupole=zero6, octopole=zero10, axis=4,
+                    anchors=[-5, 0, 0], alphas=zero3, thole=0.0, cov12=[-5], cov13=[-4, -2, -1]),
+    }
+    coords = [
+        [1.6118739816, -7.7986654421, -9.3388011053], [0.4344388195, -8.6290855266, -9.4591523136],
+        [-0.2932869802, -8.1452383606, -9.0381926002], [1.5101797393, -6.7361319725, -9.6470249020],
+        [1.8800020341, -7.7312323778, -8.2627522535], [2.3828031354, -8.2685172700, -9.9862796759],
+        [-2.3016642008, -3.3801483374, -4.5239842701], [-2.6774345292, -3.8370280231, -3.2318499504],
+        [-1.9568218092, -3.4707595618, -2.6956925837], [-1.4748236015, -3.9573461155, -4.9903514535],
+        [-3.2561339708, -3.3690912389, -5.0924789477], [-1.9925289806, -2.3413378186, -4.2797935219]]
+    order = ["C1", "O1", "HO1", "H1A", "H1B", "H1C"]
+    atoms = []
+    for a in range(12):
+        t = types[order[a % 6]]
+        z = a + t["anchors"][0]
+        x = -1 if t["anchors"][1] == 0 else a + t["anchors"][1]
+        y = -1 if t["anchors"][2] == 0 else a + t["anchors"][2]
+        atoms.append(dict(charge=t["charge"], dipole=t["dipole"], quadrupole=t["quadrupole"],
+                          octopole=t["octopole"], axisType=t["axis"], atomZ=z, atomX=x, atomY=y,
+                          thole=t["thole"], alphas=t["alphas"],
+                          cov12=[a + c for c in t["cov12"]], cov13=[a + c for c in t["cov13"]], cov14=[]))
+    return dict(atoms=atoms, positions=[[c * NM for c in p] for p in coords])
+
+
+def charge_square():
+    pos = [[0.1, 0.0, 0.0], [0.0, 0.0, 0.0], [0.0, 0.1, 0.0], [0.1, 0.1, 0.0]]
+    q = [1.0, 1.0, -1.0, -1.0]
+    cov = [([1], [2], [3]), ([0, 2], [3], []), ([1, 3], [0], []), ([2], [1], [0])]
+    atoms = []
+    for a in range(4):
+        atoms.append(dict(charge=q[a], dipole=[0.0] * 3, quadrupole=[0.0] * 6, octopole=[0.0] * 10,
+                          axisType=5, atomZ=0, atomX=0, atomY=0, thole=0.0, alphas=[0.0] * 3,
+                          cov12=cov[a][0], cov13=cov[a][1], cov14=cov[a][2]))
+    return dict(atoms=atoms, positions=pos)
+
+
+def main():
+    water = water_dimer()
+    methanol = methanol_dimer()
+    square = charge_square()
+    cases = []
+    # methanol PME (TestReferenceMPIDForce.cpp:972-1027, 1081-1137)
+    mpme = dict(system="methanol", method=1, alpha=4.5, grid=64, cutoff=1.2, box=2.461817,
+                default_thole=3.0)
+    cases.append(dict(name="methanol_pme_direct", polarization=1, target_eps=1e-5, **mpme,
+                      energy=100.048119, source="TestReferenceMPIDForce.cpp:1003-1016",
+                      forces=[[0.4407512632, 0.9533272891, 0.2662227116], [2.777987186, 1.840858052, -1.333640734],
+                              [-171.6344629, -55.43492185, 12.32086698], [54.15942246, 41.78995026, -18.66314014],
+                              [67.2048408, 12.09783528, 23.99010209], [46.52456681, -2.469100554, -16.59349719],
+                              [0.9792294678, -1.511490917, -2.188302524], [2.487119928, -3.695492757, -6.600407624],
+                              [12.53120547, -46.68073149, 204.8161], [13.32630506, -12.47148185, -64.76480367],
+                              [-26.51166625, 3.163119144, -46.15867739], [-2.28426176, 62.41964403, -85.09330594]]))
+    cases.append(dict(name="methanol_pme_mutual", polarization=0, target_eps=1e-9, **mpme,
+                      energy=100.0480699, source="TestReferenceMPIDForce.cpp:1113-1126",
+                      forces=[[0.4388490535, 0.9521698628, 0.2671848666], [2.77743403, 1.838052623, -1.324369675],
+                              [-171.6330632, -55.43091584, 12.31209076], [54.15929787, 41.79141719, -18.66382841],
+                              [67.20544012, 12.09872923, 23.98850965], [46.52468443, -2.468565193, -16.59413613],
+                              [0.9812786184, -1.522494372, -2.19205581], [2.482142255, -3.688423769, -6.596946686],
+                              [12.53264302, -46.6801623, 204.8169521], [13.32737364, -12.47126727, -64.76450554],
+                              [-26.51129625, 3.16340949, -46.15822949], [-2.283744668, 62.41956649, -85.09314563]]))
+    # water PME (TestReferenceMPIDForce.cpp:1350-1400, 1449-1501)
+    wpme = dict(system="water", method=1, alpha=3.0, grid=64, cutoff=0.6, box=2.0, default_thole=3.0)
+    cases.append(dict(name="water_pme_direct", polarization=1, target_eps=1e-5, **wpme,
+                      energy=-2.523318862, source="TestReferenceMPIDForce.cpp:1381-1388",
+                      forces=[[-138.9578383, -183.3187212, 31.05996292], [36.78883138, -5.591080652, 7.601999899],
+                              [41.46403045, 118.9693325, 34.16137849], [-116.5222458, -100.9480058, -32.82501978],
+                              [126.6226866, 166.1966239, -17.03839082], [50.6045891, 4.691845173, -22.96008637]]))
+    cases.append(dict(name="water_pme_mutual", polarization=0, target_eps=1e-8, **wpme,
+                      energy=-2.533082539, source="TestReferenceMPIDForce.cpp:1482-1489",
+                      forces=[[-140.0801113, -184.8502938, 30.90206227], [37.10990648, -5.575145037, 7.692659824],
+                              [41.55181662, 119.50422, 34.31895915], [-117.0338412, -101.5516429, -33.10846732],
+                              [127.7947382, 167.7834207, -16.87461724], [50.65754476, 4.689434353, -22.93075376]]))
+    # 1-4 scaling with PME (TestReferenceMPIDForce.cpp:1648-1700): closed form, tolerance 1e-3
+    for s14, e in ((1.0, -1389.35), (0.5, -1389.35 / 2), (0.0, 0.0)):
+        cases.append(dict(name=f"charge_square_pme_s14_{s14}", system="square", method=1, alpha=0.001, grid=64,
+                          cutoff=0.4, box=4.0, default_thole=5.0, polarization=1, target_eps=1e-5,
+                          scale14=s14, energy=e, energy_tol=1e-3, source="TestReferenceMPIDForce.cpp:1648-1700"))
+    # The NoCutoff and Extrapolated goldens (energies only) are recorded for later
+    # rows of the build; they are not exercised by the PME path.
+    other = dict(
+        methanol_nocut_direct=100.1426571, methanol_nocut_mutual=100.1424251,
+        methanol_pme_extrapolated=100.0480906, methanol_nocut_extrapolated=100.1424271,
+        water_nocut_direct=-1.949902453, water_nocut_mutual=-1.952917117,
+        water_pme_extrapolated=-2.527846018, water_nocut_extrapolated=-1.94668563)
+    out = dict(systems=dict(water=water, methanol=methanol, square=square), cases=cases,
+               other_goldens=other, tolerance=1e-4)
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_tests.json")
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1)
+    print("wrote", path)
+
+
+if __name__ == "__main__":
+    main()
