@@ -1,0 +1,217 @@
+"""Throughput benchmark: MPIDForce energy+force evaluations per second on one
+MI355X for the 98,304-atom mutual-polarisation water box (water32768: 32,768
+waters, 9.932 nm box, 128^3 PME grid, 0.9 nm cutoff, octopoles, anisotropic
+polarisability, induced dipoles converged to 1e-6 Debye).
+
+One step = one full evaluation (lab-frame rotation, permanent field, mutual
+induced-dipole solve, real-space and reciprocal energy/forces/torques, torque
+to force) with positions and forces already resident in device memory.  The
+solver starts from mu0 = alpha E every step, as the reference does.
+
+Prints ONE JSON line (rank 0).  Extra fields: the all-FP64 path's rate on the
+same box, mixed-vs-FP64 accuracy, per-kernel device times from HIP events on
+the evaluator's stream, the dominant kernel's roofline and the CPU baseline.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# Algorithmic work per unit, used for the roofline ("Measurement"):
+# floating-point operations per pair inside the cutoff for each pair kernel
+# (static count of the FP32 pair functions), and compulsory HBM bytes per atom
+# or grid point for the grid kernels.
+FLOPS_PER_PAIR = {"electrostatic_pairs": 2650, "fixed_pairs": 420, "induced_pairs": 95}
+MI355X_PEAK = {"hbm_GBps": 8000.0, "fp32_TFLOPs": 157.3, "fp64_TFLOPs": 78.6}
+
+
+def count_pairs(pos, box, cutoff):
+    from scipy.spatial import cKDTree
+    L = np.asarray(box, dtype=np.float64)
+    p = np.mod(pos, L)
+    p = np.minimum(p, np.nextafter(L, 0))
+    t = cKDTree(p, boxsize=L)
+    c = t.count_neighbors(t, cutoff)
+    return int((c - len(p)) // 2)
+
+
+def cpu_baseline(seconds_hint=20.0):
+    """FP64 oracle (single thread) on the 12,288-atom box; scaled to the
+    98,304-atom box by the atom and grid-point ratio (8x)."""
+    from mpidopenmmplugin_amd.waterbox import make_waterbox
+    from oracle import oracle
+    f, pos, box = make_waterbox("water4096")
+    t = time.perf_counter()
+    r = oracle.evaluate(f.flatten(), pos, box)
+    dt = time.perf_counter() - t
+    return {
+        "value": (1.0 / dt) / 8.0,
+        "unit": "evals/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"one full evaluation of the 12,288-atom water box (64^3 grid, mutual to 1e-6 Debye, "
+                   f"{r['iterations']} DIIS iterations) by the FP64 oracle on 1 host thread took {dt:.1f} s; "
+                   "value = 1/(8 x that), the 98,304-atom box having 8x the atoms and grid points"),
+        "sample_seconds": dt,
+    }
+
+
+def run(args):
+    import torch
+    from mpidopenmmplugin_amd.kernel import HipCalcMPIDForceKernel
+    from mpidopenmmplugin_amd.waterbox import CONFIGS, make_waterbox
+
+    t0 = time.perf_counter()
+    force, pos, box = make_waterbox(args.workload)
+    n = force.getNumMultipoles()
+    cfg = CONFIGS[args.workload]
+    dev = torch.device("cuda:0")
+    d_pos = torch.tensor(pos, dtype=torch.float64, device=dev).contiguous()
+    d_f = torch.zeros((n, 3), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+
+    def make(precision, eps=None):
+        k = HipCalcMPIDForceKernel(precision)
+        f = force
+        if eps is not None:
+            f, _, _ = make_waterbox(args.workload, target_eps=eps)
+        k.initialize(n, f)
+        return k
+
+    def timed_rate(k, steps, warmup):
+        for _ in range(warmup):
+            k.execute_device(d_pos.data_ptr(), box, d_f.data_ptr())
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        e = None
+        for _ in range(steps):
+            e = k.execute_device(d_pos.data_ptr(), box, d_f.data_ptr())
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        return dt / steps, e
+
+    km = make(args.precision)
+    ms_step, energy = timed_rate(km, args.steps, args.warmup)
+    st = km.stats()
+    print(f"[bench] {args.workload} {args.precision}: {1000 * ms_step:.2f} ms/step, solver {st}", file=sys.stderr,
+          flush=True)
+
+    # per-kernel device time (HIP events on the evaluator's stream), separate pass
+    km.setProfiling(True)
+    km.profile(reset=True)
+    nprof = max(1, min(3, args.steps))
+    for _ in range(nprof):
+        km.execute_device(d_pos.data_ptr(), box, d_f.data_ptr())
+    prof = km.profile(reset=True)
+    km.setProfiling(False)
+    kernels = {k: {"ms_per_step": v[0] / nprof, "launches_per_step": v[1] / nprof} for k, v in prof.items()}
+
+    result = {}
+    # FP64 path on the same box and its accuracy reference
+    fp64 = None
+    if not args.skip_fp64:
+        kd = make("double")
+        ms64, e64 = timed_rate(kd, max(1, args.steps // 2), 1)
+        fp64 = {"ms_per_step": 1000 * ms64, "evals_per_s": 1.0 / ms64, "stats": kd.stats()}
+        print(f"[bench] {args.workload} double: {1000 * ms64:.2f} ms/step", file=sys.stderr, flush=True)
+        del kd
+        # accuracy of the timed path against the FP64 path converged to 1e-10 Debye
+        kref = make("double", eps=1e-10)
+        e_ref, f_ref = kref.execute(pos, box)
+        mu_ref = kref.getInducedDipoles(pos, box)
+        del kref
+        kchk = make(args.precision)
+        e_m, f_m = kchk.execute(pos, box)
+        mu_m = kchk.getInducedDipoles(pos, box)
+        del kchk
+        result["accuracy_vs_fp64"] = {
+            "energy_rel": abs(e_m - e_ref) / abs(e_ref),
+            "force_rel_rms": float(np.sqrt(((f_m - f_ref) ** 2).sum() / (f_ref ** 2).sum())),
+            "induced_dipole_rel_rms": float(np.sqrt(((mu_m - mu_ref) ** 2).sum() / (mu_ref ** 2).sum())),
+            "reference": "FP64 GPU path converged to 1e-10 Debye",
+        }
+
+    # roofline of the dominant kernel family
+    npairs = count_pairs(pos, box, cfg["cutoff"])
+    dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
+    launches = max(1, kernels[dom]["launches_per_step"])
+    avg_ms = kernels[dom]["ms_per_step"] / launches
+    if dom in FLOPS_PER_PAIR:
+        flops = FLOPS_PER_PAIR[dom] * npairs
+        achieved = flops / (avg_ms * 1e-3) / 1e12
+        peak = MI355X_PEAK["fp32_TFLOPs"] if args.precision == "mixed" else MI355X_PEAK["fp64_TFLOPs"]
+        roof = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                "traffic": None, "kernel": dom, "avg_launch_ms": avg_ms,
+                "note": f"vector FP32 arithmetic (no MFMA); {FLOPS_PER_PAIR[dom]} flops/pair x {npairs} pairs"}
+    else:
+        G = cfg["grid"] ** 3
+        esz = 4 if args.precision == "mixed" else 8
+        bytes_per = {"fft": 2 * 2 * esz * G, "spread": 2 * esz * G + n * 80, "gather": 2 * esz * G + n * 140,
+                     "convolve": 2 * 2 * esz * G}
+        b = bytes_per.get(dom, n * 100)
+        achieved = b / (avg_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": MI355X_PEAK["hbm_GBps"], "unit": "GB/s",
+                "frac": achieved / MI355X_PEAK["hbm_GBps"], "traffic": None, "kernel": dom, "avg_launch_ms": avg_ms}
+
+    out = {
+        "metric": "energy+force evals/s, 98k-atom mutual-pol water box; force RMS err vs FP64",
+        "value": 1.0 / ms_step,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000 * ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 pair/reciprocal arithmetic, fp64 solver reductions" if args.precision == "mixed" else "f64",
+        "data": "synthetic (seeded water box generator)",
+        "config": {"workload": args.workload, "atoms": n, "box_nm": float(box[0]), "pme_grid": cfg["grid"],
+                   "cutoff_nm": cfg["cutoff"], "bspline_order": 6, "polarization": "mutual",
+                   "target_epsilon_debye": 1e-6, "precision": args.precision, "pairs_within_cutoff": npairs},
+        "energy_kJ_mol": energy,
+        "solver": st,
+        "kernels": kernels,
+        "roofline": roof,
+        "setup_s": setup_s,
+    }
+    if fp64:
+        out["fp64_path"] = {"evals_per_s": fp64["evals_per_s"], "ms_per_step": fp64["ms_per_step"]}
+        out["speedup_vs_fp64"] = fp64["ms_per_step"] / (1000 * ms_step)
+    out.update(result)
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "double"])
+    ap.add_argument("--workload", default="water32768", choices=["water216", "water4096", "water32768"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--skip-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build evaluates one system on one GPU; run with --gpus 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    rank = int(os.environ.get("RANK", "0"))
+    out = run(args)
+    if rank == 0:
+        print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,151 @@
+/*
+ * C ABI of the MI355X (gfx950) MPIDForce PME + induced-dipole evaluator.
+ *
+ * This is the drop-in boundary that replaces the body of the reference's
+ * platform kernel, class CalcMPIDForceKernel
+ * (openmmapi/include/openmm/mpidKernels.h:50-104).  Each entry point names the
+ * virtual method it stands in for.  Plain pointers and sizes only; all device
+ * memory is owned by the context.  Not thread safe per context; distinct
+ * contexts may be used from distinct threads.
+ *
+ * Orderings are those of the MPIDForce API (openmmapi/include/openmm/MPIDForce.h:236-237):
+ *   dipole      X Y Z
+ *   quadrupole  XX XY YY XZ YZ ZZ
+ *   octopole    XXX XXY XYY YYY XXZ XYZ YYZ XZZ YZZ ZZZ
+ * Units: nm, e, kJ/mol.  Positions are [n][3] doubles; box is three box
+ * vectors, row-major [3][3] (only rectangular boxes are supported).
+ */
+#ifndef MPID_HIP_H
+#define MPID_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mpid_context mpid_context;
+
+/* status codes (0 = success) */
+enum {
+    MPID_OK = 0,
+    MPID_ERR_INVALID = 1,        /* invalid parameter (OpenMMException in the reference) */
+    MPID_ERR_NOT_CONVERGED = 2,  /* mutual induced dipoles did not converge */
+    MPID_ERR_HIP = 3,            /* HIP runtime failure */
+    MPID_ERR_UNSUPPORTED = 4     /* feature outside this build (NoCutoff, triclinic box, ...) */
+};
+
+enum { MPID_NONBONDED_NOCUTOFF = 0, MPID_NONBONDED_PME = 1 };
+enum { MPID_POLARIZATION_MUTUAL = 0, MPID_POLARIZATION_DIRECT = 1, MPID_POLARIZATION_EXTRAPOLATED = 2 };
+/* MIXED: FP32 pair and reciprocal arithmetic, 64-bit fixed-point accumulation,
+ * FP64 solver reductions and FP64 iterative refinement of the induced dipoles.
+ * DOUBLE: the same kernels in FP64 throughout. */
+enum { MPID_PRECISION_MIXED = 0, MPID_PRECISION_DOUBLE = 1 };
+/* Option flags.
+ * CONSERVATIVE_TORQUES: add the half of the induced-induced torque about the
+ * pair axis that the reference omits (MPIDReferenceForce.cpp:4854-4855).  Off
+ * by default, which reproduces the reference; with it the forces on
+ * anisotropically polarizable sites are exact gradients of the energy. */
+enum { MPID_FLAG_CONSERVATIVE_TORQUES = 1 };
+
+/* Global settings of the force (MPIDForce.h:118-470 setters). */
+typedef struct {
+    int nonbonded_method;   /* MPID_NONBONDED_PME */
+    int polarization;       /* MPID_POLARIZATION_MUTUAL or _DIRECT */
+    double cutoff;          /* nm */
+    double ewald_alpha;     /* nm^-1, must be > 0 */
+    int grid[3];            /* PME grid, sizes of the form 2^a 3^b 5^c */
+    double default_thole;   /* MPIDForce::getDefaultTholeWidth */
+    double scale14;         /* MPIDForce::get14ScaleFactor */
+    int max_iterations;     /* MPIDForce::getMutualInducedMaxIterations */
+    double target_epsilon;  /* MPIDForce::getMutualInducedTargetEpsilon (RMS Debye) */
+    int precision;          /* MPID_PRECISION_* */
+    int flags;              /* MPID_FLAG_* bits, 0 for the reference behaviour */
+} mpid_options;
+
+/* Per-particle parameters (MPIDForce::addMultipole, setCovalentMap).
+ * Covalent lists in CSR form: the atoms in list t (0 = Covalent12, 1 = 13,
+ * 2 = 14, 3 = 15) of atom a are
+ * covalent_atoms[covalent_offsets[4a+t] .. covalent_offsets[4a+t+1]). */
+typedef struct {
+    int n;
+    const double* charges;       /* [n]   */
+    const double* dipoles;       /* [3n]  molecular frame */
+    const double* quadrupoles;   /* [6n]  molecular frame */
+    const double* octopoles;     /* [10n] molecular frame */
+    const int* axis_types;       /* [n]   MPIDForce::MultipoleAxisTypes */
+    const int* atom_z;           /* [n]   */
+    const int* atom_x;           /* [n]   */
+    const int* atom_y;           /* [n]   */
+    const double* tholes;        /* [n]   */
+    const double* polarity;      /* [3n]  molecular-frame xx yy zz */
+    const int* covalent_offsets; /* [4n+1] */
+    const int* covalent_atoms;
+} mpid_parameters;
+
+typedef struct {
+    int iterations;          /* solver iterations of the last execute */
+    double epsilon;          /* final RMS induced-dipole residual, Debye */
+    int converged;
+    int refinements;         /* FP64 refinement passes (mixed precision) */
+    int matvecs;             /* matrix-vector products */
+    int ntiles;              /* 64x64 block pairs within the cutoff */
+    double energy_terms[3];  /* real space; reciprocal + self; variational solver correction */
+} mpid_stats;
+
+/* Per-kernel-family device time of the executes since profiling was enabled
+ * (HIP events on the context's stream). */
+enum {
+    MPID_PROF_SORT = 0, MPID_PROF_ROTATE, MPID_PROF_TILES, MPID_PROF_SPREAD, MPID_PROF_FFT, MPID_PROF_CONVOLVE,
+    MPID_PROF_GATHER, MPID_PROF_FIXED_PAIRS, MPID_PROF_INDUCED_PAIRS, MPID_PROF_ELECTROSTATIC_PAIRS,
+    MPID_PROF_RECIP_ATOMS, MPID_PROF_SOLVER, MPID_PROF_OTHER, MPID_PROF_COUNT
+};
+
+/* CalcMPIDForceKernel::initialize (mpidKernels.h:67).  Validates as
+ * MPIDForceImpl::initialize (openmmapi/src/MPIDForceImpl.cpp:51-150). */
+int mpid_create(const mpid_parameters* params, const mpid_options* opts, mpid_context** out);
+
+/* CalcMPIDForceKernel::copyParametersToContext (mpidKernels.h:93).  The
+ * particle count must not change. */
+int mpid_update_parameters(mpid_context* ctx, const mpid_parameters* params);
+
+/* CalcMPIDForceKernel::execute (mpidKernels.h:77): host positions; forces are
+ * ADDED into forces_inout ([n][3], may be NULL); energy (kJ/mol) written to
+ * *energy_out (may be NULL). */
+int mpid_execute(mpid_context* ctx, const double* positions, const double* box, int include_forces,
+                 int include_energy, double* forces_inout, double* energy_out);
+
+/* Same, with positions and forces in device memory (HIP pointers on the
+ * context's device).  The call is synchronous. */
+int mpid_execute_device(mpid_context* ctx, const double* d_positions, const double* box, double* d_forces_inout,
+                        double* energy_out);
+
+/* CalcMPIDForceKernel::getInducedDipoles / getLabFramePermanentDipoles /
+ * getTotalDipoles (mpidKernels.h:79-81): [n][3] e nm at the given positions
+ * (re-evaluated when they differ from the last execute). */
+int mpid_get_induced_dipoles(mpid_context* ctx, const double* positions, const double* box, double* out);
+int mpid_get_lab_frame_permanent_dipoles(mpid_context* ctx, const double* positions, const double* box, double* out);
+int mpid_get_total_dipoles(mpid_context* ctx, const double* positions, const double* box, double* out);
+
+/* CalcMPIDForceKernel::getPMEParameters (mpidKernels.h:103). */
+int mpid_get_pme_parameters(const mpid_context* ctx, double* alpha, int* nx, int* ny, int* nz);
+
+/* Inspection of intermediate results for parity tests:
+ * permanent field E (before the polarisability), [n][3]; and the field
+ * T*mu (real space + reciprocal + self) of given induced dipoles, [n][3]. */
+int mpid_compute_fixed_field(mpid_context* ctx, const double* positions, const double* box, double* field_out);
+int mpid_compute_induced_field(mpid_context* ctx, const double* positions, const double* box, const double* mu,
+                               double* field_out);
+
+int mpid_get_stats(const mpid_context* ctx, mpid_stats* stats);
+int mpid_set_profiling(mpid_context* ctx, int enable);
+/* times_ms[MPID_PROF_COUNT], counts[MPID_PROF_COUNT] (launches); resets them when reset != 0 */
+int mpid_get_profile(mpid_context* ctx, double* times_ms, int* counts, int reset);
+
+const char* mpid_last_error(const mpid_context* ctx);
+/* error of the last failed mpid_create on this thread */
+const char* mpid_create_error(void);
+void mpid_destroy(mpid_context* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,151 @@
+// Host-side launch interface of the MPIDForce gfx950 kernels.  Every launcher
+// enqueues on the given stream and returns immediately.
+#pragma once
+
+#include "common.h"
+
+namespace mpid {
+
+template <class R> struct R4T;
+template <> struct R4T<float> { typedef float4 type; };
+template <> struct R4T<double> { typedef double4 type; };
+template <class R> using R4 = typename R4T<R>::type;
+
+template <class R> struct R2T;
+template <> struct R2T<float> { typedef float2 type; };
+template <> struct R2T<double> { typedef double2 type; };
+template <class R> using R2 = typename R2T<R>::type;
+
+// Parameters of the real-space pair kernels.
+template <class R> struct PairParams {
+    Box<R> box;
+    int n;
+    R rc2;
+    R alpha;
+    R aln[5];          // (2 alpha^2)^k / (alpha sqrt(pi)), k = 1..4 (index 0 unused)
+    R default_thole;
+    R scale14;
+    int mutual;        // 1 when the induced-induced terms enter the forces
+    int fix_torque;    // 1: energy-conserving induced-induced torques (MPID_FLAG_CONSERVATIVE_TORQUES)
+};
+
+// Precision-specific per-atom inputs of the pair kernels (sorted order).
+template <class R> struct PairAtoms {
+    const R4<R>* pos;      // x, y, z, q
+    const R* lab;          // [n*20] lab-frame multipoles
+    const R2<R>* td;       // thole, damping factor
+    const int* aniso;      // [n]
+    const int* partners;   // [n*kMaxPartners] (sorted index << 2 | class)
+    const int* npart;      // [n]
+};
+
+// ---- atoms / frames / solver vectors (kernels_atoms.hip) ---------------------
+void launch_morton_keys(hipStream_t s, int n, const double* pos, Box<double> box, unsigned* keys, int* vals);
+void launch_invert_perm(hipStream_t s, int n, const int* perm, int* inv);
+void launch_gather_sorted(hipStream_t s, int n, const int* perm, const int* inv, const double* pos, Box<double> box,
+                          const double* mol, const int4* axes, const double* thole, const double* pol,
+                          const int* partners, const int* npart, double4* spos, double* smol, int4* saxes,
+                          double* sthole, double* spol, int* spartners, int* snpart);
+void launch_rotate(hipStream_t s, int n, Box<double> box, const double4* pos, const int4* axes, const double* mol,
+                   const double* pol, double* lab, double* labpol, double* sqrtpol, double* damp, int* aniso);
+template <class R>
+void launch_convert(hipStream_t s, int n, const double4* pos, const double* lab, const double* thole,
+                    const double* damp, R4<R>* rpos, R* rlab, R2<R>* rtd);
+void launch_torque_to_force(hipStream_t s, int n, Box<double> box, const double4* pos, const int4* axes,
+                            const acc_t* torque, acc_t* force);
+void launch_output_forces(hipStream_t s, int n, const int* perm, const acc_t* force, double* out, int accumulate);
+void launch_unpermute3(hipStream_t s, int n, const int* perm, const double* in, double* out);
+void launch_zero_acc(hipStream_t s, acc_t* p, size_t count);
+size_t sort_temp_bytes(int n);
+int launch_sort_pairs(hipStream_t s, void* temp, size_t bytes, unsigned* kin, unsigned* kout, int* vin, int* vout, int n);
+// out = acc + erecip + selfc * mu (field of the induced dipoles, double)
+template <class R>
+void launch_field_sum(hipStream_t s, int n, const acc_t* acc, const R* erec, const R* mu, double selfc, double* out);
+template <class R> void launch_fill(hipStream_t s, R* p, size_t count, R v);
+
+// fixed field: E0 = recip (from phi) + self + real-space accumulator
+template <class R>
+void launch_fixed_field_finish(hipStream_t s, int n, const R* phi, const double* lab, const acc_t* acc,
+                               double3 gscale, double selfc, double* E0);
+// mu0 = alpha_lab E0 ; y0 = S E0 ; b = S E0
+void launch_solver_init(hipStream_t s, int n, const double* sqrtpol, const double* E0, double* b, double* y);
+// mu = S y (double and precision copies)
+template <class R> void launch_apply_S(hipStream_t s, int n, const double* sqrtpol, const double* y, R* mu, double* mud);
+// Ap = p - S (acc + Erecip + selfc mu)
+template <class R>
+void launch_matvec_finish(hipStream_t s, int n, const double* sqrtpol, const acc_t* acc, const R* erecip,
+                          const R* mu, double selfc, const double* p, double* Ap);
+// deterministic FP64 reductions
+void launch_dot(hipStream_t s, int m, const double* a, const double* b, double* partial, double* result);
+void launch_cg_step(hipStream_t s, int n, const double* sqrtpol, const double* scal /*rr, pAp*/, const double* p,
+                    const double* Ap, double* x, double* r, double* partial, double* result /*rr_new, ss*/);
+void launch_cg_direction(hipStream_t s, int m, const double* rr_old, const double* rr_new, const double* r,
+                         double* p);
+void launch_residual_norm(hipStream_t s, int n, const double* sqrtpol, const double* r, double* partial,
+                          double* result);
+void launch_axpy(hipStream_t s, int m, double a, const double* x, double* y);
+void launch_sub(hipStream_t s, int m, const double* a, const double* b, double* out);
+// reciprocal-space energy, forces and torques, self terms (per atom)
+template <class R>
+void launch_recip_atoms(hipStream_t s, int n, int mutual, const double* phi, const R* phidp, const double* lab,
+                        const double* mu, const int* aniso, double3 gscale, double alpha, acc_t* force,
+                        acc_t* torque, acc_t* energy);
+
+// ---- PME (kernels_pme.hip) ----------------------------------------------------
+template <class R> struct PmeGeom {
+    int n[3];
+    R gscale[3];      // n_k / L_k
+    Box<R> box;
+};
+// Morton-aligned spreading tiles: 2^(3m) tiles of T_k grid points, windows of W_k points
+struct TileGeom {
+    int m;               // Morton level
+    int tiles_per_axis;  // 2^m
+    int ntiles;          // 2^(3m)
+    int T[3];
+    int W[3];
+};
+void launch_tile_ranges(hipStream_t s, int n, const unsigned* keys_sorted, int shift, int* tstart, int* tend);
+size_t spread_tiled_lds_bytes(const TileGeom& tg, int double_precision);
+template <class R>
+void launch_spread_fixed_tiled(hipStream_t s, const R4<R>* pos, const R* lab, PmeGeom<R> g, TileGeom tg,
+                               const int* tstart, const int* tend, R* scratch, R2<R>* grid);
+template <class R>
+void launch_spread_induced_tiled(hipStream_t s, const R4<R>* pos, const R* mu, PmeGeom<R> g, TileGeom tg,
+                                 const int* tstart, const int* tend, R* scratch, R2<R>* grid);
+template <class R> void launch_fold(hipStream_t s, PmeGeom<R> g, TileGeom tg, const R* scratch, R2<R>* grid);
+template <class R>
+void launch_spread_fixed(hipStream_t s, int n, const R4<R>* pos, const R* lab, PmeGeom<R> g, R2<R>* grid);
+template <class R>
+void launch_spread_induced(hipStream_t s, int n, const R4<R>* pos, const R* mu, PmeGeom<R> g, R2<R>* grid);
+template <class R>
+void launch_convolve(hipStream_t s, PmeGeom<R> g, const R* moduli /*nx+ny+nz*/, R alpha, R2<R>* grid);
+template <class R>
+void launch_gather_phi(hipStream_t s, int n, const R4<R>* pos, PmeGeom<R> g, const R2<R>* grid, R* phi);
+template <class R>
+void launch_gather_field(hipStream_t s, int n, const R4<R>* pos, PmeGeom<R> g, const R2<R>* grid, R* field);
+// in-place unnormalised 3D complex FFT, [nx][ny][nz] layout, z fastest
+template <class R> struct FftPlan {
+    int n[3];
+    const R2<R>* tw[3];  // per-axis twiddles exp(-2 pi i k / n), k < n
+    int radix[3][16];
+    int nradix[3];
+};
+template <class R> void launch_fft3d(hipStream_t s, const FftPlan<R>& plan, R2<R>* grid, int sign);
+
+// ---- real space (kernels_pairs.hip) -------------------------------------------
+template <class R>
+void launch_block_bounds(hipStream_t s, int n, const R4<R>* pos, float4* bmin, float4* bmax);
+void launch_build_tiles(hipStream_t s, int nblocks, const float4* bmin, const float4* bmax, Box<float> box,
+                        float cutoff, int2* tiles, int* count, int capacity);
+template <class R>
+void launch_fixed_field_pairs(hipStream_t s, int ntiles, const int2* tiles, PairAtoms<R> a, PairParams<R> p,
+                              acc_t* field);
+template <class R>
+void launch_induced_field_pairs(hipStream_t s, int ntiles, const int2* tiles, PairAtoms<R> a, const R* mu,
+                                PairParams<R> p, acc_t* field);
+template <class R>
+void launch_electrostatics_pairs(hipStream_t s, int ntiles, const int2* tiles, PairAtoms<R> a, const R* mu,
+                                 PairParams<R> p, acc_t* force, acc_t* torque, acc_t* energy);
+
+}  // namespace mpid

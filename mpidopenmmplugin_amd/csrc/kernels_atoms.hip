@@ -1,0 +1,859 @@
+// Per-atom kernels of the MPIDForce path on gfx950: spatial sort keys, the
+// gather into sorted order, lab-frame rotation of multipoles and
+// polarisability tensors, solver vector operations with deterministic FP64
+// reductions, reciprocal-space energies/forces/torques, self terms and the
+// torque-to-force chain rule.
+//
+// Follows platforms/reference/src/SimTKReference/MPIDReferenceForce.cpp:
+// loadParticleData :282-350, checkChiral :357-397, applyRotationMatrix
+// :399-648, recordFixedMultipoleField :4026-4037, the PME self field :2940-2944,
+// computeReciprocalSpaceFixedMultipoleForceAndEnergy :3739-3866,
+// computeReciprocalSpaceInducedDipoleForceAndEnergy :3871-4024,
+// calculatePmeSelfEnergy :4283-4318, calculatePmeSelfTorque :4320-4333,
+// mapTorqueToForceForParticle :1895-2110.
+#include "engine.h"
+
+#include <cstring>
+#include <rocprim/rocprim.hpp>
+
+namespace mpid {
+
+typedef V3<double> D3;
+
+__device__ __forceinline__ D3 d3(double4 p) { return mk(p.x, p.y, p.z); }
+__device__ __forceinline__ double dnorm(D3& v) {
+    double n = sqrt(dot(v, v));
+    if (n > 0.0) v = v * (1.0 / n);
+    return n;
+}
+
+// ---------------------------------------------------------------------------
+// sorting
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned spread_bits(unsigned v) {
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__global__ void k_morton_keys(int n, const double* __restrict__ pos, Box<double> box, unsigned* keys, int* vals) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned c[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        double f = pos[3 * (size_t)i + k] * box.inv[k];
+        f -= floor(f);
+        int q = (int)(f * 1024.0);
+        c[k] = (unsigned)min(max(q, 0), 1023);
+    }
+    keys[i] = (spread_bits(c[0]) << 2) | (spread_bits(c[1]) << 1) | spread_bits(c[2]);
+    vals[i] = i;
+}
+
+__global__ void k_invert_perm(int n, const int* __restrict__ perm, int* inv) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) inv[perm[s]] = s;
+}
+
+__global__ void k_gather_sorted(int n, const int* __restrict__ perm, const int* __restrict__ inv,
+                                const double* __restrict__ pos, Box<double> box, const double* __restrict__ mol,
+                                const int4* __restrict__ axes, const double* __restrict__ thole,
+                                const double* __restrict__ pol, const int* __restrict__ partners,
+                                const int* __restrict__ npart, double4* spos, double* smol, int4* saxes,
+                                double* sthole, double* spol, int* spartners, int* snpart) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int a = perm[s];
+    double x[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        double v = pos[3 * (size_t)a + k];
+        v -= box.L[k] * floor(v * box.inv[k]);
+        x[k] = v;
+    }
+    spos[s] = make_double4(x[0], x[1], x[2], mol[(size_t)a * kMP]);
+    for (int k = 0; k < kMP; k++) smol[(size_t)s * kMP + k] = mol[(size_t)a * kMP + k];
+    int4 ax = axes[a];
+    saxes[s] = make_int4(ax.x >= 0 ? inv[ax.x] : -1, ax.y >= 0 ? inv[ax.y] : -1, ax.z >= 0 ? inv[ax.z] : -1, ax.w);
+    sthole[s] = thole[a];
+    for (int k = 0; k < 3; k++) spol[3 * (size_t)s + k] = pol[3 * (size_t)a + k];
+    const int np = npart[a];
+    snpart[s] = np;
+    for (int k = 0; k < np; k++) {
+        int e = partners[(size_t)a * kMaxPartners + k];
+        spartners[(size_t)s * kMaxPartners + k] = (inv[e >> 2] << 2) | (e & 3);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// lab frame
+// ---------------------------------------------------------------------------
+// Molecular frame axes of atom s (rows x, y, z in lab components); identity
+// for NoAxisType.  ZOnly picks its x axis from the lab axes.
+__device__ void frame_axes(int s, const double4* __restrict__ pos, int4 ax, const Box<double>& box, D3 e[3]) {
+    const int t = ax.w;
+    if (t == 5 || ax.x < 0) {
+        e[0] = mk(1.0, 0.0, 0.0);
+        e[1] = mk(0.0, 1.0, 0.0);
+        e[2] = mk(0.0, 0.0, 1.0);
+        return;
+    }
+    const D3 pi = d3(pos[s]);
+    D3 vz = min_image(d3(pos[ax.x]) - pi, box);
+    dnorm(vz);
+    D3 vx;
+    if (t == 4) {
+        vx = fabs(vz.x) < 0.866 ? mk(1.0, 0.0, 0.0) : mk(0.0, 1.0, 0.0);
+    } else {
+        vx = min_image(d3(pos[ax.y]) - pi, box);
+        if (t == 1) {
+            dnorm(vx);
+            vz = vz + vx;
+            dnorm(vz);
+        } else if (t == 2) {
+            dnorm(vx);
+            D3 vy = min_image(d3(pos[ax.z]) - pi, box);
+            dnorm(vy);
+            vx = vx + vy;
+            dnorm(vx);
+        } else if (t == 3) {
+            dnorm(vx);
+            D3 vy = min_image(d3(pos[ax.z]) - pi, box);
+            dnorm(vy);
+            vz = vz + vx + vy;
+            dnorm(vz);
+        }
+    }
+    vx = vx - vz * dot(vz, vx);
+    dnorm(vx);
+    e[0] = vx;
+    e[1] = cross(vz, vx);
+    e[2] = vz;
+}
+
+__device__ __forceinline__ int i2(int a, int b) {
+    // XX XY XZ YY YZ ZZ
+    return a <= b ? (a == 0 ? b : (a == 1 ? 2 + b : 5)) : (b == 0 ? a : (b == 1 ? 2 + a : 5));
+}
+__device__ __forceinline__ int i3(int a, int b, int c) {
+    // sort
+    int lo = min(a, min(b, c)), hi = max(a, max(b, c)), mid = a + b + c - lo - hi;
+    const int t[3][3][3] = {{{0, 1, 2}, {1, 3, 4}, {2, 4, 5}}, {{1, 3, 4}, {3, 6, 7}, {4, 7, 8}},
+                            {{2, 4, 5}, {4, 7, 8}, {5, 8, 9}}};
+    return t[lo][mid][hi];
+}
+
+// axes: x = z-axis atom, y = x-axis atom, z = y-axis atom, w = axis type (all sorted indices)
+__global__ __launch_bounds__(128) void k_rotate(int n, Box<double> box, const double4* __restrict__ pos,
+                                                 const int4* __restrict__ axes, const double* __restrict__ mol,
+                                                 const double* __restrict__ pol, double* lab, double* labpol,
+                                                 double* sqrtpol, double* damp, int* aniso) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int4 ax = axes[s];
+    double m[kMP];
+    for (int k = 0; k < kMP; k++) m[k] = mol[(size_t)s * kMP + k];
+    // chirality (ZThenX with a y atom): flip y-odd dipole/quadrupole components
+    if (ax.w == 0 && ax.z >= 0) {
+        const D3 py = d3(pos[ax.z]);
+        D3 ad = min_image(d3(pos[s]) - py, box), bd = min_image(d3(pos[ax.x]) - py, box),
+           cd = min_image(d3(pos[ax.y]) - py, box);
+        if (dot(cross(bd, cd), ad) < 0.0) {
+            m[2] = -m[2];
+            m[5] = -m[5];
+            m[8] = -m[8];
+        }
+    }
+    D3 e[3];
+    frame_axes(s, pos, ax, box, e);
+    double E[3][3];  // E[k][a] = component a of molecular axis k
+    for (int k = 0; k < 3; k++) {
+        E[k][0] = e[k].x;
+        E[k][1] = e[k].y;
+        E[k][2] = e[k].z;
+    }
+    double* o = lab + (size_t)s * kMP;
+    o[0] = m[0];
+    for (int a = 0; a < 3; a++) {
+        double v = 0;
+        for (int k = 0; k < 3; k++) v += E[k][a] * m[1 + k];
+        o[1 + a] = v;
+    }
+    for (int a = 0; a < 3; a++)
+        for (int b = a; b < 3; b++) {
+            double v = 0;
+            for (int k = 0; k < 3; k++)
+                for (int l = 0; l < 3; l++) v += E[k][a] * E[l][b] * m[4 + i2(k, l)];
+            o[4 + i2(a, b)] = v;
+        }
+    for (int a = 0; a < 3; a++)
+        for (int b = a; b < 3; b++)
+            for (int c = b; c < 3; c++) {
+                double v = 0;
+                for (int k = 0; k < 3; k++)
+                    for (int l = 0; l < 3; l++)
+                        for (int q = 0; q < 3; q++) v += E[k][a] * E[l][b] * E[q][c] * m[10 + i3(k, l, q)];
+                o[10 + i3(a, b, c)] = v;
+            }
+    const double p0 = pol[3 * (size_t)s], p1 = pol[3 * (size_t)s + 1], p2 = pol[3 * (size_t)s + 2];
+    const double pk[3] = {p0, p1, p2};
+    const double sk[3] = {sqrt(fmax(p0, 0.0)), sqrt(fmax(p1, 0.0)), sqrt(fmax(p2, 0.0))};
+    for (int a = 0; a < 3; a++)
+        for (int b = a; b < 3; b++) {
+            double v = 0, w = 0;
+            for (int k = 0; k < 3; k++) {
+                v += E[k][a] * E[k][b] * pk[k];
+                w += E[k][a] * E[k][b] * sk[k];
+            }
+            labpol[6 * (size_t)s + i2(a, b)] = v;
+            sqrtpol[6 * (size_t)s + i2(a, b)] = w;
+        }
+    // MPIDForce.h:531 / MPIDReferenceKernels.cpp:134
+    damp[s] = pow((p0 + p1 + p2) / 3.0, 1.0 / 6.0);
+    aniso[s] = (p0 != p1 || p0 != p2) ? 1 : 0;
+}
+
+template <class R>
+__global__ void k_convert(int n, const double4* __restrict__ pos, const double* __restrict__ lab,
+                          const double* __restrict__ thole, const double* __restrict__ damp, R4<R>* rpos, R* rlab,
+                          R2<R>* rtd) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    double4 p = pos[s];
+    rpos[s] = R4<R>{(R)p.x, (R)p.y, (R)p.z, (R)p.w};
+    for (int k = 0; k < kMP; k++) rlab[(size_t)s * kMP + k] = (R)lab[(size_t)s * kMP + k];
+    rtd[s] = R2<R>{(R)thole[s], (R)damp[s]};
+}
+
+// ---------------------------------------------------------------------------
+// fields and solver vectors
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ D3 sym_mul(const double* S, D3 v) {
+    return mk(S[0] * v.x + S[1] * v.y + S[2] * v.z, S[1] * v.x + S[3] * v.y + S[4] * v.z,
+              S[2] * v.x + S[4] * v.y + S[5] * v.z);
+}
+
+template <class R>
+__global__ void k_fixed_field_finish(int n, const R* __restrict__ phi, const double* __restrict__ lab,
+                                     const acc_t* __restrict__ acc, double3 g, double selfc, double* E0) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const R* ph = phi + (size_t)s * 35;
+    const double* m = lab + (size_t)s * kMP;
+    E0[3 * (size_t)s + 0] = -g.x * (double)ph[1] + selfc * m[1] + acc_get(acc[3 * (size_t)s + 0]);
+    E0[3 * (size_t)s + 1] = -g.y * (double)ph[2] + selfc * m[2] + acc_get(acc[3 * (size_t)s + 1]);
+    E0[3 * (size_t)s + 2] = -g.z * (double)ph[3] + selfc * m[3] + acc_get(acc[3 * (size_t)s + 2]);
+}
+
+__global__ void k_solver_init(int n, const double* __restrict__ S, const double* __restrict__ E0, double* b,
+                              double* y) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    D3 v = sym_mul(S + 6 * (size_t)s, mk(E0[3 * (size_t)s], E0[3 * (size_t)s + 1], E0[3 * (size_t)s + 2]));
+    b[3 * (size_t)s] = y[3 * (size_t)s] = v.x;
+    b[3 * (size_t)s + 1] = y[3 * (size_t)s + 1] = v.y;
+    b[3 * (size_t)s + 2] = y[3 * (size_t)s + 2] = v.z;
+}
+
+template <class R>
+__global__ void k_apply_S(int n, const double* __restrict__ S, const double* __restrict__ y, R* mu, double* mud) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    D3 v = sym_mul(S + 6 * (size_t)s, mk(y[3 * (size_t)s], y[3 * (size_t)s + 1], y[3 * (size_t)s + 2]));
+    mu[3 * (size_t)s] = (R)v.x;
+    mu[3 * (size_t)s + 1] = (R)v.y;
+    mu[3 * (size_t)s + 2] = (R)v.z;
+    if (mud) {
+        mud[3 * (size_t)s] = v.x;
+        mud[3 * (size_t)s + 1] = v.y;
+        mud[3 * (size_t)s + 2] = v.z;
+    }
+}
+
+template <class R>
+__global__ void k_matvec_finish(int n, const double* __restrict__ S, const acc_t* __restrict__ acc,
+                                const R* __restrict__ erec, const R* __restrict__ mu, double selfc,
+                                const double* __restrict__ p, double* Ap) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    D3 E;
+    E.x = acc_get(acc[3 * (size_t)s]) + (double)erec[3 * (size_t)s] + selfc * (double)mu[3 * (size_t)s];
+    E.y = acc_get(acc[3 * (size_t)s + 1]) + (double)erec[3 * (size_t)s + 1] + selfc * (double)mu[3 * (size_t)s + 1];
+    E.z = acc_get(acc[3 * (size_t)s + 2]) + (double)erec[3 * (size_t)s + 2] + selfc * (double)mu[3 * (size_t)s + 2];
+    D3 v = sym_mul(S + 6 * (size_t)s, E);
+    Ap[3 * (size_t)s] = p[3 * (size_t)s] - v.x;
+    Ap[3 * (size_t)s + 1] = p[3 * (size_t)s + 1] - v.y;
+    Ap[3 * (size_t)s + 2] = p[3 * (size_t)s + 2] - v.z;
+}
+
+// Deterministic block reductions: 256 threads, fixed tree order.
+constexpr int kRedThreads = 256;
+constexpr int kRedBlocks = 512;
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = kRedThreads / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_dot_partial(int m, const double* __restrict__ a,
+                                                             const double* __restrict__ b, double* partial) {
+    __shared__ double sh[kRedThreads];
+    double v = 0;
+    for (int i = blockIdx.x * kRedThreads + threadIdx.x; i < m; i += kRedBlocks * kRedThreads) v += a[i] * b[i];
+    v = block_sum(v, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = v;
+}
+
+// Sums kRedBlocks partials per output slot; nout outputs stored back to back.
+__global__ __launch_bounds__(kRedThreads) void k_reduce_partials(const double* __restrict__ partial, int nout,
+                                                                 double* result) {
+    __shared__ double sh[kRedThreads];
+    for (int o = 0; o < nout; o++) {
+        double v = 0;
+        for (int i = threadIdx.x; i < kRedBlocks; i += kRedThreads) v += partial[o * kRedBlocks + i];
+        v = block_sum(v, sh);
+        if (threadIdx.x == 0) result[o] = v;
+    }
+}
+
+// x += a p, r -= a Ap with a = rr / pAp; partial sums of r.r and |S r|^2
+__global__ __launch_bounds__(kRedThreads) void k_cg_step(int n, const double* __restrict__ S,
+                                                         const double* __restrict__ scal,
+                                                         const double* __restrict__ p, const double* __restrict__ Ap,
+                                                         double* x, double* r, double* partial) {
+    __shared__ double sh[kRedThreads];
+    const double pAp = scal[1];
+    const double a = pAp != 0.0 ? scal[0] / pAp : 0.0;
+    double rr = 0, ss = 0;
+    for (int s = blockIdx.x * kRedThreads + threadIdx.x; s < n; s += kRedBlocks * kRedThreads) {
+        D3 rv;
+        for (int k = 0; k < 3; k++) {
+            const size_t id = 3 * (size_t)s + k;
+            x[id] += a * p[id];
+            const double nr = r[id] - a * Ap[id];
+            r[id] = nr;
+            rr += nr * nr;
+        }
+        rv = mk(r[3 * (size_t)s], r[3 * (size_t)s + 1], r[3 * (size_t)s + 2]);
+        D3 sr = sym_mul(S + 6 * (size_t)s, rv);
+        ss += dot(sr, sr);
+    }
+    rr = block_sum(rr, sh);
+    ss = block_sum(ss, sh);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = rr;
+        partial[kRedBlocks + blockIdx.x] = ss;
+    }
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_residual_norm(int n, const double* __restrict__ S,
+                                                               const double* __restrict__ r, double* partial) {
+    __shared__ double sh[kRedThreads];
+    double rr = 0, ss = 0;
+    for (int s = blockIdx.x * kRedThreads + threadIdx.x; s < n; s += kRedBlocks * kRedThreads) {
+        D3 rv = mk(r[3 * (size_t)s], r[3 * (size_t)s + 1], r[3 * (size_t)s + 2]);
+        rr += dot(rv, rv);
+        D3 sr = sym_mul(S + 6 * (size_t)s, rv);
+        ss += dot(sr, sr);
+    }
+    rr = block_sum(rr, sh);
+    ss = block_sum(ss, sh);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = rr;
+        partial[kRedBlocks + blockIdx.x] = ss;
+    }
+}
+
+__global__ void k_cg_direction(int m, const double* __restrict__ rr_old, const double* __restrict__ rr_new,
+                               const double* __restrict__ r, double* p) {
+    const double beta = rr_old[0] != 0.0 ? rr_new[0] / rr_old[0] : 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) p[i] = r[i] + beta * p[i];
+}
+
+__global__ void k_axpy(int m, double a, const double* __restrict__ x, double* y) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) y[i] += a * x[i];
+}
+__global__ void k_sub(int m, const double* __restrict__ a, const double* __restrict__ b, double* out) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) out[i] = a[i] - b[i];
+}
+
+__global__ void k_zero_acc(acc_t* p, size_t count) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+template <class R> __global__ void k_fill(R* p, size_t count, R v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// reciprocal-space energies, forces, torques and self terms (per atom)
+// ---------------------------------------------------------------------------
+// phi index of the derivative with exponents (a, b, c), a+b+c <= 4
+__device__ __forceinline__ int phi_idx(int a, int b, int c) {
+    const int t[5][5][5] = {
+        {{0, 3, 6, 12, 22}, {2, 9, 18, 28, -1}, {5, 16, 31, -1, -1}, {11, 26, -1, -1, -1}, {21, -1, -1, -1, -1}},
+        {{1, 8, 17, 27, -1}, {7, 19, 34, -1, -1}, {15, 33, -1, -1, -1}, {25, -1, -1, -1, -1}, {-1, -1, -1, -1, -1}},
+        {{4, 14, 30, -1, -1}, {13, 32, -1, -1, -1}, {29, -1, -1, -1, -1}, {-1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1}},
+        {{10, 24, -1, -1, -1}, {23, -1, -1, -1, -1}, {-1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1}},
+        {{20, -1, -1, -1, -1}, {-1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1}}};
+    return t[a][b][c];
+}
+
+// exponents (x, y, z counts) of multipole component k of a 20-record
+__device__ __forceinline__ void comp_exp(int k, int e[3]) {
+    const int t[20][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {2, 0, 0}, {1, 1, 0}, {1, 0, 1},
+                          {0, 2, 0}, {0, 1, 1}, {0, 0, 2}, {3, 0, 0}, {2, 1, 0}, {2, 0, 1}, {1, 2, 0},
+                          {1, 1, 1}, {1, 0, 2}, {0, 3, 0}, {0, 2, 1}, {0, 1, 2}, {0, 0, 3}};
+    e[0] = t[k][0];
+    e[1] = t[k][1];
+    e[2] = t[k][2];
+}
+// multiplicity of the unique component among all index orderings
+__device__ __forceinline__ double comp_mult(int k) {
+    const double t[20] = {1, 1, 1, 1, 1, 2, 2, 1, 2, 1, 1, 3, 3, 3, 6, 3, 1, 3, 3, 1};
+    return t[k];
+}
+__device__ __forceinline__ int levi(int a, int b, int c) {
+    if (a == b || b == c || a == c) return 0;
+    return ((a == 0 && b == 1) || (a == 1 && b == 2) || (a == 2 && b == 0)) ? 1 : -1;
+}
+
+// Torque -dE/dtheta of moments (d, Q, O) in a potential whose Cartesian
+// derivative tensors are c1, c2 (6), c3 (10); E = d.c1 + Q:c2 + O:c3 over all
+// index orderings.
+__device__ D3 multipole_torque(const double* d, const double* Q, const double* O, const double* c1, const double* c2,
+                               const double* c3) {
+    double t[3];
+    for (int k = 0; k < 3; k++) {
+        double s = 0;
+        for (int a = 0; a < 3; a++)
+            for (int i = 0; i < 3; i++) s += levi(a, k, i) * d[i] * c1[a];
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) {
+                double dq = 0;
+                for (int i = 0; i < 3; i++) dq += levi(a, k, i) * Q[i2(i, b)] + levi(b, k, i) * Q[i2(a, i)];
+                s += dq * c2[i2(a, b)];
+            }
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++)
+                for (int c = 0; c < 3; c++) {
+                    double dq = 0;
+                    for (int i = 0; i < 3; i++)
+                        dq += levi(a, k, i) * O[i3(i, b, c)] + levi(b, k, i) * O[i3(a, i, c)] +
+                              levi(c, k, i) * O[i3(a, b, i)];
+                    s += dq * c3[i3(a, b, c)];
+                }
+        t[k] = -s;
+    }
+    return mk(t[0], t[1], t[2]);
+}
+
+// Cartesian derivative tensors of the potential from fractional ones
+// (rectangular box: each derivative index scales by n_k/L_k).
+template <class R>
+__device__ void cart_potential(const R* ph, const double g[3], double* c1, double* c2, double* c3) {
+    for (int a = 0; a < 3; a++) {
+        int e[3] = {0, 0, 0};
+        e[a]++;
+        c1[a] = g[a] * (double)ph[phi_idx(e[0], e[1], e[2])];
+    }
+    for (int a = 0; a < 3; a++)
+        for (int b = a; b < 3; b++) {
+            int e[3] = {0, 0, 0};
+            e[a]++;
+            e[b]++;
+            c2[i2(a, b)] = g[a] * g[b] * (double)ph[phi_idx(e[0], e[1], e[2])];
+        }
+    for (int a = 0; a < 3; a++)
+        for (int b = a; b < 3; b++)
+            for (int c = b; c < 3; c++) {
+                int e[3] = {0, 0, 0};
+                e[a]++;
+                e[b]++;
+                e[c]++;
+                c3[i3(a, b, c)] = g[a] * g[b] * g[c] * (double)ph[phi_idx(e[0], e[1], e[2])];
+            }
+}
+
+template <class R>
+__global__ __launch_bounds__(128) void k_recip_atoms(int n, int mutual, const double* __restrict__ phi,
+                                                      const R* __restrict__ phidp, const double* __restrict__ lab,
+                                                      const double* __restrict__ mu, const int* __restrict__ aniso,
+                                                      double3 gs, double alpha, acc_t* force, acc_t* torque,
+                                                      acc_t* energy) {
+    __shared__ double sh[128];
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    double en = 0;
+    if (s < n) {
+        const double g[3] = {gs.x, gs.y, gs.z};
+        const double* m = lab + (size_t)s * kMP;
+        const double* ph = phi + (size_t)s * 35;
+        const R* pd = phidp + (size_t)s * 35;
+        const double u[3] = {mu[3 * (size_t)s], mu[3 * (size_t)s + 1], mu[3 * (size_t)s + 2]};
+        const bool an = aniso[s] != 0;
+        // fractional multipoles
+        double mf[20];
+        for (int k = 0; k < 20; k++) {
+            int e[3];
+            comp_exp(k, e);
+            double sc = comp_mult(k);
+            for (int a = 0; a < 3; a++)
+                for (int p = 0; p < e[a]; p++) sc *= g[a];
+            mf[k] = sc * m[k];
+        }
+        // fixed: energy 1/2 eps M.phi, force -eps grad, torque
+        double ef = 0, f[3] = {0, 0, 0}, fi[3] = {0, 0, 0};
+        for (int k = 0; k < 20; k++) {
+            int e[3];
+            comp_exp(k, e);
+            ef += mf[k] * (double)ph[phi_idx(e[0], e[1], e[2])];
+            for (int a = 0; a < 3; a++) {
+                int e2[3] = {e[0], e[1], e[2]};
+                e2[a]++;
+                const int id = phi_idx(e2[0], e2[1], e2[2]);
+                f[a] += mf[k] * (double)ph[id];
+                fi[a] += 2.0 * mf[k] * (double)pd[id];
+            }
+        }
+        // induced: energy 1/4 eps 2 u'.grad phi, force
+        const double uf[3] = {g[0] * u[0], g[1] * u[1], g[2] * u[2]};
+        double ei = 0;
+        for (int k = 0; k < 3; k++) {
+            int e[3] = {0, 0, 0};
+            e[k]++;
+            ei += 2.0 * uf[k] * (double)ph[phi_idx(e[0], e[1], e[2])];
+            for (int a = 0; a < 3; a++) {
+                int e2[3] = {0, 0, 0};
+                e2[k]++;
+                e2[a]++;
+                const int id = phi_idx(e2[0], e2[1], e2[2]);
+                fi[a] += 2.0 * uf[k] * (double)ph[id];
+                if (mutual) fi[a] += 2.0 * uf[k] * (double)pd[id];
+            }
+        }
+        double F[3];
+        for (int a = 0; a < 3; a++) F[a] = -g[a] * (kElectric * f[a] + 0.5 * kElectric * fi[a]);
+        // torques
+        double c1[3], c2[6], c3[10];
+        double dd[3] = {m[1], m[2], m[3]};
+        if (an) {
+            dd[0] += u[0];
+            dd[1] += u[1];
+            dd[2] += u[2];
+        }
+        cart_potential<double>(ph, g, c1, c2, c3);
+        D3 T = multipole_torque(dd, m + 4, m + 10, c1, c2, c3) * kElectric;
+        double di[3] = {m[1], m[2], m[3]};
+        if (mutual && an) {
+            di[0] += u[0];
+            di[1] += u[1];
+            di[2] += u[2];
+        }
+        cart_potential<R>(pd, g, c1, c2, c3);
+        T = T + multipole_torque(di, m + 4, m + 10, c1, c2, c3) * kElectric;
+        // self energy and torque
+        const double sqp = 1.7320508075688772;
+        const double sq[5] = {3.0 * m[9], 2.0 * sqp * m[6], 2.0 * sqp * m[8], sqp * (m[4] - m[7]), 2.0 * sqp * m[5]};
+        const double r32 = 1.2247448713915890, r35 = 0.7745966692414834, r110 = 0.31622776601683794;
+        const double so[7] = {15.0 * m[19], 15.0 * r32 * m[15], 15.0 * r32 * m[18], 15.0 * r35 * (m[12] - m[17]),
+                              30.0 * r35 * m[14], 15.0 * r110 * (m[10] - 3.0 * m[13]),
+                              15.0 * r110 * (3.0 * m[11] - m[16])};
+        double qq = 0, oo = 0;
+        for (int k = 0; k < 5; k++) qq += sq[k] * sq[k];
+        for (int k = 0; k < 7; k++) oo += so[k] * so[k];
+        const double a2 = alpha * alpha;
+        const double dud = m[1] * (m[1] + u[0]) + m[2] * (m[2] + u[1]) + m[3] * (m[3] + u[2]);
+        const double eself = -alpha * kElectric / kSqrtPi *
+                             (m[0] * m[0] + (2.0 / 3.0) * a2 * dud + (4.0 / 15.0) * a2 * a2 * qq +
+                              (8.0 / 105.0) * a2 * a2 * a2 * oo);
+        if (!an) {
+            const double term = (2.0 / 3.0) * kElectric * alpha * a2 / kSqrtPi;
+            T = T + cross(mk(m[1], m[2], m[3]), mk(2.0 * u[0], 2.0 * u[1], 2.0 * u[2])) * term;
+        }
+        en = 0.5 * kElectric * ef + 0.25 * kElectric * ei + eself;
+        acc_add(&force[3 * (size_t)s + 0], F[0]);
+        acc_add(&force[3 * (size_t)s + 1], F[1]);
+        acc_add(&force[3 * (size_t)s + 2], F[2]);
+        acc_add(&torque[3 * (size_t)s + 0], T.x);
+        acc_add(&torque[3 * (size_t)s + 1], T.y);
+        acc_add(&torque[3 * (size_t)s + 2], T.z);
+    }
+    sh[threadIdx.x] = en;
+    __syncthreads();
+    for (int off = 64; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) acc_add(energy, sh[0]);
+}
+
+// ---------------------------------------------------------------------------
+// torque -> forces on the frame atoms
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(128) void k_torque_to_force(int n, Box<double> box, const double4* __restrict__ pos,
+                                                          const int4* __restrict__ axes,
+                                                          const acc_t* __restrict__ torque, acc_t* force) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int4 ax = axes[s];
+    const int t = ax.w;
+    if (t == 5 || ax.x < 0) return;
+    const D3 tq = mk(acc_get(torque[3 * (size_t)s]), acc_get(torque[3 * (size_t)s + 1]),
+                     acc_get(torque[3 * (size_t)s + 2]));
+    const D3 pI = d3(pos[s]);
+    const int iz = ax.x, ix = ax.y, iy = ax.z;
+    D3 U = min_image(d3(pos[iz]) - pI, box);
+    const double nU = dnorm(U);
+    D3 V;
+    if (t == 4) V = fabs(U.x) < 0.866 ? mk(1.0, 0.0, 0.0) : mk(0.0, 1.0, 0.0);
+    else V = min_image(d3(pos[ix]) - pI, box);
+    const double nV = dnorm(V);
+    D3 W;
+    if ((t == 2 || t == 3) && iy >= 0) W = min_image(d3(pos[iy]) - pI, box);
+    else W = cross(U, V);
+    const double nW = dnorm(W);
+    D3 UV = cross(V, U), UW = cross(W, U), VW = cross(W, V);
+    dnorm(UV);
+    dnorm(UW);
+    dnorm(VW);
+    const double cUV = dot(U, V), sUV = sqrt(1.0 - cUV * cUV);
+    const double cUW = dot(U, W), sUW = sqrt(1.0 - cUW * cUW);
+    const double cVW = dot(V, W), sVW = sqrt(1.0 - cVW * cVW);
+    const double dU = -dot(U, tq), dV = -dot(V, tq), dW = -dot(W, tq);
+    D3 fZ = mk(0.0, 0.0, 0.0), fX = fZ, fY = fZ;
+    if (t == 0 || t == 1) {
+        double f1 = dV / (nU * sUV), f2 = dW / nU, f3 = -dU / (nV * sUV), f4 = 0.0;
+        if (t == 1) {
+            f2 *= 0.5;
+            f4 = 0.5 * dW / nV;
+        }
+        fZ = UV * f1 + UW * f2;
+        fX = UV * f3 + VW * f4;
+    } else if (t == 2) {
+        D3 Rv = V + W;
+        D3 Sv = cross(U, Rv);
+        dnorm(Rv);
+        dnorm(Sv);
+        D3 UR = cross(Rv, U), US = cross(Sv, U), VS = cross(Sv, V), WS = cross(Sv, W);
+        dnorm(UR);
+        dnorm(US);
+        dnorm(VS);
+        dnorm(WS);
+        const double cUR = dot(U, Rv), sUR = sqrt(1.0 - cUR * cUR);
+        const double cVS = dot(V, Sv), sVS = sqrt(1.0 - cVS * cVS);
+        const double cWS = dot(W, Sv), sWS = sqrt(1.0 - cWS * cWS);
+        D3 t1 = V - Sv * cVS, t2 = W - Sv * cWS;
+        dnorm(t1);
+        dnorm(t2);
+        const double ut1c = dot(U, t1), ut1s = sqrt(1.0 - ut1c * ut1c);
+        const double ut2c = dot(U, t2), ut2s = sqrt(1.0 - ut2c * ut2c);
+        const double dR = -dot(Rv, tq), dS = -dot(Sv, tq);
+        const double f1 = dR / (nU * sUR), f2 = dS / nU, f3 = dU / (nV * (ut1s + ut2s)), f4 = dU / (nW * (ut1s + ut2s));
+        fZ = UR * f1 + US * f2;
+        fX = (Sv * sVS - t1 * cVS) * f3;
+        fY = (Sv * sWS - t2 * cWS) * f4;
+    } else if (t == 3) {
+        fZ = (UW * (dW / (nU * sUW)) + UV * (dV / (nU * sUV)) - UW * (dU / (nU * sUW)) - UV * (dU / (nU * sUV))) *
+             (1.0 / 3.0);
+        fX = (VW * (dW / (nV * sVW)) - UV * (dU / (nV * sUV)) - VW * (dV / (nV * sVW)) + UV * (dV / (nV * sUV))) *
+             (1.0 / 3.0);
+        fY = (UW * (-dU / (nW * sUW)) - VW * (dV / (nW * sVW)) + UW * (dW / (nW * sUW)) + VW * (dW / (nW * sVW))) *
+             (1.0 / 3.0);
+        if (iy < 0) fY = mk(0.0, 0.0, 0.0);
+    } else if (t == 4) {
+        fZ = UV * (dV / (nU * sUV)) + UW * (dW / nU);
+    }
+    const D3 fI = fZ + fX + fY;
+    acc_add(&force[3 * (size_t)iz + 0], -fZ.x);
+    acc_add(&force[3 * (size_t)iz + 1], -fZ.y);
+    acc_add(&force[3 * (size_t)iz + 2], -fZ.z);
+    if (t != 4) {
+        acc_add(&force[3 * (size_t)ix + 0], -fX.x);
+        acc_add(&force[3 * (size_t)ix + 1], -fX.y);
+        acc_add(&force[3 * (size_t)ix + 2], -fX.z);
+    }
+    if ((t == 2 || t == 3) && iy >= 0) {
+        acc_add(&force[3 * (size_t)iy + 0], -fY.x);
+        acc_add(&force[3 * (size_t)iy + 1], -fY.y);
+        acc_add(&force[3 * (size_t)iy + 2], -fY.z);
+    }
+    acc_add(&force[3 * (size_t)s + 0], fI.x);
+    acc_add(&force[3 * (size_t)s + 1], fI.y);
+    acc_add(&force[3 * (size_t)s + 2], fI.z);
+}
+
+__global__ void k_output_forces(int n, const int* __restrict__ perm, const acc_t* __restrict__ force, double* out,
+                                int accumulate) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int a = perm[s];
+    for (int k = 0; k < 3; k++) {
+        const double v = acc_get(force[3 * (size_t)s + k]);
+        if (accumulate) out[3 * (size_t)a + k] += v;
+        else out[3 * (size_t)a + k] = v;
+    }
+}
+
+__global__ void k_unpermute3(int n, const int* __restrict__ perm, const double* __restrict__ in, double* out) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int a = perm[s];
+    for (int k = 0; k < 3; k++) out[3 * (size_t)a + k] = in[3 * (size_t)s + k];
+}
+
+template <class R>
+__global__ void k_field_sum(int n, const acc_t* __restrict__ acc, const R* __restrict__ erec, const R* __restrict__ mu,
+                            double selfc, double* out) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    for (int k = 0; k < 3; k++) {
+        const size_t id = 3 * (size_t)s + k;
+        out[id] = acc_get(acc[id]) + (double)erec[id] + selfc * (double)mu[id];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+size_t sort_temp_bytes(int n) {
+    size_t bytes = 0;
+    (void)rocprim::radix_sort_pairs(nullptr, bytes, (unsigned*)nullptr, (unsigned*)nullptr, (int*)nullptr, (int*)nullptr,
+                              (size_t)n, 0, 30);
+    return bytes;
+}
+int launch_sort_pairs(hipStream_t s, void* temp, size_t bytes, unsigned* kin, unsigned* kout, int* vin, int* vout,
+                      int n) {
+    return (int)rocprim::radix_sort_pairs(temp, bytes, kin, kout, vin, vout, (size_t)n, 0, 30, s);
+}
+template <class R>
+void launch_field_sum(hipStream_t s, int n, const acc_t* acc, const R* erec, const R* mu, double selfc, double* out) {
+    hipLaunchKernelGGL(k_field_sum<R>, dim3((n + 255) / 256), dim3(256), 0, s, n, acc, erec, mu, selfc, out);
+}
+
+static inline dim3 blocks_for(size_t n, int t) { return dim3((unsigned)((n + t - 1) / t)); }
+
+void launch_morton_keys(hipStream_t s, int n, const double* pos, Box<double> box, unsigned* keys, int* vals) {
+    hipLaunchKernelGGL(k_morton_keys, blocks_for(n, 256), dim3(256), 0, s, n, pos, box, keys, vals);
+}
+void launch_invert_perm(hipStream_t s, int n, const int* perm, int* inv) {
+    hipLaunchKernelGGL(k_invert_perm, blocks_for(n, 256), dim3(256), 0, s, n, perm, inv);
+}
+void launch_gather_sorted(hipStream_t s, int n, const int* perm, const int* inv, const double* pos, Box<double> box,
+                          const double* mol, const int4* axes, const double* thole, const double* pol,
+                          const int* partners, const int* npart, double4* spos, double* smol, int4* saxes,
+                          double* sthole, double* spol, int* spartners, int* snpart) {
+    hipLaunchKernelGGL(k_gather_sorted, blocks_for(n, 128), dim3(128), 0, s, n, perm, inv, pos, box, mol, axes, thole,
+                       pol, partners, npart, spos, smol, saxes, sthole, spol, spartners, snpart);
+}
+void launch_rotate(hipStream_t s, int n, Box<double> box, const double4* pos, const int4* axes, const double* mol,
+                   const double* pol, double* lab, double* labpol, double* sqrtpol, double* damp, int* aniso) {
+    hipLaunchKernelGGL(k_rotate, blocks_for(n, 128), dim3(128), 0, s, n, box, pos, axes, mol, pol, lab, labpol,
+                       sqrtpol, damp, aniso);
+}
+template <class R>
+void launch_convert(hipStream_t s, int n, const double4* pos, const double* lab, const double* thole,
+                    const double* damp, R4<R>* rpos, R* rlab, R2<R>* rtd) {
+    hipLaunchKernelGGL(k_convert<R>, blocks_for(n, 256), dim3(256), 0, s, n, pos, lab, thole, damp, rpos, rlab, rtd);
+}
+void launch_torque_to_force(hipStream_t s, int n, Box<double> box, const double4* pos, const int4* axes,
+                            const acc_t* torque, acc_t* force) {
+    hipLaunchKernelGGL(k_torque_to_force, blocks_for(n, 128), dim3(128), 0, s, n, box, pos, axes, torque, force);
+}
+void launch_output_forces(hipStream_t s, int n, const int* perm, const acc_t* force, double* out, int accumulate) {
+    hipLaunchKernelGGL(k_output_forces, blocks_for(n, 256), dim3(256), 0, s, n, perm, force, out, accumulate);
+}
+void launch_unpermute3(hipStream_t s, int n, const int* perm, const double* in, double* out) {
+    hipLaunchKernelGGL(k_unpermute3, blocks_for(n, 256), dim3(256), 0, s, n, perm, in, out);
+}
+void launch_zero_acc(hipStream_t s, acc_t* p, size_t count) {
+    int b = (int)std::min<size_t>((count + 255) / 256, 4096);
+    if (b < 1) b = 1;
+    hipLaunchKernelGGL(k_zero_acc, dim3(b), dim3(256), 0, s, p, count);
+}
+template <class R> void launch_fill(hipStream_t s, R* p, size_t count, R v) {
+    int b = (int)std::min<size_t>((count + 255) / 256, 8192);
+    if (b < 1) b = 1;
+    hipLaunchKernelGGL(k_fill<R>, dim3(b), dim3(256), 0, s, p, count, v);
+}
+template <class R>
+void launch_fixed_field_finish(hipStream_t s, int n, const R* phi, const double* lab, const acc_t* acc, double3 g,
+                               double selfc, double* E0) {
+    hipLaunchKernelGGL(k_fixed_field_finish<R>, blocks_for(n, 256), dim3(256), 0, s, n, phi, lab, acc, g, selfc, E0);
+}
+void launch_solver_init(hipStream_t s, int n, const double* S, const double* E0, double* b, double* y) {
+    hipLaunchKernelGGL(k_solver_init, blocks_for(n, 256), dim3(256), 0, s, n, S, E0, b, y);
+}
+template <class R> void launch_apply_S(hipStream_t s, int n, const double* S, const double* y, R* mu, double* mud) {
+    hipLaunchKernelGGL(k_apply_S<R>, blocks_for(n, 256), dim3(256), 0, s, n, S, y, mu, mud);
+}
+template <class R>
+void launch_matvec_finish(hipStream_t s, int n, const double* S, const acc_t* acc, const R* erecip, const R* mu,
+                          double selfc, const double* p, double* Ap) {
+    hipLaunchKernelGGL(k_matvec_finish<R>, blocks_for(n, 256), dim3(256), 0, s, n, S, acc, erecip, mu, selfc, p, Ap);
+}
+void launch_dot(hipStream_t s, int m, const double* a, const double* b, double* partial, double* result) {
+    hipLaunchKernelGGL(k_dot_partial, dim3(kRedBlocks), dim3(kRedThreads), 0, s, m, a, b, partial);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kRedThreads), 0, s, partial, 1, result);
+}
+void launch_cg_step(hipStream_t s, int n, const double* S, const double* scal, const double* p, const double* Ap,
+                    double* x, double* r, double* partial, double* result) {
+    hipLaunchKernelGGL(k_cg_step, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, S, scal, p, Ap, x, r, partial);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kRedThreads), 0, s, partial, 2, result);
+}
+void launch_residual_norm(hipStream_t s, int n, const double* S, const double* r, double* partial, double* result) {
+    hipLaunchKernelGGL(k_residual_norm, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, S, r, partial);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kRedThreads), 0, s, partial, 2, result);
+}
+void launch_cg_direction(hipStream_t s, int m, const double* rr_old, const double* rr_new, const double* r,
+                         double* p) {
+    int b = std::min((m + 255) / 256, 4096);
+    if (b < 1) b = 1;
+    hipLaunchKernelGGL(k_cg_direction, dim3(b), dim3(256), 0, s, m, rr_old, rr_new, r, p);
+}
+void launch_axpy(hipStream_t s, int m, double a, const double* x, double* y) {
+    int b = std::min((m + 255) / 256, 4096);
+    if (b < 1) b = 1;
+    hipLaunchKernelGGL(k_axpy, dim3(b), dim3(256), 0, s, m, a, x, y);
+}
+void launch_sub(hipStream_t s, int m, const double* a, const double* b_, double* out) {
+    int b = std::min((m + 255) / 256, 4096);
+    if (b < 1) b = 1;
+    hipLaunchKernelGGL(k_sub, dim3(b), dim3(256), 0, s, m, a, b_, out);
+}
+template <class R>
+void launch_recip_atoms(hipStream_t s, int n, int mutual, const double* phi, const R* phidp, const double* lab,
+                        const double* mu, const int* aniso, double3 gscale, double alpha, acc_t* force, acc_t* torque,
+                        acc_t* energy) {
+    hipLaunchKernelGGL(k_recip_atoms<R>, blocks_for(n, 128), dim3(128), 0, s, n, mutual, phi, phidp, lab, mu, aniso,
+                       gscale, alpha, force, torque, energy);
+}
+
+#define MPID_INSTANTIATE(R)                                                                                         \
+    template void launch_convert<R>(hipStream_t, int, const double4*, const double*, const double*, const double*,  \
+                                    R4<R>*, R*, R2<R>*);                                                            \
+    template void launch_fill<R>(hipStream_t, R*, size_t, R);                                                        \
+    template void launch_fixed_field_finish<R>(hipStream_t, int, const R*, const double*, const acc_t*, double3,    \
+                                               double, double*);                                                    \
+    template void launch_apply_S<R>(hipStream_t, int, const double*, const double*, R*, double*);                   \
+    template void launch_matvec_finish<R>(hipStream_t, int, const double*, const acc_t*, const R*, const R*, double, \
+                                          const double*, double*);                                                  \
+    template void launch_recip_atoms<R>(hipStream_t, int, int, const double*, const R*, const double*, const double*,\
+                                        const int*, double3, double, acc_t*, acc_t*, acc_t*);                      \
+    template void launch_field_sum<R>(hipStream_t, int, const acc_t*, const R*, const R*, double, double*);
+MPID_INSTANTIATE(float)
+MPID_INSTANTIATE(double)
+#undef MPID_INSTANTIATE
+
+}  // namespace mpid

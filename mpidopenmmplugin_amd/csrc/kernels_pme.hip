@@ -1,0 +1,739 @@
+// Reciprocal-space (PME) kernels of the MPIDForce path on gfx950: order-6
+// B-spline spreading of multipoles and induced dipoles, a hand-written
+// Stockham FFT, the influence-function convolution and the gathering of the
+// potential and its fractional derivatives.
+//
+// Follows MPIDReferencePmeForce (platforms/reference/src/SimTKReference/
+// MPIDReferenceForce.cpp): computeBSplinePoint :2956-3044, computeMPIDBsplines
+// :3049-3075, transformMultipolesToFractionalCoordinates :3077-3169,
+// spreadFixedMultipolesOntoGrid :3269-3327, spreadInducedDipolesOnGrid
+// :3532-3573, performMPIDReciprocalConvolution :3329-3366,
+// computeFixedPotentialFromGrid :3368-3530.  The reference's grid is complex
+// with x slowest and z fastest; so is this one.
+#include "engine.h"
+
+#include <algorithm>
+
+namespace mpid {
+
+// Order-6 B-spline weights of the 6 grid points and their first ND derivatives:
+// Cox-de Boor recursion, derivatives as backward differences of lower orders.
+template <class R, int ND>
+__device__ __forceinline__ void bspline6(R w, R th[kOrder][ND + 1]) {
+    R c[kOrder + 1][kOrder + 1];
+    c[1][0] = R(1);
+#pragma unroll
+    for (int p = 2; p <= kOrder; p++) {
+#pragma unroll
+        for (int m = 0; m < p; m++) {
+            R left = (m >= 1) ? c[p - 1][m - 1] : R(0);
+            R right = (m <= p - 2) ? c[p - 1][m] : R(0);
+            c[p][m] = ((w + R(p - 1 - m)) * left + (R(m + 1) - w) * right) * R(1.0 / (p - 1));
+        }
+    }
+#pragma unroll
+    for (int r = 0; r <= ND; r++) {
+        R cur[kOrder + 1];
+        const int p = kOrder - r;
+#pragma unroll
+        for (int m = 0; m < kOrder + 1; m++) cur[m] = (m < p) ? c[p][m] : R(0);
+        int len = p;
+#pragma unroll
+        for (int s = 0; s < r; s++) {
+            R nxt[kOrder + 1];
+#pragma unroll
+            for (int m = 0; m < kOrder + 1; m++)
+                nxt[m] = ((m >= 1 && m - 1 < len) ? cur[m - 1] : R(0)) - ((m < len) ? cur[m] : R(0));
+            len++;
+#pragma unroll
+            for (int m = 0; m < kOrder + 1; m++) cur[m] = nxt[m];
+        }
+#pragma unroll
+        for (int m = 0; m < kOrder; m++) th[m][r] = cur[m];
+    }
+}
+
+__device__ __forceinline__ unsigned spread_bits_pme(unsigned v) {
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+// Grid origin and spline offset of one atom along axis k, as the reference
+// computes them (position wrapped to [-L/2, L/2), fractional coordinate
+// shifted by one half box).
+template <class R>
+__device__ __forceinline__ void grid_coord(R x, R L, R inv, int n, int& ig, R& w) {
+    x -= L * floor(x * inv + R(0.5));
+    R f = x * inv;
+    R fr = R(n) * (f - R((int)(f + R(0.5))) + R(0.5));
+    int ifr = (int)floor(fr);
+    w = fr - R(ifr);
+    ig = ifr - kOrder + 1;
+    ig %= n;
+    if (ig < 0) ig += n;
+}
+
+// ---------------------------------------------------------------------------
+// spreading
+// ---------------------------------------------------------------------------
+template <class R>
+__global__ __launch_bounds__(256) void k_spread_fixed(int n, const R4<R>* __restrict__ pos, const R* __restrict__ lab,
+                                                       PmeGeom<R> g, R2<R>* __restrict__ grid) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const R4<R> p = pos[i];
+    int ig[3];
+    R w[3];
+    grid_coord(p.x, g.box.L[0], g.box.inv[0], g.n[0], ig[0], w[0]);
+    grid_coord(p.y, g.box.L[1], g.box.inv[1], g.n[1], ig[1], w[1]);
+    grid_coord(p.z, g.box.L[2], g.box.inv[2], g.n[2], ig[2], w[2]);
+    R tx[kOrder][4], ty[kOrder][4], tz[kOrder][4];
+    bspline6<R, 3>(w[0], tx);
+    bspline6<R, 3>(w[1], ty);
+    bspline6<R, 3>(w[2], tz);
+    const R* m = lab + (size_t)i * kMP;
+    const R gx = g.gscale[0], gy = g.gscale[1], gz = g.gscale[2];
+    // fractional multipoles (rectangular box: each Cartesian index scales by n_k/L_k)
+    const R q = m[0];
+    const R dx = gx * m[1], dy = gy * m[2], dz = gz * m[3];
+    const R qxx = gx * gx * m[4], qxy = R(2) * gx * gy * m[5], qxz = R(2) * gx * gz * m[6];
+    const R qyy = gy * gy * m[7], qyz = R(2) * gy * gz * m[8], qzz = gz * gz * m[9];
+    const R oxxx = gx * gx * gx * m[10], oxxy = R(3) * gx * gx * gy * m[11], oxxz = R(3) * gx * gx * gz * m[12];
+    const R oxyy = R(3) * gx * gy * gy * m[13], oxyz = R(6) * gx * gy * gz * m[14], oxzz = R(3) * gx * gz * gz * m[15];
+    const R oyyy = gy * gy * gy * m[16], oyyz = R(3) * gy * gy * gz * m[17], oyzz = R(3) * gy * gz * gz * m[18];
+    const R ozzz = gz * gz * gz * m[19];
+    const int ny = g.n[1], nz = g.n[2];
+    for (int iy = 0; iy < kOrder; iy++) {
+        int y = ig[1] + iy;
+        if (y >= ny) y -= ny;
+        const R u0 = ty[iy][0], u1 = ty[iy][1], u2 = ty[iy][2], u3 = ty[iy][3];
+        for (int iz = 0; iz < kOrder; iz++) {
+            int z = ig[2] + iz;
+            if (z >= nz) z -= nz;
+            const R v0 = tz[iz][0], v1 = tz[iz][1], v2 = tz[iz][2], v3 = tz[iz][3];
+            const R t0 = q * u0 * v0 + dy * u1 * v0 + dz * u0 * v1 + qyy * u2 * v0 + qzz * u0 * v2 + qyz * u1 * v1 +
+                         oyyy * u3 * v0 + oyyz * u2 * v1 + oyzz * u1 * v2 + ozzz * u0 * v3;
+            const R t1 = dx * u0 * v0 + qxy * u1 * v0 + qxz * u0 * v1 + oxyy * u2 * v0 + oxyz * u1 * v1 + oxzz * u0 * v2;
+            const R t2 = qxx * u0 * v0 + oxxy * u1 * v0 + oxxz * u0 * v1;
+            const R t3 = oxxx * u0 * v0;
+#pragma unroll
+            for (int ix = 0; ix < kOrder; ix++) {
+                int x = ig[0] + ix;
+                if (x >= g.n[0]) x -= g.n[0];
+                const R v = t0 * tx[ix][0] + t1 * tx[ix][1] + t2 * tx[ix][2] + t3 * tx[ix][3];
+                atomicAdd(&grid[((size_t)x * ny + y) * nz + z].x, v);
+            }
+        }
+    }
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void k_spread_induced(int n, const R4<R>* __restrict__ pos, const R* __restrict__ mu,
+                                                         PmeGeom<R> g, R2<R>* __restrict__ grid) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const R4<R> p = pos[i];
+    int ig[3];
+    R w[3];
+    grid_coord(p.x, g.box.L[0], g.box.inv[0], g.n[0], ig[0], w[0]);
+    grid_coord(p.y, g.box.L[1], g.box.inv[1], g.n[1], ig[1], w[1]);
+    grid_coord(p.z, g.box.L[2], g.box.inv[2], g.n[2], ig[2], w[2]);
+    R tx[kOrder][2], ty[kOrder][2], tz[kOrder][2];
+    bspline6<R, 1>(w[0], tx);
+    bspline6<R, 1>(w[1], ty);
+    bspline6<R, 1>(w[2], tz);
+    const R ux = g.gscale[0] * mu[3 * (size_t)i], uy = g.gscale[1] * mu[3 * (size_t)i + 1],
+            uz = g.gscale[2] * mu[3 * (size_t)i + 2];
+    const int ny = g.n[1], nz = g.n[2];
+    for (int iy = 0; iy < kOrder; iy++) {
+        int y = ig[1] + iy;
+        if (y >= ny) y -= ny;
+        for (int iz = 0; iz < kOrder; iz++) {
+            int z = ig[2] + iz;
+            if (z >= nz) z -= nz;
+            const R t0 = uy * ty[iy][1] * tz[iz][0] + uz * ty[iy][0] * tz[iz][1];
+            const R t1 = ux * ty[iy][0] * tz[iz][0];
+#pragma unroll
+            for (int ix = 0; ix < kOrder; ix++) {
+                int x = ig[0] + ix;
+                if (x >= g.n[0]) x -= g.n[0];
+                atomicAdd(&grid[((size_t)x * ny + y) * nz + z].x, t0 * tx[ix][0] + t1 * tx[ix][1]);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tiled spreading without global atomics.
+//
+// The atoms are sorted along a Morton curve of 1024^3 cells, so the atoms
+// inside one cell of the coarser 2^m-per-axis Morton level are a contiguous
+// range.  With n_k = 2^m * T_k, such a cell spans T_k grid spacings and the
+// stencils of its atoms fall in a window of W_k <= T_k + 8 points that starts
+// at c_k T_k + n_k/2 - 6 (the reference places atom x at n x / L + n / 2,
+// MPIDReferenceForce.cpp:3061-3062; one point of slack on each side absorbs
+// the rounding of the Morton key).  One workgroup per cell spreads its atoms
+// into an LDS copy of the window, then stores the window to scratch; a fold
+// pass adds, for every grid point, the windows that cover it.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned compact_bits(unsigned v) {
+    v &= 0x09249249u;
+    v = (v | (v >> 2)) & 0x030c30c3u;
+    v = (v | (v >> 4)) & 0x0300f00fu;
+    v = (v | (v >> 8)) & 0x030000ffu;
+    v = (v | (v >> 16)) & 0x000003ffu;
+    return v;
+}
+
+__global__ void k_tile_ranges(int n, const unsigned* __restrict__ keys, int shift, int* tstart, int* tend) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const unsigned t = keys[s] >> shift;
+    if (s == 0 || (keys[s - 1] >> shift) != t) tstart[t] = s;
+    if (s == n - 1 || (keys[s + 1] >> shift) != t) tend[t] = s + 1;
+}
+
+template <class R, int MODE>
+__global__ __launch_bounds__(256) void k_spread_tiled(const R4<R>* __restrict__ pos, const R* __restrict__ src,
+                                                       PmeGeom<R> g, TileGeom tg, const int* __restrict__ tstart,
+                                                       const int* __restrict__ tend, R* __restrict__ scratch,
+                                                       R2<R>* __restrict__ grid) {
+    constexpr int ND = MODE == 0 ? 3 : 1;   // spline derivatives needed
+    constexpr int NC = MODE == 0 ? 20 : 3;  // fractional components
+    constexpr int CH = 32;                  // atoms staged per pass
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int W0 = tg.W[0], W1 = tg.W[1], W2 = tg.W[2];
+    const int WV = W0 * W1 * W2;
+    R* win = reinterpret_cast<R*>(smem_raw);
+    R* sth = win + WV;                                // [CH][3][6][ND+1]
+    R* smf = sth + CH * 3 * kOrder * (ND + 1);        // [CH][NC]
+    int* sig = reinterpret_cast<int*>(smf + CH * NC); // [CH][3] local window origin of the stencil
+    const int t = blockIdx.x;
+    const int c[3] = {(int)compact_bits((unsigned)t >> 2), (int)compact_bits((unsigned)t >> 1),
+                      (int)compact_bits((unsigned)t)};
+    int w0[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) w0[k] = ((c[k] * tg.T[k] + (g.n[k] >> 1) - 6) % g.n[k] + g.n[k]) % g.n[k];
+    for (int i = threadIdx.x; i < WV; i += blockDim.x) win[i] = R(0);
+    const int a_begin = tstart[t], a_end = tend[t];
+    __syncthreads();
+    for (int a0 = a_begin; a0 < a_end; a0 += CH) {
+        const int cnt = min(CH, a_end - a0);
+        if ((int)threadIdx.x < cnt) {
+            const int a = a0 + threadIdx.x;
+            const R4<R> p = pos[a];
+            const R px[3] = {p.x, p.y, p.z};
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                int ig;
+                R w;
+                grid_coord(px[k], g.box.L[k], g.box.inv[k], g.n[k], ig, w);
+                R th[kOrder][ND + 1];
+                bspline6<R, ND>(w, th);
+                for (int m = 0; m < kOrder; m++)
+                    for (int r = 0; r <= ND; r++) sth[((threadIdx.x * 3 + k) * kOrder + m) * (ND + 1) + r] = th[m][r];
+                sig[threadIdx.x * 3 + k] = (ig - w0[k] + g.n[k]) % g.n[k];
+            }
+            R* mf = smf + threadIdx.x * NC;
+            const R gx = g.gscale[0], gy = g.gscale[1], gz = g.gscale[2];
+            if (MODE == 0) {
+                const R* m = src + (size_t)a * kMP;
+                mf[0] = m[0];
+                mf[1] = gx * m[1]; mf[2] = gy * m[2]; mf[3] = gz * m[3];
+                mf[4] = gx * gx * m[4]; mf[5] = R(2) * gx * gy * m[5]; mf[6] = R(2) * gx * gz * m[6];
+                mf[7] = gy * gy * m[7]; mf[8] = R(2) * gy * gz * m[8]; mf[9] = gz * gz * m[9];
+                mf[10] = gx * gx * gx * m[10]; mf[11] = R(3) * gx * gx * gy * m[11];
+                mf[12] = R(3) * gx * gx * gz * m[12]; mf[13] = R(3) * gx * gy * gy * m[13];
+                mf[14] = R(6) * gx * gy * gz * m[14]; mf[15] = R(3) * gx * gz * gz * m[15];
+                mf[16] = gy * gy * gy * m[16]; mf[17] = R(3) * gy * gy * gz * m[17];
+                mf[18] = R(3) * gy * gz * gz * m[18]; mf[19] = gz * gz * gz * m[19];
+            } else {
+                mf[0] = gx * src[3 * (size_t)a];
+                mf[1] = gy * src[3 * (size_t)a + 1];
+                mf[2] = gz * src[3 * (size_t)a + 2];
+            }
+        }
+        __syncthreads();
+        // one item = (atom, iy, iz): 6 points along x
+        for (int it = threadIdx.x; it < cnt * kOrder * kOrder; it += blockDim.x) {
+            const int la = it / (kOrder * kOrder);
+            const int iy = (it / kOrder) % kOrder, iz = it % kOrder;
+            const R* tx = sth + ((la * 3 + 0) * kOrder) * (ND + 1);
+            const R* ty = sth + ((la * 3 + 1) * kOrder + iy) * (ND + 1);
+            const R* tz = sth + ((la * 3 + 2) * kOrder + iz) * (ND + 1);
+            const R* mf = smf + la * NC;
+            R t0, t1, t2 = R(0), t3 = R(0);
+            if (MODE == 0) {
+                const R u0 = ty[0], u1 = ty[1], u2 = ty[2], u3 = ty[ND >= 3 ? 3 : 0];
+                const R v0 = tz[0], v1 = tz[1], v2 = tz[2], v3 = tz[ND >= 3 ? 3 : 0];
+                t0 = mf[0] * u0 * v0 + mf[2] * u1 * v0 + mf[3] * u0 * v1 + mf[7] * u2 * v0 + mf[9] * u0 * v2 +
+                     mf[8] * u1 * v1 + mf[16] * u3 * v0 + mf[17] * u2 * v1 + mf[18] * u1 * v2 + mf[19] * u0 * v3;
+                t1 = mf[1] * u0 * v0 + mf[5] * u1 * v0 + mf[6] * u0 * v1 + mf[13] * u2 * v0 + mf[14] * u1 * v1 +
+                     mf[15] * u0 * v2;
+                t2 = mf[4] * u0 * v0 + mf[11] * u1 * v0 + mf[12] * u0 * v1;
+                t3 = mf[10] * u0 * v0;
+            } else {
+                t0 = mf[1] * ty[1] * tz[0] + mf[2] * ty[0] * tz[1];
+                t1 = mf[0] * ty[0] * tz[0];
+            }
+            const int ly = sig[la * 3 + 1] + iy, lz = sig[la * 3 + 2] + iz;
+            const int lx0 = sig[la * 3 + 0];
+            const bool inside_yz = ly < W1 && lz < W2;
+#pragma unroll
+            for (int ix = 0; ix < kOrder; ix++) {
+                R v = t0 * tx[ix * (ND + 1)] + t1 * tx[ix * (ND + 1) + 1];
+                if (MODE == 0) v += t2 * tx[ix * (ND + 1) + 2] + t3 * tx[ix * (ND + 1) + ND];
+                const int lx = lx0 + ix;
+                if (inside_yz && lx < W0) {
+                    atomicAdd(&win[(lx * W1 + ly) * W2 + lz], v);
+                } else {
+                    // outside the window (not expected): straight to the grid
+                    const int x = (w0[0] + lx) % g.n[0], y = (w0[1] + ly) % g.n[1], z = (w0[2] + lz) % g.n[2];
+                    atomicAdd(&grid[((size_t)x * g.n[1] + y) * g.n[2] + z].x, v);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    R* out = scratch + (size_t)t * WV;
+    for (int i = threadIdx.x; i < WV; i += blockDim.x) out[i] = win[i];
+}
+
+// grid(x,y,z) = sum of the windows covering the point (plus what is already there)
+template <class R>
+__global__ __launch_bounds__(256) void k_fold(PmeGeom<R> g, TileGeom tg, const R* __restrict__ scratch,
+                                               R2<R>* __restrict__ grid) {
+    const size_t total = (size_t)g.n[0] * g.n[1] * g.n[2];
+    const int WV = tg.W[0] * tg.W[1] * tg.W[2];
+    const int nt = tg.tiles_per_axis;
+    for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < total; id += (size_t)gridDim.x * blockDim.x) {
+        const int gz = (int)(id % g.n[2]);
+        const int gy = (int)((id / g.n[2]) % g.n[1]);
+        const int gx = (int)(id / ((size_t)g.n[1] * g.n[2]));
+        const int gg[3] = {gx, gy, gz};
+        int cc[3], rr[3], nj[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int q = ((gg[k] - (g.n[k] >> 1) + 6) % g.n[k] + g.n[k]) % g.n[k];
+            cc[k] = q / tg.T[k];
+            rr[k] = q - cc[k] * tg.T[k];
+            nj[k] = min((tg.W[k] - 1 - rr[k]) / tg.T[k] + 1, nt);  // windows c, c-1, ... covering the point
+        }
+        R sum = grid[id].x;
+        for (int jx = 0; jx < nj[0]; jx++)
+            for (int jy = 0; jy < nj[1]; jy++)
+                for (int jz = 0; jz < nj[2]; jz++) {
+                    const unsigned tx = (unsigned)((cc[0] - jx + nt) % nt), ty = (unsigned)((cc[1] - jy + nt) % nt),
+                                   tz = (unsigned)((cc[2] - jz + nt) % nt);
+                    const unsigned tile = (spread_bits_pme(tx) << 2) | (spread_bits_pme(ty) << 1) | spread_bits_pme(tz);
+                    const int lx = rr[0] + jx * tg.T[0], ly = rr[1] + jy * tg.T[1], lz = rr[2] + jz * tg.T[2];
+                    sum += scratch[(size_t)tile * WV + (lx * tg.W[1] + ly) * tg.W[2] + lz];
+                }
+        grid[id] = R2<R>{sum, R(0)};
+    }
+}
+
+// ---------------------------------------------------------------------------
+// convolution with the PME influence function
+// ---------------------------------------------------------------------------
+template <class R>
+__global__ __launch_bounds__(256) void k_convolve(PmeGeom<R> g, const R* __restrict__ moduli, R alpha, R2<R>* grid) {
+    const size_t total = (size_t)g.n[0] * g.n[1] * g.n[2];
+    const R expf = R(M_PI * M_PI) / (alpha * alpha);
+    const R sc = R(1.0 / M_PI) * g.box.inv[0] * g.box.inv[1] * g.box.inv[2];
+    for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < total; id += (size_t)gridDim.x * blockDim.x) {
+        const int kz = (int)(id % g.n[2]);
+        const int ky = (int)((id / g.n[2]) % g.n[1]);
+        const int kx = (int)(id / ((size_t)g.n[1] * g.n[2]));
+        if (id == 0) {
+            grid[0] = R2<R>{R(0), R(0)};
+            continue;
+        }
+        const int mx = kx < (g.n[0] + 1) / 2 ? kx : kx - g.n[0];
+        const int my = ky < (g.n[1] + 1) / 2 ? ky : ky - g.n[1];
+        const int mz = kz < (g.n[2] + 1) / 2 ? kz : kz - g.n[2];
+        const R hx = R(mx) * g.box.inv[0], hy = R(my) * g.box.inv[1], hz = R(mz) * g.box.inv[2];
+        const R m2 = hx * hx + hy * hy + hz * hz;
+        const R den = m2 * moduli[kx] * moduli[g.n[0] + ky] * moduli[g.n[0] + g.n[1] + kz];
+        const R et = sc * exp_r(-expf * m2) / den;
+        R2<R> v = grid[id];
+        v.x *= et;
+        v.y *= et;
+        grid[id] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// gathering
+// ---------------------------------------------------------------------------
+// Fractional potential derivatives phi_abc, a+b+c <= 4, in the reference's
+// order (MPIDReferenceForce.cpp:3498-3532).
+template <class R>
+__global__ __launch_bounds__(128) void k_gather_phi(int n, const R4<R>* __restrict__ pos, PmeGeom<R> g,
+                                                     const R2<R>* __restrict__ grid, R* __restrict__ phi) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const R4<R> p = pos[i];
+    int ig[3];
+    R w[3];
+    grid_coord(p.x, g.box.L[0], g.box.inv[0], g.n[0], ig[0], w[0]);
+    grid_coord(p.y, g.box.L[1], g.box.inv[1], g.n[1], ig[1], w[1]);
+    grid_coord(p.z, g.box.L[2], g.box.inv[2], g.n[2], ig[2], w[2]);
+    R tx[kOrder][5], ty[kOrder][5], tz[kOrder][5];
+    bspline6<R, 4>(w[0], tx);
+    bspline6<R, 4>(w[1], ty);
+    bspline6<R, 4>(w[2], tz);
+    const int ny = g.n[1], nz = g.n[2];
+    R s000 = 0, s100 = 0, s010 = 0, s001 = 0, s200 = 0, s020 = 0, s002 = 0, s110 = 0, s101 = 0, s011 = 0;
+    R s300 = 0, s030 = 0, s003 = 0, s210 = 0, s201 = 0, s120 = 0, s021 = 0, s102 = 0, s012 = 0, s111 = 0;
+    R s400 = 0, s040 = 0, s004 = 0, s310 = 0, s301 = 0, s130 = 0, s031 = 0, s103 = 0, s013 = 0, s220 = 0;
+    R s202 = 0, s022 = 0, s211 = 0, s121 = 0, s112 = 0;
+    for (int iz = 0; iz < kOrder; iz++) {
+        int z = ig[2] + iz;
+        if (z >= nz) z -= nz;
+        R b00 = 0, b10 = 0, b01 = 0, b20 = 0, b11 = 0, b02 = 0, b30 = 0, b21 = 0, b12 = 0, b03 = 0, b40 = 0, b04 = 0,
+          b31 = 0, b13 = 0, b22 = 0;
+        for (int iy = 0; iy < kOrder; iy++) {
+            int y = ig[1] + iy;
+            if (y >= ny) y -= ny;
+            R a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
+#pragma unroll
+            for (int ix = 0; ix < kOrder; ix++) {
+                int x = ig[0] + ix;
+                if (x >= g.n[0]) x -= g.n[0];
+                const R q = grid[((size_t)x * ny + y) * nz + z].x;
+                a0 += q * tx[ix][0];
+                a1 += q * tx[ix][1];
+                a2 += q * tx[ix][2];
+                a3 += q * tx[ix][3];
+                a4 += q * tx[ix][4];
+            }
+            const R u0 = ty[iy][0], u1 = ty[iy][1], u2 = ty[iy][2], u3 = ty[iy][3], u4 = ty[iy][4];
+            b00 += a0 * u0; b10 += a1 * u0; b01 += a0 * u1; b20 += a2 * u0; b11 += a1 * u1;
+            b02 += a0 * u2; b30 += a3 * u0; b21 += a2 * u1; b12 += a1 * u2; b03 += a0 * u3;
+            b40 += a4 * u0; b04 += a0 * u4; b31 += a3 * u1; b13 += a1 * u3; b22 += a2 * u2;
+        }
+        const R v0 = tz[iz][0], v1 = tz[iz][1], v2 = tz[iz][2], v3 = tz[iz][3], v4 = tz[iz][4];
+        s000 += b00 * v0; s100 += b10 * v0; s010 += b01 * v0; s001 += b00 * v1;
+        s200 += b20 * v0; s020 += b02 * v0; s002 += b00 * v2; s110 += b11 * v0; s101 += b10 * v1; s011 += b01 * v1;
+        s300 += b30 * v0; s030 += b03 * v0; s003 += b00 * v3; s210 += b21 * v0; s201 += b20 * v1;
+        s120 += b12 * v0; s021 += b02 * v1; s102 += b10 * v2; s012 += b01 * v2; s111 += b11 * v1;
+        s400 += b40 * v0; s040 += b04 * v0; s004 += b00 * v4; s310 += b31 * v0; s301 += b30 * v1;
+        s130 += b13 * v0; s031 += b03 * v1; s103 += b10 * v3; s013 += b01 * v3; s220 += b22 * v0;
+        s202 += b20 * v2; s022 += b02 * v2; s211 += b21 * v1; s121 += b12 * v1; s112 += b11 * v2;
+    }
+    R* o = phi + (size_t)i * 35;
+    o[0] = s000; o[1] = s100; o[2] = s010; o[3] = s001; o[4] = s200; o[5] = s020; o[6] = s002;
+    o[7] = s110; o[8] = s101; o[9] = s011; o[10] = s300; o[11] = s030; o[12] = s003; o[13] = s210;
+    o[14] = s201; o[15] = s120; o[16] = s021; o[17] = s102; o[18] = s012; o[19] = s111; o[20] = s400;
+    o[21] = s040; o[22] = s004; o[23] = s310; o[24] = s301; o[25] = s130; o[26] = s031; o[27] = s103;
+    o[28] = s013; o[29] = s220; o[30] = s202; o[31] = s022; o[32] = s211; o[33] = s121; o[34] = s112;
+}
+
+// Field -grad(phi) only (the matrix-vector product of the solver needs no more).
+template <class R>
+__global__ __launch_bounds__(256) void k_gather_field(int n, const R4<R>* __restrict__ pos, PmeGeom<R> g,
+                                                       const R2<R>* __restrict__ grid, R* __restrict__ field) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const R4<R> p = pos[i];
+    int ig[3];
+    R w[3];
+    grid_coord(p.x, g.box.L[0], g.box.inv[0], g.n[0], ig[0], w[0]);
+    grid_coord(p.y, g.box.L[1], g.box.inv[1], g.n[1], ig[1], w[1]);
+    grid_coord(p.z, g.box.L[2], g.box.inv[2], g.n[2], ig[2], w[2]);
+    R tx[kOrder][2], ty[kOrder][2], tz[kOrder][2];
+    bspline6<R, 1>(w[0], tx);
+    bspline6<R, 1>(w[1], ty);
+    bspline6<R, 1>(w[2], tz);
+    const int ny = g.n[1], nz = g.n[2];
+    R s100 = 0, s010 = 0, s001 = 0;
+    for (int iz = 0; iz < kOrder; iz++) {
+        int z = ig[2] + iz;
+        if (z >= nz) z -= nz;
+        R b00 = 0, b10 = 0, b01 = 0;
+        for (int iy = 0; iy < kOrder; iy++) {
+            int y = ig[1] + iy;
+            if (y >= ny) y -= ny;
+            R a0 = 0, a1 = 0;
+#pragma unroll
+            for (int ix = 0; ix < kOrder; ix++) {
+                int x = ig[0] + ix;
+                if (x >= g.n[0]) x -= g.n[0];
+                const R q = grid[((size_t)x * ny + y) * nz + z].x;
+                a0 += q * tx[ix][0];
+                a1 += q * tx[ix][1];
+            }
+            b00 += a0 * ty[iy][0];
+            b10 += a1 * ty[iy][0];
+            b01 += a0 * ty[iy][1];
+        }
+        s100 += b10 * tz[iz][0];
+        s010 += b01 * tz[iz][0];
+        s001 += b00 * tz[iz][1];
+    }
+    field[3 * (size_t)i + 0] = -g.gscale[0] * s100;
+    field[3 * (size_t)i + 1] = -g.gscale[1] * s010;
+    field[3 * (size_t)i + 2] = -g.gscale[2] * s001;
+}
+
+// ---------------------------------------------------------------------------
+// Stockham FFT: each workgroup transforms T lines of length n held in LDS;
+// each stage applies one radix (2, 3, 4 or 5) to every line.
+// ---------------------------------------------------------------------------
+template <class R> struct Cx {
+    R x, y;
+};
+template <class R> __device__ __forceinline__ Cx<R> cmul(Cx<R> a, Cx<R> b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <class R> __device__ __forceinline__ Cx<R> cadd(Cx<R> a, Cx<R> b) { return {a.x + b.x, a.y + b.y}; }
+template <class R> __device__ __forceinline__ Cx<R> csub(Cx<R> a, Cx<R> b) { return {a.x - b.x, a.y - b.y}; }
+// multiply by s*i, s = +-1
+template <class R> __device__ __forceinline__ Cx<R> cmuli(Cx<R> a, R s) { return {-s * a.y, s * a.x}; }
+
+template <class R, int RAD> __device__ __forceinline__ void dft_small(Cx<R>* v, R s) {
+    if (RAD == 2) {
+        Cx<R> a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if (RAD == 4) {
+        Cx<R> a0 = v[0], a1 = v[1], a2 = v[2], a3 = v[3];
+        Cx<R> t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = cmuli(csub(a1, a3), s);
+        v[0] = cadd(t0, t2);
+        v[1] = cadd(t1, t3);
+        v[2] = csub(t0, t2);
+        v[3] = csub(t1, t3);
+    } else if (RAD == 3) {
+        const R h = R(0.86602540378443865);
+        Cx<R> a0 = v[0], t1 = cadd(v[1], v[2]), t2 = cmuli(csub(v[1], v[2]), s * h);
+        Cx<R> m = {a0.x - R(0.5) * t1.x, a0.y - R(0.5) * t1.y};
+        v[0] = cadd(a0, t1);
+        v[1] = cadd(m, t2);
+        v[2] = csub(m, t2);
+    } else if (RAD == 5) {
+        const R c1 = R(0.30901699437494742), c2 = R(-0.80901699437494742);
+        const R s1 = R(0.95105651629515357), s2 = R(0.58778525229247313);
+        Cx<R> a0 = v[0];
+        Cx<R> t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]), t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+        Cx<R> m1 = {a0.x + c1 * t1.x + c2 * t2.x, a0.y + c1 * t1.y + c2 * t2.y};
+        Cx<R> m2 = {a0.x + c2 * t1.x + c1 * t2.x, a0.y + c2 * t1.y + c1 * t2.y};
+        Cx<R> n1 = cmuli(Cx<R>{s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y}, s);
+        Cx<R> n2 = cmuli(Cx<R>{s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y}, s);
+        v[0] = cadd(a0, cadd(t1, t2));
+        v[1] = cadd(m1, n1);
+        v[4] = csub(m1, n1);
+        v[2] = cadd(m2, n2);
+        v[3] = csub(m2, n2);
+    }
+}
+
+template <class R, int RAD>
+__device__ __forceinline__ void stockham_stage(const Cx<R>* in, Cx<R>* out, int n, int T, int Ns,
+                                               const R2<R>* __restrict__ tw, R s) {
+    const int nb = n / RAD;
+    const int twstep = n / (Ns * RAD);
+    for (int b = threadIdx.x; b < T * nb; b += blockDim.x) {
+        const int t = b / nb, j = b - t * nb;
+        const Cx<R>* li = in + t * n;
+        Cx<R>* lo = out + t * n;
+        const int kk = j % Ns;
+        Cx<R> v[RAD];
+#pragma unroll
+        for (int r = 0; r < RAD; r++) {
+            Cx<R> a = li[j + r * nb];
+            if (r > 0 && kk > 0) {
+                // table holds exp(-2 pi i k / n); exp(s 2 pi i k / n) = (w.x, -s w.y)
+                const R2<R> w = tw[kk * r * twstep];
+                a = cmul(a, Cx<R>{w.x, -s * w.y});
+            }
+            v[r] = a;
+        }
+        dft_small<R, RAD>(v, s);
+        const int o = (j / Ns) * Ns * RAD + kk;
+#pragma unroll
+        for (int r = 0; r < RAD; r++) lo[o + r * Ns] = v[r];
+    }
+}
+
+// mode 0: lines along z (contiguous), lines enumerated by (x,y)
+// mode 1: lines along y, batches of consecutive z for one x
+// mode 2: lines along x, batches of consecutive z for one y
+template <class R>
+__global__ __launch_bounds__(256) void k_fft_lines(R2<R>* data, int nx, int ny, int nz, int mode, int T,
+                                                    const R2<R>* __restrict__ tw, int nrad, int4 radA, int4 radB,
+                                                    R s) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    Cx<R>* bufA = reinterpret_cast<Cx<R>*>(smem_raw);
+    const int n = mode == 0 ? nz : (mode == 1 ? ny : nx);
+    Cx<R>* bufB = bufA + (size_t)T * n;
+    size_t base;
+    size_t es, ts;
+    int count;
+    if (mode == 0) {
+        const size_t l0 = (size_t)blockIdx.x * T;
+        const size_t nl = (size_t)nx * ny;
+        count = (int)min((size_t)T, nl - l0);
+        base = l0 * nz;
+        es = 1;
+        ts = nz;
+    } else {
+        const int zb = (nz + T - 1) / T;
+        const int o = blockIdx.x / zb;
+        const int z0 = (blockIdx.x - o * zb) * T;
+        count = min(T, nz - z0);
+        if (mode == 1) {
+            base = (size_t)o * ny * nz + z0;
+            es = nz;
+        } else {
+            base = (size_t)o * nz + z0;
+            es = (size_t)ny * nz;
+        }
+        ts = 1;
+    }
+    const int total = T * n;
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+        int t, k;
+        if (mode == 0) { t = idx / n; k = idx - t * n; } else { k = idx / T; t = idx - k * T; }
+        Cx<R> v = {R(0), R(0)};
+        if (t < count) {
+            const R2<R> a = data[base + (size_t)t * ts + (size_t)k * es];
+            v.x = a.x;
+            v.y = a.y;
+        }
+        bufA[t * n + k] = v;
+    }
+    __syncthreads();
+    int rads[8] = {radA.x, radA.y, radA.z, radA.w, radB.x, radB.y, radB.z, radB.w};
+    int Ns = 1;
+    Cx<R>* in = bufA;
+    Cx<R>* out = bufB;
+    for (int st = 0; st < nrad; st++) {
+        const int rad = rads[st];
+        if (rad == 4) stockham_stage<R, 4>(in, out, n, T, Ns, tw, s);
+        else if (rad == 2) stockham_stage<R, 2>(in, out, n, T, Ns, tw, s);
+        else if (rad == 3) stockham_stage<R, 3>(in, out, n, T, Ns, tw, s);
+        else stockham_stage<R, 5>(in, out, n, T, Ns, tw, s);
+        Ns *= rad;
+        __syncthreads();
+        Cx<R>* tmp = in;
+        in = out;
+        out = tmp;
+    }
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+        int t, k;
+        if (mode == 0) { t = idx / n; k = idx - t * n; } else { k = idx / T; t = idx - k * T; }
+        if (t < count) {
+            const Cx<R> v = in[t * n + k];
+            data[base + (size_t)t * ts + (size_t)k * es] = R2<R>{v.x, v.y};
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+template <class R>
+void launch_spread_fixed(hipStream_t s, int n, const R4<R>* pos, const R* lab, PmeGeom<R> g, R2<R>* grid) {
+    hipLaunchKernelGGL(k_spread_fixed<R>, dim3((n + 255) / 256), dim3(256), 0, s, n, pos, lab, g, grid);
+}
+template <class R>
+void launch_spread_induced(hipStream_t s, int n, const R4<R>* pos, const R* mu, PmeGeom<R> g, R2<R>* grid) {
+    hipLaunchKernelGGL(k_spread_induced<R>, dim3((n + 255) / 256), dim3(256), 0, s, n, pos, mu, g, grid);
+}
+void launch_tile_ranges(hipStream_t s, int n, const unsigned* keys_sorted, int shift, int* tstart, int* tend) {
+    hipLaunchKernelGGL(k_tile_ranges, dim3((n + 255) / 256), dim3(256), 0, s, n, keys_sorted, shift, tstart, tend);
+}
+
+template <class R, int MODE> static size_t tiled_lds(const TileGeom& tg) {
+    constexpr int ND = MODE == 0 ? 3 : 1;
+    constexpr int NC = MODE == 0 ? 20 : 3;
+    return (size_t)tg.W[0] * tg.W[1] * tg.W[2] * sizeof(R) + 32 * (3 * kOrder * (ND + 1) + NC) * sizeof(R) +
+           32 * 3 * sizeof(int);
+}
+size_t spread_tiled_lds_bytes(const TileGeom& tg, int double_precision) {
+    return double_precision ? tiled_lds<double, 0>(tg) : tiled_lds<float, 0>(tg);
+}
+
+template <class R>
+void launch_spread_fixed_tiled(hipStream_t s, const R4<R>* pos, const R* lab, PmeGeom<R> g, TileGeom tg,
+                               const int* tstart, const int* tend, R* scratch, R2<R>* grid) {
+    const size_t shm = tiled_lds<R, 0>(tg);
+    k_spread_tiled<R, 0><<<dim3(tg.ntiles), dim3(256), shm, s>>>(pos, lab, g, tg, tstart, tend, scratch, grid);
+}
+template <class R>
+void launch_spread_induced_tiled(hipStream_t s, const R4<R>* pos, const R* mu, PmeGeom<R> g, TileGeom tg,
+                                 const int* tstart, const int* tend, R* scratch, R2<R>* grid) {
+    const size_t shm = tiled_lds<R, 1>(tg);
+    k_spread_tiled<R, 1><<<dim3(tg.ntiles), dim3(256), shm, s>>>(pos, mu, g, tg, tstart, tend, scratch, grid);
+}
+template <class R> void launch_fold(hipStream_t s, PmeGeom<R> g, TileGeom tg, const R* scratch, R2<R>* grid) {
+    size_t total = (size_t)g.n[0] * g.n[1] * g.n[2];
+    int blocks = (int)std::min<size_t>((total + 255) / 256, 16384);
+    hipLaunchKernelGGL(k_fold<R>, dim3(blocks), dim3(256), 0, s, g, tg, scratch, grid);
+}
+
+template <class R>
+void launch_convolve(hipStream_t s, PmeGeom<R> g, const R* moduli, R alpha, R2<R>* grid) {
+    size_t total = (size_t)g.n[0] * g.n[1] * g.n[2];
+    int blocks = (int)std::min<size_t>((total + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_convolve<R>, dim3(blocks), dim3(256), 0, s, g, moduli, alpha, grid);
+}
+template <class R>
+void launch_gather_phi(hipStream_t s, int n, const R4<R>* pos, PmeGeom<R> g, const R2<R>* grid, R* phi) {
+    hipLaunchKernelGGL(k_gather_phi<R>, dim3((n + 127) / 128), dim3(128), 0, s, n, pos, g, grid, phi);
+}
+template <class R>
+void launch_gather_field(hipStream_t s, int n, const R4<R>* pos, PmeGeom<R> g, const R2<R>* grid, R* field) {
+    hipLaunchKernelGGL(k_gather_field<R>, dim3((n + 255) / 256), dim3(256), 0, s, n, pos, g, grid, field);
+}
+
+template <class R> void launch_fft3d(hipStream_t s, const FftPlan<R>& plan, R2<R>* grid, int sign) {
+    const int nx = plan.n[0], ny = plan.n[1], nz = plan.n[2];
+    const R sg = (R)sign;
+    // pass order z, y, x
+    for (int pass = 0; pass < 3; pass++) {
+        const int axis = 2 - pass;  // 2 = z, 1 = y, 0 = x
+        const int mode = pass;      // 0: z lines, 1: y lines, 2: x lines
+        const int n = plan.n[axis];
+        if (n == 1) continue;
+        int T = (int)(32768 / (2 * (size_t)n * sizeof(R2<R>)));
+        if (T > 64) T = 64;
+        if (T < 1) T = 1;
+        int blocks;
+        if (mode == 0) {
+            blocks = (int)(((size_t)nx * ny + T - 1) / T);
+        } else {
+            const int zb = (nz + T - 1) / T;
+            blocks = (mode == 1 ? nx : ny) * zb;
+        }
+        const int* r = plan.radix[axis];
+        int4 ra = make_int4(r[0], r[1], r[2], r[3]);
+        int4 rb = make_int4(r[4], r[5], r[6], r[7]);
+        size_t shm = 2 * (size_t)T * n * sizeof(R2<R>);
+        hipLaunchKernelGGL(k_fft_lines<R>, dim3(blocks), dim3(256), shm, s, grid, nx, ny, nz, mode, T, plan.tw[axis],
+                           plan.nradix[axis], ra, rb, sg);
+    }
+}
+
+#define MPID_INSTANTIATE(R)                                                                                   \
+    template void launch_spread_fixed<R>(hipStream_t, int, const R4<R>*, const R*, PmeGeom<R>, R2<R>*);       \
+    template void launch_spread_induced<R>(hipStream_t, int, const R4<R>*, const R*, PmeGeom<R>, R2<R>*);     \
+    template void launch_convolve<R>(hipStream_t, PmeGeom<R>, const R*, R, R2<R>*);                          \
+    template void launch_gather_phi<R>(hipStream_t, int, const R4<R>*, PmeGeom<R>, const R2<R>*, R*);        \
+    template void launch_gather_field<R>(hipStream_t, int, const R4<R>*, PmeGeom<R>, const R2<R>*, R*);      \
+    template void launch_fft3d<R>(hipStream_t, const FftPlan<R>&, R2<R>*, int);                              \
+    template void launch_spread_fixed_tiled<R>(hipStream_t, const R4<R>*, const R*, PmeGeom<R>, TileGeom,    \
+                                               const int*, const int*, R*, R2<R>*);                          \
+    template void launch_spread_induced_tiled<R>(hipStream_t, const R4<R>*, const R*, PmeGeom<R>, TileGeom,  \
+                                                 const int*, const int*, R*, R2<R>*);                        \
+    template void launch_fold<R>(hipStream_t, PmeGeom<R>, TileGeom, const R*, R2<R>*);
+MPID_INSTANTIATE(float)
+MPID_INSTANTIATE(double)
+#undef MPID_INSTANTIATE
+
+}  // namespace mpid

@@ -1,0 +1,1060 @@
+// Host orchestration and C ABI of the MPIDForce gfx950 evaluator.
+//
+// One mpid_context plays the role of the reference's platform kernel object
+// (CalcMPIDForceKernel, openmmapi/include/openmm/mpidKernels.h:50-104):
+// mpid_create ~ initialize, mpid_execute ~ execute.  The evaluation sequence
+// follows MPIDReferencePmeForce::calculateForceAndEnergy
+// (platforms/reference/src/SimTKReference/MPIDReferenceForce.cpp:2193-2270):
+// lab-frame rotation, permanent field (reciprocal + self + real space),
+// induced dipoles, then real-space and reciprocal energies, forces and torques,
+// and finally the torque-to-force chain rule.
+//
+// Device data live in Morton-sorted atom order for the whole evaluation; the
+// permutation is rebuilt from the positions on every call.  The mutual
+// induced dipoles are found by conjugate gradients on the symmetrised
+// operator I - S T S (S = alpha^(1/2) per site), which has the same solution
+// as the reference's DIIS iteration and is stopped by the same RMS-Debye
+// criterion (MPIDReferenceForce.cpp:1219).
+#include "engine.h"
+#include "../../include/mpid_hip.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+using namespace mpid;
+
+namespace {
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct MpidError : std::runtime_error {
+    int code;
+    MpidError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIP_CHECK(call)                                                                                         \
+    do {                                                                                                        \
+        hipError_t e_ = (call);                                                                                 \
+        if (e_ != hipSuccess)                                                                                   \
+            throw HipError(std::string("HIP error ") + hipGetErrorString(e_) + " at " + __FILE__ + ":" +        \
+                           std::to_string(__LINE__));                                                           \
+    } while (0)
+
+// Owning device buffer.
+template <class T> struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void resize(size_t count) {
+        if (count == n && p) return;
+        release();
+        if (count == 0) return;
+        HIP_CHECK(hipMalloc((void**)&p, count * sizeof(T)));
+        n = count;
+    }
+    void upload(const T* h, size_t count, hipStream_t s) {
+        resize(count);
+        HIP_CHECK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+};
+
+// B-spline moduli |b(k)|^2 with the Euler-spline fix and the optimal-zeta
+// correction (initializeBSplineModuli, MPIDReferenceForce.cpp:2720-2810).
+std::vector<double> bspline_moduli(int n) {
+    // order-6 cardinal B-spline at the integers 1..5 from the Cox-de Boor recursion at w = 0
+    double c[7][7] = {};
+    c[1][0] = 1.0;
+    const double w = 0.0;
+    for (int p = 2; p <= kOrder; p++)
+        for (int m = 0; m < p; m++) {
+            double left = m >= 1 ? c[p - 1][m - 1] : 0.0, right = m <= p - 2 ? c[p - 1][m] : 0.0;
+            c[p][m] = ((w + (p - 1 - m)) * left + (m + 1 - w) * right) / (p - 1);
+        }
+    std::vector<double> b(std::max(n, kOrder + 1), 0.0), mod(n);
+    for (int m = 1; m <= kOrder; m++) b[m] = c[kOrder][m - 1];
+    for (int i = 0; i < n; i++) {
+        double s1 = 0, s2 = 0;
+        for (int m = 0; m < n; m++) {
+            double arg = 2.0 * M_PI * i * m / n;
+            s1 += b[m] * std::cos(arg);
+            s2 += b[m] * std::sin(arg);
+        }
+        mod[i] = s1 * s1 + s2 * s2;
+    }
+    const double eps = 1e-7;
+    if (mod[0] < eps) mod[0] = 0.5 * mod[1];
+    for (int i = 1; i < n - 1; i++)
+        if (mod[i] < eps) mod[i] = 0.5 * (mod[i - 1] + mod[i + 1]);
+    if (mod[n - 1] < eps) mod[n - 1] = 0.5 * mod[n - 2];
+    for (int i = 1; i <= n; i++) {
+        int k = i - 1;
+        if (i > n / 2) k -= n;
+        double zeta = 1.0;
+        if (k != 0) {
+            double s1 = 1.0, s2 = 1.0, f = M_PI * k / n;
+            for (int j = 1; j <= 50; j++) {
+                double a = f / (f + M_PI * j);
+                s1 += std::pow(a, kOrder);
+                s2 += std::pow(a, 2 * kOrder);
+            }
+            for (int j = 1; j <= 50; j++) {
+                double a = f / (f - M_PI * j);
+                s1 += std::pow(a, kOrder);
+                s2 += std::pow(a, 2 * kOrder);
+            }
+            zeta = s2 / s1;
+        }
+        mod[i - 1] *= zeta * zeta;
+    }
+    return mod;
+}
+
+bool factor_radices(int n, int* out, int& count) {
+    count = 0;
+    while (n % 4 == 0) { out[count++] = 4; n /= 4; }
+    while (n % 2 == 0) { out[count++] = 2; n /= 2; }
+    while (n % 3 == 0) { out[count++] = 3; n /= 3; }
+    while (n % 5 == 0) { out[count++] = 5; n /= 5; }
+    return n == 1 && count <= 8;
+}
+
+enum ProfCat {
+    P_SORT = MPID_PROF_SORT, P_ROTATE = MPID_PROF_ROTATE, P_TILES = MPID_PROF_TILES, P_SPREAD = MPID_PROF_SPREAD,
+    P_FFT = MPID_PROF_FFT, P_CONV = MPID_PROF_CONVOLVE, P_GATHER = MPID_PROF_GATHER,
+    P_FIXED = MPID_PROF_FIXED_PAIRS, P_INDUCED = MPID_PROF_INDUCED_PAIRS, P_ELEC = MPID_PROF_ELECTROSTATIC_PAIRS,
+    P_RECIP = MPID_PROF_RECIP_ATOMS, P_SOLVER = MPID_PROF_SOLVER, P_OTHER = MPID_PROF_OTHER
+};
+
+// Precision-specific device state.
+template <class R> struct Prec {
+    DBuf<R4<R>> pos;
+    DBuf<R> lab;
+    DBuf<R2<R>> td;
+    DBuf<R> mu;       // induced dipoles in this precision (matvec input)
+    DBuf<R> erec;     // reciprocal induced field
+    DBuf<R> phi, phidp;
+    DBuf<R2<R>> grid;
+    DBuf<R> moduli;
+    DBuf<R2<R>> tw[3];
+    DBuf<R> scratch;  // spreading windows of the Morton tiles
+    FftPlan<R> plan;
+    bool ready = false;
+};
+
+}  // namespace
+
+struct mpid_context {
+    int n = 0;
+    mpid_options opt{};
+    hipStream_t stream = nullptr;
+    std::string error;
+    mpid_stats stats{};
+
+    // static parameters (original order)
+    DBuf<double> mol, thole, pol;
+    DBuf<int4> axes;
+    DBuf<int> partners, npart;
+    std::vector<double> h_charges;
+
+    // sorted per-evaluation state
+    DBuf<double> pos_in;
+    DBuf<unsigned> keys, keys_sorted;
+    DBuf<int> vals, perm, inv;
+    DBuf<unsigned char> sort_tmp;
+    DBuf<double4> spos;
+    DBuf<double> smol, sthole, spol, lab, labpol, sqrtpol, damp;
+    DBuf<int4> saxes;
+    DBuf<int> spart, snpart, aniso;
+    // Morton-aligned PME spreading tiles
+    TileGeom tgeom{};
+    bool tiled_spread = false;
+    DBuf<int> tstart, tend;
+    // tiles
+    DBuf<float4> bmin, bmax;
+    DBuf<int2> tiles;
+    DBuf<int> tile_count;
+    int ntiles = 0;
+    // accumulators
+    DBuf<acc_t> field_acc, force_acc, torque_acc, energy_acc;
+    // solver
+    DBuf<double> E0, b, x, r, p, Ap, dx, rt, mud, partial, scal, tmp3n, fout;
+    // precisions
+    Prec<float> pf;
+    Prec<double> pd;
+
+    // last evaluation
+    std::vector<double> last_pos, last_box;
+    bool have_last = false;
+    bool mu_valid = false;
+    double energy_correction = 0.0;
+
+    // profiling
+    bool profiling = false;
+    struct Ev { int cat; hipEvent_t a, b; };
+    std::vector<Ev> events;
+    std::vector<hipEvent_t> pool;
+    size_t pool_used = 0;
+    double prof_ms[MPID_PROF_COUNT] = {};
+    int prof_count[MPID_PROF_COUNT] = {};
+
+    ~mpid_context() {
+        for (auto e : pool) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    hipEvent_t get_event() {
+        if (pool_used == pool.size()) {
+            hipEvent_t e;
+            HIP_CHECK(hipEventCreate(&e));
+            pool.push_back(e);
+        }
+        return pool[pool_used++];
+    }
+    template <class F> void timed(int cat, F&& f) {
+        if (!profiling) {
+            f();
+            return;
+        }
+        hipEvent_t a = get_event(), b = get_event();
+        HIP_CHECK(hipEventRecord(a, stream));
+        f();
+        HIP_CHECK(hipEventRecord(b, stream));
+        events.push_back({cat, a, b});
+    }
+    void collect_profile() {
+        if (!profiling) return;
+        HIP_CHECK(hipStreamSynchronize(stream));
+        for (auto& e : events) {
+            float ms = 0.f;
+            HIP_CHECK(hipEventElapsedTime(&ms, e.a, e.b));
+            prof_ms[e.cat] += ms;
+            prof_count[e.cat] += 1;
+        }
+        events.clear();
+        pool_used = 0;
+    }
+    void check_launch() { HIP_CHECK(hipGetLastError()); }
+
+    // ---- set-up --------------------------------------------------------------
+    void set_parameters(const mpid_parameters* P, bool first) {
+        if (!P || P->n <= 0) throw MpidError(MPID_ERR_INVALID, "MPIDForce: no particles");
+        if (!first && P->n != n)
+            throw MpidError(MPID_ERR_INVALID, "updateParametersInContext: The number of multipoles has changed");
+        n = P->n;
+        // validation, MPIDForceImpl.cpp:51-150
+        for (int i = 0; i < n; i++) {
+            const double* q = P->quadrupoles + 6 * i;
+            const double* o = P->octopoles + 10 * i;
+            double tr = std::fabs(q[0] + q[2] + q[5]);
+            if (tr > 1e-5)
+                throw MpidError(MPID_ERR_INVALID, "MPIDForce: quadrupole for particle=" + std::to_string(i) +
+                                                      " has nonzero trace: " + std::to_string(tr) +
+                                                      "; MPID plugin assumes traceless quadrupole.");
+            if (std::fabs(o[0] + o[2] + o[7]) > 1e-5 || std::fabs(o[1] + o[3] + o[8]) > 1e-5 ||
+                std::fabs(o[4] + o[6] + o[9]) > 1e-5)
+                throw MpidError(MPID_ERR_INVALID, "MPIDForce: octopole for particle=" + std::to_string(i) +
+                                                      " has nonzero trace; MPID plugin assumes traceless octopoles.");
+            int t = P->axis_types[i];
+            if (t < 0 || t > 5)
+                throw MpidError(MPID_ERR_INVALID, "MPIDForce: axis type=" + std::to_string(t) + " not currently handled");
+            if (t != 5 && (P->atom_z[i] < 0 || P->atom_z[i] >= n))
+                throw MpidError(MPID_ERR_INVALID, "MPIDForce: invalid z axis particle: " + std::to_string(P->atom_z[i]));
+            if (t != 5 && t != 4 && (P->atom_x[i] < 0 || P->atom_x[i] >= n))
+                throw MpidError(MPID_ERR_INVALID, "MPIDForce: invalid x axis particle: " + std::to_string(P->atom_x[i]));
+            if ((t == 2 || t == 3) && (P->atom_y[i] < 0 || P->atom_y[i] >= n))
+                throw MpidError(MPID_ERR_INVALID, "MPIDForce: invalid y axis particle: " + std::to_string(P->atom_y[i]));
+            for (int k = 0; k < 3; k++)
+                if (!(P->polarity[3 * i + k] >= 0.0))
+                    throw MpidError(MPID_ERR_INVALID, "MPIDForce: negative polarizability for particle " + std::to_string(i));
+        }
+        // molecular-frame records in the internal orderings
+        std::vector<double> m((size_t)n * kMP);
+        std::vector<int4> ax(n);
+        for (int i = 0; i < n; i++) {
+            double* r = &m[(size_t)i * kMP];
+            const double* q = P->quadrupoles + 6 * i;
+            const double* o = P->octopoles + 10 * i;
+            r[0] = P->charges[i];
+            r[1] = P->dipoles[3 * i];
+            r[2] = P->dipoles[3 * i + 1];
+            r[3] = P->dipoles[3 * i + 2];
+            // XX XY YY XZ YZ ZZ -> XX XY XZ YY YZ ZZ  (MPIDReferenceForce.cpp:305-310)
+            r[4] = q[0]; r[5] = q[1]; r[6] = q[3]; r[7] = q[2]; r[8] = q[4]; r[9] = q[5];
+            // XXX XXY XYY YYY XXZ XYZ YYZ XZZ YZZ ZZZ -> XXX XXY XXZ XYY XYZ XZZ YYY YYZ YZZ ZZZ (:312-321)
+            r[10] = o[0]; r[11] = o[1]; r[12] = o[4]; r[13] = o[2]; r[14] = o[5];
+            r[15] = o[7]; r[16] = o[3]; r[17] = o[6]; r[18] = o[8]; r[19] = o[9];
+            int t = P->axis_types[i];
+            ax[i] = make_int4(t == 5 ? -1 : P->atom_z[i], (t == 5 || t == 4) ? -1 : P->atom_x[i],
+                              (t == 5 || t == 4) ? -1 : P->atom_y[i], t);
+        }
+        // covalent scale classes: the scale of the pair (i<j) comes from the
+        // lists of i, later list types overwrite earlier ones; only classes
+        // with a scale other than 1 are stored, for both atoms of the pair
+        // (setupScaleMaps / getMultipoleScaleFactor, MPIDReferenceForce.cpp:190-241).
+        std::vector<std::vector<std::pair<int, int>>> plist(n);
+        for (int a = 0; a < n; a++) {
+            std::vector<std::pair<int, int>> mine;  // (partner, class) with partner > a
+            for (int t = 0; t < 4; t++)
+                for (int k = P->covalent_offsets[4 * a + t]; k < P->covalent_offsets[4 * a + t + 1]; k++) {
+                    int bidx = P->covalent_atoms[k];
+                    if (bidx < 0 || bidx >= n)
+                        throw MpidError(MPID_ERR_INVALID, "MPIDForce: covalent atom index out of range");
+                    if (bidx <= a) continue;
+                    bool found = false;
+                    for (auto& e : mine)
+                        if (e.first == bidx) { e.second = t; found = true; }
+                    if (!found) mine.push_back({bidx, t});
+                }
+            for (auto& e : mine) {
+                int cls;
+                if (e.second <= 1) cls = kScaleZero;            // 1-2, 1-3: scale 0
+                else if (e.second == 2) cls = kScale14;         // 1-4: scale14
+                else continue;                                  // 1-5: scale 1
+                plist[a].push_back({e.first, cls});
+                plist[e.first].push_back({a, cls});
+            }
+        }
+        std::vector<int> part((size_t)n * kMaxPartners, 0), np(n);
+        for (int a = 0; a < n; a++) {
+            if ((int)plist[a].size() > kMaxPartners)
+                throw MpidError(MPID_ERR_UNSUPPORTED, "MPIDForce: more than " + std::to_string(kMaxPartners) +
+                                                          " scaled covalent partners on particle " + std::to_string(a));
+            np[a] = (int)plist[a].size();
+            for (int k = 0; k < np[a]; k++) part[(size_t)a * kMaxPartners + k] = (plist[a][k].first << 2) | plist[a][k].second;
+        }
+        std::vector<double> th(P->tholes, P->tholes + n), po(P->polarity, P->polarity + 3 * n);
+        mol.upload(m.data(), m.size(), stream);
+        axes.upload(ax.data(), ax.size(), stream);
+        thole.upload(th.data(), th.size(), stream);
+        pol.upload(po.data(), po.size(), stream);
+        partners.upload(part.data(), part.size(), stream);
+        npart.upload(np.data(), np.size(), stream);
+        HIP_CHECK(hipStreamSynchronize(stream));
+        have_last = false;
+        mu_valid = false;
+    }
+
+    void set_options(const mpid_options* o) {
+        if (!o) throw MpidError(MPID_ERR_INVALID, "missing options");
+        opt = *o;
+        if (opt.nonbonded_method != MPID_NONBONDED_PME)
+            throw MpidError(MPID_ERR_UNSUPPORTED, "only the PME nonbonded method is implemented on this platform");
+        if (opt.polarization != MPID_POLARIZATION_MUTUAL && opt.polarization != MPID_POLARIZATION_DIRECT)
+            throw MpidError(MPID_ERR_UNSUPPORTED, "only Mutual and Direct polarization are implemented on this platform");
+        if (!(opt.ewald_alpha > 0.0))
+            throw MpidError(MPID_ERR_INVALID, "the Ewald alpha must be given explicitly (setPMEParameters)");
+        if (!(opt.cutoff > 0.0)) throw MpidError(MPID_ERR_INVALID, "invalid cutoff");
+        if (opt.precision != MPID_PRECISION_MIXED && opt.precision != MPID_PRECISION_DOUBLE)
+            throw MpidError(MPID_ERR_INVALID, "invalid precision");
+        if (opt.max_iterations < 0) throw MpidError(MPID_ERR_INVALID, "invalid iteration limit");
+        if (opt.flags & ~MPID_FLAG_CONSERVATIVE_TORQUES) throw MpidError(MPID_ERR_INVALID, "unknown option flags");
+        for (int k = 0; k < 3; k++) {
+            int rr[16], c;
+            if (opt.grid[k] < kOrder || opt.grid[k] > 1024 || !factor_radices(opt.grid[k], rr, c))
+                throw MpidError(MPID_ERR_UNSUPPORTED, "PME grid dimension " + std::to_string(opt.grid[k]) +
+                                                          " is not of the form 2^a 3^b 5^c between 6 and 1024");
+        }
+    }
+
+    template <class R> void init_prec(Prec<R>& P) {
+        if (P.ready) return;
+        const size_t G = (size_t)opt.grid[0] * opt.grid[1] * opt.grid[2];
+        P.pos.resize(n);
+        P.lab.resize((size_t)n * kMP);
+        P.td.resize(n);
+        P.mu.resize(3 * (size_t)n);
+        P.erec.resize(3 * (size_t)n);
+        P.phi.resize(35 * (size_t)n);
+        P.phidp.resize(35 * (size_t)n);
+        P.grid.resize(G);
+        std::vector<R> mods;
+        for (int k = 0; k < 3; k++) {
+            std::vector<double> m = bspline_moduli(opt.grid[k]);
+            for (double v : m) mods.push_back((R)v);
+            const int nk = opt.grid[k];
+            std::vector<R2<R>> tw(nk);
+            for (int j = 0; j < nk; j++) {
+                double ang = -2.0 * M_PI * j / nk;
+                tw[j].x = (R)std::cos(ang);
+                tw[j].y = (R)std::sin(ang);
+            }
+            P.tw[k].upload(tw.data(), tw.size(), stream);
+            P.plan.n[k] = nk;
+            P.plan.tw[k] = P.tw[k].p;
+            for (int j = 0; j < 16; j++) P.plan.radix[k][j] = 0;
+            factor_radices(nk, P.plan.radix[k], P.plan.nradix[k]);
+        }
+        P.moduli.upload(mods.data(), mods.size(), stream);
+        if (tiled_spread) P.scratch.resize((size_t)tgeom.ntiles * tgeom.W[0] * tgeom.W[1] * tgeom.W[2]);
+        HIP_CHECK(hipStreamSynchronize(stream));
+        P.ready = true;
+    }
+
+    void alloc_state() {
+        const size_t n3 = 3 * (size_t)n;
+        pos_in.resize(n3);
+        keys.resize(n);
+        keys_sorted.resize(n);
+        vals.resize(n);
+        perm.resize(n);
+        inv.resize(n);
+        sort_tmp.resize(std::max<size_t>(sort_temp_bytes(n), 1));
+        spos.resize(n);
+        smol.resize((size_t)n * kMP);
+        saxes.resize(n);
+        sthole.resize(n);
+        spol.resize(n3);
+        spart.resize((size_t)n * kMaxPartners);
+        snpart.resize(n);
+        lab.resize((size_t)n * kMP);
+        labpol.resize(6 * (size_t)n);
+        sqrtpol.resize(6 * (size_t)n);
+        damp.resize(n);
+        aniso.resize(n);
+        const int nb = (n + kTile - 1) / kTile;
+        bmin.resize(nb);
+        bmax.resize(nb);
+        tiles.resize((size_t)nb * (nb + 1) / 2);
+        tile_count.resize(1);
+        field_acc.resize(n3);
+        force_acc.resize(n3);
+        torque_acc.resize(n3);
+        energy_acc.resize(1);
+        E0.resize(n3);
+        b.resize(n3);
+        x.resize(n3);
+        r.resize(n3);
+        p.resize(n3);
+        Ap.resize(n3);
+        dx.resize(n3);
+        rt.resize(n3);
+        mud.resize(n3);
+        tmp3n.resize(n3);
+        fout.resize(n3);
+        partial.resize(2 * 512);
+        scal.resize(8);
+        choose_tiles();
+        if (opt.precision == MPID_PRECISION_MIXED) init_prec(pf);
+        init_prec(pd);  // FP64 path (also the refinement residual of the mixed path)
+    }
+
+    // Largest Morton level m with n_k = 2^m T_k, T_k >= 6, and a window that fits in LDS.
+    void choose_tiles() {
+        tiled_spread = false;
+        for (int m = 10; m >= 1; m--) {
+            const int a = 1 << m;
+            bool ok = true;
+            for (int k = 0; k < 3; k++)
+                if (opt.grid[k] % a != 0 || opt.grid[k] / a < 6) ok = false;
+            if (!ok) continue;
+            TileGeom tg{};
+            tg.m = m;
+            tg.tiles_per_axis = a;
+            tg.ntiles = a * a * a;
+            for (int k = 0; k < 3; k++) {
+                tg.T[k] = opt.grid[k] / a;
+                tg.W[k] = std::min(tg.T[k] + 8, opt.grid[k]);
+            }
+            if (spread_tiled_lds_bytes(tg, 1) > 65536) continue;
+            tgeom = tg;
+            tiled_spread = true;
+            break;
+        }
+        if (tiled_spread) {
+            tstart.resize(tgeom.ntiles);
+            tend.resize(tgeom.ntiles);
+        }
+    }
+
+    // ---- evaluation ------------------------------------------------------------
+    Box<double> dbox;
+    double box_vec[9];
+
+    void set_box(const double* boxv) {
+        if (!boxv) throw MpidError(MPID_ERR_INVALID, "PME requires periodic box vectors");
+        for (int k = 0; k < 9; k++) box_vec[k] = boxv[k];
+        if (boxv[1] != 0 || boxv[2] != 0 || boxv[3] != 0 || boxv[5] != 0 || boxv[6] != 0 || boxv[7] != 0)
+            throw MpidError(MPID_ERR_UNSUPPORTED, "only rectangular periodic boxes are supported on this platform");
+        for (int k = 0; k < 3; k++) {
+            double L = boxv[4 * k];
+            if (!(L > 0.0)) throw MpidError(MPID_ERR_INVALID, "Box size of zero is invalid.");
+            // MPIDReferenceKernels.cpp:193-197 and MPIDForceImpl.cpp:61-66
+            if (L < 1.999999 * opt.cutoff)
+                throw MpidError(MPID_ERR_INVALID,
+                                "The periodic box size has decreased to less than twice the nonbonded cutoff.");
+            dbox.L[k] = L;
+            dbox.inv[k] = 1.0 / L;
+        }
+    }
+    template <class R> Box<R> rbox() const {
+        Box<R> bx;
+        for (int k = 0; k < 3; k++) {
+            bx.L[k] = (R)dbox.L[k];
+            bx.inv[k] = (R)dbox.inv[k];
+        }
+        return bx;
+    }
+    template <class R> PmeGeom<R> geom() const {
+        PmeGeom<R> g;
+        for (int k = 0; k < 3; k++) {
+            g.n[k] = opt.grid[k];
+            g.gscale[k] = (R)(opt.grid[k] * dbox.inv[k]);
+        }
+        g.box = rbox<R>();
+        return g;
+    }
+    template <class R> PairParams<R> pair_params() const {
+        PairParams<R> P;
+        P.box = rbox<R>();
+        P.n = n;
+        P.rc2 = (R)(opt.cutoff * opt.cutoff);
+        P.alpha = (R)opt.ewald_alpha;
+        double a = opt.ewald_alpha, aln = 1.0 / (kSqrtPi * a);
+        P.aln[0] = (R)0;
+        for (int k = 1; k <= 4; k++) {
+            aln *= 2.0 * a * a;
+            P.aln[k] = (R)aln;
+        }
+        P.default_thole = (R)opt.default_thole;
+        P.scale14 = (R)opt.scale14;
+        P.mutual = opt.polarization == MPID_POLARIZATION_MUTUAL ? 1 : 0;
+        P.fix_torque = (opt.flags & MPID_FLAG_CONSERVATIVE_TORQUES) ? 1 : 0;
+        return P;
+    }
+    template <class R> PairAtoms<R> pair_atoms(Prec<R>& P) const {
+        PairAtoms<R> A;
+        A.pos = P.pos.p;
+        A.lab = P.lab.p;
+        A.td = P.td.p;
+        A.aniso = aniso.p;
+        A.partners = spart.p;
+        A.npart = snpart.p;
+        return A;
+    }
+    double selfc() const { return (4.0 / 3.0) * opt.ewald_alpha * opt.ewald_alpha * opt.ewald_alpha / kSqrtPi; }
+    double3 gscale3() const {
+        return make_double3(opt.grid[0] * dbox.inv[0], opt.grid[1] * dbox.inv[1], opt.grid[2] * dbox.inv[2]);
+    }
+
+    // positions (device, original order) -> sorted lab-frame state
+    void prepare(const double* d_pos) {
+        timed(P_SORT, [&] {
+            launch_morton_keys(stream, n, d_pos, dbox, keys.p, vals.p);
+            int rc = launch_sort_pairs(stream, sort_tmp.p, sort_tmp.n, keys.p, keys_sorted.p, vals.p, perm.p, n);
+            if (rc != 0) throw HipError("rocprim radix sort failed");
+            launch_invert_perm(stream, n, perm.p, inv.p);
+            launch_gather_sorted(stream, n, perm.p, inv.p, d_pos, dbox, mol.p, axes.p, thole.p, pol.p, partners.p,
+                                 npart.p, spos.p, smol.p, saxes.p, sthole.p, spol.p, spart.p, snpart.p);
+        });
+        if (tiled_spread) {
+            timed(P_SORT, [&] {
+                HIP_CHECK(hipMemsetAsync(tstart.p, 0, tstart.n * sizeof(int), stream));
+                HIP_CHECK(hipMemsetAsync(tend.p, 0, tend.n * sizeof(int), stream));
+                launch_tile_ranges(stream, n, keys_sorted.p, 3 * (10 - tgeom.m), tstart.p, tend.p);
+            });
+        }
+        timed(P_ROTATE, [&] {
+            launch_rotate(stream, n, dbox, spos.p, saxes.p, smol.p, spol.p, lab.p, labpol.p, sqrtpol.p, damp.p,
+                          aniso.p);
+            if (opt.precision == MPID_PRECISION_MIXED)
+                launch_convert<float>(stream, n, spos.p, lab.p, sthole.p, damp.p, pf.pos.p, pf.lab.p, pf.td.p);
+            launch_convert<double>(stream, n, spos.p, lab.p, sthole.p, damp.p, pd.pos.p, pd.lab.p, pd.td.p);
+        });
+        check_launch();
+        // tiles
+        const int nb = (n + kTile - 1) / kTile;
+        timed(P_TILES, [&] {
+            launch_block_bounds<double>(stream, n, pd.pos.p, bmin.p, bmax.p);
+            HIP_CHECK(hipMemsetAsync(tile_count.p, 0, sizeof(int), stream));
+            launch_build_tiles(stream, nb, bmin.p, bmax.p, rbox<float>(), (float)opt.cutoff, tiles.p, tile_count.p,
+                               (int)tiles.n);
+        });
+        check_launch();
+        int cnt = 0;
+        HIP_CHECK(hipMemcpyAsync(&cnt, tile_count.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        if (cnt > (int)tiles.n) throw MpidError(MPID_ERR_HIP, "tile list overflow");
+        ntiles = cnt;
+        stats.ntiles = cnt;
+    }
+
+    template <class R> void recip_cycle_fixed(Prec<R>& P) {
+        PmeGeom<R> g = geom<R>();
+        const size_t G = P.grid.n;
+        timed(P_SPREAD, [&] {
+            launch_fill<R>(stream, (R*)P.grid.p, 2 * G, R(0));
+            if (tiled_spread) {
+                launch_spread_fixed_tiled<R>(stream, P.pos.p, P.lab.p, g, tgeom, tstart.p, tend.p, P.scratch.p,
+                                             P.grid.p);
+                launch_fold<R>(stream, g, tgeom, P.scratch.p, P.grid.p);
+            } else {
+                launch_spread_fixed<R>(stream, n, P.pos.p, P.lab.p, g, P.grid.p);
+            }
+        });
+        timed(P_FFT, [&] { launch_fft3d<R>(stream, P.plan, P.grid.p, -1); });
+        timed(P_CONV, [&] { launch_convolve<R>(stream, g, P.moduli.p, (R)opt.ewald_alpha, P.grid.p); });
+        timed(P_FFT, [&] { launch_fft3d<R>(stream, P.plan, P.grid.p, +1); });
+        timed(P_GATHER, [&] { launch_gather_phi<R>(stream, n, P.pos.p, g, P.grid.p, P.phi.p); });
+        check_launch();
+    }
+
+    template <class R> void recip_cycle_induced(Prec<R>& P, bool full) {
+        PmeGeom<R> g = geom<R>();
+        const size_t G = P.grid.n;
+        timed(P_SPREAD, [&] {
+            launch_fill<R>(stream, (R*)P.grid.p, 2 * G, R(0));
+            if (tiled_spread) {
+                launch_spread_induced_tiled<R>(stream, P.pos.p, P.mu.p, g, tgeom, tstart.p, tend.p, P.scratch.p,
+                                               P.grid.p);
+                launch_fold<R>(stream, g, tgeom, P.scratch.p, P.grid.p);
+            } else {
+                launch_spread_induced<R>(stream, n, P.pos.p, P.mu.p, g, P.grid.p);
+            }
+        });
+        timed(P_FFT, [&] { launch_fft3d<R>(stream, P.plan, P.grid.p, -1); });
+        timed(P_CONV, [&] { launch_convolve<R>(stream, g, P.moduli.p, (R)opt.ewald_alpha, P.grid.p); });
+        timed(P_FFT, [&] { launch_fft3d<R>(stream, P.plan, P.grid.p, +1); });
+        timed(P_GATHER, [&] {
+            if (full) launch_gather_phi<R>(stream, n, P.pos.p, g, P.grid.p, P.phidp.p);
+            else launch_gather_field<R>(stream, n, P.pos.p, g, P.grid.p, P.erec.p);
+        });
+        check_launch();
+    }
+
+    // The permanent multipoles go through the reciprocal sum once per
+    // evaluation, always in FP64: each atom's potential contains its own
+    // smeared moments, whose energy nearly cancels the self energy, so FP32
+    // rounding of that potential would dominate the error of the total.
+    template <class R> void fixed_field(Prec<R>& P) {
+        recip_cycle_fixed(pd);
+        timed(P_FIXED, [&] {
+            launch_zero_acc(stream, field_acc.p, field_acc.n);
+            launch_fixed_field_pairs<R>(stream, ntiles, tiles.p, pair_atoms(P), pair_params<R>(), field_acc.p);
+        });
+        timed(P_OTHER, [&] {
+            launch_fixed_field_finish<double>(stream, n, pd.phi.p, lab.p, field_acc.p, gscale3(), selfc(), E0.p);
+        });
+        check_launch();
+    }
+
+    // field T*mu of the induced dipoles in P.mu: accumulator + P.erec
+    template <class R> void induced_field(Prec<R>& P) {
+        timed(P_INDUCED, [&] {
+            launch_zero_acc(stream, field_acc.p, field_acc.n);
+            launch_induced_field_pairs<R>(stream, ntiles, tiles.p, pair_atoms(P), P.mu.p, pair_params<R>(),
+                                          field_acc.p);
+        });
+        recip_cycle_induced(P, false);
+    }
+
+    // Av = v - S T S v
+    template <class R> void matvec(Prec<R>& P, const double* v, double* Av) {
+        timed(P_SOLVER, [&] { launch_apply_S<R>(stream, n, sqrtpol.p, v, P.mu.p, nullptr); });
+        induced_field(P);
+        timed(P_SOLVER, [&] {
+            launch_matvec_finish<R>(stream, n, sqrtpol.p, field_acc.p, P.erec.p, P.mu.p, selfc(), v, Av);
+        });
+        check_launch();
+        stats.matvecs++;
+    }
+
+    double read_scalar(const double* d) {
+        double h = 0;
+        HIP_CHECK(hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        return h;
+    }
+    double debye_eps(double ss) const { return kDebye * std::sqrt(ss / n); }
+
+    // Conjugate gradients on (I - S T S) sol = rhs.  With zero_start the
+    // iteration starts from sol = 0 (r = rhs), else from the current sol.
+    // Stops when the RMS-Debye norm of S r falls below stop_eps; returns it.
+    template <class R>
+    double cg(Prec<R>& P, double* sol, const double* rhs, bool zero_start, double stop_eps, int& iters, int max_iter) {
+        const int n3 = 3 * n;
+        timed(P_SOLVER, [&] {
+            if (zero_start) {
+                launch_fill<double>(stream, sol, n3, 0.0);
+                HIP_CHECK(hipMemcpyAsync(r.p, rhs, n3 * sizeof(double), hipMemcpyDeviceToDevice, stream));
+            }
+        });
+        if (!zero_start) {
+            matvec(P, sol, Ap.p);
+            timed(P_SOLVER, [&] { launch_sub(stream, n3, rhs, Ap.p, r.p); });
+        }
+        timed(P_SOLVER, [&] {
+            HIP_CHECK(hipMemcpyAsync(p.p, r.p, n3 * sizeof(double), hipMemcpyDeviceToDevice, stream));
+            launch_residual_norm(stream, n, sqrtpol.p, r.p, partial.p, scal.p + 2);  // scal[2] = rr, scal[3] = ss
+            HIP_CHECK(hipMemcpyAsync(scal.p, scal.p + 2, sizeof(double), hipMemcpyDeviceToDevice, stream));
+        });
+        double eps = debye_eps(read_scalar(scal.p + 3));
+        while (eps >= stop_eps && iters < max_iter) {
+            matvec(P, p.p, Ap.p);
+            timed(P_SOLVER, [&] {
+                launch_dot(stream, n3, p.p, Ap.p, partial.p, scal.p + 1);
+                launch_cg_step(stream, n, sqrtpol.p, scal.p, p.p, Ap.p, sol, r.p, partial.p, scal.p + 2);
+                launch_cg_direction(stream, n3, scal.p, scal.p + 2, r.p, p.p);
+                HIP_CHECK(hipMemcpyAsync(scal.p, scal.p + 2, sizeof(double), hipMemcpyDeviceToDevice, stream));
+            });
+            check_launch();
+            iters++;
+            eps = debye_eps(read_scalar(scal.p + 3));
+        }
+        return eps;
+    }
+
+    // residual with the FP64 operator: rt = b - A x; returns its RMS-Debye norm
+    double true_residual() {
+        const int n3 = 3 * n;
+        matvec(pd, x.p, Ap.p);
+        timed(P_SOLVER, [&] {
+            launch_sub(stream, n3, b.p, Ap.p, rt.p);
+            launch_residual_norm(stream, n, sqrtpol.p, rt.p, partial.p, scal.p + 4);
+        });
+        return debye_eps(read_scalar(scal.p + 5));
+    }
+
+    template <class R> void solve(Prec<R>& P) {
+        stats.iterations = 0;
+        stats.refinements = 0;
+        stats.matvecs = 0;
+        timed(P_SOLVER, [&] { launch_solver_init(stream, n, sqrtpol.p, E0.p, b.p, x.p); });
+        check_launch();
+        energy_correction = 0.0;
+        if (opt.polarization == MPID_POLARIZATION_DIRECT) {
+            stats.converged = 1;
+            stats.epsilon = 0.0;
+            return;
+        }
+        int iters = 0;
+        double eps = cg(P, x.p, b.p, false, opt.target_epsilon, iters, opt.max_iterations);
+        if (opt.precision == MPID_PRECISION_MIXED) {
+            // FP64 iterative refinement: the residual of the FP32 solution is
+            // formed with the FP64 operator, the correction is solved with the
+            // FP32 operator and added in FP64, until the target is met.
+            eps = true_residual();
+            int refine = 0;
+            while (eps >= opt.target_epsilon && refine < 8 && iters < opt.max_iterations) {
+                refine++;
+                const int before = iters;
+                cg(P, dx.p, rt.p, true, std::max(0.3 * opt.target_epsilon, 1e-4 * eps), iters, opt.max_iterations);
+                if (iters == before) iters++;
+                timed(P_SOLVER, [&] { launch_axpy(stream, 3 * n, 1.0, dx.p, x.p); });
+                eps = true_residual();
+            }
+            stats.refinements = refine;
+        }
+        // Variational correction of the energy: the energy expression of the
+        // reference is linear in the induced dipoles, U(mu) = E(mu) - eps/2 y.r
+        // (r = b - A y) is stationary at the solution, so it carries no
+        // first-order error from the solver tolerance.
+        timed(P_SOLVER, [&] {
+            launch_dot(stream, 3 * n, x.p, opt.precision == MPID_PRECISION_MIXED ? rt.p : r.p, partial.p, scal.p + 6);
+        });
+        energy_correction = -0.5 * kElectric * read_scalar(scal.p + 6);
+        stats.iterations = iters;
+        stats.epsilon = eps;
+        stats.converged = eps < opt.target_epsilon ? 1 : 0;
+        if (!stats.converged) {
+            char buf[160];
+            std::snprintf(buf, sizeof(buf), "Induced dipoles did not converge:  iterations=%d eps=%g", iters, eps);
+            throw MpidError(MPID_ERR_NOT_CONVERGED, buf);
+        }
+    }
+
+    template <class R> double forces_and_energy(Prec<R>& P, double* d_forces_out) {
+        // induced dipoles: mu = S x (double for the per-atom terms, R for the kernels)
+        timed(P_SOLVER, [&] { launch_apply_S<R>(stream, n, sqrtpol.p, x.p, P.mu.p, mud.p); });
+        mu_valid = true;
+        recip_cycle_induced(P, true);  // phidp
+        timed(P_ELEC, [&] {
+            launch_zero_acc(stream, force_acc.p, force_acc.n);
+            launch_zero_acc(stream, torque_acc.p, torque_acc.n);
+            launch_zero_acc(stream, energy_acc.p, 1);
+            launch_electrostatics_pairs<R>(stream, ntiles, tiles.p, pair_atoms(P), P.mu.p, pair_params<R>(),
+                                           force_acc.p, torque_acc.p, energy_acc.p);
+        });
+        check_launch();
+        acc_t ereal = 0;
+        HIP_CHECK(hipMemcpyAsync(&ereal, energy_acc.p, sizeof(acc_t), hipMemcpyDeviceToHost, stream));
+        timed(P_RECIP, [&] {
+            launch_recip_atoms<R>(stream, n, opt.polarization == MPID_POLARIZATION_MUTUAL ? 1 : 0, pd.phi.p,
+                                  P.phidp.p, lab.p, mud.p, aniso.p, gscale3(), opt.ewald_alpha, force_acc.p,
+                                  torque_acc.p, energy_acc.p);
+        });
+        timed(P_OTHER, [&] {
+            launch_torque_to_force(stream, n, dbox, spos.p, saxes.p, torque_acc.p, force_acc.p);
+            if (d_forces_out) launch_output_forces(stream, n, perm.p, force_acc.p, d_forces_out, 1);
+        });
+        check_launch();
+        acc_t e = 0;
+        HIP_CHECK(hipMemcpyAsync(&e, energy_acc.p, sizeof(acc_t), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        stats.energy_terms[0] = acc_get(ereal);
+        stats.energy_terms[1] = acc_get(e) - acc_get(ereal);
+        stats.energy_terms[2] = energy_correction;
+        return acc_get(e) + energy_correction;
+    }
+
+    template <class R> double run(Prec<R>& P, const double* d_pos, double* d_forces_out) {
+        prepare(d_pos);
+        fixed_field(P);
+        solve(P);
+        double e = forces_and_energy(P, d_forces_out);
+        collect_profile();
+        return e;
+    }
+
+    double execute_device(const double* d_pos, const double* boxv, double* d_forces) {
+        set_box(boxv);
+        double e = opt.precision == MPID_PRECISION_MIXED ? run(pf, d_pos, d_forces) : run(pd, d_pos, d_forces);
+        return e;
+    }
+
+    void remember(const double* pos) {
+        last_pos.assign(pos, pos + 3 * (size_t)n);
+        last_box.assign(box_vec, box_vec + 9);
+        have_last = true;
+    }
+    bool same_as_last(const double* pos, const double* boxv) const {
+        if (!have_last || !mu_valid) return false;
+        return std::memcmp(last_pos.data(), pos, 3 * (size_t)n * sizeof(double)) == 0 &&
+               std::memcmp(last_box.data(), boxv, 9 * sizeof(double)) == 0;
+    }
+
+    double execute_host(const double* pos, const double* boxv, double* forces_inout) {
+        set_box(boxv);
+        pos_in.upload(pos, 3 * (size_t)n, stream);
+        if (forces_inout) {
+            fout.upload(forces_inout, 3 * (size_t)n, stream);
+        }
+        double e = execute_device(pos_in.p, boxv, forces_inout ? fout.p : nullptr);
+        if (forces_inout) {
+            HIP_CHECK(hipMemcpyAsync(forces_inout, fout.p, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost,
+                                     stream));
+            HIP_CHECK(hipStreamSynchronize(stream));
+        }
+        remember(pos);
+        return e;
+    }
+
+    void ensure(const double* pos, const double* boxv) {
+        if (!same_as_last(pos, boxv)) execute_host(pos, boxv, nullptr);
+    }
+    void download_unpermuted(const double* d_sorted, double* out) {
+        launch_unpermute3(stream, n, perm.p, d_sorted, tmp3n.p);
+        check_launch();
+        HIP_CHECK(hipMemcpyAsync(out, tmp3n.p, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+    }
+    void lab_dipoles(double* out) {
+        std::vector<double> l((size_t)n * kMP), d(3 * (size_t)n);
+        std::vector<int> pm(n);
+        HIP_CHECK(hipMemcpyAsync(l.data(), lab.p, l.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipMemcpyAsync(pm.data(), perm.p, n * sizeof(int), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        for (int s = 0; s < n; s++)
+            for (int k = 0; k < 3; k++) out[3 * (size_t)pm[s] + k] = l[(size_t)s * kMP + 1 + k];
+    }
+
+    void compute_fixed_field(const double* pos, const double* boxv, double* out) {
+        set_box(boxv);
+        pos_in.upload(pos, 3 * (size_t)n, stream);
+        prepare(pos_in.p);
+        if (opt.precision == MPID_PRECISION_MIXED) fixed_field(pf); else fixed_field(pd);
+        download_unpermuted(E0.p, out);
+        have_last = false;
+        collect_profile();
+    }
+    void compute_induced_field(const double* pos, const double* boxv, const double* mu, double* out) {
+        set_box(boxv);
+        pos_in.upload(pos, 3 * (size_t)n, stream);
+        prepare(pos_in.p);
+        // mu (original order) -> sorted precision copy
+        std::vector<int> pm(n);
+        HIP_CHECK(hipMemcpyAsync(pm.data(), perm.p, n * sizeof(int), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        auto run_field = [&](auto& P) {
+            using R = typename std::remove_reference<decltype(*P.mu.p)>::type;
+            std::vector<R> ms(3 * (size_t)n);
+            for (int s = 0; s < n; s++)
+                for (int k = 0; k < 3; k++) ms[3 * (size_t)s + k] = (R)mu[3 * (size_t)pm[s] + k];
+            HIP_CHECK(hipMemcpyAsync(P.mu.p, ms.data(), ms.size() * sizeof(R), hipMemcpyHostToDevice, stream));
+            induced_field(P);
+            launch_field_sum<R>(stream, n, field_acc.p, P.erec.p, P.mu.p, selfc(), tmp3n.p);
+            check_launch();
+            std::vector<double> f(3 * (size_t)n);
+            HIP_CHECK(hipMemcpyAsync(f.data(), tmp3n.p, f.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+            HIP_CHECK(hipStreamSynchronize(stream));
+            for (int s = 0; s < n; s++)
+                for (int k = 0; k < 3; k++) out[3 * (size_t)pm[s] + k] = f[3 * (size_t)s + k];
+        };
+        if (opt.precision == MPID_PRECISION_MIXED) run_field(pf); else run_field(pd);
+        have_last = false;
+        collect_profile();
+    }
+};
+
+namespace {
+thread_local std::string g_create_error;
+
+template <class F> int guarded(mpid_context* ctx, F&& f) {
+    try {
+        f();
+        if (ctx) ctx->error.clear();
+        return MPID_OK;
+    } catch (const MpidError& e) {
+        if (ctx) ctx->error = e.what();
+        return e.code;
+    } catch (const HipError& e) {
+        if (ctx) ctx->error = e.what();
+        return MPID_ERR_HIP;
+    } catch (const std::exception& e) {
+        if (ctx) ctx->error = e.what();
+        return MPID_ERR_HIP;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+int mpid_create(const mpid_parameters* params, const mpid_options* opts, mpid_context** out) {
+    if (!out) return MPID_ERR_INVALID;
+    *out = nullptr;
+    std::unique_ptr<mpid_context> ctx(new mpid_context());
+    int rc = guarded(ctx.get(), [&] {
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+            throw MpidError(MPID_ERR_HIP, "no HIP device is available");
+        HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        ctx->set_options(opts);
+        ctx->set_parameters(params, true);
+        ctx->alloc_state();
+    });
+    if (rc != MPID_OK) {
+        g_create_error = ctx->error;
+        return rc;
+    }
+    *out = ctx.release();
+    return MPID_OK;
+}
+
+const char* mpid_create_error(void) { return g_create_error.c_str(); }
+
+int mpid_update_parameters(mpid_context* ctx, const mpid_parameters* params) {
+    if (!ctx) return MPID_ERR_INVALID;
+    return guarded(ctx, [&] { ctx->set_parameters(params, false); });
+}
+
+int mpid_execute(mpid_context* ctx, const double* positions, const double* box, int include_forces,
+                 int include_energy, double* forces_inout, double* energy_out) {
+    if (!ctx || !positions) return MPID_ERR_INVALID;
+    return guarded(ctx, [&] {
+        double e = ctx->execute_host(positions, box, include_forces ? forces_inout : nullptr);
+        if (energy_out && include_energy) *energy_out = e;
+        else if (energy_out) *energy_out = 0.0;
+    });
+}
+
+int mpid_execute_device(mpid_context* ctx, const double* d_positions, const double* box, double* d_forces_inout,
+                        double* energy_out) {
+    if (!ctx || !d_positions) return MPID_ERR_INVALID;
+    return guarded(ctx, [&] {
+        double e = ctx->execute_device(d_positions, box, d_forces_inout);
+        ctx->have_last = false;
+        if (energy_out) *energy_out = e;
+    });
+}
+
+int mpid_get_induced_dipoles(mpid_context* ctx, const double* positions, const double* box, double* out) {
+    if (!ctx || !positions || !out) return MPID_ERR_INVALID;
+    return guarded(ctx, [&] {
+        ctx->ensure(positions, box);
+        ctx->download_unpermuted(ctx->mud.p, out);
+    });
+}
+
+int mpid_get_lab_frame_permanent_dipoles(mpid_context* ctx, const double* positions, const double* box, double* out) {
+    if (!ctx || !positions || !out) return MPID_ERR_INVALID;
+    return guarded(ctx, [&] {
+        ctx->ensure(positions, box);
+        ctx->lab_dipoles(out);
+    });
+}
+
+int mpid_get_total_dipoles(mpid_context* ctx, const double* positions, const double* box, double* out) {
+    if (!ctx || !positions || !out) return MPID_ERR_INVALID;
+    return guarded(ctx, [&] {
+        ctx->ensure(positions, box);
+        std::vector<double> ind(3 * (size_t)ctx->n);
+        ctx->download_unpermuted(ctx->mud.p, ind.data());
+        ctx->lab_dipoles(out);
+        for (size_t k = 0; k < ind.size(); k++) out[k] += ind[k];
+    });
+}
+
+int mpid_get_pme_parameters(const mpid_context* ctx, double* alpha, int* nx, int* ny, int* nz) {
+    if (!ctx) return MPID_ERR_INVALID;
+    if (alpha) *alpha = ctx->opt.ewald_alpha;
+    if (nx) *nx = ctx->opt.grid[0];
+    if (ny) *ny = ctx->opt.grid[1];
+    if (nz) *nz = ctx->opt.grid[2];
+    return MPID_OK;
+}
+
+int mpid_compute_fixed_field(mpid_context* ctx, const double* positions, const double* box, double* field_out) {
+    if (!ctx || !positions || !field_out) return MPID_ERR_INVALID;
+    return guarded(ctx, [&] { ctx->compute_fixed_field(positions, box, field_out); });
+}
+
+int mpid_compute_induced_field(mpid_context* ctx, const double* positions, const double* box, const double* mu,
+                               double* field_out) {
+    if (!ctx || !positions || !mu || !field_out) return MPID_ERR_INVALID;
+    return guarded(ctx, [&] { ctx->compute_induced_field(positions, box, mu, field_out); });
+}
+
+int mpid_get_stats(const mpid_context* ctx, mpid_stats* stats) {
+    if (!ctx || !stats) return MPID_ERR_INVALID;
+    *stats = ctx->stats;
+    return MPID_OK;
+}
+
+int mpid_set_profiling(mpid_context* ctx, int enable) {
+    if (!ctx) return MPID_ERR_INVALID;
+    ctx->profiling = enable != 0;
+    return MPID_OK;
+}
+
+int mpid_get_profile(mpid_context* ctx, double* times_ms, int* counts, int reset) {
+    if (!ctx) return MPID_ERR_INVALID;
+    for (int k = 0; k < MPID_PROF_COUNT; k++) {
+        if (times_ms) times_ms[k] = ctx->prof_ms[k];
+        if (counts) counts[k] = ctx->prof_count[k];
+        if (reset) {
+            ctx->prof_ms[k] = 0;
+            ctx->prof_count[k] = 0;
+        }
+    }
+    return MPID_OK;
+}
+
+const char* mpid_last_error(const mpid_context* ctx) { return ctx ? ctx->error.c_str() : "null context"; }
+
+void mpid_destroy(mpid_context* ctx) { delete ctx; }
+
+}  // extern "C"

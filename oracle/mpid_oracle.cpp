@@ -227,7 +227,8 @@ void bspline(double w, double theta[kOrder][5]) {
 // convolution kernel is real and even in k, so the sign convention is
 // immaterial.
 // ---------------------------------------------------------------------------
-void fft1d(cplx* x, int n, int stride, int sign, std::vector<cplx>& scratch) {
+// W[k] = exp(sign 2 pi i k / ntop); a sub-transform of length n uses W[k * ntop / n].
+void fft1d(cplx* x, int n, int stride, const std::vector<cplx>& W, int ntop) {
     if (n == 1) return;
     int p = 0;
     for (int f : {4, 2, 3, 5})
@@ -238,39 +239,45 @@ void fft1d(cplx* x, int n, int stride, int sign, std::vector<cplx>& scratch) {
         if (p == 0) p = n;
     }
     int m = n / p;
-    // gather the p decimated sub-sequences
+    int step = ntop / n;
     std::vector<cplx> sub(n);
     for (int r = 0; r < p; r++)
         for (int k = 0; k < m; k++) sub[r * m + k] = x[(k * p + r) * stride];
-    for (int r = 0; r < p; r++) fft1d(&sub[r * m], m, 1, sign, scratch);
+    for (int r = 0; r < p; r++) fft1d(&sub[r * m], m, 1, W, ntop);
     for (int k = 0; k < n; k++) {
         cplx s = 0;
         int kk = k % m;
-        for (int r = 0; r < p; r++) {
-            double ang = sign * 2.0 * M_PI * (double)((long long)r * k % n) / n;
-            s += sub[r * m + kk] * cplx(std::cos(ang), std::sin(ang));
-        }
+        for (int r = 0; r < p; r++) s += sub[r * m + kk] * W[(size_t)((long long)r * k % n) * step];
         x[k * stride] = s;
     }
 }
 
+std::vector<cplx> twiddles(int n, int sign) {
+    std::vector<cplx> W(n);
+    for (int k = 0; k < n; k++) {
+        double ang = sign * 2.0 * M_PI * k / n;
+        W[k] = cplx(std::cos(ang), std::sin(ang));
+    }
+    return W;
+}
+
 void fft3d(std::vector<cplx>& g, int nx, int ny, int nz, int sign) {
-    std::vector<cplx> scratch;
     std::vector<cplx> line;
+    std::vector<cplx> Wz = twiddles(nz, sign), Wy = twiddles(ny, sign), Wx = twiddles(nx, sign);
     for (int x = 0; x < nx; x++)
-        for (int y = 0; y < ny; y++) fft1d(&g[((size_t)x * ny + y) * nz], nz, 1, sign, scratch);
+        for (int y = 0; y < ny; y++) fft1d(&g[((size_t)x * ny + y) * nz], nz, 1, Wz, nz);
     line.resize(ny);
     for (int x = 0; x < nx; x++)
         for (int z = 0; z < nz; z++) {
             for (int y = 0; y < ny; y++) line[y] = g[((size_t)x * ny + y) * nz + z];
-            fft1d(line.data(), ny, 1, sign, scratch);
+            fft1d(line.data(), ny, 1, Wy, ny);
             for (int y = 0; y < ny; y++) g[((size_t)x * ny + y) * nz + z] = line[y];
         }
     line.resize(nx);
     for (int y = 0; y < ny; y++)
         for (int z = 0; z < nz; z++) {
             for (int x = 0; x < nx; x++) line[x] = g[((size_t)x * ny + y) * nz + z];
-            fft1d(line.data(), nx, 1, sign, scratch);
+            fft1d(line.data(), nx, 1, Wx, nx);
             for (int x = 0; x < nx; x++) g[((size_t)x * ny + y) * nz + z] = line[x];
         }
 }
@@ -1268,6 +1275,12 @@ struct Oracle {
             iEJ[1] += eC * ub[0] * ua[1];
             iEI[2] += eC * ua[1] * ub[2];
             iEJ[2] += eC * ub[1] * ua[2];
+            if (S.flags & 1) {
+                // the reference (:4854-4855) keeps only one of the two terms of
+                // this rotational derivative; flag 1 adds the other one
+                iEI[2] -= eC * ua[2] * ub[1];
+                iEJ[2] -= eC * ub[2] * ua[1];
+            }
             fIZ += dC * (ua[1] * ub[1] + ua[2] * ub[2]);
             fJZ += dC * (ub[1] * ua[1] + ub[2] * ua[2]);
         }

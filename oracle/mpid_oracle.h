@@ -46,6 +46,7 @@ typedef struct {
     int max_iter;
     double target_eps;
     double box[9];              /* three box vectors, row-major */
+    int flags;                  /* 1: restore the omitted induced-induced z torque */
 } oracle_system;
 
 /* Every pointer may be NULL.  Internal orderings: quadrupole QXX QXY QXZ QYY

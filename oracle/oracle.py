@@ -36,6 +36,7 @@ class _System(ctypes.Structure):
         ("default_thole", ctypes.c_double), ("scale14", ctypes.c_double),
         ("max_iter", ctypes.c_int), ("target_eps", ctypes.c_double),
         ("box", ctypes.c_double * 9),
+        ("flags", ctypes.c_int),
     ]
 
 
@@ -79,7 +80,7 @@ def _ptr(a, kind):
     return a.ctypes.data_as(_dp if kind == "d" else _ip)
 
 
-def _system(flat, box, polarization=None):
+def _system(flat, box, polarization=None, flags=0):
     keep = {}
 
     def arr(name, dtype):
@@ -117,13 +118,17 @@ def _system(flat, box, polarization=None):
         b = np.diag(b).reshape(-1)
     for k in range(9):
         s.box[k] = float(b[k])
+    s.flags = int(flags)
     return s, keep
 
 
-def evaluate(flat, positions, box, polarization=None, details=False):
-    """Energy (kJ/mol), forces (kJ/mol/nm, [n,3]) and optional intermediates."""
+def evaluate(flat, positions, box, polarization=None, details=False, conservative_torques=False):
+    """Energy (kJ/mol), forces (kJ/mol/nm, [n,3]) and optional intermediates.
+
+    ``conservative_torques`` adds the induced-induced torque term that the
+    reference omits (MPIDReferenceForce.cpp:4854-4855)."""
     n = int(flat["n"])
-    s, keep = _system(flat, box, polarization)
+    s, keep = _system(flat, box, polarization, 1 if conservative_torques else 0)
     pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1)
     bufs = {}
     out = _Out()

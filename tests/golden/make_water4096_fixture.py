@@ -1,0 +1,35 @@
+"""Writes tests/golden/water4096_oracle.npz: the FP64 oracle's energy, forces,
+permanent field and induced dipoles for the 12,288-atom water box (octopoles,
+anisotropic polarisability, 64^3 grid), mutual polarisation converged to
+1e-10 Debye.  The GPU box has no reference sources and the oracle needs about a
+minute here, so the result is committed as a fixture.
+
+Run:  python tests/golden/make_water4096_fixture.py
+"""
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from mpidopenmmplugin_amd.waterbox import make_waterbox  # noqa: E402
+from oracle import oracle  # noqa: E402
+
+
+def main():
+    f, pos, box = make_waterbox("water4096", target_eps=1e-10)
+    t = time.time()
+    res = oracle.evaluate(f.flatten(), pos, box, details=True)
+    print(f"oracle: {time.time() - t:.1f} s, {res['iterations']} DIIS iterations, eps {res['eps']:.2e}")
+    out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "water4096_oracle.npz")
+    np.savez_compressed(out, energy=np.float64(res["energy"]), energy_terms=np.array(res["energy_terms"]),
+                        forces=res["forces"], induced=res["induced"], fixed_field=res["fixed_field"],
+                        positions_checksum=np.float64(pos.sum()))
+    print("wrote", out, os.path.getsize(out), "bytes")
+
+
+if __name__ == "__main__":
+    main()

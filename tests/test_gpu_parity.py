@@ -1,0 +1,228 @@
+"""Parity of the gfx950 path (through the C ABI) with the FP64 oracle and with
+the golden values of the reference's own tests.
+
+Tolerances, stated per result:
+  * DOUBLE precision path vs FP64 oracle: 1e-8 relative (same arithmetic,
+    different summation order and solver).
+  * MIXED precision path: force relative RMS <= 1e-5, energy relative <= 1e-6,
+    induced-dipole relative RMS <= 1e-5 against the FP64 oracle (the targets of
+    the workload definition), and the reference tests' own rule (1e-4) against
+    their goldens.
+"""
+import numpy as np
+import pytest
+
+from mpidopenmmplugin_amd import MPIDForce, MPIDForceError
+from mpidopenmmplugin_amd.kernel import MPIDContext, MPIDConvergenceError, HipCalcMPIDForceKernel
+from mpidopenmmplugin_amd.waterbox import make_waterbox
+from oracle import oracle
+from tests.helpers import assert_equal_tol, assert_equal_vec, force_from_case, load_reference_tests
+
+pytestmark = pytest.mark.gpu
+
+CASES = load_reference_tests()["cases"]
+
+
+def rel_rms(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum())
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+@pytest.mark.parametrize("case", CASES, ids=[c["name"] for c in CASES])
+def test_reference_goldens(case, precision, reference_tests):
+    print(f"[gpu] golden {case['name']} {precision}", flush=True)
+    f, pos, box = force_from_case(case, reference_tests["systems"])
+    ctx = MPIDContext(f, box, precision=precision)
+    ctx.setPositions(pos)
+    st = ctx.getState()
+    tol = case.get("energy_tol", 1e-4)
+    assert_equal_tol(case["energy"], st.getPotentialEnergy(), tol)
+    if "forces" in case:
+        assert_equal_vec(case["forces"], st.getForces(), 1e-4)
+    if precision == "double" and "forces" in case:
+        ref = oracle.evaluate(f.flatten(), pos, box)
+        assert abs(st.getPotentialEnergy() - ref["energy"]) <= 1e-8 * abs(ref["energy"])
+        assert rel_rms(st.getForces(), ref["forces"]) <= 1e-8
+
+
+@pytest.fixture(scope="module")
+def water216():
+    """The 648-atom workload; the oracle is converged to 1e-10 Debye so that
+    solver noise does not enter the comparison."""
+    f, pos, box = make_waterbox("water216")
+    tight, _, _ = make_waterbox("water216", target_eps=1e-10)
+    ref = oracle.evaluate(tight.flatten(), pos, box, details=True)
+    return f, pos, box, ref
+
+
+@pytest.mark.parametrize("precision,tol", [("double", 1e-10), ("mixed", 2e-6)])
+def test_fixed_field(water216, precision, tol):
+    print(f"[gpu] fixed field water216 {precision}", flush=True)
+    f, pos, box, ref = water216
+    k = HipCalcMPIDForceKernel(precision)
+    k.initialize(f.getNumMultipoles(), f)
+    E = k.computeFixedField(pos, box)
+    assert rel_rms(E, ref["fixed_field"]) <= tol
+
+
+@pytest.mark.parametrize("precision,tol", [("double", 1e-9), ("mixed", 2e-6)])
+def test_induced_field_matvec(water216, precision, tol):
+    print(f"[gpu] induced field water216 {precision}", flush=True)
+    f, pos, box, ref = water216
+    rng = np.random.default_rng(3)
+    mu = rng.normal(size=pos.shape) * 2e-3
+    want = oracle.induced_field(f.flatten(), pos, box, mu)
+    k = HipCalcMPIDForceKernel(precision)
+    k.initialize(f.getNumMultipoles(), f)
+    got = k.computeInducedField(pos, box, mu)
+    assert rel_rms(got, want) <= tol
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_full_evaluation_water216(water216, precision):
+    """At the workload's own convergence target (1e-6 Debye): energy within
+    1e-6, forces and induced dipoles within 1e-5 relative RMS of the FP64 oracle."""
+    print(f"[gpu] full water216 {precision}", flush=True)
+    f, pos, box, ref = water216
+    ctx = MPIDContext(f, box, precision=precision)
+    ctx.setPositions(pos)
+    st = ctx.getState()
+    e, F = st.getPotentialEnergy(), st.getForces()
+    mu = ctx.getInducedDipoles()
+    assert abs(e - ref["energy"]) <= 1e-6 * abs(ref["energy"])
+    assert rel_rms(F, ref["forces"]) <= 1e-5
+    assert rel_rms(mu, ref["induced"]) <= 1e-5
+    lab = ctx.getLabFramePermanentDipoles()
+    assert np.allclose(lab, ref["lab_dipoles"], rtol=0, atol=1e-12)
+    tot = ctx.getTotalDipoles()
+    assert np.allclose(tot, lab + mu, atol=1e-12)
+
+
+def test_full_evaluation_water216_tight(water216):
+    """FP64 path converged as tightly as the oracle: agreement to 1e-9."""
+    f, pos, box, ref = water216
+    g, _, _ = make_waterbox("water216", target_eps=1e-10)
+    ctx = MPIDContext(g, box, precision="double")
+    ctx.setPositions(pos)
+    st = ctx.getState()
+    assert abs(st.getPotentialEnergy() - ref["energy"]) <= 1e-9 * abs(ref["energy"])
+    assert rel_rms(st.getForces(), ref["forces"]) <= 1e-8
+    assert rel_rms(ctx.getInducedDipoles(), ref["induced"]) <= 1e-8
+
+
+@pytest.mark.parametrize("precision,tol", [("double", 1e-12), ("mixed", 1e-6)])
+def test_repeated_evaluations_agree(water216, precision, tol):
+    """Per-atom sums are fixed point (order independent); the PME grid uses
+    floating-point atomics, so repeated evaluations agree to rounding."""
+    print(f"[gpu] repeat {precision}", flush=True)
+    f, pos, box, ref = water216
+    ctx = MPIDContext(f, box, precision=precision)
+    ctx.setPositions(pos)
+    a = ctx.getState()
+    ctx.setPositions(pos + 0.0)
+    b = ctx.getState()
+    assert abs(a.getPotentialEnergy() - b.getPotentialEnergy()) <= tol * abs(a.getPotentialEnergy())
+    assert rel_rms(a.getForces(), b.getForces()) <= 10 * tol
+
+
+def test_forces_accumulate_into_buffer(water216):
+    f, pos, box, ref = water216
+    k = HipCalcMPIDForceKernel("double")
+    k.initialize(f.getNumMultipoles(), f)
+    base = np.ones((f.getNumMultipoles(), 3))
+    e, out = k.execute(pos, box, forces=base.copy())
+    e2, out2 = k.execute(pos, box)
+    assert np.allclose(out, out2 + 1.0, atol=1e-9)
+
+
+def test_nonconvergence_is_an_error(water216):
+    f, pos, box, ref = water216
+    f2, _, _ = make_waterbox("water216")
+    f2.setMutualInducedMaxIterations(1)
+    f2.setMutualInducedTargetEpsilon(1e-12)
+    ctx = MPIDContext(f2, box, precision="double")
+    ctx.setPositions(pos)
+    with pytest.raises(MPIDConvergenceError, match="did not converge"):
+        ctx.getState()
+
+
+def test_invalid_parameters_raise():
+    f, pos, box = make_waterbox("water216")
+    f.setCutoffDistance(1.0)  # more than half of the 1.862 nm box
+    ctx = MPIDContext(f, box, precision="double")
+    ctx.setPositions(pos)
+    with pytest.raises(MPIDForceError, match="twice the nonbonded cutoff"):
+        ctx.getState()
+    g, pos, box = make_waterbox("water216")
+    q = list(g.getMultipoleParameters(0))
+    q[2] = [1e-3, 0, 0, 0, 0, 0]  # non-traceless quadrupole
+    g.setMultipoleParameters(0, *q)
+    with pytest.raises(MPIDForceError, match="nonzero trace"):
+        MPIDContext(g, box, precision="double")
+
+
+def test_update_parameters_in_context(water216):
+    """copyParametersToContext (without the reference platform's octopole bug,
+    platforms/reference/src/MPIDReferenceKernels.cpp:371-376)."""
+    f, pos, box, ref = water216
+    g, _, _ = make_waterbox("water216", target_eps=1e-10)
+    ctx = MPIDContext(g, box, precision="double")
+    ctx.setPositions(pos)
+    e0 = ctx.getState().getPotentialEnergy()
+    q = list(g.getMultipoleParameters(0))
+    q[0] = -0.7
+    g.setMultipoleParameters(0, *q)
+    ctx.updateParametersInContext()
+    e1 = ctx.getState().getPotentialEnergy()
+    want = oracle.evaluate(g.flatten(), pos, box)["energy"]
+    assert abs(e1 - want) <= 1e-9 * abs(want)
+    assert abs(e0 - ref["energy"]) <= 1e-9 * abs(ref["energy"])
+
+
+@pytest.fixture(scope="module")
+def water4096():
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "water4096_oracle.npz")
+    ref = np.load(path)
+    f, pos, box = make_waterbox("water4096")
+    assert abs(pos.sum() - float(ref["positions_checksum"])) < 1e-9, "generator drifted from the fixture"
+    return f, pos, box, ref
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_water4096_octopoles_anisotropic(water4096, precision):
+    """12,288 atoms, octopoles and anisotropic polarisability, 64^3 grid,
+    against the committed FP64 oracle fixture (converged to 1e-10 Debye)."""
+    print(f"[gpu] water4096 {precision}", flush=True)
+    f, pos, box, ref = water4096
+    ctx = MPIDContext(f, box, precision=precision)
+    ctx.setPositions(pos)
+    st = ctx.getState()
+    e_ref = float(ref["energy"])
+    assert abs(st.getPotentialEnergy() - e_ref) <= 1e-6 * abs(e_ref)
+    assert rel_rms(st.getForces(), ref["forces"]) <= 1e-5
+    assert rel_rms(ctx.getInducedDipoles(), ref["induced"]) <= 1e-5
+
+
+@pytest.mark.parametrize("conservative", [False, True], ids=["reference_torques", "conservative_torques"])
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_water216x_octopoles_anisotropic(precision, conservative):
+    """Octopoles and anisotropic polarisability (the parameter set of the
+    benchmark box) on 648 atoms, in both torque conventions."""
+    print(f"[gpu] water216x {precision} conservative={conservative}", flush=True)
+    tight, pos, box = make_waterbox("water216x", target_eps=1e-10)
+    ref = oracle.evaluate(tight.flatten(), pos, box, details=True, conservative_torques=conservative)
+    if precision == "double":
+        ctx = MPIDContext(tight, box, precision="double", conservative_torques=conservative)
+        e_tol, f_tol = 1e-9, 1e-8
+    else:
+        f, _, _ = make_waterbox("water216x")
+        ctx = MPIDContext(f, box, precision="mixed", conservative_torques=conservative)
+        e_tol, f_tol = 1e-6, 1e-5
+    ctx.setPositions(pos)
+    st = ctx.getState()
+    assert abs(st.getPotentialEnergy() - ref["energy"]) <= e_tol * abs(ref["energy"])
+    assert rel_rms(st.getForces(), ref["forces"]) <= f_tol
+    assert rel_rms(ctx.getInducedDipoles(), ref["induced"]) <= f_tol
