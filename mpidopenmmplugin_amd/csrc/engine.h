@@ -39,6 +39,14 @@ template <class R> struct PairAtoms {
     const int* npart;      // [n]
 };
 
+// Full per-atom neighbour lists (kernels_nlist.hip).  Entry k of atom i is at
+// ((k / kNlSplit) * n + i) * kNlSplit + k % kNlSplit.
+constexpr int kNlSplit = 4;  // threads per atom in the list-driven kernels
+struct NbrList {
+    const int* nbr;    // [n * cap] sorted indices of the neighbours
+    const int* count;  // [n]
+};
+
 // ---- atoms / frames / solver vectors (kernels_atoms.hip) ---------------------
 void launch_morton_keys(hipStream_t s, int n, const double* pos, Box<double> box, unsigned* keys, int* vals);
 void launch_invert_perm(hipStream_t s, int n, const int* perm, int* inv);
@@ -139,13 +147,25 @@ void launch_block_bounds(hipStream_t s, int n, const R4<R>* pos, float4* bmin, f
 void launch_build_tiles(hipStream_t s, int nblocks, const float4* bmin, const float4* bmax, Box<float> box,
                         float cutoff, int2* tiles, int* count, int capacity);
 template <class R>
-void launch_fixed_field_pairs(hipStream_t s, int ntiles, const int2* tiles, PairAtoms<R> a, PairParams<R> p,
-                              acc_t* field);
-template <class R>
-void launch_induced_field_pairs(hipStream_t s, int ntiles, const int2* tiles, PairAtoms<R> a, const R* mu,
-                                PairParams<R> p, acc_t* field);
-template <class R>
 void launch_electrostatics_pairs(hipStream_t s, int ntiles, const int2* tiles, PairAtoms<R> a, const R* mu,
                                  PairParams<R> p, acc_t* force, acc_t* torque, acc_t* energy);
+// ---- neighbour lists (kernels_nlist.hip) ---------------------------------------
+void launch_block_rows(hipStream_t s, int nb, const float4* bmin, const float4* bmax, Box<float> box, float cutoff,
+                       const int* start, int* counts, int* rows);
+void launch_exclusive_scan(hipStream_t s, int m, const int* counts, int* start);
+void launch_nl_build(hipStream_t s, int n, const int* rstart, const int* rows, const double4* pos, const int* partners,
+                     const int* npart, Box<double> box, double rc2, int cap, int* nbr, int* count, int* maxcount);
+template <class R>
+void launch_nl_fixed_field(hipStream_t s, int n, NbrList l, PairAtoms<R> a, PairParams<R> p, acc_t* field);
+template <class R>
+void launch_nl_induced_field(hipStream_t s, int n, NbrList l, PairAtoms<R> a, const R* mu, PairParams<R> p,
+                             acc_t* field);
+// the scaled covalent pairs, which the list and tile kernels leave out
+template <class R> void launch_cov_fixed_field(hipStream_t s, int n, PairAtoms<R> a, PairParams<R> p, acc_t* field);
+template <class R>
+void launch_cov_induced_field(hipStream_t s, int n, PairAtoms<R> a, const R* mu, PairParams<R> p, acc_t* field);
+template <class R>
+void launch_cov_electrostatics(hipStream_t s, int n, PairAtoms<R> a, const R* mu, PairParams<R> p, acc_t* force,
+                               acc_t* torque, acc_t* energy);
 
 }  // namespace mpid

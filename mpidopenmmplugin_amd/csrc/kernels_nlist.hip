@@ -1,0 +1,259 @@
+// Neighbour lists and the list-driven field kernels of the MPIDForce PME path.
+//
+// The solver applies the real-space dipole field operator once per iteration,
+// so the pairs within the cutoff are found once per evaluation and kept as a
+// full (both directions) per-atom list.  Each atom then sums only its own
+// field: no atomics per pair, no second pass over pair slots that fail the
+// cutoff, and the same list serves the permanent-field pass.
+//
+// Construction, all in the Morton-sorted order of the atoms:
+//   1. block rows: for every 64-atom block I, the blocks J whose bounding
+//      boxes come within the cutoff (both J < I and J >= I), in ascending J;
+//   2. atom lists: lane i of block I tests the atoms of each J in its row and
+//      appends those within the cutoff, skipping itself and its scaled
+//      covalent partners (those pairs go to the covalent-pair kernels).
+// Both steps are deterministic, so the summation order of every field is.
+//
+// List layout: entry k of atom i sits at ((k / S) * n + i) * S + k % S with
+// S = kNlSplit, so the S threads that share an atom read S adjacent words and
+// a wavefront reads 64 consecutive words per step.
+#include "pair_math.h"
+
+namespace mpid {
+
+__device__ __forceinline__ size_t nl_index(int i, int k, int n) {
+    return ((size_t)(k / kNlSplit) * n + i) * kNlSplit + (k % kNlSplit);
+}
+
+// Bounding boxes of blocks I and J within the cutoff (minimum image between
+// the box centres, with a little slack for the float rounding of the bounds).
+__device__ __forceinline__ bool blocks_within(float4 a0, float4 a1, float4 b0, float4 b1, const Box<float>& box,
+                                              float cutoff) {
+    const float ca[3] = {0.5f * (a0.x + a1.x), 0.5f * (a0.y + a1.y), 0.5f * (a0.z + a1.z)};
+    const float ha[3] = {0.5f * (a1.x - a0.x), 0.5f * (a1.y - a0.y), 0.5f * (a1.z - a0.z)};
+    const float cb[3] = {0.5f * (b0.x + b1.x), 0.5f * (b0.y + b1.y), 0.5f * (b0.z + b1.z)};
+    const float hb[3] = {0.5f * (b1.x - b0.x), 0.5f * (b1.y - b0.y), 0.5f * (b1.z - b0.z)};
+    float d2 = 0.f;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        float d = cb[k] - ca[k];
+        d -= box.L[k] * floorf(d * box.inv[k] + 0.5f);
+        const float g = fmaxf(0.f, fabsf(d) - ha[k] - hb[k]);
+        d2 += g * g;
+    }
+    const float rc = cutoff * 1.0001f + 1e-5f;
+    return d2 <= rc * rc;
+}
+
+// One wavefront per block I.  Count mode (rows == nullptr) writes counts[I];
+// fill mode writes the row at rows[start[I] ...] in ascending J.
+__global__ __launch_bounds__(64) void k_block_rows(int nb, const float4* __restrict__ bmin,
+                                                   const float4* __restrict__ bmax, Box<float> box, float cutoff,
+                                                   const int* __restrict__ start, int* __restrict__ counts,
+                                                   int* __restrict__ rows) {
+    const int I = blockIdx.x, lane = threadIdx.x;
+    const float4 a0 = bmin[I], a1 = bmax[I];
+    const int base = rows ? start[I] : 0;
+    int c = 0;
+    for (int J0 = 0; J0 < nb; J0 += kTile) {
+        const int J = J0 + lane;
+        const bool hit = J < nb && blocks_within(a0, a1, bmin[J], bmax[J], box, cutoff);
+        const unsigned long long m = __ballot(hit);
+        if (hit && rows) rows[base + c + __popcll(m & ((1ull << lane) - 1ull))] = J;
+        c += __popcll(m);
+    }
+    if (lane == 0 && !rows) counts[I] = c;
+}
+
+// start[k] = counts[0] + ... + counts[k-1] for k = 0..m (one workgroup).
+__global__ __launch_bounds__(1024) void k_exclusive_scan(int m, const int* __restrict__ counts, int* __restrict__ start) {
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    const int per = (m + 1023) / 1024;
+    const int b = t * per, e = min(b + per, m);
+    int s = 0;
+    for (int k = b; k < e; k++) s += counts[k];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = t ? part[t - 1] : 0;
+    for (int k = b; k < e; k++) {
+        start[k] = run;
+        run += counts[k];
+    }
+    if (t == 1023) start[m] = part[1023];
+}
+
+// Lane i of block I collects every j within the cutoff (FP64 distances, the
+// test of the reference: r2 > cutoff^2 is out).  Entries beyond cap are
+// counted but not stored; the host grows the list and rebuilds.
+__global__ __launch_bounds__(64) void k_nl_build(int n, const int* __restrict__ rstart, const int* __restrict__ rows,
+                                                 const double4* __restrict__ pos, const int* __restrict__ partners,
+                                                 const int* __restrict__ npart, Box<double> box, double rc2, int cap,
+                                                 int* __restrict__ nbr, int* __restrict__ count,
+                                                 int* __restrict__ maxcount) {
+    __shared__ double4 sp[kTile];
+    const int I = blockIdx.x, lane = threadIdx.x;
+    const int i = I * kTile + lane;
+    const bool iv = i < n;
+    const int ic = iv ? i : n - 1;
+    const double4 pi = pos[ic];
+    const int* parts = partners + (size_t)ic * kMaxPartners;
+    const int np = iv ? npart[ic] : 0;
+    int cnt = 0;
+    for (int r = rstart[I]; r < rstart[I + 1]; r++) {
+        const int J = rows[r];
+        __syncthreads();
+        {
+            const int j = J * kTile + lane;
+            sp[lane] = pos[j < n ? j : n - 1];
+        }
+        __syncthreads();
+        unsigned long long skip = 0;
+        for (int k = 0; k < np; k++) {
+            const int j = parts[k] >> 2;
+            if ((j >> 6) == J) skip |= 1ull << (j & 63);
+        }
+        if (J == I) skip |= 1ull << lane;
+        const int jn = min(kTile, n - J * kTile);
+        if (!iv) continue;
+        for (int jj = 0; jj < jn; jj++) {
+            if ((skip >> jj) & 1ull) continue;
+            const double4 pj = sp[jj];
+            const V3<double> dr = min_image(mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z), box);
+            if (dot(dr, dr) > rc2) continue;
+            if (cnt < cap) nbr[nl_index(i, cnt, n)] = J * kTile + jj;
+            cnt++;
+        }
+    }
+    if (iv) {
+        count[i] = min(cnt, cap);
+        atomicMax(maxcount, cnt);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// list-driven field kernels: kNlSplit threads per atom, each taking every
+// kNlSplit-th entry; the partial fields meet in a lane reduction and are
+// added once per atom.
+// ---------------------------------------------------------------------------
+template <class R>
+__global__ __launch_bounds__(256) void k_nl_fixed_field(NbrList L, PairAtoms<R> A, PairParams<R> P,
+                                                        acc_t* __restrict__ field) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = g / kNlSplit, sub = g % kNlSplit;
+    const int n = P.n;
+    const bool iv = i < n;
+    const int ic = iv ? i : n - 1;
+    const R4<R> pi = A.pos[ic];
+    const R2<R> tdi = A.td[ic];
+    R mi[kMP];
+#pragma unroll
+    for (int k = 0; k < kMP; k++) mi[k] = A.lab[(size_t)ic * kMP + k];
+    const int cnt = iv ? L.count[ic] : 0;
+    R ex = 0, ey = 0, ez = 0;
+    for (int k = sub; k < cnt; k += kNlSplit) {
+        const int j = L.nbr[nl_index(ic, k, n)];
+        const R4<R> pj = A.pos[j];
+        const V3<R> dr = min_image(mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z), P.box);
+        R mj[kMP];
+#pragma unroll
+        for (int c = 0; c < kMP; c++) mj[c] = A.lab[(size_t)j * kMP + c];
+        V3<R> fi, fj;
+        fixed_field_pair<R>(dr, dot(dr, dr), mi, mj, tdi, A.td[j], R(1), P, fi, fj);
+        ex += fi.x;
+        ey += fi.y;
+        ez += fi.z;
+    }
+#pragma unroll
+    for (int off = 1; off < kNlSplit; off <<= 1) {
+        ex += __shfl_xor(ex, off);
+        ey += __shfl_xor(ey, off);
+        ez += __shfl_xor(ez, off);
+    }
+    if (iv && sub == 0) {
+        acc_add(&field[3 * (size_t)i + 0], ex);
+        acc_add(&field[3 * (size_t)i + 1], ey);
+        acc_add(&field[3 * (size_t)i + 2], ez);
+    }
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void k_nl_induced_field(NbrList L, PairAtoms<R> A, const R* __restrict__ mu,
+                                                          PairParams<R> P, acc_t* __restrict__ field) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = g / kNlSplit, sub = g % kNlSplit;
+    const int n = P.n;
+    const bool iv = i < n;
+    const int ic = iv ? i : n - 1;
+    const R4<R> pi = A.pos[ic];
+    const R2<R> tdi = A.td[ic];
+    const V3<R> ui = mk(mu[3 * (size_t)ic], mu[3 * (size_t)ic + 1], mu[3 * (size_t)ic + 2]);
+    const int cnt = iv ? L.count[ic] : 0;
+    R ex = 0, ey = 0, ez = 0;
+    for (int k = sub; k < cnt; k += kNlSplit) {
+        const int j = L.nbr[nl_index(ic, k, n)];
+        const R4<R> pj = A.pos[j];
+        const V3<R> dr = min_image(mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z), P.box);
+        const V3<R> uj = mk(mu[3 * (size_t)j], mu[3 * (size_t)j + 1], mu[3 * (size_t)j + 2]);
+        V3<R> fi, fj;
+        induced_field_pair<R>(dr, dot(dr, dr), ui, uj, tdi, A.td[j], R(1), P, fi, fj);
+        ex += fi.x;
+        ey += fi.y;
+        ez += fi.z;
+    }
+#pragma unroll
+    for (int off = 1; off < kNlSplit; off <<= 1) {
+        ex += __shfl_xor(ex, off);
+        ey += __shfl_xor(ey, off);
+        ez += __shfl_xor(ez, off);
+    }
+    if (iv && sub == 0) {
+        acc_add(&field[3 * (size_t)i + 0], ex);
+        acc_add(&field[3 * (size_t)i + 1], ey);
+        acc_add(&field[3 * (size_t)i + 2], ez);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+void launch_block_rows(hipStream_t s, int nb, const float4* bmin, const float4* bmax, Box<float> box, float cutoff,
+                       const int* start, int* counts, int* rows) {
+    hipLaunchKernelGGL(k_block_rows, dim3(nb), dim3(kTile), 0, s, nb, bmin, bmax, box, cutoff, start, counts, rows);
+}
+void launch_exclusive_scan(hipStream_t s, int m, const int* counts, int* start) {
+    hipLaunchKernelGGL(k_exclusive_scan, dim3(1), dim3(1024), 0, s, m, counts, start);
+}
+void launch_nl_build(hipStream_t s, int n, const int* rstart, const int* rows, const double4* pos, const int* partners,
+                     const int* npart, Box<double> box, double rc2, int cap, int* nbr, int* count, int* maxcount) {
+    const int nb = (n + kTile - 1) / kTile;
+    hipLaunchKernelGGL(k_nl_build, dim3(nb), dim3(kTile), 0, s, n, rstart, rows, pos, partners, npart, box, rc2, cap,
+                       nbr, count, maxcount);
+}
+template <class R>
+void launch_nl_fixed_field(hipStream_t s, int n, NbrList l, PairAtoms<R> a, PairParams<R> p, acc_t* field) {
+    const size_t threads = (size_t)n * kNlSplit;
+    hipLaunchKernelGGL(k_nl_fixed_field<R>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, l, a, p, field);
+}
+template <class R>
+void launch_nl_induced_field(hipStream_t s, int n, NbrList l, PairAtoms<R> a, const R* mu, PairParams<R> p,
+                             acc_t* field) {
+    const size_t threads = (size_t)n * kNlSplit;
+    hipLaunchKernelGGL(k_nl_induced_field<R>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, l, a, mu, p,
+                       field);
+}
+
+#define MPID_INSTANTIATE(R)                                                                                         \
+    template void launch_nl_fixed_field<R>(hipStream_t, int, NbrList, PairAtoms<R>, PairParams<R>, acc_t*);          \
+    template void launch_nl_induced_field<R>(hipStream_t, int, NbrList, PairAtoms<R>, const R*, PairParams<R>, acc_t*);
+MPID_INSTANTIATE(float)
+MPID_INSTANTIATE(double)
+#undef MPID_INSTANTIATE
+
+}  // namespace mpid

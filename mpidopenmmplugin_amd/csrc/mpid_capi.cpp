@@ -189,6 +189,10 @@ struct mpid_context {
     DBuf<int2> tiles;
     DBuf<int> tile_count;
     int ntiles = 0;
+    // neighbour lists: block rows (CSR) and per-atom lists of capacity nl_cap
+    DBuf<int> brow_count, brow_start, brows, nbr, ncount, nl_scalars;
+    int nl_cap = 0;
+    int nl_max = 0;
     // accumulators
     DBuf<acc_t> field_acc, force_acc, torque_acc, energy_acc;
     // solver
@@ -431,6 +435,10 @@ struct mpid_context {
         bmax.resize(nb);
         tiles.resize((size_t)nb * (nb + 1) / 2);
         tile_count.resize(1);
+        brow_count.resize(nb);
+        brow_start.resize(nb + 1);
+        ncount.resize(n);
+        nl_scalars.resize(1);
         field_acc.resize(n3);
         force_acc.resize(n3);
         torque_acc.resize(n3);
@@ -585,13 +593,52 @@ struct mpid_context {
                                (int)tiles.n);
         });
         check_launch();
-        int cnt = 0;
+        timed(P_TILES, [&] {
+            launch_block_rows(stream, nb, bmin.p, bmax.p, rbox<float>(), (float)opt.cutoff, nullptr, brow_count.p,
+                              nullptr);
+            launch_exclusive_scan(stream, nb, brow_count.p, brow_start.p);
+        });
+        check_launch();
+        int cnt = 0, rows_total = 0;
         HIP_CHECK(hipMemcpyAsync(&cnt, tile_count.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipMemcpyAsync(&rows_total, brow_start.p + nb, sizeof(int), hipMemcpyDeviceToHost, stream));
         HIP_CHECK(hipStreamSynchronize(stream));
         if (cnt > (int)tiles.n) throw MpidError(MPID_ERR_HIP, "tile list overflow");
         ntiles = cnt;
         stats.ntiles = cnt;
+        if ((size_t)rows_total > brows.n) brows.resize(rows_total + rows_total / 4 + 64);
+        build_neighbour_lists(nb);
     }
+
+    // Full per-atom lists.  The capacity starts from the mean number of atoms
+    // in a cutoff sphere; a list that overflows is rebuilt with a larger one.
+    void build_neighbour_lists(int nb) {
+        if (nl_cap == 0) {
+            const double vol = dbox.L[0] * dbox.L[1] * dbox.L[2];
+            const double mean = n / vol * (4.0 / 3.0) * M_PI * opt.cutoff * opt.cutoff * opt.cutoff;
+            nl_cap = std::min<int>((int)(1.3 * mean) + 64, n);
+        }
+        timed(P_TILES, [&] {
+            launch_block_rows(stream, nb, bmin.p, bmax.p, rbox<float>(), (float)opt.cutoff, brow_start.p, nullptr,
+                              brows.p);
+        });
+        for (int attempt = 0; attempt < 2; attempt++) {
+            nl_cap = (nl_cap + kNlSplit - 1) / kNlSplit * kNlSplit;
+            nbr.resize((size_t)n * nl_cap);
+            timed(P_TILES, [&] {
+                HIP_CHECK(hipMemsetAsync(nl_scalars.p, 0, sizeof(int), stream));
+                launch_nl_build(stream, n, brow_start.p, brows.p, pd.pos.p, spart.p, snpart.p, dbox,
+                                opt.cutoff * opt.cutoff, nl_cap, nbr.p, ncount.p, nl_scalars.p);
+            });
+            check_launch();
+            HIP_CHECK(hipMemcpyAsync(&nl_max, nl_scalars.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+            HIP_CHECK(hipStreamSynchronize(stream));
+            if (nl_max <= nl_cap) return;
+            nl_cap = nl_max + nl_max / 8 + kNlSplit;
+        }
+        throw MpidError(MPID_ERR_HIP, "neighbour list overflow after regrowth");
+    }
+    NbrList nlist() const { return NbrList{nbr.p, ncount.p}; }
 
     template <class R> void recip_cycle_fixed(Prec<R>& P) {
         PmeGeom<R> g = geom<R>();
@@ -644,7 +691,8 @@ struct mpid_context {
         recip_cycle_fixed(pd);
         timed(P_FIXED, [&] {
             launch_zero_acc(stream, field_acc.p, field_acc.n);
-            launch_fixed_field_pairs<R>(stream, ntiles, tiles.p, pair_atoms(P), pair_params<R>(), field_acc.p);
+            launch_nl_fixed_field<R>(stream, n, nlist(), pair_atoms(P), pair_params<R>(), field_acc.p);
+            launch_cov_fixed_field<double>(stream, n, pair_atoms(pd), pair_params<double>(), field_acc.p);
         });
         timed(P_OTHER, [&] {
             launch_fixed_field_finish<double>(stream, n, pd.phi.p, lab.p, field_acc.p, gscale3(), selfc(), E0.p);
@@ -656,8 +704,8 @@ struct mpid_context {
     template <class R> void induced_field(Prec<R>& P) {
         timed(P_INDUCED, [&] {
             launch_zero_acc(stream, field_acc.p, field_acc.n);
-            launch_induced_field_pairs<R>(stream, ntiles, tiles.p, pair_atoms(P), P.mu.p, pair_params<R>(),
-                                          field_acc.p);
+            launch_nl_induced_field<R>(stream, n, nlist(), pair_atoms(P), P.mu.p, pair_params<R>(), field_acc.p);
+            launch_cov_induced_field<R>(stream, n, pair_atoms(P), P.mu.p, pair_params<R>(), field_acc.p);
         });
         recip_cycle_induced(P, false);
     }
@@ -788,6 +836,8 @@ struct mpid_context {
             launch_zero_acc(stream, energy_acc.p, 1);
             launch_electrostatics_pairs<R>(stream, ntiles, tiles.p, pair_atoms(P), P.mu.p, pair_params<R>(),
                                            force_acc.p, torque_acc.p, energy_acc.p);
+            launch_cov_electrostatics<double>(stream, n, pair_atoms(pd), mud.p, pair_params<double>(), force_acc.p,
+                                              torque_acc.p, energy_acc.p);
         });
         check_launch();
         acc_t ereal = 0;

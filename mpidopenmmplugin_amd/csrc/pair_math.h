@@ -1,0 +1,508 @@
+// Pair interaction formulas shared by the tile, neighbour-list and covalent
+// pair kernels (device functions, inlined into each kernel).
+//
+// Physics follows MPIDReferencePmeForce
+// (platforms/reference/src/SimTKReference/MPIDReferenceForce.cpp):
+//   fixed field     :2812-2920 (+ getDampedInverseDistances :2678-2718)
+//   induced field   :4161-4281
+//   electrostatics  :4335-4920 (quasi-internal frame)
+#pragma once
+
+#include "engine.h"
+
+namespace mpid {
+
+// ---------------------------------------------------------------------------
+// shared helpers
+// ---------------------------------------------------------------------------
+// Scale of a covalent partner entry (partner << 2 | class).
+template <class R> __device__ __forceinline__ R partner_scale(int e, R s14) {
+    return ((e & 3) == kScaleZero) ? R(0) : s14;
+}
+
+// Quadrupole times vector and octopole contracted twice with a vector.
+template <class R> __device__ __forceinline__ V3<R> quad_dot(const R* Q, V3<R> v) {
+    return {Q[0] * v.x + Q[1] * v.y + Q[2] * v.z, Q[1] * v.x + Q[3] * v.y + Q[4] * v.z,
+            Q[2] * v.x + Q[4] * v.y + Q[5] * v.z};
+}
+// M = O . v (symmetric 3x3, stored XX XY XZ YY YZ ZZ)
+template <class R> __device__ __forceinline__ void oct_dot(const R* O, V3<R> v, R* M) {
+    M[0] = O[0] * v.x + O[1] * v.y + O[2] * v.z;  // xx.
+    M[1] = O[1] * v.x + O[3] * v.y + O[4] * v.z;  // xy.
+    M[2] = O[2] * v.x + O[4] * v.y + O[5] * v.z;  // xz.
+    M[3] = O[3] * v.x + O[6] * v.y + O[7] * v.z;  // yy.
+    M[4] = O[4] * v.x + O[7] * v.y + O[8] * v.z;  // yz.
+    M[5] = O[5] * v.x + O[8] * v.y + O[9] * v.z;  // zz.
+}
+template <class R> __device__ __forceinline__ R quad_form(const R* M, V3<R> a, V3<R> b) {
+    return dot(a, quad_dot(M, b));
+}
+
+// ---------------------------------------------------------------------------
+// fixed multipole field
+// ---------------------------------------------------------------------------
+// Field at i from the permanent moments of j (fi) and at j from i (fj), with
+// dr = r_j - r_i inside the cutoff and ds the multipole/polarization scale of
+// the pair (MPIDReferenceForce.cpp:2812-2920).
+template <class R>
+__device__ __forceinline__ void fixed_field_pair(V3<R> dr, R r2, const R* mi, const R* mj, R2<R> tdi, R2<R> tdj,
+                                                 R ds, const PairParams<R>& P, V3<R>& fi, V3<R>& fj) {
+    const R rinv = rsqrt_r(r2);
+    const R r = r2 * rinv;
+    const R rinv2 = rinv * rinv;
+    const R ar = P.alpha * r;
+    const R ex2 = exp_r(-ar * ar);
+    const R bn0 = erfc_r(ar) * rinv;
+    const R bn1 = (bn0 + P.aln[1] * ex2) * rinv2;
+    const R bn2 = (R(3) * bn1 + P.aln[2] * ex2) * rinv2;
+    const R bn3 = (R(5) * bn2 + P.aln[3] * ex2) * rinv2;
+    const R bn4 = (R(7) * bn3 + P.aln[4] * ex2) * rinv2;
+    // Thole-damped bare terms: (1 - ds*lambda_k) * (2k-1)!! / r^(2k+1)
+    R l3 = R(1), l5 = R(1), l7 = R(1), l9 = R(1);
+    const R damp = tdi.y * tdj.y;
+    if (damp != R(0)) {
+        const R a = (ds == R(0)) ? tdi.x + tdj.x : P.default_thole;
+        const R u = a * r / damp;
+        if (u < R(50)) {
+            const R e = exp_r(-u), u2 = u * u, u3 = u2 * u, u4 = u3 * u, u5 = u4 * u;
+            const R p3 = R(1) + u + R(0.5) * u2;
+            const R p5 = p3 + u3 * R(1.0 / 6.0);
+            const R p7 = p5 + u4 * R(1.0 / 30.0);
+            const R p9 = p5 + u4 * R(4.0 / 105.0) + u5 * R(1.0 / 210.0);
+            l3 = R(1) - e * p3;
+            l5 = R(1) - e * p5;
+            l7 = R(1) - e * p7;
+            l9 = R(1) - e * p9;
+        }
+    }
+    const R rinv3 = rinv * rinv2, rinv5 = rinv3 * rinv2, rinv7 = rinv5 * rinv2, rinv9 = rinv7 * rinv2;
+    const R g1 = bn1 - (R(1) - ds * l3) * rinv3;
+    const R g2 = bn2 - R(3) * (R(1) - ds * l5) * rinv5;
+    const R g3 = bn3 - R(15) * (R(1) - ds * l7) * rinv7;
+    const R g4 = bn4 - R(105) * (R(1) - ds * l9) * rinv9;
+    // site j moments contracted with r
+    const V3<R> dj = mk(mj[1], mj[2], mj[3]);
+    const V3<R> qj = quad_dot(mj + 4, dr);
+    R Mj[6];
+    oct_dot(mj + 10, dr, Mj);
+    const V3<R> oj = quad_dot(Mj, dr);
+    const R djr = dot(dj, dr), qjr = dot(qj, dr), ojr = dot(oj, dr);
+    const V3<R> di = mk(mi[1], mi[2], mi[3]);
+    const V3<R> qi = quad_dot(mi + 4, dr);
+    R Mi[6];
+    oct_dot(mi + 10, dr, Mi);
+    const V3<R> oi = quad_dot(Mi, dr);
+    const R dir = dot(di, dr), qir = dot(qi, dr), oir = dot(oi, dr);
+    fi = qj * (R(2) * g2) - dj * g1 - dr * (g1 * mj[0] - g2 * djr + g3 * qjr - g4 * ojr) - oj * (R(3) * g3);
+    fj = qi * (R(-2) * g2) - di * g1 + dr * (g1 * mi[0] + g2 * dir + g3 * qir + g4 * oir) - oi * (R(3) * g3);
+}
+
+// Lanes of block J that are covalent partners of the lane's atom: those pairs
+// are left to the covalent-pair kernels.
+__device__ __forceinline__ unsigned long long partner_mask(const int* __restrict__ parts, int np, int J) {
+    unsigned long long m = 0;
+    for (int k = 0; k < np; k++) {
+        const int j = parts[k] >> 2;
+        if ((j >> 6) == J) m |= 1ull << (j & 63);
+    }
+    return m;
+}
+
+// ---------------------------------------------------------------------------
+// induced dipole field (the real-space part of the solver's matrix-vector product)
+// ---------------------------------------------------------------------------
+// MPIDReferenceForce.cpp:4161-4281; ps selects the Thole width of the pair.
+template <class R>
+__device__ __forceinline__ void induced_field_pair(V3<R> dr, R r2, V3<R> ui, V3<R> uj, R2<R> tdi, R2<R> tdj, R ps,
+                                                   const PairParams<R>& P, V3<R>& fi, V3<R>& fj) {
+    const R rinv = rsqrt_r(r2);
+    const R r = r2 * rinv;
+    const R rinv2 = rinv * rinv;
+    const R ar = P.alpha * r;
+    const R ex2 = exp_r(-ar * ar);
+    const R bn0 = erfc_r(ar) * rinv;
+    const R bn1 = (bn0 + P.aln[1] * ex2) * rinv2;
+    const R bn2 = (R(3) * bn1 + P.aln[2] * ex2) * rinv2;
+    // (1 - lambda) of the Thole damping, computed directly
+    R m3 = R(0), m5 = R(0);
+    const R damp = tdi.y * tdj.y;
+    if (damp != R(0)) {
+        const R a = (ps == R(0)) ? tdi.x + tdj.x : P.default_thole;
+        const R u = a * r / damp;
+        if (u < R(50)) {
+            const R e = exp_r(-u);
+            m3 = e * (R(1) + u + R(0.5) * u * u);
+            m5 = m3 + e * u * u * u * R(1.0 / 6.0);
+        }
+    }
+    const R rinv3 = rinv * rinv2;
+    const R f1 = m3 * rinv3 - bn1;
+    const R f2 = bn2 - R(3) * m5 * rinv3 * rinv2;
+    const R ujr = dot(uj, dr) * f2, uir = dot(ui, dr) * f2;
+    fi = dr * ujr + uj * f1;
+    fj = dr * uir + ui * f1;
+}
+
+// ---------------------------------------------------------------------------
+// Final real-space energy, force and torque in the quasi-internal (QI) frame.
+// ---------------------------------------------------------------------------
+// Spherical moments in a frame (ex, ey, ez): Cartesian tensors are contracted
+// with the frame axes and projected as in MPIDReferenceForce.cpp:321-342.
+template <class R>
+__device__ __forceinline__ void qi_moments(const R* m, V3<R> ex, V3<R> ey, V3<R> ez, R* q) {
+    const R s3 = R(1.7320508075688772);
+    q[0] = m[0];
+    const V3<R> d = mk(m[1], m[2], m[3]);
+    q[1] = dot(ez, d);
+    q[2] = dot(ex, d);
+    q[3] = dot(ey, d);
+    const R* Q = m + 4;
+    const V3<R> Qz = quad_dot(Q, ez), Qx = quad_dot(Q, ex), Qy = quad_dot(Q, ey);
+    const R qzz = dot(ez, Qz), qxz = dot(ex, Qz), qyz = dot(ey, Qz);
+    const R qxx = dot(ex, Qx), qyy = dot(ey, Qy), qxy = dot(ey, Qx);
+    q[4] = R(3) * qzz;
+    q[5] = R(2) * s3 * qxz;
+    q[6] = R(2) * s3 * qyz;
+    q[7] = s3 * (qxx - qyy);
+    q[8] = R(2) * s3 * qxy;
+    const R* O = m + 10;
+    R Mz[6], Mx[6], My[6];
+    oct_dot(O, ez, Mz);
+    oct_dot(O, ex, Mx);
+    oct_dot(O, ey, My);
+    const V3<R> wz = quad_dot(Mz, ez);
+    const R ozzz = dot(ez, wz), oxzz = dot(ex, wz), oyzz = dot(ey, wz);
+    const R oxxz = quad_form(Mz, ex, ex), oyyz = quad_form(Mz, ey, ey), oxyz = quad_form(Mz, ex, ey);
+    const R oxxx = quad_form(Mx, ex, ex), oxyy = quad_form(Mx, ey, ey), oxxy = quad_form(Mx, ex, ey);
+    const R oyyy = quad_form(My, ey, ey);
+    const R r32 = R(1.2247448713915890), r35 = R(0.7745966692414834), r110 = R(0.31622776601683794);
+    q[9] = R(15) * ozzz;
+    q[10] = R(15) * r32 * oxzz;
+    q[11] = R(15) * r32 * oyzz;
+    q[12] = R(15) * r35 * (oxxz - oyyz);
+    q[13] = R(30) * r35 * oxyz;
+    q[14] = R(15) * r110 * (oxxx - R(3) * oxyy);
+    q[15] = R(15) * r110 * (R(3) * oxxy - oyyy);
+}
+
+// sum_i (dq_i/dtheta_k) V_i for rotations about the QI x, y and z axes.  The
+// generator tables were derived symbolically from the Cartesian definition of
+// the moments; they coincide with the hand-written tables at
+// MPIDReferenceForce.cpp:4438-4455.
+template <class R> __device__ __forceinline__ void qi_torque_sums(const R* q, const R* V, R& tx, R& ty, R& tz) {
+    const R s3 = R(1.7320508075688772), s6 = R(2.449489742783178), s52 = R(1.5811388300841898),
+            s32 = R(1.2247448713915890);
+    tx = q[3] * V[1] - q[1] * V[3] + s3 * q[6] * V[4] + q[8] * V[5] + (-s3 * q[4] - q[7]) * V[6] + q[6] * V[7] -
+         q[5] * V[8] + s6 * q[11] * V[9] + s52 * q[13] * V[10] + (-s52 * q[12] - s6 * q[9]) * V[11] +
+         (s52 * q[11] + s32 * q[15]) * V[12] + (-s52 * q[10] - s32 * q[14]) * V[13] + s32 * q[13] * V[14] -
+         s32 * q[12] * V[15];
+    ty = -q[2] * V[1] + q[1] * V[2] - s3 * q[5] * V[4] + (s3 * q[4] - q[7]) * V[5] - q[8] * V[6] + q[5] * V[7] +
+         q[6] * V[8] - s6 * q[10] * V[9] + (-s52 * q[12] + s6 * q[9]) * V[10] - s52 * q[13] * V[11] +
+         (s52 * q[10] - s32 * q[14]) * V[12] + (s52 * q[11] - s32 * q[15]) * V[13] + s32 * q[12] * V[14] +
+         s32 * q[13] * V[15];
+    tz = -q[3] * V[2] + q[2] * V[3] - q[6] * V[5] + q[5] * V[6] - R(2) * q[8] * V[7] + R(2) * q[7] * V[8] -
+         q[11] * V[10] + q[10] * V[11] - R(2) * q[13] * V[12] + R(2) * q[12] * V[13] - R(3) * q[15] * V[14] +
+         R(3) * q[14] * V[15];
+}
+
+template <class R> struct ElecOut {
+    V3<R> f;    // force on j (force on i is -f)
+    V3<R> ti, tj;
+    R energy;
+};
+
+template <class R>
+__device__ __forceinline__ ElecOut<R> elec_pair(V3<R> dr, R r2, const R* ma, const R* mb, V3<R> mua, V3<R> mub,
+                                                R2<R> tda, R2<R> tdb, bool anisoA, bool anisoB, R ms, R ps,
+                                                const PairParams<R>& P) {
+    const R rinv = rsqrt_r(r2);
+    const R r = r2 * rinv;
+    // QI frame (formQIRotationMatrix, MPIDReferenceForce.cpp:650-681): z along r,
+    // x = the lab x axis made orthogonal to z (the lab y axis when r lies on x).
+    // Written as (s, -zx zy / s, -zx zz / s), s = sqrt(zy^2 + zz^2), which is
+    // the same vector without the cancellation of 1 - zx^2.  The choice of x
+    // matters: the reference's induced-induced z torque is not invariant
+    // under rotations about r (see P.fix_torque).
+    const V3<R> ez = dr * rinv;
+    V3<R> ex;
+    const R syz2 = ez.y * ez.y + ez.z * ez.z;
+    if (dr.y != R(0) || dr.z != R(0)) {
+        const R is = rsqrt_r(syz2);
+        ex = mk(syz2 * is, -ez.x * ez.y * is, -ez.x * ez.z * is);
+    } else {
+        ex = mk(R(0), R(1), R(0));
+    }
+    const V3<R> ey = cross(ez, ex);
+    R qa[16], qb[16];
+    qi_moments(ma, ex, ey, ez, qa);
+    qi_moments(mb, ex, ey, ez, qb);
+    R ua[3] = {R(0.5) * dot(ez, mua), R(0.5) * dot(ex, mua), R(0.5) * dot(ey, mua)};
+    R ub[3] = {R(0.5) * dot(ez, mub), R(0.5) * dot(ex, mub), R(0.5) * dot(ey, mub)};
+
+    // Ewald and Thole radial factors.  b'_k = 1 + bVec[k] of the reference
+    // (computed from erfc, so ms + bVec[k] = (ms - 1) + b'_k loses nothing).
+    R ri[9];
+    ri[1] = R(kElectric) * rinv;
+#pragma unroll
+    for (int k = 2; k < 9; k++) ri[k] = ri[k - 1] * rinv;
+    R ar[10];
+    ar[1] = P.alpha * r;
+#pragma unroll
+    for (int k = 2; k < 10; k++) ar[k] = ar[k - 1] * ar[1];
+    const R X = R(2.0 / kSqrtPi) * exp_r(-ar[2]);
+    const R b2 = erfc_r(ar[1]) + ar[1] * X;
+    const R b3 = b2 + R(2.0 / 3.0) * ar[3] * X;
+    const R b4 = b3 + R(4.0 / 15.0) * ar[5] * X;
+    const R b5 = b4 + R(8.0 / 105.0) * ar[7] * X;
+    const R m1 = ms - R(1);
+    const R mB2 = m1 + b2, mB3 = m1 + b3, mB4 = m1 + b4, mB5 = m1 + b5;
+    const R dmp = tda.y * tdb.y;
+    const R a = (ps == R(0)) ? tda.x + tdb.x : P.default_thole;
+    const R u = fabs(dmp) > R(1e-5) ? r / dmp : R(1e10);
+    // exp(-au) vanishes for au >= 50; the polynomial argument is clamped there so
+    // that its powers stay finite in FP32 (u = 1e10 when a damping factor is 0)
+    const R au = fmin(a * u, R(50));
+    const R e = au < R(50) ? exp_r(-au) : R(0);
+    const R au2 = au * au, au3 = au2 * au, au4 = au3 * au, au5 = au4 * au, au6 = au5 * au;
+    // th - 1 for each Thole factor (MPIDReferenceForce.cpp:4509-4523)
+    const R base = R(1) + au + R(0.5) * au2;
+    const R t_c = -e * base;
+    const R t_d0 = -e * (base + au3 * R(0.25));
+    const R t_d1 = t_c;
+    const R t_q0 = -e * (base + au3 * R(1.0 / 6.0) + au4 * R(1.0 / 18.0));
+    const R t_q1 = -e * (base + au3 * R(1.0 / 6.0));
+    const R t_o0 = -e * (base + au3 * R(1.0 / 6.0) + au4 * R(1.0 / 24.0) + au5 * R(1.0 / 120.0));
+    const R t_o1 = -e * (base + au3 * R(1.0 / 6.0) + au4 * R(1.0 / 30.0));
+    const R dt_c = -e * (base + au3 * R(0.25));
+    const R dt_d0 = -e * (base + au3 * R(1.0 / 6.0) + au4 * R(1.0 / 12.0));
+    const R dt_d1 = -e * (base + au3 * R(1.0 / 6.0));
+    const R dt_q0 = -e * (base + au3 * R(1.0 / 6.0) + au4 * R(1.0 / 24.0) + au5 * R(1.0 / 72.0));
+    const R dt_q1 = -e * (base + au3 * R(1.0 / 6.0) + au4 * R(1.0 / 24.0));
+    const R dt_o0 = -e * (base + au3 * R(1.0 / 6.0) + au4 * R(1.0 / 24.0) + au5 * R(1.0 / 120.0) + au6 * R(1.0 / 600.0));
+    const R dt_o1 = -e * (base + au3 * R(1.0 / 6.0) + au4 * R(1.0 / 25.0) + au5 * R(1.0 / 150.0));
+    // scale*th + bVec[k] = (scale - 1) + scale*(th - 1) + b'_k
+    const R p1 = ps - R(1);
+    auto sc = [&](R th1) { return p1 + ps * th1; };
+    const R X3 = ar[3] * X, X5 = ar[5] * X, X7 = ar[7] * X, X9 = ar[9] * X;
+
+    R Va[16], Vb[16], VaR[16], VbR[16], Vad[3], Vbd[3];
+    R eP, dP, eU, dU;
+    // C-C (m=0)
+    eP = ri[1] * (mB2 - ar[1] * X);
+    dP = R(-0.5) * mB2 * ri[2];
+    Va[0] = eP * qb[0]; Vb[0] = eP * qa[0]; VaR[0] = dP * qb[0]; VbR[0] = dP * qa[0];
+    // C-D, C-Uind (m=0)
+    eP = ri[2] * mB2;
+    eU = R(2) * ri[2] * (sc(t_c) + b2);
+    dP = -ri[3] * (mB2 + X3);
+    dU = R(-4) * ri[3] * (sc(dt_c) + b2 + X3);
+    Va[0] += -(eP * qb[1] + eU * ub[0]); Vb[1] = -(eP * qa[0]);
+    VaR[0] += -(dP * qb[1] + dU * ub[0]); VbR[1] = -(dP * qa[0]);
+    Vbd[0] = -(eU * qa[0]);
+    Va[1] = eP * qb[0]; Vb[0] += eP * qa[1] + eU * ua[0];
+    VaR[1] = dP * qb[0]; VbR[0] += dP * qa[1] + dU * ua[0];
+    Vad[0] = eU * qb[0];
+    // D-D, D-Uind (m=0)
+    eP = R(-2.0 / 3.0) * ri[3] * (R(3) * mB3 + X3);
+    eU = R(-4.0 / 3.0) * ri[3] * (R(3) * (sc(t_d0) + b3) + X3);
+    dP = ri[4] * (R(3) * mB3 + R(2) * X5);
+    dU = R(2) * ri[4] * (R(6) * (sc(dt_d0) + b3) + R(4) * X5);
+    Va[1] += eP * qb[1] + eU * ub[0]; Vb[1] += eP * qa[1] + eU * ua[0];
+    VaR[1] += dP * qb[1] + dU * ub[0]; VbR[1] += dP * qa[1] + dU * ua[0];
+    Vad[0] += eU * qb[1]; Vbd[0] += eU * qa[1];
+    // D-D, D-Uind (m=1)
+    eP = ri[3] * (mB3 - R(2.0 / 3.0) * X3);
+    eU = R(2) * ri[3] * (sc(t_d1) + b3 - R(2.0 / 3.0) * X3);
+    dP = R(-1.5) * ri[4] * mB3;
+    dU = R(-6) * ri[4] * (sc(dt_d1) + b3);
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        Va[2 + c] = eP * qb[2 + c] + eU * ub[1 + c]; Vb[2 + c] = eP * qa[2 + c] + eU * ua[1 + c];
+        VaR[2 + c] = dP * qb[2 + c] + dU * ub[1 + c]; VbR[2 + c] = dP * qa[2 + c] + dU * ua[1 + c];
+        Vad[1 + c] = eU * qb[2 + c]; Vbd[1 + c] = eU * qa[2 + c];
+    }
+    // C-Q, Q-C (m=0)
+    eP = mB3 * ri[3];
+    dP = R(-1.0 / 3.0) * ri[4] * (R(4.5) * mB3 + R(2) * X5);
+    Va[0] += eP * qb[4]; Vb[4] = eP * qa[0]; VaR[0] += dP * qb[4]; VbR[4] = dP * qa[0];
+    Va[4] = eP * qb[0]; Vb[0] += eP * qa[4]; VaR[4] = dP * qb[0]; VbR[0] += dP * qa[4];
+    // D-Q, Uind-Q (m=0)
+    eP = ri[4] * (R(3) * mB3 + R(4.0 / 3.0) * X5);
+    eU = R(2) * ri[4] * (R(3) * (sc(t_q0) + b3) + R(4.0 / 3.0) * X5);
+    dP = R(-4.0 / 3.0) * ri[5] * (R(4.5) * mB3 + (R(1) + ar[2]) * X5);
+    dU = R(-8.0 / 3.0) * ri[5] * (R(9) * (sc(dt_q0) + b3) + R(2) * (R(1) + ar[2]) * X5);
+    Va[1] += eP * qb[4]; Vb[4] += eP * qa[1] + eU * ua[0];
+    VaR[1] += dP * qb[4]; VbR[4] += dP * qa[1] + dU * ua[0];
+    Vad[0] += eU * qb[4];
+    Va[4] += -(eP * qb[1] + eU * ub[0]); Vb[1] += -(eP * qa[4]);
+    VaR[4] += -(dP * qb[1] + dU * ub[0]); VbR[1] += -(dP * qa[4]);
+    Vbd[0] += -(eU * qa[4]);
+    // D-Q, Uind-Q (m=1)
+    eP = R(-1.7320508075688772) * ri[4] * mB3;
+    eU = R(-2 * 1.7320508075688772) * ri[4] * (sc(t_q1) + b3);
+    dP = R(4.0 / 1.7320508075688772) * ri[5] * (R(1.5) * mB3 + R(0.5) * X5);
+    dU = R(8.0 / 1.7320508075688772) * ri[5] * (R(3) * (sc(dt_q1) + b3) + X5);
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        Va[2 + c] += eP * qb[5 + c]; Vb[5 + c] = eP * qa[2 + c] + eU * ua[1 + c];
+        VaR[2 + c] += dP * qb[5 + c]; VbR[5 + c] = dP * qa[2 + c] + dU * ua[1 + c];
+        Vad[1 + c] += eU * qb[5 + c];
+        Va[5 + c] = -(eP * qb[2 + c] + eU * ub[1 + c]); Vb[2 + c] += -(eP * qa[5 + c]);
+        VaR[5 + c] = -(dP * qb[2 + c] + dU * ub[1 + c]); VbR[2 + c] += -(dP * qa[5 + c]);
+        Vbd[1 + c] += -(eU * qa[5 + c]);
+    }
+    // Q-Q (m=0, 1, 2)
+    eP = ri[5] * (R(6) * mB4 + R(4.0 / 45.0) * (R(-3) + R(10) * ar[2]) * X5);
+    dP = R(-1.0 / 9.0) * ri[6] * (R(135) * mB4 + R(4) * (R(1) + R(2) * ar[2]) * X7);
+    Va[4] += eP * qb[4]; Vb[4] += eP * qa[4]; VaR[4] += dP * qb[4]; VbR[4] += dP * qa[4];
+    eP = R(-4.0 / 15.0) * ri[5] * (R(15) * mB4 + X5);
+    dP = ri[6] * (R(10) * mB4 + R(4.0 / 3.0) * X7);
+#pragma unroll
+    for (int c = 5; c < 7; c++) {
+        Va[c] += eP * qb[c]; Vb[c] += eP * qa[c]; VaR[c] += dP * qb[c]; VbR[c] += dP * qa[c];
+    }
+    eP = ri[5] * (mB4 - R(4.0 / 15.0) * X5);
+    dP = R(-2.5) * mB4 * ri[6];
+#pragma unroll
+    for (int c = 7; c < 9; c++) {
+        Va[c] = eP * qb[c]; Vb[c] = eP * qa[c]; VaR[c] = dP * qb[c]; VbR[c] = dP * qa[c];
+    }
+    // C-O, O-C (m=0)
+    eP = ri[4] * (-mB3 - R(4.0 / 15.0) * X5);
+    dP = R(2.0 / 15.0) * ri[5] * (R(15) * mB3 + R(2) * (R(2) * X5 + X7));
+    Va[0] += eP * qb[9]; Vb[9] = eP * qa[0]; VaR[0] += dP * qb[9]; VbR[9] = dP * qa[0];
+    Va[9] = -eP * qb[0]; Vb[0] -= eP * qa[9]; VaR[9] = -dP * qb[0]; VbR[0] -= dP * qa[9];
+    // D-O, Uind-O (m=0)
+    eP = R(-4) * ri[5] * (mB4 + R(2.0 / 15.0) * X7);
+    eU = R(-8) * ri[5] * (sc(t_o0) + b4 + R(2.0 / 15.0) * X7);
+    dP = R(2.0 / 15.0) * ri[6] * (R(75) * mB4 + R(4) * (R(1) + ar[2]) * X7);
+    dU = R(8.0 / 15.0) * ri[6] * (R(75) * (sc(dt_o0) + b4) + R(4) * (R(1) + ar[2]) * X7);
+    Va[1] += eP * qb[9]; Vb[9] += eP * qa[1] + eU * ua[0];
+    VaR[1] += dP * qb[9]; VbR[9] += dP * qa[1] + dU * ua[0];
+    Va[9] += eP * qb[1] + eU * ub[0]; Vb[1] += eP * qa[9];
+    VaR[9] += dP * qb[1] + dU * ub[0]; VbR[1] += dP * qa[9];
+    Vad[0] += eU * qb[9]; Vbd[0] += eU * qa[9];
+    // D-O, Uind-O (m=1)
+    const R s6 = R(2.449489742783178);
+    const R c23 = R(0.2 * 0.816496580927726);
+    eP = s6 * mB4 * ri[5];
+    eU = R(2) * s6 * (sc(t_o1) + b4) * ri[5];
+    dP = R(-0.5) * c23 * ri[6] * (R(75) * mB4 + R(8) * X7);
+    dU = R(-2) * c23 * ri[6] * (R(75) * (sc(dt_o1) + b4) + R(8) * X7);
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        Va[2 + c] += eP * qb[10 + c]; Vb[10 + c] = eP * qa[2 + c] + eU * ua[1 + c];
+        VaR[2 + c] += dP * qb[10 + c]; VbR[10 + c] = dP * qa[2 + c] + dU * ua[1 + c];
+        Vad[1 + c] += eU * qb[10 + c];
+        Va[10 + c] = eP * qb[2 + c] + eU * ub[1 + c]; Vb[2 + c] += eP * qa[10 + c];
+        VaR[10 + c] = dP * qb[2 + c] + dU * ub[1 + c]; VbR[2 + c] += dP * qa[10 + c];
+        Vbd[1 + c] += eU * qa[10 + c];
+    }
+    // Q-O, O-Q (m=0)
+    eP = ri[6] * (R(-10) * mB4 - R(8.0 / 45.0) * (R(3) + R(2) * ar[2]) * X7);
+    dP = R(2.0 / 45.0) * ri[7] * (R(675) * mB4 + R(2) * (R(27) + R(4) * ar[4]) * X7);
+    Va[4] += eP * qb[9]; Vb[9] += eP * qa[4]; VaR[4] += dP * qb[9]; VbR[9] += dP * qa[4];
+    Va[9] -= eP * qb[4]; Vb[4] -= eP * qa[9]; VaR[9] -= dP * qb[4]; VbR[4] -= dP * qa[9];
+    // Q-O, O-Q (m=1)
+    eP = R(5 * 1.4142135623730951) * ri[6] * (mB4 + R(8.0 / 75.0) * X7);
+    dP = R(-0.5 * 2.8284271247461903 / 15.0) * ri[7] * (R(225) * mB4 + R(8) * (R(2) + ar[2]) * X7);
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        Va[5 + c] += eP * qb[10 + c]; Vb[10 + c] += eP * qa[5 + c];
+        VaR[5 + c] += dP * qb[10 + c]; VbR[10 + c] += dP * qa[5 + c];
+        Va[10 + c] -= eP * qb[5 + c]; Vb[5 + c] -= eP * qa[10 + c];
+        VaR[10 + c] -= dP * qb[5 + c]; VbR[5 + c] -= dP * qa[10 + c];
+    }
+    // Q-O, O-Q (m=2)
+    eP = R(-2.23606797749979) * mB4 * ri[6];
+    dP = R(2.23606797749979 / 15.0) * ri[7] * (R(45) * mB4 + R(4) * X7);
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        Va[7 + c] += eP * qb[12 + c]; Vb[12 + c] = eP * qa[7 + c];
+        VaR[7 + c] += dP * qb[12 + c]; VbR[12 + c] = dP * qa[7 + c];
+        Va[12 + c] = -eP * qb[7 + c]; Vb[7 + c] -= eP * qa[12 + c];
+        VaR[12 + c] = -dP * qb[7 + c]; VbR[7 + c] -= dP * qa[12 + c];
+    }
+    // O-O (m=0..3)
+    eP = ri[7] * (R(-20) * mB5 - R(8.0 / 1575.0) * (R(15) + R(28) * ar[2] + R(28) * ar[4]) * X7);
+    dP = R(2.0 / 225.0) * ri[8] * (R(7875) * mB5 + R(4) * (R(41) - R(4) * ar[2] + R(4) * ar[4]) * X9);
+    Va[9] += eP * qb[9]; Vb[9] += eP * qa[9]; VaR[9] += dP * qb[9]; VbR[9] += dP * qa[9];
+    eP = ri[7] * (R(15) * mB5 + R(8.0 / 525.0) * (R(-5) + R(28) * ar[2]) * X7);
+    dP = R(-1.0 / 150.0) * ri[8] * (R(7875) * mB5 + R(32) * (R(3) + R(2) * ar[2]) * X9);
+#pragma unroll
+    for (int c = 10; c < 12; c++) {
+        Va[c] += eP * qb[c]; Vb[c] += eP * qa[c]; VaR[c] += dP * qb[c]; VbR[c] += dP * qa[c];
+    }
+    eP = ri[7] * (R(-6) * mB5 - R(8.0 / 105.0) * X7);
+    dP = R(0.5) * ri[8] * (R(42) * mB5 + R(16.0 / 15.0) * X9);
+#pragma unroll
+    for (int c = 12; c < 14; c++) {
+        Va[c] += eP * qb[c]; Vb[c] += eP * qa[c]; VaR[c] += dP * qb[c]; VbR[c] += dP * qa[c];
+    }
+    eP = ri[7] * (mB5 - R(8.0 / 105.0) * X7);
+    dP = R(-3.5) * mB5 * ri[8];
+#pragma unroll
+    for (int c = 14; c < 16; c++) {
+        Va[c] = eP * qb[c]; Vb[c] = eP * qa[c]; VaR[c] = dP * qb[c]; VbR[c] = dP * qa[c];
+    }
+
+    ElecOut<R> o;
+    R energy = R(0), fIZ = R(0), fJZ = R(0);
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        energy += qa[k] * Va[k] + qb[k] * Vb[k];
+        fIZ += qa[k] * VaR[k];
+        fJZ += qb[k] * VbR[k];
+    }
+    o.energy = R(0.5) * energy;
+    R EIX, EIY, EIZ, EJX, EJY, EJZ;
+    qi_torque_sums(qa, Va, EIX, EIY, EIZ);
+    qi_torque_sums(qb, Vb, EJX, EJY, EJZ);
+    // induced torque intermediates
+    R iEIX = ua[2] * Vad[0] - ua[0] * Vad[2];
+    R iEJX = ub[2] * Vbd[0] - ub[0] * Vbd[2];
+    R iEIY = ua[0] * Vad[1] - ua[1] * Vad[0];
+    R iEJY = ub[0] * Vbd[1] - ub[1] * Vbd[0];
+    R iEIZ = ua[1] * Vad[2] - ua[2] * Vad[1];
+    R iEJZ = ub[1] * Vbd[2] - ub[2] * Vbd[1];
+    if (P.mutual) {
+        // induced-induced terms, never scaled (uScale = 1)
+        R eC = R(-8.0 / 3.0) * ri[3] * (R(3) * (t_d0 + b3) + X3);
+        R dC = R(2) * ri[4] * (R(6) * (dt_d0 + b3) + R(4) * X5);
+        iEIX += eC * ua[2] * ub[0];
+        iEJX += eC * ub[2] * ua[0];
+        iEIY -= eC * ua[1] * ub[0];
+        iEJY -= eC * ub[1] * ua[0];
+        fIZ += dC * ua[0] * ub[0];
+        fJZ += dC * ub[0] * ua[0];
+        eC = R(4) * ri[3] * (t_d1 + b3 - R(2.0 / 3.0) * X3);
+        dC = R(-6) * ri[4] * (dt_d1 + b3);
+        iEIX -= eC * ua[0] * ub[2];
+        iEJX -= eC * ub[0] * ua[2];
+        iEIY += eC * ua[0] * ub[1];
+        iEJY += eC * ub[0] * ua[1];
+        iEIZ += eC * ua[1] * ub[2];
+        iEJZ += eC * ub[1] * ua[2];
+        if (P.fix_torque) {
+            // the second half of the rotational derivative about r, absent at
+            // MPIDReferenceForce.cpp:4854-4855; with it the torques on
+            // anisotropic sites are exact derivatives of the energy
+            iEIZ -= eC * ua[2] * ub[1];
+            iEJZ -= eC * ub[2] * ua[1];
+        }
+        fIZ += dC * (ua[1] * ub[1] + ua[2] * ub[2]);
+        fJZ += dC * (ub[1] * ua[1] + ub[2] * ua[2]);
+    }
+    const R qfx = rinv * (EIY + EJY + iEIY + iEJY);
+    const R qfy = -rinv * (EIX + EJX + iEIX + iEJX);
+    const R qfz = -(fJZ + fIZ);
+    R tix = -EIX, tiy = -EIY, tiz = -EIZ, tjx = -EJX, tjy = -EJY, tjz = -EJZ;
+    if (anisoA) { tix -= iEIX; tiy -= iEIY; tiz -= iEIZ; }
+    if (anisoB) { tjx -= iEJX; tjy -= iEJY; tjz -= iEJZ; }
+    o.f = ex * qfx + ey * qfy + ez * qfz;
+    o.ti = ex * tix + ey * tiy + ez * tiz;
+    o.tj = ex * tjx + ey * tjy + ez * tjz;
+    return o;
+}
+
+}  // namespace mpid
