@@ -106,14 +106,16 @@ def run(args):
           flush=True)
 
     # per-kernel device time (HIP events on the evaluator's stream), separate pass
-    km.setProfiling(True)
-    km.profile(reset=True)
-    nprof = max(1, min(3, args.steps))
-    for _ in range(nprof):
-        km.execute_device(d_pos.data_ptr(), box, d_f.data_ptr())
-    prof = km.profile(reset=True)
-    km.setProfiling(False)
-    kernels = {k: {"ms_per_step": v[0] / nprof, "launches_per_step": v[1] / nprof} for k, v in prof.items()}
+    def kernel_times(k, nprof):
+        k.setProfiling(True)
+        k.profile(reset=True)
+        for _ in range(nprof):
+            k.execute_device(d_pos.data_ptr(), box, d_f.data_ptr())
+        prof = k.profile(reset=True)
+        k.setProfiling(False)
+        return {name: {"ms_per_step": v[0] / nprof, "launches_per_step": v[1] / nprof} for name, v in prof.items()}
+
+    kernels = kernel_times(km, max(1, min(3, args.steps)))
 
     result = {}
     # FP64 path on the same box and its accuracy reference
@@ -121,7 +123,8 @@ def run(args):
     if not args.skip_fp64:
         kd = make("double")
         ms64, e64 = timed_rate(kd, max(1, args.steps // 2), 1)
-        fp64 = {"ms_per_step": 1000 * ms64, "evals_per_s": 1.0 / ms64, "stats": kd.stats()}
+        fp64 = {"ms_per_step": 1000 * ms64, "evals_per_s": 1.0 / ms64, "stats": kd.stats(),
+                "kernels": {k: round(v["ms_per_step"], 4) for k, v in kernel_times(kd, 2).items()}}
         print(f"[bench] {args.workload} double: {1000 * ms64:.2f} ms/step", file=sys.stderr, flush=True)
         del kd
         # accuracy of the timed path against the FP64 path converged to 1e-10 Debye
@@ -185,7 +188,8 @@ def run(args):
         "setup_s": setup_s,
     }
     if fp64:
-        out["fp64_path"] = {"evals_per_s": fp64["evals_per_s"], "ms_per_step": fp64["ms_per_step"]}
+        out["fp64_path"] = {"evals_per_s": fp64["evals_per_s"], "ms_per_step": fp64["ms_per_step"],
+                            "kernels_ms_per_step": fp64["kernels"]}
         out["speedup_vs_fp64"] = fp64["ms_per_step"] / (1000 * ms_step)
     out.update(result)
     if not args.no_cpu_baseline:

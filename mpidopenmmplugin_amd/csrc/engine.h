@@ -31,7 +31,7 @@ template <class R> struct PairParams {
 
 // Precision-specific per-atom inputs of the pair kernels (sorted order).
 template <class R> struct PairAtoms {
-    const R4<R>* pos;      // x, y, z, q
+    const R4<R>* pos;      // x, y, z, damping factor
     const R* lab;          // [n*20] lab-frame multipoles
     const R2<R>* td;       // thole, damping factor
     const int* aniso;      // [n]
@@ -117,44 +117,68 @@ void launch_tile_ranges(hipStream_t s, int n, const unsigned* keys_sorted, int s
 size_t spread_tiled_lds_bytes(const TileGeom& tg, int double_precision);
 template <class R>
 void launch_spread_fixed_tiled(hipStream_t s, const R4<R>* pos, const R* lab, PmeGeom<R> g, TileGeom tg,
-                               const int* tstart, const int* tend, R* scratch, R2<R>* grid);
+                               const int* tstart, const int* tend, R* scratch, R* grid);
 template <class R>
 void launch_spread_induced_tiled(hipStream_t s, const R4<R>* pos, const R* mu, PmeGeom<R> g, TileGeom tg,
-                                 const int* tstart, const int* tend, R* scratch, R2<R>* grid);
-template <class R> void launch_fold(hipStream_t s, PmeGeom<R> g, TileGeom tg, const R* scratch, R2<R>* grid);
+                                 const int* tstart, const int* tend, R* scratch, R* grid);
+template <class R> void launch_fold(hipStream_t s, PmeGeom<R> g, TileGeom tg, const R* scratch, R* grid);
 template <class R>
-void launch_spread_fixed(hipStream_t s, int n, const R4<R>* pos, const R* lab, PmeGeom<R> g, R2<R>* grid);
+void launch_spread_fixed(hipStream_t s, int n, const R4<R>* pos, const R* lab, PmeGeom<R> g, R* grid);
 template <class R>
-void launch_spread_induced(hipStream_t s, int n, const R4<R>* pos, const R* mu, PmeGeom<R> g, R2<R>* grid);
+void launch_spread_induced(hipStream_t s, int n, const R4<R>* pos, const R* mu, PmeGeom<R> g, R* grid);
 template <class R>
-void launch_convolve(hipStream_t s, PmeGeom<R> g, const R* moduli /*nx+ny+nz*/, R alpha, R2<R>* grid);
+void launch_gather_phi(hipStream_t s, int n, const R4<R>* pos, PmeGeom<R> g, const R* grid, R* phi);
 template <class R>
-void launch_gather_phi(hipStream_t s, int n, const R4<R>* pos, PmeGeom<R> g, const R2<R>* grid, R* phi);
-template <class R>
-void launch_gather_field(hipStream_t s, int n, const R4<R>* pos, PmeGeom<R> g, const R2<R>* grid, R* field);
-// in-place unnormalised 3D complex FFT, [nx][ny][nz] layout, z fastest
-template <class R> struct FftPlan {
-    int n[3];
-    const R2<R>* tw[3];  // per-axis twiddles exp(-2 pi i k / n), k < n
-    int radix[3][16];
-    int nradix[3];
+void launch_gather_field(hipStream_t s, int n, const R4<R>* pos, PmeGeom<R> g, const R* grid, R* field);
+// ---- FFT and convolution (kernels_fft.hip) ------------------------------------
+// x / d = (x * m) >> 40 with m = ceil(2^40 / d), exact for x < 2^20.
+struct FastDiv {
+    unsigned d;
+    unsigned long long m;
 };
-template <class R> void launch_fft3d(hipStream_t s, const FftPlan<R>& plan, R2<R>* grid, int sign);
+FastDiv make_fastdiv(unsigned d);
+constexpr int kFftMaxStages = 8;
+struct FftStage {
+    int rad;      // 4, 2, 3 or 5
+    FastDiv nb;   // n / rad
+    FastDiv ns;   // product of the earlier radices
+    int twstep;   // n / (ns * rad)
+};
+struct FftAxis {
+    int n, nst;
+    FastDiv nd;
+    FftStage st[kFftMaxStages];
+};
+bool make_fft_axis(int n, FftAxis& ax);  // false unless n = 2^a 3^b 5^c with at most 8 stages
+template <class R> struct FftPlan {
+    FftAxis ax[3];
+    const R2<R>* tw[3];  // per-axis twiddles exp(-2 pi i k / n), k < n
+};
+template <class R> struct ConvParams {
+    R expf;          // pi^2 / alpha^2
+    R sc;            // 1 / (pi V)
+    R inv[3];        // 1 / box edge
+    const R* moduli; // |b(k)|^2, [n0 | n1 | n2]
+};
+// Q (real grid) -> S (half spectrum) -> convolution -> Q (potential grid)
+template <class R>
+void launch_recip_convolution(hipStream_t s, const FftPlan<R>& p, R* Q, R2<R>* S, ConvParams<R> cp);
 
 // ---- real space (kernels_pairs.hip) -------------------------------------------
 template <class R>
 void launch_block_bounds(hipStream_t s, int n, const R4<R>* pos, float4* bmin, float4* bmax);
-void launch_build_tiles(hipStream_t s, int nblocks, const float4* bmin, const float4* bmax, Box<float> box,
-                        float cutoff, int2* tiles, int* count, int capacity);
+// one wavefront per block-row entry r; entries with J < I return at once
 template <class R>
-void launch_electrostatics_pairs(hipStream_t s, int ntiles, const int2* tiles, PairAtoms<R> a, const R* mu,
-                                 PairParams<R> p, acc_t* force, acc_t* torque, acc_t* energy);
+void launch_electrostatics_pairs(hipStream_t s, int nrows, const int* rows, const int* owner,
+                                 const unsigned long long* tmask, PairAtoms<R> a, const R* mu, PairParams<R> p,
+                                 acc_t* force, acc_t* torque, acc_t* energy);
 // ---- neighbour lists (kernels_nlist.hip) ---------------------------------------
 void launch_block_rows(hipStream_t s, int nb, const float4* bmin, const float4* bmax, Box<float> box, float cutoff,
-                       const int* start, int* counts, int* rows);
+                       const int* start, int* counts, int* rows, int* owner);
 void launch_exclusive_scan(hipStream_t s, int m, const int* counts, int* start);
-void launch_nl_build(hipStream_t s, int n, const int* rstart, const int* rows, const double4* pos, const int* partners,
-                     const int* npart, Box<double> box, double rc2, int cap, int* nbr, int* count, int* maxcount);
+void launch_nl_build(hipStream_t s, int n, const int* rstart, const int* rows, const float4* bmin, const float4* bmax,
+                     const double4* pos, const int* partners, const int* npart, Box<double> box, double rc2, int cap,
+                     int* nbr, int* count, int* maxcount, unsigned long long* tmask);
 template <class R>
 void launch_nl_fixed_field(hipStream_t s, int n, NbrList l, PairAtoms<R> a, PairParams<R> p, acc_t* field);
 template <class R>

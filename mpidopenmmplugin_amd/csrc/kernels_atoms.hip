@@ -224,7 +224,7 @@ __global__ void k_convert(int n, const double4* __restrict__ pos, const double* 
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     double4 p = pos[s];
-    rpos[s] = R4<R>{(R)p.x, (R)p.y, (R)p.z, (R)p.w};
+    rpos[s] = R4<R>{(R)p.x, (R)p.y, (R)p.z, (R)damp[s]};  // w: damping factor
     for (int k = 0; k < kMP; k++) rlab[(size_t)s * kMP + k] = (R)lab[(size_t)s * kMP + k];
     rtd[s] = R2<R>{(R)thole[s], (R)damp[s]};
 }

@@ -1,0 +1,363 @@
+// 3D FFT and influence-function convolution of the PME reciprocal sum.
+//
+// The charge grid Q is real, [n0][n1][n2] with z fastest (the reference's
+// x*ny*nz + y*nz + z order, MPIDReferenceForce.cpp:3269-3327).  Its spectrum
+// is kept as the half S[n0][n1][n2/2+1]: the grid is real and the influence
+// function even, so the other half is the complex conjugate and is never
+// stored.  One reciprocal cycle is five passes over the grid:
+//
+//   1. z, real to half-complex   Q -> S      (forward)
+//   2. y, complex, in place      S           (forward)
+//   3. x, in place: forward transform, multiply by the influence function
+//      (performMPIDReciprocalConvolution, MPIDReferenceForce.cpp:3329-3366),
+//      backward transform, all while the lines sit in LDS
+//   4. y, complex, in place      S           (backward)
+//   5. z, half-complex to real   S -> Q      (backward)
+//
+// Like the reference's fftpack transforms, neither direction is normalised;
+// the convolution constants carry the scale.
+//
+// Every pass loads a batch of T lines into LDS, runs Stockham stages of radix
+// 4, 2, 3 and 5 (twiddles staged in LDS) and writes the lines back.  Index
+// arithmetic uses precomputed multiply-shift division by the stage constants.
+#include "engine.h"
+
+namespace mpid {
+
+__device__ __forceinline__ unsigned fdiv(unsigned x, FastDiv f) {
+    return (unsigned)(((unsigned long long)x * f.m) >> 40);
+}
+
+template <class R> struct Cx {
+    R x, y;
+};
+template <class R> __device__ __forceinline__ Cx<R> cmul(Cx<R> a, Cx<R> b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <class R> __device__ __forceinline__ Cx<R> cadd(Cx<R> a, Cx<R> b) { return {a.x + b.x, a.y + b.y}; }
+template <class R> __device__ __forceinline__ Cx<R> csub(Cx<R> a, Cx<R> b) { return {a.x - b.x, a.y - b.y}; }
+// multiply by s*i, s = +-1
+template <class R> __device__ __forceinline__ Cx<R> cmuli(Cx<R> a, R s) { return {-s * a.y, s * a.x}; }
+
+template <class R, int RAD> __device__ __forceinline__ void dft_small(Cx<R>* v, R s) {
+    if (RAD == 2) {
+        const Cx<R> a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if (RAD == 4) {
+        const Cx<R> a0 = v[0], a1 = v[1], a2 = v[2], a3 = v[3];
+        const Cx<R> t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = cmuli(csub(a1, a3), s);
+        v[0] = cadd(t0, t2);
+        v[1] = cadd(t1, t3);
+        v[2] = csub(t0, t2);
+        v[3] = csub(t1, t3);
+    } else if (RAD == 3) {
+        const R h = R(0.86602540378443865);
+        const Cx<R> a0 = v[0], t1 = cadd(v[1], v[2]), t2 = cmuli(csub(v[1], v[2]), s * h);
+        const Cx<R> m = {a0.x - R(0.5) * t1.x, a0.y - R(0.5) * t1.y};
+        v[0] = cadd(a0, t1);
+        v[1] = cadd(m, t2);
+        v[2] = csub(m, t2);
+    } else if (RAD == 5) {
+        const R c1 = R(0.30901699437494742), c2 = R(-0.80901699437494742);
+        const R s1 = R(0.95105651629515357), s2 = R(0.58778525229247313);
+        const Cx<R> a0 = v[0];
+        const Cx<R> t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]), t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+        const Cx<R> m1 = {a0.x + c1 * t1.x + c2 * t2.x, a0.y + c1 * t1.y + c2 * t2.y};
+        const Cx<R> m2 = {a0.x + c2 * t1.x + c1 * t2.x, a0.y + c2 * t1.y + c1 * t2.y};
+        const Cx<R> n1 = cmuli(Cx<R>{s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y}, s);
+        const Cx<R> n2 = cmuli(Cx<R>{s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y}, s);
+        v[0] = cadd(a0, cadd(t1, t2));
+        v[1] = cadd(m1, n1);
+        v[4] = csub(m1, n1);
+        v[2] = cadd(m2, n2);
+        v[3] = csub(m2, n2);
+    }
+}
+
+// One Stockham stage on T lines of length n: butterfly j of line t reads
+// in[t][j + r nb] (r < RAD), twiddles by w^(kk r), and writes
+// out[t][(j / Ns) Ns RAD + kk + r Ns], kk = j mod Ns.
+template <class R, int RAD>
+__device__ __forceinline__ void stockham_stage(const Cx<R>* in, Cx<R>* out, int n, int T, const FftStage& st,
+                                               const Cx<R>* tw, R s) {
+    const int nb = (int)st.nb.d, Ns = (int)st.ns.d;
+    for (int b = threadIdx.x; b < T * nb; b += blockDim.x) {
+        const int t = (int)fdiv((unsigned)b, st.nb), j = b - t * nb;
+        const int jq = (int)fdiv((unsigned)j, st.ns), kk = j - jq * Ns;
+        const Cx<R>* li = in + t * n;
+        Cx<R>* lo = out + t * n;
+        Cx<R> v[RAD];
+#pragma unroll
+        for (int r = 0; r < RAD; r++) {
+            Cx<R> a = li[j + r * nb];
+            if (r > 0 && kk > 0) {
+                // table: exp(-2 pi i k / n); exp(s 2 pi i k / n) = (w.x, -s w.y)
+                const Cx<R> w = tw[kk * r * st.twstep];
+                a = cmul(a, Cx<R>{w.x, -s * w.y});
+            }
+            v[r] = a;
+        }
+        dft_small<R, RAD>(v, s);
+        const int o = jq * Ns * RAD + kk;
+#pragma unroll
+        for (int r = 0; r < RAD; r++) lo[o + r * Ns] = v[r];
+    }
+}
+
+// All stages of one axis; returns the buffer that holds the result.
+template <class R>
+__device__ __forceinline__ Cx<R>* fft_in_lds(Cx<R>* a, Cx<R>* b, const FftAxis& ax, int T, const Cx<R>* tw, R s) {
+    Cx<R>* in = a;
+    Cx<R>* out = b;
+    for (int k = 0; k < ax.nst; k++) {
+        const FftStage& st = ax.st[k];
+        switch (st.rad) {
+            case 4: stockham_stage<R, 4>(in, out, ax.n, T, st, tw, s); break;
+            case 2: stockham_stage<R, 2>(in, out, ax.n, T, st, tw, s); break;
+            case 3: stockham_stage<R, 3>(in, out, ax.n, T, st, tw, s); break;
+            default: stockham_stage<R, 5>(in, out, ax.n, T, st, tw, s); break;
+        }
+        __syncthreads();
+        Cx<R>* t = in;
+        in = out;
+        out = t;
+    }
+    return in;
+}
+
+template <class R> __device__ __forceinline__ void stage_twiddles(Cx<R>* tw_s, const R2<R>* tw, int n) {
+    for (int k = threadIdx.x; k < n; k += blockDim.x) tw_s[k] = Cx<R>{tw[k].x, tw[k].y};
+}
+
+// Pass 1 (dir = -1): lines along z, real Q -> half spectrum S.
+// Pass 5 (dir = +1): half spectrum S -> real Q (Hermitian completion).
+template <class R>
+__global__ __launch_bounds__(256) void k_fft_z(R* __restrict__ Q, R2<R>* __restrict__ S, FftAxis ax, int T,
+                                               const R2<R>* __restrict__ tw, int nlines, FastDiv nh, int dir) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int n = ax.n;
+    Cx<R>* A = reinterpret_cast<Cx<R>*>(smem_raw);
+    Cx<R>* B = A + (size_t)T * n;
+    Cx<R>* W = B + (size_t)T * n;
+    const int n2h = (int)nh.d;
+    const int l0 = blockIdx.x * T;
+    const int count = min(T, nlines - l0);
+    stage_twiddles(W, tw, n);
+    if (dir < 0) {
+        const R* src = Q + (size_t)l0 * n;
+        for (int idx = threadIdx.x; idx < T * n; idx += blockDim.x) {
+            const int t = (int)fdiv((unsigned)idx, ax.nd);
+            A[idx] = Cx<R>{t < count ? src[idx] : R(0), R(0)};
+        }
+    } else {
+        for (int idx = threadIdx.x; idx < T * n; idx += blockDim.x) {
+            const int t = (int)fdiv((unsigned)idx, ax.nd), k = idx - t * n;
+            Cx<R> v = {R(0), R(0)};
+            if (t < count) {
+                const R2<R>* line = S + (size_t)(l0 + t) * n2h;
+                if (k < n2h) {
+                    v = Cx<R>{line[k].x, line[k].y};
+                } else {
+                    const R2<R> c = line[n - k];
+                    v = Cx<R>{c.x, -c.y};
+                }
+            }
+            A[idx] = v;
+        }
+    }
+    __syncthreads();
+    const Cx<R>* res = fft_in_lds<R>(A, B, ax, T, W, (R)dir);
+    if (dir < 0) {
+        for (int idx = threadIdx.x; idx < T * n2h; idx += blockDim.x) {
+            const int t = (int)fdiv((unsigned)idx, nh), k = idx - t * n2h;
+            if (t < count) {
+                const Cx<R> v = res[t * n + k];
+                S[(size_t)(l0 + t) * n2h + k] = R2<R>{v.x, v.y};
+            }
+        }
+    } else {
+        R* dst = Q + (size_t)l0 * n;
+        for (int idx = threadIdx.x; idx < T * n; idx += blockDim.x) {
+            const int t = (int)fdiv((unsigned)idx, ax.nd);
+            if (t < count) dst[idx] = res[idx].x;
+        }
+    }
+}
+
+// Passes 2 and 4: lines along y of S, a batch = T consecutive kz of one x.
+template <class R>
+__global__ __launch_bounds__(256) void k_fft_y(R2<R>* __restrict__ S, FftAxis ax, int T, int Tshift,
+                                               const R2<R>* __restrict__ tw, int n2h, int nkb, int dir) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int n = ax.n;
+    Cx<R>* A = reinterpret_cast<Cx<R>*>(smem_raw);
+    Cx<R>* B = A + (size_t)T * n;
+    Cx<R>* W = B + (size_t)T * n;
+    const int x = blockIdx.x / nkb, kz0 = (blockIdx.x - x * nkb) * T;
+    const int count = min(T, n2h - kz0);
+    R2<R>* base = S + (size_t)x * n * n2h + kz0;
+    stage_twiddles(W, tw, n);
+    for (int idx = threadIdx.x; idx < T * n; idx += blockDim.x) {
+        const int k = idx >> Tshift, t = idx & (T - 1);
+        Cx<R> v = {R(0), R(0)};
+        if (t < count) {
+            const R2<R> a = base[(size_t)k * n2h + t];
+            v = Cx<R>{a.x, a.y};
+        }
+        A[t * n + k] = v;
+    }
+    __syncthreads();
+    const Cx<R>* res = fft_in_lds<R>(A, B, ax, T, W, (R)dir);
+    for (int idx = threadIdx.x; idx < T * n; idx += blockDim.x) {
+        const int k = idx >> Tshift, t = idx & (T - 1);
+        if (t < count) {
+            const Cx<R> v = res[t * n + k];
+            base[(size_t)k * n2h + t] = R2<R>{v.x, v.y};
+        }
+    }
+}
+
+// Pass 3: lines along x of S (a batch = T consecutive kz of one y): forward
+// transform, influence function, backward transform.
+template <class R>
+__global__ __launch_bounds__(256) void k_fft_x_convolve(R2<R>* __restrict__ S, FftAxis ax, int T, int Tshift,
+                                                        const R2<R>* __restrict__ tw, int n1, int n2, int n2h, int nkb,
+                                                        ConvParams<R> cp) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int n = ax.n;
+    Cx<R>* A = reinterpret_cast<Cx<R>*>(smem_raw);
+    Cx<R>* B = A + (size_t)T * n;
+    Cx<R>* W = B + (size_t)T * n;
+    const int y = blockIdx.x / nkb, kz0 = (blockIdx.x - y * nkb) * T;
+    const int count = min(T, n2h - kz0);
+    const size_t xs = (size_t)n1 * n2h;
+    R2<R>* base = S + (size_t)y * n2h + kz0;
+    stage_twiddles(W, tw, n);
+    for (int idx = threadIdx.x; idx < T * n; idx += blockDim.x) {
+        const int k = idx >> Tshift, t = idx & (T - 1);
+        Cx<R> v = {R(0), R(0)};
+        if (t < count) {
+            const R2<R> a = base[(size_t)k * xs + t];
+            v = Cx<R>{a.x, a.y};
+        }
+        A[t * n + k] = v;
+    }
+    __syncthreads();
+    Cx<R>* res = fft_in_lds<R>(A, B, ax, T, W, R(-1));
+    // influence function exp(-pi^2 m^2 / alpha^2) / (pi V m^2 |b(m)|^2), m = 0 removed
+    const int my = y < (n1 + 1) / 2 ? y : y - n1;
+    const R hy = R(my) * cp.inv[1];
+    const R by = cp.moduli[n + y];
+    for (int idx = threadIdx.x; idx < T * n; idx += blockDim.x) {
+        const int kx = idx >> Tshift, t = idx & (T - 1);
+        const int kz = kz0 + t;
+        R et = R(0);
+        if (t < count && (kx | y | kz) != 0) {
+            const int mx = kx < (n + 1) / 2 ? kx : kx - n;
+            const int mz = kz < (n2 + 1) / 2 ? kz : kz - n2;
+            const R hx = R(mx) * cp.inv[0], hz = R(mz) * cp.inv[2];
+            const R m2 = hx * hx + hy * hy + hz * hz;
+            const R den = m2 * cp.moduli[kx] * by * cp.moduli[n + n1 + kz];
+            et = cp.sc * exp_r(-cp.expf * m2) / den;
+        }
+        Cx<R>& v = res[t * n + kx];
+        v.x *= et;
+        v.y *= et;
+    }
+    __syncthreads();
+    Cx<R>* other = (res == A) ? B : A;
+    const Cx<R>* out = fft_in_lds<R>(res, other, ax, T, W, R(1));
+    for (int idx = threadIdx.x; idx < T * n; idx += blockDim.x) {
+        const int k = idx >> Tshift, t = idx & (T - 1);
+        if (t < count) {
+            const Cx<R> v = out[t * n + k];
+            base[(size_t)k * xs + t] = R2<R>{v.x, v.y};
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+FastDiv make_fastdiv(unsigned d) {
+    FastDiv f;
+    f.d = d;
+    f.m = ((1ull << 40) + d - 1) / d;  // ceil(2^40 / d): exact for x < 2^40 / d, here x < 2^20
+    return f;
+}
+
+bool make_fft_axis(int n, FftAxis& ax) {
+    int rad[16], c = 0, m = n;
+    while (m % 4 == 0) { rad[c++] = 4; m /= 4; }
+    while (m % 2 == 0) { rad[c++] = 2; m /= 2; }
+    while (m % 3 == 0) { rad[c++] = 3; m /= 3; }
+    while (m % 5 == 0) { rad[c++] = 5; m /= 5; }
+    if (m != 1 || c > kFftMaxStages) return false;
+    ax.n = n;
+    ax.nst = c;
+    ax.nd = make_fastdiv((unsigned)n);
+    int Ns = 1;
+    for (int k = 0; k < c; k++) {
+        ax.st[k].rad = rad[k];
+        ax.st[k].nb = make_fastdiv((unsigned)(n / rad[k]));
+        ax.st[k].ns = make_fastdiv((unsigned)Ns);
+        ax.st[k].twstep = n / (Ns * rad[k]);
+        Ns *= rad[k];
+    }
+    return true;
+}
+
+// lines per workgroup: a power of two, two line buffers within 32 KB
+template <class R> static int lines_per_group(int n) {
+    int T = 16;
+    while (T > 1 && 2 * (size_t)T * n * sizeof(R2<R>) > 32768) T >>= 1;
+    return T;
+}
+static int log2i(int v) {
+    int s = 0;
+    while ((1 << s) < v) s++;
+    return s;
+}
+template <class R> static size_t fft_lds(int T, int n) { return (2 * (size_t)T * n + n) * sizeof(R2<R>); }
+
+template <class R>
+void launch_recip_convolution(hipStream_t s, const FftPlan<R>& p, R* Q, R2<R>* S, ConvParams<R> cp) {
+    const int n0 = p.ax[0].n, n1 = p.ax[1].n, n2 = p.ax[2].n;
+    const int n2h = n2 / 2 + 1;
+    const FastDiv nh = make_fastdiv((unsigned)n2h);
+    // 1. z forward
+    {
+        const int T = lines_per_group<R>(n2);
+        const int nlines = n0 * n1;
+        hipLaunchKernelGGL(k_fft_z<R>, dim3((nlines + T - 1) / T), dim3(256), fft_lds<R>(T, n2), s, Q, S, p.ax[2], T,
+                           p.tw[2], nlines, nh, -1);
+    }
+    // 2. y forward
+    const int Ty = lines_per_group<R>(n1), nkby = (n2h + Ty - 1) / Ty;
+    hipLaunchKernelGGL(k_fft_y<R>, dim3(n0 * nkby), dim3(256), fft_lds<R>(Ty, n1), s, S, p.ax[1], Ty, log2i(Ty),
+                       p.tw[1], n2h, nkby, -1);
+    // 3. x forward, convolution, x backward
+    {
+        const int Tx = lines_per_group<R>(n0), nkbx = (n2h + Tx - 1) / Tx;
+        hipLaunchKernelGGL(k_fft_x_convolve<R>, dim3(n1 * nkbx), dim3(256), fft_lds<R>(Tx, n0), s, S, p.ax[0], Tx,
+                           log2i(Tx), p.tw[0], n1, n2, n2h, nkbx, cp);
+    }
+    // 4. y backward
+    hipLaunchKernelGGL(k_fft_y<R>, dim3(n0 * nkby), dim3(256), fft_lds<R>(Ty, n1), s, S, p.ax[1], Ty, log2i(Ty),
+                       p.tw[1], n2h, nkby, 1);
+    // 5. z backward
+    {
+        const int T = lines_per_group<R>(n2);
+        const int nlines = n0 * n1;
+        hipLaunchKernelGGL(k_fft_z<R>, dim3((nlines + T - 1) / T), dim3(256), fft_lds<R>(T, n2), s, Q, S, p.ax[2], T,
+                           p.tw[2], nlines, nh, 1);
+    }
+}
+
+template void launch_recip_convolution<float>(hipStream_t, const FftPlan<float>&, float*, R2<float>*,
+                                              ConvParams<float>);
+template void launch_recip_convolution<double>(hipStream_t, const FftPlan<double>&, double*, R2<double>*,
+                                               ConvParams<double>);
+
+}  // namespace mpid

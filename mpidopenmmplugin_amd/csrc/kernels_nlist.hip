@@ -46,11 +46,12 @@ __device__ __forceinline__ bool blocks_within(float4 a0, float4 a1, float4 b0, f
 }
 
 // One wavefront per block I.  Count mode (rows == nullptr) writes counts[I];
-// fill mode writes the row at rows[start[I] ...] in ascending J.
+// fill mode writes the row at rows[start[I] ...] in ascending J, and I into
+// owner[] at the same positions.
 __global__ __launch_bounds__(64) void k_block_rows(int nb, const float4* __restrict__ bmin,
                                                    const float4* __restrict__ bmax, Box<float> box, float cutoff,
                                                    const int* __restrict__ start, int* __restrict__ counts,
-                                                   int* __restrict__ rows) {
+                                                   int* __restrict__ rows, int* __restrict__ owner) {
     const int I = blockIdx.x, lane = threadIdx.x;
     const float4 a0 = bmin[I], a1 = bmax[I];
     const int base = rows ? start[I] : 0;
@@ -59,7 +60,11 @@ __global__ __launch_bounds__(64) void k_block_rows(int nb, const float4* __restr
         const int J = J0 + lane;
         const bool hit = J < nb && blocks_within(a0, a1, bmin[J], bmax[J], box, cutoff);
         const unsigned long long m = __ballot(hit);
-        if (hit && rows) rows[base + c + __popcll(m & ((1ull << lane) - 1ull))] = J;
+        if (hit && rows) {
+            const int at = base + c + __popcll(m & ((1ull << lane) - 1ull));
+            rows[at] = J;
+            owner[at] = I;
+        }
         c += __popcll(m);
     }
     if (lane == 0 && !rows) counts[I] = c;
@@ -91,12 +96,15 @@ __global__ __launch_bounds__(1024) void k_exclusive_scan(int m, const int* __res
 
 // Lane i of block I collects every j within the cutoff (FP64 distances, the
 // test of the reference: r2 > cutoff^2 is out).  Entries beyond cap are
-// counted but not stored; the host grows the list and rebuilds.
+// counted but not stored; the host grows the list and rebuilds.  For every
+// row entry r with J >= I it also writes the lane's tile mask
+// tmask[r * 64 + lane]: bit jj for each listed j = J * 64 + jj, with j > i.
 __global__ __launch_bounds__(64) void k_nl_build(int n, const int* __restrict__ rstart, const int* __restrict__ rows,
+                                                 const float4* __restrict__ bmin, const float4* __restrict__ bmax,
                                                  const double4* __restrict__ pos, const int* __restrict__ partners,
                                                  const int* __restrict__ npart, Box<double> box, double rc2, int cap,
                                                  int* __restrict__ nbr, int* __restrict__ count,
-                                                 int* __restrict__ maxcount) {
+                                                 int* __restrict__ maxcount, unsigned long long* __restrict__ tmask) {
     __shared__ double4 sp[kTile];
     const int I = blockIdx.x, lane = threadIdx.x;
     const int i = I * kTile + lane;
@@ -105,14 +113,36 @@ __global__ __launch_bounds__(64) void k_nl_build(int n, const int* __restrict__ 
     const double4 pi = pos[ic];
     const int* parts = partners + (size_t)ic * kMaxPartners;
     const int np = iv ? npart[ic] : 0;
+    // bounding box of block I (centre, half extent) for the candidate filter
+    const float4 b0 = bmin[I], b1 = bmax[I];
+    const float cI[3] = {0.5f * (b0.x + b1.x), 0.5f * (b0.y + b1.y), 0.5f * (b0.z + b1.z)};
+    const float hI[3] = {0.5f * (b1.x - b0.x), 0.5f * (b1.y - b0.y), 0.5f * (b1.z - b0.z)};
+    const float rcf = sqrtf((float)rc2) * 1.0001f + 1e-5f;
+    const Box<float> fbox{{(float)box.L[0], (float)box.L[1], (float)box.L[2]},
+                          {(float)box.inv[0], (float)box.inv[1], (float)box.inv[2]}};
     int cnt = 0;
     for (int r = rstart[I]; r < rstart[I + 1]; r++) {
         const int J = rows[r];
         __syncthreads();
+        // each lane stages one atom of J and tests it against block I's box
+        // (minimum image of the distance to the box; never rejects a pair
+        // within the cutoff, also when the box is wide)
+        bool cand;
         {
             const int j = J * kTile + lane;
-            sp[lane] = pos[j < n ? j : n - 1];
+            const double4 pj = pos[j < n ? j : n - 1];
+            sp[lane] = pj;
+            const float d[3] = {(float)pj.x - cI[0], (float)pj.y - cI[1], (float)pj.z - cI[2]};
+            float d2 = 0.f;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const float dk = d[k] - fbox.L[k] * floorf(d[k] * fbox.inv[k] + 0.5f);
+                const float g = fmaxf(0.f, fabsf(dk) - hI[k]);
+                d2 += g * g;
+            }
+            cand = j < n && d2 <= rcf * rcf;
         }
+        unsigned long long cmask = __ballot(cand);
         __syncthreads();
         unsigned long long skip = 0;
         for (int k = 0; k < np; k++) {
@@ -120,16 +150,22 @@ __global__ __launch_bounds__(64) void k_nl_build(int n, const int* __restrict__ 
             if ((j >> 6) == J) skip |= 1ull << (j & 63);
         }
         if (J == I) skip |= 1ull << lane;
-        const int jn = min(kTile, n - J * kTile);
-        if (!iv) continue;
-        for (int jj = 0; jj < jn; jj++) {
+        if (!iv) cmask = 0;  // lanes past the last atom list nothing
+        // tile-mask bits: j > i, i.e. all of J when J > I, those above the lane when J == I
+        const unsigned long long upper = J > I ? ~0ull : (J == I ? ~((2ull << lane) - 1ull) : 0ull);
+        unsigned long long tm = 0;
+        while (cmask) {
+            const int jj = __ffsll((long long)cmask) - 1;
+            cmask &= cmask - 1ull;
             if ((skip >> jj) & 1ull) continue;
             const double4 pj = sp[jj];
             const V3<double> dr = min_image(mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z), box);
             if (dot(dr, dr) > rc2) continue;
             if (cnt < cap) nbr[nl_index(i, cnt, n)] = J * kTile + jj;
             cnt++;
+            tm |= 1ull << jj;
         }
+        if (J >= I) tmask[(size_t)r * kTile + lane] = tm & upper;
     }
     if (iv) {
         count[i] = min(cnt, cap);
@@ -145,8 +181,8 @@ __global__ __launch_bounds__(64) void k_nl_build(int n, const int* __restrict__ 
 template <class R>
 __global__ __launch_bounds__(256) void k_nl_fixed_field(NbrList L, PairAtoms<R> A, PairParams<R> P,
                                                         acc_t* __restrict__ field) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = g / kNlSplit, sub = g % kNlSplit;
+    const unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = (int)(g / kNlSplit), sub = (int)(g % kNlSplit);
     const int n = P.n;
     const bool iv = i < n;
     const int ic = iv ? i : n - 1;
@@ -156,16 +192,22 @@ __global__ __launch_bounds__(256) void k_nl_fixed_field(NbrList L, PairAtoms<R> 
 #pragma unroll
     for (int k = 0; k < kMP; k++) mi[k] = A.lab[(size_t)ic * kMP + k];
     const int cnt = iv ? L.count[ic] : 0;
+    // entry k of atom i: lp advances by n * kNlSplit per step of kNlSplit
+    const int* lp = L.nbr + (size_t)ic * kNlSplit + sub;
+    const size_t stride = (size_t)n * kNlSplit;
+    int jn = sub < cnt ? lp[0] : 0;
     R ex = 0, ey = 0, ez = 0;
     for (int k = sub; k < cnt; k += kNlSplit) {
-        const int j = L.nbr[nl_index(ic, k, n)];
+        const int j = jn;
+        lp += stride;
+        if (k + kNlSplit < cnt) jn = *lp;  // next index in flight during this pair
         const R4<R> pj = A.pos[j];
         const V3<R> dr = min_image(mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z), P.box);
         R mj[kMP];
 #pragma unroll
         for (int c = 0; c < kMP; c++) mj[c] = A.lab[(size_t)j * kMP + c];
         V3<R> fi, fj;
-        fixed_field_pair<R>(dr, dot(dr, dr), mi, mj, tdi, A.td[j], R(1), P, fi, fj);
+        fixed_field_pair<R>(dr, dot(dr, dr), mi, mj, tdi, R2<R>{R(0), pj.w}, R(1), P, fi, fj);
         ex += fi.x;
         ey += fi.y;
         ez += fi.z;
@@ -186,23 +228,28 @@ __global__ __launch_bounds__(256) void k_nl_fixed_field(NbrList L, PairAtoms<R> 
 template <class R>
 __global__ __launch_bounds__(256) void k_nl_induced_field(NbrList L, PairAtoms<R> A, const R* __restrict__ mu,
                                                           PairParams<R> P, acc_t* __restrict__ field) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = g / kNlSplit, sub = g % kNlSplit;
+    const unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = (int)(g / kNlSplit), sub = (int)(g % kNlSplit);
     const int n = P.n;
     const bool iv = i < n;
     const int ic = iv ? i : n - 1;
     const R4<R> pi = A.pos[ic];
-    const R2<R> tdi = A.td[ic];
     const V3<R> ui = mk(mu[3 * (size_t)ic], mu[3 * (size_t)ic + 1], mu[3 * (size_t)ic + 2]);
     const int cnt = iv ? L.count[ic] : 0;
+    const int* lp = L.nbr + (size_t)ic * kNlSplit + sub;
+    const size_t stride = (size_t)n * kNlSplit;
+    int jn = sub < cnt ? lp[0] : 0;
     R ex = 0, ey = 0, ez = 0;
     for (int k = sub; k < cnt; k += kNlSplit) {
-        const int j = L.nbr[nl_index(ic, k, n)];
+        const int j = jn;
+        lp += stride;
+        if (k + kNlSplit < cnt) jn = *lp;
         const R4<R> pj = A.pos[j];
         const V3<R> dr = min_image(mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z), P.box);
         const V3<R> uj = mk(mu[3 * (size_t)j], mu[3 * (size_t)j + 1], mu[3 * (size_t)j + 2]);
         V3<R> fi, fj;
-        induced_field_pair<R>(dr, dot(dr, dr), ui, uj, tdi, A.td[j], R(1), P, fi, fj);
+        // listed pairs are not covalent partners: polarization scale 1, default Thole width
+        induced_field_pair<R>(dr, dot(dr, dr), ui, uj, pi.w, pj.w, P.default_thole, P, fi, fj);
         ex += fi.x;
         ey += fi.y;
         ez += fi.z;
@@ -224,17 +271,19 @@ __global__ __launch_bounds__(256) void k_nl_induced_field(NbrList L, PairAtoms<R
 // launchers
 // ---------------------------------------------------------------------------
 void launch_block_rows(hipStream_t s, int nb, const float4* bmin, const float4* bmax, Box<float> box, float cutoff,
-                       const int* start, int* counts, int* rows) {
-    hipLaunchKernelGGL(k_block_rows, dim3(nb), dim3(kTile), 0, s, nb, bmin, bmax, box, cutoff, start, counts, rows);
+                       const int* start, int* counts, int* rows, int* owner) {
+    hipLaunchKernelGGL(k_block_rows, dim3(nb), dim3(kTile), 0, s, nb, bmin, bmax, box, cutoff, start, counts, rows,
+                       owner);
 }
 void launch_exclusive_scan(hipStream_t s, int m, const int* counts, int* start) {
     hipLaunchKernelGGL(k_exclusive_scan, dim3(1), dim3(1024), 0, s, m, counts, start);
 }
-void launch_nl_build(hipStream_t s, int n, const int* rstart, const int* rows, const double4* pos, const int* partners,
-                     const int* npart, Box<double> box, double rc2, int cap, int* nbr, int* count, int* maxcount) {
+void launch_nl_build(hipStream_t s, int n, const int* rstart, const int* rows, const float4* bmin, const float4* bmax,
+                     const double4* pos, const int* partners, const int* npart, Box<double> box, double rc2, int cap,
+                     int* nbr, int* count, int* maxcount, unsigned long long* tmask) {
     const int nb = (n + kTile - 1) / kTile;
-    hipLaunchKernelGGL(k_nl_build, dim3(nb), dim3(kTile), 0, s, n, rstart, rows, pos, partners, npart, box, rc2, cap,
-                       nbr, count, maxcount);
+    hipLaunchKernelGGL(k_nl_build, dim3(nb), dim3(kTile), 0, s, n, rstart, rows, bmin, bmax, pos, partners, npart, box,
+                       rc2, cap, nbr, count, maxcount, tmask);
 }
 template <class R>
 void launch_nl_fixed_field(hipStream_t s, int n, NbrList l, PairAtoms<R> a, PairParams<R> p, acc_t* field) {

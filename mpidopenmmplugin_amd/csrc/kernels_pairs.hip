@@ -2,12 +2,16 @@
 // path on gfx950, and the covalent-pair kernels.
 //
 // Atoms are sorted along a Morton curve and cut into blocks of 64 (one
-// wavefront).  A tile is a pair of blocks whose bounding boxes come within the
-// cutoff.  One wavefront owns one tile: lane t holds atom i = I*64+t in
-// registers, the 64 atoms of block J sit in LDS, and in step s lane t meets
-// j = J*64 + (t+s)%64.  In every step the 64 lanes touch 64 distinct j, so the
-// j-side accumulators live in LDS without atomics; both sides are flushed to
-// the fixed-point global accumulators once per tile.
+// wavefront).  A tile is a pair of blocks (I, J >= I) whose bounding boxes
+// come within the cutoff; the neighbour-list build leaves, for every tile and
+// every atom i of block I, a 64-bit mask of the atoms j of block J that are
+// within the cutoff and are not covalent partners of i (j > i when I == J).
+// One wavefront owns one tile: the set bits are ranked with a wavefront
+// prefix sum, and the wavefront then walks the tile's pairs 64 at a time,
+// lane t taking the pair of rank t + 64c.  Every lane thus works on a pair
+// inside the cutoff, whatever the tile's fill.  Both sides accumulate in LDS
+// (LDS atomics) and are flushed to the fixed-point global accumulators once
+// per tile.
 //
 // The pair formulas are in pair_math.h (MPIDReferenceForce.cpp:4335-4920).
 #include "pair_math.h"
@@ -48,123 +52,154 @@ __global__ __launch_bounds__(64) void k_block_bounds(int n, const R4<R>* __restr
     }
 }
 
-__global__ void k_build_tiles(int nb, const float4* __restrict__ bmin, const float4* __restrict__ bmax,
-                              Box<float> box, float cutoff, int2* tiles, int* count, int capacity) {
-    int I = blockIdx.y;
-    int J = blockIdx.x * blockDim.x + threadIdx.x;
-    if (J >= nb || J < I) return;
-    float4 a0 = bmin[I], a1 = bmax[I], b0 = bmin[J], b1 = bmax[J];
-    float ca[3] = {0.5f * (a0.x + a1.x), 0.5f * (a0.y + a1.y), 0.5f * (a0.z + a1.z)};
-    float ha[3] = {0.5f * (a1.x - a0.x), 0.5f * (a1.y - a0.y), 0.5f * (a1.z - a0.z)};
-    float cb[3] = {0.5f * (b0.x + b1.x), 0.5f * (b0.y + b1.y), 0.5f * (b0.z + b1.z)};
-    float hb[3] = {0.5f * (b1.x - b0.x), 0.5f * (b1.y - b0.y), 0.5f * (b1.z - b0.z)};
-    float d2 = 0.f;
+// Position of the k-th (0-based) set bit of m; m has more than k set bits.
+__device__ __forceinline__ int select_bit(unsigned long long m, int k) {
+    int pos = 0;
+    unsigned v = (unsigned)m;
+    const int c = __popc(v);
+    if (k >= c) {
+        k -= c;
+        v = (unsigned)(m >> 32);
+        pos = 32;
+    }
 #pragma unroll
-    for (int k = 0; k < 3; k++) {
-        float d = cb[k] - ca[k];
-        d -= box.L[k] * floorf(d * box.inv[k] + 0.5f);
-        float g = fmaxf(0.f, fabsf(d) - ha[k] - hb[k]);
-        d2 += g * g;
+    for (int w = 16; w >= 1; w >>= 1) {
+        const int cl = __popc(v & ((1u << w) - 1u));
+        if (k >= cl) {
+            k -= cl;
+            v >>= w;
+            pos += w;
+        }
     }
-    // a little slack absorbs float rounding of the bounds
-    float rc = cutoff * 1.0001f + 1e-5f;
-    if (d2 <= rc * rc) {
-        int slot = atomicAdd(count, 1);
-        if (slot < capacity) tiles[slot] = make_int2(I, J);
-    }
+    return pos;
 }
 
+constexpr int kElecThreads = 256;  // four wavefronts share one tile
+
 template <class R>
-__global__ __launch_bounds__(64) void k_electrostatics(const int2* __restrict__ tiles, PairAtoms<R> A,
-                                                        const R* __restrict__ mu, PairParams<R> P,
-                                                        acc_t* __restrict__ force, acc_t* __restrict__ torque,
-                                                        acc_t* __restrict__ energy) {
-    __shared__ R4<R> sp[kTile];
-    __shared__ R sm[kTile][kMP];
-    __shared__ R smu[kTile][3];
-    __shared__ R2<R> std_[kTile];
-    __shared__ int san[kTile];
-    __shared__ R sacc[kTile][6];
-    const int lane = threadIdx.x;
-    const int2 t = tiles[blockIdx.x];
-    const int I = t.x, J = t.y;
+__global__ __launch_bounds__(kElecThreads) void k_electrostatics(const int* __restrict__ rows,
+                                                                  const int* __restrict__ owner,
+                                                                  const unsigned long long* __restrict__ tmask,
+                                                                  PairAtoms<R> A, const R* __restrict__ mu,
+                                                                  PairParams<R> P, acc_t* __restrict__ force,
+                                                                  acc_t* __restrict__ torque,
+                                                                  acc_t* __restrict__ energy) {
+    constexpr int kS = kMP + 1;  // odd LDS row stride: rows of different atoms start in different banks
+    __shared__ unsigned long long smask[kTile];
+    __shared__ int soff[kTile];
+    __shared__ int stotal;
+    // [0]: atoms of block I, [1]: atoms of block J; FP64 in both precisions
+    // (LDS atomic adds of doubles keep up on gfx950, those of floats stall)
+    __shared__ double acc[2][kTile][6];
+    __shared__ R4<R> spos[2][kTile];
+    __shared__ R smp[2][kTile * kS];
+    __shared__ R smu[2][kTile * 3];
+    __shared__ int san[2][kTile];
+    __shared__ double sen[kElecThreads / kTile];
+    const int r = blockIdx.x, tid = threadIdx.x, lane = tid & (kTile - 1), wave = tid / kTile;
+    const int I = owner[r], J = rows[r];
+    if (J < I) return;  // each block pair once (uniform over the workgroup)
+    if (wave == 0) {
+        // rank the tile's pairs: prefix sum of the lanes' mask popcounts
+        const unsigned long long m = tmask[(size_t)r * kTile + lane];
+        const int c = __popcll(m);
+        int incl = c;
+#pragma unroll
+        for (int off = 1; off < kTile; off <<= 1) {
+            const int v = __shfl_up(incl, off);
+            if (lane >= off) incl += v;
+        }
+        smask[lane] = m;
+        soff[lane] = incl - c;
+        if (lane == kTile - 1) stotal = incl;
+    }
+    __syncthreads();
+    const int total = stotal;
+    if (total == 0) return;
     const int n = P.n;
-    {
-        int j = J * kTile + lane;
-        int jc = j < n ? j : n - 1;
-        sp[lane] = A.pos[jc];
+    // stage both blocks in LDS (thread tid: side tid / 128, atom tid % 64 for the per-atom fields)
+    if (tid < 2 * kTile) {
+        const int side = tid / kTile;
+        const int a = min((side == 0 ? I : J) * kTile + lane, n - 1);
+        spos[side][lane] = A.pos[a];
+        smu[side][3 * lane + 0] = mu[3 * (size_t)a + 0];
+        smu[side][3 * lane + 1] = mu[3 * (size_t)a + 1];
+        smu[side][3 * lane + 2] = mu[3 * (size_t)a + 2];
+        san[side][lane] = A.aniso[a];
 #pragma unroll
-        for (int k = 0; k < kMP; k++) sm[lane][k] = A.lab[(size_t)jc * kMP + k];
-        smu[lane][0] = mu[3 * (size_t)jc + 0];
-        smu[lane][1] = mu[3 * (size_t)jc + 1];
-        smu[lane][2] = mu[3 * (size_t)jc + 2];
-        std_[lane] = A.td[jc];
-        san[lane] = A.aniso[jc];
-#pragma unroll
-        for (int k = 0; k < 6; k++) sacc[lane][k] = R(0);
+        for (int k = 0; k < 6; k++) acc[side][lane][k] = 0.0;
     }
-    const int i = I * kTile + lane;
-    const bool iv = i < n;
-    const int ic = iv ? i : n - 1;
-    const R4<R> pi = A.pos[ic];
-    R mi[kMP];
 #pragma unroll
-    for (int k = 0; k < kMP; k++) mi[k] = A.lab[(size_t)ic * kMP + k];
-    const V3<R> ui = mk(mu[3 * (size_t)ic], mu[3 * (size_t)ic + 1], mu[3 * (size_t)ic + 2]);
-    const R2<R> tdi = A.td[ic];
-    const bool ani = A.aniso[ic] != 0;
-    const unsigned long long pm = partner_mask(A.partners + (size_t)ic * kMaxPartners, iv ? A.npart[ic] : 0, J);
-    R fx = 0, fy = 0, fz = 0, tx = 0, ty = 0, tz = 0;
-    // energies summed in FP64: the real-space total is a large number that the
-    // reciprocal and self terms cancel almost entirely
+    for (int side = 0; side < 2; side++) {
+        // the 64 x 20 multipole block is contiguous; read it coalesced
+        const int a0 = (side == 0 ? I : J) * kTile;
+        const R* src = A.lab + (size_t)a0 * kMP;
+        const int lim = (min(a0 + kTile, n) - a0) * kMP;
+        for (int e = tid; e < kTile * kMP; e += kElecThreads) {
+            const int at = e / kMP, k = e - at * kMP;
+            smp[side][at * kS + k] = e < lim ? src[e] : R(0);
+        }
+    }
+    __syncthreads();
     double en = 0.0;
-    __syncthreads();
-    for (int s = 0; s < kTile; s++) {
-        const int jj = (lane + s) & (kTile - 1);
-        const int j = J * kTile + jj;
-        if (!iv || j >= n || (I == J && jj <= lane) || ((pm >> jj) & 1ull)) continue;
-        const R4<R> pj = sp[jj];
-        V3<R> dr = min_image(mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z), P.box);
-        const R r2 = dot(dr, dr);
-        if (r2 > P.rc2) continue;
-        ElecOut<R> o = elec_pair<R>(dr, r2, mi, sm[jj], ui, mk(smu[jj][0], smu[jj][1], smu[jj][2]), tdi, std_[jj],
-                                    ani, san[jj] != 0, R(1), R(1), P);
+    for (int q = tid; q < total; q += kElecThreads) {
+        // owning lane: the last il with soff[il] <= q
+        int il = 0;
+#pragma unroll
+        for (int step = 32; step >= 1; step >>= 1)
+            if (soff[il + step] <= q) il += step;
+        const int jj = select_bit(smask[il], q - soff[il]);
+        const R4<R> pi = spos[0][il], pj = spos[1][jj];
+        const V3<R> dr = min_image(mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z), P.box);
+        R mi[kMP], mj[kMP];
+#pragma unroll
+        for (int k = 0; k < kMP; k++) {
+            mi[k] = smp[0][il * kS + k];
+            mj[k] = smp[1][jj * kS + k];
+        }
+        const V3<R> ui = mk(smu[0][3 * il], smu[0][3 * il + 1], smu[0][3 * il + 2]);
+        const V3<R> uj = mk(smu[1][3 * jj], smu[1][3 * jj + 1], smu[1][3 * jj + 2]);
+        const ElecOut<R> o = elec_pair<R>(dr, dot(dr, dr), mi, mj, ui, uj, R2<R>{R(0), pi.w}, R2<R>{R(0), pj.w},
+                                          san[0][il] != 0, san[1][jj] != 0, R(1), R(1), P);
         en += (double)o.energy;
-        fx -= o.f.x;
-        fy -= o.f.y;
-        fz -= o.f.z;
-        tx += o.ti.x;
-        ty += o.ti.y;
-        tz += o.ti.z;
-        sacc[jj][0] += o.f.x;
-        sacc[jj][1] += o.f.y;
-        sacc[jj][2] += o.f.z;
-        sacc[jj][3] += o.tj.x;
-        sacc[jj][4] += o.tj.y;
-        sacc[jj][5] += o.tj.z;
+        atomicAdd(&acc[0][il][0], -(double)o.f.x);
+        atomicAdd(&acc[0][il][1], -(double)o.f.y);
+        atomicAdd(&acc[0][il][2], -(double)o.f.z);
+        atomicAdd(&acc[0][il][3], (double)o.ti.x);
+        atomicAdd(&acc[0][il][4], (double)o.ti.y);
+        atomicAdd(&acc[0][il][5], (double)o.ti.z);
+        atomicAdd(&acc[1][jj][0], (double)o.f.x);
+        atomicAdd(&acc[1][jj][1], (double)o.f.y);
+        atomicAdd(&acc[1][jj][2], (double)o.f.z);
+        atomicAdd(&acc[1][jj][3], (double)o.tj.x);
+        atomicAdd(&acc[1][jj][4], (double)o.tj.y);
+        atomicAdd(&acc[1][jj][5], (double)o.tj.z);
     }
-    __syncthreads();
-    if (iv) {
-        acc_add(&force[3 * (size_t)i + 0], fx);
-        acc_add(&force[3 * (size_t)i + 1], fy);
-        acc_add(&force[3 * (size_t)i + 2], fz);
-        acc_add(&torque[3 * (size_t)i + 0], tx);
-        acc_add(&torque[3 * (size_t)i + 1], ty);
-        acc_add(&torque[3 * (size_t)i + 2], tz);
-    }
-    const int j = J * kTile + lane;
-    if (j < n) {
-        acc_add(&force[3 * (size_t)j + 0], sacc[lane][0]);
-        acc_add(&force[3 * (size_t)j + 1], sacc[lane][1]);
-        acc_add(&force[3 * (size_t)j + 2], sacc[lane][2]);
-        acc_add(&torque[3 * (size_t)j + 0], sacc[lane][3]);
-        acc_add(&torque[3 * (size_t)j + 1], sacc[lane][4]);
-        acc_add(&torque[3 * (size_t)j + 2], sacc[lane][5]);
-    }
-    // wavefront sum of the pair energies, one fixed-point add per tile
+    // wavefront sums of the pair energies
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) en += __shfl_xor(en, off);
-    if (lane == 0) acc_add(energy, en);
+    if (lane == 0) sen[wave] = en;
+    __syncthreads();
+    if (tid < 2 * kTile) {
+        const int side = tid / kTile;
+        const int a = (side == 0 ? I : J) * kTile + lane;
+        if (a < n) {
+            const double* v = acc[side][lane];
+            if (v[0] != 0.0 || v[1] != 0.0 || v[2] != 0.0 || v[3] != 0.0 || v[4] != 0.0 || v[5] != 0.0) {
+                acc_add(&force[3 * (size_t)a + 0], v[0]);
+                acc_add(&force[3 * (size_t)a + 1], v[1]);
+                acc_add(&force[3 * (size_t)a + 2], v[2]);
+                acc_add(&torque[3 * (size_t)a + 0], v[3]);
+                acc_add(&torque[3 * (size_t)a + 1], v[4]);
+                acc_add(&torque[3 * (size_t)a + 2], v[5]);
+            }
+        }
+    }
+    if (tid == 0) {
+        double e = 0.0;
+        for (int w = 0; w < kElecThreads / kTile; w++) e += sen[w];
+        acc_add(energy, e);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -230,8 +265,10 @@ __global__ __launch_bounds__(256) void k_cov_induced_field(PairAtoms<R> A, const
         const R r2 = dot(dr, dr);
         if (r2 > P.rc2) continue;
         V3<R> fi, fj;
-        induced_field_pair<R>(dr, r2, ui, mk(mu[3 * (size_t)j], mu[3 * (size_t)j + 1], mu[3 * (size_t)j + 2]), tdi,
-                              A.td[j], partner_scale<R>(e, P.scale14), P, fi, fj);
+        const R2<R> tdj = A.td[j];
+        const R a = partner_scale<R>(e, P.scale14) == R(0) ? tdi.x + tdj.x : P.default_thole;
+        induced_field_pair<R>(dr, r2, ui, mk(mu[3 * (size_t)j], mu[3 * (size_t)j + 1], mu[3 * (size_t)j + 2]), tdi.y,
+                              tdj.y, a, P, fi, fj);
         ex += fi.x;
         ey += fi.y;
         ez += fi.z;
@@ -311,17 +348,14 @@ void launch_block_bounds(hipStream_t s, int n, const R4<R>* pos, float4* bmin, f
     hipLaunchKernelGGL(k_block_bounds<R>, dim3(nb), dim3(kTile), 0, s, n, pos, bmin, bmax);
 }
 
-void launch_build_tiles(hipStream_t s, int nb, const float4* bmin, const float4* bmax, Box<float> box, float cutoff,
-                        int2* tiles, int* count, int capacity) {
-    dim3 grid((nb + 63) / 64, nb);
-    hipLaunchKernelGGL(k_build_tiles, grid, dim3(64), 0, s, nb, bmin, bmax, box, cutoff, tiles, count, capacity);
-}
 
 template <class R>
-void launch_electrostatics_pairs(hipStream_t s, int ntiles, const int2* tiles, PairAtoms<R> a, const R* mu,
-                                 PairParams<R> p, acc_t* force, acc_t* torque, acc_t* energy) {
-    if (ntiles <= 0) return;
-    hipLaunchKernelGGL(k_electrostatics<R>, dim3(ntiles), dim3(kTile), 0, s, tiles, a, mu, p, force, torque, energy);
+void launch_electrostatics_pairs(hipStream_t s, int nrows, const int* rows, const int* owner,
+                                 const unsigned long long* tmask, PairAtoms<R> a, const R* mu, PairParams<R> p,
+                                 acc_t* force, acc_t* torque, acc_t* energy) {
+    if (nrows <= 0) return;
+    hipLaunchKernelGGL(k_electrostatics<R>, dim3(nrows), dim3(kElecThreads), 0, s, rows, owner, tmask, a, mu, p, force,
+                       torque, energy);
 }
 
 template <class R> void launch_cov_fixed_field(hipStream_t s, int n, PairAtoms<R> a, PairParams<R> p, acc_t* field) {
@@ -340,8 +374,9 @@ void launch_cov_electrostatics(hipStream_t s, int n, PairAtoms<R> a, const R* mu
 
 #define MPID_INSTANTIATE(R)                                                                                        \
     template void launch_block_bounds<R>(hipStream_t, int, const R4<R>*, float4*, float4*);                         \
-    template void launch_electrostatics_pairs<R>(hipStream_t, int, const int2*, PairAtoms<R>, const R*,            \
-                                                 PairParams<R>, acc_t*, acc_t*, acc_t*);             \
+    template void launch_electrostatics_pairs<R>(hipStream_t, int, const int*, const int*,                         \
+                                                 const unsigned long long*, PairAtoms<R>, const R*, PairParams<R>,  \
+                                                 acc_t*, acc_t*, acc_t*);                                           \
     template void launch_cov_fixed_field<R>(hipStream_t, int, PairAtoms<R>, PairParams<R>, acc_t*);                \
     template void launch_cov_induced_field<R>(hipStream_t, int, PairAtoms<R>, const R*, PairParams<R>, acc_t*);    \
     template void launch_cov_electrostatics<R>(hipStream_t, int, PairAtoms<R>, const R*, PairParams<R>, acc_t*,    \

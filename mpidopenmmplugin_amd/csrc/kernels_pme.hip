@@ -1,15 +1,15 @@
 // Reciprocal-space (PME) kernels of the MPIDForce path on gfx950: order-6
-// B-spline spreading of multipoles and induced dipoles, a hand-written
-// Stockham FFT, the influence-function convolution and the gathering of the
-// potential and its fractional derivatives.
+// B-spline spreading of multipoles and induced dipoles onto the real charge
+// grid, and the gathering of the potential and its fractional derivatives
+// (the FFT and convolution are in kernels_fft.hip).
 //
 // Follows MPIDReferencePmeForce (platforms/reference/src/SimTKReference/
 // MPIDReferenceForce.cpp): computeBSplinePoint :2956-3044, computeMPIDBsplines
 // :3049-3075, transformMultipolesToFractionalCoordinates :3077-3169,
 // spreadFixedMultipolesOntoGrid :3269-3327, spreadInducedDipolesOnGrid
-// :3532-3573, performMPIDReciprocalConvolution :3329-3366,
-// computeFixedPotentialFromGrid :3368-3530.  The reference's grid is complex
-// with x slowest and z fastest; so is this one.
+// :3532-3573, computeFixedPotentialFromGrid :3368-3530.  The grid has x
+// slowest and z fastest, as the reference's; it holds only the real part,
+// which is all the reference ever fills.
 #include "engine.h"
 
 #include <algorithm>
@@ -82,7 +82,7 @@ __device__ __forceinline__ void grid_coord(R x, R L, R inv, int n, int& ig, R& w
 // ---------------------------------------------------------------------------
 template <class R>
 __global__ __launch_bounds__(256) void k_spread_fixed(int n, const R4<R>* __restrict__ pos, const R* __restrict__ lab,
-                                                       PmeGeom<R> g, R2<R>* __restrict__ grid) {
+                                                       PmeGeom<R> g, R* __restrict__ grid) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const R4<R> p = pos[i];
@@ -125,7 +125,7 @@ __global__ __launch_bounds__(256) void k_spread_fixed(int n, const R4<R>* __rest
                 int x = ig[0] + ix;
                 if (x >= g.n[0]) x -= g.n[0];
                 const R v = t0 * tx[ix][0] + t1 * tx[ix][1] + t2 * tx[ix][2] + t3 * tx[ix][3];
-                atomicAdd(&grid[((size_t)x * ny + y) * nz + z].x, v);
+                atomicAdd(&grid[((size_t)x * ny + y) * nz + z], v);
             }
         }
     }
@@ -133,7 +133,7 @@ __global__ __launch_bounds__(256) void k_spread_fixed(int n, const R4<R>* __rest
 
 template <class R>
 __global__ __launch_bounds__(256) void k_spread_induced(int n, const R4<R>* __restrict__ pos, const R* __restrict__ mu,
-                                                         PmeGeom<R> g, R2<R>* __restrict__ grid) {
+                                                         PmeGeom<R> g, R* __restrict__ grid) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const R4<R> p = pos[i];
@@ -161,7 +161,7 @@ __global__ __launch_bounds__(256) void k_spread_induced(int n, const R4<R>* __re
             for (int ix = 0; ix < kOrder; ix++) {
                 int x = ig[0] + ix;
                 if (x >= g.n[0]) x -= g.n[0];
-                atomicAdd(&grid[((size_t)x * ny + y) * nz + z].x, t0 * tx[ix][0] + t1 * tx[ix][1]);
+                atomicAdd(&grid[((size_t)x * ny + y) * nz + z], t0 * tx[ix][0] + t1 * tx[ix][1]);
             }
         }
     }
@@ -201,15 +201,17 @@ template <class R, int MODE>
 __global__ __launch_bounds__(256) void k_spread_tiled(const R4<R>* __restrict__ pos, const R* __restrict__ src,
                                                        PmeGeom<R> g, TileGeom tg, const int* __restrict__ tstart,
                                                        const int* __restrict__ tend, R* __restrict__ scratch,
-                                                       R2<R>* __restrict__ grid) {
+                                                       R* __restrict__ grid) {
     constexpr int ND = MODE == 0 ? 3 : 1;   // spline derivatives needed
     constexpr int NC = MODE == 0 ? 20 : 3;  // fractional components
     constexpr int CH = 32;                  // atoms staged per pass
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int W0 = tg.W[0], W1 = tg.W[1], W2 = tg.W[2];
     const int WV = W0 * W1 * W2;
-    R* win = reinterpret_cast<R*>(smem_raw);
-    R* sth = win + WV;                                // [CH][3][6][ND+1]
+    // The window accumulates in FP64 for both precisions: on gfx950 LDS
+    // atomic adds of doubles (ds_add_f64) keep up, those of floats stall.
+    double* win = reinterpret_cast<double*>(smem_raw);
+    R* sth = reinterpret_cast<R*>(win + WV);          // [CH][3][6][ND+1]
     R* smf = sth + CH * 3 * kOrder * (ND + 1);        // [CH][NC]
     int* sig = reinterpret_cast<int*>(smf + CH * NC); // [CH][3] local window origin of the stencil
     const int t = blockIdx.x;
@@ -218,7 +220,7 @@ __global__ __launch_bounds__(256) void k_spread_tiled(const R4<R>* __restrict__ 
     int w0[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) w0[k] = ((c[k] * tg.T[k] + (g.n[k] >> 1) - 6) % g.n[k] + g.n[k]) % g.n[k];
-    for (int i = threadIdx.x; i < WV; i += blockDim.x) win[i] = R(0);
+    for (int i = threadIdx.x; i < WV; i += blockDim.x) win[i] = 0.0;
     const int a_begin = tstart[t], a_end = tend[t];
     __syncthreads();
     for (int a0 = a_begin; a0 < a_end; a0 += CH) {
@@ -289,24 +291,24 @@ __global__ __launch_bounds__(256) void k_spread_tiled(const R4<R>* __restrict__ 
                 if (MODE == 0) v += t2 * tx[ix * (ND + 1) + 2] + t3 * tx[ix * (ND + 1) + ND];
                 const int lx = lx0 + ix;
                 if (inside_yz && lx < W0) {
-                    atomicAdd(&win[(lx * W1 + ly) * W2 + lz], v);
+                    atomicAdd(&win[(lx * W1 + ly) * W2 + lz], (double)v);
                 } else {
                     // outside the window (not expected): straight to the grid
                     const int x = (w0[0] + lx) % g.n[0], y = (w0[1] + ly) % g.n[1], z = (w0[2] + lz) % g.n[2];
-                    atomicAdd(&grid[((size_t)x * g.n[1] + y) * g.n[2] + z].x, v);
+                    atomicAdd(&grid[((size_t)x * g.n[1] + y) * g.n[2] + z], v);
                 }
             }
         }
         __syncthreads();
     }
     R* out = scratch + (size_t)t * WV;
-    for (int i = threadIdx.x; i < WV; i += blockDim.x) out[i] = win[i];
+    for (int i = threadIdx.x; i < WV; i += blockDim.x) out[i] = (R)win[i];
 }
 
 // grid(x,y,z) = sum of the windows covering the point (plus what is already there)
 template <class R>
 __global__ __launch_bounds__(256) void k_fold(PmeGeom<R> g, TileGeom tg, const R* __restrict__ scratch,
-                                               R2<R>* __restrict__ grid) {
+                                               R* __restrict__ grid) {
     const size_t total = (size_t)g.n[0] * g.n[1] * g.n[2];
     const int WV = tg.W[0] * tg.W[1] * tg.W[2];
     const int nt = tg.tiles_per_axis;
@@ -323,7 +325,7 @@ __global__ __launch_bounds__(256) void k_fold(PmeGeom<R> g, TileGeom tg, const R
             rr[k] = q - cc[k] * tg.T[k];
             nj[k] = min((tg.W[k] - 1 - rr[k]) / tg.T[k] + 1, nt);  // windows c, c-1, ... covering the point
         }
-        R sum = grid[id].x;
+        R sum = grid[id];
         for (int jx = 0; jx < nj[0]; jx++)
             for (int jy = 0; jy < nj[1]; jy++)
                 for (int jz = 0; jz < nj[2]; jz++) {
@@ -333,37 +335,7 @@ __global__ __launch_bounds__(256) void k_fold(PmeGeom<R> g, TileGeom tg, const R
                     const int lx = rr[0] + jx * tg.T[0], ly = rr[1] + jy * tg.T[1], lz = rr[2] + jz * tg.T[2];
                     sum += scratch[(size_t)tile * WV + (lx * tg.W[1] + ly) * tg.W[2] + lz];
                 }
-        grid[id] = R2<R>{sum, R(0)};
-    }
-}
-
-// ---------------------------------------------------------------------------
-// convolution with the PME influence function
-// ---------------------------------------------------------------------------
-template <class R>
-__global__ __launch_bounds__(256) void k_convolve(PmeGeom<R> g, const R* __restrict__ moduli, R alpha, R2<R>* grid) {
-    const size_t total = (size_t)g.n[0] * g.n[1] * g.n[2];
-    const R expf = R(M_PI * M_PI) / (alpha * alpha);
-    const R sc = R(1.0 / M_PI) * g.box.inv[0] * g.box.inv[1] * g.box.inv[2];
-    for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < total; id += (size_t)gridDim.x * blockDim.x) {
-        const int kz = (int)(id % g.n[2]);
-        const int ky = (int)((id / g.n[2]) % g.n[1]);
-        const int kx = (int)(id / ((size_t)g.n[1] * g.n[2]));
-        if (id == 0) {
-            grid[0] = R2<R>{R(0), R(0)};
-            continue;
-        }
-        const int mx = kx < (g.n[0] + 1) / 2 ? kx : kx - g.n[0];
-        const int my = ky < (g.n[1] + 1) / 2 ? ky : ky - g.n[1];
-        const int mz = kz < (g.n[2] + 1) / 2 ? kz : kz - g.n[2];
-        const R hx = R(mx) * g.box.inv[0], hy = R(my) * g.box.inv[1], hz = R(mz) * g.box.inv[2];
-        const R m2 = hx * hx + hy * hy + hz * hz;
-        const R den = m2 * moduli[kx] * moduli[g.n[0] + ky] * moduli[g.n[0] + g.n[1] + kz];
-        const R et = sc * exp_r(-expf * m2) / den;
-        R2<R> v = grid[id];
-        v.x *= et;
-        v.y *= et;
-        grid[id] = v;
+        grid[id] = sum;
     }
 }
 
@@ -374,7 +346,7 @@ __global__ __launch_bounds__(256) void k_convolve(PmeGeom<R> g, const R* __restr
 // order (MPIDReferenceForce.cpp:3498-3532).
 template <class R>
 __global__ __launch_bounds__(128) void k_gather_phi(int n, const R4<R>* __restrict__ pos, PmeGeom<R> g,
-                                                     const R2<R>* __restrict__ grid, R* __restrict__ phi) {
+                                                     const R* __restrict__ grid, R* __restrict__ phi) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const R4<R> p = pos[i];
@@ -405,7 +377,7 @@ __global__ __launch_bounds__(128) void k_gather_phi(int n, const R4<R>* __restri
             for (int ix = 0; ix < kOrder; ix++) {
                 int x = ig[0] + ix;
                 if (x >= g.n[0]) x -= g.n[0];
-                const R q = grid[((size_t)x * ny + y) * nz + z].x;
+                const R q = grid[((size_t)x * ny + y) * nz + z];
                 a0 += q * tx[ix][0];
                 a1 += q * tx[ix][1];
                 a2 += q * tx[ix][2];
@@ -437,7 +409,7 @@ __global__ __launch_bounds__(128) void k_gather_phi(int n, const R4<R>* __restri
 // Field -grad(phi) only (the matrix-vector product of the solver needs no more).
 template <class R>
 __global__ __launch_bounds__(256) void k_gather_field(int n, const R4<R>* __restrict__ pos, PmeGeom<R> g,
-                                                       const R2<R>* __restrict__ grid, R* __restrict__ field) {
+                                                       const R* __restrict__ grid, R* __restrict__ field) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const R4<R> p = pos[i];
@@ -464,7 +436,7 @@ __global__ __launch_bounds__(256) void k_gather_field(int n, const R4<R>* __rest
             for (int ix = 0; ix < kOrder; ix++) {
                 int x = ig[0] + ix;
                 if (x >= g.n[0]) x -= g.n[0];
-                const R q = grid[((size_t)x * ny + y) * nz + z].x;
+                const R q = grid[((size_t)x * ny + y) * nz + z];
                 a0 += q * tx[ix][0];
                 a1 += q * tx[ix][1];
             }
@@ -482,167 +454,14 @@ __global__ __launch_bounds__(256) void k_gather_field(int n, const R4<R>* __rest
 }
 
 // ---------------------------------------------------------------------------
-// Stockham FFT: each workgroup transforms T lines of length n held in LDS;
-// each stage applies one radix (2, 3, 4 or 5) to every line.
-// ---------------------------------------------------------------------------
-template <class R> struct Cx {
-    R x, y;
-};
-template <class R> __device__ __forceinline__ Cx<R> cmul(Cx<R> a, Cx<R> b) {
-    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
-}
-template <class R> __device__ __forceinline__ Cx<R> cadd(Cx<R> a, Cx<R> b) { return {a.x + b.x, a.y + b.y}; }
-template <class R> __device__ __forceinline__ Cx<R> csub(Cx<R> a, Cx<R> b) { return {a.x - b.x, a.y - b.y}; }
-// multiply by s*i, s = +-1
-template <class R> __device__ __forceinline__ Cx<R> cmuli(Cx<R> a, R s) { return {-s * a.y, s * a.x}; }
-
-template <class R, int RAD> __device__ __forceinline__ void dft_small(Cx<R>* v, R s) {
-    if (RAD == 2) {
-        Cx<R> a = v[0], b = v[1];
-        v[0] = cadd(a, b);
-        v[1] = csub(a, b);
-    } else if (RAD == 4) {
-        Cx<R> a0 = v[0], a1 = v[1], a2 = v[2], a3 = v[3];
-        Cx<R> t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = cmuli(csub(a1, a3), s);
-        v[0] = cadd(t0, t2);
-        v[1] = cadd(t1, t3);
-        v[2] = csub(t0, t2);
-        v[3] = csub(t1, t3);
-    } else if (RAD == 3) {
-        const R h = R(0.86602540378443865);
-        Cx<R> a0 = v[0], t1 = cadd(v[1], v[2]), t2 = cmuli(csub(v[1], v[2]), s * h);
-        Cx<R> m = {a0.x - R(0.5) * t1.x, a0.y - R(0.5) * t1.y};
-        v[0] = cadd(a0, t1);
-        v[1] = cadd(m, t2);
-        v[2] = csub(m, t2);
-    } else if (RAD == 5) {
-        const R c1 = R(0.30901699437494742), c2 = R(-0.80901699437494742);
-        const R s1 = R(0.95105651629515357), s2 = R(0.58778525229247313);
-        Cx<R> a0 = v[0];
-        Cx<R> t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]), t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
-        Cx<R> m1 = {a0.x + c1 * t1.x + c2 * t2.x, a0.y + c1 * t1.y + c2 * t2.y};
-        Cx<R> m2 = {a0.x + c2 * t1.x + c1 * t2.x, a0.y + c2 * t1.y + c1 * t2.y};
-        Cx<R> n1 = cmuli(Cx<R>{s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y}, s);
-        Cx<R> n2 = cmuli(Cx<R>{s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y}, s);
-        v[0] = cadd(a0, cadd(t1, t2));
-        v[1] = cadd(m1, n1);
-        v[4] = csub(m1, n1);
-        v[2] = cadd(m2, n2);
-        v[3] = csub(m2, n2);
-    }
-}
-
-template <class R, int RAD>
-__device__ __forceinline__ void stockham_stage(const Cx<R>* in, Cx<R>* out, int n, int T, int Ns,
-                                               const R2<R>* __restrict__ tw, R s) {
-    const int nb = n / RAD;
-    const int twstep = n / (Ns * RAD);
-    for (int b = threadIdx.x; b < T * nb; b += blockDim.x) {
-        const int t = b / nb, j = b - t * nb;
-        const Cx<R>* li = in + t * n;
-        Cx<R>* lo = out + t * n;
-        const int kk = j % Ns;
-        Cx<R> v[RAD];
-#pragma unroll
-        for (int r = 0; r < RAD; r++) {
-            Cx<R> a = li[j + r * nb];
-            if (r > 0 && kk > 0) {
-                // table holds exp(-2 pi i k / n); exp(s 2 pi i k / n) = (w.x, -s w.y)
-                const R2<R> w = tw[kk * r * twstep];
-                a = cmul(a, Cx<R>{w.x, -s * w.y});
-            }
-            v[r] = a;
-        }
-        dft_small<R, RAD>(v, s);
-        const int o = (j / Ns) * Ns * RAD + kk;
-#pragma unroll
-        for (int r = 0; r < RAD; r++) lo[o + r * Ns] = v[r];
-    }
-}
-
-// mode 0: lines along z (contiguous), lines enumerated by (x,y)
-// mode 1: lines along y, batches of consecutive z for one x
-// mode 2: lines along x, batches of consecutive z for one y
-template <class R>
-__global__ __launch_bounds__(256) void k_fft_lines(R2<R>* data, int nx, int ny, int nz, int mode, int T,
-                                                    const R2<R>* __restrict__ tw, int nrad, int4 radA, int4 radB,
-                                                    R s) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    Cx<R>* bufA = reinterpret_cast<Cx<R>*>(smem_raw);
-    const int n = mode == 0 ? nz : (mode == 1 ? ny : nx);
-    Cx<R>* bufB = bufA + (size_t)T * n;
-    size_t base;
-    size_t es, ts;
-    int count;
-    if (mode == 0) {
-        const size_t l0 = (size_t)blockIdx.x * T;
-        const size_t nl = (size_t)nx * ny;
-        count = (int)min((size_t)T, nl - l0);
-        base = l0 * nz;
-        es = 1;
-        ts = nz;
-    } else {
-        const int zb = (nz + T - 1) / T;
-        const int o = blockIdx.x / zb;
-        const int z0 = (blockIdx.x - o * zb) * T;
-        count = min(T, nz - z0);
-        if (mode == 1) {
-            base = (size_t)o * ny * nz + z0;
-            es = nz;
-        } else {
-            base = (size_t)o * nz + z0;
-            es = (size_t)ny * nz;
-        }
-        ts = 1;
-    }
-    const int total = T * n;
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        int t, k;
-        if (mode == 0) { t = idx / n; k = idx - t * n; } else { k = idx / T; t = idx - k * T; }
-        Cx<R> v = {R(0), R(0)};
-        if (t < count) {
-            const R2<R> a = data[base + (size_t)t * ts + (size_t)k * es];
-            v.x = a.x;
-            v.y = a.y;
-        }
-        bufA[t * n + k] = v;
-    }
-    __syncthreads();
-    int rads[8] = {radA.x, radA.y, radA.z, radA.w, radB.x, radB.y, radB.z, radB.w};
-    int Ns = 1;
-    Cx<R>* in = bufA;
-    Cx<R>* out = bufB;
-    for (int st = 0; st < nrad; st++) {
-        const int rad = rads[st];
-        if (rad == 4) stockham_stage<R, 4>(in, out, n, T, Ns, tw, s);
-        else if (rad == 2) stockham_stage<R, 2>(in, out, n, T, Ns, tw, s);
-        else if (rad == 3) stockham_stage<R, 3>(in, out, n, T, Ns, tw, s);
-        else stockham_stage<R, 5>(in, out, n, T, Ns, tw, s);
-        Ns *= rad;
-        __syncthreads();
-        Cx<R>* tmp = in;
-        in = out;
-        out = tmp;
-    }
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        int t, k;
-        if (mode == 0) { t = idx / n; k = idx - t * n; } else { k = idx / T; t = idx - k * T; }
-        if (t < count) {
-            const Cx<R> v = in[t * n + k];
-            data[base + (size_t)t * ts + (size_t)k * es] = R2<R>{v.x, v.y};
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 template <class R>
-void launch_spread_fixed(hipStream_t s, int n, const R4<R>* pos, const R* lab, PmeGeom<R> g, R2<R>* grid) {
+void launch_spread_fixed(hipStream_t s, int n, const R4<R>* pos, const R* lab, PmeGeom<R> g, R* grid) {
     hipLaunchKernelGGL(k_spread_fixed<R>, dim3((n + 255) / 256), dim3(256), 0, s, n, pos, lab, g, grid);
 }
 template <class R>
-void launch_spread_induced(hipStream_t s, int n, const R4<R>* pos, const R* mu, PmeGeom<R> g, R2<R>* grid) {
+void launch_spread_induced(hipStream_t s, int n, const R4<R>* pos, const R* mu, PmeGeom<R> g, R* grid) {
     hipLaunchKernelGGL(k_spread_induced<R>, dim3((n + 255) / 256), dim3(256), 0, s, n, pos, mu, g, grid);
 }
 void launch_tile_ranges(hipStream_t s, int n, const unsigned* keys_sorted, int shift, int* tstart, int* tend) {
@@ -652,7 +471,7 @@ void launch_tile_ranges(hipStream_t s, int n, const unsigned* keys_sorted, int s
 template <class R, int MODE> static size_t tiled_lds(const TileGeom& tg) {
     constexpr int ND = MODE == 0 ? 3 : 1;
     constexpr int NC = MODE == 0 ? 20 : 3;
-    return (size_t)tg.W[0] * tg.W[1] * tg.W[2] * sizeof(R) + 32 * (3 * kOrder * (ND + 1) + NC) * sizeof(R) +
+    return (size_t)tg.W[0] * tg.W[1] * tg.W[2] * sizeof(double) + 32 * (3 * kOrder * (ND + 1) + NC) * sizeof(R) +
            32 * 3 * sizeof(int);
 }
 size_t spread_tiled_lds_bytes(const TileGeom& tg, int double_precision) {
@@ -661,77 +480,41 @@ size_t spread_tiled_lds_bytes(const TileGeom& tg, int double_precision) {
 
 template <class R>
 void launch_spread_fixed_tiled(hipStream_t s, const R4<R>* pos, const R* lab, PmeGeom<R> g, TileGeom tg,
-                               const int* tstart, const int* tend, R* scratch, R2<R>* grid) {
+                               const int* tstart, const int* tend, R* scratch, R* grid) {
     const size_t shm = tiled_lds<R, 0>(tg);
     k_spread_tiled<R, 0><<<dim3(tg.ntiles), dim3(256), shm, s>>>(pos, lab, g, tg, tstart, tend, scratch, grid);
 }
 template <class R>
 void launch_spread_induced_tiled(hipStream_t s, const R4<R>* pos, const R* mu, PmeGeom<R> g, TileGeom tg,
-                                 const int* tstart, const int* tend, R* scratch, R2<R>* grid) {
+                                 const int* tstart, const int* tend, R* scratch, R* grid) {
     const size_t shm = tiled_lds<R, 1>(tg);
     k_spread_tiled<R, 1><<<dim3(tg.ntiles), dim3(256), shm, s>>>(pos, mu, g, tg, tstart, tend, scratch, grid);
 }
-template <class R> void launch_fold(hipStream_t s, PmeGeom<R> g, TileGeom tg, const R* scratch, R2<R>* grid) {
+template <class R> void launch_fold(hipStream_t s, PmeGeom<R> g, TileGeom tg, const R* scratch, R* grid) {
     size_t total = (size_t)g.n[0] * g.n[1] * g.n[2];
     int blocks = (int)std::min<size_t>((total + 255) / 256, 16384);
     hipLaunchKernelGGL(k_fold<R>, dim3(blocks), dim3(256), 0, s, g, tg, scratch, grid);
 }
 
 template <class R>
-void launch_convolve(hipStream_t s, PmeGeom<R> g, const R* moduli, R alpha, R2<R>* grid) {
-    size_t total = (size_t)g.n[0] * g.n[1] * g.n[2];
-    int blocks = (int)std::min<size_t>((total + 255) / 256, 8192);
-    hipLaunchKernelGGL(k_convolve<R>, dim3(blocks), dim3(256), 0, s, g, moduli, alpha, grid);
-}
-template <class R>
-void launch_gather_phi(hipStream_t s, int n, const R4<R>* pos, PmeGeom<R> g, const R2<R>* grid, R* phi) {
+void launch_gather_phi(hipStream_t s, int n, const R4<R>* pos, PmeGeom<R> g, const R* grid, R* phi) {
     hipLaunchKernelGGL(k_gather_phi<R>, dim3((n + 127) / 128), dim3(128), 0, s, n, pos, g, grid, phi);
 }
 template <class R>
-void launch_gather_field(hipStream_t s, int n, const R4<R>* pos, PmeGeom<R> g, const R2<R>* grid, R* field) {
+void launch_gather_field(hipStream_t s, int n, const R4<R>* pos, PmeGeom<R> g, const R* grid, R* field) {
     hipLaunchKernelGGL(k_gather_field<R>, dim3((n + 255) / 256), dim3(256), 0, s, n, pos, g, grid, field);
 }
 
-template <class R> void launch_fft3d(hipStream_t s, const FftPlan<R>& plan, R2<R>* grid, int sign) {
-    const int nx = plan.n[0], ny = plan.n[1], nz = plan.n[2];
-    const R sg = (R)sign;
-    // pass order z, y, x
-    for (int pass = 0; pass < 3; pass++) {
-        const int axis = 2 - pass;  // 2 = z, 1 = y, 0 = x
-        const int mode = pass;      // 0: z lines, 1: y lines, 2: x lines
-        const int n = plan.n[axis];
-        if (n == 1) continue;
-        int T = (int)(32768 / (2 * (size_t)n * sizeof(R2<R>)));
-        if (T > 64) T = 64;
-        if (T < 1) T = 1;
-        int blocks;
-        if (mode == 0) {
-            blocks = (int)(((size_t)nx * ny + T - 1) / T);
-        } else {
-            const int zb = (nz + T - 1) / T;
-            blocks = (mode == 1 ? nx : ny) * zb;
-        }
-        const int* r = plan.radix[axis];
-        int4 ra = make_int4(r[0], r[1], r[2], r[3]);
-        int4 rb = make_int4(r[4], r[5], r[6], r[7]);
-        size_t shm = 2 * (size_t)T * n * sizeof(R2<R>);
-        hipLaunchKernelGGL(k_fft_lines<R>, dim3(blocks), dim3(256), shm, s, grid, nx, ny, nz, mode, T, plan.tw[axis],
-                           plan.nradix[axis], ra, rb, sg);
-    }
-}
-
 #define MPID_INSTANTIATE(R)                                                                                   \
-    template void launch_spread_fixed<R>(hipStream_t, int, const R4<R>*, const R*, PmeGeom<R>, R2<R>*);       \
-    template void launch_spread_induced<R>(hipStream_t, int, const R4<R>*, const R*, PmeGeom<R>, R2<R>*);     \
-    template void launch_convolve<R>(hipStream_t, PmeGeom<R>, const R*, R, R2<R>*);                          \
-    template void launch_gather_phi<R>(hipStream_t, int, const R4<R>*, PmeGeom<R>, const R2<R>*, R*);        \
-    template void launch_gather_field<R>(hipStream_t, int, const R4<R>*, PmeGeom<R>, const R2<R>*, R*);      \
-    template void launch_fft3d<R>(hipStream_t, const FftPlan<R>&, R2<R>*, int);                              \
+    template void launch_spread_fixed<R>(hipStream_t, int, const R4<R>*, const R*, PmeGeom<R>, R*);           \
+    template void launch_spread_induced<R>(hipStream_t, int, const R4<R>*, const R*, PmeGeom<R>, R*);         \
+    template void launch_gather_phi<R>(hipStream_t, int, const R4<R>*, PmeGeom<R>, const R*, R*);            \
+    template void launch_gather_field<R>(hipStream_t, int, const R4<R>*, PmeGeom<R>, const R*, R*);          \
     template void launch_spread_fixed_tiled<R>(hipStream_t, const R4<R>*, const R*, PmeGeom<R>, TileGeom,    \
-                                               const int*, const int*, R*, R2<R>*);                          \
+                                               const int*, const int*, R*, R*);                              \
     template void launch_spread_induced_tiled<R>(hipStream_t, const R4<R>*, const R*, PmeGeom<R>, TileGeom,  \
-                                                 const int*, const int*, R*, R2<R>*);                        \
-    template void launch_fold<R>(hipStream_t, PmeGeom<R>, TileGeom, const R*, R2<R>*);
+                                                 const int*, const int*, R*, R*);                            \
+    template void launch_fold<R>(hipStream_t, PmeGeom<R>, TileGeom, const R*, R*);
 MPID_INSTANTIATE(float)
 MPID_INSTANTIATE(double)
 #undef MPID_INSTANTIATE

@@ -148,7 +148,8 @@ template <class R> struct Prec {
     DBuf<R> mu;       // induced dipoles in this precision (matvec input)
     DBuf<R> erec;     // reciprocal induced field
     DBuf<R> phi, phidp;
-    DBuf<R2<R>> grid;
+    DBuf<R> grid;       // real charge / potential grid [n0][n1][n2]
+    DBuf<R2<R>> spec;   // half spectrum [n0][n1][n2/2+1]
     DBuf<R> moduli;
     DBuf<R2<R>> tw[3];
     DBuf<R> scratch;  // spreading windows of the Morton tiles
@@ -184,13 +185,13 @@ struct mpid_context {
     TileGeom tgeom{};
     bool tiled_spread = false;
     DBuf<int> tstart, tend;
-    // tiles
+    // 64-atom blocks: bounding boxes, block rows (CSR: for each block I the
+    // blocks J within the cutoff, with their owner I), per-atom neighbour lists
+    // of capacity nl_cap, and the tile masks of the row entries with J >= I
     DBuf<float4> bmin, bmax;
-    DBuf<int2> tiles;
-    DBuf<int> tile_count;
-    int ntiles = 0;
-    // neighbour lists: block rows (CSR) and per-atom lists of capacity nl_cap
-    DBuf<int> brow_count, brow_start, brows, nbr, ncount, nl_scalars;
+    DBuf<int> brow_count, brow_start, brows, brow_owner, nbr, ncount, nl_scalars;
+    DBuf<unsigned long long> tmask;
+    int rows_total = 0;
     int nl_cap = 0;
     int nl_max = 0;
     // accumulators
@@ -369,7 +370,9 @@ struct mpid_context {
         if (opt.flags & ~MPID_FLAG_CONSERVATIVE_TORQUES) throw MpidError(MPID_ERR_INVALID, "unknown option flags");
         for (int k = 0; k < 3; k++) {
             int rr[16], c;
-            if (opt.grid[k] < kOrder || opt.grid[k] > 1024 || !factor_radices(opt.grid[k], rr, c))
+            FftAxis ax;
+            if (opt.grid[k] < kOrder || opt.grid[k] > 1024 || !factor_radices(opt.grid[k], rr, c) ||
+                !make_fft_axis(opt.grid[k], ax))
                 throw MpidError(MPID_ERR_UNSUPPORTED, "PME grid dimension " + std::to_string(opt.grid[k]) +
                                                           " is not of the form 2^a 3^b 5^c between 6 and 1024");
         }
@@ -386,6 +389,7 @@ struct mpid_context {
         P.phi.resize(35 * (size_t)n);
         P.phidp.resize(35 * (size_t)n);
         P.grid.resize(G);
+        P.spec.resize((size_t)opt.grid[0] * opt.grid[1] * (opt.grid[2] / 2 + 1));
         std::vector<R> mods;
         for (int k = 0; k < 3; k++) {
             std::vector<double> m = bspline_moduli(opt.grid[k]);
@@ -398,10 +402,8 @@ struct mpid_context {
                 tw[j].y = (R)std::sin(ang);
             }
             P.tw[k].upload(tw.data(), tw.size(), stream);
-            P.plan.n[k] = nk;
             P.plan.tw[k] = P.tw[k].p;
-            for (int j = 0; j < 16; j++) P.plan.radix[k][j] = 0;
-            factor_radices(nk, P.plan.radix[k], P.plan.nradix[k]);
+            if (!make_fft_axis(nk, P.plan.ax[k])) throw MpidError(MPID_ERR_UNSUPPORTED, "PME grid size not supported");
         }
         P.moduli.upload(mods.data(), mods.size(), stream);
         if (tiled_spread) P.scratch.resize((size_t)tgeom.ntiles * tgeom.W[0] * tgeom.W[1] * tgeom.W[2]);
@@ -433,8 +435,6 @@ struct mpid_context {
         const int nb = (n + kTile - 1) / kTile;
         bmin.resize(nb);
         bmax.resize(nb);
-        tiles.resize((size_t)nb * (nb + 1) / 2);
-        tile_count.resize(1);
         brow_count.resize(nb);
         brow_start.resize(nb + 1);
         ncount.resize(n);
@@ -584,29 +584,24 @@ struct mpid_context {
             launch_convert<double>(stream, n, spos.p, lab.p, sthole.p, damp.p, pd.pos.p, pd.lab.p, pd.td.p);
         });
         check_launch();
-        // tiles
+        // blocks, block rows, neighbour lists and tile masks
         const int nb = (n + kTile - 1) / kTile;
         timed(P_TILES, [&] {
             launch_block_bounds<double>(stream, n, pd.pos.p, bmin.p, bmax.p);
-            HIP_CHECK(hipMemsetAsync(tile_count.p, 0, sizeof(int), stream));
-            launch_build_tiles(stream, nb, bmin.p, bmax.p, rbox<float>(), (float)opt.cutoff, tiles.p, tile_count.p,
-                               (int)tiles.n);
-        });
-        check_launch();
-        timed(P_TILES, [&] {
             launch_block_rows(stream, nb, bmin.p, bmax.p, rbox<float>(), (float)opt.cutoff, nullptr, brow_count.p,
-                              nullptr);
+                              nullptr, nullptr);
             launch_exclusive_scan(stream, nb, brow_count.p, brow_start.p);
         });
         check_launch();
-        int cnt = 0, rows_total = 0;
-        HIP_CHECK(hipMemcpyAsync(&cnt, tile_count.p, sizeof(int), hipMemcpyDeviceToHost, stream));
         HIP_CHECK(hipMemcpyAsync(&rows_total, brow_start.p + nb, sizeof(int), hipMemcpyDeviceToHost, stream));
         HIP_CHECK(hipStreamSynchronize(stream));
-        if (cnt > (int)tiles.n) throw MpidError(MPID_ERR_HIP, "tile list overflow");
-        ntiles = cnt;
-        stats.ntiles = cnt;
-        if ((size_t)rows_total > brows.n) brows.resize(rows_total + rows_total / 4 + 64);
+        if ((size_t)rows_total > brows.n) {
+            const size_t cap = rows_total + rows_total / 4 + 64;
+            brows.resize(cap);
+            brow_owner.resize(cap);
+            tmask.resize(cap * kTile);
+        }
+        stats.ntiles = (rows_total + nb) / 2;  // block pairs with J >= I
         build_neighbour_lists(nb);
     }
 
@@ -620,15 +615,15 @@ struct mpid_context {
         }
         timed(P_TILES, [&] {
             launch_block_rows(stream, nb, bmin.p, bmax.p, rbox<float>(), (float)opt.cutoff, brow_start.p, nullptr,
-                              brows.p);
+                              brows.p, brow_owner.p);
         });
         for (int attempt = 0; attempt < 2; attempt++) {
             nl_cap = (nl_cap + kNlSplit - 1) / kNlSplit * kNlSplit;
             nbr.resize((size_t)n * nl_cap);
             timed(P_TILES, [&] {
                 HIP_CHECK(hipMemsetAsync(nl_scalars.p, 0, sizeof(int), stream));
-                launch_nl_build(stream, n, brow_start.p, brows.p, pd.pos.p, spart.p, snpart.p, dbox,
-                                opt.cutoff * opt.cutoff, nl_cap, nbr.p, ncount.p, nl_scalars.p);
+                launch_nl_build(stream, n, brow_start.p, brows.p, bmin.p, bmax.p, pd.pos.p, spart.p, snpart.p, dbox,
+                                opt.cutoff * opt.cutoff, nl_cap, nbr.p, ncount.p, nl_scalars.p, tmask.p);
             });
             check_launch();
             HIP_CHECK(hipMemcpyAsync(&nl_max, nl_scalars.p, sizeof(int), hipMemcpyDeviceToHost, stream));
@@ -640,11 +635,20 @@ struct mpid_context {
     }
     NbrList nlist() const { return NbrList{nbr.p, ncount.p}; }
 
+    template <class R> ConvParams<R> conv_params(const Prec<R>& P) const {
+        ConvParams<R> c;
+        c.expf = (R)(M_PI * M_PI / (opt.ewald_alpha * opt.ewald_alpha));
+        c.sc = (R)(1.0 / M_PI * dbox.inv[0] * dbox.inv[1] * dbox.inv[2]);
+        for (int k = 0; k < 3; k++) c.inv[k] = (R)dbox.inv[k];
+        c.moduli = P.moduli.p;
+        return c;
+    }
+
     template <class R> void recip_cycle_fixed(Prec<R>& P) {
         PmeGeom<R> g = geom<R>();
         const size_t G = P.grid.n;
         timed(P_SPREAD, [&] {
-            launch_fill<R>(stream, (R*)P.grid.p, 2 * G, R(0));
+            launch_fill<R>(stream, P.grid.p, G, R(0));
             if (tiled_spread) {
                 launch_spread_fixed_tiled<R>(stream, P.pos.p, P.lab.p, g, tgeom, tstart.p, tend.p, P.scratch.p,
                                              P.grid.p);
@@ -653,9 +657,7 @@ struct mpid_context {
                 launch_spread_fixed<R>(stream, n, P.pos.p, P.lab.p, g, P.grid.p);
             }
         });
-        timed(P_FFT, [&] { launch_fft3d<R>(stream, P.plan, P.grid.p, -1); });
-        timed(P_CONV, [&] { launch_convolve<R>(stream, g, P.moduli.p, (R)opt.ewald_alpha, P.grid.p); });
-        timed(P_FFT, [&] { launch_fft3d<R>(stream, P.plan, P.grid.p, +1); });
+        timed(P_FFT, [&] { launch_recip_convolution<R>(stream, P.plan, P.grid.p, P.spec.p, conv_params(P)); });
         timed(P_GATHER, [&] { launch_gather_phi<R>(stream, n, P.pos.p, g, P.grid.p, P.phi.p); });
         check_launch();
     }
@@ -664,7 +666,7 @@ struct mpid_context {
         PmeGeom<R> g = geom<R>();
         const size_t G = P.grid.n;
         timed(P_SPREAD, [&] {
-            launch_fill<R>(stream, (R*)P.grid.p, 2 * G, R(0));
+            launch_fill<R>(stream, P.grid.p, G, R(0));
             if (tiled_spread) {
                 launch_spread_induced_tiled<R>(stream, P.pos.p, P.mu.p, g, tgeom, tstart.p, tend.p, P.scratch.p,
                                                P.grid.p);
@@ -673,9 +675,7 @@ struct mpid_context {
                 launch_spread_induced<R>(stream, n, P.pos.p, P.mu.p, g, P.grid.p);
             }
         });
-        timed(P_FFT, [&] { launch_fft3d<R>(stream, P.plan, P.grid.p, -1); });
-        timed(P_CONV, [&] { launch_convolve<R>(stream, g, P.moduli.p, (R)opt.ewald_alpha, P.grid.p); });
-        timed(P_FFT, [&] { launch_fft3d<R>(stream, P.plan, P.grid.p, +1); });
+        timed(P_FFT, [&] { launch_recip_convolution<R>(stream, P.plan, P.grid.p, P.spec.p, conv_params(P)); });
         timed(P_GATHER, [&] {
             if (full) launch_gather_phi<R>(stream, n, P.pos.p, g, P.grid.p, P.phidp.p);
             else launch_gather_field<R>(stream, n, P.pos.p, g, P.grid.p, P.erec.p);
@@ -834,8 +834,8 @@ struct mpid_context {
             launch_zero_acc(stream, force_acc.p, force_acc.n);
             launch_zero_acc(stream, torque_acc.p, torque_acc.n);
             launch_zero_acc(stream, energy_acc.p, 1);
-            launch_electrostatics_pairs<R>(stream, ntiles, tiles.p, pair_atoms(P), P.mu.p, pair_params<R>(),
-                                           force_acc.p, torque_acc.p, energy_acc.p);
+            launch_electrostatics_pairs<R>(stream, rows_total, brows.p, brow_owner.p, tmask.p, pair_atoms(P), P.mu.p,
+                                           pair_params<R>(), force_acc.p, torque_acc.p, energy_acc.p);
             launch_cov_electrostatics<double>(stream, n, pair_atoms(pd), mud.p, pair_params<double>(), force_acc.p,
                                               torque_acc.p, energy_acc.p);
         });

@@ -16,17 +16,17 @@ namespace mpid {
 // shared helpers
 // ---------------------------------------------------------------------------
 // Scale of a covalent partner entry (partner << 2 | class).
-template <class R> __device__ __forceinline__ R partner_scale(int e, R s14) {
+template <class R> MPID_HD R partner_scale(int e, R s14) {
     return ((e & 3) == kScaleZero) ? R(0) : s14;
 }
 
 // Quadrupole times vector and octopole contracted twice with a vector.
-template <class R> __device__ __forceinline__ V3<R> quad_dot(const R* Q, V3<R> v) {
+template <class R> MPID_HD V3<R> quad_dot(const R* Q, V3<R> v) {
     return {Q[0] * v.x + Q[1] * v.y + Q[2] * v.z, Q[1] * v.x + Q[3] * v.y + Q[4] * v.z,
             Q[2] * v.x + Q[4] * v.y + Q[5] * v.z};
 }
 // M = O . v (symmetric 3x3, stored XX XY XZ YY YZ ZZ)
-template <class R> __device__ __forceinline__ void oct_dot(const R* O, V3<R> v, R* M) {
+template <class R> MPID_HD void oct_dot(const R* O, V3<R> v, R* M) {
     M[0] = O[0] * v.x + O[1] * v.y + O[2] * v.z;  // xx.
     M[1] = O[1] * v.x + O[3] * v.y + O[4] * v.z;  // xy.
     M[2] = O[2] * v.x + O[4] * v.y + O[5] * v.z;  // xz.
@@ -34,7 +34,7 @@ template <class R> __device__ __forceinline__ void oct_dot(const R* O, V3<R> v, 
     M[4] = O[4] * v.x + O[7] * v.y + O[8] * v.z;  // yz.
     M[5] = O[5] * v.x + O[8] * v.y + O[9] * v.z;  // zz.
 }
-template <class R> __device__ __forceinline__ R quad_form(const R* M, V3<R> a, V3<R> b) {
+template <class R> MPID_HD R quad_form(const R* M, V3<R> a, V3<R> b) {
     return dot(a, quad_dot(M, b));
 }
 
@@ -45,14 +45,14 @@ template <class R> __device__ __forceinline__ R quad_form(const R* M, V3<R> a, V
 // dr = r_j - r_i inside the cutoff and ds the multipole/polarization scale of
 // the pair (MPIDReferenceForce.cpp:2812-2920).
 template <class R>
-__device__ __forceinline__ void fixed_field_pair(V3<R> dr, R r2, const R* mi, const R* mj, R2<R> tdi, R2<R> tdj,
+MPID_HD void fixed_field_pair(V3<R> dr, R r2, const R* mi, const R* mj, R2<R> tdi, R2<R> tdj,
                                                  R ds, const PairParams<R>& P, V3<R>& fi, V3<R>& fj) {
     const R rinv = rsqrt_r(r2);
     const R r = r2 * rinv;
     const R rinv2 = rinv * rinv;
     const R ar = P.alpha * r;
     const R ex2 = exp_r(-ar * ar);
-    const R bn0 = erfc_r(ar) * rinv;
+    const R bn0 = erfc_r(ar, ex2) * rinv;
     const R bn1 = (bn0 + P.aln[1] * ex2) * rinv2;
     const R bn2 = (R(3) * bn1 + P.aln[2] * ex2) * rinv2;
     const R bn3 = (R(5) * bn2 + P.aln[3] * ex2) * rinv2;
@@ -62,7 +62,7 @@ __device__ __forceinline__ void fixed_field_pair(V3<R> dr, R r2, const R* mi, co
     const R damp = tdi.y * tdj.y;
     if (damp != R(0)) {
         const R a = (ds == R(0)) ? tdi.x + tdj.x : P.default_thole;
-        const R u = a * r / damp;
+        const R u = div_r(a * r, damp);
         if (u < R(50)) {
             const R e = exp_r(-u), u2 = u * u, u3 = u2 * u, u4 = u3 * u, u5 = u4 * u;
             const R p3 = R(1) + u + R(0.5) * u2;
@@ -111,24 +111,25 @@ __device__ __forceinline__ unsigned long long partner_mask(const int* __restrict
 // ---------------------------------------------------------------------------
 // induced dipole field (the real-space part of the solver's matrix-vector product)
 // ---------------------------------------------------------------------------
-// MPIDReferenceForce.cpp:4161-4281; ps selects the Thole width of the pair.
+// MPIDReferenceForce.cpp:4161-4281.  dampi, dampj: damping factors of the
+// two sites; a: the Thole width of the pair (the sum of the sites' Thole
+// factors for pairs with polarization scale 0, the default width otherwise).
 template <class R>
-__device__ __forceinline__ void induced_field_pair(V3<R> dr, R r2, V3<R> ui, V3<R> uj, R2<R> tdi, R2<R> tdj, R ps,
+MPID_HD void induced_field_pair(V3<R> dr, R r2, V3<R> ui, V3<R> uj, R dampi, R dampj, R a,
                                                    const PairParams<R>& P, V3<R>& fi, V3<R>& fj) {
     const R rinv = rsqrt_r(r2);
     const R r = r2 * rinv;
     const R rinv2 = rinv * rinv;
     const R ar = P.alpha * r;
     const R ex2 = exp_r(-ar * ar);
-    const R bn0 = erfc_r(ar) * rinv;
+    const R bn0 = erfc_r(ar, ex2) * rinv;
     const R bn1 = (bn0 + P.aln[1] * ex2) * rinv2;
     const R bn2 = (R(3) * bn1 + P.aln[2] * ex2) * rinv2;
     // (1 - lambda) of the Thole damping, computed directly
     R m3 = R(0), m5 = R(0);
-    const R damp = tdi.y * tdj.y;
+    const R damp = dampi * dampj;
     if (damp != R(0)) {
-        const R a = (ps == R(0)) ? tdi.x + tdj.x : P.default_thole;
-        const R u = a * r / damp;
+        const R u = div_r(a * r, damp);
         if (u < R(50)) {
             const R e = exp_r(-u);
             m3 = e * (R(1) + u + R(0.5) * u * u);
@@ -149,7 +150,7 @@ __device__ __forceinline__ void induced_field_pair(V3<R> dr, R r2, V3<R> ui, V3<
 // Spherical moments in a frame (ex, ey, ez): Cartesian tensors are contracted
 // with the frame axes and projected as in MPIDReferenceForce.cpp:321-342.
 template <class R>
-__device__ __forceinline__ void qi_moments(const R* m, V3<R> ex, V3<R> ey, V3<R> ez, R* q) {
+MPID_HD void qi_moments(const R* m, V3<R> ex, V3<R> ey, V3<R> ez, R* q) {
     const R s3 = R(1.7320508075688772);
     q[0] = m[0];
     const V3<R> d = mk(m[1], m[2], m[3]);
@@ -189,7 +190,7 @@ __device__ __forceinline__ void qi_moments(const R* m, V3<R> ex, V3<R> ey, V3<R>
 // generator tables were derived symbolically from the Cartesian definition of
 // the moments; they coincide with the hand-written tables at
 // MPIDReferenceForce.cpp:4438-4455.
-template <class R> __device__ __forceinline__ void qi_torque_sums(const R* q, const R* V, R& tx, R& ty, R& tz) {
+template <class R> MPID_HD void qi_torque_sums(const R* q, const R* V, R& tx, R& ty, R& tz) {
     const R s3 = R(1.7320508075688772), s6 = R(2.449489742783178), s52 = R(1.5811388300841898),
             s32 = R(1.2247448713915890);
     tx = q[3] * V[1] - q[1] * V[3] + s3 * q[6] * V[4] + q[8] * V[5] + (-s3 * q[4] - q[7]) * V[6] + q[6] * V[7] -
@@ -212,7 +213,7 @@ template <class R> struct ElecOut {
 };
 
 template <class R>
-__device__ __forceinline__ ElecOut<R> elec_pair(V3<R> dr, R r2, const R* ma, const R* mb, V3<R> mua, V3<R> mub,
+MPID_HD ElecOut<R> elec_pair(V3<R> dr, R r2, const R* ma, const R* mb, V3<R> mua, V3<R> mub,
                                                 R2<R> tda, R2<R> tdb, bool anisoA, bool anisoB, R ms, R ps,
                                                 const PairParams<R>& P) {
     const R rinv = rsqrt_r(r2);
@@ -249,8 +250,9 @@ __device__ __forceinline__ ElecOut<R> elec_pair(V3<R> dr, R r2, const R* ma, con
     ar[1] = P.alpha * r;
 #pragma unroll
     for (int k = 2; k < 10; k++) ar[k] = ar[k - 1] * ar[1];
-    const R X = R(2.0 / kSqrtPi) * exp_r(-ar[2]);
-    const R b2 = erfc_r(ar[1]) + ar[1] * X;
+    const R eX = exp_r(-ar[2]);
+    const R X = R(2.0 / kSqrtPi) * eX;
+    const R b2 = erfc_r(ar[1], eX) + ar[1] * X;
     const R b3 = b2 + R(2.0 / 3.0) * ar[3] * X;
     const R b4 = b3 + R(4.0 / 15.0) * ar[5] * X;
     const R b5 = b4 + R(8.0 / 105.0) * ar[7] * X;
@@ -258,7 +260,7 @@ __device__ __forceinline__ ElecOut<R> elec_pair(V3<R> dr, R r2, const R* ma, con
     const R mB2 = m1 + b2, mB3 = m1 + b3, mB4 = m1 + b4, mB5 = m1 + b5;
     const R dmp = tda.y * tdb.y;
     const R a = (ps == R(0)) ? tda.x + tdb.x : P.default_thole;
-    const R u = fabs(dmp) > R(1e-5) ? r / dmp : R(1e10);
+    const R u = fabs(dmp) > R(1e-5) ? div_r(r, dmp) : R(1e10);
     // exp(-au) vanishes for au >= 50; the polynomial argument is clamped there so
     // that its powers stay finite in FP32 (u = 1e10 when a damping factor is 0)
     const R au = fmin(a * u, R(50));
