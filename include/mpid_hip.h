@@ -92,11 +92,16 @@ typedef struct {
 } mpid_stats;
 
 /* Per-kernel-family device time of the executes since profiling was enabled
- * (HIP events on the context's stream). */
+ * (HIP events on the context's stream).  MPID_PROF_FFT covers the whole
+ * reciprocal convolution (five passes, the influence function inside the x
+ * pass), so MPID_PROF_CONVOLVE stays zero. */
 enum {
     MPID_PROF_SORT = 0, MPID_PROF_ROTATE, MPID_PROF_TILES, MPID_PROF_SPREAD, MPID_PROF_FFT, MPID_PROF_CONVOLVE,
     MPID_PROF_GATHER, MPID_PROF_FIXED_PAIRS, MPID_PROF_INDUCED_PAIRS, MPID_PROF_ELECTROSTATIC_PAIRS,
-    MPID_PROF_RECIP_ATOMS, MPID_PROF_SOLVER, MPID_PROF_OTHER, MPID_PROF_COUNT
+    MPID_PROF_RECIP_ATOMS, MPID_PROF_SOLVER, MPID_PROF_OTHER,
+    MPID_PROF_COVALENT_PAIRS, /* the scaled covalent pairs (FP64 permanent field and electrostatics) */
+    MPID_PROF_REFINEMENT,     /* mixed precision: the FP64 operator applications of the refinement */
+    MPID_PROF_COUNT
 };
 
 /* CalcMPIDForceKernel::initialize (mpidKernels.h:67).  Validates as

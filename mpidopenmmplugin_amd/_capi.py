@@ -23,7 +23,8 @@ MPID_PRECISION_MIXED = 0
 MPID_PRECISION_DOUBLE = 1
 
 PROFILE_NAMES = ["sort", "rotate", "tiles", "spread", "fft", "convolve", "gather", "fixed_pairs",
-                 "induced_pairs", "electrostatic_pairs", "recip_atoms", "solver", "other"]
+                 "induced_pairs", "electrostatic_pairs", "recip_atoms", "solver", "other", "covalent_pairs",
+                 "refinement"]
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)

@@ -113,14 +113,17 @@ struct TileGeom {
     int T[3];
     int W[3];
 };
-void launch_tile_ranges(hipStream_t s, int n, const unsigned* keys_sorted, int shift, int* tstart, int* tend);
+// Morton-cell ranges of the sorted atoms, or (idx != nullptr) of the subsequence idx[0 .. *count)
+void launch_tile_ranges(hipStream_t s, int n, const unsigned* keys_sorted, const int* idx, const int* count,
+                        int shift, int* tstart, int* tend);
 size_t spread_tiled_lds_bytes(const TileGeom& tg, int double_precision);
 template <class R>
 void launch_spread_fixed_tiled(hipStream_t s, const R4<R>* pos, const R* lab, PmeGeom<R> g, TileGeom tg,
                                const int* tstart, const int* tend, R* scratch, R* grid);
+// atoms != nullptr: the ranges index atoms[] (the polarisable subsequence)
 template <class R>
 void launch_spread_induced_tiled(hipStream_t s, const R4<R>* pos, const R* mu, PmeGeom<R> g, TileGeom tg,
-                                 const int* tstart, const int* tend, R* scratch, R* grid);
+                                 const int* atoms, const int* tstart, const int* tend, R* scratch, R* grid);
 template <class R> void launch_fold(hipStream_t s, PmeGeom<R> g, TileGeom tg, const R* scratch, R* grid);
 template <class R>
 void launch_spread_fixed(hipStream_t s, int n, const R4<R>* pos, const R* lab, PmeGeom<R> g, R* grid);
@@ -129,7 +132,8 @@ void launch_spread_induced(hipStream_t s, int n, const R4<R>* pos, const R* mu, 
 template <class R>
 void launch_gather_phi(hipStream_t s, int n, const R4<R>* pos, PmeGeom<R> g, const R* grid, R* phi);
 template <class R>
-void launch_gather_field(hipStream_t s, int n, const R4<R>* pos, PmeGeom<R> g, const R* grid, R* field);
+void launch_gather_field(hipStream_t s, int natoms, const int* atoms, const R4<R>* pos, PmeGeom<R> g, const R* grid,
+                         R* field);
 // ---- FFT and convolution (kernels_fft.hip) ------------------------------------
 // x / d = (x * m) >> 40 with m = ceil(2^40 / d), exact for x < 2^20.
 struct FastDiv {
@@ -176,18 +180,24 @@ void launch_electrostatics_pairs(hipStream_t s, int nrows, const int* rows, cons
 void launch_block_rows(hipStream_t s, int nb, const float4* bmin, const float4* bmax, Box<float> box, float cutoff,
                        const int* start, int* counts, int* rows, int* owner);
 void launch_exclusive_scan(hipStream_t s, int m, const int* counts, int* start);
+// pos.w of the FP64 positions is the damping factor: nonzero iff the site is polarisable
 void launch_nl_build(hipStream_t s, int n, const int* rstart, const int* rows, const float4* bmin, const float4* bmax,
                      const double4* pos, const int* partners, const int* npart, Box<double> box, double rc2, int cap,
-                     int* nbr, int* count, int* maxcount, unsigned long long* tmask);
+                     int* nbr, int* count, int* maxcount, unsigned long long* tmask, int* pnbr, int* pcount,
+                     int* polcnt);
+void launch_pol_fill(hipStream_t s, int n, const double4* pos, const int* start, int* idx);
+// fields at the atoms atoms[0..natoms) (all atoms when atoms == nullptr, natoms = n)
 template <class R>
-void launch_nl_fixed_field(hipStream_t s, int n, NbrList l, PairAtoms<R> a, PairParams<R> p, acc_t* field);
+void launch_nl_fixed_field(hipStream_t s, int natoms, const int* atoms, NbrList l, PairAtoms<R> a, PairParams<R> p,
+                           acc_t* field);
 template <class R>
-void launch_nl_induced_field(hipStream_t s, int n, NbrList l, PairAtoms<R> a, const R* mu, PairParams<R> p,
-                             acc_t* field);
+void launch_nl_induced_field(hipStream_t s, int natoms, const int* atoms, NbrList l, PairAtoms<R> a, const R* mu,
+                             PairParams<R> p, acc_t* field);
 // the scaled covalent pairs, which the list and tile kernels leave out
 template <class R> void launch_cov_fixed_field(hipStream_t s, int n, PairAtoms<R> a, PairParams<R> p, acc_t* field);
 template <class R>
-void launch_cov_induced_field(hipStream_t s, int n, PairAtoms<R> a, const R* mu, PairParams<R> p, acc_t* field);
+void launch_cov_induced_field(hipStream_t s, int n, PairAtoms<R> a, const R* mu, PairParams<R> p, int pol_only,
+                              acc_t* field);
 template <class R>
 void launch_cov_electrostatics(hipStream_t s, int n, PairAtoms<R> a, const R* mu, PairParams<R> p, acc_t* force,
                                acc_t* torque, acc_t* energy);

@@ -130,10 +130,16 @@ template <class R> __device__ __forceinline__ void stage_twiddles(Cx<R>* tw_s, c
     for (int k = threadIdx.x; k < n; k += blockDim.x) tw_s[k] = Cx<R>{tw[k].x, tw[k].y};
 }
 
+// A batch holds T*n <= 32768 / (2 * sizeof(complex)) elements, at most
+// max_per_thread() per thread of a 256-thread workgroup.  Loads go to
+// registers first, all of them in flight at once, and then to LDS.
+constexpr int kFftThreads = 256;
+template <class R> constexpr int max_per_thread() { return 32768 / (2 * 2 * (int)sizeof(R)) / kFftThreads; }
+
 // Pass 1 (dir = -1): lines along z, real Q -> half spectrum S.
 // Pass 5 (dir = +1): half spectrum S -> real Q (Hermitian completion).
 template <class R>
-__global__ __launch_bounds__(256) void k_fft_z(R* __restrict__ Q, R2<R>* __restrict__ S, FftAxis ax, int T,
+__global__ __launch_bounds__(kFftThreads) void k_fft_z(R* __restrict__ Q, R2<R>* __restrict__ S, FftAxis ax, int T,
                                                const R2<R>* __restrict__ tw, int nlines, FastDiv nh, int dir) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int n = ax.n;
@@ -144,26 +150,37 @@ __global__ __launch_bounds__(256) void k_fft_z(R* __restrict__ Q, R2<R>* __restr
     const int l0 = blockIdx.x * T;
     const int count = min(T, nlines - l0);
     stage_twiddles(W, tw, n);
+    constexpr int MP = max_per_thread<R>();
+    const int total = T * n;
     if (dir < 0) {
         const R* src = Q + (size_t)l0 * n;
-        for (int idx = threadIdx.x; idx < T * n; idx += blockDim.x) {
-            const int t = (int)fdiv((unsigned)idx, ax.nd);
-            A[idx] = Cx<R>{t < count ? src[idx] : R(0), R(0)};
+        const int lim = count * n;
+        R v[MP];
+#pragma unroll
+        for (int u = 0; u < MP; u++) {
+            const int idx = threadIdx.x + u * kFftThreads;
+            v[u] = idx < lim ? src[idx] : R(0);
+        }
+#pragma unroll
+        for (int u = 0; u < MP; u++) {
+            const int idx = threadIdx.x + u * kFftThreads;
+            if (idx < total) A[idx] = Cx<R>{v[u], R(0)};
         }
     } else {
-        for (int idx = threadIdx.x; idx < T * n; idx += blockDim.x) {
+        R2<R> v[MP];
+        bool cj[MP];
+#pragma unroll
+        for (int u = 0; u < MP; u++) {
+            const int idx = threadIdx.x + u * kFftThreads;
             const int t = (int)fdiv((unsigned)idx, ax.nd), k = idx - t * n;
-            Cx<R> v = {R(0), R(0)};
-            if (t < count) {
-                const R2<R>* line = S + (size_t)(l0 + t) * n2h;
-                if (k < n2h) {
-                    v = Cx<R>{line[k].x, line[k].y};
-                } else {
-                    const R2<R> c = line[n - k];
-                    v = Cx<R>{c.x, -c.y};
-                }
-            }
-            A[idx] = v;
+            cj[u] = k >= n2h;
+            v[u] = R2<R>{R(0), R(0)};
+            if (idx < total && t < count) v[u] = S[(size_t)(l0 + t) * n2h + (cj[u] ? n - k : k)];
+        }
+#pragma unroll
+        for (int u = 0; u < MP; u++) {
+            const int idx = threadIdx.x + u * kFftThreads;
+            if (idx < total) A[idx] = Cx<R>{v[u].x, cj[u] ? -v[u].y : v[u].y};
         }
     }
     __syncthreads();
@@ -187,7 +204,7 @@ __global__ __launch_bounds__(256) void k_fft_z(R* __restrict__ Q, R2<R>* __restr
 
 // Passes 2 and 4: lines along y of S, a batch = T consecutive kz of one x.
 template <class R>
-__global__ __launch_bounds__(256) void k_fft_y(R2<R>* __restrict__ S, FftAxis ax, int T, int Tshift,
+__global__ __launch_bounds__(kFftThreads) void k_fft_y(R2<R>* __restrict__ S, FftAxis ax, int T, int Tshift,
                                                const R2<R>* __restrict__ tw, int n2h, int nkb, int dir) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int n = ax.n;
@@ -198,14 +215,22 @@ __global__ __launch_bounds__(256) void k_fft_y(R2<R>* __restrict__ S, FftAxis ax
     const int count = min(T, n2h - kz0);
     R2<R>* base = S + (size_t)x * n * n2h + kz0;
     stage_twiddles(W, tw, n);
-    for (int idx = threadIdx.x; idx < T * n; idx += blockDim.x) {
-        const int k = idx >> Tshift, t = idx & (T - 1);
-        Cx<R> v = {R(0), R(0)};
-        if (t < count) {
-            const R2<R> a = base[(size_t)k * n2h + t];
-            v = Cx<R>{a.x, a.y};
+    constexpr int MP = max_per_thread<R>();
+    const int total = T * n;
+    {
+        R2<R> v[MP];
+#pragma unroll
+        for (int u = 0; u < MP; u++) {
+            const int idx = threadIdx.x + u * kFftThreads;
+            const int k = idx >> Tshift, t = idx & (T - 1);
+            v[u] = (idx < total && t < count) ? base[(size_t)k * n2h + t] : R2<R>{R(0), R(0)};
         }
-        A[t * n + k] = v;
+#pragma unroll
+        for (int u = 0; u < MP; u++) {
+            const int idx = threadIdx.x + u * kFftThreads;
+            const int k = idx >> Tshift, t = idx & (T - 1);
+            if (idx < total) A[t * n + k] = Cx<R>{v[u].x, v[u].y};
+        }
     }
     __syncthreads();
     const Cx<R>* res = fft_in_lds<R>(A, B, ax, T, W, (R)dir);
@@ -221,7 +246,7 @@ __global__ __launch_bounds__(256) void k_fft_y(R2<R>* __restrict__ S, FftAxis ax
 // Pass 3: lines along x of S (a batch = T consecutive kz of one y): forward
 // transform, influence function, backward transform.
 template <class R>
-__global__ __launch_bounds__(256) void k_fft_x_convolve(R2<R>* __restrict__ S, FftAxis ax, int T, int Tshift,
+__global__ __launch_bounds__(kFftThreads) void k_fft_x_convolve(R2<R>* __restrict__ S, FftAxis ax, int T, int Tshift,
                                                         const R2<R>* __restrict__ tw, int n1, int n2, int n2h, int nkb,
                                                         ConvParams<R> cp) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -234,14 +259,22 @@ __global__ __launch_bounds__(256) void k_fft_x_convolve(R2<R>* __restrict__ S, F
     const size_t xs = (size_t)n1 * n2h;
     R2<R>* base = S + (size_t)y * n2h + kz0;
     stage_twiddles(W, tw, n);
-    for (int idx = threadIdx.x; idx < T * n; idx += blockDim.x) {
-        const int k = idx >> Tshift, t = idx & (T - 1);
-        Cx<R> v = {R(0), R(0)};
-        if (t < count) {
-            const R2<R> a = base[(size_t)k * xs + t];
-            v = Cx<R>{a.x, a.y};
+    constexpr int MP = max_per_thread<R>();
+    const int total = T * n;
+    {
+        R2<R> v[MP];
+#pragma unroll
+        for (int u = 0; u < MP; u++) {
+            const int idx = threadIdx.x + u * kFftThreads;
+            const int k = idx >> Tshift, t = idx & (T - 1);
+            v[u] = (idx < total && t < count) ? base[(size_t)k * xs + t] : R2<R>{R(0), R(0)};
         }
-        A[t * n + k] = v;
+#pragma unroll
+        for (int u = 0; u < MP; u++) {
+            const int idx = threadIdx.x + u * kFftThreads;
+            const int k = idx >> Tshift, t = idx & (T - 1);
+            if (idx < total) A[t * n + k] = Cx<R>{v[u].x, v[u].y};
+        }
     }
     __syncthreads();
     Cx<R>* res = fft_in_lds<R>(A, B, ax, T, W, R(-1));
@@ -330,27 +363,27 @@ void launch_recip_convolution(hipStream_t s, const FftPlan<R>& p, R* Q, R2<R>* S
     {
         const int T = lines_per_group<R>(n2);
         const int nlines = n0 * n1;
-        hipLaunchKernelGGL(k_fft_z<R>, dim3((nlines + T - 1) / T), dim3(256), fft_lds<R>(T, n2), s, Q, S, p.ax[2], T,
+        hipLaunchKernelGGL(k_fft_z<R>, dim3((nlines + T - 1) / T), dim3(kFftThreads), fft_lds<R>(T, n2), s, Q, S, p.ax[2], T,
                            p.tw[2], nlines, nh, -1);
     }
     // 2. y forward
     const int Ty = lines_per_group<R>(n1), nkby = (n2h + Ty - 1) / Ty;
-    hipLaunchKernelGGL(k_fft_y<R>, dim3(n0 * nkby), dim3(256), fft_lds<R>(Ty, n1), s, S, p.ax[1], Ty, log2i(Ty),
+    hipLaunchKernelGGL(k_fft_y<R>, dim3(n0 * nkby), dim3(kFftThreads), fft_lds<R>(Ty, n1), s, S, p.ax[1], Ty, log2i(Ty),
                        p.tw[1], n2h, nkby, -1);
     // 3. x forward, convolution, x backward
     {
         const int Tx = lines_per_group<R>(n0), nkbx = (n2h + Tx - 1) / Tx;
-        hipLaunchKernelGGL(k_fft_x_convolve<R>, dim3(n1 * nkbx), dim3(256), fft_lds<R>(Tx, n0), s, S, p.ax[0], Tx,
+        hipLaunchKernelGGL(k_fft_x_convolve<R>, dim3(n1 * nkbx), dim3(kFftThreads), fft_lds<R>(Tx, n0), s, S, p.ax[0], Tx,
                            log2i(Tx), p.tw[0], n1, n2, n2h, nkbx, cp);
     }
     // 4. y backward
-    hipLaunchKernelGGL(k_fft_y<R>, dim3(n0 * nkby), dim3(256), fft_lds<R>(Ty, n1), s, S, p.ax[1], Ty, log2i(Ty),
+    hipLaunchKernelGGL(k_fft_y<R>, dim3(n0 * nkby), dim3(kFftThreads), fft_lds<R>(Ty, n1), s, S, p.ax[1], Ty, log2i(Ty),
                        p.tw[1], n2h, nkby, 1);
     // 5. z backward
     {
         const int T = lines_per_group<R>(n2);
         const int nlines = n0 * n1;
-        hipLaunchKernelGGL(k_fft_z<R>, dim3((nlines + T - 1) / T), dim3(256), fft_lds<R>(T, n2), s, Q, S, p.ax[2], T,
+        hipLaunchKernelGGL(k_fft_z<R>, dim3((nlines + T - 1) / T), dim3(kFftThreads), fft_lds<R>(T, n2), s, Q, S, p.ax[2], T,
                            p.tw[2], nlines, nh, 1);
     }
 }

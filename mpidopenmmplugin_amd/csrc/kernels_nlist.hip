@@ -13,6 +13,10 @@
 //      appends those within the cutoff, skipping itself and its scaled
 //      covalent partners (those pairs go to the covalent-pair kernels).
 // Both steps are deterministic, so the summation order of every field is.
+// A second list keeps only the polarisable neighbours: an atom without
+// polarisability carries no induced dipole (mu = S y with S = 0) and its
+// row of the solver's operator is multiplied by S = 0, so the solver's field
+// passes run over the polarisable atoms and their polarisable neighbours only.
 //
 // List layout: entry k of atom i sits at ((k / S) * n + i) * S + k % S with
 // S = kNlSplit, so the S threads that share an atom read S adjacent words and
@@ -104,7 +108,9 @@ __global__ __launch_bounds__(64) void k_nl_build(int n, const int* __restrict__ 
                                                  const double4* __restrict__ pos, const int* __restrict__ partners,
                                                  const int* __restrict__ npart, Box<double> box, double rc2, int cap,
                                                  int* __restrict__ nbr, int* __restrict__ count,
-                                                 int* __restrict__ maxcount, unsigned long long* __restrict__ tmask) {
+                                                 int* __restrict__ maxcount, unsigned long long* __restrict__ tmask,
+                                                 int* __restrict__ pnbr, int* __restrict__ pcount,
+                                                 int* __restrict__ polcnt) {
     __shared__ double4 sp[kTile];
     const int I = blockIdx.x, lane = threadIdx.x;
     const int i = I * kTile + lane;
@@ -120,9 +126,14 @@ __global__ __launch_bounds__(64) void k_nl_build(int n, const int* __restrict__ 
     const float rcf = sqrtf((float)rc2) * 1.0001f + 1e-5f;
     const Box<float> fbox{{(float)box.L[0], (float)box.L[1], (float)box.L[2]},
                           {(float)box.inv[0], (float)box.inv[1], (float)box.inv[2]}};
-    int cnt = 0;
-    for (int r = rstart[I]; r < rstart[I + 1]; r++) {
+    int cnt = 0, pcnt = 0;
+    const int r0 = rstart[I], r1 = rstart[I + 1];
+    // the next row's atom is loaded one row ahead, so its latency overlaps the current row
+    double4 pnext = pos[r0 < r1 ? min(rows[r0] * kTile + lane, n - 1) : 0];
+    for (int r = r0; r < r1; r++) {
         const int J = rows[r];
+        const double4 pj = pnext;
+        if (r + 1 < r1) pnext = pos[min(rows[r + 1] * kTile + lane, n - 1)];
         __syncthreads();
         // each lane stages one atom of J and tests it against block I's box
         // (minimum image of the distance to the box; never rejects a pair
@@ -130,7 +141,6 @@ __global__ __launch_bounds__(64) void k_nl_build(int n, const int* __restrict__ 
         bool cand;
         {
             const int j = J * kTile + lane;
-            const double4 pj = pos[j < n ? j : n - 1];
             sp[lane] = pj;
             const float d[3] = {(float)pj.x - cI[0], (float)pj.y - cI[1], (float)pj.z - cI[2]};
             float d2 = 0.f;
@@ -164,13 +174,31 @@ __global__ __launch_bounds__(64) void k_nl_build(int n, const int* __restrict__ 
             if (cnt < cap) nbr[nl_index(i, cnt, n)] = J * kTile + jj;
             cnt++;
             tm |= 1ull << jj;
+            if (pj.w != 0.0) {  // j polarisable (damping factor nonzero)
+                if (pcnt < cap) pnbr[nl_index(i, pcnt, n)] = J * kTile + jj;
+                pcnt++;
+            }
         }
         if (J >= I) tmask[(size_t)r * kTile + lane] = tm & upper;
     }
     if (iv) {
         count[i] = min(cnt, cap);
+        pcount[i] = min(pcnt, cap);
         atomicMax(maxcount, cnt);
     }
+    const unsigned long long pol = __ballot(iv && pi.w != 0.0);
+    if (lane == 0) polcnt[I] = __popcll(pol);
+}
+
+// Sorted indices of the polarisable atoms, in order: block I writes its
+// polarisable lanes from start[I].
+__global__ __launch_bounds__(64) void k_pol_fill(int n, const double4* __restrict__ pos, const int* __restrict__ start,
+                                                 int* __restrict__ idx) {
+    const int I = blockIdx.x, lane = threadIdx.x;
+    const int i = I * kTile + lane;
+    const bool p = i < n && pos[i].w != 0.0;
+    const unsigned long long m = __ballot(p);
+    if (p) idx[start[I] + __popcll(m & ((1ull << lane) - 1ull))] = i;
 }
 
 // ---------------------------------------------------------------------------
@@ -179,13 +207,14 @@ __global__ __launch_bounds__(64) void k_nl_build(int n, const int* __restrict__ 
 // added once per atom.
 // ---------------------------------------------------------------------------
 template <class R>
-__global__ __launch_bounds__(256) void k_nl_fixed_field(NbrList L, PairAtoms<R> A, PairParams<R> P,
-                                                        acc_t* __restrict__ field) {
+__global__ __launch_bounds__(256) void k_nl_fixed_field(NbrList L, const int* __restrict__ atoms, int natoms,
+                                                        PairAtoms<R> A, PairParams<R> P, acc_t* __restrict__ field) {
     const unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = (int)(g / kNlSplit), sub = (int)(g % kNlSplit);
+    const int t = (int)(g / kNlSplit), sub = (int)(g % kNlSplit);
     const int n = P.n;
-    const bool iv = i < n;
-    const int ic = iv ? i : n - 1;
+    const bool iv = t < natoms;
+    const int i = atoms ? atoms[iv ? t : natoms - 1] : (iv ? t : n - 1);
+    const int ic = i;
     const R4<R> pi = A.pos[ic];
     const R2<R> tdi = A.td[ic];
     R mi[kMP];
@@ -226,13 +255,15 @@ __global__ __launch_bounds__(256) void k_nl_fixed_field(NbrList L, PairAtoms<R> 
 }
 
 template <class R>
-__global__ __launch_bounds__(256) void k_nl_induced_field(NbrList L, PairAtoms<R> A, const R* __restrict__ mu,
-                                                          PairParams<R> P, acc_t* __restrict__ field) {
+__global__ __launch_bounds__(256) void k_nl_induced_field(NbrList L, const int* __restrict__ atoms, int natoms,
+                                                          PairAtoms<R> A, const R* __restrict__ mu, PairParams<R> P,
+                                                          acc_t* __restrict__ field) {
     const unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = (int)(g / kNlSplit), sub = (int)(g % kNlSplit);
+    const int t = (int)(g / kNlSplit), sub = (int)(g % kNlSplit);
     const int n = P.n;
-    const bool iv = i < n;
-    const int ic = iv ? i : n - 1;
+    const bool iv = t < natoms;
+    const int i = atoms ? atoms[iv ? t : natoms - 1] : (iv ? t : n - 1);
+    const int ic = i;
     const R4<R> pi = A.pos[ic];
     const V3<R> ui = mk(mu[3 * (size_t)ic], mu[3 * (size_t)ic + 1], mu[3 * (size_t)ic + 2]);
     const int cnt = iv ? L.count[ic] : 0;
@@ -280,27 +311,38 @@ void launch_exclusive_scan(hipStream_t s, int m, const int* counts, int* start) 
 }
 void launch_nl_build(hipStream_t s, int n, const int* rstart, const int* rows, const float4* bmin, const float4* bmax,
                      const double4* pos, const int* partners, const int* npart, Box<double> box, double rc2, int cap,
-                     int* nbr, int* count, int* maxcount, unsigned long long* tmask) {
+                     int* nbr, int* count, int* maxcount, unsigned long long* tmask, int* pnbr, int* pcount,
+                     int* polcnt) {
     const int nb = (n + kTile - 1) / kTile;
     hipLaunchKernelGGL(k_nl_build, dim3(nb), dim3(kTile), 0, s, n, rstart, rows, bmin, bmax, pos, partners, npart, box,
-                       rc2, cap, nbr, count, maxcount, tmask);
+                       rc2, cap, nbr, count, maxcount, tmask, pnbr, pcount, polcnt);
+}
+void launch_pol_fill(hipStream_t s, int n, const double4* pos, const int* start, int* idx) {
+    const int nb = (n + kTile - 1) / kTile;
+    hipLaunchKernelGGL(k_pol_fill, dim3(nb), dim3(kTile), 0, s, n, pos, start, idx);
 }
 template <class R>
-void launch_nl_fixed_field(hipStream_t s, int n, NbrList l, PairAtoms<R> a, PairParams<R> p, acc_t* field) {
-    const size_t threads = (size_t)n * kNlSplit;
-    hipLaunchKernelGGL(k_nl_fixed_field<R>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, l, a, p, field);
+void launch_nl_fixed_field(hipStream_t s, int natoms, const int* atoms, NbrList l, PairAtoms<R> a, PairParams<R> p,
+                           acc_t* field) {
+    if (natoms <= 0) return;
+    const size_t threads = (size_t)natoms * kNlSplit;
+    hipLaunchKernelGGL(k_nl_fixed_field<R>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, l, atoms, natoms,
+                       a, p, field);
 }
 template <class R>
-void launch_nl_induced_field(hipStream_t s, int n, NbrList l, PairAtoms<R> a, const R* mu, PairParams<R> p,
-                             acc_t* field) {
-    const size_t threads = (size_t)n * kNlSplit;
-    hipLaunchKernelGGL(k_nl_induced_field<R>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, l, a, mu, p,
-                       field);
+void launch_nl_induced_field(hipStream_t s, int natoms, const int* atoms, NbrList l, PairAtoms<R> a, const R* mu,
+                             PairParams<R> p, acc_t* field) {
+    if (natoms <= 0) return;
+    const size_t threads = (size_t)natoms * kNlSplit;
+    hipLaunchKernelGGL(k_nl_induced_field<R>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, l, atoms,
+                       natoms, a, mu, p, field);
 }
 
 #define MPID_INSTANTIATE(R)                                                                                         \
-    template void launch_nl_fixed_field<R>(hipStream_t, int, NbrList, PairAtoms<R>, PairParams<R>, acc_t*);          \
-    template void launch_nl_induced_field<R>(hipStream_t, int, NbrList, PairAtoms<R>, const R*, PairParams<R>, acc_t*);
+    template void launch_nl_fixed_field<R>(hipStream_t, int, const int*, NbrList, PairAtoms<R>, PairParams<R>,       \
+                                           acc_t*);                                                                \
+    template void launch_nl_induced_field<R>(hipStream_t, int, const int*, NbrList, PairAtoms<R>, const R*,          \
+                                             PairParams<R>, acc_t*);
 MPID_INSTANTIATE(float)
 MPID_INSTANTIATE(double)
 #undef MPID_INSTANTIATE

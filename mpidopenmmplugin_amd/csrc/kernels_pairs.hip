@@ -246,8 +246,9 @@ __global__ __launch_bounds__(256) void k_cov_fixed_field(PairAtoms<R> A, PairPar
 }
 
 template <class R>
+// pol_only: only pairs of two polarisable sites (the solver's operator)
 __global__ __launch_bounds__(256) void k_cov_induced_field(PairAtoms<R> A, const R* __restrict__ mu, PairParams<R> P,
-                                                           acc_t* __restrict__ field) {
+                                                           int pol_only, acc_t* __restrict__ field) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.n) return;
     const int np = A.npart[i];
@@ -255,11 +256,13 @@ __global__ __launch_bounds__(256) void k_cov_induced_field(PairAtoms<R> A, const
     const R4<R> pi = A.pos[i];
     const V3<R> ui = mk(mu[3 * (size_t)i], mu[3 * (size_t)i + 1], mu[3 * (size_t)i + 2]);
     const R2<R> tdi = A.td[i];
+    if (pol_only && tdi.y == R(0)) return;
     R ex = 0, ey = 0, ez = 0;
     for (int k = 0; k < np; k++) {
         const int e = A.partners[(size_t)i * kMaxPartners + k];
         const int j = e >> 2;
         if (j <= i) continue;
+        if (pol_only && A.td[j].y == R(0)) continue;
         const R4<R> pj = A.pos[j];
         const V3<R> dr = min_image(mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z), P.box);
         const R r2 = dot(dr, dr);
@@ -362,8 +365,9 @@ template <class R> void launch_cov_fixed_field(hipStream_t s, int n, PairAtoms<R
     hipLaunchKernelGGL(k_cov_fixed_field<R>, dim3((n + 255) / 256), dim3(256), 0, s, a, p, field);
 }
 template <class R>
-void launch_cov_induced_field(hipStream_t s, int n, PairAtoms<R> a, const R* mu, PairParams<R> p, acc_t* field) {
-    hipLaunchKernelGGL(k_cov_induced_field<R>, dim3((n + 255) / 256), dim3(256), 0, s, a, mu, p, field);
+void launch_cov_induced_field(hipStream_t s, int n, PairAtoms<R> a, const R* mu, PairParams<R> p, int pol_only,
+                              acc_t* field) {
+    hipLaunchKernelGGL(k_cov_induced_field<R>, dim3((n + 255) / 256), dim3(256), 0, s, a, mu, p, pol_only, field);
 }
 template <class R>
 void launch_cov_electrostatics(hipStream_t s, int n, PairAtoms<R> a, const R* mu, PairParams<R> p, acc_t* force,
@@ -378,7 +382,8 @@ void launch_cov_electrostatics(hipStream_t s, int n, PairAtoms<R> a, const R* mu
                                                  const unsigned long long*, PairAtoms<R>, const R*, PairParams<R>,  \
                                                  acc_t*, acc_t*, acc_t*);                                           \
     template void launch_cov_fixed_field<R>(hipStream_t, int, PairAtoms<R>, PairParams<R>, acc_t*);                \
-    template void launch_cov_induced_field<R>(hipStream_t, int, PairAtoms<R>, const R*, PairParams<R>, acc_t*);    \
+    template void launch_cov_induced_field<R>(hipStream_t, int, PairAtoms<R>, const R*, PairParams<R>, int,        \
+                                              acc_t*);                                                            \
     template void launch_cov_electrostatics<R>(hipStream_t, int, PairAtoms<R>, const R*, PairParams<R>, acc_t*,    \
                                                acc_t*, acc_t*);
 MPID_INSTANTIATE(float)

@@ -189,17 +189,23 @@ __device__ __forceinline__ unsigned compact_bits(unsigned v) {
     return v;
 }
 
-__global__ void k_tile_ranges(int n, const unsigned* __restrict__ keys, int shift, int* tstart, int* tend) {
+// Range [tstart[t], tend[t]) of the atoms of Morton cell t.  With idx, over
+// the subsequence idx[0 .. *count) of the sorted atoms (still in Morton order).
+__global__ void k_tile_ranges(int n, const unsigned* __restrict__ keys, const int* __restrict__ idx,
+                              const int* __restrict__ count, int shift, int* tstart, int* tend) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const unsigned t = keys[s] >> shift;
-    if (s == 0 || (keys[s - 1] >> shift) != t) tstart[t] = s;
-    if (s == n - 1 || (keys[s + 1] >> shift) != t) tend[t] = s + 1;
+    const int m = count ? *count : n;
+    if (s >= m) return;
+    auto key = [&](int k) { return keys[idx ? idx[k] : k] >> shift; };
+    const unsigned t = key(s);
+    if (s == 0 || key(s - 1) != t) tstart[t] = s;
+    if (s == m - 1 || key(s + 1) != t) tend[t] = s + 1;
 }
 
 template <class R, int MODE>
 __global__ __launch_bounds__(256) void k_spread_tiled(const R4<R>* __restrict__ pos, const R* __restrict__ src,
-                                                       PmeGeom<R> g, TileGeom tg, const int* __restrict__ tstart,
+                                                       PmeGeom<R> g, TileGeom tg, const int* __restrict__ atoms,
+                                                       const int* __restrict__ tstart,
                                                        const int* __restrict__ tend, R* __restrict__ scratch,
                                                        R* __restrict__ grid) {
     constexpr int ND = MODE == 0 ? 3 : 1;   // spline derivatives needed
@@ -226,7 +232,7 @@ __global__ __launch_bounds__(256) void k_spread_tiled(const R4<R>* __restrict__ 
     for (int a0 = a_begin; a0 < a_end; a0 += CH) {
         const int cnt = min(CH, a_end - a0);
         if ((int)threadIdx.x < cnt) {
-            const int a = a0 + threadIdx.x;
+            const int a = atoms ? atoms[a0 + threadIdx.x] : a0 + threadIdx.x;
             const R4<R> p = pos[a];
             const R px[3] = {p.x, p.y, p.z};
 #pragma unroll
@@ -325,15 +331,24 @@ __global__ __launch_bounds__(256) void k_fold(PmeGeom<R> g, TileGeom tg, const R
             rr[k] = q - cc[k] * tg.T[k];
             nj[k] = min((tg.W[k] - 1 - rr[k]) / tg.T[k] + 1, nt);  // windows c, c-1, ... covering the point
         }
+        // at most 3 windows per dimension cover a point (W <= T + 8, T >= 6);
+        // the loop is unrolled so that all the loads are in flight together
         R sum = grid[id];
-        for (int jx = 0; jx < nj[0]; jx++)
-            for (int jy = 0; jy < nj[1]; jy++)
-                for (int jz = 0; jz < nj[2]; jz++) {
-                    const unsigned tx = (unsigned)((cc[0] - jx + nt) % nt), ty = (unsigned)((cc[1] - jy + nt) % nt),
-                                   tz = (unsigned)((cc[2] - jz + nt) % nt);
-                    const unsigned tile = (spread_bits_pme(tx) << 2) | (spread_bits_pme(ty) << 1) | spread_bits_pme(tz);
-                    const int lx = rr[0] + jx * tg.T[0], ly = rr[1] + jy * tg.T[1], lz = rr[2] + jz * tg.T[2];
-                    sum += scratch[(size_t)tile * WV + (lx * tg.W[1] + ly) * tg.W[2] + lz];
+#pragma unroll
+        for (int jx = 0; jx < 3; jx++)
+#pragma unroll
+            for (int jy = 0; jy < 3; jy++)
+#pragma unroll
+                for (int jz = 0; jz < 3; jz++) {
+                    if (jx < nj[0] && jy < nj[1] && jz < nj[2]) {
+                        const unsigned tx = (unsigned)((cc[0] - jx + nt) % nt),
+                                       ty = (unsigned)((cc[1] - jy + nt) % nt),
+                                       tz = (unsigned)((cc[2] - jz + nt) % nt);
+                        const unsigned tile =
+                            (spread_bits_pme(tx) << 2) | (spread_bits_pme(ty) << 1) | spread_bits_pme(tz);
+                        const int lx = rr[0] + jx * tg.T[0], ly = rr[1] + jy * tg.T[1], lz = rr[2] + jz * tg.T[2];
+                        sum += scratch[(size_t)tile * WV + (lx * tg.W[1] + ly) * tg.W[2] + lz];
+                    }
                 }
         grid[id] = sum;
     }
@@ -342,13 +357,19 @@ __global__ __launch_bounds__(256) void k_fold(PmeGeom<R> g, TileGeom tg, const R
 // ---------------------------------------------------------------------------
 // gathering
 // ---------------------------------------------------------------------------
+// Gathering: 8 lanes per atom, lane iz < 6 sums the 36 points of its z plane
+// of the stencil (all 36 loads issued together), and the planes meet in a
+// lane reduction.
+
 // Fractional potential derivatives phi_abc, a+b+c <= 4, in the reference's
 // order (MPIDReferenceForce.cpp:3498-3532).
 template <class R>
-__global__ __launch_bounds__(128) void k_gather_phi(int n, const R4<R>* __restrict__ pos, PmeGeom<R> g,
+__global__ __launch_bounds__(256) void k_gather_phi(int n, const R4<R>* __restrict__ pos, PmeGeom<R> g,
                                                      const R* __restrict__ grid, R* __restrict__ phi) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = gt >> 3, iz = gt & 7;
+    const bool tv = t < n;
+    const int i = tv ? t : n - 1;
     const R4<R> p = pos[i];
     int ig[3];
     R w[3];
@@ -360,29 +381,37 @@ __global__ __launch_bounds__(128) void k_gather_phi(int n, const R4<R>* __restri
     bspline6<R, 4>(w[1], ty);
     bspline6<R, 4>(w[2], tz);
     const int ny = g.n[1], nz = g.n[2];
-    R s000 = 0, s100 = 0, s010 = 0, s001 = 0, s200 = 0, s020 = 0, s002 = 0, s110 = 0, s101 = 0, s011 = 0;
-    R s300 = 0, s030 = 0, s003 = 0, s210 = 0, s201 = 0, s120 = 0, s021 = 0, s102 = 0, s012 = 0, s111 = 0;
-    R s400 = 0, s040 = 0, s004 = 0, s310 = 0, s301 = 0, s130 = 0, s031 = 0, s103 = 0, s013 = 0, s220 = 0;
-    R s202 = 0, s022 = 0, s211 = 0, s121 = 0, s112 = 0;
-    for (int iz = 0; iz < kOrder; iz++) {
+    R sv[35];
+#pragma unroll
+    for (int k = 0; k < 35; k++) sv[k] = R(0);
+    if (tv && iz < kOrder) {
         int z = ig[2] + iz;
         if (z >= nz) z -= nz;
-        R b00 = 0, b10 = 0, b01 = 0, b20 = 0, b11 = 0, b02 = 0, b30 = 0, b21 = 0, b12 = 0, b03 = 0, b40 = 0, b04 = 0,
-          b31 = 0, b13 = 0, b22 = 0;
+        R q[kOrder][kOrder];
+#pragma unroll
         for (int iy = 0; iy < kOrder; iy++) {
             int y = ig[1] + iy;
             if (y >= ny) y -= ny;
-            R a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
 #pragma unroll
             for (int ix = 0; ix < kOrder; ix++) {
                 int x = ig[0] + ix;
                 if (x >= g.n[0]) x -= g.n[0];
-                const R q = grid[((size_t)x * ny + y) * nz + z];
-                a0 += q * tx[ix][0];
-                a1 += q * tx[ix][1];
-                a2 += q * tx[ix][2];
-                a3 += q * tx[ix][3];
-                a4 += q * tx[ix][4];
+                q[iy][ix] = grid[((size_t)x * ny + y) * nz + z];
+            }
+        }
+        R b00 = 0, b10 = 0, b01 = 0, b20 = 0, b11 = 0, b02 = 0, b30 = 0, b21 = 0, b12 = 0, b03 = 0, b40 = 0, b04 = 0,
+          b31 = 0, b13 = 0, b22 = 0;
+#pragma unroll
+        for (int iy = 0; iy < kOrder; iy++) {
+            R a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
+#pragma unroll
+            for (int ix = 0; ix < kOrder; ix++) {
+                const R v = q[iy][ix];
+                a0 += v * tx[ix][0];
+                a1 += v * tx[ix][1];
+                a2 += v * tx[ix][2];
+                a3 += v * tx[ix][3];
+                a4 += v * tx[ix][4];
             }
             const R u0 = ty[iy][0], u1 = ty[iy][1], u2 = ty[iy][2], u3 = ty[iy][3], u4 = ty[iy][4];
             b00 += a0 * u0; b10 += a1 * u0; b01 += a0 * u1; b20 += a2 * u0; b11 += a1 * u1;
@@ -390,28 +419,38 @@ __global__ __launch_bounds__(128) void k_gather_phi(int n, const R4<R>* __restri
             b40 += a4 * u0; b04 += a0 * u4; b31 += a3 * u1; b13 += a1 * u3; b22 += a2 * u2;
         }
         const R v0 = tz[iz][0], v1 = tz[iz][1], v2 = tz[iz][2], v3 = tz[iz][3], v4 = tz[iz][4];
-        s000 += b00 * v0; s100 += b10 * v0; s010 += b01 * v0; s001 += b00 * v1;
-        s200 += b20 * v0; s020 += b02 * v0; s002 += b00 * v2; s110 += b11 * v0; s101 += b10 * v1; s011 += b01 * v1;
-        s300 += b30 * v0; s030 += b03 * v0; s003 += b00 * v3; s210 += b21 * v0; s201 += b20 * v1;
-        s120 += b12 * v0; s021 += b02 * v1; s102 += b10 * v2; s012 += b01 * v2; s111 += b11 * v1;
-        s400 += b40 * v0; s040 += b04 * v0; s004 += b00 * v4; s310 += b31 * v0; s301 += b30 * v1;
-        s130 += b13 * v0; s031 += b03 * v1; s103 += b10 * v3; s013 += b01 * v3; s220 += b22 * v0;
-        s202 += b20 * v2; s022 += b02 * v2; s211 += b21 * v1; s121 += b12 * v1; s112 += b11 * v2;
+        // order of the reference's 35 derivatives (tuv000 ... tuv112)
+        sv[0] = b00 * v0; sv[1] = b10 * v0; sv[2] = b01 * v0; sv[3] = b00 * v1;
+        sv[4] = b20 * v0; sv[5] = b02 * v0; sv[6] = b00 * v2; sv[7] = b11 * v0; sv[8] = b10 * v1; sv[9] = b01 * v1;
+        sv[10] = b30 * v0; sv[11] = b03 * v0; sv[12] = b00 * v3; sv[13] = b21 * v0; sv[14] = b20 * v1;
+        sv[15] = b12 * v0; sv[16] = b02 * v1; sv[17] = b10 * v2; sv[18] = b01 * v2; sv[19] = b11 * v1;
+        sv[20] = b40 * v0; sv[21] = b04 * v0; sv[22] = b00 * v4; sv[23] = b31 * v0; sv[24] = b30 * v1;
+        sv[25] = b13 * v0; sv[26] = b03 * v1; sv[27] = b10 * v3; sv[28] = b01 * v3; sv[29] = b22 * v0;
+        sv[30] = b20 * v2; sv[31] = b02 * v2; sv[32] = b21 * v1; sv[33] = b12 * v1; sv[34] = b11 * v2;
     }
-    R* o = phi + (size_t)i * 35;
-    o[0] = s000; o[1] = s100; o[2] = s010; o[3] = s001; o[4] = s200; o[5] = s020; o[6] = s002;
-    o[7] = s110; o[8] = s101; o[9] = s011; o[10] = s300; o[11] = s030; o[12] = s003; o[13] = s210;
-    o[14] = s201; o[15] = s120; o[16] = s021; o[17] = s102; o[18] = s012; o[19] = s111; o[20] = s400;
-    o[21] = s040; o[22] = s004; o[23] = s310; o[24] = s301; o[25] = s130; o[26] = s031; o[27] = s103;
-    o[28] = s013; o[29] = s220; o[30] = s202; o[31] = s022; o[32] = s211; o[33] = s121; o[34] = s112;
+#pragma unroll
+    for (int k = 0; k < 35; k++) {
+        sv[k] += __shfl_xor(sv[k], 1);
+        sv[k] += __shfl_xor(sv[k], 2);
+        sv[k] += __shfl_xor(sv[k], 4);
+    }
+    if (tv && iz == 0) {
+        R* o = phi + (size_t)i * 35;
+#pragma unroll
+        for (int k = 0; k < 35; k++) o[k] = sv[k];
+    }
 }
 
 // Field -grad(phi) only (the matrix-vector product of the solver needs no more).
 template <class R>
-__global__ __launch_bounds__(256) void k_gather_field(int n, const R4<R>* __restrict__ pos, PmeGeom<R> g,
+__global__ __launch_bounds__(256) void k_gather_field(int natoms, const int* __restrict__ atoms,
+                                                       const R4<R>* __restrict__ pos, PmeGeom<R> g,
                                                        const R* __restrict__ grid, R* __restrict__ field) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = gt >> 3, iz = gt & 7;
+    const bool tv = t < natoms;
+    const int tc = tv ? t : natoms - 1;
+    const int i = atoms ? atoms[tc] : tc;
     const R4<R> p = pos[i];
     int ig[3];
     R w[3];
@@ -424,33 +463,49 @@ __global__ __launch_bounds__(256) void k_gather_field(int n, const R4<R>* __rest
     bspline6<R, 1>(w[2], tz);
     const int ny = g.n[1], nz = g.n[2];
     R s100 = 0, s010 = 0, s001 = 0;
-    for (int iz = 0; iz < kOrder; iz++) {
+    if (tv && iz < kOrder) {
         int z = ig[2] + iz;
         if (z >= nz) z -= nz;
-        R b00 = 0, b10 = 0, b01 = 0;
+        R q[kOrder][kOrder];
+#pragma unroll
         for (int iy = 0; iy < kOrder; iy++) {
             int y = ig[1] + iy;
             if (y >= ny) y -= ny;
-            R a0 = 0, a1 = 0;
 #pragma unroll
             for (int ix = 0; ix < kOrder; ix++) {
                 int x = ig[0] + ix;
                 if (x >= g.n[0]) x -= g.n[0];
-                const R q = grid[((size_t)x * ny + y) * nz + z];
-                a0 += q * tx[ix][0];
-                a1 += q * tx[ix][1];
+                q[iy][ix] = grid[((size_t)x * ny + y) * nz + z];
+            }
+        }
+        R b00 = 0, b10 = 0, b01 = 0;
+#pragma unroll
+        for (int iy = 0; iy < kOrder; iy++) {
+            R a0 = 0, a1 = 0;
+#pragma unroll
+            for (int ix = 0; ix < kOrder; ix++) {
+                a0 += q[iy][ix] * tx[ix][0];
+                a1 += q[iy][ix] * tx[ix][1];
             }
             b00 += a0 * ty[iy][0];
             b10 += a1 * ty[iy][0];
             b01 += a0 * ty[iy][1];
         }
-        s100 += b10 * tz[iz][0];
-        s010 += b01 * tz[iz][0];
-        s001 += b00 * tz[iz][1];
+        s100 = b10 * tz[iz][0];
+        s010 = b01 * tz[iz][0];
+        s001 = b00 * tz[iz][1];
     }
-    field[3 * (size_t)i + 0] = -g.gscale[0] * s100;
-    field[3 * (size_t)i + 1] = -g.gscale[1] * s010;
-    field[3 * (size_t)i + 2] = -g.gscale[2] * s001;
+#pragma unroll
+    for (int off = 1; off < 8; off <<= 1) {
+        s100 += __shfl_xor(s100, off);
+        s010 += __shfl_xor(s010, off);
+        s001 += __shfl_xor(s001, off);
+    }
+    if (tv && iz == 0) {
+        field[3 * (size_t)i + 0] = -g.gscale[0] * s100;
+        field[3 * (size_t)i + 1] = -g.gscale[1] * s010;
+        field[3 * (size_t)i + 2] = -g.gscale[2] * s001;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -464,8 +519,10 @@ template <class R>
 void launch_spread_induced(hipStream_t s, int n, const R4<R>* pos, const R* mu, PmeGeom<R> g, R* grid) {
     hipLaunchKernelGGL(k_spread_induced<R>, dim3((n + 255) / 256), dim3(256), 0, s, n, pos, mu, g, grid);
 }
-void launch_tile_ranges(hipStream_t s, int n, const unsigned* keys_sorted, int shift, int* tstart, int* tend) {
-    hipLaunchKernelGGL(k_tile_ranges, dim3((n + 255) / 256), dim3(256), 0, s, n, keys_sorted, shift, tstart, tend);
+void launch_tile_ranges(hipStream_t s, int n, const unsigned* keys_sorted, const int* idx, const int* count,
+                        int shift, int* tstart, int* tend) {
+    hipLaunchKernelGGL(k_tile_ranges, dim3((n + 255) / 256), dim3(256), 0, s, n, keys_sorted, idx, count, shift, tstart,
+                       tend);
 }
 
 template <class R, int MODE> static size_t tiled_lds(const TileGeom& tg) {
@@ -482,13 +539,14 @@ template <class R>
 void launch_spread_fixed_tiled(hipStream_t s, const R4<R>* pos, const R* lab, PmeGeom<R> g, TileGeom tg,
                                const int* tstart, const int* tend, R* scratch, R* grid) {
     const size_t shm = tiled_lds<R, 0>(tg);
-    k_spread_tiled<R, 0><<<dim3(tg.ntiles), dim3(256), shm, s>>>(pos, lab, g, tg, tstart, tend, scratch, grid);
+    k_spread_tiled<R, 0><<<dim3(tg.ntiles), dim3(256), shm, s>>>(pos, lab, g, tg, nullptr, tstart, tend, scratch,
+                                                                  grid);
 }
 template <class R>
 void launch_spread_induced_tiled(hipStream_t s, const R4<R>* pos, const R* mu, PmeGeom<R> g, TileGeom tg,
-                                 const int* tstart, const int* tend, R* scratch, R* grid) {
+                                 const int* atoms, const int* tstart, const int* tend, R* scratch, R* grid) {
     const size_t shm = tiled_lds<R, 1>(tg);
-    k_spread_tiled<R, 1><<<dim3(tg.ntiles), dim3(256), shm, s>>>(pos, mu, g, tg, tstart, tend, scratch, grid);
+    k_spread_tiled<R, 1><<<dim3(tg.ntiles), dim3(256), shm, s>>>(pos, mu, g, tg, atoms, tstart, tend, scratch, grid);
 }
 template <class R> void launch_fold(hipStream_t s, PmeGeom<R> g, TileGeom tg, const R* scratch, R* grid) {
     size_t total = (size_t)g.n[0] * g.n[1] * g.n[2];
@@ -498,22 +556,28 @@ template <class R> void launch_fold(hipStream_t s, PmeGeom<R> g, TileGeom tg, co
 
 template <class R>
 void launch_gather_phi(hipStream_t s, int n, const R4<R>* pos, PmeGeom<R> g, const R* grid, R* phi) {
-    hipLaunchKernelGGL(k_gather_phi<R>, dim3((n + 127) / 128), dim3(128), 0, s, n, pos, g, grid, phi);
+    const size_t threads = 8 * (size_t)n;
+    hipLaunchKernelGGL(k_gather_phi<R>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, n, pos, g, grid, phi);
 }
 template <class R>
-void launch_gather_field(hipStream_t s, int n, const R4<R>* pos, PmeGeom<R> g, const R* grid, R* field) {
-    hipLaunchKernelGGL(k_gather_field<R>, dim3((n + 255) / 256), dim3(256), 0, s, n, pos, g, grid, field);
+void launch_gather_field(hipStream_t s, int natoms, const int* atoms, const R4<R>* pos, PmeGeom<R> g, const R* grid,
+                         R* field) {
+    if (natoms <= 0) return;
+    const size_t threads = 8 * (size_t)natoms;
+    hipLaunchKernelGGL(k_gather_field<R>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, natoms, atoms, pos,
+                       g, grid, field);
 }
 
 #define MPID_INSTANTIATE(R)                                                                                   \
     template void launch_spread_fixed<R>(hipStream_t, int, const R4<R>*, const R*, PmeGeom<R>, R*);           \
     template void launch_spread_induced<R>(hipStream_t, int, const R4<R>*, const R*, PmeGeom<R>, R*);         \
     template void launch_gather_phi<R>(hipStream_t, int, const R4<R>*, PmeGeom<R>, const R*, R*);            \
-    template void launch_gather_field<R>(hipStream_t, int, const R4<R>*, PmeGeom<R>, const R*, R*);          \
+    template void launch_gather_field<R>(hipStream_t, int, const int*, const R4<R>*, PmeGeom<R>, const R*,   \
+                                         R*);                                                                \
     template void launch_spread_fixed_tiled<R>(hipStream_t, const R4<R>*, const R*, PmeGeom<R>, TileGeom,    \
                                                const int*, const int*, R*, R*);                              \
     template void launch_spread_induced_tiled<R>(hipStream_t, const R4<R>*, const R*, PmeGeom<R>, TileGeom,  \
-                                                 const int*, const int*, R*, R*);                            \
+                                                 const int*, const int*, const int*, R*, R*);                \
     template void launch_fold<R>(hipStream_t, PmeGeom<R>, TileGeom, const R*, R*);
 MPID_INSTANTIATE(float)
 MPID_INSTANTIATE(double)

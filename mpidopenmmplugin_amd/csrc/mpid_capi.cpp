@@ -137,7 +137,8 @@ enum ProfCat {
     P_SORT = MPID_PROF_SORT, P_ROTATE = MPID_PROF_ROTATE, P_TILES = MPID_PROF_TILES, P_SPREAD = MPID_PROF_SPREAD,
     P_FFT = MPID_PROF_FFT, P_CONV = MPID_PROF_CONVOLVE, P_GATHER = MPID_PROF_GATHER,
     P_FIXED = MPID_PROF_FIXED_PAIRS, P_INDUCED = MPID_PROF_INDUCED_PAIRS, P_ELEC = MPID_PROF_ELECTROSTATIC_PAIRS,
-    P_RECIP = MPID_PROF_RECIP_ATOMS, P_SOLVER = MPID_PROF_SOLVER, P_OTHER = MPID_PROF_OTHER
+    P_RECIP = MPID_PROF_RECIP_ATOMS, P_SOLVER = MPID_PROF_SOLVER, P_OTHER = MPID_PROF_OTHER,
+    P_COV = MPID_PROF_COVALENT_PAIRS, P_REFINE = MPID_PROF_REFINEMENT
 };
 
 // Precision-specific device state.
@@ -184,12 +185,16 @@ struct mpid_context {
     // Morton-aligned PME spreading tiles
     TileGeom tgeom{};
     bool tiled_spread = false;
-    DBuf<int> tstart, tend;
+    DBuf<int> tstart, tend;     // over all sorted atoms
+    DBuf<int> tpstart, tpend;   // over the polarisable atoms (indices into pol_idx)
     // 64-atom blocks: bounding boxes, block rows (CSR: for each block I the
     // blocks J within the cutoff, with their owner I), per-atom neighbour lists
     // of capacity nl_cap, and the tile masks of the row entries with J >= I
     DBuf<float4> bmin, bmax;
     DBuf<int> brow_count, brow_start, brows, brow_owner, nbr, ncount, nl_scalars;
+    // polarisable neighbours only (pnbr/pcount), and the sorted indices of the polarisable atoms
+    DBuf<int> pnbr, pcount, polcnt, polstart, pol_idx;
+    int npol = 0;
     DBuf<unsigned long long> tmask;
     int rows_total = 0;
     int nl_cap = 0;
@@ -230,11 +235,13 @@ struct mpid_context {
         }
         return pool[pool_used++];
     }
+    int prof_override = -1;  // when set, every timed region is booked to this category
     template <class F> void timed(int cat, F&& f) {
         if (!profiling) {
             f();
             return;
         }
+        if (prof_override >= 0) cat = prof_override;
         hipEvent_t a = get_event(), b = get_event();
         HIP_CHECK(hipEventRecord(a, stream));
         f();
@@ -386,6 +393,9 @@ struct mpid_context {
         P.td.resize(n);
         P.mu.resize(3 * (size_t)n);
         P.erec.resize(3 * (size_t)n);
+        // the solver gathers the field at polarisable sites only; the other
+        // entries stay zero (they are multiplied by S = 0, which needs them finite)
+        HIP_CHECK(hipMemsetAsync(P.erec.p, 0, P.erec.n * sizeof(R), stream));
         P.phi.resize(35 * (size_t)n);
         P.phidp.resize(35 * (size_t)n);
         P.grid.resize(G);
@@ -438,6 +448,10 @@ struct mpid_context {
         brow_count.resize(nb);
         brow_start.resize(nb + 1);
         ncount.resize(n);
+        pcount.resize(n);
+        polcnt.resize(nb);
+        polstart.resize(nb + 1);
+        pol_idx.resize(n);
         nl_scalars.resize(1);
         field_acc.resize(n3);
         force_acc.resize(n3);
@@ -477,7 +491,9 @@ struct mpid_context {
             for (int k = 0; k < 3; k++) {
                 tg.T[k] = opt.grid[k] / a;
                 tg.W[k] = std::min(tg.T[k] + 8, opt.grid[k]);
+                if ((tg.W[k] - 1) / tg.T[k] + 1 > 3) ok = false;  // the fold sums at most 3 windows per dimension
             }
+            if (!ok) continue;
             if (spread_tiled_lds_bytes(tg, 1) > 65536) continue;
             tgeom = tg;
             tiled_spread = true;
@@ -486,6 +502,8 @@ struct mpid_context {
         if (tiled_spread) {
             tstart.resize(tgeom.ntiles);
             tend.resize(tgeom.ntiles);
+            tpstart.resize(tgeom.ntiles);
+            tpend.resize(tgeom.ntiles);
         }
     }
 
@@ -573,7 +591,7 @@ struct mpid_context {
             timed(P_SORT, [&] {
                 HIP_CHECK(hipMemsetAsync(tstart.p, 0, tstart.n * sizeof(int), stream));
                 HIP_CHECK(hipMemsetAsync(tend.p, 0, tend.n * sizeof(int), stream));
-                launch_tile_ranges(stream, n, keys_sorted.p, 3 * (10 - tgeom.m), tstart.p, tend.p);
+                launch_tile_ranges(stream, n, keys_sorted.p, nullptr, nullptr, 3 * (10 - tgeom.m), tstart.p, tend.p);
             });
         }
         timed(P_ROTATE, [&] {
@@ -620,13 +638,24 @@ struct mpid_context {
         for (int attempt = 0; attempt < 2; attempt++) {
             nl_cap = (nl_cap + kNlSplit - 1) / kNlSplit * kNlSplit;
             nbr.resize((size_t)n * nl_cap);
+            pnbr.resize((size_t)n * nl_cap);
             timed(P_TILES, [&] {
                 HIP_CHECK(hipMemsetAsync(nl_scalars.p, 0, sizeof(int), stream));
                 launch_nl_build(stream, n, brow_start.p, brows.p, bmin.p, bmax.p, pd.pos.p, spart.p, snpart.p, dbox,
-                                opt.cutoff * opt.cutoff, nl_cap, nbr.p, ncount.p, nl_scalars.p, tmask.p);
+                                opt.cutoff * opt.cutoff, nl_cap, nbr.p, ncount.p, nl_scalars.p, tmask.p, pnbr.p,
+                                pcount.p, polcnt.p);
+                launch_exclusive_scan(stream, nb, polcnt.p, polstart.p);
+                launch_pol_fill(stream, n, pd.pos.p, polstart.p, pol_idx.p);
+                if (tiled_spread) {
+                    HIP_CHECK(hipMemsetAsync(tpstart.p, 0, tpstart.n * sizeof(int), stream));
+                    HIP_CHECK(hipMemsetAsync(tpend.p, 0, tpend.n * sizeof(int), stream));
+                    launch_tile_ranges(stream, n, keys_sorted.p, pol_idx.p, polstart.p + nb, 3 * (10 - tgeom.m),
+                                       tpstart.p, tpend.p);
+                }
             });
             check_launch();
             HIP_CHECK(hipMemcpyAsync(&nl_max, nl_scalars.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+            HIP_CHECK(hipMemcpyAsync(&npol, polstart.p + nb, sizeof(int), hipMemcpyDeviceToHost, stream));
             HIP_CHECK(hipStreamSynchronize(stream));
             if (nl_max <= nl_cap) return;
             nl_cap = nl_max + nl_max / 8 + kNlSplit;
@@ -634,6 +663,7 @@ struct mpid_context {
         throw MpidError(MPID_ERR_HIP, "neighbour list overflow after regrowth");
     }
     NbrList nlist() const { return NbrList{nbr.p, ncount.p}; }
+    NbrList nlist_pol() const { return NbrList{pnbr.p, pcount.p}; }
 
     template <class R> ConvParams<R> conv_params(const Prec<R>& P) const {
         ConvParams<R> c;
@@ -662,14 +692,22 @@ struct mpid_context {
         check_launch();
     }
 
-    template <class R> void recip_cycle_induced(Prec<R>& P, bool full) {
+    // Reciprocal part of the field of the induced dipoles in P.mu.  Unless
+    // all_atoms, mu is known to vanish on unpolarisable sites: only the
+    // polarisable ones are spread, and (full == false) only their field is
+    // gathered.  full: the 35 potential derivatives at every atom (forces).
+    template <class R> void recip_cycle_induced(Prec<R>& P, bool full, bool all_atoms = false) {
         PmeGeom<R> g = geom<R>();
         const size_t G = P.grid.n;
         timed(P_SPREAD, [&] {
             launch_fill<R>(stream, P.grid.p, G, R(0));
             if (tiled_spread) {
-                launch_spread_induced_tiled<R>(stream, P.pos.p, P.mu.p, g, tgeom, tstart.p, tend.p, P.scratch.p,
-                                               P.grid.p);
+                if (all_atoms)
+                    launch_spread_induced_tiled<R>(stream, P.pos.p, P.mu.p, g, tgeom, nullptr, tstart.p, tend.p,
+                                                   P.scratch.p, P.grid.p);
+                else
+                    launch_spread_induced_tiled<R>(stream, P.pos.p, P.mu.p, g, tgeom, pol_idx.p, tpstart.p, tpend.p,
+                                                   P.scratch.p, P.grid.p);
                 launch_fold<R>(stream, g, tgeom, P.scratch.p, P.grid.p);
             } else {
                 launch_spread_induced<R>(stream, n, P.pos.p, P.mu.p, g, P.grid.p);
@@ -678,7 +716,8 @@ struct mpid_context {
         timed(P_FFT, [&] { launch_recip_convolution<R>(stream, P.plan, P.grid.p, P.spec.p, conv_params(P)); });
         timed(P_GATHER, [&] {
             if (full) launch_gather_phi<R>(stream, n, P.pos.p, g, P.grid.p, P.phidp.p);
-            else launch_gather_field<R>(stream, n, P.pos.p, g, P.grid.p, P.erec.p);
+            else if (all_atoms) launch_gather_field<R>(stream, n, nullptr, P.pos.p, g, P.grid.p, P.erec.p);
+            else launch_gather_field<R>(stream, npol, pol_idx.p, P.pos.p, g, P.grid.p, P.erec.p);
         });
         check_launch();
     }
@@ -687,11 +726,16 @@ struct mpid_context {
     // evaluation, always in FP64: each atom's potential contains its own
     // smeared moments, whose energy nearly cancels the self energy, so FP32
     // rounding of that potential would dominate the error of the total.
-    template <class R> void fixed_field(Prec<R>& P) {
+    // Permanent field.  The solver needs it at the polarisable sites only;
+    // all_atoms computes it everywhere (the inspection entry point).
+    template <class R> void fixed_field(Prec<R>& P, bool all_atoms = false) {
         recip_cycle_fixed(pd);
         timed(P_FIXED, [&] {
             launch_zero_acc(stream, field_acc.p, field_acc.n);
-            launch_nl_fixed_field<R>(stream, n, nlist(), pair_atoms(P), pair_params<R>(), field_acc.p);
+            launch_nl_fixed_field<R>(stream, all_atoms ? n : npol, all_atoms ? nullptr : pol_idx.p, nlist(),
+                                     pair_atoms(P), pair_params<R>(), field_acc.p);
+        });
+        timed(P_COV, [&] {
             launch_cov_fixed_field<double>(stream, n, pair_atoms(pd), pair_params<double>(), field_acc.p);
         });
         timed(P_OTHER, [&] {
@@ -700,14 +744,25 @@ struct mpid_context {
         check_launch();
     }
 
-    // field T*mu of the induced dipoles in P.mu: accumulator + P.erec
-    template <class R> void induced_field(Prec<R>& P) {
+    // Field T*mu of the induced dipoles in P.mu: accumulator + P.erec.  In the
+    // solver (all_atoms false) mu vanishes on unpolarisable sites and the field
+    // is only used on polarisable ones, so only pairs of polarisable sites are
+    // visited; all_atoms gives the field of an arbitrary mu everywhere.
+    template <class R> void induced_field(Prec<R>& P, bool all_atoms = false) {
         timed(P_INDUCED, [&] {
             launch_zero_acc(stream, field_acc.p, field_acc.n);
-            launch_nl_induced_field<R>(stream, n, nlist(), pair_atoms(P), P.mu.p, pair_params<R>(), field_acc.p);
-            launch_cov_induced_field<R>(stream, n, pair_atoms(P), P.mu.p, pair_params<R>(), field_acc.p);
+            if (all_atoms)
+                launch_nl_induced_field<R>(stream, n, nullptr, nlist(), pair_atoms(P), P.mu.p, pair_params<R>(),
+                                           field_acc.p);
+            else
+                launch_nl_induced_field<R>(stream, npol, pol_idx.p, nlist_pol(), pair_atoms(P), P.mu.p,
+                                           pair_params<R>(), field_acc.p);
         });
-        recip_cycle_induced(P, false);
+        timed(P_COV, [&] {
+            launch_cov_induced_field<R>(stream, n, pair_atoms(P), P.mu.p, pair_params<R>(), all_atoms ? 0 : 1,
+                                        field_acc.p);
+        });
+        recip_cycle_induced(P, false, all_atoms);
     }
 
     // Av = v - S T S v
@@ -769,7 +824,9 @@ struct mpid_context {
     // residual with the FP64 operator: rt = b - A x; returns its RMS-Debye norm
     double true_residual() {
         const int n3 = 3 * n;
+        prof_override = P_REFINE;
         matvec(pd, x.p, Ap.p);
+        prof_override = -1;
         timed(P_SOLVER, [&] {
             launch_sub(stream, n3, b.p, Ap.p, rt.p);
             launch_residual_norm(stream, n, sqrtpol.p, rt.p, partial.p, scal.p + 4);
@@ -836,6 +893,8 @@ struct mpid_context {
             launch_zero_acc(stream, energy_acc.p, 1);
             launch_electrostatics_pairs<R>(stream, rows_total, brows.p, brow_owner.p, tmask.p, pair_atoms(P), P.mu.p,
                                            pair_params<R>(), force_acc.p, torque_acc.p, energy_acc.p);
+        });
+        timed(P_COV, [&] {
             launch_cov_electrostatics<double>(stream, n, pair_atoms(pd), mud.p, pair_params<double>(), force_acc.p,
                                               torque_acc.p, energy_acc.p);
         });
@@ -926,7 +985,7 @@ struct mpid_context {
         set_box(boxv);
         pos_in.upload(pos, 3 * (size_t)n, stream);
         prepare(pos_in.p);
-        if (opt.precision == MPID_PRECISION_MIXED) fixed_field(pf); else fixed_field(pd);
+        if (opt.precision == MPID_PRECISION_MIXED) fixed_field(pf, true); else fixed_field(pd, true);
         download_unpermuted(E0.p, out);
         have_last = false;
         collect_profile();
@@ -945,7 +1004,7 @@ struct mpid_context {
             for (int s = 0; s < n; s++)
                 for (int k = 0; k < 3; k++) ms[3 * (size_t)s + k] = (R)mu[3 * (size_t)pm[s] + k];
             HIP_CHECK(hipMemcpyAsync(P.mu.p, ms.data(), ms.size() * sizeof(R), hipMemcpyHostToDevice, stream));
-            induced_field(P);
+            induced_field(P, true);
             launch_field_sum<R>(stream, n, field_acc.p, P.erec.p, P.mu.p, selfc(), tmp3n.p);
             check_launch();
             std::vector<double> f(3 * (size_t)n);
