@@ -25,39 +25,63 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# Algorithmic floating-point operations per unordered pair inside the cutoff,
-# as counted by mpidopenmmplugin_amd/csrc/opcount.cpp (make -C
-# mpidopenmmplugin_amd/csrc opcount): (both sites polarisable, so the Thole
-# terms are evaluated; a site without polarisability).
-PAIR_FLOPS = {"fixed_pairs": (283, 257), "induced_pairs": (66, 53), "electrostatic_pairs": (2115, 2113)}
+# Algorithmic floating-point operations of the pair formulas, as counted by
+# mpidopenmmplugin_amd/csrc/opcount.cpp (make -C mpidopenmmplugin_amd/csrc
+# opcount): (both sites polarisable, so the Thole terms are evaluated; a site
+# without polarisability).  The field formulas are one-sided: the radial
+# factors of a pair, and the field one site's moments produce at the other;
+# elec_pair is the quasi-internal-frame interaction without its charge-charge
+# channel, which charge_pair evaluates (in FP64) for every pair.
+PAIR_FLOPS = {"fixed_field_radial": (73, 47), "fixed_field_from": (105, 105), "induced_field_radial": (36, 23),
+              "induced_field_from": (15, 15), "elec_pair": (2106, 2104), "charge_pair": (19, 19)}
+
+
+def pair_kernel_flops(cat, pc):
+    """Algorithmic FLOPs of one launch of a pair-kernel family: the work the
+    result needs, each pair's radial factors once.  pc: pair counts."""
+    F = PAIR_FLOPS
+    if cat == "fixed_pairs":  # permanent field at every polarisable site
+        return (pc["pol"] * (F["fixed_field_radial"][0] + 2 * F["fixed_field_from"][0]) +
+                pc["mixed"] * (F["fixed_field_radial"][1] + F["fixed_field_from"][0]))
+    if cat == "induced_pairs":  # solver operator: pairs of polarisable sites, both sides
+        return pc["pol"] * (F["induced_field_radial"][0] + 2 * F["induced_field_from"][0])
+    if cat == "electrostatic_pairs":  # the charge-charge channel of every pair, the rest where moments exist
+        return (pc["all"] * F["charge_pair"][0] + pc["pol"] * F["elec_pair"][0] +
+                (pc["all"] - pc["pol"] - pc["cc"]) * F["elec_pair"][1])
+    return None
+
+
 MI355X_PEAK = {"hbm_GBps": 8000.0, "fp32_TFLOPs": 157.3, "fp64_TFLOPs": 78.6}
 
 
-def count_pairs(pos, box, cutoff, flat=None):
+def count_pairs(pos, box, cutoff, flat):
     """Unordered pairs within the cutoff that the pair kernels evaluate (the
     scaled covalent partner pairs, which run in the FP64 covalent kernels,
-    excluded): (all such pairs, those with both sites polarisable)."""
+    excluded), split by kind: all, both sites polarisable ("pol"), exactly one
+    ("mixed"), both bare charges without polarisability ("cc")."""
     from scipy.spatial import cKDTree
     L = np.asarray(box, dtype=np.float64)
     p = np.mod(pos, L)
     p = np.minimum(p, np.nextafter(L, 0))
     t = cKDTree(p, boxsize=L)
     pairs = t.query_pairs(cutoff, output_type="ndarray")
-    pol = np.ones(len(p), dtype=bool)
-    if flat is not None:
-        n = len(p)
-        pol = np.asarray(flat["polarity"]).reshape(-1, 3).sum(axis=1) != 0
-        off = np.asarray(flat["cov_offsets"])
-        atoms = np.asarray(flat["cov_atoms"])
-        # Covalent12..14 entries (types 0-2) carry scales other than 1
-        cnt = off[3::4][:n] - off[0::4][:n]
+    n = len(p)
+    pol = np.asarray(flat["polarity"]).reshape(-1, 3).sum(axis=1) != 0
+    moments = np.concatenate([np.asarray(flat[k]).reshape(n, -1) for k in ("dipoles", "quadrupoles", "octopoles")], 1)
+    bare = (~pol) & np.all(moments == 0.0, axis=1)
+    off = np.asarray(flat["cov_offsets"])
+    atoms = np.asarray(flat["cov_atoms"])
+    # Covalent12..14 entries (types 0-2) carry scales other than 1
+    cnt = off[3::4][:n] - off[0::4][:n]
+    if cnt.sum():
         a = np.repeat(np.arange(n), cnt)
-        b = np.concatenate([atoms[off[4 * i]:off[4 * i + 3]] for i in range(n)]) if cnt.sum() else np.zeros(0, int)
+        b = np.concatenate([atoms[off[4 * i]:off[4 * i + 3]] for i in range(n)])
         ckey = np.unique(np.minimum(a, b).astype(np.int64) * n + np.maximum(a, b))
         pkey = np.minimum(pairs[:, 0], pairs[:, 1]).astype(np.int64) * n + np.maximum(pairs[:, 0], pairs[:, 1])
         pairs = pairs[~np.isin(pkey, ckey)]
-    both = pol[pairs[:, 0]] & pol[pairs[:, 1]]
-    return int(len(pairs)), int(both.sum())
+    pi, pj = pairs[:, 0], pairs[:, 1]
+    return {"all": int(len(pairs)), "pol": int((pol[pi] & pol[pj]).sum()), "mixed": int((pol[pi] ^ pol[pj]).sum()),
+            "cc": int((bare[pi] & bare[pj]).sum())}
 
 
 def grid_bytes(cat, cfg, n, launches, mixed):
@@ -181,7 +205,7 @@ def run(args):
 
     # roofline of every kernel family; the dominant one is reported
     flat = force.flatten()
-    npairs, npol = count_pairs(pos, box, cfg["cutoff"], flat)
+    pc = count_pairs(pos, box, cfg["cutoff"], flat)
     mixed = args.precision == "mixed"
     rooflines = {}
     for cat, kv in kernels.items():
@@ -189,9 +213,8 @@ def run(args):
         if ms <= 0 or nl <= 0:
             continue
         avg_ms = ms / nl
-        if cat in PAIR_FLOPS:
-            c1, c0 = PAIR_FLOPS[cat]
-            flops = c1 * npol + c0 * (npairs - npol)  # per launch
+        flops = pair_kernel_flops(cat, pc)  # per launch
+        if flops is not None:
             achieved = flops / (avg_ms * 1e-3) / 1e12
             peak = MI355X_PEAK["fp32_TFLOPs"] if mixed else MI355X_PEAK["fp64_TFLOPs"]
             rooflines[cat] = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
@@ -232,8 +255,7 @@ def run(args):
         "data": "synthetic (seeded water box generator)",
         "config": {"workload": args.workload, "atoms": n, "box_nm": float(box[0]), "pme_grid": cfg["grid"],
                    "cutoff_nm": cfg["cutoff"], "bspline_order": 6, "polarization": "mutual",
-                   "target_epsilon_debye": 1e-6, "precision": args.precision, "pairs_evaluated": npairs,
-                   "pairs_both_polarisable": npol},
+                   "target_epsilon_debye": 1e-6, "precision": args.precision, "pairs": pc},
         "energy_kJ_mol": energy,
         "solver": st,
         "kernels": kernels,

@@ -54,8 +54,10 @@ void launch_gather_sorted(hipStream_t s, int n, const int* perm, const int* inv,
                           const double* mol, const int4* axes, const double* thole, const double* pol,
                           const int* partners, const int* npart, double4* spos, double* smol, int4* saxes,
                           double* sthole, double* spol, int* spartners, int* snpart);
+// chg[s] = 1: the site carries a charge only (no higher moment, no polarisability)
 void launch_rotate(hipStream_t s, int n, Box<double> box, const double4* pos, const int4* axes, const double* mol,
-                   const double* pol, double* lab, double* labpol, double* sqrtpol, double* damp, int* aniso);
+                   const double* pol, double* lab, double* labpol, double* sqrtpol, double* damp, int* aniso,
+                   int* chg);
 template <class R>
 void launch_convert(hipStream_t s, int n, const double4* pos, const double* lab, const double* thole,
                     const double* damp, R4<R>* rpos, R* rlab, R2<R>* rtd);
@@ -172,10 +174,14 @@ void launch_recip_convolution(hipStream_t s, const FftPlan<R>& p, R* Q, R2<R>* S
 template <class R>
 void launch_block_bounds(hipStream_t s, int n, const R4<R>* pos, float4* bmin, float4* bmax);
 // one wavefront per block-row entry r; entries with J < I return at once
+// tmask: pairs with a multipole or a polarisable site; tmaskcc: pairs of two
+// bare charges.  The charge-charge channel of all pairs is evaluated in FP64
+// from pos64 (x, y, z) and the FP64 lab-frame charges lab64[20 i].
 template <class R>
 void launch_electrostatics_pairs(hipStream_t s, int nrows, const int* rows, const int* owner,
-                                 const unsigned long long* tmask, PairAtoms<R> a, const R* mu, PairParams<R> p,
-                                 acc_t* force, acc_t* torque, acc_t* energy);
+                                 const unsigned long long* tmask, const unsigned long long* tmaskcc, PairAtoms<R> a,
+                                 const R* mu, PairParams<R> p, const double4* pos64, const double* lab64,
+                                 PairParams<double> p64, acc_t* force, acc_t* torque, acc_t* energy);
 // ---- neighbour lists (kernels_nlist.hip) ---------------------------------------
 void launch_block_rows(hipStream_t s, int nb, const float4* bmin, const float4* bmax, Box<float> box, float cutoff,
                        const int* start, int* counts, int* rows, int* owner);
@@ -184,7 +190,7 @@ void launch_exclusive_scan(hipStream_t s, int m, const int* counts, int* start);
 void launch_nl_build(hipStream_t s, int n, const int* rstart, const int* rows, const float4* bmin, const float4* bmax,
                      const double4* pos, const int* partners, const int* npart, Box<double> box, double rc2, int cap,
                      int* nbr, int* count, int* maxcount, unsigned long long* tmask, int* pnbr, int* pcount,
-                     int* polcnt);
+                     int* polcnt, const int* chg, unsigned long long* tmaskcc);
 void launch_pol_fill(hipStream_t s, int n, const double4* pos, const int* start, int* idx);
 // fields at the atoms atoms[0..natoms) (all atoms when atoms == nullptr, natoms = n)
 template <class R>

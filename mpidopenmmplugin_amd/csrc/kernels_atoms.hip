@@ -151,12 +151,16 @@ __device__ __forceinline__ int i3(int a, int b, int c) {
 __global__ __launch_bounds__(128) void k_rotate(int n, Box<double> box, const double4* __restrict__ pos,
                                                  const int4* __restrict__ axes, const double* __restrict__ mol,
                                                  const double* __restrict__ pol, double* lab, double* labpol,
-                                                 double* sqrtpol, double* damp, int* aniso) {
+                                                 double* sqrtpol, double* damp, int* aniso, int* chg) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const int4 ax = axes[s];
     double m[kMP];
-    for (int k = 0; k < kMP; k++) m[k] = mol[(size_t)s * kMP + k];
+    bool bare = true;  // charge only: no higher moment, no polarisability
+    for (int k = 0; k < kMP; k++) {
+        m[k] = mol[(size_t)s * kMP + k];
+        if (k > 0 && m[k] != 0.0) bare = false;
+    }
     // chirality (ZThenX with a y atom): flip y-odd dipole/quadrupole components
     if (ax.w == 0 && ax.z >= 0) {
         const D3 py = d3(pos[ax.z]);
@@ -215,6 +219,7 @@ __global__ __launch_bounds__(128) void k_rotate(int n, Box<double> box, const do
     // MPIDForce.h:531 / MPIDReferenceKernels.cpp:134
     damp[s] = pow((p0 + p1 + p2) / 3.0, 1.0 / 6.0);
     aniso[s] = (p0 != p1 || p0 != p2) ? 1 : 0;
+    chg[s] = (bare && p0 == 0.0 && p1 == 0.0 && p2 == 0.0) ? 1 : 0;
 }
 
 template <class R>
@@ -758,9 +763,10 @@ void launch_gather_sorted(hipStream_t s, int n, const int* perm, const int* inv,
                        pol, partners, npart, spos, smol, saxes, sthole, spol, spartners, snpart);
 }
 void launch_rotate(hipStream_t s, int n, Box<double> box, const double4* pos, const int4* axes, const double* mol,
-                   const double* pol, double* lab, double* labpol, double* sqrtpol, double* damp, int* aniso) {
+                   const double* pol, double* lab, double* labpol, double* sqrtpol, double* damp, int* aniso,
+                   int* chg) {
     hipLaunchKernelGGL(k_rotate, blocks_for(n, 128), dim3(128), 0, s, n, box, pos, axes, mol, pol, lab, labpol,
-                       sqrtpol, damp, aniso);
+                       sqrtpol, damp, aniso, chg);
 }
 template <class R>
 void launch_convert(hipStream_t s, int n, const double4* pos, const double* lab, const double* thole,

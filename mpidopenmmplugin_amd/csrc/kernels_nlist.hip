@@ -101,8 +101,9 @@ __global__ __launch_bounds__(1024) void k_exclusive_scan(int m, const int* __res
 // Lane i of block I collects every j within the cutoff (FP64 distances, the
 // test of the reference: r2 > cutoff^2 is out).  Entries beyond cap are
 // counted but not stored; the host grows the list and rebuilds.  For every
-// row entry r with J >= I it also writes the lane's tile mask
-// tmask[r * 64 + lane]: bit jj for each listed j = J * 64 + jj, with j > i.
+// row entry r with J >= I it also writes the lane's tile masks: bit jj for
+// each listed j = J * 64 + jj with j > i, in tmaskcc[r * 64 + lane] when i
+// and j are both bare charges (chg), in tmask[r * 64 + lane] otherwise.
 __global__ __launch_bounds__(64) void k_nl_build(int n, const int* __restrict__ rstart, const int* __restrict__ rows,
                                                  const float4* __restrict__ bmin, const float4* __restrict__ bmax,
                                                  const double4* __restrict__ pos, const int* __restrict__ partners,
@@ -110,7 +111,8 @@ __global__ __launch_bounds__(64) void k_nl_build(int n, const int* __restrict__ 
                                                  int* __restrict__ nbr, int* __restrict__ count,
                                                  int* __restrict__ maxcount, unsigned long long* __restrict__ tmask,
                                                  int* __restrict__ pnbr, int* __restrict__ pcount,
-                                                 int* __restrict__ polcnt) {
+                                                 int* __restrict__ polcnt, const int* __restrict__ chg,
+                                                 unsigned long long* __restrict__ tmaskcc) {
     __shared__ double4 sp[kTile];
     const int I = blockIdx.x, lane = threadIdx.x;
     const int i = I * kTile + lane;
@@ -126,14 +128,22 @@ __global__ __launch_bounds__(64) void k_nl_build(int n, const int* __restrict__ 
     const float rcf = sqrtf((float)rc2) * 1.0001f + 1e-5f;
     const Box<float> fbox{{(float)box.L[0], (float)box.L[1], (float)box.L[2]},
                           {(float)box.inv[0], (float)box.inv[1], (float)box.inv[2]}};
+    const bool ibare = iv && chg[ic] != 0;
     int cnt = 0, pcnt = 0;
     const int r0 = rstart[I], r1 = rstart[I + 1];
     // the next row's atom is loaded one row ahead, so its latency overlaps the current row
-    double4 pnext = pos[r0 < r1 ? min(rows[r0] * kTile + lane, n - 1) : 0];
+    int jnext = r0 < r1 ? min(rows[r0] * kTile + lane, n - 1) : 0;
+    double4 pnext = pos[jnext];
+    int cnext = chg[jnext];
     for (int r = r0; r < r1; r++) {
         const int J = rows[r];
         const double4 pj = pnext;
-        if (r + 1 < r1) pnext = pos[min(rows[r + 1] * kTile + lane, n - 1)];
+        const unsigned long long jbare = __ballot(cnext != 0 && J * kTile + lane < n);
+        if (r + 1 < r1) {
+            jnext = min(rows[r + 1] * kTile + lane, n - 1);
+            pnext = pos[jnext];
+            cnext = chg[jnext];
+        }
         __syncthreads();
         // each lane stages one atom of J and tests it against block I's box
         // (minimum image of the distance to the box; never rejects a pair
@@ -179,7 +189,12 @@ __global__ __launch_bounds__(64) void k_nl_build(int n, const int* __restrict__ 
                 pcnt++;
             }
         }
-        if (J >= I) tmask[(size_t)r * kTile + lane] = tm & upper;
+        if (J >= I) {
+            // pairs of two bare charges go to their own list (their interaction is q q erfc(a r) / r)
+            const unsigned long long cc = ibare ? (tm & jbare) : 0ull;
+            tmask[(size_t)r * kTile + lane] = (tm & ~cc) & upper;
+            tmaskcc[(size_t)r * kTile + lane] = cc & upper;
+        }
     }
     if (iv) {
         count[i] = min(cnt, cap);
@@ -235,8 +250,9 @@ __global__ __launch_bounds__(256) void k_nl_fixed_field(NbrList L, const int* __
         R mj[kMP];
 #pragma unroll
         for (int c = 0; c < kMP; c++) mj[c] = A.lab[(size_t)j * kMP + c];
-        V3<R> fi, fj;
-        fixed_field_pair<R>(dr, dot(dr, dr), mi, mj, tdi, R2<R>{R(0), pj.w}, R(1), P, fi, fj);
+        // listed pairs are not covalent partners: scale 1, default Thole width
+        const FixedRadial<R> gr = fixed_field_radial<R>(dot(dr, dr), tdi, R2<R>{R(0), pj.w}, R(1), P);
+        const V3<R> fi = fixed_field_from<R>(mj, dr, gr);
         ex += fi.x;
         ey += fi.y;
         ez += fi.z;
@@ -278,9 +294,9 @@ __global__ __launch_bounds__(256) void k_nl_induced_field(NbrList L, const int* 
         const R4<R> pj = A.pos[j];
         const V3<R> dr = min_image(mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z), P.box);
         const V3<R> uj = mk(mu[3 * (size_t)j], mu[3 * (size_t)j + 1], mu[3 * (size_t)j + 2]);
-        V3<R> fi, fj;
         // listed pairs are not covalent partners: polarization scale 1, default Thole width
-        induced_field_pair<R>(dr, dot(dr, dr), ui, uj, pi.w, pj.w, P.default_thole, P, fi, fj);
+        const V3<R> fi =
+            induced_field_from<R>(uj, dr, induced_field_radial<R>(dot(dr, dr), pi.w, pj.w, P.default_thole, P));
         ex += fi.x;
         ey += fi.y;
         ez += fi.z;
@@ -312,10 +328,10 @@ void launch_exclusive_scan(hipStream_t s, int m, const int* counts, int* start) 
 void launch_nl_build(hipStream_t s, int n, const int* rstart, const int* rows, const float4* bmin, const float4* bmax,
                      const double4* pos, const int* partners, const int* npart, Box<double> box, double rc2, int cap,
                      int* nbr, int* count, int* maxcount, unsigned long long* tmask, int* pnbr, int* pcount,
-                     int* polcnt) {
+                     int* polcnt, const int* chg, unsigned long long* tmaskcc) {
     const int nb = (n + kTile - 1) / kTile;
     hipLaunchKernelGGL(k_nl_build, dim3(nb), dim3(kTile), 0, s, n, rstart, rows, bmin, bmax, pos, partners, npart, box,
-                       rc2, cap, nbr, count, maxcount, tmask, pnbr, pcount, polcnt);
+                       rc2, cap, nbr, count, maxcount, tmask, pnbr, pcount, polcnt, chg, tmaskcc);
 }
 void launch_pol_fill(hipStream_t s, int n, const double4* pos, const int* start, int* idx) {
     const int nb = (n + kTile - 1) / kTile;

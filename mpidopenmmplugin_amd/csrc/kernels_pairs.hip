@@ -80,14 +80,18 @@ template <class R>
 __global__ __launch_bounds__(kElecThreads) void k_electrostatics(const int* __restrict__ rows,
                                                                   const int* __restrict__ owner,
                                                                   const unsigned long long* __restrict__ tmask,
+                                                                  const unsigned long long* __restrict__ tmaskcc,
                                                                   PairAtoms<R> A, const R* __restrict__ mu,
-                                                                  PairParams<R> P, acc_t* __restrict__ force,
+                                                                  PairParams<R> P, const double4* __restrict__ pos64,
+                                                                  const double* __restrict__ lab64,
+                                                                  PairParams<double> P64, acc_t* __restrict__ force,
                                                                   acc_t* __restrict__ torque,
                                                                   acc_t* __restrict__ energy) {
     constexpr int kS = kMP + 1;  // odd LDS row stride: rows of different atoms start in different banks
-    __shared__ unsigned long long smask[kTile];
-    __shared__ int soff[kTile];
-    __shared__ int stotal;
+    // [0]: the general pairs, [1]: pairs of two bare charges
+    __shared__ unsigned long long smask[2][kTile];
+    __shared__ int soff[2][kTile];
+    __shared__ int stotal[2];
     // [0]: atoms of block I, [1]: atoms of block J; FP64 in both precisions
     // (LDS atomic adds of doubles keep up on gfx950, those of floats stall)
     __shared__ double acc[2][kTile][6];
@@ -95,13 +99,14 @@ __global__ __launch_bounds__(kElecThreads) void k_electrostatics(const int* __re
     __shared__ R smp[2][kTile * kS];
     __shared__ R smu[2][kTile * 3];
     __shared__ int san[2][kTile];
+    __shared__ double4 spos64[2][kTile];  // x, y, z, charge in FP64 for the charge-charge channel
     __shared__ double sen[kElecThreads / kTile];
     const int r = blockIdx.x, tid = threadIdx.x, lane = tid & (kTile - 1), wave = tid / kTile;
     const int I = owner[r], J = rows[r];
     if (J < I) return;  // each block pair once (uniform over the workgroup)
-    if (wave == 0) {
+    if (wave < 2) {
         // rank the tile's pairs: prefix sum of the lanes' mask popcounts
-        const unsigned long long m = tmask[(size_t)r * kTile + lane];
+        const unsigned long long m = (wave == 0 ? tmask : tmaskcc)[(size_t)r * kTile + lane];
         const int c = __popcll(m);
         int incl = c;
 #pragma unroll
@@ -109,13 +114,13 @@ __global__ __launch_bounds__(kElecThreads) void k_electrostatics(const int* __re
             const int v = __shfl_up(incl, off);
             if (lane >= off) incl += v;
         }
-        smask[lane] = m;
-        soff[lane] = incl - c;
-        if (lane == kTile - 1) stotal = incl;
+        smask[wave][lane] = m;
+        soff[wave][lane] = incl - c;
+        if (lane == kTile - 1) stotal[wave] = incl;
     }
     __syncthreads();
-    const int total = stotal;
-    if (total == 0) return;
+    const int total = stotal[0], total_cc = stotal[1];
+    if (total == 0 && total_cc == 0) return;
     const int n = P.n;
     // stage both blocks in LDS (thread tid: side tid / 128, atom tid % 64 for the per-atom fields)
     if (tid < 2 * kTile) {
@@ -126,6 +131,8 @@ __global__ __launch_bounds__(kElecThreads) void k_electrostatics(const int* __re
         smu[side][3 * lane + 1] = mu[3 * (size_t)a + 1];
         smu[side][3 * lane + 2] = mu[3 * (size_t)a + 2];
         san[side][lane] = A.aniso[a];
+        const double4 p64 = pos64[a];
+        spos64[side][lane] = make_double4(p64.x, p64.y, p64.z, lab64[(size_t)a * kMP]);
 #pragma unroll
         for (int k = 0; k < 6; k++) acc[side][lane][k] = 0.0;
     }
@@ -142,13 +149,39 @@ __global__ __launch_bounds__(kElecThreads) void k_electrostatics(const int* __re
     }
     __syncthreads();
     double en = 0.0;
+    // The charge-charge channel of every pair, in FP64 in both precision
+    // modes: it carries most of the magnitude of the real-space energy, and
+    // FP32 rounding of the charges, alpha and the Coulomb constant would bias
+    // all its terms alike.  For pairs of two bare charges it is the whole
+    // interaction.
+    for (int q = tid; q < total_cc + total; q += kElecThreads) {
+        const int list = q < total_cc ? 1 : 0, qq = q < total_cc ? q : q - total_cc;
+        int il = 0;
+#pragma unroll
+        for (int step = 32; step >= 1; step >>= 1)
+            if (soff[list][il + step] <= qq) il += step;
+        const int jj = select_bit(smask[list][il], qq - soff[list][il]);
+        const double4 pi = spos64[0][il], pj = spos64[1][jj];
+        const V3<double> dr = min_image(mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z), P64.box);
+        V3<double> f;
+        double e;
+        charge_pair<double>(dr, dot(dr, dr), pi.w, pj.w, P64, f, e);
+        en += e;
+        atomicAdd(&acc[0][il][0], -f.x);
+        atomicAdd(&acc[0][il][1], -f.y);
+        atomicAdd(&acc[0][il][2], -f.z);
+        atomicAdd(&acc[1][jj][0], f.x);
+        atomicAdd(&acc[1][jj][1], f.y);
+        atomicAdd(&acc[1][jj][2], f.z);
+    }
+    // the rest of the general pairs' interaction, in the working precision
     for (int q = tid; q < total; q += kElecThreads) {
         // owning lane: the last il with soff[il] <= q
         int il = 0;
 #pragma unroll
         for (int step = 32; step >= 1; step >>= 1)
-            if (soff[il + step] <= q) il += step;
-        const int jj = select_bit(smask[il], q - soff[il]);
+            if (soff[0][il + step] <= q) il += step;
+        const int jj = select_bit(smask[0][il], q - soff[0][il]);
         const R4<R> pi = spos[0][il], pj = spos[1][jj];
         const V3<R> dr = min_image(mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z), P.box);
         R mi[kMP], mj[kMP];
@@ -160,7 +193,7 @@ __global__ __launch_bounds__(kElecThreads) void k_electrostatics(const int* __re
         const V3<R> ui = mk(smu[0][3 * il], smu[0][3 * il + 1], smu[0][3 * il + 2]);
         const V3<R> uj = mk(smu[1][3 * jj], smu[1][3 * jj + 1], smu[1][3 * jj + 2]);
         const ElecOut<R> o = elec_pair<R>(dr, dot(dr, dr), mi, mj, ui, uj, R2<R>{R(0), pi.w}, R2<R>{R(0), pj.w},
-                                          san[0][il] != 0, san[1][jj] != 0, R(1), R(1), P);
+                                          san[0][il] != 0, san[1][jj] != 0, R(1), R(1), P, false);
         en += (double)o.energy;
         atomicAdd(&acc[0][il][0], -(double)o.f.x);
         atomicAdd(&acc[0][il][1], -(double)o.f.y);
@@ -354,11 +387,12 @@ void launch_block_bounds(hipStream_t s, int n, const R4<R>* pos, float4* bmin, f
 
 template <class R>
 void launch_electrostatics_pairs(hipStream_t s, int nrows, const int* rows, const int* owner,
-                                 const unsigned long long* tmask, PairAtoms<R> a, const R* mu, PairParams<R> p,
-                                 acc_t* force, acc_t* torque, acc_t* energy) {
+                                 const unsigned long long* tmask, const unsigned long long* tmaskcc, PairAtoms<R> a,
+                                 const R* mu, PairParams<R> p, const double4* pos64, const double* lab64,
+                                 PairParams<double> p64, acc_t* force, acc_t* torque, acc_t* energy) {
     if (nrows <= 0) return;
-    hipLaunchKernelGGL(k_electrostatics<R>, dim3(nrows), dim3(kElecThreads), 0, s, rows, owner, tmask, a, mu, p, force,
-                       torque, energy);
+    hipLaunchKernelGGL(k_electrostatics<R>, dim3(nrows), dim3(kElecThreads), 0, s, rows, owner, tmask, tmaskcc, a, mu,
+                       p, pos64, lab64, p64, force, torque, energy);
 }
 
 template <class R> void launch_cov_fixed_field(hipStream_t s, int n, PairAtoms<R> a, PairParams<R> p, acc_t* field) {
@@ -379,8 +413,9 @@ void launch_cov_electrostatics(hipStream_t s, int n, PairAtoms<R> a, const R* mu
 #define MPID_INSTANTIATE(R)                                                                                        \
     template void launch_block_bounds<R>(hipStream_t, int, const R4<R>*, float4*, float4*);                         \
     template void launch_electrostatics_pairs<R>(hipStream_t, int, const int*, const int*,                         \
-                                                 const unsigned long long*, PairAtoms<R>, const R*, PairParams<R>,  \
-                                                 acc_t*, acc_t*, acc_t*);                                           \
+                                                 const unsigned long long*, const unsigned long long*, PairAtoms<R>, \
+                                                 const R*, PairParams<R>, const double4*, const double*,            \
+                                                 PairParams<double>, acc_t*, acc_t*, acc_t*);                       \
     template void launch_cov_fixed_field<R>(hipStream_t, int, PairAtoms<R>, PairParams<R>, acc_t*);                \
     template void launch_cov_induced_field<R>(hipStream_t, int, PairAtoms<R>, const R*, PairParams<R>, int,        \
                                               acc_t*);                                                            \

@@ -181,7 +181,7 @@ struct mpid_context {
     DBuf<double4> spos;
     DBuf<double> smol, sthole, spol, lab, labpol, sqrtpol, damp;
     DBuf<int4> saxes;
-    DBuf<int> spart, snpart, aniso;
+    DBuf<int> spart, snpart, aniso, chg;
     // Morton-aligned PME spreading tiles
     TileGeom tgeom{};
     bool tiled_spread = false;
@@ -195,7 +195,7 @@ struct mpid_context {
     // polarisable neighbours only (pnbr/pcount), and the sorted indices of the polarisable atoms
     DBuf<int> pnbr, pcount, polcnt, polstart, pol_idx;
     int npol = 0;
-    DBuf<unsigned long long> tmask;
+    DBuf<unsigned long long> tmask, tmaskcc;
     int rows_total = 0;
     int nl_cap = 0;
     int nl_max = 0;
@@ -442,6 +442,7 @@ struct mpid_context {
         sqrtpol.resize(6 * (size_t)n);
         damp.resize(n);
         aniso.resize(n);
+        chg.resize(n);
         const int nb = (n + kTile - 1) / kTile;
         bmin.resize(nb);
         bmax.resize(nb);
@@ -596,7 +597,7 @@ struct mpid_context {
         }
         timed(P_ROTATE, [&] {
             launch_rotate(stream, n, dbox, spos.p, saxes.p, smol.p, spol.p, lab.p, labpol.p, sqrtpol.p, damp.p,
-                          aniso.p);
+                          aniso.p, chg.p);
             if (opt.precision == MPID_PRECISION_MIXED)
                 launch_convert<float>(stream, n, spos.p, lab.p, sthole.p, damp.p, pf.pos.p, pf.lab.p, pf.td.p);
             launch_convert<double>(stream, n, spos.p, lab.p, sthole.p, damp.p, pd.pos.p, pd.lab.p, pd.td.p);
@@ -618,6 +619,7 @@ struct mpid_context {
             brows.resize(cap);
             brow_owner.resize(cap);
             tmask.resize(cap * kTile);
+            tmaskcc.resize(cap * kTile);
         }
         stats.ntiles = (rows_total + nb) / 2;  // block pairs with J >= I
         build_neighbour_lists(nb);
@@ -643,7 +645,7 @@ struct mpid_context {
                 HIP_CHECK(hipMemsetAsync(nl_scalars.p, 0, sizeof(int), stream));
                 launch_nl_build(stream, n, brow_start.p, brows.p, bmin.p, bmax.p, pd.pos.p, spart.p, snpart.p, dbox,
                                 opt.cutoff * opt.cutoff, nl_cap, nbr.p, ncount.p, nl_scalars.p, tmask.p, pnbr.p,
-                                pcount.p, polcnt.p);
+                                pcount.p, polcnt.p, chg.p, tmaskcc.p);
                 launch_exclusive_scan(stream, nb, polcnt.p, polstart.p);
                 launch_pol_fill(stream, n, pd.pos.p, polstart.p, pol_idx.p);
                 if (tiled_spread) {
@@ -891,8 +893,9 @@ struct mpid_context {
             launch_zero_acc(stream, force_acc.p, force_acc.n);
             launch_zero_acc(stream, torque_acc.p, torque_acc.n);
             launch_zero_acc(stream, energy_acc.p, 1);
-            launch_electrostatics_pairs<R>(stream, rows_total, brows.p, brow_owner.p, tmask.p, pair_atoms(P), P.mu.p,
-                                           pair_params<R>(), force_acc.p, torque_acc.p, energy_acc.p);
+            launch_electrostatics_pairs<R>(stream, rows_total, brows.p, brow_owner.p, tmask.p, tmaskcc.p,
+                                           pair_atoms(P), P.mu.p, pair_params<R>(), pd.pos.p, pd.lab.p,
+                                           pair_params<double>(), force_acc.p, torque_acc.p, energy_acc.p);
         });
         timed(P_COV, [&] {
             launch_cov_electrostatics<double>(stream, n, pair_atoms(pd), mud.p, pair_params<double>(), force_acc.p,

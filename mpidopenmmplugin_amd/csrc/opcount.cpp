@@ -5,11 +5,13 @@
 // arithmetic operation it takes part in: +, -, * and / count one each (so a
 // fused multiply-add counts two), exp, erfc and reciprocal square root one
 // each; sign changes, comparisons and selects count nothing.  Each function
-// is evaluated for one unordered pair in both Thole branches (both sites
-// polarisable, so the damping terms are evaluated; or a damping factor of
-// zero), which are the two costs the roofline weights by the pair mix.
+// is evaluated for one pair in both Thole branches (both sites polarisable,
+// so the damping terms are evaluated; or a damping factor of zero), which
+// are the two costs the roofline weights by the pair mix.
 //
 // Prints one line per formula:  name  flops_damped  flops_undamped
+// (the field formulas one-sided: radial factors of the pair, and the field
+// that one site's moments produce at the other)
 // Build and run:  make -C mpidopenmmplugin_amd/csrc opcount
 #include <cmath>
 #include <cstdio>
@@ -95,24 +97,40 @@ int main() {
     }
     const V3<Cnt> ua = mk(Cnt(0.004), Cnt(-0.002), Cnt(0.003)), ub = mk(Cnt(-0.001), Cnt(0.005), Cnt(0.002));
     const PairParams<Cnt> P = params(true);
-    long fixed[2], induced[2], elec[2];
+    // one-sided pieces (radial factors, field from one site) and the
+    // two-sided electrostatics; radial factors with and without damping
+    long frad[2], ffrom = 0, irad[2], ifrom = 0, elec[2];
     for (int d = 0; d < 2; d++) {
         const double damp = d == 0 ? 0.316 : 0.0;  // d = 0: both sites polarisable
         R2<Cnt> ta{Cnt(1.3), Cnt(damp)}, tb{Cnt(1.3), Cnt(0.316)};
-        V3<Cnt> fi, fj;
         Ops::n = 0;
-        fixed_field_pair<Cnt>(dr, r2, ma, mb, ta, tb, Cnt(1.0), P, fi, fj);
-        fixed[d] = Ops::n;
+        const FixedRadial<Cnt> g = fixed_field_radial<Cnt>(r2, ta, tb, Cnt(1.0), P);
+        frad[d] = Ops::n;
         Ops::n = 0;
-        induced_field_pair<Cnt>(dr, r2, ua, ub, ta.y, tb.y, P.default_thole, P, fi, fj);
-        induced[d] = Ops::n;
+        (void)fixed_field_from<Cnt>(mb, dr, g);
+        ffrom = Ops::n;
         Ops::n = 0;
-        ElecOut<Cnt> o = elec_pair<Cnt>(dr, r2, ma, mb, ua, ub, ta, tb, true, true, Cnt(1.0), Cnt(1.0), P);
+        const InducedRadial<Cnt> f = induced_field_radial<Cnt>(r2, ta.y, tb.y, P.default_thole, P);
+        irad[d] = Ops::n;
+        Ops::n = 0;
+        (void)induced_field_from<Cnt>(ub, dr, f);
+        ifrom = Ops::n;
+        Ops::n = 0;
+        // the charge-charge channel is counted apart (charge_pair)
+        ElecOut<Cnt> o = elec_pair<Cnt>(dr, r2, ma, mb, ua, ub, ta, tb, true, true, Cnt(1.0), Cnt(1.0), P, false);
         (void)o;
         elec[d] = Ops::n;
     }
-    std::printf("fixed_field_pair %ld %ld\n", fixed[0], fixed[1]);
-    std::printf("induced_field_pair %ld %ld\n", induced[0], induced[1]);
+    V3<Cnt> fc;
+    Cnt ec;
+    Ops::n = 0;
+    charge_pair<Cnt>(dr, r2, Cnt(0.4), Cnt(0.4), P, fc, ec);
+    const long cc = Ops::n;
+    std::printf("fixed_field_radial %ld %ld\n", frad[0], frad[1]);
+    std::printf("fixed_field_from %ld %ld\n", ffrom, ffrom);
+    std::printf("induced_field_radial %ld %ld\n", irad[0], irad[1]);
+    std::printf("induced_field_from %ld %ld\n", ifrom, ifrom);
     std::printf("elec_pair %ld %ld\n", elec[0], elec[1]);
+    std::printf("charge_pair %ld %ld\n", cc, cc);
     return 0;
 }

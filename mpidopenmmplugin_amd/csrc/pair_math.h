@@ -41,12 +41,15 @@ template <class R> MPID_HD R quad_form(const R* M, V3<R> a, V3<R> b) {
 // ---------------------------------------------------------------------------
 // fixed multipole field
 // ---------------------------------------------------------------------------
-// Field at i from the permanent moments of j (fi) and at j from i (fj), with
-// dr = r_j - r_i inside the cutoff and ds the multipole/polarization scale of
-// the pair (MPIDReferenceForce.cpp:2812-2920).
+// Radial factors of the permanent field of a pair at distance r^2 = r2: the
+// Ewald terms bn1..bn4 minus the Thole-damped bare terms (1 - ds lambda_k)
+// (2k-1)!! / r^(2k+1), ds the multipole/polarization scale of the pair
+// (MPIDReferenceForce.cpp:2812-2920, getDampedInverseDistances :2678-2718).
+template <class R> struct FixedRadial {
+    R g1, g2, g3, g4;
+};
 template <class R>
-MPID_HD void fixed_field_pair(V3<R> dr, R r2, const R* mi, const R* mj, R2<R> tdi, R2<R> tdj,
-                                                 R ds, const PairParams<R>& P, V3<R>& fi, V3<R>& fj) {
+MPID_HD FixedRadial<R> fixed_field_radial(R r2, R2<R> tdi, R2<R> tdj, R ds, const PairParams<R>& P) {
     const R rinv = rsqrt_r(r2);
     const R r = r2 * rinv;
     const R rinv2 = rinv * rinv;
@@ -57,7 +60,6 @@ MPID_HD void fixed_field_pair(V3<R> dr, R r2, const R* mi, const R* mj, R2<R> td
     const R bn2 = (R(3) * bn1 + P.aln[2] * ex2) * rinv2;
     const R bn3 = (R(5) * bn2 + P.aln[3] * ex2) * rinv2;
     const R bn4 = (R(7) * bn3 + P.aln[4] * ex2) * rinv2;
-    // Thole-damped bare terms: (1 - ds*lambda_k) * (2k-1)!! / r^(2k+1)
     R l3 = R(1), l5 = R(1), l7 = R(1), l9 = R(1);
     const R damp = tdi.y * tdj.y;
     if (damp != R(0)) {
@@ -76,25 +78,34 @@ MPID_HD void fixed_field_pair(V3<R> dr, R r2, const R* mi, const R* mj, R2<R> td
         }
     }
     const R rinv3 = rinv * rinv2, rinv5 = rinv3 * rinv2, rinv7 = rinv5 * rinv2, rinv9 = rinv7 * rinv2;
-    const R g1 = bn1 - (R(1) - ds * l3) * rinv3;
-    const R g2 = bn2 - R(3) * (R(1) - ds * l5) * rinv5;
-    const R g3 = bn3 - R(15) * (R(1) - ds * l7) * rinv7;
-    const R g4 = bn4 - R(105) * (R(1) - ds * l9) * rinv9;
-    // site j moments contracted with r
-    const V3<R> dj = mk(mj[1], mj[2], mj[3]);
-    const V3<R> qj = quad_dot(mj + 4, dr);
-    R Mj[6];
-    oct_dot(mj + 10, dr, Mj);
-    const V3<R> oj = quad_dot(Mj, dr);
-    const R djr = dot(dj, dr), qjr = dot(qj, dr), ojr = dot(oj, dr);
-    const V3<R> di = mk(mi[1], mi[2], mi[3]);
-    const V3<R> qi = quad_dot(mi + 4, dr);
-    R Mi[6];
-    oct_dot(mi + 10, dr, Mi);
-    const V3<R> oi = quad_dot(Mi, dr);
-    const R dir = dot(di, dr), qir = dot(qi, dr), oir = dot(oi, dr);
-    fi = qj * (R(2) * g2) - dj * g1 - dr * (g1 * mj[0] - g2 * djr + g3 * qjr - g4 * ojr) - oj * (R(3) * g3);
-    fj = qi * (R(-2) * g2) - di * g1 + dr * (g1 * mi[0] + g2 * dir + g3 * qir + g4 * oir) - oi * (R(3) * g3);
+    FixedRadial<R> g;
+    g.g1 = bn1 - (R(1) - ds * l3) * rinv3;
+    g.g2 = bn2 - R(3) * (R(1) - ds * l5) * rinv5;
+    g.g3 = bn3 - R(15) * (R(1) - ds * l7) * rinv7;
+    g.g4 = bn4 - R(105) * (R(1) - ds * l9) * rinv9;
+    return g;
+}
+
+// Field at site i from the permanent moments m of a site at r_i + dr.  The
+// field at that site from the moments of i is fixed_field_from(m_i, -dr, g).
+template <class R> MPID_HD V3<R> fixed_field_from(const R* m, V3<R> dr, const FixedRadial<R>& g) {
+    const V3<R> d = mk(m[1], m[2], m[3]);
+    const V3<R> q = quad_dot(m + 4, dr);
+    R M[6];
+    oct_dot(m + 10, dr, M);
+    const V3<R> o = quad_dot(M, dr);
+    const R dr_d = dot(d, dr), dr_q = dot(q, dr), dr_o = dot(o, dr);
+    return q * (R(2) * g.g2) - d * g.g1 - dr * (g.g1 * m[0] - g.g2 * dr_d + g.g3 * dr_q - g.g4 * dr_o) -
+           o * (R(3) * g.g3);
+}
+
+// Both sides: field at i from j (fi) and at j from i (fj), dr = r_j - r_i.
+template <class R>
+MPID_HD void fixed_field_pair(V3<R> dr, R r2, const R* mi, const R* mj, R2<R> tdi, R2<R> tdj, R ds,
+                              const PairParams<R>& P, V3<R>& fi, V3<R>& fj) {
+    const FixedRadial<R> g = fixed_field_radial<R>(r2, tdi, tdj, ds, P);
+    fi = fixed_field_from<R>(mj, dr, g);
+    fj = fixed_field_from<R>(mi, mk(-dr.x, -dr.y, -dr.z), g);
 }
 
 // Lanes of block J that are covalent partners of the lane's atom: those pairs
@@ -111,12 +122,16 @@ __device__ __forceinline__ unsigned long long partner_mask(const int* __restrict
 // ---------------------------------------------------------------------------
 // induced dipole field (the real-space part of the solver's matrix-vector product)
 // ---------------------------------------------------------------------------
-// MPIDReferenceForce.cpp:4161-4281.  dampi, dampj: damping factors of the
-// two sites; a: the Thole width of the pair (the sum of the sites' Thole
+// MPIDReferenceForce.cpp:4161-4281.  Radial factors f1, f2 of the dipole
+// field tensor of a pair: field at i from a dipole u at r_i + dr is
+// dr (u.dr) f2 + u f1 (symmetric in the two sites).  dampi, dampj: damping
+// factors; a: the Thole width of the pair (the sum of the sites' Thole
 // factors for pairs with polarization scale 0, the default width otherwise).
+template <class R> struct InducedRadial {
+    R f1, f2;
+};
 template <class R>
-MPID_HD void induced_field_pair(V3<R> dr, R r2, V3<R> ui, V3<R> uj, R dampi, R dampj, R a,
-                                                   const PairParams<R>& P, V3<R>& fi, V3<R>& fj) {
+MPID_HD InducedRadial<R> induced_field_radial(R r2, R dampi, R dampj, R a, const PairParams<R>& P) {
     const R rinv = rsqrt_r(r2);
     const R r = r2 * rinv;
     const R rinv2 = rinv * rinv;
@@ -137,11 +152,20 @@ MPID_HD void induced_field_pair(V3<R> dr, R r2, V3<R> ui, V3<R> uj, R dampi, R d
         }
     }
     const R rinv3 = rinv * rinv2;
-    const R f1 = m3 * rinv3 - bn1;
-    const R f2 = bn2 - R(3) * m5 * rinv3 * rinv2;
-    const R ujr = dot(uj, dr) * f2, uir = dot(ui, dr) * f2;
-    fi = dr * ujr + uj * f1;
-    fj = dr * uir + ui * f1;
+    InducedRadial<R> f;
+    f.f1 = m3 * rinv3 - bn1;
+    f.f2 = bn2 - R(3) * m5 * rinv3 * rinv2;
+    return f;
+}
+template <class R> MPID_HD V3<R> induced_field_from(V3<R> u, V3<R> dr, const InducedRadial<R>& f) {
+    return dr * (dot(u, dr) * f.f2) + u * f.f1;
+}
+template <class R>
+MPID_HD void induced_field_pair(V3<R> dr, R r2, V3<R> ui, V3<R> uj, R dampi, R dampj, R a, const PairParams<R>& P,
+                                V3<R>& fi, V3<R>& fj) {
+    const InducedRadial<R> f = induced_field_radial<R>(r2, dampi, dampj, a, P);
+    fi = induced_field_from<R>(uj, dr, f);
+    fj = induced_field_from<R>(ui, dr, f);
 }
 
 // ---------------------------------------------------------------------------
@@ -206,6 +230,22 @@ template <class R> MPID_HD void qi_torque_sums(const R* q, const R* V, R& tx, R&
          R(3) * q[14] * V[15];
 }
 
+// The charge-charge channel of a pair (scale 1): energy q_a q_b erfc(alpha r)/r
+// and the force on b along r.  For two bare charges (no higher moment, no
+// polarisability) it is the whole of the quasi-internal-frame interaction
+// below.
+template <class R>
+MPID_HD void charge_pair(V3<R> dr, R r2, R qa, R qb, const PairParams<R>& P, V3<R>& f, R& energy) {
+    const R rinv = rsqrt_r(r2);
+    const R r = r2 * rinv;
+    const R ar = P.alpha * r;
+    const R e = exp_r(-ar * ar);
+    const R ec = erfc_r(ar, e);
+    const R c = R(kElectric) * qa * qb * rinv;
+    energy = c * ec;
+    f = dr * (c * (ec + ar * R(2.0 / kSqrtPi) * e) * rinv * rinv);
+}
+
 template <class R> struct ElecOut {
     V3<R> f;    // force on j (force on i is -f)
     V3<R> ti, tj;
@@ -215,7 +255,7 @@ template <class R> struct ElecOut {
 template <class R>
 MPID_HD ElecOut<R> elec_pair(V3<R> dr, R r2, const R* ma, const R* mb, V3<R> mua, V3<R> mub,
                                                 R2<R> tda, R2<R> tdb, bool anisoA, bool anisoB, R ms, R ps,
-                                                const PairParams<R>& P) {
+                                                const PairParams<R>& P, bool with_cc = true) {
     const R rinv = rsqrt_r(r2);
     const R r = r2 * rinv;
     // QI frame (formQIRotationMatrix, MPIDReferenceForce.cpp:650-681): z along r,
@@ -289,10 +329,15 @@ MPID_HD ElecOut<R> elec_pair(V3<R> dr, R r2, const R* ma, const R* mb, V3<R> mua
 
     R Va[16], Vb[16], VaR[16], VbR[16], Vad[3], Vbd[3];
     R eP, dP, eU, dU;
-    // C-C (m=0)
-    eP = ri[1] * (mB2 - ar[1] * X);
-    dP = R(-0.5) * mB2 * ri[2];
-    Va[0] = eP * qb[0]; Vb[0] = eP * qa[0]; VaR[0] = dP * qb[0]; VbR[0] = dP * qa[0];
+    // C-C (m=0); without with_cc the caller evaluates this channel itself
+    // (charge_pair, which it equals for scale 1)
+    if (with_cc) {
+        eP = ri[1] * (mB2 - ar[1] * X);
+        dP = R(-0.5) * mB2 * ri[2];
+        Va[0] = eP * qb[0]; Vb[0] = eP * qa[0]; VaR[0] = dP * qb[0]; VbR[0] = dP * qa[0];
+    } else {
+        Va[0] = Vb[0] = VaR[0] = VbR[0] = R(0);
+    }
     // C-D, C-Uind (m=0)
     eP = ri[2] * mB2;
     eU = R(2) * ri[2] * (sc(t_c) + b2);

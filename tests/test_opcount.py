@@ -20,7 +20,7 @@ def counts():
     res = {}
     for line in out.splitlines():
         parts = line.split()
-        if len(parts) == 3 and parts[0].endswith("_pair"):
+        if len(parts) == 3 and parts[0].startswith(("fixed", "induced", "elec", "charge")):
             res[parts[0]] = (int(parts[1]), int(parts[2]))
     return res
 
@@ -28,16 +28,20 @@ def counts():
 def test_bench_uses_the_measured_counts(counts):
     sys.path.insert(0, ROOT)
     import bench
-    assert bench.PAIR_FLOPS["fixed_pairs"] == counts["fixed_field_pair"]
-    assert bench.PAIR_FLOPS["induced_pairs"] == counts["induced_field_pair"]
-    assert bench.PAIR_FLOPS["electrostatic_pairs"] == counts["elec_pair"]
+    assert set(bench.PAIR_FLOPS) == set(counts)
+    for name, value in counts.items():
+        assert bench.PAIR_FLOPS[name] == value, name
 
 
 def test_counts_follow_the_thole_branch(counts):
     # damping terms cost operations only when both sites are polarisable
-    for name in ("fixed_field_pair", "induced_field_pair"):
+    for name in ("fixed_field_radial", "induced_field_radial"):
         damped, undamped = counts[name]
         assert damped > undamped > 0
+    for name in ("fixed_field_from", "induced_field_from", "charge_pair"):
+        assert counts[name][0] == counts[name][1] > 0
     # the electrostatics clamps the Thole argument instead of branching: one exp apart
     d, u = counts["elec_pair"]
     assert 0 < d - u <= 4
+    # a pair of bare charges costs a small fraction of the general interaction
+    assert counts["charge_pair"][0] * 50 < counts["elec_pair"][1]
