@@ -143,6 +143,9 @@ struct FastDiv {
     unsigned long long m;
 };
 FastDiv make_fastdiv(unsigned d);
+__device__ __forceinline__ unsigned fdiv(unsigned x, FastDiv f) {
+    return (unsigned)(((unsigned long long)x * f.m) >> 40);
+}
 constexpr int kFftMaxStages = 8;
 struct FftStage {
     int rad;      // 4, 2, 3 or 5
@@ -166,6 +169,11 @@ template <class R> struct ConvParams {
     R inv[3];        // 1 / box edge
     const R* moduli; // |b(k)|^2, [n0 | n1 | n2]
 };
+// Fold for tilings with W = T + 8 <= 2T in every dimension: one workgroup per
+// Morton cell sums, for each point of its T^3 core, its own window and those
+// of its lower neighbours (launch_fold handles any tiling)
+bool fold_by_tiles(const TileGeom& tg);
+template <class R> void launch_fold_tiles(hipStream_t s, PmeGeom<R> g, TileGeom tg, const R* scratch, R* grid);
 // Q (real grid) -> S (half spectrum) -> convolution -> Q (potential grid)
 template <class R>
 void launch_recip_convolution(hipStream_t s, const FftPlan<R>& p, R* Q, R2<R>* S, ConvParams<R> cp);

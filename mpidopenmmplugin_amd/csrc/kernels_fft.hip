@@ -24,10 +24,6 @@
 
 namespace mpid {
 
-__device__ __forceinline__ unsigned fdiv(unsigned x, FastDiv f) {
-    return (unsigned)(((unsigned long long)x * f.m) >> 40);
-}
-
 template <class R> struct Cx {
     R x, y;
 };
@@ -130,11 +126,19 @@ template <class R> __device__ __forceinline__ void stage_twiddles(Cx<R>* tw_s, c
     for (int k = threadIdx.x; k < n; k += blockDim.x) tw_s[k] = Cx<R>{tw[k].x, tw[k].y};
 }
 
-// A batch holds T*n <= 32768 / (2 * sizeof(complex)) elements, at most
-// max_per_thread() per thread of a 256-thread workgroup.  Loads go to
-// registers first, all of them in flight at once, and then to LDS.
+// A batch holds T*n <= kFftLdsBytes / (2 * sizeof(complex)) elements (two
+// line buffers), at most max_per_thread() per thread of a 256-thread
+// workgroup.  Loads go to registers first, all of them in flight at once, and
+// then to LDS.  The budget keeps several workgroups resident per CU so that
+// one's loads overlap another's stages.
 constexpr int kFftThreads = 256;
-template <class R> constexpr int max_per_thread() { return 32768 / (2 * 2 * (int)sizeof(R)) / kFftThreads; }
+constexpr int kFftLdsBytes = 16384;
+constexpr int kFftMaxN = 1024;  // largest grid dimension accepted (one line per batch at worst)
+template <class R> constexpr int max_per_thread() {
+    constexpr int budget = kFftLdsBytes / (2 * 2 * (int)sizeof(R));
+    constexpr int elems = budget > kFftMaxN ? budget : kFftMaxN;
+    return (elems + kFftThreads - 1) / kFftThreads;
+}
 
 // Pass 1 (dir = -1): lines along z, real Q -> half spectrum S.
 // Pass 5 (dir = +1): half spectrum S -> real Q (Hermitian completion).
@@ -341,10 +345,11 @@ bool make_fft_axis(int n, FftAxis& ax) {
     return true;
 }
 
-// lines per workgroup: a power of two, two line buffers within 32 KB
+// lines per workgroup: a power of two, two line buffers within kFftLdsBytes
+// (one line when even that exceeds it: n up to 1024 needs 16 KB in FP64)
 template <class R> static int lines_per_group(int n) {
     int T = 16;
-    while (T > 1 && 2 * (size_t)T * n * sizeof(R2<R>) > 32768) T >>= 1;
+    while (T > 1 && 2 * (size_t)T * n * sizeof(R2<R>) > (size_t)kFftLdsBytes) T >>= 1;
     return T;
 }
 static int log2i(int v) {

@@ -354,6 +354,53 @@ __global__ __launch_bounds__(256) void k_fold(PmeGeom<R> g, TileGeom tg, const R
     }
 }
 
+// When W = T + 8 <= 2T in every dimension, the point l (local to the core
+// of its cell c) lies in window c at l and in window c - 1 at l + T when
+// l < 8, and in no other window.  One workgroup per cell then sums the
+// (at most 8) windows over its core, the neighbour cells' indices computed
+// once per workgroup.
+template <class R>
+__global__ __launch_bounds__(256) void k_fold_tiles(PmeGeom<R> g, TileGeom tg, FastDiv t2, FastDiv t12,
+                                                     const R* __restrict__ scratch, R* __restrict__ grid) {
+    const int t = blockIdx.x;
+    const int nt = tg.tiles_per_axis;
+    const int c[3] = {(int)compact_bits((unsigned)t >> 2), (int)compact_bits((unsigned)t >> 1),
+                      (int)compact_bits((unsigned)t)};
+    int w0[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) w0[k] = ((c[k] * tg.T[k] + (g.n[k] >> 1) - 6) % g.n[k] + g.n[k]) % g.n[k];
+    size_t base[8];
+#pragma unroll
+    for (int d = 0; d < 8; d++) {
+        const unsigned tx = (unsigned)((c[0] - ((d >> 2) & 1) + nt) % nt);
+        const unsigned ty = (unsigned)((c[1] - ((d >> 1) & 1) + nt) % nt);
+        const unsigned tz = (unsigned)((c[2] - (d & 1) + nt) % nt);
+        const unsigned tile = (spread_bits_pme(tx) << 2) | (spread_bits_pme(ty) << 1) | spread_bits_pme(tz);
+        base[d] = (size_t)tile * tg.W[0] * tg.W[1] * tg.W[2];
+    }
+    const int T0 = tg.T[0], T1 = tg.T[1], T2 = tg.T[2];
+    const int W1 = tg.W[1], W2 = tg.W[2];
+    const int ov0 = tg.W[0] - T0, ov1 = W1 - T1, ov2 = W2 - T2;  // = 8
+    const int core = T0 * T1 * T2;
+    for (int p = threadIdx.x; p < core; p += blockDim.x) {
+        const int lx = (int)fdiv((unsigned)p, t12), rem = p - lx * T1 * T2;
+        const int ly = (int)fdiv((unsigned)rem, t2), lz = rem - ly * T2;
+        int gx = w0[0] + lx, gy = w0[1] + ly, gz = w0[2] + lz;
+        if (gx >= g.n[0]) gx -= g.n[0];
+        if (gy >= g.n[1]) gy -= g.n[1];
+        if (gz >= g.n[2]) gz -= g.n[2];
+        const size_t id = ((size_t)gx * g.n[1] + gy) * g.n[2] + gz;
+        R sum = grid[id];
+#pragma unroll
+        for (int d = 0; d < 8; d++) {
+            const int dx = (d >> 2) & 1, dy = (d >> 1) & 1, dz = d & 1;
+            if ((!dx || lx < ov0) && (!dy || ly < ov1) && (!dz || lz < ov2))
+                sum += scratch[base[d] + ((size_t)(lx + dx * T0) * W1 + (ly + dy * T1)) * W2 + lz + dz * T2];
+        }
+        grid[id] = sum;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // gathering
 // ---------------------------------------------------------------------------
@@ -554,6 +601,16 @@ template <class R> void launch_fold(hipStream_t s, PmeGeom<R> g, TileGeom tg, co
     hipLaunchKernelGGL(k_fold<R>, dim3(blocks), dim3(256), 0, s, g, tg, scratch, grid);
 }
 
+bool fold_by_tiles(const TileGeom& tg) {
+    for (int k = 0; k < 3; k++)
+        if (tg.W[k] != tg.T[k] + 8 || tg.T[k] < 8) return false;
+    return tg.tiles_per_axis >= 2;
+}
+template <class R> void launch_fold_tiles(hipStream_t s, PmeGeom<R> g, TileGeom tg, const R* scratch, R* grid) {
+    const FastDiv t2 = make_fastdiv((unsigned)tg.T[2]), t12 = make_fastdiv((unsigned)(tg.T[1] * tg.T[2]));
+    hipLaunchKernelGGL(k_fold_tiles<R>, dim3(tg.ntiles), dim3(256), 0, s, g, tg, t2, t12, scratch, grid);
+}
+
 template <class R>
 void launch_gather_phi(hipStream_t s, int n, const R4<R>* pos, PmeGeom<R> g, const R* grid, R* phi) {
     const size_t threads = 8 * (size_t)n;
@@ -578,7 +635,8 @@ void launch_gather_field(hipStream_t s, int natoms, const int* atoms, const R4<R
                                                const int*, const int*, R*, R*);                              \
     template void launch_spread_induced_tiled<R>(hipStream_t, const R4<R>*, const R*, PmeGeom<R>, TileGeom,  \
                                                  const int*, const int*, const int*, R*, R*);                \
-    template void launch_fold<R>(hipStream_t, PmeGeom<R>, TileGeom, const R*, R*);
+    template void launch_fold<R>(hipStream_t, PmeGeom<R>, TileGeom, const R*, R*);                          \
+    template void launch_fold_tiles<R>(hipStream_t, PmeGeom<R>, TileGeom, const R*, R*);
 MPID_INSTANTIATE(float)
 MPID_INSTANTIATE(double)
 #undef MPID_INSTANTIATE

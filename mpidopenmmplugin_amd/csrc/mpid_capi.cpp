@@ -684,7 +684,8 @@ struct mpid_context {
             if (tiled_spread) {
                 launch_spread_fixed_tiled<R>(stream, P.pos.p, P.lab.p, g, tgeom, tstart.p, tend.p, P.scratch.p,
                                              P.grid.p);
-                launch_fold<R>(stream, g, tgeom, P.scratch.p, P.grid.p);
+                if (fold_by_tiles(tgeom)) launch_fold_tiles<R>(stream, g, tgeom, P.scratch.p, P.grid.p);
+                else launch_fold<R>(stream, g, tgeom, P.scratch.p, P.grid.p);
             } else {
                 launch_spread_fixed<R>(stream, n, P.pos.p, P.lab.p, g, P.grid.p);
             }
@@ -710,7 +711,8 @@ struct mpid_context {
                 else
                     launch_spread_induced_tiled<R>(stream, P.pos.p, P.mu.p, g, tgeom, pol_idx.p, tpstart.p, tpend.p,
                                                    P.scratch.p, P.grid.p);
-                launch_fold<R>(stream, g, tgeom, P.scratch.p, P.grid.p);
+                if (fold_by_tiles(tgeom)) launch_fold_tiles<R>(stream, g, tgeom, P.scratch.p, P.grid.p);
+                else launch_fold<R>(stream, g, tgeom, P.scratch.p, P.grid.p);
             } else {
                 launch_spread_induced<R>(stream, n, P.pos.p, P.mu.p, g, P.grid.p);
             }
