@@ -6,12 +6,12 @@
 // come within the cutoff; the neighbour-list build leaves, for every tile and
 // every atom i of block I, a 64-bit mask of the atoms j of block J that are
 // within the cutoff and are not covalent partners of i (j > i when I == J).
-// One wavefront owns one tile: the set bits are ranked with a wavefront
-// prefix sum, and the wavefront then walks the tile's pairs 64 at a time,
-// lane t taking the pair of rank t + 64c.  Every lane thus works on a pair
-// inside the cutoff, whatever the tile's fill.  Both sides accumulate in LDS
-// (LDS atomics) and are flushed to the fixed-point global accumulators once
-// per tile.
+// One workgroup owns one tile: its first wavefront ranks the set bits with a
+// prefix sum and writes the tile's pairs into an LDS list; the workgroup then
+// walks the list, thread t taking entries t, t + 256, ...  Every thread thus
+// works on a pair inside the cutoff, whatever the tile's fill.  Both sides
+// accumulate in LDS (atomic adds) and are flushed to the fixed-point global
+// accumulators once per tile.
 //
 // The pair formulas are in pair_math.h (MPIDReferenceForce.cpp:4335-4920).
 #include "pair_math.h"
@@ -52,32 +52,10 @@ __global__ __launch_bounds__(64) void k_block_bounds(int n, const R4<R>* __restr
     }
 }
 
-// Position of the k-th (0-based) set bit of m; m has more than k set bits.
-__device__ __forceinline__ int select_bit(unsigned long long m, int k) {
-    int pos = 0;
-    unsigned v = (unsigned)m;
-    const int c = __popc(v);
-    if (k >= c) {
-        k -= c;
-        v = (unsigned)(m >> 32);
-        pos = 32;
-    }
-#pragma unroll
-    for (int w = 16; w >= 1; w >>= 1) {
-        const int cl = __popc(v & ((1u << w) - 1u));
-        if (k >= cl) {
-            k -= cl;
-            v >>= w;
-            pos += w;
-        }
-    }
-    return pos;
-}
-
 constexpr int kElecThreads = 256;  // four wavefronts share one tile
 
 template <class R>
-__global__ __launch_bounds__(kElecThreads) void k_electrostatics(const int* __restrict__ rows,
+__device__ __forceinline__ void electrostatics_tile(const int* __restrict__ rows,
                                                                   const int* __restrict__ owner,
                                                                   const unsigned long long* __restrict__ tmask,
                                                                   const unsigned long long* __restrict__ tmaskcc,
@@ -88,9 +66,9 @@ __global__ __launch_bounds__(kElecThreads) void k_electrostatics(const int* __re
                                                                   acc_t* __restrict__ torque,
                                                                   acc_t* __restrict__ energy) {
     constexpr int kS = kMP + 1;  // odd LDS row stride: rows of different atoms start in different banks
-    // [0]: the general pairs, [1]: pairs of two bare charges
-    __shared__ unsigned long long smask[2][kTile];
-    __shared__ int soff[2][kTile];
+    // the tile's pairs as (il << 8 | jj): the pairs of two bare charges
+    // first, then the general ones (at most 64 x 64 in all)
+    __shared__ unsigned short splist[kTile * kTile];
     __shared__ int stotal[2];
     // [0]: atoms of block I, [1]: atoms of block J; FP64 in both precisions
     // (LDS atomic adds of doubles keep up on gfx950, those of floats stall)
@@ -104,19 +82,32 @@ __global__ __launch_bounds__(kElecThreads) void k_electrostatics(const int* __re
     const int r = blockIdx.x, tid = threadIdx.x, lane = tid & (kTile - 1), wave = tid / kTile;
     const int I = owner[r], J = rows[r];
     if (J < I) return;  // each block pair once (uniform over the workgroup)
-    if (wave < 2) {
-        // rank the tile's pairs: prefix sum of the lanes' mask popcounts
-        const unsigned long long m = (wave == 0 ? tmask : tmaskcc)[(size_t)r * kTile + lane];
-        const int c = __popcll(m);
-        int incl = c;
+    if (wave == 0) {
+        // rank the tile's pairs (prefix sums of the lanes' popcounts) and
+        // write them out: bare-charge pairs at [0, total_cc), the others after
+        const unsigned long long mc = tmaskcc[(size_t)r * kTile + lane];
+        const unsigned long long mg = tmask[(size_t)r * kTile + lane];
+        const int cc = __popcll(mc), cg = __popcll(mg);
+        int ic = cc, ig = cg;
 #pragma unroll
         for (int off = 1; off < kTile; off <<= 1) {
-            const int v = __shfl_up(incl, off);
-            if (lane >= off) incl += v;
+            const int vc = __shfl_up(ic, off), vg = __shfl_up(ig, off);
+            if (lane >= off) {
+                ic += vc;
+                ig += vg;
+            }
         }
-        smask[wave][lane] = m;
-        soff[wave][lane] = incl - c;
-        if (lane == kTile - 1) stotal[wave] = incl;
+        const int tc = __shfl(ic, kTile - 1), tg = __shfl(ig, kTile - 1);
+        int o = ic - cc;
+        for (unsigned long long b = mc; b; b &= b - 1ull)
+            splist[o++] = (unsigned short)((lane << 8) | (__ffsll((long long)b) - 1));
+        o = tc + ig - cg;
+        for (unsigned long long b = mg; b; b &= b - 1ull)
+            splist[o++] = (unsigned short)((lane << 8) | (__ffsll((long long)b) - 1));
+        if (lane == 0) {
+            stotal[0] = tg;
+            stotal[1] = tc;
+        }
     }
     __syncthreads();
     const int total = stotal[0], total_cc = stotal[1];
@@ -155,12 +146,7 @@ __global__ __launch_bounds__(kElecThreads) void k_electrostatics(const int* __re
     // all its terms alike.  For pairs of two bare charges it is the whole
     // interaction.
     for (int q = tid; q < total_cc + total; q += kElecThreads) {
-        const int list = q < total_cc ? 1 : 0, qq = q < total_cc ? q : q - total_cc;
-        int il = 0;
-#pragma unroll
-        for (int step = 32; step >= 1; step >>= 1)
-            if (soff[list][il + step] <= qq) il += step;
-        const int jj = select_bit(smask[list][il], qq - soff[list][il]);
+        const int ent = splist[q], il = ent >> 8, jj = ent & 0xff;
         const double4 pi = spos64[0][il], pj = spos64[1][jj];
         const V3<double> dr = min_image(mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z), P64.box);
         V3<double> f;
@@ -176,12 +162,7 @@ __global__ __launch_bounds__(kElecThreads) void k_electrostatics(const int* __re
     }
     // the rest of the general pairs' interaction, in the working precision
     for (int q = tid; q < total; q += kElecThreads) {
-        // owning lane: the last il with soff[il] <= q
-        int il = 0;
-#pragma unroll
-        for (int step = 32; step >= 1; step >>= 1)
-            if (soff[0][il + step] <= q) il += step;
-        const int jj = select_bit(smask[0][il], q - soff[0][il]);
+        const int ent = splist[total_cc + q], il = ent >> 8, jj = ent & 0xff;
         const R4<R> pi = spos[0][il], pj = spos[1][jj];
         const V3<R> dr = min_image(mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z), P.box);
         R mi[kMP], mj[kMP];
@@ -234,6 +215,31 @@ __global__ __launch_bounds__(kElecThreads) void k_electrostatics(const int* __re
         acc_add(energy, e);
     }
 }
+
+// FP32: at most 128 VGPRs, so that four wavefronts fit per SIMD
+__global__ __launch_bounds__(kElecThreads) __attribute__((amdgpu_waves_per_eu(4)))
+void k_electrostatics_f32(const int* __restrict__ rows, const int* __restrict__ owner,
+                          const unsigned long long* __restrict__ tmask, const unsigned long long* __restrict__ tmaskcc,
+                          PairAtoms<float> A, const float* __restrict__ mu, PairParams<float> P,
+                          const double4* __restrict__ pos64, const double* __restrict__ lab64, PairParams<double> P64,
+                          acc_t* __restrict__ force, acc_t* __restrict__ torque, acc_t* __restrict__ energy) {
+    electrostatics_tile<float>(rows, owner, tmask, tmaskcc, A, mu, P, pos64, lab64, P64, force, torque, energy);
+}
+__global__ __launch_bounds__(kElecThreads)
+void k_electrostatics_f64(const int* __restrict__ rows, const int* __restrict__ owner,
+                          const unsigned long long* __restrict__ tmask, const unsigned long long* __restrict__ tmaskcc,
+                          PairAtoms<double> A, const double* __restrict__ mu, PairParams<double> P,
+                          const double4* __restrict__ pos64, const double* __restrict__ lab64, PairParams<double> P64,
+                          acc_t* __restrict__ force, acc_t* __restrict__ torque, acc_t* __restrict__ energy) {
+    electrostatics_tile<double>(rows, owner, tmask, tmaskcc, A, mu, P, pos64, lab64, P64, force, torque, energy);
+}
+template <class R> struct ElecKernel;
+template <> struct ElecKernel<float> {
+    static constexpr auto fn = k_electrostatics_f32;
+};
+template <> struct ElecKernel<double> {
+    static constexpr auto fn = k_electrostatics_f64;
+};
 
 // ---------------------------------------------------------------------------
 // Covalent pairs.  The pairs on the scaled covalent lists (1-2, 1-3 and 1-4
@@ -391,8 +397,8 @@ void launch_electrostatics_pairs(hipStream_t s, int nrows, const int* rows, cons
                                  const R* mu, PairParams<R> p, const double4* pos64, const double* lab64,
                                  PairParams<double> p64, acc_t* force, acc_t* torque, acc_t* energy) {
     if (nrows <= 0) return;
-    hipLaunchKernelGGL(k_electrostatics<R>, dim3(nrows), dim3(kElecThreads), 0, s, rows, owner, tmask, tmaskcc, a, mu,
-                       p, pos64, lab64, p64, force, torque, energy);
+    hipLaunchKernelGGL(ElecKernel<R>::fn, dim3(nrows), dim3(kElecThreads), 0, s, rows, owner, tmask, tmaskcc, a, mu, p,
+                       pos64, lab64, p64, force, torque, energy);
 }
 
 template <class R> void launch_cov_fixed_field(hipStream_t s, int n, PairAtoms<R> a, PairParams<R> p, acc_t* field) {
