@@ -65,7 +65,10 @@ __device__ __forceinline__ void electrostatics_tile(const int* __restrict__ rows
                                                                   PairParams<double> P64, acc_t* __restrict__ force,
                                                                   acc_t* __restrict__ torque,
                                                                   acc_t* __restrict__ energy) {
-    constexpr int kS = kMP + 1;  // odd LDS row stride: rows of different atoms start in different banks
+    // Multipole rows of 20 reals, read as 16-byte vectors: with a 20-word
+    // stride, 16 lanes reading 16 different rows cover each bank once.
+    constexpr int kS = kMP;
+    constexpr int kV = 16 / (int)sizeof(R);  // reals per 16-byte vector
     // the tile's pairs as (il << 8 | jj): the pairs of two bare charges
     // first, then the general ones (at most 64 x 64 in all)
     __shared__ unsigned short splist[kTile * kTile];
@@ -74,7 +77,7 @@ __device__ __forceinline__ void electrostatics_tile(const int* __restrict__ rows
     // (LDS atomic adds of doubles keep up on gfx950, those of floats stall)
     __shared__ double acc[2][kTile][6];
     __shared__ R4<R> spos[2][kTile];
-    __shared__ R smp[2][kTile * kS];
+    __shared__ __align__(16) R smp[2][kTile * kS];
     __shared__ R smu[2][kTile * 3];
     __shared__ int san[2][kTile];
     __shared__ double4 spos64[2][kTile];  // x, y, z, charge in FP64 for the charge-charge channel
@@ -166,10 +169,19 @@ __device__ __forceinline__ void electrostatics_tile(const int* __restrict__ rows
         const R4<R> pi = spos[0][il], pj = spos[1][jj];
         const V3<R> dr = min_image(mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z), P.box);
         R mi[kMP], mj[kMP];
+        {
+            typedef R vec __attribute__((ext_vector_type(kV)));
+            const vec* ri = reinterpret_cast<const vec*>(&smp[0][il * kS]);
+            const vec* rj = reinterpret_cast<const vec*>(&smp[1][jj * kS]);
 #pragma unroll
-        for (int k = 0; k < kMP; k++) {
-            mi[k] = smp[0][il * kS + k];
-            mj[k] = smp[1][jj * kS + k];
+            for (int v = 0; v < kMP / kV; v++) {
+                const vec a = ri[v], b = rj[v];
+#pragma unroll
+                for (int c = 0; c < kV; c++) {
+                    mi[v * kV + c] = a[c];
+                    mj[v * kV + c] = b[c];
+                }
+            }
         }
         const V3<R> ui = mk(smu[0][3 * il], smu[0][3 * il + 1], smu[0][3 * il + 2]);
         const V3<R> uj = mk(smu[1][3 * jj], smu[1][3 * jj + 1], smu[1][3 * jj + 2]);
