@@ -31,9 +31,11 @@ sys.path.insert(0, ROOT)
 # without polarisability).  The field formulas are one-sided: the radial
 # factors of a pair, and the field one site's moments produce at the other;
 # elec_pair is the quasi-internal-frame interaction without its charge-charge
-# channel, which charge_pair evaluates (in FP64) for every pair.
+# channel, which charge_pair evaluates (in FP64) for every pair; elec_pair_bare
+# is elec_pair for a pair whose second site is a bare charge.
 PAIR_FLOPS = {"fixed_field_radial": (73, 47), "fixed_field_from": (105, 105), "induced_field_radial": (36, 23),
-              "induced_field_from": (15, 15), "elec_pair": (2106, 2104), "charge_pair": (19, 19)}
+              "induced_field_from": (15, 15), "elec_pair": (2106, 2104), "elec_pair_bare": (570, 568),
+              "charge_pair": (19, 19)}
 
 
 def pair_kernel_flops(cat, pc):
@@ -46,8 +48,9 @@ def pair_kernel_flops(cat, pc):
     if cat == "induced_pairs":  # solver operator: pairs of polarisable sites, both sides
         return pc["pol"] * (F["induced_field_radial"][0] + 2 * F["induced_field_from"][0])
     if cat == "electrostatic_pairs":  # the charge-charge channel of every pair, the rest where moments exist
+        general = pc["all"] - pc["cc"] - pc["bare1"]
         return (pc["all"] * F["charge_pair"][0] + pc["pol"] * F["elec_pair"][0] +
-                (pc["all"] - pc["pol"] - pc["cc"]) * F["elec_pair"][1])
+                (general - pc["pol"]) * F["elec_pair"][1] + pc["bare1"] * F["elec_pair_bare"][1])
     return None
 
 
@@ -58,7 +61,8 @@ def count_pairs(pos, box, cutoff, flat):
     """Unordered pairs within the cutoff that the pair kernels evaluate (the
     scaled covalent partner pairs, which run in the FP64 covalent kernels,
     excluded), split by kind: all, both sites polarisable ("pol"), exactly one
-    ("mixed"), both bare charges without polarisability ("cc")."""
+    ("mixed"), both bare charges without polarisability ("cc"), exactly one
+    bare charge ("bare1")."""
     from scipy.spatial import cKDTree
     L = np.asarray(box, dtype=np.float64)
     p = np.mod(pos, L)
@@ -81,7 +85,7 @@ def count_pairs(pos, box, cutoff, flat):
         pairs = pairs[~np.isin(pkey, ckey)]
     pi, pj = pairs[:, 0], pairs[:, 1]
     return {"all": int(len(pairs)), "pol": int((pol[pi] & pol[pj]).sum()), "mixed": int((pol[pi] ^ pol[pj]).sum()),
-            "cc": int((bare[pi] & bare[pj]).sum())}
+            "cc": int((bare[pi] & bare[pj]).sum()), "bare1": int((bare[pi] ^ bare[pj]).sum())}
 
 
 def grid_bytes(cat, cfg, n, launches, mixed):

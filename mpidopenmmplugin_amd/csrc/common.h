@@ -46,13 +46,14 @@ template <class R> MPID_HD V3<R> cross(V3<R> a, V3<R> b) {
 
 // Precision-dispatched math.  FP32 uses the hardware reciprocal and
 // reciprocal square root (1 ulp; pair distances are never denormal) and a
-// fitted erfc; FP64 uses the correctly rounded library functions.
+// fitted erfc; FP64 uses the library functions (also on the host, where the
+// pair formulas are instantiated for operation counts and identity checks).
 __device__ __forceinline__ float rsqrt_r(float x) { return __builtin_amdgcn_rsqf(x); }
-__device__ __forceinline__ double rsqrt_r(double x) { return 1.0 / sqrt(x); }
+MPID_HD double rsqrt_r(double x) { return 1.0 / sqrt(x); }
 __device__ __forceinline__ float div_r(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
-__device__ __forceinline__ double div_r(double a, double b) { return a / b; }
+MPID_HD double div_r(double a, double b) { return a / b; }
 __device__ __forceinline__ float exp_r(float x) { return expf(x); }
-__device__ __forceinline__ double exp_r(double x) { return exp(x); }
+MPID_HD double exp_r(double x) { return exp(x); }
 
 // erfc(x) for x >= 0, given e = exp(-x^2) (which the Ewald terms need anyway).
 // FP32: erfc(x) = e * t * Q(t), t = 1 / (1 + 0.485 x), with Q a degree-8
@@ -70,7 +71,7 @@ __device__ __forceinline__ float erfc_r(float x, float e) {
     for (int k = 7; k >= 0; k--) q = fmaf(q, t, kErfcQ[k]);
     return e * t * q;
 }
-__device__ __forceinline__ double erfc_r(double x, double) { return erfc(x); }
+MPID_HD double erfc_r(double x, double) { return erfc(x); }
 
 // Rectangular periodic box as seen by the kernels.
 template <class R> struct Box {

@@ -183,13 +183,16 @@ template <class R>
 void launch_block_bounds(hipStream_t s, int n, const R4<R>* pos, float4* bmin, float4* bmax);
 // one wavefront per block-row entry r; entries with J < I return at once
 // tmask: pairs with a multipole or a polarisable site; tmaskcc: pairs of two
-// bare charges.  The charge-charge channel of all pairs is evaluated in FP64
-// from pos64 (x, y, z) and the FP64 lab-frame charges lab64[20 i].
+// bare charges; chg[i] != 0: atom i is a bare charge (pairs of tmask with one
+// such site take the reduced formula).  The charge-charge channel of all
+// pairs is evaluated in FP64 from pos64 (x, y, z) and the FP64 lab-frame
+// charges lab64[20 i].
 template <class R>
 void launch_electrostatics_pairs(hipStream_t s, int nrows, const int* rows, const int* owner,
-                                 const unsigned long long* tmask, const unsigned long long* tmaskcc, PairAtoms<R> a,
-                                 const R* mu, PairParams<R> p, const double4* pos64, const double* lab64,
-                                 PairParams<double> p64, acc_t* force, acc_t* torque, acc_t* energy);
+                                 const unsigned long long* tmask, const unsigned long long* tmaskcc, const int* chg,
+                                 PairAtoms<R> a, const R* mu, PairParams<R> p, const double4* pos64,
+                                 const double* lab64, PairParams<double> p64, acc_t* force, acc_t* torque,
+                                 acc_t* energy);
 // ---- neighbour lists (kernels_nlist.hip) ---------------------------------------
 void launch_block_rows(hipStream_t s, int nb, const float4* bmin, const float4* bmax, Box<float> box, float cutoff,
                        const int* start, int* counts, int* rows, int* owner);

@@ -281,7 +281,6 @@ __global__ __launch_bounds__(256) void k_nl_induced_field(NbrList L, const int* 
     const int i = atoms ? atoms[iv ? t : natoms - 1] : (iv ? t : n - 1);
     const int ic = i;
     const R4<R> pi = A.pos[ic];
-    const V3<R> ui = mk(mu[3 * (size_t)ic], mu[3 * (size_t)ic + 1], mu[3 * (size_t)ic + 2]);
     const int cnt = iv ? L.count[ic] : 0;
     const int* lp = L.nbr + (size_t)ic * kNlSplit + sub;
     const size_t stride = (size_t)n * kNlSplit;

@@ -6,12 +6,14 @@
 // come within the cutoff; the neighbour-list build leaves, for every tile and
 // every atom i of block I, a 64-bit mask of the atoms j of block J that are
 // within the cutoff and are not covalent partners of i (j > i when I == J).
-// One workgroup owns one tile: its first wavefront ranks the set bits with a
-// prefix sum and writes the tile's pairs into an LDS list; the workgroup then
-// walks the list, thread t taking entries t, t + 256, ...  Every thread thus
-// works on a pair inside the cutoff, whatever the tile's fill.  Both sides
-// accumulate in LDS (atomic adds) and are flushed to the fixed-point global
-// accumulators once per tile.
+// One workgroup (four wavefronts) owns one tile: it ranks the set bits with
+// prefix sums and writes the tile's pairs into an LDS list, sorted into three
+// classes (two bare charges, one bare charge, general), and then walks each
+// class in chunks of 64 entries, one chunk per wavefront.  Every thread thus
+// works on a pair inside the cutoff, whatever the tile's fill, and each class
+// runs its own formula without divergence.  Both sides accumulate in LDS
+// (atomic adds) and are flushed to the fixed-point global accumulators once
+// per tile.
 //
 // The pair formulas are in pair_math.h (MPIDReferenceForce.cpp:4335-4920).
 #include "pair_math.h"
@@ -59,6 +61,7 @@ __device__ __forceinline__ void electrostatics_tile(const int* __restrict__ rows
                                                                   const int* __restrict__ owner,
                                                                   const unsigned long long* __restrict__ tmask,
                                                                   const unsigned long long* __restrict__ tmaskcc,
+                                                                  const int* __restrict__ chg,
                                                                   PairAtoms<R> A, const R* __restrict__ mu,
                                                                   PairParams<R> P, const double4* __restrict__ pos64,
                                                                   const double* __restrict__ lab64,
@@ -69,13 +72,12 @@ __device__ __forceinline__ void electrostatics_tile(const int* __restrict__ rows
     // stride, 16 lanes reading 16 different rows cover each bank once.
     constexpr int kS = kMP;
     constexpr int kV = 16 / (int)sizeof(R);  // reals per 16-byte vector
-    // the tile's pairs as (il << 8 | jj): the pairs of two bare charges
-    // first, then the general ones (at most 64 x 64 in all)
+    // the tile's pairs as (il << 8 | jj), at most 64 x 64 in all
     __shared__ unsigned short splist[kTile * kTile];
-    __shared__ int stotal[2];
-    // [0]: atoms of block I, [1]: atoms of block J; FP64 in both precisions
-    // (LDS atomic adds of doubles keep up on gfx950, those of floats stall)
-    __shared__ double acc[2][kTile][6];
+    // [side][component][atom], side 0: atoms of block I, 1: atoms of block J;
+    // components: force x, y, z, torque x, y, z.  FP64 in both precisions
+    // (LDS atomic adds of doubles keep up on gfx950, those of floats stall).
+    __shared__ double acc[2][6][kTile];
     __shared__ R4<R> spos[2][kTile];
     __shared__ __align__(16) R smp[2][kTile * kS];
     __shared__ R smu[2][kTile * 3];
@@ -85,40 +87,56 @@ __device__ __forceinline__ void electrostatics_tile(const int* __restrict__ rows
     const int r = blockIdx.x, tid = threadIdx.x, lane = tid & (kTile - 1), wave = tid / kTile;
     const int I = owner[r], J = rows[r];
     if (J < I) return;  // each block pair once (uniform over the workgroup)
-    if (wave == 0) {
-        // rank the tile's pairs (prefix sums of the lanes' popcounts) and
-        // write them out: bare-charge pairs at [0, total_cc), the others after
-        const unsigned long long mc = tmaskcc[(size_t)r * kTile + lane];
-        const unsigned long long mg = tmask[(size_t)r * kTile + lane];
-        const int cc = __popcll(mc), cg = __popcll(mg);
-        int ic = cc, ig = cg;
+    const int n = P.n;
+    // A workgroup's wavefronts sit on fixed SIMDs of a CU, and a tile's work
+    // is uneven: one wavefront takes the general pairs (about four times the
+    // cost of a pair with a bare charge).  The roles therefore rotate over
+    // the wavefronts from tile to tile (a multiplicative hash of the tile
+    // index), so that the workgroups sharing a CU load its SIMDs alike.
+    const int role = (wave - (int)(((unsigned)r * 0x9E3779B1u) >> 30)) & 3;
+
+    // Every wavefront ranks the tile's pairs (prefix sums of the lanes'
+    // popcounts): pairs of two bare charges at [0, tc), general pairs at
+    // [tc, tc + t2), pairs with one bare charge after.  Bit 15 of a one-bare
+    // entry says that i (not j) is the bare site.  Roles 1, 2 and 3 write out
+    // one class each.
+    const bool bi = chg[min(I * kTile + lane, n - 1)] != 0;
+    const unsigned long long bj = __ballot(chg[min(J * kTile + lane, n - 1)] != 0);
+    const unsigned long long mc = tmaskcc[(size_t)r * kTile + lane];
+    const unsigned long long mg = tmask[(size_t)r * kTile + lane];
+    const unsigned long long m1 = bi ? mg : (mg & bj);
+    const unsigned long long m2 = bi ? 0ull : (mg & ~bj);
+    const int cc = __popcll(mc), c1 = __popcll(m1), c2 = __popcll(m2);
+    int ic = cc, i1 = c1, i2 = c2;
 #pragma unroll
-        for (int off = 1; off < kTile; off <<= 1) {
-            const int vc = __shfl_up(ic, off), vg = __shfl_up(ig, off);
-            if (lane >= off) {
-                ic += vc;
-                ig += vg;
-            }
+    for (int off = 1; off < kTile; off <<= 1) {
+        const int vc = __shfl_up(ic, off), v1 = __shfl_up(i1, off), v2 = __shfl_up(i2, off);
+        if (lane >= off) {
+            ic += vc;
+            i1 += v1;
+            i2 += v2;
         }
-        const int tc = __shfl(ic, kTile - 1), tg = __shfl(ig, kTile - 1);
+    }
+    const int tc = __shfl(ic, kTile - 1), t1 = __shfl(i1, kTile - 1), t2 = __shfl(i2, kTile - 1);
+    if (tc + t1 + t2 == 0) return;  // uniform over the workgroup
+    if (role == 1) {
         int o = ic - cc;
         for (unsigned long long b = mc; b; b &= b - 1ull)
             splist[o++] = (unsigned short)((lane << 8) | (__ffsll((long long)b) - 1));
-        o = tc + ig - cg;
-        for (unsigned long long b = mg; b; b &= b - 1ull)
+    } else if (role == 2) {
+        int o = tc + t2 + i1 - c1;
+        const int flag = bi ? 0x8000 : 0;
+        for (unsigned long long b = m1; b; b &= b - 1ull)
+            splist[o++] = (unsigned short)(flag | (lane << 8) | (__ffsll((long long)b) - 1));
+    } else if (role == 3) {
+        int o = tc + i2 - c2;
+        for (unsigned long long b = m2; b; b &= b - 1ull)
             splist[o++] = (unsigned short)((lane << 8) | (__ffsll((long long)b) - 1));
-        if (lane == 0) {
-            stotal[0] = tg;
-            stotal[1] = tc;
-        }
     }
-    __syncthreads();
-    const int total = stotal[0], total_cc = stotal[1];
-    if (total == 0 && total_cc == 0) return;
-    const int n = P.n;
-    // stage both blocks in LDS (thread tid: side tid / 128, atom tid % 64 for the per-atom fields)
-    if (tid < 2 * kTile) {
-        const int side = tid / kTile;
+    // stage both blocks in LDS: roles 0 and 3 the per-atom fields of sides
+    // 0 and 1, all threads the multipole rows
+    if (role == 0 || role == 3) {
+        const int side = role == 0 ? 0 : 1;
         const int a = min((side == 0 ? I : J) * kTile + lane, n - 1);
         spos[side][lane] = A.pos[a];
         smu[side][3 * lane + 0] = mu[3 * (size_t)a + 0];
@@ -128,7 +146,7 @@ __device__ __forceinline__ void electrostatics_tile(const int* __restrict__ rows
         const double4 p64 = pos64[a];
         spos64[side][lane] = make_double4(p64.x, p64.y, p64.z, lab64[(size_t)a * kMP]);
 #pragma unroll
-        for (int k = 0; k < 6; k++) acc[side][lane][k] = 0.0;
+        for (int k = 0; k < 6; k++) acc[side][k][lane] = 0.0;
     }
 #pragma unroll
     for (int side = 0; side < 2; side++) {
@@ -142,13 +160,20 @@ __device__ __forceinline__ void electrostatics_tile(const int* __restrict__ rows
         }
     }
     __syncthreads();
+    // Within a wavefront's chunk of 64 entries, lane l takes entry
+    // 8 (l mod 8) + l / 8: entries of one atom i are adjacent in the list,
+    // and spreading them over the lanes keeps the atomic adds of one
+    // instruction off a common address.
+    const int perm = ((lane & 7) << 3) | (lane >> 3);
     double en = 0.0;
-    // The charge-charge channel of every pair, in FP64 in both precision
-    // modes: it carries most of the magnitude of the real-space energy, and
-    // FP32 rounding of the charges, alpha and the Coulomb constant would bias
-    // all its terms alike.  For pairs of two bare charges it is the whole
-    // interaction.
-    for (int q = tid; q < total_cc + total; q += kElecThreads) {
+    // The charge-charge channel runs in FP64 in both precision modes for
+    // every pair: it carries most of the magnitude of the real-space energy,
+    // and FP32 rounding of the charges, alpha and the Coulomb constant would
+    // bias all its terms alike.  This loop takes it for the pairs of two bare
+    // charges (where it is the whole interaction) and for the general pairs
+    // (apart from their formula, whose registers it would otherwise hold);
+    // the one-bare loop takes it with the rest.  Chunks start at role 3.
+    for (int q = ((3 - role) & 3) * kTile + perm; q < tc + t2; q += kElecThreads) {
         const int ent = splist[q], il = ent >> 8, jj = ent & 0xff;
         const double4 pi = spos64[0][il], pj = spos64[1][jj];
         const V3<double> dr = min_image(mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z), P64.box);
@@ -156,16 +181,60 @@ __device__ __forceinline__ void electrostatics_tile(const int* __restrict__ rows
         double e;
         charge_pair<double>(dr, dot(dr, dr), pi.w, pj.w, P64, f, e);
         en += e;
-        atomicAdd(&acc[0][il][0], -f.x);
-        atomicAdd(&acc[0][il][1], -f.y);
-        atomicAdd(&acc[0][il][2], -f.z);
-        atomicAdd(&acc[1][jj][0], f.x);
-        atomicAdd(&acc[1][jj][1], f.y);
-        atomicAdd(&acc[1][jj][2], f.z);
+        atomicAdd(&acc[0][0][il], -f.x);
+        atomicAdd(&acc[0][1][il], -f.y);
+        atomicAdd(&acc[0][2][il], -f.z);
+        atomicAdd(&acc[1][0][jj], f.x);
+        atomicAdd(&acc[1][1][jj], f.y);
+        atomicAdd(&acc[1][2][jj], f.z);
     }
-    // the rest of the general pairs' interaction, in the working precision
-    for (int q = tid; q < total; q += kElecThreads) {
-        const int ent = splist[total_cc + q], il = ent >> 8, jj = ent & 0xff;
+    // Pairs with one bare charge b: the FP64 charge-charge channel, and b's
+    // charge against a's moments and induced dipole in the working
+    // precision.  Chunks start at role 1.
+    for (int q = ((role - 1) & 3) * kTile + perm; q < t1; q += kElecThreads) {
+        const int ent = splist[tc + t2 + q], il = (ent >> 8) & 0x3f, jj = ent & 0xff;
+        const bool ibare = (ent & 0x8000) != 0;
+        const int sa = ibare ? 1 : 0, ia = ibare ? jj : il;
+        const int sb = 1 - sa, ib = ibare ? il : jj;
+        const R4<R> pa = spos[sa][ia], pb = spos[sb][ib];
+        const V3<R> dr = min_image(mk(pb.x - pa.x, pb.y - pa.y, pb.z - pa.z), P.box);
+        R ma[kMP];
+        {
+            typedef R vec __attribute__((ext_vector_type(kV)));
+            const vec* ra = reinterpret_cast<const vec*>(&smp[sa][ia * kS]);
+#pragma unroll
+            for (int v = 0; v < kMP / kV; v++) {
+                const vec x = ra[v];
+#pragma unroll
+                for (int c = 0; c < kV; c++) ma[v * kV + c] = x[c];
+            }
+        }
+        const V3<R> ua = mk(smu[sa][3 * ia], smu[sa][3 * ia + 1], smu[sa][3 * ia + 2]);
+        const ElecBareOut<R> o = elec_pair_bare<R>(dr, dot(dr, dr), ma, smp[sb][ib * kS], ua, R2<R>{R(0), pa.w},
+                                                   R2<R>{R(0), pb.w}, san[sa][ia] != 0, R(1), R(1), P);
+        const double4 qa = spos64[sa][ia], qb = spos64[sb][ib];
+        const V3<double> dr64 = min_image(mk(qb.x - qa.x, qb.y - qa.y, qb.z - qa.z), P64.box);
+        V3<double> f;
+        double e;
+        charge_pair<double>(dr64, dot(dr64, dr64), qa.w, qb.w, P64, f, e);
+        en += e + (double)o.energy;
+        f.x += (double)o.f.x;
+        f.y += (double)o.f.y;
+        f.z += (double)o.f.z;
+        atomicAdd(&acc[sa][0][ia], -f.x);
+        atomicAdd(&acc[sa][1][ia], -f.y);
+        atomicAdd(&acc[sa][2][ia], -f.z);
+        atomicAdd(&acc[sa][3][ia], (double)o.ta.x);
+        atomicAdd(&acc[sa][4][ia], (double)o.ta.y);
+        atomicAdd(&acc[sa][5][ia], (double)o.ta.z);
+        atomicAdd(&acc[sb][0][ib], f.x);
+        atomicAdd(&acc[sb][1][ib], f.y);
+        atomicAdd(&acc[sb][2][ib], f.z);
+    }
+    // General pairs: the quasi-internal-frame interaction without its
+    // charge-charge channel, in the working precision.  Chunks start at role 0.
+    for (int q = role * kTile + perm; q < t2; q += kElecThreads) {
+        const int ent = splist[tc + q], il = ent >> 8, jj = ent & 0xff;
         const R4<R> pi = spos[0][il], pj = spos[1][jj];
         const V3<R> dr = min_image(mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z), P.box);
         R mi[kMP], mj[kMP];
@@ -188,29 +257,33 @@ __device__ __forceinline__ void electrostatics_tile(const int* __restrict__ rows
         const ElecOut<R> o = elec_pair<R>(dr, dot(dr, dr), mi, mj, ui, uj, R2<R>{R(0), pi.w}, R2<R>{R(0), pj.w},
                                           san[0][il] != 0, san[1][jj] != 0, R(1), R(1), P, false);
         en += (double)o.energy;
-        atomicAdd(&acc[0][il][0], -(double)o.f.x);
-        atomicAdd(&acc[0][il][1], -(double)o.f.y);
-        atomicAdd(&acc[0][il][2], -(double)o.f.z);
-        atomicAdd(&acc[0][il][3], (double)o.ti.x);
-        atomicAdd(&acc[0][il][4], (double)o.ti.y);
-        atomicAdd(&acc[0][il][5], (double)o.ti.z);
-        atomicAdd(&acc[1][jj][0], (double)o.f.x);
-        atomicAdd(&acc[1][jj][1], (double)o.f.y);
-        atomicAdd(&acc[1][jj][2], (double)o.f.z);
-        atomicAdd(&acc[1][jj][3], (double)o.tj.x);
-        atomicAdd(&acc[1][jj][4], (double)o.tj.y);
-        atomicAdd(&acc[1][jj][5], (double)o.tj.z);
+        const V3<R> f = o.f;
+        atomicAdd(&acc[0][0][il], -(double)f.x);
+        atomicAdd(&acc[0][1][il], -(double)f.y);
+        atomicAdd(&acc[0][2][il], -(double)f.z);
+        atomicAdd(&acc[0][3][il], (double)o.ti.x);
+        atomicAdd(&acc[0][4][il], (double)o.ti.y);
+        atomicAdd(&acc[0][5][il], (double)o.ti.z);
+        atomicAdd(&acc[1][0][jj], (double)f.x);
+        atomicAdd(&acc[1][1][jj], (double)f.y);
+        atomicAdd(&acc[1][2][jj], (double)f.z);
+        atomicAdd(&acc[1][3][jj], (double)o.tj.x);
+        atomicAdd(&acc[1][4][jj], (double)o.tj.y);
+        atomicAdd(&acc[1][5][jj], (double)o.tj.z);
     }
     // wavefront sums of the pair energies
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) en += __shfl_xor(en, off);
     if (lane == 0) sen[wave] = en;
     __syncthreads();
-    if (tid < 2 * kTile) {
-        const int side = tid / kTile;
+    // flush: role 3 side 0, role 1 side 1 (the two lightest pair loads)
+    if (role == 3 || role == 1) {
+        const int side = role == 3 ? 0 : 1;
         const int a = (side == 0 ? I : J) * kTile + lane;
         if (a < n) {
-            const double* v = acc[side][lane];
+            double v[6];
+#pragma unroll
+            for (int k = 0; k < 6; k++) v[k] = acc[side][k][lane];
             if (v[0] != 0.0 || v[1] != 0.0 || v[2] != 0.0 || v[3] != 0.0 || v[4] != 0.0 || v[5] != 0.0) {
                 acc_add(&force[3 * (size_t)a + 0], v[0]);
                 acc_add(&force[3 * (size_t)a + 1], v[1]);
@@ -232,18 +305,18 @@ __device__ __forceinline__ void electrostatics_tile(const int* __restrict__ rows
 __global__ __launch_bounds__(kElecThreads) __attribute__((amdgpu_waves_per_eu(4)))
 void k_electrostatics_f32(const int* __restrict__ rows, const int* __restrict__ owner,
                           const unsigned long long* __restrict__ tmask, const unsigned long long* __restrict__ tmaskcc,
-                          PairAtoms<float> A, const float* __restrict__ mu, PairParams<float> P,
+                          const int* __restrict__ chg, PairAtoms<float> A, const float* __restrict__ mu, PairParams<float> P,
                           const double4* __restrict__ pos64, const double* __restrict__ lab64, PairParams<double> P64,
                           acc_t* __restrict__ force, acc_t* __restrict__ torque, acc_t* __restrict__ energy) {
-    electrostatics_tile<float>(rows, owner, tmask, tmaskcc, A, mu, P, pos64, lab64, P64, force, torque, energy);
+    electrostatics_tile<float>(rows, owner, tmask, tmaskcc, chg, A, mu, P, pos64, lab64, P64, force, torque, energy);
 }
 __global__ __launch_bounds__(kElecThreads)
 void k_electrostatics_f64(const int* __restrict__ rows, const int* __restrict__ owner,
                           const unsigned long long* __restrict__ tmask, const unsigned long long* __restrict__ tmaskcc,
-                          PairAtoms<double> A, const double* __restrict__ mu, PairParams<double> P,
+                          const int* __restrict__ chg, PairAtoms<double> A, const double* __restrict__ mu, PairParams<double> P,
                           const double4* __restrict__ pos64, const double* __restrict__ lab64, PairParams<double> P64,
                           acc_t* __restrict__ force, acc_t* __restrict__ torque, acc_t* __restrict__ energy) {
-    electrostatics_tile<double>(rows, owner, tmask, tmaskcc, A, mu, P, pos64, lab64, P64, force, torque, energy);
+    electrostatics_tile<double>(rows, owner, tmask, tmaskcc, chg, A, mu, P, pos64, lab64, P64, force, torque, energy);
 }
 template <class R> struct ElecKernel;
 template <> struct ElecKernel<float> {
@@ -405,11 +478,13 @@ void launch_block_bounds(hipStream_t s, int n, const R4<R>* pos, float4* bmin, f
 
 template <class R>
 void launch_electrostatics_pairs(hipStream_t s, int nrows, const int* rows, const int* owner,
-                                 const unsigned long long* tmask, const unsigned long long* tmaskcc, PairAtoms<R> a,
-                                 const R* mu, PairParams<R> p, const double4* pos64, const double* lab64,
-                                 PairParams<double> p64, acc_t* force, acc_t* torque, acc_t* energy) {
+                                 const unsigned long long* tmask, const unsigned long long* tmaskcc, const int* chg,
+                                 PairAtoms<R> a, const R* mu, PairParams<R> p, const double4* pos64,
+                                 const double* lab64, PairParams<double> p64, acc_t* force, acc_t* torque,
+                                 acc_t* energy) {
     if (nrows <= 0) return;
-    hipLaunchKernelGGL(ElecKernel<R>::fn, dim3(nrows), dim3(kElecThreads), 0, s, rows, owner, tmask, tmaskcc, a, mu, p,
+    hipLaunchKernelGGL(ElecKernel<R>::fn, dim3(nrows), dim3(kElecThreads), 0, s, rows, owner, tmask, tmaskcc, chg, a,
+                       mu, p,
                        pos64, lab64, p64, force, torque, energy);
 }
 
@@ -431,7 +506,8 @@ void launch_cov_electrostatics(hipStream_t s, int n, PairAtoms<R> a, const R* mu
 #define MPID_INSTANTIATE(R)                                                                                        \
     template void launch_block_bounds<R>(hipStream_t, int, const R4<R>*, float4*, float4*);                         \
     template void launch_electrostatics_pairs<R>(hipStream_t, int, const int*, const int*,                         \
-                                                 const unsigned long long*, const unsigned long long*, PairAtoms<R>, \
+                                                 const unsigned long long*, const unsigned long long*, const int*,   \
+                                                 PairAtoms<R>,                                                      \
                                                  const R*, PairParams<R>, const double4*, const double*,            \
                                                  PairParams<double>, acc_t*, acc_t*, acc_t*);                       \
     template void launch_cov_fixed_field<R>(hipStream_t, int, PairAtoms<R>, PairParams<R>, acc_t*);                \

@@ -895,7 +895,7 @@ struct mpid_context {
             launch_zero_acc(stream, force_acc.p, force_acc.n);
             launch_zero_acc(stream, torque_acc.p, torque_acc.n);
             launch_zero_acc(stream, energy_acc.p, 1);
-            launch_electrostatics_pairs<R>(stream, rows_total, brows.p, brow_owner.p, tmask.p, tmaskcc.p,
+            launch_electrostatics_pairs<R>(stream, rows_total, brows.p, brow_owner.p, tmask.p, tmaskcc.p, chg.p,
                                            pair_atoms(P), P.mu.p, pair_params<R>(), pd.pos.p, pd.lab.p,
                                            pair_params<double>(), force_acc.p, torque_acc.p, energy_acc.p);
         });

@@ -99,7 +99,7 @@ int main() {
     const PairParams<Cnt> P = params(true);
     // one-sided pieces (radial factors, field from one site) and the
     // two-sided electrostatics; radial factors with and without damping
-    long frad[2], ffrom = 0, irad[2], ifrom = 0, elec[2];
+    long frad[2], ffrom = 0, irad[2], ifrom = 0, elec[2], bare[2];
     for (int d = 0; d < 2; d++) {
         const double damp = d == 0 ? 0.316 : 0.0;  // d = 0: both sites polarisable
         R2<Cnt> ta{Cnt(1.3), Cnt(damp)}, tb{Cnt(1.3), Cnt(0.316)};
@@ -120,6 +120,11 @@ int main() {
         ElecOut<Cnt> o = elec_pair<Cnt>(dr, r2, ma, mb, ua, ub, ta, tb, true, true, Cnt(1.0), Cnt(1.0), P, false);
         (void)o;
         elec[d] = Ops::n;
+        // a pair with one bare charge (b: charge only, no polarisability)
+        Ops::n = 0;
+        ElecBareOut<Cnt> ob = elec_pair_bare<Cnt>(dr, r2, ma, mb[0], ua, ta, tb, true, Cnt(1.0), Cnt(1.0), P);
+        (void)ob;
+        bare[d] = Ops::n;
     }
     V3<Cnt> fc;
     Cnt ec;
@@ -131,6 +136,7 @@ int main() {
     std::printf("induced_field_radial %ld %ld\n", irad[0], irad[1]);
     std::printf("induced_field_from %ld %ld\n", ifrom, ifrom);
     std::printf("elec_pair %ld %ld\n", elec[0], elec[1]);
+    std::printf("elec_pair_bare %ld %ld\n", bare[0], bare[1]);
     std::printf("charge_pair %ld %ld\n", cc, cc);
     return 0;
 }

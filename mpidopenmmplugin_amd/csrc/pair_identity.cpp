@@ -1,0 +1,108 @@
+// Host check of the reduced pair formula elec_pair_bare against the general
+// elec_pair (both from pair_math.h, instantiated here in double precision).
+//
+// For a pair whose site b is a bare charge (b's dipole, quadrupole, octopole
+// and induced dipole all zero), elec_pair_bare must give the same energy,
+// force and torque on a as elec_pair, and elec_pair must give b no torque.
+// Each random pair is also evaluated with the sites' roles swapped
+// (dr -> -dr, a <-> b), which puts the pair in a different quasi-internal
+// frame: the interaction of a point charge with a's moments does not depend
+// on the frame, so the two must agree to rounding.
+//
+// Prints, per output, the largest difference over all pairs relative to the
+// largest magnitude of that output (energy_rel, force_rel, torque_rel), the
+// largest torque given to a bare charge, the magnitudes, and max_rel_diff.
+// Build and run:  make -C mpidopenmmplugin_amd/csrc pair_identity
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <random>
+
+#include "pair_math.h"
+
+using namespace mpid;
+
+static PairParams<double> params(int fix_torque) {
+    PairParams<double> P{};
+    const double alpha = 2.9203, rc = 0.9;
+    P.box.L[0] = P.box.L[1] = P.box.L[2] = 9.932;
+    P.box.inv[0] = P.box.inv[1] = P.box.inv[2] = 1.0 / 9.932;
+    P.n = 2;
+    P.rc2 = rc * rc;
+    P.alpha = alpha;
+    double aln = 1.0 / (kSqrtPi * alpha);
+    P.aln[0] = 0.0;
+    for (int k = 1; k <= 4; k++) {
+        aln *= 2.0 * alpha * alpha;
+        P.aln[k] = aln;
+    }
+    P.default_thole = 1.3;
+    P.scale14 = 1.0;
+    P.mutual = 1;
+    P.fix_torque = fix_torque;
+    return P;
+}
+
+namespace {
+struct MaxRel {
+    double diff = 0.0, mag = 0.0;
+    void add(double a, double b) {
+        diff = std::max(diff, std::fabs(a - b));
+        mag = std::max(mag, std::max(std::fabs(a), std::fabs(b)));
+    }
+    double rel() const { return mag > 0.0 ? diff / mag : diff; }
+};
+}  // namespace
+
+int main() {
+    std::mt19937_64 rng(20261202);
+    std::uniform_real_distribution<double> U(-1.0, 1.0);
+    MaxRel de, df, dt, dtb;
+    for (int trial = 0; trial < 4000; trial++) {
+        const PairParams<double> P = params(trial & 1);
+        V3<double> dr;
+        const double len = 0.1 + 0.75 * (0.5 + 0.5 * U(rng));
+        if (trial % 97 == 0) {
+            dr = mk(len, 0.0, 0.0);  // r along the lab x axis: the frame's other branch
+        } else {
+            dr = mk(U(rng), U(rng), U(rng));
+            dr = dr * (len / std::sqrt(dot(dr, dr)));
+        }
+        const double r2 = dot(dr, dr);
+        double ma[kMP], mb[kMP];
+        ma[0] = 0.8 * U(rng);
+        for (int k = 1; k < 4; k++) ma[k] = 0.02 * U(rng);
+        for (int k = 4; k < 10; k++) ma[k] = 1e-3 * U(rng);
+        for (int k = 10; k < kMP; k++) ma[k] = 1e-4 * U(rng);
+        for (int k = 0; k < kMP; k++) mb[k] = 0.0;
+        mb[0] = 0.5 * U(rng);
+        const V3<double> mua = mk(0.01 * U(rng), 0.01 * U(rng), 0.01 * U(rng)), zero = mk(0.0, 0.0, 0.0);
+        // damping factors: both present, or b's zero (a bare site's usual value)
+        const double dmpa = 0.3 + 0.05 * U(rng), dmpb = (trial % 3 == 0) ? 0.0 : 0.2 + 0.05 * U(rng);
+        const R2<double> ta{0.39, dmpa}, tb{0.41, dmpb};
+        const bool anisoA = (trial % 4) >= 2;
+        const double ms = (trial % 5 == 0) ? 0.5 : 1.0;
+        const double ps = (trial % 7 == 0) ? 0.0 : ((trial % 7 == 1) ? 0.5 : 1.0);
+
+        const ElecOut<double> g = elec_pair<double>(dr, r2, ma, mb, mua, zero, ta, tb, anisoA, false, ms, ps, P, false);
+        const ElecBareOut<double> b = elec_pair_bare<double>(dr, r2, ma, mb[0], mua, ta, tb, anisoA, ms, ps, P);
+        // roles swapped: the bare charge is site a of the general formula
+        const ElecOut<double> s =
+            elec_pair<double>(dr * -1.0, r2, mb, ma, zero, mua, tb, ta, false, anisoA, ms, ps, P, false);
+        de.add(g.energy, b.energy);
+        de.add(s.energy, b.energy);
+        df.add(g.f.x, b.f.x); df.add(g.f.y, b.f.y); df.add(g.f.z, b.f.z);
+        df.add(-s.f.x, b.f.x); df.add(-s.f.y, b.f.y); df.add(-s.f.z, b.f.z);
+        dt.add(g.ti.x, b.ta.x); dt.add(g.ti.y, b.ta.y); dt.add(g.ti.z, b.ta.z);
+        dt.add(s.tj.x, b.ta.x); dt.add(s.tj.y, b.ta.y); dt.add(s.tj.z, b.ta.z);
+        // the bare charge takes no torque in either orientation
+        dtb.add(g.tj.x, 0.0); dtb.add(g.tj.y, 0.0); dtb.add(g.tj.z, 0.0);
+        dtb.add(s.ti.x, 0.0); dtb.add(s.ti.y, 0.0); dtb.add(s.ti.z, 0.0);
+    }
+    const double worst = std::max(std::max(de.rel(), df.rel()), dt.rel());
+    std::printf("energy_rel %.3e\nforce_rel %.3e\ntorque_rel %.3e\nbare_torque_abs %.3e\n", de.rel(), df.rel(),
+                dt.rel(), dtb.diff);
+    std::printf("energy_mag %.3e\nforce_mag %.3e\ntorque_mag %.3e\n", de.mag, df.mag, dt.mag);
+    std::printf("max_rel_diff %.3e\n", worst);
+    return 0;
+}

@@ -252,20 +252,14 @@ template <class R> struct ElecOut {
     R energy;
 };
 
-template <class R>
-MPID_HD ElecOut<R> elec_pair(V3<R> dr, R r2, const R* ma, const R* mb, V3<R> mua, V3<R> mub,
-                                                R2<R> tda, R2<R> tdb, bool anisoA, bool anisoB, R ms, R ps,
-                                                const PairParams<R>& P, bool with_cc = true) {
-    const R rinv = rsqrt_r(r2);
-    const R r = r2 * rinv;
-    // QI frame (formQIRotationMatrix, MPIDReferenceForce.cpp:650-681): z along r,
-    // x = the lab x axis made orthogonal to z (the lab y axis when r lies on x).
-    // Written as (s, -zx zy / s, -zx zz / s), s = sqrt(zy^2 + zz^2), which is
-    // the same vector without the cancellation of 1 - zx^2.  The choice of x
-    // matters: the reference's induced-induced z torque is not invariant
-    // under rotations about r (see P.fix_torque).
-    const V3<R> ez = dr * rinv;
-    V3<R> ex;
+// QI frame (formQIRotationMatrix, MPIDReferenceForce.cpp:650-681): z along r,
+// x = the lab x axis made orthogonal to z (the lab y axis when r lies on x).
+// Written as (s, -zx zy / s, -zx zz / s), s = sqrt(zy^2 + zz^2), which is the
+// same vector without the cancellation of 1 - zx^2.  The choice of x matters:
+// the reference's induced-induced z torque is not invariant under rotations
+// about r (see PairParams::fix_torque).
+template <class R> MPID_HD void qi_frame(V3<R> dr, R rinv, V3<R>& ex, V3<R>& ey, V3<R>& ez) {
+    ez = dr * rinv;
     const R syz2 = ez.y * ez.y + ez.z * ez.z;
     if (dr.y != R(0) || dr.z != R(0)) {
         const R is = rsqrt_r(syz2);
@@ -273,7 +267,17 @@ MPID_HD ElecOut<R> elec_pair(V3<R> dr, R r2, const R* ma, const R* mb, V3<R> mua
     } else {
         ex = mk(R(0), R(1), R(0));
     }
-    const V3<R> ey = cross(ez, ex);
+    ey = cross(ez, ex);
+}
+
+template <class R>
+MPID_HD ElecOut<R> elec_pair(V3<R> dr, R r2, const R* ma, const R* mb, V3<R> mua, V3<R> mub,
+                                                R2<R> tda, R2<R> tdb, bool anisoA, bool anisoB, R ms, R ps,
+                                                const PairParams<R>& P, bool with_cc = true) {
+    const R rinv = rsqrt_r(r2);
+    const R r = r2 * rinv;
+    V3<R> ex, ey, ez;
+    qi_frame(dr, rinv, ex, ey, ez);
     R qa[16], qb[16];
     qi_moments(ma, ex, ey, ez, qa);
     qi_moments(mb, ex, ey, ez, qb);
@@ -549,6 +553,93 @@ MPID_HD ElecOut<R> elec_pair(V3<R> dr, R r2, const R* ma, const R* mb, V3<R> mua
     o.f = ex * qfx + ey * qfy + ez * qfz;
     o.ti = ex * tix + ey * tiy + ez * tiz;
     o.tj = ex * tjx + ey * tjy + ez * tjz;
+    return o;
+}
+
+template <class R> struct ElecBareOut {
+    V3<R> f;   // force on b (force on a is -f)
+    V3<R> ta;  // torque on a (b, a point charge, takes none)
+    R energy;
+};
+
+// elec_pair with with_cc = false for a pair whose site b is a bare charge:
+// qb[k] = 0 for k >= 1 and no induced dipole (no polarisability).  Every
+// term of elec_pair with a factor qb[k >= 1] or ub vanishes, and so do b's
+// torque and all induced-induced terms; what remains is b's charge against
+// a's dipole, quadrupole, octopole (the m = 0 components along r) and a's
+// induced dipole.  Same formulas, same order of operations, for those terms.
+template <class R>
+MPID_HD ElecBareOut<R> elec_pair_bare(V3<R> dr, R r2, const R* ma, R qb, V3<R> mua, R2<R> tda, R2<R> tdb,
+                                       bool anisoA, R ms, R ps, const PairParams<R>& P) {
+    const R rinv = rsqrt_r(r2);
+    const R r = r2 * rinv;
+    V3<R> ex, ey, ez;
+    qi_frame(dr, rinv, ex, ey, ez);
+    R qa[16];
+    qi_moments(ma, ex, ey, ez, qa);
+    const R ua0 = R(0.5) * dot(ez, mua), ua1 = R(0.5) * dot(ex, mua), ua2 = R(0.5) * dot(ey, mua);
+
+    R ri[6];
+    ri[1] = R(kElectric) * rinv;
+#pragma unroll
+    for (int k = 2; k < 6; k++) ri[k] = ri[k - 1] * rinv;
+    R ar[8];
+    ar[1] = P.alpha * r;
+#pragma unroll
+    for (int k = 2; k < 8; k++) ar[k] = ar[k - 1] * ar[1];
+    const R eX = exp_r(-ar[2]);
+    const R X = R(2.0 / kSqrtPi) * eX;
+    const R b2 = erfc_r(ar[1], eX) + ar[1] * X;
+    const R b3 = b2 + R(2.0 / 3.0) * ar[3] * X;
+    const R m1 = ms - R(1);
+    const R mB2 = m1 + b2, mB3 = m1 + b3;
+    const R dmp = tda.y * tdb.y;
+    const R a = (ps == R(0)) ? tda.x + tdb.x : P.default_thole;
+    const R u = fabs(dmp) > R(1e-5) ? div_r(r, dmp) : R(1e10);
+    const R au = fmin(a * u, R(50));
+    const R e = au < R(50) ? exp_r(-au) : R(0);
+    const R au2 = au * au, au3 = au2 * au;
+    const R base = R(1) + au + R(0.5) * au2;
+    const R t_c = -e * base;
+    const R dt_c = -e * (base + au3 * R(0.25));
+    const R p1 = ps - R(1);
+    const R X3 = ar[3] * X, X5 = ar[5] * X, X7 = ar[7] * X;
+
+    // C-D, C-Uind (m=0)
+    R eP = ri[2] * mB2;
+    const R eU = R(2) * ri[2] * (p1 + ps * t_c + b2);
+    R dP = -ri[3] * (mB2 + X3);
+    const R dU = R(-4) * ri[3] * (p1 + ps * dt_c + b2 + X3);
+    const R Va1 = eP * qb, VaR1 = dP * qb, Vad0 = eU * qb;
+    R Vb0 = eP * qa[1] + eU * ua0;
+    R VbR0 = dP * qa[1] + dU * ua0;
+    // C-Q (m=0)
+    eP = mB3 * ri[3];
+    dP = R(-1.0 / 3.0) * ri[4] * (R(4.5) * mB3 + R(2) * X5);
+    const R Va4 = eP * qb, VaR4 = dP * qb;
+    Vb0 += eP * qa[4];
+    VbR0 += dP * qa[4];
+    // C-O (m=0)
+    eP = ri[4] * (-mB3 - R(4.0 / 15.0) * X5);
+    dP = R(2.0 / 15.0) * ri[5] * (R(15) * mB3 + R(2) * (R(2) * X5 + X7));
+    const R Va9 = -eP * qb, VaR9 = -dP * qb;
+    Vb0 -= eP * qa[9];
+    VbR0 -= dP * qa[9];
+
+    ElecBareOut<R> o;
+    o.energy = R(0.5) * (qa[1] * Va1 + qa[4] * Va4 + qa[9] * Va9 + qb * Vb0);
+    const R fIZ = qa[1] * VaR1 + qa[4] * VaR4 + qa[9] * VaR9, fJZ = qb * VbR0;
+    // qi_torque_sums with V nonzero at 1, 4 and 9 only (the z sum vanishes)
+    const R EIX = qa[3] * Va1 + R(1.7320508075688772) * qa[6] * Va4 + R(2.449489742783178) * qa[11] * Va9;
+    const R EIY = -qa[2] * Va1 - R(1.7320508075688772) * qa[5] * Va4 - R(2.449489742783178) * qa[10] * Va9;
+    const R iEIX = ua2 * Vad0, iEIY = -ua1 * Vad0;
+    const R qfx = rinv * (EIY + iEIY);
+    const R qfy = -rinv * (EIX + iEIX);
+    const R qfz = -(fJZ + fIZ);
+    R tix = -EIX, tiy = -EIY;
+    if (anisoA) { tix -= iEIX; tiy -= iEIY; }
+    o.f = ex * qfx + ey * qfy + ez * qfz;
+    o.ta = ex * tix + ey * tiy;
     return o;
 }
 
