@@ -39,12 +39,14 @@ template <class R> struct PairAtoms {
     const int* npart;      // [n]
 };
 
-// Full per-atom neighbour lists (kernels_nlist.hip).  Entry k of atom i is at
-// ((k / kNlSplit) * n + i) * kNlSplit + k % kNlSplit.
 constexpr int kNlSplit = 4;  // threads per atom in the list-driven kernels
+// Per-atom neighbour lists (kernels_nlist.hip).  Entry k of list row l sits at
+// ((k / kNlSplit) * rows + l) * kNlSplit + k % kNlSplit; a kernel given an
+// atoms index reads row t for atoms[t] (rows = the index's length), one
+// without reads row i for atom i (rows = n).
 struct NbrList {
-    const int* nbr;    // [n * cap] sorted indices of the neighbours
-    const int* count;  // [n]
+    const int* nbr;    // sorted indices of the neighbours
+    const int* count;  // [n], by atom index
 };
 
 // ---- atoms / frames / solver vectors (kernels_atoms.hip) ---------------------
@@ -198,10 +200,19 @@ void launch_block_rows(hipStream_t s, int nb, const float4* bmin, const float4* 
                        const int* start, int* counts, int* rows, int* owner);
 void launch_exclusive_scan(hipStream_t s, int m, const int* counts, int* start);
 // pos.w of the FP64 positions is the damping factor: nonzero iff the site is polarisable
-void launch_nl_build(hipStream_t s, int n, const int* rstart, const int* rows, const float4* bmin, const float4* bmax,
-                     const double4* pos, const int* partners, const int* npart, Box<double> box, double rc2, int cap,
-                     int* nbr, int* count, int* maxcount, unsigned long long* tmask, int* pnbr, int* pcount,
-                     int* polcnt, const int* chg, unsigned long long* tmaskcc);
+// Masks, then lists: all = true gives the full list of every atom stored by
+// atom index; all = false the full and polarisable-neighbour lists of the
+// polarisable atoms, stored by position in their sorted index (polstart,
+// the exclusive scan of polcnt, is written on the way).  nmask: 64 words per
+// row entry (nrows entries); polmask: one word per block; scalars[0]:
+// largest stored list length, scalars[1]: nonzero if a block pair lacked its
+// mirror entry.
+void launch_nl_build(hipStream_t s, int n, int nrows, bool all, const int* rstart, const int* rows,
+                     const int* owner, const float4* bmin, const float4* bmax, const double4* pos,
+                     const int* partners, const int* npart, Box<double> box, double rc2, int cap,
+                     unsigned long long* nmask, unsigned long long* polmask, int* polcnt, int* polstart, int* nbr,
+                     int* count, int* scalars, unsigned long long* tmask, int* pnbr, int* pcount, const int* chg,
+                     unsigned long long* tmaskcc);
 void launch_pol_fill(hipStream_t s, int n, const double4* pos, const int* start, int* idx);
 // fields at the atoms atoms[0..natoms) (all atoms when atoms == nullptr, natoms = n)
 template <class R>

@@ -9,24 +9,30 @@
 // Construction, all in the Morton-sorted order of the atoms:
 //   1. block rows: for every 64-atom block I, the blocks J whose bounding
 //      boxes come within the cutoff (both J < I and J >= I), in ascending J;
-//   2. atom lists: lane i of block I tests the atoms of each J in its row and
-//      appends those within the cutoff, skipping itself and its scaled
-//      covalent partners (those pairs go to the covalent-pair kernels).
+//   2. neighbour masks: one wavefront per block pair (I, J >= I) tests each
+//      pair once and writes the 64-bit masks of both directions (the second
+//      by a bit-matrix transpose), skipping self pairs and scaled covalent
+//      partners (those pairs go to the covalent-pair kernels);
+//   3. atom lists: lane i of block I walks its row in ascending J and appends
+//      the atoms of each mask.
 // Both steps are deterministic, so the summation order of every field is.
-// A second list keeps only the polarisable neighbours: an atom without
+// The solver needs fields at polarisable sites only: an atom without
 // polarisability carries no induced dipole (mu = S y with S = 0) and its
-// row of the solver's operator is multiplied by S = 0, so the solver's field
-// passes run over the polarisable atoms and their polarisable neighbours only.
+// row of the solver's operator is multiplied by S = 0.  An evaluation
+// therefore lists the polarisable atoms only, each with its full list (for
+// the permanent field) and a list of its polarisable neighbours (for the
+// solver's operator); the inspection entry points, which want fields at
+// every atom, ask for full lists of all atoms.
 //
-// List layout: entry k of atom i sits at ((k / S) * n + i) * S + k % S with
-// S = kNlSplit, so the S threads that share an atom read S adjacent words and
-// a wavefront reads 64 consecutive words per step.
+// List layout: entry k of list row l sits at ((k / S) * rows + l) * S + k % S
+// with S = kNlSplit, so the S threads that share an atom read S adjacent
+// words and a wavefront reads 64 consecutive words per step.
 #include "pair_math.h"
 
 namespace mpid {
 
-__device__ __forceinline__ size_t nl_index(int i, int k, int n) {
-    return ((size_t)(k / kNlSplit) * n + i) * kNlSplit + (k % kNlSplit);
+__device__ __forceinline__ size_t nl_index(int l, int k, int rows) {
+    return ((size_t)(k / kNlSplit) * rows + l) * kNlSplit + (k % kNlSplit);
 }
 
 // Bounding boxes of blocks I and J within the cutoff (minimum image between
@@ -98,111 +104,183 @@ __global__ __launch_bounds__(1024) void k_exclusive_scan(int m, const int* __res
     if (t == 1023) start[m] = part[1023];
 }
 
-// Lane i of block I collects every j within the cutoff (FP64 distances, the
-// test of the reference: r2 > cutoff^2 is out).  Entries beyond cap are
-// counted but not stored; the host grows the list and rebuilds.  For every
-// row entry r with J >= I it also writes the lane's tile masks: bit jj for
-// each listed j = J * 64 + jj with j > i, in tmaskcc[r * 64 + lane] when i
-// and j are both bare charges (chg), in tmask[r * 64 + lane] otherwise.
-__global__ __launch_bounds__(64) void k_nl_build(int n, const int* __restrict__ rstart, const int* __restrict__ rows,
-                                                 const float4* __restrict__ bmin, const float4* __restrict__ bmax,
-                                                 const double4* __restrict__ pos, const int* __restrict__ partners,
-                                                 const int* __restrict__ npart, Box<double> box, double rc2, int cap,
-                                                 int* __restrict__ nbr, int* __restrict__ count,
-                                                 int* __restrict__ maxcount, unsigned long long* __restrict__ tmask,
-                                                 int* __restrict__ pnbr, int* __restrict__ pcount,
-                                                 int* __restrict__ polcnt, const int* __restrict__ chg,
-                                                 unsigned long long* __restrict__ tmaskcc) {
+// Phase 1 of the atom lists, one wavefront per row entry r = (I, J), J >= I
+// (entries with J < I return): lane i of block I tests the atoms of J (FP64
+// distances, the test of the reference: r2 > cutoff^2 is out), skipping
+// itself and its scaled covalent partners (those pairs go to the
+// covalent-pair kernels), and writes the 64-bit neighbour mask of i within J
+// to nmask[r * 64 + lane].  For J > I the 64 x 64 bit matrix is transposed
+// (one ballot per column) into the masks of J's atoms within I, written at
+// the mirrored entry (J, I) of J's row; the block-box test is symmetric, so
+// that entry exists (if not, flag[0] is set and the host raises an error).
+// Tile masks: bit jj for each listed j = J * 64 + jj with j > i, in
+// tmaskcc[r * 64 + lane] when i and j are both bare charges (chg), in
+// tmask[r * 64 + lane] otherwise.  The diagonal entry (I, I) writes
+// polmask[I], the polarisable atoms of block I, and their number polcnt[I].
+__global__ __launch_bounds__(64) void k_nl_masks(int n, const int* __restrict__ rstart, const int* __restrict__ rows,
+                                                 const int* __restrict__ owner, const float4* __restrict__ bmin,
+                                                 const float4* __restrict__ bmax, const double4* __restrict__ pos,
+                                                 const int* __restrict__ partners, const int* __restrict__ npart,
+                                                 Box<double> box, double rc2, const int* __restrict__ chg,
+                                                 unsigned long long* __restrict__ nmask,
+                                                 unsigned long long* __restrict__ tmask,
+                                                 unsigned long long* __restrict__ tmaskcc,
+                                                 unsigned long long* __restrict__ polmask, int* __restrict__ polcnt,
+                                                 int* __restrict__ flag) {
     __shared__ double4 sp[kTile];
-    const int I = blockIdx.x, lane = threadIdx.x;
+    const int r = blockIdx.x, lane = threadIdx.x;
+    const int I = owner[r], J = rows[r];
+    if (J < I) return;  // uniform: written by the mirrored entry
+    const int i = I * kTile + lane, j = J * kTile + lane;
+    const bool iv = i < n, jv = j < n;
+    const int ic = iv ? i : n - 1, jc = jv ? j : n - 1;
+    const double4 pi = pos[ic], pj = pos[jc];
+    const bool ibare = iv && chg[ic] != 0;
+    const unsigned long long jbare = __ballot(jv && chg[jc] != 0);
+    if (J == I) {
+        // the polarisable atoms of the block (damping factor nonzero)
+        const unsigned long long pm = __ballot(jv && pj.w != 0.0);
+        if (lane == 0) {
+            polmask[I] = pm;
+            polcnt[I] = __popcll(pm);
+        }
+    }
+    sp[lane] = pj;
+    // candidates: atoms of J whose distance to block I's box is within the
+    // cutoff (minimum image; never rejects a pair within the cutoff)
+    bool cand;
+    {
+        const float4 b0 = bmin[I], b1 = bmax[I];
+        const float cI[3] = {0.5f * (b0.x + b1.x), 0.5f * (b0.y + b1.y), 0.5f * (b0.z + b1.z)};
+        const float hI[3] = {0.5f * (b1.x - b0.x), 0.5f * (b1.y - b0.y), 0.5f * (b1.z - b0.z)};
+        const float rcf = sqrtf((float)rc2) * 1.0001f + 1e-5f;
+        const float L[3] = {(float)box.L[0], (float)box.L[1], (float)box.L[2]};
+        const float inv[3] = {(float)box.inv[0], (float)box.inv[1], (float)box.inv[2]};
+        const float d[3] = {(float)pj.x - cI[0], (float)pj.y - cI[1], (float)pj.z - cI[2]};
+        float d2 = 0.f;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const float dk = d[k] - L[k] * floorf(d[k] * inv[k] + 0.5f);
+            const float g = fmaxf(0.f, fabsf(dk) - hI[k]);
+            d2 += g * g;
+        }
+        cand = jv && d2 <= rcf * rcf;
+    }
+    unsigned long long cmask = __ballot(cand);
+    unsigned long long skip = J == I ? 1ull << lane : 0ull;
+    const int np = iv ? npart[ic] : 0;
+    const int* parts = partners + (size_t)ic * kMaxPartners;
+    for (int k = 0; k < np; k++) {
+        const int p = parts[k] >> 2;
+        if ((p >> 6) == J) skip |= 1ull << (p & 63);
+    }
+    if (!iv) cmask = 0;  // lanes past the last atom list nothing
+    __syncthreads();
+    unsigned long long m = 0;
+    while (cmask) {
+        const int jj = __ffsll((long long)cmask) - 1;
+        cmask &= cmask - 1ull;
+        const double4 q = sp[jj];
+        const V3<double> dr = min_image(mk(q.x - pi.x, q.y - pi.y, q.z - pi.z), box);
+        if (dot(dr, dr) <= rc2) m |= 1ull << jj;
+    }
+    m &= ~skip;
+    nmask[(size_t)r * kTile + lane] = m;
+    // tile-mask bits: j > i, i.e. all of J when J > I, those above the lane when J == I;
+    // pairs of two bare charges go to their own list (their interaction is q q erfc(a r) / r)
+    const unsigned long long upper = J > I ? ~0ull : ~((2ull << lane) - 1ull);
+    const unsigned long long cc = ibare ? (m & jbare) : 0ull;
+    tmask[(size_t)r * kTile + lane] = (m & ~cc) & upper;
+    tmaskcc[(size_t)r * kTile + lane] = cc & upper;
+    if (J > I) {
+        unsigned long long t = 0;
+        for (int k = 0; k < kTile; k++) {
+            const unsigned long long col = __ballot((m >> k) & 1ull);
+            if (lane == k) t = col;
+        }
+        // the mirrored entry: I in the ascending row of J
+        int lo = rstart[J], hi = rstart[J + 1] - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (rows[mid] < I) lo = mid + 1;
+            else hi = mid;
+        }
+        if (lo <= hi && rows[lo] == I) nmask[(size_t)lo * kTile + lane] = t;
+        else if (lane == 0) atomicOr(flag, 1);
+    }
+}
+
+// Phase 2, one workgroup of kFillWaves wavefronts per block I: the row of I
+// is cut into kFillWaves contiguous chunks in ascending J.  Each wavefront
+// counts its chunk's entries per atom, a prefix over the chunks gives each
+// its offsets, and lane i then appends the atoms of each mask in ascending
+// order.  all != 0: the full list of every atom, stored by atom index.
+// all == 0: for the polarisable atoms only, the full list and the list of
+// polarisable neighbours (polmask of the neighbour's block), stored by the
+// atom's position p in the sorted index of polarisable atoms
+// (p = polstart[I] + rank in the block, npol = polstart[nb] rows).
+// count[i] (and pcount[i] when all == 0) are stored for every atom.  Entries
+// beyond cap are counted but not stored; maxcount takes the largest stored
+// list's length, and the host grows the lists and rebuilds.
+constexpr int kFillWaves = 16;
+__global__ __launch_bounds__(kFillWaves * kTile) void k_nl_fill(
+    int n, int all, const int* __restrict__ rstart, const int* __restrict__ rows,
+    const unsigned long long* __restrict__ polmask, const int* __restrict__ polstart,
+    const unsigned long long* __restrict__ nmask, int cap, int* __restrict__ nbr, int* __restrict__ count,
+    int* __restrict__ maxcount, int* __restrict__ pnbr, int* __restrict__ pcount) {
+    __shared__ int scnt[kFillWaves][kTile], spcnt[kFillWaves][kTile];
+    const int I = blockIdx.x, lane = threadIdx.x & (kTile - 1), w = threadIdx.x / kTile;
+    const int nb = (n + kTile - 1) / kTile;
     const int i = I * kTile + lane;
     const bool iv = i < n;
-    const int ic = iv ? i : n - 1;
-    const double4 pi = pos[ic];
-    const int* parts = partners + (size_t)ic * kMaxPartners;
-    const int np = iv ? npart[ic] : 0;
-    // bounding box of block I (centre, half extent) for the candidate filter
-    const float4 b0 = bmin[I], b1 = bmax[I];
-    const float cI[3] = {0.5f * (b0.x + b1.x), 0.5f * (b0.y + b1.y), 0.5f * (b0.z + b1.z)};
-    const float hI[3] = {0.5f * (b1.x - b0.x), 0.5f * (b1.y - b0.y), 0.5f * (b1.z - b0.z)};
-    const float rcf = sqrtf((float)rc2) * 1.0001f + 1e-5f;
-    const Box<float> fbox{{(float)box.L[0], (float)box.L[1], (float)box.L[2]},
-                          {(float)box.inv[0], (float)box.inv[1], (float)box.inv[2]}};
-    const bool ibare = iv && chg[ic] != 0;
-    int cnt = 0, pcnt = 0;
+    const unsigned long long pmI = polmask[I];
+    const bool ipol = ((pmI >> lane) & 1ull) != 0;
+    const bool store = iv && (all || ipol);
+    const int li = all ? i : polstart[I] + __popcll(pmI & ((1ull << lane) - 1ull));
+    const int nrows = all ? n : polstart[nb];
     const int r0 = rstart[I], r1 = rstart[I + 1];
-    // the next row's atom is loaded one row ahead, so its latency overlaps the current row
-    int jnext = r0 < r1 ? min(rows[r0] * kTile + lane, n - 1) : 0;
-    double4 pnext = pos[jnext];
-    int cnext = chg[jnext];
-    for (int r = r0; r < r1; r++) {
-        const int J = rows[r];
-        const double4 pj = pnext;
-        const unsigned long long jbare = __ballot(cnext != 0 && J * kTile + lane < n);
-        if (r + 1 < r1) {
-            jnext = min(rows[r + 1] * kTile + lane, n - 1);
-            pnext = pos[jnext];
-            cnext = chg[jnext];
+    const int per = (r1 - r0 + kFillWaves - 1) / kFillWaves;
+    const int a = min(r0 + w * per, r1), b = min(a + per, r1);
+    int c = 0, pc = 0;
+    for (int r = a; r < b; r++) {
+        const unsigned long long m = iv ? nmask[(size_t)r * kTile + lane] : 0ull;
+        c += __popcll(m);
+        pc += __popcll(m & polmask[rows[r]]);
+    }
+    scnt[w][lane] = c;
+    spcnt[w][lane] = pc;
+    __syncthreads();
+    int cnt = 0, pcnt = 0, tot = 0, ptot = 0;
+    for (int v = 0; v < kFillWaves; v++) {
+        if (v == w) {
+            cnt = tot;
+            pcnt = ptot;
         }
-        __syncthreads();
-        // each lane stages one atom of J and tests it against block I's box
-        // (minimum image of the distance to the box; never rejects a pair
-        // within the cutoff, also when the box is wide)
-        bool cand;
-        {
-            const int j = J * kTile + lane;
-            sp[lane] = pj;
-            const float d[3] = {(float)pj.x - cI[0], (float)pj.y - cI[1], (float)pj.z - cI[2]};
-            float d2 = 0.f;
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                const float dk = d[k] - fbox.L[k] * floorf(d[k] * fbox.inv[k] + 0.5f);
-                const float g = fmaxf(0.f, fabsf(dk) - hI[k]);
-                d2 += g * g;
-            }
-            cand = j < n && d2 <= rcf * rcf;
-        }
-        unsigned long long cmask = __ballot(cand);
-        __syncthreads();
-        unsigned long long skip = 0;
-        for (int k = 0; k < np; k++) {
-            const int j = parts[k] >> 2;
-            if ((j >> 6) == J) skip |= 1ull << (j & 63);
-        }
-        if (J == I) skip |= 1ull << lane;
-        if (!iv) cmask = 0;  // lanes past the last atom list nothing
-        // tile-mask bits: j > i, i.e. all of J when J > I, those above the lane when J == I
-        const unsigned long long upper = J > I ? ~0ull : (J == I ? ~((2ull << lane) - 1ull) : 0ull);
-        unsigned long long tm = 0;
-        while (cmask) {
-            const int jj = __ffsll((long long)cmask) - 1;
-            cmask &= cmask - 1ull;
-            if ((skip >> jj) & 1ull) continue;
-            const double4 pj = sp[jj];
-            const V3<double> dr = min_image(mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z), box);
-            if (dot(dr, dr) > rc2) continue;
-            if (cnt < cap) nbr[nl_index(i, cnt, n)] = J * kTile + jj;
-            cnt++;
-            tm |= 1ull << jj;
-            if (pj.w != 0.0) {  // j polarisable (damping factor nonzero)
-                if (pcnt < cap) pnbr[nl_index(i, pcnt, n)] = J * kTile + jj;
-                pcnt++;
+        tot += scnt[v][lane];
+        ptot += spcnt[v][lane];
+    }
+    if (store) {
+        for (int r = a; r < b; r++) {
+            unsigned long long m = nmask[(size_t)r * kTile + lane];
+            const int J = rows[r];
+            const unsigned long long polJ = all ? 0ull : polmask[J];
+            while (m) {
+                const int jj = __ffsll((long long)m) - 1;
+                m &= m - 1ull;
+                if (cnt < cap) nbr[nl_index(li, cnt, nrows)] = J * kTile + jj;
+                cnt++;
+                if ((polJ >> jj) & 1ull) {
+                    if (pcnt < cap) pnbr[nl_index(li, pcnt, nrows)] = J * kTile + jj;
+                    pcnt++;
+                }
             }
         }
-        if (J >= I) {
-            // pairs of two bare charges go to their own list (their interaction is q q erfc(a r) / r)
-            const unsigned long long cc = ibare ? (tm & jbare) : 0ull;
-            tmask[(size_t)r * kTile + lane] = (tm & ~cc) & upper;
-            tmaskcc[(size_t)r * kTile + lane] = cc & upper;
-        }
     }
-    if (iv) {
-        count[i] = min(cnt, cap);
-        pcount[i] = min(pcnt, cap);
-        atomicMax(maxcount, cnt);
+    if (w == 0 && iv) {
+        count[i] = min(tot, cap);
+        if (!all) pcount[i] = min(ptot, cap);
+        if (store) atomicMax(maxcount, tot);
     }
-    const unsigned long long pol = __ballot(iv && pi.w != 0.0);
-    if (lane == 0) polcnt[I] = __popcll(pol);
 }
 
 // Sorted indices of the polarisable atoms, in order: block I writes its
@@ -237,8 +315,10 @@ __global__ __launch_bounds__(256) void k_nl_fixed_field(NbrList L, const int* __
     for (int k = 0; k < kMP; k++) mi[k] = A.lab[(size_t)ic * kMP + k];
     const int cnt = iv ? L.count[ic] : 0;
     // entry k of atom i: lp advances by n * kNlSplit per step of kNlSplit
-    const int* lp = L.nbr + (size_t)ic * kNlSplit + sub;
-    const size_t stride = (size_t)n * kNlSplit;
+    // lists of an atoms index are stored by position in the index
+    const int li = atoms ? (iv ? t : natoms - 1) : ic;
+    const int* lp = L.nbr + (size_t)li * kNlSplit + sub;
+    const size_t stride = (size_t)(atoms ? natoms : n) * kNlSplit;
     int jn = sub < cnt ? lp[0] : 0;
     R ex = 0, ey = 0, ez = 0;
     for (int k = sub; k < cnt; k += kNlSplit) {
@@ -282,8 +362,10 @@ __global__ __launch_bounds__(256) void k_nl_induced_field(NbrList L, const int* 
     const int ic = i;
     const R4<R> pi = A.pos[ic];
     const int cnt = iv ? L.count[ic] : 0;
-    const int* lp = L.nbr + (size_t)ic * kNlSplit + sub;
-    const size_t stride = (size_t)n * kNlSplit;
+    // lists of an atoms index are stored by position in the index
+    const int li = atoms ? (iv ? t : natoms - 1) : ic;
+    const int* lp = L.nbr + (size_t)li * kNlSplit + sub;
+    const size_t stride = (size_t)(atoms ? natoms : n) * kNlSplit;
     int jn = sub < cnt ? lp[0] : 0;
     R ex = 0, ey = 0, ez = 0;
     for (int k = sub; k < cnt; k += kNlSplit) {
@@ -324,13 +406,19 @@ void launch_block_rows(hipStream_t s, int nb, const float4* bmin, const float4* 
 void launch_exclusive_scan(hipStream_t s, int m, const int* counts, int* start) {
     hipLaunchKernelGGL(k_exclusive_scan, dim3(1), dim3(1024), 0, s, m, counts, start);
 }
-void launch_nl_build(hipStream_t s, int n, const int* rstart, const int* rows, const float4* bmin, const float4* bmax,
-                     const double4* pos, const int* partners, const int* npart, Box<double> box, double rc2, int cap,
-                     int* nbr, int* count, int* maxcount, unsigned long long* tmask, int* pnbr, int* pcount,
-                     int* polcnt, const int* chg, unsigned long long* tmaskcc) {
+void launch_nl_build(hipStream_t s, int n, int nrows, bool all, const int* rstart, const int* rows,
+                     const int* owner, const float4* bmin, const float4* bmax, const double4* pos,
+                     const int* partners, const int* npart, Box<double> box, double rc2, int cap,
+                     unsigned long long* nmask, unsigned long long* polmask, int* polcnt, int* polstart, int* nbr,
+                     int* count, int* scalars, unsigned long long* tmask, int* pnbr, int* pcount, const int* chg,
+                     unsigned long long* tmaskcc) {
     const int nb = (n + kTile - 1) / kTile;
-    hipLaunchKernelGGL(k_nl_build, dim3(nb), dim3(kTile), 0, s, n, rstart, rows, bmin, bmax, pos, partners, npart, box,
-                       rc2, cap, nbr, count, maxcount, tmask, pnbr, pcount, polcnt, chg, tmaskcc);
+    if (nrows > 0)
+        hipLaunchKernelGGL(k_nl_masks, dim3(nrows), dim3(kTile), 0, s, n, rstart, rows, owner, bmin, bmax, pos,
+                           partners, npart, box, rc2, chg, nmask, tmask, tmaskcc, polmask, polcnt, scalars + 1);
+    launch_exclusive_scan(s, nb, polcnt, polstart);
+    hipLaunchKernelGGL(k_nl_fill, dim3(nb), dim3(kFillWaves * kTile), 0, s, n, all ? 1 : 0, rstart, rows, polmask,
+                       polstart, nmask, cap, nbr, count, scalars, pnbr, pcount);
 }
 void launch_pol_fill(hipStream_t s, int n, const double4* pos, const int* start, int* idx) {
     const int nb = (n + kTile - 1) / kTile;

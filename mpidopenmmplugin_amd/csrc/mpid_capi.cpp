@@ -21,10 +21,12 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 using namespace mpid;
@@ -48,6 +50,14 @@ struct MpidError : std::runtime_error {
     } while (0)
 
 // Owning device buffer.
+// MPID_DEBUG_POISON set in the environment: every new device allocation of a
+// floating-point type is filled with NaN bytes, so that a kernel reading
+// memory that nothing wrote shows up as NaN in the results.
+inline bool debug_poison() {
+    static const bool on = std::getenv("MPID_DEBUG_POISON") != nullptr;
+    return on;
+}
+
 template <class T> struct DBuf {
     T* p = nullptr;
     size_t n = 0;
@@ -66,6 +76,7 @@ template <class T> struct DBuf {
         if (count == 0) return;
         HIP_CHECK(hipMalloc((void**)&p, count * sizeof(T)));
         n = count;
+        if (!std::is_integral<T>::value && debug_poison()) HIP_CHECK(hipMemset(p, 0xFF, count * sizeof(T)));
     }
     void upload(const T* h, size_t count, hipStream_t s) {
         resize(count);
@@ -195,8 +206,9 @@ struct mpid_context {
     // polarisable neighbours only (pnbr/pcount), and the sorted indices of the polarisable atoms
     DBuf<int> pnbr, pcount, polcnt, polstart, pol_idx;
     int npol = 0;
-    DBuf<unsigned long long> tmask, tmaskcc;
+    DBuf<unsigned long long> tmask, tmaskcc, nmask, polmask;
     int rows_total = 0;
+    bool all_lists = false;  // lists of the last build: every atom (true) or the polarisable ones
     int nl_cap = 0;
     int nl_max = 0;
     // accumulators
@@ -451,9 +463,10 @@ struct mpid_context {
         ncount.resize(n);
         pcount.resize(n);
         polcnt.resize(nb);
+        polmask.resize(nb);
         polstart.resize(nb + 1);
         pol_idx.resize(n);
-        nl_scalars.resize(1);
+        nl_scalars.resize(2);
         field_acc.resize(n3);
         force_acc.resize(n3);
         torque_acc.resize(n3);
@@ -579,7 +592,10 @@ struct mpid_context {
     }
 
     // positions (device, original order) -> sorted lab-frame state
-    void prepare(const double* d_pos) {
+    // all: build the lists of every atom (the inspection entry points, whose
+    // field passes visit all atoms) instead of those of the polarisable atoms
+    void prepare(const double* d_pos, bool all = false) {
+        all_lists = all;
         timed(P_SORT, [&] {
             launch_morton_keys(stream, n, d_pos, dbox, keys.p, vals.p);
             int rc = launch_sort_pairs(stream, sort_tmp.p, sort_tmp.n, keys.p, keys_sorted.p, vals.p, perm.p, n);
@@ -620,13 +636,16 @@ struct mpid_context {
             brow_owner.resize(cap);
             tmask.resize(cap * kTile);
             tmaskcc.resize(cap * kTile);
+            nmask.resize(cap * kTile);
         }
         stats.ntiles = (rows_total + nb) / 2;  // block pairs with J >= I
         build_neighbour_lists(nb);
     }
 
-    // Full per-atom lists.  The capacity starts from the mean number of atoms
-    // in a cutoff sphere; a list that overflows is rebuilt with a larger one.
+    // Per-atom lists: of the polarisable atoms (by position in pol_idx) for an
+    // evaluation, of every atom (by atom index) when all_lists is set.  The
+    // capacity starts from the mean number of atoms in a cutoff sphere; a
+    // list that overflows is rebuilt with a larger one.
     void build_neighbour_lists(int nb) {
         if (nl_cap == 0) {
             const double vol = dbox.L[0] * dbox.L[1] * dbox.L[2];
@@ -642,11 +661,11 @@ struct mpid_context {
             nbr.resize((size_t)n * nl_cap);
             pnbr.resize((size_t)n * nl_cap);
             timed(P_TILES, [&] {
-                HIP_CHECK(hipMemsetAsync(nl_scalars.p, 0, sizeof(int), stream));
-                launch_nl_build(stream, n, brow_start.p, brows.p, bmin.p, bmax.p, pd.pos.p, spart.p, snpart.p, dbox,
-                                opt.cutoff * opt.cutoff, nl_cap, nbr.p, ncount.p, nl_scalars.p, tmask.p, pnbr.p,
-                                pcount.p, polcnt.p, chg.p, tmaskcc.p);
-                launch_exclusive_scan(stream, nb, polcnt.p, polstart.p);
+                HIP_CHECK(hipMemsetAsync(nl_scalars.p, 0, 2 * sizeof(int), stream));
+                launch_nl_build(stream, n, rows_total, all_lists, brow_start.p, brows.p, brow_owner.p, bmin.p, bmax.p,
+                                pd.pos.p, spart.p, snpart.p, dbox, opt.cutoff * opt.cutoff, nl_cap, nmask.p,
+                                polmask.p, polcnt.p, polstart.p, nbr.p, ncount.p, nl_scalars.p, tmask.p, pnbr.p,
+                                pcount.p, chg.p, tmaskcc.p);
                 launch_pol_fill(stream, n, pd.pos.p, polstart.p, pol_idx.p);
                 if (tiled_spread) {
                     HIP_CHECK(hipMemsetAsync(tpstart.p, 0, tpstart.n * sizeof(int), stream));
@@ -656,9 +675,12 @@ struct mpid_context {
                 }
             });
             check_launch();
-            HIP_CHECK(hipMemcpyAsync(&nl_max, nl_scalars.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+            int scal[2];
+            HIP_CHECK(hipMemcpyAsync(scal, nl_scalars.p, 2 * sizeof(int), hipMemcpyDeviceToHost, stream));
             HIP_CHECK(hipMemcpyAsync(&npol, polstart.p + nb, sizeof(int), hipMemcpyDeviceToHost, stream));
             HIP_CHECK(hipStreamSynchronize(stream));
+            if (scal[1] != 0) throw MpidError(MPID_ERR_HIP, "neighbour masks: a block pair lacks its mirror entry");
+            nl_max = scal[0];
             if (nl_max <= nl_cap) return;
             nl_cap = nl_max + nl_max / 8 + kNlSplit;
         }
@@ -732,7 +754,13 @@ struct mpid_context {
     // rounding of that potential would dominate the error of the total.
     // Permanent field.  The solver needs it at the polarisable sites only;
     // all_atoms computes it everywhere (the inspection entry point).
+    // the lists of the last build must be the ones the field pass reads
+    void check_lists(bool all_atoms) const {
+        if (all_atoms != all_lists)
+            throw MpidError(MPID_ERR_HIP, "internal: neighbour lists built for another set of atoms");
+    }
     template <class R> void fixed_field(Prec<R>& P, bool all_atoms = false) {
+        check_lists(all_atoms);
         recip_cycle_fixed(pd);
         timed(P_FIXED, [&] {
             launch_zero_acc(stream, field_acc.p, field_acc.n);
@@ -753,6 +781,7 @@ struct mpid_context {
     // is only used on polarisable ones, so only pairs of polarisable sites are
     // visited; all_atoms gives the field of an arbitrary mu everywhere.
     template <class R> void induced_field(Prec<R>& P, bool all_atoms = false) {
+        check_lists(all_atoms);
         timed(P_INDUCED, [&] {
             launch_zero_acc(stream, field_acc.p, field_acc.n);
             if (all_atoms)
@@ -989,7 +1018,7 @@ struct mpid_context {
     void compute_fixed_field(const double* pos, const double* boxv, double* out) {
         set_box(boxv);
         pos_in.upload(pos, 3 * (size_t)n, stream);
-        prepare(pos_in.p);
+        prepare(pos_in.p, true);
         if (opt.precision == MPID_PRECISION_MIXED) fixed_field(pf, true); else fixed_field(pd, true);
         download_unpermuted(E0.p, out);
         have_last = false;
@@ -998,7 +1027,7 @@ struct mpid_context {
     void compute_induced_field(const double* pos, const double* boxv, const double* mu, double* out) {
         set_box(boxv);
         pos_in.upload(pos, 3 * (size_t)n, stream);
-        prepare(pos_in.p);
+        prepare(pos_in.p, true);
         // mu (original order) -> sorted precision copy
         std::vector<int> pm(n);
         HIP_CHECK(hipMemcpyAsync(pm.data(), perm.p, n * sizeof(int), hipMemcpyDeviceToHost, stream));
