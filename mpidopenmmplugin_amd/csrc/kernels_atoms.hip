@@ -380,6 +380,77 @@ __global__ __launch_bounds__(kRedThreads) void k_residual_norm(int n, const doub
     }
 }
 
+// The solver's operator output fused with the CG inner product: Ap as in
+// k_matvec_finish and the partial sums of p.Ap per block (kRedBlocks blocks).
+template <class R>
+__global__ __launch_bounds__(kRedThreads) void k_matvec_finish_dot(int n, const double* __restrict__ S,
+                                                                   const acc_t* __restrict__ acc,
+                                                                   const R* __restrict__ erec,
+                                                                   const R* __restrict__ mu, double selfc,
+                                                                   const double* __restrict__ p, double* Ap,
+                                                                   double* partial) {
+    __shared__ double sh[kRedThreads];
+    double v = 0;
+    for (int s = blockIdx.x * kRedThreads + threadIdx.x; s < n; s += kRedBlocks * kRedThreads) {
+        D3 E;
+        E.x = acc_get(acc[3 * (size_t)s]) + (double)erec[3 * (size_t)s] + selfc * (double)mu[3 * (size_t)s];
+        E.y = acc_get(acc[3 * (size_t)s + 1]) + (double)erec[3 * (size_t)s + 1] +
+              selfc * (double)mu[3 * (size_t)s + 1];
+        E.z = acc_get(acc[3 * (size_t)s + 2]) + (double)erec[3 * (size_t)s + 2] +
+              selfc * (double)mu[3 * (size_t)s + 2];
+        const D3 w = sym_mul(S + 6 * (size_t)s, E);
+        const D3 pv = mk(p[3 * (size_t)s], p[3 * (size_t)s + 1], p[3 * (size_t)s + 2]);
+        const D3 a = mk(pv.x - w.x, pv.y - w.y, pv.z - w.z);
+        Ap[3 * (size_t)s] = a.x;
+        Ap[3 * (size_t)s + 1] = a.y;
+        Ap[3 * (size_t)s + 2] = a.z;
+        v += dot(pv, a);
+    }
+    v = block_sum(v, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = v;
+}
+
+// One block: the partial sums of k_cg_step give rr_new (scal[2], scal[0]) and
+// ss (scal[3]); beta = rr_new / rr_old goes to scal[7].  Thread 0 reads the
+// old scal[0] before it writes the new one.
+__global__ __launch_bounds__(kRedThreads) void k_cg_reduce_beta(const double* __restrict__ partial, double* scal) {
+    __shared__ double sh[kRedThreads];
+    double rr = 0, ss = 0;
+    for (int i = threadIdx.x; i < kRedBlocks; i += kRedThreads) {
+        rr += partial[i];
+        ss += partial[kRedBlocks + i];
+    }
+    rr = block_sum(rr, sh);
+    ss = block_sum(ss, sh);
+    if (threadIdx.x == 0) {
+        const double old = scal[0];
+        scal[7] = old != 0.0 ? rr / old : 0.0;
+        scal[0] = rr;
+        scal[2] = rr;
+        scal[3] = ss;
+    }
+}
+
+// p = r + beta p (beta = scal[7]), fused with the next operator input mu = S p.
+template <class R>
+__global__ void k_cg_direction_apply(int n, const double* __restrict__ S, const double* __restrict__ scal,
+                                     const double* __restrict__ r, double* p, R* mu) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const double beta = scal[7];
+    D3 pv;
+    pv.x = r[3 * (size_t)s] + beta * p[3 * (size_t)s];
+    pv.y = r[3 * (size_t)s + 1] + beta * p[3 * (size_t)s + 1];
+    pv.z = r[3 * (size_t)s + 2] + beta * p[3 * (size_t)s + 2];
+    p[3 * (size_t)s] = pv.x;
+    p[3 * (size_t)s + 1] = pv.y;
+    p[3 * (size_t)s + 2] = pv.z;
+    const D3 v = sym_mul(S + 6 * (size_t)s, pv);
+    mu[3 * (size_t)s] = (R)v.x;
+    mu[3 * (size_t)s + 1] = (R)v.y;
+    mu[3 * (size_t)s + 2] = (R)v.z;
+}
+
 __global__ void k_cg_direction(int m, const double* __restrict__ rr_old, const double* __restrict__ rr_new,
                                const double* __restrict__ r, double* p) {
     const double beta = rr_old[0] != 0.0 ? rr_new[0] / rr_old[0] : 0.0;
@@ -822,6 +893,19 @@ void launch_residual_norm(hipStream_t s, int n, const double* S, const double* r
     hipLaunchKernelGGL(k_residual_norm, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, S, r, partial);
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kRedThreads), 0, s, partial, 2, result);
 }
+// One CG iteration after the operator's field is complete (scal slots:
+// [0] rr, [1] pAp, [2] rr_new, [3] ss, [7] beta): Ap and pAp, x += a p,
+// r -= a Ap with their norms, then the next direction p and its mu = S p.
+template <class R>
+void launch_cg_iteration(hipStream_t s, int n, const double* S, const acc_t* acc, const R* erec, R* mu, double selfc,
+                         double* p, double* Ap, double* x, double* r, double* partial, double* scal) {
+    hipLaunchKernelGGL(k_matvec_finish_dot<R>, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, S, acc, erec, mu, selfc,
+                       p, Ap, partial);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kRedThreads), 0, s, partial, 1, scal + 1);
+    hipLaunchKernelGGL(k_cg_step, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, S, scal, p, Ap, x, r, partial);
+    hipLaunchKernelGGL(k_cg_reduce_beta, dim3(1), dim3(kRedThreads), 0, s, partial, scal);
+    hipLaunchKernelGGL(k_cg_direction_apply<R>, blocks_for(n, 256), dim3(256), 0, s, n, S, scal, r, p, mu);
+}
 void launch_cg_direction(hipStream_t s, int m, const double* rr_old, const double* rr_new, const double* r,
                          double* p) {
     int b = std::min((m + 255) / 256, 4096);
@@ -855,6 +939,8 @@ void launch_recip_atoms(hipStream_t s, int n, int mutual, const double* phi, con
     template void launch_apply_S<R>(hipStream_t, int, const double*, const double*, R*, double*);                   \
     template void launch_matvec_finish<R>(hipStream_t, int, const double*, const acc_t*, const R*, const R*, double, \
                                           const double*, double*);                                                  \
+    template void launch_cg_iteration<R>(hipStream_t, int, const double*, const acc_t*, const R*, R*, double,       \
+                                         double*, double*, double*, double*, double*, double*);                     \
     template void launch_recip_atoms<R>(hipStream_t, int, int, const double*, const R*, const double*, const double*,\
                                         const int*, double3, double, acc_t*, acc_t*, acc_t*);                      \
     template void launch_field_sum<R>(hipStream_t, int, const acc_t*, const R*, const R*, double, double*);

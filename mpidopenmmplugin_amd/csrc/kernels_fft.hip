@@ -2,9 +2,10 @@
 //
 // The charge grid Q is real, [n0][n1][n2] with z fastest (the reference's
 // x*ny*nz + y*nz + z order, MPIDReferenceForce.cpp:3269-3327).  Its spectrum
-// is kept as the half S[n0][n1][n2/2+1]: the grid is real and the influence
+// is kept as the half kz < n2/2+1: the grid is real and the influence
 // function even, so the other half is the complex conjugate and is never
-// stored.  One reciprocal cycle is five passes over the grid:
+// stored.  The half is stored plane by plane, S[kz][x][y], so that every pass
+// moves contiguous runs.  One reciprocal cycle is five passes over the grid:
 //
 //   1. z, real to half-complex   Q -> S      (forward)
 //   2. y, complex, in place      S           (forward)
@@ -127,30 +128,39 @@ template <class R> __device__ __forceinline__ void stage_twiddles(Cx<R>* tw_s, c
 }
 
 // A batch holds T*n <= kFftLdsBytes / (2 * sizeof(complex)) elements (two
-// line buffers), at most max_per_thread() per thread of a 256-thread
-// workgroup.  Loads go to registers first, all of them in flight at once, and
-// then to LDS.  The budget keeps several workgroups resident per CU so that
-// one's loads overlap another's stages.
-constexpr int kFftThreads = 256;
-constexpr int kFftLdsBytes = 16384;
+// line buffers), at most max_per_thread() per thread.  Loads go to registers
+// first, all of them in flight at once, and then to LDS.  T = 16 FP32 or 8
+// FP64 lines of 128 make every global access a run of at least 128
+// contiguous bytes (see the spectrum layout below).  A pass runs as few
+// workgroups as there are batches, so its time is one workgroup's latency:
+// the workgroup has one thread per radix-4 butterfly of a stage (512 FP32,
+// 256 FP64 threads for n = 128), so that each stage is one step per thread.
+constexpr int kFftLdsBytes = 32768;
 constexpr int kFftMaxN = 1024;  // largest grid dimension accepted (one line per batch at worst)
+template <class R> constexpr int fft_threads() { return sizeof(R) == 4 ? 512 : 256; }
 template <class R> constexpr int max_per_thread() {
     constexpr int budget = kFftLdsBytes / (2 * 2 * (int)sizeof(R));
     constexpr int elems = budget > kFftMaxN ? budget : kFftMaxN;
-    return (elems + kFftThreads - 1) / kFftThreads;
+    return (elems + fft_threads<R>() - 1) / fft_threads<R>();
 }
 
-// Pass 1 (dir = -1): lines along z, real Q -> half spectrum S.
-// Pass 5 (dir = +1): half spectrum S -> real Q (Hermitian completion).
+// Spectrum layout: S[kz][l], l = x * n1 + y, kz < n2 / 2 + 1 (N01 = n0 * n1
+// words per plane).  Pass 1 writes, for each kz, T consecutive lines l as one
+// contiguous run; pass 2 (lines along y) reads and writes whole contiguous
+// blocks of T lines; pass 3 (lines along x) reads runs of T consecutive y.
+
+// Pass 1 (dir = -1): lines l along z, real Q[l][z] -> S[kz][l].
+// Pass 5 (dir = +1): S[kz][l] -> real Q[l][z] (Hermitian completion).
+// A batch is T consecutive lines l0 .. l0 + T - 1.
 template <class R>
-__global__ __launch_bounds__(kFftThreads) void k_fft_z(R* __restrict__ Q, R2<R>* __restrict__ S, FftAxis ax, int T,
-                                               const R2<R>* __restrict__ tw, int nlines, FastDiv nh, int dir) {
+__global__ __launch_bounds__(fft_threads<R>()) void k_fft_z(R* __restrict__ Q, R2<R>* __restrict__ S, FftAxis ax, int T,
+                                                       int Tshift, const R2<R>* __restrict__ tw, int nlines, int dir) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int n = ax.n;
     Cx<R>* A = reinterpret_cast<Cx<R>*>(smem_raw);
     Cx<R>* B = A + (size_t)T * n;
     Cx<R>* W = B + (size_t)T * n;
-    const int n2h = (int)nh.d;
+    const int n2h = n / 2 + 1;
     const int l0 = blockIdx.x * T;
     const int count = min(T, nlines - l0);
     stage_twiddles(W, tw, n);
@@ -162,39 +172,41 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_z(R* __restrict__ Q, R2<R>*
         R v[MP];
 #pragma unroll
         for (int u = 0; u < MP; u++) {
-            const int idx = threadIdx.x + u * kFftThreads;
+            const int idx = threadIdx.x + u * fft_threads<R>();
             v[u] = idx < lim ? src[idx] : R(0);
         }
 #pragma unroll
         for (int u = 0; u < MP; u++) {
-            const int idx = threadIdx.x + u * kFftThreads;
+            const int idx = threadIdx.x + u * fft_threads<R>();
             if (idx < total) A[idx] = Cx<R>{v[u], R(0)};
         }
     } else {
+        // element (t, k) of the batch, t fastest: runs of T along l
         R2<R> v[MP];
         bool cj[MP];
 #pragma unroll
         for (int u = 0; u < MP; u++) {
-            const int idx = threadIdx.x + u * kFftThreads;
-            const int t = (int)fdiv((unsigned)idx, ax.nd), k = idx - t * n;
+            const int idx = threadIdx.x + u * fft_threads<R>();
+            const int k = idx >> Tshift, t = idx & (T - 1);
             cj[u] = k >= n2h;
             v[u] = R2<R>{R(0), R(0)};
-            if (idx < total && t < count) v[u] = S[(size_t)(l0 + t) * n2h + (cj[u] ? n - k : k)];
+            if (idx < total && t < count) v[u] = S[(size_t)(cj[u] ? n - k : k) * nlines + l0 + t];
         }
 #pragma unroll
         for (int u = 0; u < MP; u++) {
-            const int idx = threadIdx.x + u * kFftThreads;
-            if (idx < total) A[idx] = Cx<R>{v[u].x, cj[u] ? -v[u].y : v[u].y};
+            const int idx = threadIdx.x + u * fft_threads<R>();
+            const int k = idx >> Tshift, t = idx & (T - 1);
+            if (idx < total) A[t * n + k] = Cx<R>{v[u].x, cj[u] ? -v[u].y : v[u].y};
         }
     }
     __syncthreads();
     const Cx<R>* res = fft_in_lds<R>(A, B, ax, T, W, (R)dir);
     if (dir < 0) {
         for (int idx = threadIdx.x; idx < T * n2h; idx += blockDim.x) {
-            const int t = (int)fdiv((unsigned)idx, nh), k = idx - t * n2h;
+            const int k = idx >> Tshift, t = idx & (T - 1);
             if (t < count) {
                 const Cx<R> v = res[t * n + k];
-                S[(size_t)(l0 + t) * n2h + k] = R2<R>{v.x, v.y};
+                S[(size_t)k * nlines + l0 + t] = R2<R>{v.x, v.y};
             }
         }
     } else {
@@ -206,62 +218,57 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_z(R* __restrict__ Q, R2<R>*
     }
 }
 
-// Passes 2 and 4: lines along y of S, a batch = T consecutive kz of one x.
+// Passes 2 and 4: lines along y, a batch = T consecutive x of one kz plane,
+// a contiguous block of T * n1 words.
 template <class R>
-__global__ __launch_bounds__(kFftThreads) void k_fft_y(R2<R>* __restrict__ S, FftAxis ax, int T, int Tshift,
-                                               const R2<R>* __restrict__ tw, int n2h, int nkb, int dir) {
+__global__ __launch_bounds__(fft_threads<R>()) void k_fft_y(R2<R>* __restrict__ S, FftAxis ax, int T,
+                                                       const R2<R>* __restrict__ tw, int n0, int nxb, int dir) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int n = ax.n;
     Cx<R>* A = reinterpret_cast<Cx<R>*>(smem_raw);
     Cx<R>* B = A + (size_t)T * n;
     Cx<R>* W = B + (size_t)T * n;
-    const int x = blockIdx.x / nkb, kz0 = (blockIdx.x - x * nkb) * T;
-    const int count = min(T, n2h - kz0);
-    R2<R>* base = S + (size_t)x * n * n2h + kz0;
+    const int kz = blockIdx.x / nxb, x0 = (blockIdx.x - kz * nxb) * T;
+    const int count = min(T, n0 - x0);
+    R2<R>* base = S + ((size_t)kz * n0 + x0) * n;
     stage_twiddles(W, tw, n);
     constexpr int MP = max_per_thread<R>();
-    const int total = T * n;
+    const int lim = count * n;
     {
         R2<R> v[MP];
 #pragma unroll
         for (int u = 0; u < MP; u++) {
-            const int idx = threadIdx.x + u * kFftThreads;
-            const int k = idx >> Tshift, t = idx & (T - 1);
-            v[u] = (idx < total && t < count) ? base[(size_t)k * n2h + t] : R2<R>{R(0), R(0)};
+            const int idx = threadIdx.x + u * fft_threads<R>();
+            v[u] = idx < lim ? base[idx] : R2<R>{R(0), R(0)};
         }
 #pragma unroll
         for (int u = 0; u < MP; u++) {
-            const int idx = threadIdx.x + u * kFftThreads;
-            const int k = idx >> Tshift, t = idx & (T - 1);
-            if (idx < total) A[t * n + k] = Cx<R>{v[u].x, v[u].y};
+            const int idx = threadIdx.x + u * fft_threads<R>();
+            if (idx < T * n) A[idx] = Cx<R>{v[u].x, v[u].y};
         }
     }
     __syncthreads();
     const Cx<R>* res = fft_in_lds<R>(A, B, ax, T, W, (R)dir);
-    for (int idx = threadIdx.x; idx < T * n; idx += blockDim.x) {
-        const int k = idx >> Tshift, t = idx & (T - 1);
-        if (t < count) {
-            const Cx<R> v = res[t * n + k];
-            base[(size_t)k * n2h + t] = R2<R>{v.x, v.y};
-        }
+    for (int idx = threadIdx.x; idx < lim; idx += blockDim.x) {
+        const Cx<R> v = res[idx];
+        base[idx] = R2<R>{v.x, v.y};
     }
 }
 
-// Pass 3: lines along x of S (a batch = T consecutive kz of one y): forward
-// transform, influence function, backward transform.
+// Pass 3: lines along x (a batch = T consecutive y of one kz plane, runs of T
+// words): forward transform, influence function, backward transform.
 template <class R>
-__global__ __launch_bounds__(kFftThreads) void k_fft_x_convolve(R2<R>* __restrict__ S, FftAxis ax, int T, int Tshift,
-                                                        const R2<R>* __restrict__ tw, int n1, int n2, int n2h, int nkb,
-                                                        ConvParams<R> cp) {
+__global__ __launch_bounds__(fft_threads<R>()) void k_fft_x_convolve(R2<R>* __restrict__ S, FftAxis ax, int T, int Tshift,
+                                                                const R2<R>* __restrict__ tw, int n1, int n2, int nyb,
+                                                                ConvParams<R> cp) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int n = ax.n;
     Cx<R>* A = reinterpret_cast<Cx<R>*>(smem_raw);
     Cx<R>* B = A + (size_t)T * n;
     Cx<R>* W = B + (size_t)T * n;
-    const int y = blockIdx.x / nkb, kz0 = (blockIdx.x - y * nkb) * T;
-    const int count = min(T, n2h - kz0);
-    const size_t xs = (size_t)n1 * n2h;
-    R2<R>* base = S + (size_t)y * n2h + kz0;
+    const int kz = blockIdx.x / nyb, y0 = (blockIdx.x - kz * nyb) * T;
+    const int count = min(T, n1 - y0);
+    R2<R>* base = S + (size_t)kz * n * n1 + y0;
     stage_twiddles(W, tw, n);
     constexpr int MP = max_per_thread<R>();
     const int total = T * n;
@@ -269,13 +276,13 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_convolve(R2<R>* __restric
         R2<R> v[MP];
 #pragma unroll
         for (int u = 0; u < MP; u++) {
-            const int idx = threadIdx.x + u * kFftThreads;
+            const int idx = threadIdx.x + u * fft_threads<R>();
             const int k = idx >> Tshift, t = idx & (T - 1);
-            v[u] = (idx < total && t < count) ? base[(size_t)k * xs + t] : R2<R>{R(0), R(0)};
+            v[u] = (idx < total && t < count) ? base[(size_t)k * n1 + t] : R2<R>{R(0), R(0)};
         }
 #pragma unroll
         for (int u = 0; u < MP; u++) {
-            const int idx = threadIdx.x + u * kFftThreads;
+            const int idx = threadIdx.x + u * fft_threads<R>();
             const int k = idx >> Tshift, t = idx & (T - 1);
             if (idx < total) A[t * n + k] = Cx<R>{v[u].x, v[u].y};
         }
@@ -283,19 +290,19 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_convolve(R2<R>* __restric
     __syncthreads();
     Cx<R>* res = fft_in_lds<R>(A, B, ax, T, W, R(-1));
     // influence function exp(-pi^2 m^2 / alpha^2) / (pi V m^2 |b(m)|^2), m = 0 removed
-    const int my = y < (n1 + 1) / 2 ? y : y - n1;
-    const R hy = R(my) * cp.inv[1];
-    const R by = cp.moduli[n + y];
+    const int mz = kz < (n2 + 1) / 2 ? kz : kz - n2;
+    const R hz = R(mz) * cp.inv[2];
+    const R bz = cp.moduli[n + n1 + kz];
     for (int idx = threadIdx.x; idx < T * n; idx += blockDim.x) {
         const int kx = idx >> Tshift, t = idx & (T - 1);
-        const int kz = kz0 + t;
+        const int y = y0 + t;
         R et = R(0);
         if (t < count && (kx | y | kz) != 0) {
             const int mx = kx < (n + 1) / 2 ? kx : kx - n;
-            const int mz = kz < (n2 + 1) / 2 ? kz : kz - n2;
-            const R hx = R(mx) * cp.inv[0], hz = R(mz) * cp.inv[2];
+            const int my = y < (n1 + 1) / 2 ? y : y - n1;
+            const R hx = R(mx) * cp.inv[0], hy = R(my) * cp.inv[1];
             const R m2 = hx * hx + hy * hy + hz * hz;
-            const R den = m2 * cp.moduli[kx] * by * cp.moduli[n + n1 + kz];
+            const R den = m2 * cp.moduli[kx] * cp.moduli[n + y] * bz;
             et = cp.sc * exp_r(-cp.expf * m2) / den;
         }
         Cx<R>& v = res[t * n + kx];
@@ -309,7 +316,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_convolve(R2<R>* __restric
         const int k = idx >> Tshift, t = idx & (T - 1);
         if (t < count) {
             const Cx<R> v = out[t * n + k];
-            base[(size_t)k * xs + t] = R2<R>{v.x, v.y};
+            base[(size_t)k * n1 + t] = R2<R>{v.x, v.y};
         }
     }
 }
@@ -345,8 +352,9 @@ bool make_fft_axis(int n, FftAxis& ax) {
     return true;
 }
 
-// lines per workgroup: a power of two, two line buffers within kFftLdsBytes
-// (one line when even that exceeds it: n up to 1024 needs 16 KB in FP64)
+// lines per workgroup: a power of two up to 16, two line buffers within
+// kFftLdsBytes (one line when even that exceeds it: n up to 1024 needs 32 KB
+// in FP64)
 template <class R> static int lines_per_group(int n) {
     int T = 16;
     while (T > 1 && 2 * (size_t)T * n * sizeof(R2<R>) > (size_t)kFftLdsBytes) T >>= 1;
@@ -363,34 +371,25 @@ template <class R>
 void launch_recip_convolution(hipStream_t s, const FftPlan<R>& p, R* Q, R2<R>* S, ConvParams<R> cp) {
     const int n0 = p.ax[0].n, n1 = p.ax[1].n, n2 = p.ax[2].n;
     const int n2h = n2 / 2 + 1;
-    const FastDiv nh = make_fastdiv((unsigned)n2h);
+    const int nlines = n0 * n1;
+    const int Tz = lines_per_group<R>(n2);
+    const int Ty = lines_per_group<R>(n1), nxb = (n0 + Ty - 1) / Ty;
+    const int Tx = lines_per_group<R>(n0), nyb = (n1 + Tx - 1) / Tx;
     // 1. z forward
-    {
-        const int T = lines_per_group<R>(n2);
-        const int nlines = n0 * n1;
-        hipLaunchKernelGGL(k_fft_z<R>, dim3((nlines + T - 1) / T), dim3(kFftThreads), fft_lds<R>(T, n2), s, Q, S, p.ax[2], T,
-                           p.tw[2], nlines, nh, -1);
-    }
+    hipLaunchKernelGGL(k_fft_z<R>, dim3((nlines + Tz - 1) / Tz), dim3(fft_threads<R>()), fft_lds<R>(Tz, n2), s, Q, S,
+                       p.ax[2], Tz, log2i(Tz), p.tw[2], nlines, -1);
     // 2. y forward
-    const int Ty = lines_per_group<R>(n1), nkby = (n2h + Ty - 1) / Ty;
-    hipLaunchKernelGGL(k_fft_y<R>, dim3(n0 * nkby), dim3(kFftThreads), fft_lds<R>(Ty, n1), s, S, p.ax[1], Ty, log2i(Ty),
-                       p.tw[1], n2h, nkby, -1);
+    hipLaunchKernelGGL(k_fft_y<R>, dim3(n2h * nxb), dim3(fft_threads<R>()), fft_lds<R>(Ty, n1), s, S, p.ax[1], Ty, p.tw[1],
+                       n0, nxb, -1);
     // 3. x forward, convolution, x backward
-    {
-        const int Tx = lines_per_group<R>(n0), nkbx = (n2h + Tx - 1) / Tx;
-        hipLaunchKernelGGL(k_fft_x_convolve<R>, dim3(n1 * nkbx), dim3(kFftThreads), fft_lds<R>(Tx, n0), s, S, p.ax[0], Tx,
-                           log2i(Tx), p.tw[0], n1, n2, n2h, nkbx, cp);
-    }
+    hipLaunchKernelGGL(k_fft_x_convolve<R>, dim3(n2h * nyb), dim3(fft_threads<R>()), fft_lds<R>(Tx, n0), s, S, p.ax[0], Tx,
+                       log2i(Tx), p.tw[0], n1, n2, nyb, cp);
     // 4. y backward
-    hipLaunchKernelGGL(k_fft_y<R>, dim3(n0 * nkby), dim3(kFftThreads), fft_lds<R>(Ty, n1), s, S, p.ax[1], Ty, log2i(Ty),
-                       p.tw[1], n2h, nkby, 1);
+    hipLaunchKernelGGL(k_fft_y<R>, dim3(n2h * nxb), dim3(fft_threads<R>()), fft_lds<R>(Ty, n1), s, S, p.ax[1], Ty, p.tw[1],
+                       n0, nxb, 1);
     // 5. z backward
-    {
-        const int T = lines_per_group<R>(n2);
-        const int nlines = n0 * n1;
-        hipLaunchKernelGGL(k_fft_z<R>, dim3((nlines + T - 1) / T), dim3(kFftThreads), fft_lds<R>(T, n2), s, Q, S, p.ax[2], T,
-                           p.tw[2], nlines, nh, 1);
-    }
+    hipLaunchKernelGGL(k_fft_z<R>, dim3((nlines + Tz - 1) / Tz), dim3(fft_threads<R>()), fft_lds<R>(Tz, n2), s, Q, S,
+                       p.ax[2], Tz, log2i(Tz), p.tw[2], nlines, 1);
 }
 
 template void launch_recip_convolution<float>(hipStream_t, const FftPlan<float>&, float*, R2<float>*,

@@ -175,6 +175,17 @@ struct mpid_context {
     int n = 0;
     mpid_options opt{};
     hipStream_t stream = nullptr;
+    // The reciprocal-space chains (spread, FFT, gather) run on rstream beside
+    // the real-space kernels on stream: their launches hold few workgroups and
+    // leave most CUs free.  fork() makes rstream wait for the work issued to
+    // stream so far; join() makes stream wait for the work issued to rstream.
+    hipStream_t rstream = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // CG iterations are replayed from a graph captured once per solve (its
+    // kernel arguments are those of that solve); hscal is the pinned host
+    // copy of the residual norm that each replay leaves behind.
+    hipGraphExec_t cg_exec = nullptr;
+    double* hscal = nullptr;
     std::string error;
     mpid_stats stats{};
 
@@ -236,6 +247,11 @@ struct mpid_context {
 
     ~mpid_context() {
         for (auto e : pool) (void)hipEventDestroy(e);
+        if (cg_exec) (void)hipGraphExecDestroy(cg_exec);
+        if (hscal) (void)hipHostFree(hscal);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (rstream) (void)hipStreamDestroy(rstream);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -248,20 +264,31 @@ struct mpid_context {
         return pool[pool_used++];
     }
     int prof_override = -1;  // when set, every timed region is booked to this category
-    template <class F> void timed(int cat, F&& f) {
+    // device time of the work that f issues to stream s, booked to cat
+    template <class F> void timed_on(hipStream_t s, int cat, F&& f) {
         if (!profiling) {
             f();
             return;
         }
         if (prof_override >= 0) cat = prof_override;
         hipEvent_t a = get_event(), b = get_event();
-        HIP_CHECK(hipEventRecord(a, stream));
+        HIP_CHECK(hipEventRecord(a, s));
         f();
-        HIP_CHECK(hipEventRecord(b, stream));
+        HIP_CHECK(hipEventRecord(b, s));
         events.push_back({cat, a, b});
+    }
+    template <class F> void timed(int cat, F&& f) { timed_on(stream, cat, std::forward<F>(f)); }
+    void fork() {
+        HIP_CHECK(hipEventRecord(ev_fork, stream));
+        HIP_CHECK(hipStreamWaitEvent(rstream, ev_fork, 0));
+    }
+    void join() {
+        HIP_CHECK(hipEventRecord(ev_join, rstream));
+        HIP_CHECK(hipStreamWaitEvent(stream, ev_join, 0));
     }
     void collect_profile() {
         if (!profiling) return;
+        HIP_CHECK(hipStreamSynchronize(rstream));
         HIP_CHECK(hipStreamSynchronize(stream));
         for (auto& e : events) {
             float ms = 0.f;
@@ -593,8 +620,11 @@ struct mpid_context {
 
     // positions (device, original order) -> sorted lab-frame state
     // all: build the lists of every atom (the inspection entry points, whose
-    // field passes visit all atoms) instead of those of the polarisable atoms
-    void prepare(const double* d_pos, bool all = false) {
+    // field passes visit all atoms) instead of those of the polarisable atoms.
+    // fixed_recip: issue the FP64 reciprocal cycle of the permanent multipoles
+    // (to rstream, as soon as the lab-frame multipoles exist, so that it runs
+    // beside the neighbour-list build); fixed_field() joins it.
+    void prepare(const double* d_pos, bool all = false, bool fixed_recip = true) {
         all_lists = all;
         timed(P_SORT, [&] {
             launch_morton_keys(stream, n, d_pos, dbox, keys.p, vals.p);
@@ -619,6 +649,7 @@ struct mpid_context {
             launch_convert<double>(stream, n, spos.p, lab.p, sthole.p, damp.p, pd.pos.p, pd.lab.p, pd.td.p);
         });
         check_launch();
+        if (fixed_recip) recip_cycle_fixed(pd);
         // blocks, block rows, neighbour lists and tile masks
         const int nb = (n + kTile - 1) / kTile;
         timed(P_TILES, [&] {
@@ -698,22 +729,25 @@ struct mpid_context {
         return c;
     }
 
+    // Reciprocal cycles: issued to rstream after a fork(); the caller joins
+    // before it reads their output (phi, phidp, erec).
     template <class R> void recip_cycle_fixed(Prec<R>& P) {
         PmeGeom<R> g = geom<R>();
+        fork();
         const size_t G = P.grid.n;
-        timed(P_SPREAD, [&] {
-            launch_fill<R>(stream, P.grid.p, G, R(0));
+        timed_on(rstream, P_SPREAD, [&] {
+            launch_fill<R>(rstream, P.grid.p, G, R(0));
             if (tiled_spread) {
-                launch_spread_fixed_tiled<R>(stream, P.pos.p, P.lab.p, g, tgeom, tstart.p, tend.p, P.scratch.p,
+                launch_spread_fixed_tiled<R>(rstream, P.pos.p, P.lab.p, g, tgeom, tstart.p, tend.p, P.scratch.p,
                                              P.grid.p);
-                if (fold_by_tiles(tgeom)) launch_fold_tiles<R>(stream, g, tgeom, P.scratch.p, P.grid.p);
-                else launch_fold<R>(stream, g, tgeom, P.scratch.p, P.grid.p);
+                if (fold_by_tiles(tgeom)) launch_fold_tiles<R>(rstream, g, tgeom, P.scratch.p, P.grid.p);
+                else launch_fold<R>(rstream, g, tgeom, P.scratch.p, P.grid.p);
             } else {
-                launch_spread_fixed<R>(stream, n, P.pos.p, P.lab.p, g, P.grid.p);
+                launch_spread_fixed<R>(rstream, n, P.pos.p, P.lab.p, g, P.grid.p);
             }
         });
-        timed(P_FFT, [&] { launch_recip_convolution<R>(stream, P.plan, P.grid.p, P.spec.p, conv_params(P)); });
-        timed(P_GATHER, [&] { launch_gather_phi<R>(stream, n, P.pos.p, g, P.grid.p, P.phi.p); });
+        timed_on(rstream, P_FFT, [&] { launch_recip_convolution<R>(rstream, P.plan, P.grid.p, P.spec.p, conv_params(P)); });
+        timed_on(rstream, P_GATHER, [&] { launch_gather_phi<R>(rstream, n, P.pos.p, g, P.grid.p, P.phi.p); });
         check_launch();
     }
 
@@ -723,27 +757,28 @@ struct mpid_context {
     // gathered.  full: the 35 potential derivatives at every atom (forces).
     template <class R> void recip_cycle_induced(Prec<R>& P, bool full, bool all_atoms = false) {
         PmeGeom<R> g = geom<R>();
+        fork();
         const size_t G = P.grid.n;
-        timed(P_SPREAD, [&] {
-            launch_fill<R>(stream, P.grid.p, G, R(0));
+        timed_on(rstream, P_SPREAD, [&] {
+            launch_fill<R>(rstream, P.grid.p, G, R(0));
             if (tiled_spread) {
                 if (all_atoms)
-                    launch_spread_induced_tiled<R>(stream, P.pos.p, P.mu.p, g, tgeom, nullptr, tstart.p, tend.p,
+                    launch_spread_induced_tiled<R>(rstream, P.pos.p, P.mu.p, g, tgeom, nullptr, tstart.p, tend.p,
                                                    P.scratch.p, P.grid.p);
                 else
-                    launch_spread_induced_tiled<R>(stream, P.pos.p, P.mu.p, g, tgeom, pol_idx.p, tpstart.p, tpend.p,
+                    launch_spread_induced_tiled<R>(rstream, P.pos.p, P.mu.p, g, tgeom, pol_idx.p, tpstart.p, tpend.p,
                                                    P.scratch.p, P.grid.p);
-                if (fold_by_tiles(tgeom)) launch_fold_tiles<R>(stream, g, tgeom, P.scratch.p, P.grid.p);
-                else launch_fold<R>(stream, g, tgeom, P.scratch.p, P.grid.p);
+                if (fold_by_tiles(tgeom)) launch_fold_tiles<R>(rstream, g, tgeom, P.scratch.p, P.grid.p);
+                else launch_fold<R>(rstream, g, tgeom, P.scratch.p, P.grid.p);
             } else {
-                launch_spread_induced<R>(stream, n, P.pos.p, P.mu.p, g, P.grid.p);
+                launch_spread_induced<R>(rstream, n, P.pos.p, P.mu.p, g, P.grid.p);
             }
         });
-        timed(P_FFT, [&] { launch_recip_convolution<R>(stream, P.plan, P.grid.p, P.spec.p, conv_params(P)); });
-        timed(P_GATHER, [&] {
-            if (full) launch_gather_phi<R>(stream, n, P.pos.p, g, P.grid.p, P.phidp.p);
-            else if (all_atoms) launch_gather_field<R>(stream, n, nullptr, P.pos.p, g, P.grid.p, P.erec.p);
-            else launch_gather_field<R>(stream, npol, pol_idx.p, P.pos.p, g, P.grid.p, P.erec.p);
+        timed_on(rstream, P_FFT, [&] { launch_recip_convolution<R>(rstream, P.plan, P.grid.p, P.spec.p, conv_params(P)); });
+        timed_on(rstream, P_GATHER, [&] {
+            if (full) launch_gather_phi<R>(rstream, n, P.pos.p, g, P.grid.p, P.phidp.p);
+            else if (all_atoms) launch_gather_field<R>(rstream, n, nullptr, P.pos.p, g, P.grid.p, P.erec.p);
+            else launch_gather_field<R>(rstream, npol, pol_idx.p, P.pos.p, g, P.grid.p, P.erec.p);
         });
         check_launch();
     }
@@ -761,7 +796,8 @@ struct mpid_context {
     }
     template <class R> void fixed_field(Prec<R>& P, bool all_atoms = false) {
         check_lists(all_atoms);
-        recip_cycle_fixed(pd);
+        // the FP64 reciprocal cycle of the permanent multipoles was issued by
+        // prepare() and runs beside the neighbour lists and these kernels
         timed(P_FIXED, [&] {
             launch_zero_acc(stream, field_acc.p, field_acc.n);
             launch_nl_fixed_field<R>(stream, all_atoms ? n : npol, all_atoms ? nullptr : pol_idx.p, nlist(),
@@ -770,6 +806,7 @@ struct mpid_context {
         timed(P_COV, [&] {
             launch_cov_fixed_field<double>(stream, n, pair_atoms(pd), pair_params<double>(), field_acc.p);
         });
+        join();
         timed(P_OTHER, [&] {
             launch_fixed_field_finish<double>(stream, n, pd.phi.p, lab.p, field_acc.p, gscale3(), selfc(), E0.p);
         });
@@ -780,8 +817,11 @@ struct mpid_context {
     // solver (all_atoms false) mu vanishes on unpolarisable sites and the field
     // is only used on polarisable ones, so only pairs of polarisable sites are
     // visited; all_atoms gives the field of an arbitrary mu everywhere.
+    // The reciprocal part goes to rstream first and runs beside the pair
+    // kernels; the caller joins before it reads erec.
     template <class R> void induced_field(Prec<R>& P, bool all_atoms = false) {
         check_lists(all_atoms);
+        recip_cycle_induced(P, false, all_atoms);
         timed(P_INDUCED, [&] {
             launch_zero_acc(stream, field_acc.p, field_acc.n);
             if (all_atoms)
@@ -795,18 +835,54 @@ struct mpid_context {
             launch_cov_induced_field<R>(stream, n, pair_atoms(P), P.mu.p, pair_params<R>(), all_atoms ? 0 : 1,
                                         field_acc.p);
         });
-        recip_cycle_induced(P, false, all_atoms);
     }
 
     // Av = v - S T S v
     template <class R> void matvec(Prec<R>& P, const double* v, double* Av) {
         timed(P_SOLVER, [&] { launch_apply_S<R>(stream, n, sqrtpol.p, v, P.mu.p, nullptr); });
         induced_field(P);
+        join();
         timed(P_SOLVER, [&] {
             launch_matvec_finish<R>(stream, n, sqrtpol.p, field_acc.p, P.erec.p, P.mu.p, selfc(), v, Av);
         });
         check_launch();
         stats.matvecs++;
+    }
+
+    // Records f (one CG iteration on stream and rstream) and a copy of the
+    // residual norm scal[3] to hscal into cg_exec: the executable graph is
+    // updated in place when its topology is unchanged, else instantiated.
+    template <class F> void capture_cg(F&& f) {
+        hipGraph_t g = nullptr;
+        HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        try {
+            f();
+            HIP_CHECK(hipMemcpyAsync(hscal, scal.p + 3, sizeof(double), hipMemcpyDeviceToHost, stream));
+        } catch (...) {
+            hipGraph_t dead = nullptr;
+            (void)hipStreamEndCapture(stream, &dead);
+            if (dead) (void)hipGraphDestroy(dead);
+            throw;
+        }
+        HIP_CHECK(hipStreamEndCapture(stream, &g));
+        bool updated = false;
+        if (cg_exec) {
+            hipGraphNode_t err = nullptr;
+            hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+            updated = hipGraphExecUpdate(cg_exec, g, &err, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+            if (!updated) {
+                (void)hipGetLastError();
+                (void)hipGraphExecDestroy(cg_exec);
+                cg_exec = nullptr;
+            }
+        }
+        hipError_t e = hipSuccess;
+        if (!updated) e = hipGraphInstantiate(&cg_exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            cg_exec = nullptr;
+            throw HipError(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
     }
 
     double read_scalar(const double* d) {
@@ -839,17 +915,38 @@ struct mpid_context {
             HIP_CHECK(hipMemcpyAsync(scal.p, scal.p + 2, sizeof(double), hipMemcpyDeviceToDevice, stream));
         });
         double eps = debye_eps(read_scalar(scal.p + 3));
-        while (eps >= stop_eps && iters < max_iter) {
-            matvec(P, p.p, Ap.p);
+        // the first direction's operator input; each iteration then leaves the
+        // next one's (mu = S p) behind
+        if (eps >= stop_eps && iters < max_iter)
+            timed(P_SOLVER, [&] { launch_apply_S<R>(stream, n, sqrtpol.p, p.p, P.mu.p, nullptr); });
+        auto body = [&] {
+            induced_field(P);
+            join();
             timed(P_SOLVER, [&] {
-                launch_dot(stream, n3, p.p, Ap.p, partial.p, scal.p + 1);
-                launch_cg_step(stream, n, sqrtpol.p, scal.p, p.p, Ap.p, sol, r.p, partial.p, scal.p + 2);
-                launch_cg_direction(stream, n3, scal.p, scal.p + 2, r.p, p.p);
-                HIP_CHECK(hipMemcpyAsync(scal.p, scal.p + 2, sizeof(double), hipMemcpyDeviceToDevice, stream));
+                launch_cg_iteration<R>(stream, n, sqrtpol.p, field_acc.p, P.erec.p, P.mu.p, selfc(), p.p, Ap.p, sol,
+                                       r.p, partial.p, scal.p);
             });
-            check_launch();
-            iters++;
-            eps = debye_eps(read_scalar(scal.p + 3));
+        };
+        if (profiling) {
+            // per-kernel timing events cannot sit in a graph: iterate eagerly
+            while (eps >= stop_eps && iters < max_iter) {
+                body();
+                check_launch();
+                stats.matvecs++;
+                iters++;
+                eps = debye_eps(read_scalar(scal.p + 3));
+            }
+            return eps;
+        }
+        if (eps >= stop_eps && iters < max_iter) {
+            capture_cg(body);
+            while (eps >= stop_eps && iters < max_iter) {
+                HIP_CHECK(hipGraphLaunch(cg_exec, stream));
+                HIP_CHECK(hipStreamSynchronize(stream));
+                stats.matvecs++;
+                iters++;
+                eps = debye_eps(hscal[0]);
+            }
         }
         return eps;
     }
@@ -919,7 +1016,7 @@ struct mpid_context {
         // induced dipoles: mu = S x (double for the per-atom terms, R for the kernels)
         timed(P_SOLVER, [&] { launch_apply_S<R>(stream, n, sqrtpol.p, x.p, P.mu.p, mud.p); });
         mu_valid = true;
-        recip_cycle_induced(P, true);  // phidp
+        recip_cycle_induced(P, true);  // phidp, on rstream beside the pair kernels
         timed(P_ELEC, [&] {
             launch_zero_acc(stream, force_acc.p, force_acc.n);
             launch_zero_acc(stream, torque_acc.p, torque_acc.n);
@@ -935,6 +1032,7 @@ struct mpid_context {
         check_launch();
         acc_t ereal = 0;
         HIP_CHECK(hipMemcpyAsync(&ereal, energy_acc.p, sizeof(acc_t), hipMemcpyDeviceToHost, stream));
+        join();
         timed(P_RECIP, [&] {
             launch_recip_atoms<R>(stream, n, opt.polarization == MPID_POLARIZATION_MUTUAL ? 1 : 0, pd.phi.p,
                                   P.phidp.p, lab.p, mud.p, aniso.p, gscale3(), opt.ewald_alpha, force_acc.p,
@@ -1027,7 +1125,7 @@ struct mpid_context {
     void compute_induced_field(const double* pos, const double* boxv, const double* mu, double* out) {
         set_box(boxv);
         pos_in.upload(pos, 3 * (size_t)n, stream);
-        prepare(pos_in.p, true);
+        prepare(pos_in.p, true, false);
         // mu (original order) -> sorted precision copy
         std::vector<int> pm(n);
         HIP_CHECK(hipMemcpyAsync(pm.data(), perm.p, n * sizeof(int), hipMemcpyDeviceToHost, stream));
@@ -1039,6 +1137,7 @@ struct mpid_context {
                 for (int k = 0; k < 3; k++) ms[3 * (size_t)s + k] = (R)mu[3 * (size_t)pm[s] + k];
             HIP_CHECK(hipMemcpyAsync(P.mu.p, ms.data(), ms.size() * sizeof(R), hipMemcpyHostToDevice, stream));
             induced_field(P, true);
+            join();
             launch_field_sum<R>(stream, n, field_acc.p, P.erec.p, P.mu.p, selfc(), tmp3n.p);
             check_launch();
             std::vector<double> f(3 * (size_t)n);
@@ -1085,6 +1184,10 @@ int mpid_create(const mpid_parameters* params, const mpid_options* opts, mpid_co
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
             throw MpidError(MPID_ERR_HIP, "no HIP device is available");
         HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        HIP_CHECK(hipStreamCreateWithFlags(&ctx->rstream, hipStreamNonBlocking));
+        HIP_CHECK(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+        HIP_CHECK(hipHostMalloc((void**)&ctx->hscal, 8 * sizeof(double), 0));
         ctx->set_options(opts);
         ctx->set_parameters(params, true);
         ctx->alloc_state();

@@ -226,3 +226,27 @@ def test_water216x_octopoles_anisotropic(precision, conservative):
     assert abs(st.getPotentialEnergy() - ref["energy"]) <= e_tol * abs(ref["energy"])
     assert rel_rms(st.getForces(), ref["forces"]) <= f_tol
     assert rel_rms(ctx.getInducedDipoles(), ref["induced"]) <= f_tol
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_graph_replays_match_eager_iterations(water4096, precision):
+    """The solver's iterations run as replays of a captured graph unless
+    per-kernel profiling is on, when they are launched one by one.  Both give
+    the same result: an eager evaluation, then two graph evaluations in a row
+    (the second replays an executable graph updated in place), each compared
+    with the eager one.  The tolerance covers the order of the FP64 atomic
+    sums in the spreading kernels, which is not fixed from run to run."""
+    print(f"[gpu] graph vs eager water4096 {precision}", flush=True)
+    f, pos, box, _ = water4096
+    k = HipCalcMPIDForceKernel(precision)
+    k.initialize(len(pos), f)
+    k.setProfiling(True)
+    e0, f0 = k.execute(pos, box)
+    it0 = k.stats()["iterations"]
+    k.setProfiling(False)
+    tol_e, tol_f = (1e-12, 1e-10) if precision == "double" else (1e-9, 1e-6)
+    for rep in range(2):
+        e1, f1 = k.execute(pos, box)
+        assert k.stats()["iterations"] == it0, rep
+        assert abs(e1 - e0) <= tol_e * abs(e0), rep
+        assert rel_rms(f1, f0) <= tol_f, rep
