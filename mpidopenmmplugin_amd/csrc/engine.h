@@ -136,6 +136,15 @@ template <class R>
 void launch_spread_induced_tiled(hipStream_t s, const R4<R>* pos, const R* mu, PmeGeom<R> g, TileGeom tg,
                                  const int* atoms, const int* tstart, const int* tend, R* scratch, R* grid);
 template <class R> void launch_fold(hipStream_t s, PmeGeom<R> g, TileGeom tg, const R* scratch, R* grid);
+// Owner-computes spreading (tiles of at least 8 points per axis): one
+// workgroup per grid core writes the whole grid, no scratch and no fold.
+bool spread_by_owner(const TileGeom& tg);
+template <class R>
+void launch_spread_fixed_owner(hipStream_t s, const R4<R>* pos, const R* lab, PmeGeom<R> g, TileGeom tg,
+                               const int* tstart, const int* tend, R* grid);
+template <class R>
+void launch_spread_induced_owner(hipStream_t s, const R4<R>* pos, const R* mu, PmeGeom<R> g, TileGeom tg,
+                                 const int* atoms, const int* tstart, const int* tend, R* grid);
 template <class R>
 void launch_spread_fixed(hipStream_t s, int n, const R4<R>* pos, const R* lab, PmeGeom<R> g, R* grid);
 template <class R>

@@ -311,6 +311,172 @@ __global__ __launch_bounds__(256) void k_spread_tiled(const R4<R>* __restrict__ 
     for (int i = threadIdx.x; i < WV; i += blockDim.x) out[i] = (R)win[i];
 }
 
+// ---------------------------------------------------------------------------
+// Owner-computes spreading (tiles of at least 8 points per axis).
+// The grid points are partitioned into cores: core c is the T0 x T1 x T2 block
+// starting at (c T + n/2 - 6) mod n, the window origin of cell c above.  An
+// atom of cell c (cell of its Morton key, from the FP64 position) has
+// fractional grid coordinate fr in [c T + n/2, (c + 1) T + n/2) mod n, so its
+// stencil start ig = floor(fr) - 5 lies in [c T + n/2 - 5, (c + 1) T + n/2 - 6]
+// up to one point of rounding, i.e. in [origin(c), origin(c + 1)]: it meets
+// the cores c and c + 1 only (T >= 8).  The workgroup of core c therefore
+// scans the cells c - 1 and c along each axis (8 cells), keeps the atoms
+// whose stencil meets the core, adds their contributions to an FP64 LDS
+// copy of the core and writes the core to the grid once.  No scratch, no
+// fold and no clearing of the grid: every point is written by its owner.
+// ---------------------------------------------------------------------------
+// candidate atoms examined per pass (the slots of their spline weights in LDS)
+template <class R, int MODE> constexpr int owner_cand() { return MODE == 0 ? 64 : 128; }
+
+template <class R, int MODE>
+__global__ __launch_bounds__(256) void k_spread_owner(const R4<R>* __restrict__ pos, const R* __restrict__ src,
+                                                      PmeGeom<R> g, TileGeom tg, const int* __restrict__ atoms,
+                                                      const int* __restrict__ tstart, const int* __restrict__ tend,
+                                                      R* __restrict__ grid) {
+    constexpr int ND = MODE == 0 ? 3 : 1;   // spline derivatives needed
+    constexpr int NC = MODE == 0 ? 20 : 3;  // fractional components
+    constexpr int kOwnerCand = owner_cand<R, MODE>();
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int T0 = tg.T[0], T1 = tg.T[1], T2 = tg.T[2];
+    const int TV = T0 * T1 * T2;
+    double* core = reinterpret_cast<double*>(smem_raw);
+    R* sth = reinterpret_cast<R*>(core + TV);                       // [kOwnerCand][3][6][ND+1]
+    R* smf = sth + kOwnerCand * 3 * kOrder * (ND + 1);              // [kOwnerCand][NC]
+    int* srel = reinterpret_cast<int*>(smf + kOwnerCand * NC);      // [kOwnerCand][3] stencil start - core origin
+    __shared__ int sbeg[8], send_[8], soff[9], snv;
+    const int t = blockIdx.x;
+    const int nt = tg.tiles_per_axis;
+    const int c[3] = {(int)compact_bits((unsigned)t >> 2), (int)compact_bits((unsigned)t >> 1),
+                      (int)compact_bits((unsigned)t)};
+    int w0[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) w0[k] = ((c[k] * tg.T[k] + (g.n[k] >> 1) - 6) % g.n[k] + g.n[k]) % g.n[k];
+    for (int i = threadIdx.x; i < TV; i += blockDim.x) core[i] = 0.0;
+    if (threadIdx.x < 8) {
+        const int d = threadIdx.x;
+        const unsigned tx = (unsigned)((c[0] - ((d >> 2) & 1) + nt) % nt),
+                       ty = (unsigned)((c[1] - ((d >> 1) & 1) + nt) % nt), tz = (unsigned)((c[2] - (d & 1) + nt) % nt);
+        const unsigned tile = (spread_bits_pme(tx) << 2) | (spread_bits_pme(ty) << 1) | spread_bits_pme(tz);
+        sbeg[d] = tstart[tile];
+        send_[d] = tend[tile];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        // cells c - 1 and c coincide only with one cell per axis: count each cell once
+        int o = 0;
+        for (int d = 0; d < 8; d++) {
+            bool dup = false;
+            for (int e = 0; e < d; e++) dup = dup || (sbeg[e] == sbeg[d] && send_[e] == send_[d] && send_[d] > sbeg[d]);
+            soff[d] = o;
+            if (!dup) o += max(0, send_[d] - sbeg[d]);
+            else send_[d] = sbeg[d];
+        }
+        soff[8] = o;
+    }
+    __syncthreads();
+    const int ncand = soff[8];
+    for (int c0 = 0; c0 < ncand; c0 += kOwnerCand) {
+        if (threadIdx.x == 0) snv = 0;
+        __syncthreads();
+        if ((int)threadIdx.x < kOwnerCand && c0 + (int)threadIdx.x < ncand) {
+            const int q = c0 + threadIdx.x;
+            int d = 0;
+            while (soff[d + 1] <= q) d++;
+            const int s = sbeg[d] + (q - soff[d]);
+            const int a = atoms ? atoms[s] : s;
+            const R4<R> p = pos[a];
+            const R px[3] = {p.x, p.y, p.z};
+            int ig[3], rel[3];
+            R w[3];
+            bool meets = true;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                grid_coord(px[k], g.box.L[k], g.box.inv[k], g.n[k], ig[k], w[k]);
+                int r = ig[k] - w0[k];
+                if (r < -(g.n[k] >> 1)) r += g.n[k];
+                if (r >= g.n[k] - (g.n[k] >> 1)) r -= g.n[k];
+                rel[k] = r;
+                meets = meets && r + kOrder - 1 >= 0 && r < tg.T[k];
+            }
+            if (meets) {
+                const int slot = atomicAdd(&snv, 1);
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    R th[kOrder][ND + 1];
+                    bspline6<R, ND>(w[k], th);
+                    for (int m = 0; m < kOrder; m++)
+                        for (int r = 0; r <= ND; r++) sth[((slot * 3 + k) * kOrder + m) * (ND + 1) + r] = th[m][r];
+                    srel[slot * 3 + k] = rel[k];
+                }
+                R* mf = smf + slot * NC;
+                const R gx = g.gscale[0], gy = g.gscale[1], gz = g.gscale[2];
+                if (MODE == 0) {
+                    const R* m = src + (size_t)a * kMP;
+                    mf[0] = m[0];
+                    mf[1] = gx * m[1]; mf[2] = gy * m[2]; mf[3] = gz * m[3];
+                    mf[4] = gx * gx * m[4]; mf[5] = R(2) * gx * gy * m[5]; mf[6] = R(2) * gx * gz * m[6];
+                    mf[7] = gy * gy * m[7]; mf[8] = R(2) * gy * gz * m[8]; mf[9] = gz * gz * m[9];
+                    mf[10] = gx * gx * gx * m[10]; mf[11] = R(3) * gx * gx * gy * m[11];
+                    mf[12] = R(3) * gx * gx * gz * m[12]; mf[13] = R(3) * gx * gy * gy * m[13];
+                    mf[14] = R(6) * gx * gy * gz * m[14]; mf[15] = R(3) * gx * gz * gz * m[15];
+                    mf[16] = gy * gy * gy * m[16]; mf[17] = R(3) * gy * gy * gz * m[17];
+                    mf[18] = R(3) * gy * gz * gz * m[18]; mf[19] = gz * gz * gz * m[19];
+                } else {
+                    mf[0] = gx * src[3 * (size_t)a];
+                    mf[1] = gy * src[3 * (size_t)a + 1];
+                    mf[2] = gz * src[3 * (size_t)a + 2];
+                }
+            }
+        }
+        __syncthreads();
+        const int nv = snv;
+        // one item = (atom, iy, iz): the points of the stencil row along x inside the core
+        for (int it = threadIdx.x; it < nv * kOrder * kOrder; it += blockDim.x) {
+            const int la = it / (kOrder * kOrder);
+            const int iy = (it / kOrder) % kOrder, iz = it % kOrder;
+            const int ly = srel[la * 3 + 1] + iy, lz = srel[la * 3 + 2] + iz;
+            if (ly < 0 || ly >= T1 || lz < 0 || lz >= T2) continue;
+            const R* tx = sth + ((la * 3 + 0) * kOrder) * (ND + 1);
+            const R* ty = sth + ((la * 3 + 1) * kOrder + iy) * (ND + 1);
+            const R* tz = sth + ((la * 3 + 2) * kOrder + iz) * (ND + 1);
+            const R* mf = smf + la * NC;
+            R t0, t1, t2 = R(0), t3 = R(0);
+            if (MODE == 0) {
+                const R u0 = ty[0], u1 = ty[1], u2 = ty[2], u3 = ty[ND >= 3 ? 3 : 0];
+                const R v0 = tz[0], v1 = tz[1], v2 = tz[2], v3 = tz[ND >= 3 ? 3 : 0];
+                t0 = mf[0] * u0 * v0 + mf[2] * u1 * v0 + mf[3] * u0 * v1 + mf[7] * u2 * v0 + mf[9] * u0 * v2 +
+                     mf[8] * u1 * v1 + mf[16] * u3 * v0 + mf[17] * u2 * v1 + mf[18] * u1 * v2 + mf[19] * u0 * v3;
+                t1 = mf[1] * u0 * v0 + mf[5] * u1 * v0 + mf[6] * u0 * v1 + mf[13] * u2 * v0 + mf[14] * u1 * v1 +
+                     mf[15] * u0 * v2;
+                t2 = mf[4] * u0 * v0 + mf[11] * u1 * v0 + mf[12] * u0 * v1;
+                t3 = mf[10] * u0 * v0;
+            } else {
+                t0 = mf[1] * ty[1] * tz[0] + mf[2] * ty[0] * tz[1];
+                t1 = mf[0] * ty[0] * tz[0];
+            }
+            const int lx0 = srel[la * 3 + 0];
+#pragma unroll
+            for (int ix = 0; ix < kOrder; ix++) {
+                const int lx = lx0 + ix;
+                if (lx < 0 || lx >= T0) continue;
+                R v = t0 * tx[ix * (ND + 1)] + t1 * tx[ix * (ND + 1) + 1];
+                if (MODE == 0) v += t2 * tx[ix * (ND + 1) + 2] + t3 * tx[ix * (ND + 1) + ND];
+                atomicAdd(&core[(lx * T1 + ly) * T2 + lz], (double)v);
+            }
+        }
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < TV; i += blockDim.x) {
+        const int lx = i / (T1 * T2), rem = i - lx * T1 * T2;
+        const int ly = rem / T2, lz = rem - ly * T2;
+        int gx = w0[0] + lx, gy = w0[1] + ly, gz = w0[2] + lz;
+        if (gx >= g.n[0]) gx -= g.n[0];
+        if (gy >= g.n[1]) gy -= g.n[1];
+        if (gz >= g.n[2]) gz -= g.n[2];
+        grid[((size_t)gx * g.n[1] + gy) * g.n[2] + gz] = (R)core[i];
+    }
+}
+
 // grid(x,y,z) = sum of the windows covering the point (plus what is already there)
 template <class R>
 __global__ __launch_bounds__(256) void k_fold(PmeGeom<R> g, TileGeom tg, const R* __restrict__ scratch,
@@ -595,6 +761,30 @@ void launch_spread_induced_tiled(hipStream_t s, const R4<R>* pos, const R* mu, P
     const size_t shm = tiled_lds<R, 1>(tg);
     k_spread_tiled<R, 1><<<dim3(tg.ntiles), dim3(256), shm, s>>>(pos, mu, g, tg, atoms, tstart, tend, scratch, grid);
 }
+template <class R, int MODE> static size_t owner_lds(const TileGeom& tg) {
+    constexpr int ND = MODE == 0 ? 3 : 1;
+    constexpr int NC = MODE == 0 ? 20 : 3;
+    constexpr int K = owner_cand<R, MODE>();
+    return (size_t)tg.T[0] * tg.T[1] * tg.T[2] * sizeof(double) + K * (3 * kOrder * (ND + 1) + NC) * sizeof(R) +
+           K * 3 * sizeof(int);
+}
+bool spread_by_owner(const TileGeom& tg) {
+    for (int k = 0; k < 3; k++)
+        if (tg.T[k] < 8) return false;
+    return owner_lds<double, 0>(tg) <= 65536;
+}
+template <class R>
+void launch_spread_fixed_owner(hipStream_t s, const R4<R>* pos, const R* lab, PmeGeom<R> g, TileGeom tg,
+                               const int* tstart, const int* tend, R* grid) {
+    k_spread_owner<R, 0><<<dim3(tg.ntiles), dim3(256), owner_lds<R, 0>(tg), s>>>(pos, lab, g, tg, nullptr, tstart,
+                                                                                   tend, grid);
+}
+template <class R>
+void launch_spread_induced_owner(hipStream_t s, const R4<R>* pos, const R* mu, PmeGeom<R> g, TileGeom tg,
+                                 const int* atoms, const int* tstart, const int* tend, R* grid) {
+    k_spread_owner<R, 1><<<dim3(tg.ntiles), dim3(256), owner_lds<R, 1>(tg), s>>>(pos, mu, g, tg, atoms, tstart, tend,
+                                                                                   grid);
+}
 template <class R> void launch_fold(hipStream_t s, PmeGeom<R> g, TileGeom tg, const R* scratch, R* grid) {
     size_t total = (size_t)g.n[0] * g.n[1] * g.n[2];
     int blocks = (int)std::min<size_t>((total + 255) / 256, 16384);
@@ -636,6 +826,10 @@ void launch_gather_field(hipStream_t s, int natoms, const int* atoms, const R4<R
     template void launch_spread_induced_tiled<R>(hipStream_t, const R4<R>*, const R*, PmeGeom<R>, TileGeom,  \
                                                  const int*, const int*, const int*, R*, R*);                \
     template void launch_fold<R>(hipStream_t, PmeGeom<R>, TileGeom, const R*, R*);                          \
+    template void launch_spread_fixed_owner<R>(hipStream_t, const R4<R>*, const R*, PmeGeom<R>, TileGeom,    \
+                                               const int*, const int*, R*);                                  \
+    template void launch_spread_induced_owner<R>(hipStream_t, const R4<R>*, const R*, PmeGeom<R>, TileGeom,  \
+                                                 const int*, const int*, const int*, R*);                    \
     template void launch_fold_tiles<R>(hipStream_t, PmeGeom<R>, TileGeom, const R*, R*);
 MPID_INSTANTIATE(float)
 MPID_INSTANTIATE(double)

@@ -185,7 +185,8 @@ struct mpid_context {
     // kernel arguments are those of that solve); hscal is the pinned host
     // copy of the residual norm that each replay leaves behind.
     hipGraphExec_t cg_exec = nullptr;
-    double* hscal = nullptr;
+    double* hscal = nullptr;  // [0] residual norm of a replay, [1] read_scalar, [2] energy-correction dot
+    hipEvent_t ev_sync = nullptr;
     std::string error;
     mpid_stats stats{};
 
@@ -207,6 +208,7 @@ struct mpid_context {
     // Morton-aligned PME spreading tiles
     TileGeom tgeom{};
     bool tiled_spread = false;
+    bool owner_spread = false;  // tiles of at least 8 points: owner-computes spreading, no scratch or fold
     DBuf<int> tstart, tend;     // over all sorted atoms
     DBuf<int> tpstart, tpend;   // over the polarisable atoms (indices into pol_idx)
     // 64-atom blocks: bounding boxes, block rows (CSR: for each block I the
@@ -222,6 +224,13 @@ struct mpid_context {
     bool all_lists = false;  // lists of the last build: every atom (true) or the polarisable ones
     int nl_cap = 0;
     int nl_max = 0;
+    // After one verified build for the current parameters, the list build of
+    // an evaluation is not waited for: npol is kept (it depends on the
+    // polarisabilities only), and the overflow and mirror checks, copied to
+    // hnl, are read at the solver's first synchronisation (check_lists_pending).
+    int npol_cached = -1;
+    bool lists_pending = false;
+    int* hnl = nullptr;  // pinned: [0] largest list length, [1] mirror flag
     // accumulators
     DBuf<acc_t> field_acc, force_acc, torque_acc, energy_acc;
     // solver
@@ -235,6 +244,7 @@ struct mpid_context {
     bool have_last = false;
     bool mu_valid = false;
     double energy_correction = 0.0;
+    bool correction_pending = false;  // the solver's correction dot is on its way to hscal[2]
 
     // profiling
     bool profiling = false;
@@ -249,7 +259,9 @@ struct mpid_context {
         for (auto e : pool) (void)hipEventDestroy(e);
         if (cg_exec) (void)hipGraphExecDestroy(cg_exec);
         if (hscal) (void)hipHostFree(hscal);
+        if (hnl) (void)hipHostFree(hnl);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_sync) (void)hipEventDestroy(ev_sync);
         if (ev_join) (void)hipEventDestroy(ev_join);
         if (rstream) (void)hipStreamDestroy(rstream);
         if (stream) (void)hipStreamDestroy(stream);
@@ -304,6 +316,7 @@ struct mpid_context {
     // ---- set-up --------------------------------------------------------------
     void set_parameters(const mpid_parameters* P, bool first) {
         if (!P || P->n <= 0) throw MpidError(MPID_ERR_INVALID, "MPIDForce: no particles");
+        npol_cached = -1;  // the polarisable set may change
         if (!first && P->n != n)
             throw MpidError(MPID_ERR_INVALID, "updateParametersInContext: The number of multipoles has changed");
         n = P->n;
@@ -455,7 +468,8 @@ struct mpid_context {
             if (!make_fft_axis(nk, P.plan.ax[k])) throw MpidError(MPID_ERR_UNSUPPORTED, "PME grid size not supported");
         }
         P.moduli.upload(mods.data(), mods.size(), stream);
-        if (tiled_spread) P.scratch.resize((size_t)tgeom.ntiles * tgeom.W[0] * tgeom.W[1] * tgeom.W[2]);
+        if (tiled_spread && !owner_spread)
+            P.scratch.resize((size_t)tgeom.ntiles * tgeom.W[0] * tgeom.W[1] * tgeom.W[2]);
         HIP_CHECK(hipStreamSynchronize(stream));
         P.ready = true;
     }
@@ -540,6 +554,7 @@ struct mpid_context {
             tiled_spread = true;
             break;
         }
+        owner_spread = tiled_spread && spread_by_owner(tgeom);
         if (tiled_spread) {
             tstart.resize(tgeom.ntiles);
             tend.resize(tgeom.ntiles);
@@ -706,16 +721,40 @@ struct mpid_context {
                 }
             });
             check_launch();
+            if (!all_lists && npol_cached >= 0) {
+                HIP_CHECK(hipMemcpyAsync(hnl, nl_scalars.p, 2 * sizeof(int), hipMemcpyDeviceToHost, stream));
+                npol = npol_cached;
+                lists_pending = true;
+                return;
+            }
             int scal[2];
             HIP_CHECK(hipMemcpyAsync(scal, nl_scalars.p, 2 * sizeof(int), hipMemcpyDeviceToHost, stream));
             HIP_CHECK(hipMemcpyAsync(&npol, polstart.p + nb, sizeof(int), hipMemcpyDeviceToHost, stream));
             HIP_CHECK(hipStreamSynchronize(stream));
             if (scal[1] != 0) throw MpidError(MPID_ERR_HIP, "neighbour masks: a block pair lacks its mirror entry");
             nl_max = scal[0];
-            if (nl_max <= nl_cap) return;
+            if (nl_max <= nl_cap) {
+                if (!all_lists) npol_cached = npol;
+                return;
+            }
             nl_cap = nl_max + nl_max / 8 + kNlSplit;
         }
         throw MpidError(MPID_ERR_HIP, "neighbour list overflow after regrowth");
+    }
+    // The checks of an unverified build (after a synchronisation that covers
+    // it).  On overflow the capacity grows and ListsOverflow asks run() to
+    // evaluate again; no output has been written by then.
+    struct ListsOverflow {};
+    void check_lists_pending() {
+        if (!lists_pending) return;
+        lists_pending = false;
+        if (hnl[1] != 0) throw MpidError(MPID_ERR_HIP, "neighbour masks: a block pair lacks its mirror entry");
+        nl_max = hnl[0];
+        if (nl_max > nl_cap) {
+            nl_cap = nl_max + nl_max / 8 + kNlSplit;
+            npol_cached = -1;
+            throw ListsOverflow{};
+        }
     }
     NbrList nlist() const { return NbrList{nbr.p, ncount.p}; }
     NbrList nlist_pol() const { return NbrList{pnbr.p, pcount.p}; }
@@ -736,6 +775,10 @@ struct mpid_context {
         fork();
         const size_t G = P.grid.n;
         timed_on(rstream, P_SPREAD, [&] {
+            if (owner_spread) {
+                launch_spread_fixed_owner<R>(rstream, P.pos.p, P.lab.p, g, tgeom, tstart.p, tend.p, P.grid.p);
+                return;
+            }
             launch_fill<R>(rstream, P.grid.p, G, R(0));
             if (tiled_spread) {
                 launch_spread_fixed_tiled<R>(rstream, P.pos.p, P.lab.p, g, tgeom, tstart.p, tend.p, P.scratch.p,
@@ -760,6 +803,15 @@ struct mpid_context {
         fork();
         const size_t G = P.grid.n;
         timed_on(rstream, P_SPREAD, [&] {
+            if (owner_spread) {
+                if (all_atoms)
+                    launch_spread_induced_owner<R>(rstream, P.pos.p, P.mu.p, g, tgeom, nullptr, tstart.p, tend.p,
+                                                   P.grid.p);
+                else
+                    launch_spread_induced_owner<R>(rstream, P.pos.p, P.mu.p, g, tgeom, pol_idx.p, tpstart.p,
+                                                   tpend.p, P.grid.p);
+                return;
+            }
             launch_fill<R>(rstream, P.grid.p, G, R(0));
             if (tiled_spread) {
                 if (all_atoms)
@@ -885,11 +937,21 @@ struct mpid_context {
         }
     }
 
+    // Waits for the work issued to stream so far by polling an event: the
+    // solver's host decisions sit on this path, and polling resumes sooner
+    // than a blocking synchronisation.
+    void wait_stream() {
+        HIP_CHECK(hipEventRecord(ev_sync, stream));
+        for (;;) {
+            const hipError_t q = hipEventQuery(ev_sync);
+            if (q == hipSuccess) return;
+            if (q != hipErrorNotReady) HIP_CHECK(q);
+        }
+    }
     double read_scalar(const double* d) {
-        double h = 0;
-        HIP_CHECK(hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost, stream));
-        HIP_CHECK(hipStreamSynchronize(stream));
-        return h;
+        HIP_CHECK(hipMemcpyAsync(hscal + 1, d, sizeof(double), hipMemcpyDeviceToHost, stream));
+        wait_stream();
+        return hscal[1];
     }
     double debye_eps(double ss) const { return kDebye * std::sqrt(ss / n); }
 
@@ -899,6 +961,18 @@ struct mpid_context {
     template <class R>
     double cg(Prec<R>& P, double* sol, const double* rhs, bool zero_start, double stop_eps, int& iters, int max_iter) {
         const int n3 = 3 * n;
+        auto body = [&] {
+            induced_field(P);
+            join();
+            timed(P_SOLVER, [&] {
+                launch_cg_iteration<R>(stream, n, sqrtpol.p, field_acc.p, P.erec.p, P.mu.p, selfc(), p.p, Ap.p, sol,
+                                       r.p, partial.p, scal.p);
+            });
+        };
+        // The iteration graph is recorded first: the host does it while the
+        // device still runs the work issued before.  (Per-kernel timing
+        // events cannot sit in a graph: with profiling the loop runs eagerly.)
+        if (!profiling && iters < max_iter) capture_cg(body);
         timed(P_SOLVER, [&] {
             if (zero_start) {
                 launch_fill<double>(stream, sol, n3, 0.0);
@@ -913,22 +987,13 @@ struct mpid_context {
             HIP_CHECK(hipMemcpyAsync(p.p, r.p, n3 * sizeof(double), hipMemcpyDeviceToDevice, stream));
             launch_residual_norm(stream, n, sqrtpol.p, r.p, partial.p, scal.p + 2);  // scal[2] = rr, scal[3] = ss
             HIP_CHECK(hipMemcpyAsync(scal.p, scal.p + 2, sizeof(double), hipMemcpyDeviceToDevice, stream));
+            // the first direction's operator input (each iteration leaves the
+            // next one's behind); issued before the norm is known
+            launch_apply_S<R>(stream, n, sqrtpol.p, p.p, P.mu.p, nullptr);
         });
         double eps = debye_eps(read_scalar(scal.p + 3));
-        // the first direction's operator input; each iteration then leaves the
-        // next one's (mu = S p) behind
-        if (eps >= stop_eps && iters < max_iter)
-            timed(P_SOLVER, [&] { launch_apply_S<R>(stream, n, sqrtpol.p, p.p, P.mu.p, nullptr); });
-        auto body = [&] {
-            induced_field(P);
-            join();
-            timed(P_SOLVER, [&] {
-                launch_cg_iteration<R>(stream, n, sqrtpol.p, field_acc.p, P.erec.p, P.mu.p, selfc(), p.p, Ap.p, sol,
-                                       r.p, partial.p, scal.p);
-            });
-        };
+        check_lists_pending();
         if (profiling) {
-            // per-kernel timing events cannot sit in a graph: iterate eagerly
             while (eps >= stop_eps && iters < max_iter) {
                 body();
                 check_launch();
@@ -938,15 +1003,12 @@ struct mpid_context {
             }
             return eps;
         }
-        if (eps >= stop_eps && iters < max_iter) {
-            capture_cg(body);
-            while (eps >= stop_eps && iters < max_iter) {
-                HIP_CHECK(hipGraphLaunch(cg_exec, stream));
-                HIP_CHECK(hipStreamSynchronize(stream));
-                stats.matvecs++;
-                iters++;
-                eps = debye_eps(hscal[0]);
-            }
+        while (eps >= stop_eps && iters < max_iter) {
+            HIP_CHECK(hipGraphLaunch(cg_exec, stream));
+            wait_stream();
+            stats.matvecs++;
+            iters++;
+            eps = debye_eps(hscal[0]);
         }
         return eps;
     }
@@ -971,7 +1033,12 @@ struct mpid_context {
         timed(P_SOLVER, [&] { launch_solver_init(stream, n, sqrtpol.p, E0.p, b.p, x.p); });
         check_launch();
         energy_correction = 0.0;
+        correction_pending = false;
         if (opt.polarization == MPID_POLARIZATION_DIRECT) {
+            if (lists_pending) {
+                wait_stream();
+                check_lists_pending();
+            }
             stats.converged = 1;
             stats.epsilon = 0.0;
             return;
@@ -1001,7 +1068,9 @@ struct mpid_context {
         timed(P_SOLVER, [&] {
             launch_dot(stream, 3 * n, x.p, opt.precision == MPID_PRECISION_MIXED ? rt.p : r.p, partial.p, scal.p + 6);
         });
-        energy_correction = -0.5 * kElectric * read_scalar(scal.p + 6);
+        // read with the energy at the end of the evaluation (forces_and_energy)
+        HIP_CHECK(hipMemcpyAsync(hscal + 2, scal.p + 6, sizeof(double), hipMemcpyDeviceToHost, stream));
+        correction_pending = true;
         stats.iterations = iters;
         stats.epsilon = eps;
         stats.converged = eps < opt.target_epsilon ? 1 : 0;
@@ -1046,6 +1115,10 @@ struct mpid_context {
         acc_t e = 0;
         HIP_CHECK(hipMemcpyAsync(&e, energy_acc.p, sizeof(acc_t), hipMemcpyDeviceToHost, stream));
         HIP_CHECK(hipStreamSynchronize(stream));
+        if (correction_pending) {
+            energy_correction = -0.5 * kElectric * hscal[2];
+            correction_pending = false;
+        }
         stats.energy_terms[0] = acc_get(ereal);
         stats.energy_terms[1] = acc_get(e) - acc_get(ereal);
         stats.energy_terms[2] = energy_correction;
@@ -1053,9 +1126,16 @@ struct mpid_context {
     }
 
     template <class R> double run(Prec<R>& P, const double* d_pos, double* d_forces_out) {
-        prepare(d_pos);
-        fixed_field(P);
-        solve(P);
+        for (int attempt = 0;; attempt++) {
+            try {
+                prepare(d_pos);
+                fixed_field(P);
+                solve(P);
+                break;
+            } catch (const ListsOverflow&) {
+                if (attempt > 0) throw MpidError(MPID_ERR_HIP, "neighbour list overflow after regrowth");
+            }
+        }
         double e = forces_and_energy(P, d_forces_out);
         collect_profile();
         return e;
@@ -1187,7 +1267,9 @@ int mpid_create(const mpid_parameters* params, const mpid_options* opts, mpid_co
         HIP_CHECK(hipStreamCreateWithFlags(&ctx->rstream, hipStreamNonBlocking));
         HIP_CHECK(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
         HIP_CHECK(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&ctx->ev_sync, hipEventDisableTiming));
         HIP_CHECK(hipHostMalloc((void**)&ctx->hscal, 8 * sizeof(double), 0));
+        HIP_CHECK(hipHostMalloc((void**)&ctx->hnl, 4 * sizeof(int), 0));
         ctx->set_options(opts);
         ctx->set_parameters(params, true);
         ctx->alloc_state();
