@@ -322,6 +322,184 @@ __global__ __launch_bounds__(fft_threads<R>()) void k_fft_x_convolve(R2<R>* __re
 }
 
 // ---------------------------------------------------------------------------
+// FP32, n0 = n1 = 128: passes 2-4 fused, one workgroup per kz plane.
+// The 128 x 128 complex plane S[kz][.][.] (128 KB) is held in LDS, and the
+// y transform, the x transform, the influence function and both inverse
+// transforms run on it without leaving the chip.  A line of 128 = 16 x 8 is
+// transformed in two register steps (four-step algorithm): 16-point DFTs at
+// stride 8 with the twiddles w^(b k1), then 8-point DFTs over contiguous
+// groups of 8, which leaves X[k1 + 16 k2] in slot 8 k1 + k2 (digit-reversed
+// order).  The influence function is applied in that order and the inverse
+// runs the steps backwards, so no permutation pass is needed.
+// LDS layout: element (x, y) at x * 152 + 9 (y / 8) + y % 8 (a pad word after
+// every group of 8 and a row pitch of 8 mod 16 complex words keep the
+// accesses of both steps, along rows and along columns, free of bank
+// conflicts).
+// ---------------------------------------------------------------------------
+namespace plane {
+constexpr int N = 128, A = 16, B = 8;
+constexpr int kThreads = 1024;
+constexpr int kPitch = (N / B) * (B + 1) + 8;  // 152
+__device__ __forceinline__ int slot(int y) { return (y >> 3) * (B + 1) + (y & 7); }
+
+typedef Cx<float> C;
+__device__ __forceinline__ C tw_s(const C* tw, int j, float s) { return C{tw[j].x, -s * tw[j].y}; }
+
+// exp(s 2 pi i j / 16), j < 16
+__device__ __forceinline__ C w16(int j, float s) {
+    constexpr float c[16] = {1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508979f,
+                             0.0f, -0.38268343236508979f, -0.70710678118654752f, -0.92387953251128674f,
+                             -1.0f, -0.92387953251128674f, -0.70710678118654752f, -0.38268343236508979f,
+                             0.0f, 0.38268343236508979f, 0.70710678118654752f, 0.92387953251128674f};
+    constexpr float sn[16] = {0.0f, 0.38268343236508979f, 0.70710678118654752f, 0.92387953251128674f,
+                              1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508979f,
+                              0.0f, -0.38268343236508979f, -0.70710678118654752f, -0.92387953251128674f,
+                              -1.0f, -0.92387953251128674f, -0.70710678118654752f, -0.38268343236508979f};
+    return C{c[j], s * sn[j]};
+}
+// natural-order DFTs: X[k] = sum_n v[n] exp(s 2 pi i n k / M)
+__device__ __forceinline__ void dft8(C* v, float s) {
+    C e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+    dft_small<float, 4>(e, s);
+    dft_small<float, 4>(o, s);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const C t = k == 0 ? o[0] : cmul(o[k], w16(2 * k, s));
+        v[k] = cadd(e[k], t);
+        v[k + 4] = csub(e[k], t);
+    }
+}
+__device__ __forceinline__ void dft16(C* v, float s) {
+    C d[4][4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+#pragma unroll
+        for (int m = 0; m < 4; m++) d[q][m] = v[4 * m + q];
+        dft_small<float, 4>(d[q], s);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        C y[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) y[q] = (q * k == 0) ? d[q][k] : cmul(d[q][k], w16(q * k, s));
+        dft_small<float, 4>(y, s);
+#pragma unroll
+        for (int l = 0; l < 4; l++) v[k + 4 * l] = y[l];
+    }
+}
+
+// address of element k of line `line`: ROW lines run along y in row x = line,
+// column lines along x in column y = line
+template <bool ROW> __device__ __forceinline__ int addr(int line, int k) {
+    return ROW ? line * kPitch + slot(k) : k * kPitch + slot(line);
+}
+
+// Step of stride 8 (16-point DFTs), one task per (line, b).  Forward: the
+// result times w^(b k1) goes to slot 8 k1 + b.  Inverse (last step): natural
+// order in, natural order out.
+template <bool ROW, bool FWD> __device__ __forceinline__ void step16(C* P, const C* tw, float s) {
+    const int t = threadIdx.x;
+    const int line = ROW ? t >> 3 : t & 127, b = ROW ? t & 7 : t >> 7;
+    C v[A];
+#pragma unroll
+    for (int a = 0; a < A; a++) v[a] = P[addr<ROW>(line, a * B + b)];
+    dft16(v, s);
+#pragma unroll
+    for (int k1 = 0; k1 < A; k1++) {
+        C r = v[k1];
+        if (FWD && b * k1 != 0) r = cmul(r, tw_s(tw, b * k1, s));
+        P[addr<ROW>(line, k1 * B + b)] = r;
+    }
+}
+// Step over contiguous groups (8-point DFTs), tasks (line, k1), two per
+// thread.  Forward: slot 8 k1 + k2 receives X[k1 + 16 k2].  Inverse (first
+// step): the inverse DFT over k2, times w^(-b k1), to slot 8 k1 + b.
+template <bool ROW, bool FWD> __device__ __forceinline__ void step8(C* P, const C* tw, float s) {
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int t = threadIdx.x + h * kThreads;
+        const int line = ROW ? t >> 4 : t & 127, k1 = ROW ? t & 15 : t >> 7;
+        C v[B];
+#pragma unroll
+        for (int j = 0; j < B; j++) v[j] = P[addr<ROW>(line, k1 * B + j)];
+        dft8(v, s);
+#pragma unroll
+        for (int j = 0; j < B; j++) {
+            C r = v[j];
+            if (!FWD && j * k1 != 0) r = cmul(r, tw_s(tw, j * k1, s));
+            P[addr<ROW>(line, k1 * B + j)] = r;
+        }
+    }
+}
+}  // namespace plane
+
+__global__ __launch_bounds__(plane::kThreads) void k_fft_plane128_convolve(R2<float>* __restrict__ S,
+                                                                           const R2<float>* __restrict__ tw,
+                                                                           int n2, ConvParams<float> cp) {
+    using namespace plane;
+    __shared__ C P[N * kPitch];
+    __shared__ C W[N];
+    const int kz = blockIdx.x;
+    R2<float>* base = S + (size_t)kz * N * N;
+    if (threadIdx.x < N) W[threadIdx.x] = C{tw[threadIdx.x].x, tw[threadIdx.x].y};
+    {
+        R2<float> v[N * N / kThreads];
+#pragma unroll
+        for (int u = 0; u < N * N / kThreads; u++) v[u] = base[threadIdx.x + u * kThreads];
+#pragma unroll
+        for (int u = 0; u < N * N / kThreads; u++) {
+            const int idx = threadIdx.x + u * kThreads;
+            P[(idx >> 7) * kPitch + slot(idx & 127)] = C{v[u].x, v[u].y};
+        }
+    }
+    __syncthreads();
+    step16<true, true>(P, W, -1.f);
+    __syncthreads();
+    step8<true, true>(P, W, -1.f);
+    __syncthreads();
+    step16<false, true>(P, W, -1.f);
+    __syncthreads();
+    step8<false, true>(P, W, -1.f);
+    __syncthreads();
+    // influence function at (kx, ky, kz), slots in digit-reversed order
+    {
+        const int mz = kz < (n2 + 1) / 2 ? kz : kz - n2;
+        const float hz = float(mz) * cp.inv[2];
+        const float bz = cp.moduli[2 * N + kz];
+        for (int idx = threadIdx.x; idx < N * N; idx += kThreads) {
+            const int sx = idx >> 7, sy = idx & 127;
+            const int kx = (sx >> 3) + A * (sx & 7), ky = (sy >> 3) + A * (sy & 7);
+            float et = 0.f;
+            if ((kx | ky | kz) != 0) {
+                const int mx = kx < (N + 1) / 2 ? kx : kx - N;
+                const int my = ky < (N + 1) / 2 ? ky : ky - N;
+                const float hx = float(mx) * cp.inv[0], hy = float(my) * cp.inv[1];
+                const float m2 = hx * hx + hy * hy + hz * hz;
+                const float den = m2 * cp.moduli[kx] * cp.moduli[N + ky] * bz;
+                et = cp.sc * exp_r(-cp.expf * m2) / den;
+            }
+            C& v = P[sx * kPitch + slot(sy)];
+            v.x *= et;
+            v.y *= et;
+        }
+    }
+    __syncthreads();
+    step8<false, false>(P, W, 1.f);
+    __syncthreads();
+    step16<false, false>(P, W, 1.f);
+    __syncthreads();
+    step8<true, false>(P, W, 1.f);
+    __syncthreads();
+    step16<true, false>(P, W, 1.f);
+    __syncthreads();
+    for (int u = 0; u < N * N / kThreads; u++) {
+        const int idx = threadIdx.x + u * kThreads;
+        const C v = P[(idx >> 7) * kPitch + slot(idx & 127)];
+        base[idx] = R2<float>{v.x, v.y};
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 FastDiv make_fastdiv(unsigned d) {
@@ -367,6 +545,17 @@ static int log2i(int v) {
 }
 template <class R> static size_t fft_lds(int T, int n) { return (2 * (size_t)T * n + n) * sizeof(R2<R>); }
 
+// passes 2-4 as one plane pass where that kernel applies (FP32, n0 = n1 = 128)
+template <class R> static bool plane_pass(hipStream_t, const FftPlan<R>&, R2<R>*, int, int, ConvParams<R>) {
+    return false;
+}
+template <>
+bool plane_pass<float>(hipStream_t s, const FftPlan<float>& p, R2<float>* S, int n2h, int n2, ConvParams<float> cp) {
+    if (p.ax[0].n != plane::N || p.ax[1].n != plane::N) return false;
+    hipLaunchKernelGGL(k_fft_plane128_convolve, dim3(n2h), dim3(plane::kThreads), 0, s, S, p.tw[0], n2, cp);
+    return true;
+}
+
 template <class R>
 void launch_recip_convolution(hipStream_t s, const FftPlan<R>& p, R* Q, R2<R>* S, ConvParams<R> cp) {
     const int n0 = p.ax[0].n, n1 = p.ax[1].n, n2 = p.ax[2].n;
@@ -378,15 +567,17 @@ void launch_recip_convolution(hipStream_t s, const FftPlan<R>& p, R* Q, R2<R>* S
     // 1. z forward
     hipLaunchKernelGGL(k_fft_z<R>, dim3((nlines + Tz - 1) / Tz), dim3(fft_threads<R>()), fft_lds<R>(Tz, n2), s, Q, S,
                        p.ax[2], Tz, log2i(Tz), p.tw[2], nlines, -1);
-    // 2. y forward
-    hipLaunchKernelGGL(k_fft_y<R>, dim3(n2h * nxb), dim3(fft_threads<R>()), fft_lds<R>(Ty, n1), s, S, p.ax[1], Ty, p.tw[1],
-                       n0, nxb, -1);
-    // 3. x forward, convolution, x backward
-    hipLaunchKernelGGL(k_fft_x_convolve<R>, dim3(n2h * nyb), dim3(fft_threads<R>()), fft_lds<R>(Tx, n0), s, S, p.ax[0], Tx,
-                       log2i(Tx), p.tw[0], n1, n2, nyb, cp);
-    // 4. y backward
-    hipLaunchKernelGGL(k_fft_y<R>, dim3(n2h * nxb), dim3(fft_threads<R>()), fft_lds<R>(Ty, n1), s, S, p.ax[1], Ty, p.tw[1],
-                       n0, nxb, 1);
+    if (!plane_pass<R>(s, p, S, n2h, n2, cp)) {
+        // 2. y forward
+        hipLaunchKernelGGL(k_fft_y<R>, dim3(n2h * nxb), dim3(fft_threads<R>()), fft_lds<R>(Ty, n1), s, S, p.ax[1], Ty,
+                           p.tw[1], n0, nxb, -1);
+        // 3. x forward, convolution, x backward
+        hipLaunchKernelGGL(k_fft_x_convolve<R>, dim3(n2h * nyb), dim3(fft_threads<R>()), fft_lds<R>(Tx, n0), s, S,
+                           p.ax[0], Tx, log2i(Tx), p.tw[0], n1, n2, nyb, cp);
+        // 4. y backward
+        hipLaunchKernelGGL(k_fft_y<R>, dim3(n2h * nxb), dim3(fft_threads<R>()), fft_lds<R>(Ty, n1), s, S, p.ax[1], Ty,
+                           p.tw[1], n0, nxb, 1);
+    }
     // 5. z backward
     hipLaunchKernelGGL(k_fft_z<R>, dim3((nlines + Tz - 1) / Tz), dim3(fft_threads<R>()), fft_lds<R>(Tz, n2), s, Q, S,
                        p.ax[2], Tz, log2i(Tz), p.tw[2], nlines, 1);
