@@ -92,9 +92,12 @@ typedef struct {
 } mpid_stats;
 
 /* Per-kernel-family device time of the executes since profiling was enabled
- * (HIP events on the context's stream).  MPID_PROF_FFT covers the whole
- * reciprocal convolution (five passes, the influence function inside the x
- * pass), so MPID_PROF_CONVOLVE stays zero. */
+ * (HIP events on the stream that runs the family's kernels; the reciprocal
+ * chains run on a second stream beside the real-space kernels, so the
+ * families' times can overlap).  MPID_PROF_FFT covers the whole reciprocal
+ * convolution (the influence function is applied inside the FFT passes), so
+ * MPID_PROF_CONVOLVE stays zero.  MPID_PROF_ELECTROSTATIC_PAIRS times the
+ * one real-space electrostatics kernel alone. */
 enum {
     MPID_PROF_SORT = 0, MPID_PROF_ROTATE, MPID_PROF_TILES, MPID_PROF_SPREAD, MPID_PROF_FFT, MPID_PROF_CONVOLVE,
     MPID_PROF_GATHER, MPID_PROF_FIXED_PAIRS, MPID_PROF_INDUCED_PAIRS, MPID_PROF_ELECTROSTATIC_PAIRS,
@@ -141,7 +144,14 @@ int mpid_compute_induced_field(mpid_context* ctx, const double* positions, const
                                double* field_out);
 
 int mpid_get_stats(const mpid_context* ctx, mpid_stats* stats);
+/* Profiling (no counterpart in the reference): mpid_set_profiling times every
+ * family (enable != 0) or none; mpid_set_profile_mask times the families
+ * whose bit (1 << MPID_PROF_x) is set.  When a family issued inside a solver
+ * iteration is timed (spread, FFT, gather, induced pairs, covalent pairs,
+ * solver), the iterations run as individual launches; otherwise they are
+ * replayed from a captured graph, as without profiling. */
 int mpid_set_profiling(mpid_context* ctx, int enable);
+int mpid_set_profile_mask(mpid_context* ctx, unsigned mask);
 /* times_ms[MPID_PROF_COUNT], counts[MPID_PROF_COUNT] (launches); resets them when reset != 0 */
 int mpid_get_profile(mpid_context* ctx, double* times_ms, int* counts, int reset);
 

@@ -86,6 +86,7 @@ SIGNATURES = {
     "mpid_compute_induced_field": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp]),
     "mpid_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Stats)]),
     "mpid_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "mpid_set_profile_mask": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint]),
     "mpid_get_profile": (ctypes.c_int, [ctypes.c_void_p, _dp, _ip, ctypes.c_int]),
     "mpid_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
     "mpid_create_error": (ctypes.c_char_p, []),

@@ -281,16 +281,13 @@ __device__ __forceinline__ void electrostatics_tile(const int* __restrict__ rows
         const int side = role == 3 ? 0 : 1;
         const int a = (side == 0 ? I : J) * kTile + lane;
         if (a < n) {
-            double v[6];
+            // component by component: a bare charge takes no torque, and an
+            // atom without pairs in the tile nothing at all
 #pragma unroll
-            for (int k = 0; k < 6; k++) v[k] = acc[side][k][lane];
-            if (v[0] != 0.0 || v[1] != 0.0 || v[2] != 0.0 || v[3] != 0.0 || v[4] != 0.0 || v[5] != 0.0) {
-                acc_add(&force[3 * (size_t)a + 0], v[0]);
-                acc_add(&force[3 * (size_t)a + 1], v[1]);
-                acc_add(&force[3 * (size_t)a + 2], v[2]);
-                acc_add(&torque[3 * (size_t)a + 0], v[3]);
-                acc_add(&torque[3 * (size_t)a + 1], v[4]);
-                acc_add(&torque[3 * (size_t)a + 2], v[5]);
+            for (int k = 0; k < 3; k++) {
+                const double f = acc[side][k][lane], t = acc[side][3 + k][lane];
+                if (f != 0.0) acc_add(&force[3 * (size_t)a + k], f);
+                if (t != 0.0) acc_add(&torque[3 * (size_t)a + k], t);
             }
         }
     }

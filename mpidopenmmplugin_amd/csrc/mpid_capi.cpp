@@ -246,8 +246,13 @@ struct mpid_context {
     double energy_correction = 0.0;
     bool correction_pending = false;  // the solver's correction dot is on its way to hscal[2]
 
-    // profiling
-    bool profiling = false;
+    // profiling: bit c of prof_mask times the work booked to category c
+    unsigned prof_mask = 0;
+    bool capturing = false;  // recording the solver's iteration graph: no timing events
+    // the categories of the work inside one solver iteration; timing any of
+    // them makes the iterations run eagerly instead of from the graph
+    static constexpr unsigned kIterationCats = (1u << P_SPREAD) | (1u << P_FFT) | (1u << P_GATHER) |
+                                               (1u << P_INDUCED) | (1u << P_COV) | (1u << P_SOLVER);
     struct Ev { int cat; hipEvent_t a, b; };
     std::vector<Ev> events;
     std::vector<hipEvent_t> pool;
@@ -278,11 +283,11 @@ struct mpid_context {
     int prof_override = -1;  // when set, every timed region is booked to this category
     // device time of the work that f issues to stream s, booked to cat
     template <class F> void timed_on(hipStream_t s, int cat, F&& f) {
-        if (!profiling) {
+        if (prof_override >= 0) cat = prof_override;
+        if (capturing || !((prof_mask >> cat) & 1u)) {
             f();
             return;
         }
-        if (prof_override >= 0) cat = prof_override;
         hipEvent_t a = get_event(), b = get_event();
         HIP_CHECK(hipEventRecord(a, s));
         f();
@@ -299,7 +304,7 @@ struct mpid_context {
         HIP_CHECK(hipStreamWaitEvent(stream, ev_join, 0));
     }
     void collect_profile() {
-        if (!profiling) return;
+        if (!prof_mask) return;
         HIP_CHECK(hipStreamSynchronize(rstream));
         HIP_CHECK(hipStreamSynchronize(stream));
         for (auto& e : events) {
@@ -907,15 +912,18 @@ struct mpid_context {
     template <class F> void capture_cg(F&& f) {
         hipGraph_t g = nullptr;
         HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        capturing = true;
         try {
             f();
             HIP_CHECK(hipMemcpyAsync(hscal, scal.p + 3, sizeof(double), hipMemcpyDeviceToHost, stream));
         } catch (...) {
+            capturing = false;
             hipGraph_t dead = nullptr;
             (void)hipStreamEndCapture(stream, &dead);
             if (dead) (void)hipGraphDestroy(dead);
             throw;
         }
+        capturing = false;
         HIP_CHECK(hipStreamEndCapture(stream, &g));
         bool updated = false;
         if (cg_exec) {
@@ -970,9 +978,11 @@ struct mpid_context {
             });
         };
         // The iteration graph is recorded first: the host does it while the
-        // device still runs the work issued before.  (Per-kernel timing
-        // events cannot sit in a graph: with profiling the loop runs eagerly.)
-        if (!profiling && iters < max_iter) capture_cg(body);
+        // device still runs the work issued before.  (Timing events cannot sit
+        // in a graph: when the iteration's categories are timed, the loop runs
+        // eagerly.)
+        const bool eager = (prof_mask & kIterationCats) != 0;
+        if (!eager && iters < max_iter) capture_cg(body);
         timed(P_SOLVER, [&] {
             if (zero_start) {
                 launch_fill<double>(stream, sol, n3, 0.0);
@@ -993,7 +1003,7 @@ struct mpid_context {
         });
         double eps = debye_eps(read_scalar(scal.p + 3));
         check_lists_pending();
-        if (profiling) {
+        if (eager) {
             while (eps >= stop_eps && iters < max_iter) {
                 body();
                 check_launch();
@@ -1086,10 +1096,13 @@ struct mpid_context {
         timed(P_SOLVER, [&] { launch_apply_S<R>(stream, n, sqrtpol.p, x.p, P.mu.p, mud.p); });
         mu_valid = true;
         recip_cycle_induced(P, true);  // phidp, on rstream beside the pair kernels
-        timed(P_ELEC, [&] {
+        timed(P_OTHER, [&] {
             launch_zero_acc(stream, force_acc.p, force_acc.n);
             launch_zero_acc(stream, torque_acc.p, torque_acc.n);
             launch_zero_acc(stream, energy_acc.p, 1);
+        });
+        // this category's events bracket the one kernel (its live time in the benchmark)
+        timed(P_ELEC, [&] {
             launch_electrostatics_pairs<R>(stream, rows_total, brows.p, brow_owner.p, tmask.p, tmaskcc.p, chg.p,
                                            pair_atoms(P), P.mu.p, pair_params<R>(), pd.pos.p, pd.lab.p,
                                            pair_params<double>(), force_acc.p, torque_acc.p, energy_acc.p);
@@ -1362,10 +1375,13 @@ int mpid_get_stats(const mpid_context* ctx, mpid_stats* stats) {
     return MPID_OK;
 }
 
-int mpid_set_profiling(mpid_context* ctx, int enable) {
+int mpid_set_profile_mask(mpid_context* ctx, unsigned mask) {
     if (!ctx) return MPID_ERR_INVALID;
-    ctx->profiling = enable != 0;
+    ctx->prof_mask = mask & ((1u << MPID_PROF_COUNT) - 1u);
     return MPID_OK;
+}
+int mpid_set_profiling(mpid_context* ctx, int enable) {
+    return mpid_set_profile_mask(ctx, enable ? ~0u : 0u);
 }
 
 int mpid_get_profile(mpid_context* ctx, double* times_ms, int* counts, int reset) {

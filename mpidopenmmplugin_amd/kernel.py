@@ -155,6 +155,15 @@ class HipCalcMPIDForceKernel:
     def setProfiling(self, enable=True):
         self._check(self._lib.mpid_set_profiling(self._ctx, int(enable)))
 
+    def setProfileFamilies(self, names):
+        """Time only the named kernel families (see _capi.PROFILE_NAMES);
+        unless one of them runs inside a solver iteration, the iterations
+        keep their graph replays."""
+        mask = 0
+        for name in names:
+            mask |= 1 << _capi.PROFILE_NAMES.index(name)
+        self._check(self._lib.mpid_set_profile_mask(self._ctx, mask))
+
     def profile(self, reset=False):
         t = np.zeros(len(_capi.PROFILE_NAMES))
         c = np.zeros(len(_capi.PROFILE_NAMES), dtype=np.int32)
