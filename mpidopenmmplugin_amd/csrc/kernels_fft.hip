@@ -500,6 +500,129 @@ __global__ __launch_bounds__(plane::kThreads) void k_fft_plane128_convolve(R2<fl
 }
 
 // ---------------------------------------------------------------------------
+// FP32, n2 = 128: the z passes with the same register steps (16-point DFTs at
+// stride 8, then 8-point DFTs over contiguous groups), 64 lines per
+// workgroup.  Forward: the lines are staged in LDS with coalesced loads, the
+// two steps run, and the second writes X[k1 + 16 k2] straight to the half
+// spectrum S[kz][l] (kz <= 64), consecutive lines to consecutive words.
+// Backward: the first inverse step reads S (the other half by Hermitian
+// symmetry), the second leaves the real lines in LDS, and they are stored
+// with coalesced writes.  LDS: line t at t * 145 + 9 (z / 8) + z % 8 (a
+// pitch of 1 mod 16 complex words: 64 lines in one access spread over all
+// banks).
+// ---------------------------------------------------------------------------
+namespace zline {
+constexpr int N = 128, A = 16, B = 8, T = 64;
+constexpr int kThreads = 512;
+constexpr int kPitch = (N / B) * (B + 1) + 1;  // 145
+__device__ __forceinline__ int at(int t, int z) { return t * kPitch + (z >> 3) * (B + 1) + (z & 7); }
+}  // namespace zline
+
+__global__ __launch_bounds__(zline::kThreads) void k_fft_z128_r2c(const float* __restrict__ Q, R2<float>* __restrict__ S,
+                                                                  const R2<float>* __restrict__ tw, int nlines) {
+    using namespace zline;
+    typedef Cx<float> C;
+    __shared__ C L[T * kPitch];
+    __shared__ C W[N];
+    const int l0 = blockIdx.x * T;
+    if (threadIdx.x < N) W[threadIdx.x] = C{tw[threadIdx.x].x, tw[threadIdx.x].y};
+    {
+        // T lines of N reals: T * N / 4 float4 loads, 4 per thread
+        const float4* src = reinterpret_cast<const float4*>(Q + (size_t)l0 * N);
+        float4 v[T * N / 4 / kThreads];
+#pragma unroll
+        for (int u = 0; u < T * N / 4 / kThreads; u++) v[u] = src[threadIdx.x + u * kThreads];
+#pragma unroll
+        for (int u = 0; u < T * N / 4 / kThreads; u++) {
+            const int f = threadIdx.x + u * kThreads, t = f >> 5, z = (f & 31) * 4;
+            L[at(t, z)] = C{v[u].x, 0.f};
+            L[at(t, z + 1)] = C{v[u].y, 0.f};
+            L[at(t, z + 2)] = C{v[u].z, 0.f};
+            L[at(t, z + 3)] = C{v[u].w, 0.f};
+        }
+    }
+    __syncthreads();
+    {
+        const int t = threadIdx.x & (T - 1), b = threadIdx.x >> 6;
+        C v[A];
+#pragma unroll
+        for (int a = 0; a < A; a++) v[a] = L[at(t, a * B + b)];
+        plane::dft16(v, -1.f);
+#pragma unroll
+        for (int k1 = 0; k1 < A; k1++) {
+            C r = v[k1];
+            if (b * k1 != 0) r = cmul(r, plane::tw_s(W, b * k1, -1.f));
+            L[at(t, k1 * B + b)] = r;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int task = threadIdx.x + h * kThreads;
+        const int t = task & (T - 1), k1 = task >> 6;
+        C v[B];
+#pragma unroll
+        for (int j = 0; j < B; j++) v[j] = L[at(t, k1 * B + j)];
+        plane::dft8(v, -1.f);
+#pragma unroll
+        for (int k2 = 0; k2 < B; k2++) {
+            const int kz = k1 + A * k2;
+            if (kz <= N / 2) S[(size_t)kz * nlines + l0 + t] = R2<float>{v[k2].x, v[k2].y};
+        }
+    }
+}
+
+__global__ __launch_bounds__(zline::kThreads) void k_fft_z128_c2r(float* __restrict__ Q, const R2<float>* __restrict__ S,
+                                                                  const R2<float>* __restrict__ tw, int nlines) {
+    using namespace zline;
+    typedef Cx<float> C;
+    __shared__ C L[T * kPitch];
+    __shared__ C W[N];
+    const int l0 = blockIdx.x * T;
+    if (threadIdx.x < N) W[threadIdx.x] = C{tw[threadIdx.x].x, tw[threadIdx.x].y};
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int task = threadIdx.x + h * kThreads;
+        const int t = task & (T - 1), k1 = task >> 6;
+        C v[B];
+#pragma unroll
+        for (int k2 = 0; k2 < B; k2++) {
+            const int kz = k1 + A * k2;
+            const bool cj = kz > N / 2;
+            const R2<float> s = S[(size_t)(cj ? N - kz : kz) * nlines + l0 + t];
+            v[k2] = C{s.x, cj ? -s.y : s.y};
+        }
+        plane::dft8(v, 1.f);
+#pragma unroll
+        for (int j = 0; j < B; j++) {
+            C r = v[j];
+            if (j * k1 != 0) r = cmul(r, plane::tw_s(W, j * k1, 1.f));
+            L[at(t, k1 * B + j)] = r;
+        }
+    }
+    __syncthreads();
+    {
+        const int t = threadIdx.x & (T - 1), b = threadIdx.x >> 6;
+        C v[A];
+#pragma unroll
+        for (int k1 = 0; k1 < A; k1++) v[k1] = L[at(t, k1 * B + b)];
+        plane::dft16(v, 1.f);
+#pragma unroll
+        for (int a = 0; a < A; a++) L[at(t, a * B + b)] = v[a];
+    }
+    __syncthreads();
+    {
+        float4* dst = reinterpret_cast<float4*>(Q + (size_t)l0 * N);
+#pragma unroll
+        for (int u = 0; u < T * N / 4 / kThreads; u++) {
+            const int f = threadIdx.x + u * kThreads, t = f >> 5, z = (f & 31) * 4;
+            dst[f] = make_float4(L[at(t, z)].x, L[at(t, z + 1)].x, L[at(t, z + 2)].x, L[at(t, z + 3)].x);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 FastDiv make_fastdiv(unsigned d) {
@@ -545,6 +668,20 @@ static int log2i(int v) {
 }
 template <class R> static size_t fft_lds(int T, int n) { return (2 * (size_t)T * n + n) * sizeof(R2<R>); }
 
+// the z passes with the register-step kernels where they apply (FP32,
+// n2 = 128, whole batches of 64 lines)
+template <class R> static bool zline_pass(hipStream_t, const FftPlan<R>&, R*, R2<R>*, int) { return false; }
+template <>
+bool zline_pass<float>(hipStream_t s, const FftPlan<float>& p, float* Q, R2<float>* S, int dir) {
+    const int nlines = p.ax[0].n * p.ax[1].n;
+    if (p.ax[2].n != zline::N || nlines % zline::T != 0) return false;
+    if (dir < 0)
+        hipLaunchKernelGGL(k_fft_z128_r2c, dim3(nlines / zline::T), dim3(zline::kThreads), 0, s, Q, S, p.tw[2], nlines);
+    else
+        hipLaunchKernelGGL(k_fft_z128_c2r, dim3(nlines / zline::T), dim3(zline::kThreads), 0, s, Q, S, p.tw[2], nlines);
+    return true;
+}
+
 // passes 2-4 as one plane pass where that kernel applies (FP32, n0 = n1 = 128)
 template <class R> static bool plane_pass(hipStream_t, const FftPlan<R>&, R2<R>*, int, int, ConvParams<R>) {
     return false;
@@ -565,8 +702,9 @@ void launch_recip_convolution(hipStream_t s, const FftPlan<R>& p, R* Q, R2<R>* S
     const int Ty = lines_per_group<R>(n1), nxb = (n0 + Ty - 1) / Ty;
     const int Tx = lines_per_group<R>(n0), nyb = (n1 + Tx - 1) / Tx;
     // 1. z forward
-    hipLaunchKernelGGL(k_fft_z<R>, dim3((nlines + Tz - 1) / Tz), dim3(fft_threads<R>()), fft_lds<R>(Tz, n2), s, Q, S,
-                       p.ax[2], Tz, log2i(Tz), p.tw[2], nlines, -1);
+    if (!zline_pass<R>(s, p, Q, S, -1))
+        hipLaunchKernelGGL(k_fft_z<R>, dim3((nlines + Tz - 1) / Tz), dim3(fft_threads<R>()), fft_lds<R>(Tz, n2), s, Q,
+                           S, p.ax[2], Tz, log2i(Tz), p.tw[2], nlines, -1);
     if (!plane_pass<R>(s, p, S, n2h, n2, cp)) {
         // 2. y forward
         hipLaunchKernelGGL(k_fft_y<R>, dim3(n2h * nxb), dim3(fft_threads<R>()), fft_lds<R>(Ty, n1), s, S, p.ax[1], Ty,
@@ -579,8 +717,9 @@ void launch_recip_convolution(hipStream_t s, const FftPlan<R>& p, R* Q, R2<R>* S
                            p.tw[1], n0, nxb, 1);
     }
     // 5. z backward
-    hipLaunchKernelGGL(k_fft_z<R>, dim3((nlines + Tz - 1) / Tz), dim3(fft_threads<R>()), fft_lds<R>(Tz, n2), s, Q, S,
-                       p.ax[2], Tz, log2i(Tz), p.tw[2], nlines, 1);
+    if (!zline_pass<R>(s, p, Q, S, 1))
+        hipLaunchKernelGGL(k_fft_z<R>, dim3((nlines + Tz - 1) / Tz), dim3(fft_threads<R>()), fft_lds<R>(Tz, n2), s, Q,
+                           S, p.ax[2], Tz, log2i(Tz), p.tw[2], nlines, 1);
 }
 
 template void launch_recip_convolution<float>(hipStream_t, const FftPlan<float>&, float*, R2<float>*,
