@@ -148,15 +148,18 @@ __device__ __forceinline__ void electrostatics_tile(const int* __restrict__ rows
 #pragma unroll
         for (int k = 0; k < 6; k++) acc[side][k][lane] = 0.0;
     }
-#pragma unroll
-    for (int side = 0; side < 2; side++) {
-        // the 64 x 20 multipole block is contiguous; read it coalesced
-        const int a0 = (side == 0 ? I : J) * kTile;
-        const R* src = A.lab + (size_t)a0 * kMP;
-        const int lim = (min(a0 + kTile, n) - a0) * kMP;
-        for (int e = tid; e < kTile * kMP; e += kElecThreads) {
-            const int at = e / kMP, k = e - at * kMP;
-            smp[side][at * kS + k] = e < lim ? src[e] : R(0);
+    {
+        // the 64 x 20 multipole block is contiguous in HBM and in LDS: copy it
+        // as 16-byte vectors (atoms past the last one read as zeros)
+        typedef R vec __attribute__((ext_vector_type(kV)));
+        constexpr int kVecs = kTile * kMP / kV;  // per side
+        for (int e = tid; e < 2 * kVecs; e += kElecThreads) {
+            const int side = e >= kVecs, v = e - side * kVecs;
+            const int a0 = (side == 0 ? I : J) * kTile;
+            const int lim = (min(a0 + kTile, n) - a0) * (kMP / kV);
+            const vec* src = reinterpret_cast<const vec*>(A.lab + (size_t)a0 * kMP);
+            vec* dst = reinterpret_cast<vec*>(&smp[side][0]);
+            dst[v] = v < lim ? src[v] : vec(R(0));
         }
     }
     __syncthreads();
