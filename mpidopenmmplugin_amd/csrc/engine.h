@@ -96,10 +96,12 @@ void launch_cg_direction(hipStream_t s, int m, const double* rr_old, const doubl
 // One CG iteration once the operator's field for mu = S p is complete (acc,
 // erec): Ap, a = rr / pAp, x += a p, r -= a Ap, the norms rr and |S r|^2,
 // beta, p = r + beta p and the next operator input mu = S p.  Scalars in
-// scal: [0] rr, [1] pAp, [2] rr_new, [3] |S r|^2, [7] beta.
+// scal: [0] rr, [1] pAp, [2] rr_new, [3] |S r|^2, [7] beta; hss receives a
+// host-visible copy of |S r|^2 (pinned, mapped host memory).
 template <class R>
 void launch_cg_iteration(hipStream_t s, int n, const double* sqrtpol, const acc_t* acc, const R* erec, R* mu,
-                         double selfc, double* p, double* Ap, double* x, double* r, double* partial, double* scal);
+                         double selfc, double* p, double* Ap, double* x, double* r, double* partial, double* scal,
+                         double* hss);
 void launch_residual_norm(hipStream_t s, int n, const double* sqrtpol, const double* r, double* partial,
                           double* result);
 void launch_axpy(hipStream_t s, int m, double a, const double* x, double* y);

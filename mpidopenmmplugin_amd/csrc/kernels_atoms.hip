@@ -411,9 +411,10 @@ __global__ __launch_bounds__(kRedThreads) void k_matvec_finish_dot(int n, const 
 }
 
 // One block: the partial sums of k_cg_step give rr_new (scal[2], scal[0]) and
-// ss (scal[3]); beta = rr_new / rr_old goes to scal[7].  Thread 0 reads the
-// old scal[0] before it writes the new one.
-__global__ __launch_bounds__(kRedThreads) void k_cg_reduce_beta(const double* __restrict__ partial, double* scal) {
+// ss (scal[3], and *hss, a host-visible copy); beta = rr_new / rr_old goes
+// to scal[7].  Thread 0 reads the old scal[0] before it writes the new one.
+__global__ __launch_bounds__(kRedThreads) void k_cg_reduce_beta(const double* __restrict__ partial, double* scal,
+                                                                double* hss) {
     __shared__ double sh[kRedThreads];
     double rr = 0, ss = 0;
     for (int i = threadIdx.x; i < kRedBlocks; i += kRedThreads) {
@@ -428,6 +429,7 @@ __global__ __launch_bounds__(kRedThreads) void k_cg_reduce_beta(const double* __
         scal[0] = rr;
         scal[2] = rr;
         scal[3] = ss;
+        *hss = ss;
     }
 }
 
@@ -896,14 +898,16 @@ void launch_residual_norm(hipStream_t s, int n, const double* S, const double* r
 // One CG iteration after the operator's field is complete (scal slots:
 // [0] rr, [1] pAp, [2] rr_new, [3] ss, [7] beta): Ap and pAp, x += a p,
 // r -= a Ap with their norms, then the next direction p and its mu = S p.
+// Deterministic reductions: per-block partials, then one block in block
+// order.
 template <class R>
 void launch_cg_iteration(hipStream_t s, int n, const double* S, const acc_t* acc, const R* erec, R* mu, double selfc,
-                         double* p, double* Ap, double* x, double* r, double* partial, double* scal) {
+                         double* p, double* Ap, double* x, double* r, double* partial, double* scal, double* hss) {
     hipLaunchKernelGGL(k_matvec_finish_dot<R>, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, S, acc, erec, mu, selfc,
                        p, Ap, partial);
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kRedThreads), 0, s, partial, 1, scal + 1);
     hipLaunchKernelGGL(k_cg_step, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, S, scal, p, Ap, x, r, partial);
-    hipLaunchKernelGGL(k_cg_reduce_beta, dim3(1), dim3(kRedThreads), 0, s, partial, scal);
+    hipLaunchKernelGGL(k_cg_reduce_beta, dim3(1), dim3(kRedThreads), 0, s, partial, scal, hss);
     hipLaunchKernelGGL(k_cg_direction_apply<R>, blocks_for(n, 256), dim3(256), 0, s, n, S, scal, r, p, mu);
 }
 void launch_cg_direction(hipStream_t s, int m, const double* rr_old, const double* rr_new, const double* r,
@@ -940,7 +944,7 @@ void launch_recip_atoms(hipStream_t s, int n, int mutual, const double* phi, con
     template void launch_matvec_finish<R>(hipStream_t, int, const double*, const acc_t*, const R*, const R*, double, \
                                           const double*, double*);                                                  \
     template void launch_cg_iteration<R>(hipStream_t, int, const double*, const acc_t*, const R*, R*, double,       \
-                                         double*, double*, double*, double*, double*, double*);                     \
+                                         double*, double*, double*, double*, double*, double*, double*);            \
     template void launch_recip_atoms<R>(hipStream_t, int, int, const double*, const R*, const double*, const double*,\
                                         const int*, double3, double, acc_t*, acc_t*, acc_t*);                      \
     template void launch_field_sum<R>(hipStream_t, int, const acc_t*, const R*, const R*, double, double*);

@@ -185,7 +185,9 @@ struct mpid_context {
     // kernel arguments are those of that solve); hscal is the pinned host
     // copy of the residual norm that each replay leaves behind.
     hipGraphExec_t cg_exec = nullptr;
-    double* hscal = nullptr;  // [0] residual norm of a replay, [1] read_scalar, [2] energy-correction dot
+    double* hscal_dev = nullptr;  // device address of hscal (pinned, mapped)
+    double* hscal = nullptr;  // [0] residual norm of a replay, [1] read_scalar, [2] energy-correction dot,
+                              // [3] (as int) the block-row count
     hipEvent_t ev_sync = nullptr;
     std::string error;
     mpid_stats stats{};
@@ -529,7 +531,7 @@ struct mpid_context {
         tmp3n.resize(n3);
         fout.resize(n3);
         partial.resize(2 * 512);
-        scal.resize(8);
+        scal.resize(16);
         choose_tiles();
         if (opt.precision == MPID_PRECISION_MIXED) init_prec(pf);
         init_prec(pd);  // FP64 path (also the refinement residual of the mixed path)
@@ -669,8 +671,10 @@ struct mpid_context {
             launch_convert<double>(stream, n, spos.p, lab.p, sthole.p, damp.p, pd.pos.p, pd.lab.p, pd.td.p);
         });
         check_launch();
-        if (fixed_recip) recip_cycle_fixed(pd);
-        // blocks, block rows, neighbour lists and tile masks
+        // blocks, block rows, neighbour lists and tile masks.  The row count
+        // is issued before the FP64 reciprocal cycle of the permanent
+        // multipoles, so that the host issues that cycle while the device
+        // counts, and it is read back only then.
         const int nb = (n + kTile - 1) / kTile;
         timed(P_TILES, [&] {
             launch_block_bounds<double>(stream, n, pd.pos.p, bmin.p, bmax.p);
@@ -679,8 +683,10 @@ struct mpid_context {
             launch_exclusive_scan(stream, nb, brow_count.p, brow_start.p);
         });
         check_launch();
-        HIP_CHECK(hipMemcpyAsync(&rows_total, brow_start.p + nb, sizeof(int), hipMemcpyDeviceToHost, stream));
-        HIP_CHECK(hipStreamSynchronize(stream));
+        HIP_CHECK(hipMemcpyAsync(hscal + 3, brow_start.p + nb, sizeof(int), hipMemcpyDeviceToHost, stream));
+        if (fixed_recip) recip_cycle_fixed(pd);
+        wait_stream();
+        std::memcpy(&rows_total, hscal + 3, sizeof(int));
         if ((size_t)rows_total > brows.n) {
             const size_t cap = rows_total + rows_total / 4 + 64;
             brows.resize(cap);
@@ -907,8 +913,8 @@ struct mpid_context {
         stats.matvecs++;
     }
 
-    // Records f (one CG iteration on stream and rstream) and a copy of the
-    // residual norm scal[3] to hscal into cg_exec: the executable graph is
+    // Records f (one CG iteration on stream and rstream, which leaves the
+    // residual norm in hscal[0]) into cg_exec: the executable graph is
     // updated in place when its topology is unchanged, else instantiated.
     template <class F> void capture_cg(F&& f) {
         hipGraph_t g = nullptr;
@@ -916,7 +922,6 @@ struct mpid_context {
         capturing = true;
         try {
             f();
-            HIP_CHECK(hipMemcpyAsync(hscal, scal.p + 3, sizeof(double), hipMemcpyDeviceToHost, stream));
         } catch (...) {
             capturing = false;
             hipGraph_t dead = nullptr;
@@ -975,7 +980,7 @@ struct mpid_context {
             join();
             timed(P_SOLVER, [&] {
                 launch_cg_iteration<R>(stream, n, sqrtpol.p, field_acc.p, P.erec.p, P.mu.p, selfc(), p.p, Ap.p, sol,
-                                       r.p, partial.p, scal.p);
+                                       r.p, partial.p, scal.p, hscal_dev);
             });
         };
         // The iteration graph is recorded first: the host does it while the
@@ -1290,7 +1295,9 @@ int mpid_create(const mpid_parameters* params, const mpid_options* opts, mpid_co
         HIP_CHECK(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
         HIP_CHECK(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
         HIP_CHECK(hipEventCreateWithFlags(&ctx->ev_sync, hipEventDisableTiming));
-        HIP_CHECK(hipHostMalloc((void**)&ctx->hscal, 8 * sizeof(double), 0));
+        HIP_CHECK(hipHostMalloc((void**)&ctx->hscal, 8 * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent));
+        HIP_CHECK(hipHostGetDevicePointer((void**)&ctx->hscal_dev, ctx->hscal, 0));
+
         HIP_CHECK(hipHostMalloc((void**)&ctx->hnl, 4 * sizeof(int), 0));
         ctx->set_options(opts);
         ctx->set_parameters(params, true);
