@@ -32,6 +32,11 @@ __device__ __forceinline__ void acc_add(acc_t* p, double v) {
 }
 __host__ __device__ __forceinline__ double acc_get(acc_t v) { return (double)(long long)v / kFixed; }
 
+// Kernels of a solver iteration take the solver's stop flag: the iteration
+// was issued ahead of the host's convergence test and the solve has stopped,
+// so the kernel returns at once (before any barrier; the flag is uniform).
+__device__ __forceinline__ bool skipped(const double* flag) { return flag != nullptr && *flag != 0.0; }
+
 template <class R> struct V3 {
     R x, y, z;
 };

@@ -27,6 +27,7 @@ template <class R> struct PairParams {
     R scale14;
     int mutual;        // 1 when the induced-induced terms enter the forces
     int fix_torque;    // 1: energy-conserving induced-induced torques (MPID_FLAG_CONSERVATIVE_TORQUES)
+    const double* skip;  // the solver's stop flag in the induced-field kernels of a solver iteration, else null
 };
 
 // Precision-specific per-atom inputs of the pair kernels (sorted order).
@@ -67,7 +68,7 @@ void launch_torque_to_force(hipStream_t s, int n, Box<double> box, const double4
                             const acc_t* torque, acc_t* force);
 void launch_output_forces(hipStream_t s, int n, const int* perm, const acc_t* force, double* out, int accumulate);
 void launch_unpermute3(hipStream_t s, int n, const int* perm, const double* in, double* out);
-void launch_zero_acc(hipStream_t s, acc_t* p, size_t count);
+void launch_zero_acc(hipStream_t s, acc_t* p, size_t count, const double* skip = nullptr);
 size_t sort_temp_bytes(int n);
 int launch_sort_pairs(hipStream_t s, void* temp, size_t bytes, unsigned* kin, unsigned* kout, int* vin, int* vout, int n);
 // out = acc + erecip + selfc * mu (field of the induced dipoles, double)
@@ -89,19 +90,26 @@ void launch_matvec_finish(hipStream_t s, int n, const double* sqrtpol, const acc
                           const R* mu, double selfc, const double* p, double* Ap);
 // deterministic FP64 reductions
 void launch_dot(hipStream_t s, int m, const double* a, const double* b, double* partial, double* result);
-void launch_cg_step(hipStream_t s, int n, const double* sqrtpol, const double* scal /*rr, pAp*/, const double* p,
-                    const double* Ap, double* x, double* r, double* partial, double* result /*rr_new, ss*/);
-void launch_cg_direction(hipStream_t s, int m, const double* rr_old, const double* rr_new, const double* r,
-                         double* p);
+// The solver's device-side state in scal: [0] rr, [1] pAp, [2] rr_new,
+// [3] |S r|^2, [7] beta, [kCgCount] iterations done, [kCgStop] 1 once the
+// stopping test has passed.  Kernels of an iteration given the stop flag
+// return at once when it is set, so that the host may issue an iteration
+// before it has read the previous one's check.
+constexpr int kCgCount = 9;
+constexpr int kCgStop = 10;
+constexpr int kScalars = 16;
+// Check 0 of a solve: the stopping test on the initial |S r|^2 (scal[3]),
+// published to scal and to hcheck (pinned, mapped host memory, two slots of
+// (|S r|^2, stop); check c goes to slot c & 1).
+void launch_cg_begin(hipStream_t s, int n, double* scal, double* hcheck, double stop_eps);
 // One CG iteration once the operator's field for mu = S p is complete (acc,
 // erec): Ap, a = rr / pAp, x += a p, r -= a Ap, the norms rr and |S r|^2,
-// beta, p = r + beta p and the next operator input mu = S p.  Scalars in
-// scal: [0] rr, [1] pAp, [2] rr_new, [3] |S r|^2, [7] beta; hss receives a
-// host-visible copy of |S r|^2 (pinned, mapped host memory).
+// beta, the iteration's check, p = r + beta p and the next operator input
+// mu = S p.
 template <class R>
 void launch_cg_iteration(hipStream_t s, int n, const double* sqrtpol, const acc_t* acc, const R* erec, R* mu,
                          double selfc, double* p, double* Ap, double* x, double* r, double* partial, double* scal,
-                         double* hss);
+                         double* hcheck, double stop_eps);
 void launch_residual_norm(hipStream_t s, int n, const double* sqrtpol, const double* r, double* partial,
                           double* result);
 void launch_axpy(hipStream_t s, int m, double a, const double* x, double* y);
@@ -117,6 +125,7 @@ template <class R> struct PmeGeom {
     int n[3];
     R gscale[3];      // n_k / L_k
     Box<R> box;
+    const double* skip;  // the solver's stop flag in the spreading and gathering of a solver iteration, else null
 };
 // Morton-aligned spreading tiles: 2^(3m) tiles of T_k grid points, windows of W_k points
 struct TileGeom {
@@ -188,6 +197,7 @@ template <class R> struct ConvParams {
     R sc;            // 1 / (pi V)
     R inv[3];        // 1 / box edge
     const R* moduli; // |b(k)|^2, [n0 | n1 | n2]
+    const double* skip;  // the solver's stop flag in the FFT passes of a solver iteration, else null
 };
 // Fold for tilings with W = T + 8 <= 2T in every dimension: one workgroup per
 // Morton cell sums, for each point of its T^3 core, its own window and those
@@ -195,6 +205,7 @@ template <class R> struct ConvParams {
 bool fold_by_tiles(const TileGeom& tg);
 template <class R> void launch_fold_tiles(hipStream_t s, PmeGeom<R> g, TileGeom tg, const R* scratch, R* grid);
 // Q (real grid) -> S (half spectrum) -> convolution -> Q (potential grid)
+// (every pass returns at once when cp.skip is set)
 template <class R>
 void launch_recip_convolution(hipStream_t s, const FftPlan<R>& p, R* Q, R2<R>* S, ConvParams<R> cp);
 

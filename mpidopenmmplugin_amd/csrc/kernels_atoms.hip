@@ -322,7 +322,8 @@ __global__ __launch_bounds__(kRedThreads) void k_dot_partial(int m, const double
 
 // Sums kRedBlocks partials per output slot; nout outputs stored back to back.
 __global__ __launch_bounds__(kRedThreads) void k_reduce_partials(const double* __restrict__ partial, int nout,
-                                                                 double* result) {
+                                                                 double* result, const double* skip) {
+    if (skipped(skip)) return;
     __shared__ double sh[kRedThreads];
     for (int o = 0; o < nout; o++) {
         double v = 0;
@@ -337,6 +338,7 @@ __global__ __launch_bounds__(kRedThreads) void k_cg_step(int n, const double* __
                                                          const double* __restrict__ scal,
                                                          const double* __restrict__ p, const double* __restrict__ Ap,
                                                          double* x, double* r, double* partial) {
+    if (scal[kCgStop] != 0.0) return;
     __shared__ double sh[kRedThreads];
     const double pAp = scal[1];
     const double a = pAp != 0.0 ? scal[0] / pAp : 0.0;
@@ -388,7 +390,8 @@ __global__ __launch_bounds__(kRedThreads) void k_matvec_finish_dot(int n, const 
                                                                    const R* __restrict__ erec,
                                                                    const R* __restrict__ mu, double selfc,
                                                                    const double* __restrict__ p, double* Ap,
-                                                                   double* partial) {
+                                                                   double* partial, const double* __restrict__ scal) {
+    if (scal[kCgStop] != 0.0) return;
     __shared__ double sh[kRedThreads];
     double v = 0;
     for (int s = blockIdx.x * kRedThreads + threadIdx.x; s < n; s += kRedBlocks * kRedThreads) {
@@ -410,11 +413,38 @@ __global__ __launch_bounds__(kRedThreads) void k_matvec_finish_dot(int n, const 
     if (threadIdx.x == 0) partial[blockIdx.x] = v;
 }
 
+// The solver's stopping test, the same expression as the host's report:
+// converged when the RMS of S r, in Debye, is below stop_eps.
+__device__ __forceinline__ double cg_stopped(double ss, int n, double stop_eps) {
+    return kDebye * sqrt(ss / (double)n) >= stop_eps ? 0.0 : 1.0;
+}
+
+// Publishes check c of a solve (the state after c iterations): scal[kCgCount]
+// = c, scal[kCgStop] = the stopping test, and in host slot c & 1 the pair
+// (|S r|^2, stop).  Slots alternate so that the host can read check c while
+// the iteration after it is already running.
+__device__ __forceinline__ void cg_publish(double* scal, double* hcheck, double c, double ss, double stop) {
+    scal[kCgCount] = c;
+    scal[kCgStop] = stop;
+    double* slot = hcheck + 2 * ((int)c & 1);
+    slot[0] = ss;
+    slot[1] = stop;
+}
+
+// Check 0 of a solve, from the initial residual's |S r|^2 in scal[3].
+__global__ void k_cg_begin(double* scal, double* hcheck, int n, double stop_eps) {
+    if (threadIdx.x != 0) return;
+    const double ss = scal[3];
+    cg_publish(scal, hcheck, 0.0, ss, cg_stopped(ss, n, stop_eps));
+}
+
 // One block: the partial sums of k_cg_step give rr_new (scal[2], scal[0]) and
-// ss (scal[3], and *hss, a host-visible copy); beta = rr_new / rr_old goes
-// to scal[7].  Thread 0 reads the old scal[0] before it writes the new one.
+// ss (scal[3]); beta = rr_new / rr_old goes to scal[7], then the iteration's
+// check is published.  Thread 0 reads the old scal[0] before it writes the new
+// one.
 __global__ __launch_bounds__(kRedThreads) void k_cg_reduce_beta(const double* __restrict__ partial, double* scal,
-                                                                double* hss) {
+                                                                double* hcheck, int n, double stop_eps) {
+    if (scal[kCgStop] != 0.0) return;
     __shared__ double sh[kRedThreads];
     double rr = 0, ss = 0;
     for (int i = threadIdx.x; i < kRedBlocks; i += kRedThreads) {
@@ -429,14 +459,17 @@ __global__ __launch_bounds__(kRedThreads) void k_cg_reduce_beta(const double* __
         scal[0] = rr;
         scal[2] = rr;
         scal[3] = ss;
-        *hss = ss;
+        cg_publish(scal, hcheck, scal[kCgCount] + 1.0, ss, cg_stopped(ss, n, stop_eps));
     }
 }
 
 // p = r + beta p (beta = scal[7]), fused with the next operator input mu = S p.
+// Skipped once the solve has stopped, also after the iteration that stops it:
+// the solver does not read p or mu after its last check.
 template <class R>
 __global__ void k_cg_direction_apply(int n, const double* __restrict__ S, const double* __restrict__ scal,
                                      const double* __restrict__ r, double* p, R* mu) {
+    if (scal[kCgStop] != 0.0) return;
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const double beta = scal[7];
@@ -453,12 +486,6 @@ __global__ void k_cg_direction_apply(int n, const double* __restrict__ S, const 
     mu[3 * (size_t)s + 2] = (R)v.z;
 }
 
-__global__ void k_cg_direction(int m, const double* __restrict__ rr_old, const double* __restrict__ rr_new,
-                               const double* __restrict__ r, double* p) {
-    const double beta = rr_old[0] != 0.0 ? rr_new[0] / rr_old[0] : 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) p[i] = r[i] + beta * p[i];
-}
-
 __global__ void k_axpy(int m, double a, const double* __restrict__ x, double* y) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) y[i] += a * x[i];
 }
@@ -466,7 +493,8 @@ __global__ void k_sub(int m, const double* __restrict__ a, const double* __restr
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) out[i] = a[i] - b[i];
 }
 
-__global__ void k_zero_acc(acc_t* p, size_t count) {
+__global__ void k_zero_acc(acc_t* p, size_t count, const double* skip) {
+    if (skipped(skip)) return;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
         p[i] = 0;
 }
@@ -856,10 +884,10 @@ void launch_output_forces(hipStream_t s, int n, const int* perm, const acc_t* fo
 void launch_unpermute3(hipStream_t s, int n, const int* perm, const double* in, double* out) {
     hipLaunchKernelGGL(k_unpermute3, blocks_for(n, 256), dim3(256), 0, s, n, perm, in, out);
 }
-void launch_zero_acc(hipStream_t s, acc_t* p, size_t count) {
+void launch_zero_acc(hipStream_t s, acc_t* p, size_t count, const double* skip) {
     int b = (int)std::min<size_t>((count + 255) / 256, 4096);
     if (b < 1) b = 1;
-    hipLaunchKernelGGL(k_zero_acc, dim3(b), dim3(256), 0, s, p, count);
+    hipLaunchKernelGGL(k_zero_acc, dim3(b), dim3(256), 0, s, p, count, skip);
 }
 template <class R> void launch_fill(hipStream_t s, R* p, size_t count, R v) {
     int b = (int)std::min<size_t>((count + 255) / 256, 8192);
@@ -884,37 +912,31 @@ void launch_matvec_finish(hipStream_t s, int n, const double* S, const acc_t* ac
 }
 void launch_dot(hipStream_t s, int m, const double* a, const double* b, double* partial, double* result) {
     hipLaunchKernelGGL(k_dot_partial, dim3(kRedBlocks), dim3(kRedThreads), 0, s, m, a, b, partial);
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kRedThreads), 0, s, partial, 1, result);
-}
-void launch_cg_step(hipStream_t s, int n, const double* S, const double* scal, const double* p, const double* Ap,
-                    double* x, double* r, double* partial, double* result) {
-    hipLaunchKernelGGL(k_cg_step, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, S, scal, p, Ap, x, r, partial);
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kRedThreads), 0, s, partial, 2, result);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kRedThreads), 0, s, partial, 1, result, nullptr);
 }
 void launch_residual_norm(hipStream_t s, int n, const double* S, const double* r, double* partial, double* result) {
     hipLaunchKernelGGL(k_residual_norm, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, S, r, partial);
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kRedThreads), 0, s, partial, 2, result);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kRedThreads), 0, s, partial, 2, result, nullptr);
 }
 // One CG iteration after the operator's field is complete (scal slots:
 // [0] rr, [1] pAp, [2] rr_new, [3] ss, [7] beta): Ap and pAp, x += a p,
-// r -= a Ap with their norms, then the next direction p and its mu = S p.
-// Deterministic reductions: per-block partials, then one block in block
-// order.
+// r -= a Ap with their norms, the iteration's check, then the next direction
+// p and its mu = S p.  Deterministic reductions: per-block partials, then
+// one block in block order.  Every kernel returns at once when the solve
+// has stopped before the iteration.
 template <class R>
 void launch_cg_iteration(hipStream_t s, int n, const double* S, const acc_t* acc, const R* erec, R* mu, double selfc,
-                         double* p, double* Ap, double* x, double* r, double* partial, double* scal, double* hss) {
+                         double* p, double* Ap, double* x, double* r, double* partial, double* scal, double* hcheck,
+                         double stop_eps) {
     hipLaunchKernelGGL(k_matvec_finish_dot<R>, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, S, acc, erec, mu, selfc,
-                       p, Ap, partial);
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kRedThreads), 0, s, partial, 1, scal + 1);
+                       p, Ap, partial, scal);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kRedThreads), 0, s, partial, 1, scal + 1, scal + kCgStop);
     hipLaunchKernelGGL(k_cg_step, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, S, scal, p, Ap, x, r, partial);
-    hipLaunchKernelGGL(k_cg_reduce_beta, dim3(1), dim3(kRedThreads), 0, s, partial, scal, hss);
+    hipLaunchKernelGGL(k_cg_reduce_beta, dim3(1), dim3(kRedThreads), 0, s, partial, scal, hcheck, n, stop_eps);
     hipLaunchKernelGGL(k_cg_direction_apply<R>, blocks_for(n, 256), dim3(256), 0, s, n, S, scal, r, p, mu);
 }
-void launch_cg_direction(hipStream_t s, int m, const double* rr_old, const double* rr_new, const double* r,
-                         double* p) {
-    int b = std::min((m + 255) / 256, 4096);
-    if (b < 1) b = 1;
-    hipLaunchKernelGGL(k_cg_direction, dim3(b), dim3(256), 0, s, m, rr_old, rr_new, r, p);
+void launch_cg_begin(hipStream_t s, int n, double* scal, double* hcheck, double stop_eps) {
+    hipLaunchKernelGGL(k_cg_begin, dim3(1), dim3(64), 0, s, scal, hcheck, n, stop_eps);
 }
 void launch_axpy(hipStream_t s, int m, double a, const double* x, double* y) {
     int b = std::min((m + 255) / 256, 4096);
@@ -944,7 +966,7 @@ void launch_recip_atoms(hipStream_t s, int n, int mutual, const double* phi, con
     template void launch_matvec_finish<R>(hipStream_t, int, const double*, const acc_t*, const R*, const R*, double, \
                                           const double*, double*);                                                  \
     template void launch_cg_iteration<R>(hipStream_t, int, const double*, const acc_t*, const R*, R*, double,       \
-                                         double*, double*, double*, double*, double*, double*, double*);            \
+                                         double*, double*, double*, double*, double*, double*, double*, double);    \
     template void launch_recip_atoms<R>(hipStream_t, int, int, const double*, const R*, const double*, const double*,\
                                         const int*, double3, double, acc_t*, acc_t*, acc_t*);                      \
     template void launch_field_sum<R>(hipStream_t, int, const acc_t*, const R*, const R*, double, double*);

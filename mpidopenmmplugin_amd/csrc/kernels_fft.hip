@@ -154,7 +154,9 @@ template <class R> constexpr int max_per_thread() {
 // A batch is T consecutive lines l0 .. l0 + T - 1.
 template <class R>
 __global__ __launch_bounds__(fft_threads<R>()) void k_fft_z(R* __restrict__ Q, R2<R>* __restrict__ S, FftAxis ax, int T,
-                                                       int Tshift, const R2<R>* __restrict__ tw, int nlines, int dir) {
+                                                       int Tshift, const R2<R>* __restrict__ tw, int nlines, int dir,
+                                                       const double* skip) {
+    if (skipped(skip)) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int n = ax.n;
     Cx<R>* A = reinterpret_cast<Cx<R>*>(smem_raw);
@@ -222,7 +224,9 @@ __global__ __launch_bounds__(fft_threads<R>()) void k_fft_z(R* __restrict__ Q, R
 // a contiguous block of T * n1 words.
 template <class R>
 __global__ __launch_bounds__(fft_threads<R>()) void k_fft_y(R2<R>* __restrict__ S, FftAxis ax, int T,
-                                                       const R2<R>* __restrict__ tw, int n0, int nxb, int dir) {
+                                                       const R2<R>* __restrict__ tw, int n0, int nxb, int dir,
+                                                       const double* skip) {
+    if (skipped(skip)) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int n = ax.n;
     Cx<R>* A = reinterpret_cast<Cx<R>*>(smem_raw);
@@ -261,6 +265,7 @@ template <class R>
 __global__ __launch_bounds__(fft_threads<R>()) void k_fft_x_convolve(R2<R>* __restrict__ S, FftAxis ax, int T, int Tshift,
                                                                 const R2<R>* __restrict__ tw, int n1, int n2, int nyb,
                                                                 ConvParams<R> cp) {
+    if (skipped(cp.skip)) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int n = ax.n;
     Cx<R>* A = reinterpret_cast<Cx<R>*>(smem_raw);
@@ -436,6 +441,7 @@ template <bool ROW, bool FWD> __device__ __forceinline__ void step8(C* P, const 
 __global__ __launch_bounds__(plane::kThreads) void k_fft_plane128_convolve(R2<float>* __restrict__ S,
                                                                            const R2<float>* __restrict__ tw,
                                                                            int n2, ConvParams<float> cp) {
+    if (skipped(cp.skip)) return;
     using namespace plane;
     __shared__ C P[N * kPitch];
     __shared__ C W[N];
@@ -519,7 +525,9 @@ __device__ __forceinline__ int at(int t, int z) { return t * kPitch + (z >> 3) *
 }  // namespace zline
 
 __global__ __launch_bounds__(zline::kThreads) void k_fft_z128_r2c(const float* __restrict__ Q, R2<float>* __restrict__ S,
-                                                                  const R2<float>* __restrict__ tw, int nlines) {
+                                                                  const R2<float>* __restrict__ tw, int nlines,
+                                                                  const double* skip) {
+    if (skipped(skip)) return;
     using namespace zline;
     typedef Cx<float> C;
     __shared__ C L[T * kPitch];
@@ -573,7 +581,9 @@ __global__ __launch_bounds__(zline::kThreads) void k_fft_z128_r2c(const float* _
 }
 
 __global__ __launch_bounds__(zline::kThreads) void k_fft_z128_c2r(float* __restrict__ Q, const R2<float>* __restrict__ S,
-                                                                  const R2<float>* __restrict__ tw, int nlines) {
+                                                                  const R2<float>* __restrict__ tw, int nlines,
+                                                                  const double* skip) {
+    if (skipped(skip)) return;
     using namespace zline;
     typedef Cx<float> C;
     __shared__ C L[T * kPitch];
@@ -670,15 +680,19 @@ template <class R> static size_t fft_lds(int T, int n) { return (2 * (size_t)T *
 
 // the z passes with the register-step kernels where they apply (FP32,
 // n2 = 128, whole batches of 64 lines)
-template <class R> static bool zline_pass(hipStream_t, const FftPlan<R>&, R*, R2<R>*, int) { return false; }
+template <class R> static bool zline_pass(hipStream_t, const FftPlan<R>&, R*, R2<R>*, int, const double*) {
+    return false;
+}
 template <>
-bool zline_pass<float>(hipStream_t s, const FftPlan<float>& p, float* Q, R2<float>* S, int dir) {
+bool zline_pass<float>(hipStream_t s, const FftPlan<float>& p, float* Q, R2<float>* S, int dir, const double* skip) {
     const int nlines = p.ax[0].n * p.ax[1].n;
     if (p.ax[2].n != zline::N || nlines % zline::T != 0) return false;
     if (dir < 0)
-        hipLaunchKernelGGL(k_fft_z128_r2c, dim3(nlines / zline::T), dim3(zline::kThreads), 0, s, Q, S, p.tw[2], nlines);
+        hipLaunchKernelGGL(k_fft_z128_r2c, dim3(nlines / zline::T), dim3(zline::kThreads), 0, s, Q, S, p.tw[2], nlines,
+                           skip);
     else
-        hipLaunchKernelGGL(k_fft_z128_c2r, dim3(nlines / zline::T), dim3(zline::kThreads), 0, s, Q, S, p.tw[2], nlines);
+        hipLaunchKernelGGL(k_fft_z128_c2r, dim3(nlines / zline::T), dim3(zline::kThreads), 0, s, Q, S, p.tw[2], nlines,
+                           skip);
     return true;
 }
 
@@ -702,24 +716,24 @@ void launch_recip_convolution(hipStream_t s, const FftPlan<R>& p, R* Q, R2<R>* S
     const int Ty = lines_per_group<R>(n1), nxb = (n0 + Ty - 1) / Ty;
     const int Tx = lines_per_group<R>(n0), nyb = (n1 + Tx - 1) / Tx;
     // 1. z forward
-    if (!zline_pass<R>(s, p, Q, S, -1))
+    if (!zline_pass<R>(s, p, Q, S, -1, cp.skip))
         hipLaunchKernelGGL(k_fft_z<R>, dim3((nlines + Tz - 1) / Tz), dim3(fft_threads<R>()), fft_lds<R>(Tz, n2), s, Q,
-                           S, p.ax[2], Tz, log2i(Tz), p.tw[2], nlines, -1);
+                           S, p.ax[2], Tz, log2i(Tz), p.tw[2], nlines, -1, cp.skip);
     if (!plane_pass<R>(s, p, S, n2h, n2, cp)) {
         // 2. y forward
         hipLaunchKernelGGL(k_fft_y<R>, dim3(n2h * nxb), dim3(fft_threads<R>()), fft_lds<R>(Ty, n1), s, S, p.ax[1], Ty,
-                           p.tw[1], n0, nxb, -1);
+                           p.tw[1], n0, nxb, -1, cp.skip);
         // 3. x forward, convolution, x backward
         hipLaunchKernelGGL(k_fft_x_convolve<R>, dim3(n2h * nyb), dim3(fft_threads<R>()), fft_lds<R>(Tx, n0), s, S,
                            p.ax[0], Tx, log2i(Tx), p.tw[0], n1, n2, nyb, cp);
         // 4. y backward
         hipLaunchKernelGGL(k_fft_y<R>, dim3(n2h * nxb), dim3(fft_threads<R>()), fft_lds<R>(Ty, n1), s, S, p.ax[1], Ty,
-                           p.tw[1], n0, nxb, 1);
+                           p.tw[1], n0, nxb, 1, cp.skip);
     }
     // 5. z backward
-    if (!zline_pass<R>(s, p, Q, S, 1))
+    if (!zline_pass<R>(s, p, Q, S, 1, cp.skip))
         hipLaunchKernelGGL(k_fft_z<R>, dim3((nlines + Tz - 1) / Tz), dim3(fft_threads<R>()), fft_lds<R>(Tz, n2), s, Q,
-                           S, p.ax[2], Tz, log2i(Tz), p.tw[2], nlines, 1);
+                           S, p.ax[2], Tz, log2i(Tz), p.tw[2], nlines, 1, cp.skip);
 }
 
 template void launch_recip_convolution<float>(hipStream_t, const FftPlan<float>&, float*, R2<float>*,

@@ -354,6 +354,7 @@ template <class R>
 __global__ __launch_bounds__(256) void k_nl_induced_field(NbrList L, const int* __restrict__ atoms, int natoms,
                                                           PairAtoms<R> A, const R* __restrict__ mu, PairParams<R> P,
                                                           acc_t* __restrict__ field) {
+    if (skipped(P.skip)) return;
     const unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
     const int t = (int)(g / kNlSplit), sub = (int)(g % kNlSplit);
     const int n = P.n;

@@ -373,6 +373,7 @@ template <class R>
 // pol_only: only pairs of two polarisable sites (the solver's operator)
 __global__ __launch_bounds__(256) void k_cov_induced_field(PairAtoms<R> A, const R* __restrict__ mu, PairParams<R> P,
                                                            int pol_only, acc_t* __restrict__ field) {
+    if (skipped(P.skip)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.n) return;
     const int np = A.npart[i];

@@ -333,6 +333,7 @@ __global__ __launch_bounds__(256) void k_spread_owner(const R4<R>* __restrict__ 
                                                       PmeGeom<R> g, TileGeom tg, const int* __restrict__ atoms,
                                                       const int* __restrict__ tstart, const int* __restrict__ tend,
                                                       R* __restrict__ grid) {
+    if (skipped(g.skip)) return;
     constexpr int ND = MODE == 0 ? 3 : 1;   // spline derivatives needed
     constexpr int NC = MODE == 0 ? 20 : 3;  // fractional components
     constexpr int kOwnerCand = owner_cand<R, MODE>();
@@ -659,6 +660,7 @@ template <class R>
 __global__ __launch_bounds__(256) void k_gather_field(int natoms, const int* __restrict__ atoms,
                                                        const R4<R>* __restrict__ pos, PmeGeom<R> g,
                                                        const R* __restrict__ grid, R* __restrict__ field) {
+    if (skipped(g.skip)) return;
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;
     const int t = gt >> 3, iz = gt & 7;
     const bool tv = t < natoms;

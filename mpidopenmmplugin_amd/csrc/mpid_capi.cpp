@@ -186,8 +186,10 @@ struct mpid_context {
     // copy of the residual norm that each replay leaves behind.
     hipGraphExec_t cg_exec = nullptr;
     double* hscal_dev = nullptr;  // device address of hscal (pinned, mapped)
-    double* hscal = nullptr;  // [0] residual norm of a replay, [1] read_scalar, [2] energy-correction dot,
-                              // [3] (as int) the block-row count
+    double* hscal = nullptr;  // [1] read_scalar, [2] energy-correction dot, [3] (as int) the block-row count,
+                              // [kCheck ..] the solver's two check slots (|S r|^2, stop)
+    static constexpr int kCheck = 8;
+    hipEvent_t ev_check[2] = {nullptr, nullptr};  // recorded after the work that publishes check c (c & 1)
     hipEvent_t ev_sync = nullptr;
     std::string error;
     mpid_stats stats{};
@@ -269,6 +271,8 @@ struct mpid_context {
         if (hnl) (void)hipHostFree(hnl);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_sync) (void)hipEventDestroy(ev_sync);
+        for (hipEvent_t e : ev_check)
+            if (e) (void)hipEventDestroy(e);
         if (ev_join) (void)hipEventDestroy(ev_join);
         if (rstream) (void)hipStreamDestroy(rstream);
         if (stream) (void)hipStreamDestroy(stream);
@@ -531,7 +535,7 @@ struct mpid_context {
         tmp3n.resize(n3);
         fout.resize(n3);
         partial.resize(2 * 512);
-        scal.resize(16);
+        scal.resize(kScalars);
         choose_tiles();
         if (opt.precision == MPID_PRECISION_MIXED) init_prec(pf);
         init_prec(pd);  // FP64 path (also the refinement residual of the mixed path)
@@ -605,8 +609,12 @@ struct mpid_context {
             g.gscale[k] = (R)(opt.grid[k] * dbox.inv[k]);
         }
         g.box = rbox<R>();
+        g.skip = iteration_skip();
         return g;
     }
+    // Kernels recorded into the solver's iteration graph take the stop flag:
+    // an iteration issued after the solve has stopped does no work.
+    const double* iteration_skip() const { return capturing ? scal.p + kCgStop : nullptr; }
     template <class R> PairParams<R> pair_params() const {
         PairParams<R> P;
         P.box = rbox<R>();
@@ -623,6 +631,7 @@ struct mpid_context {
         P.scale14 = (R)opt.scale14;
         P.mutual = opt.polarization == MPID_POLARIZATION_MUTUAL ? 1 : 0;
         P.fix_torque = (opt.flags & MPID_FLAG_CONSERVATIVE_TORQUES) ? 1 : 0;
+        P.skip = iteration_skip();
         return P;
     }
     template <class R> PairAtoms<R> pair_atoms(Prec<R>& P) const {
@@ -776,6 +785,7 @@ struct mpid_context {
         c.sc = (R)(1.0 / M_PI * dbox.inv[0] * dbox.inv[1] * dbox.inv[2]);
         for (int k = 0; k < 3; k++) c.inv[k] = (R)dbox.inv[k];
         c.moduli = P.moduli.p;
+        c.skip = iteration_skip();
         return c;
     }
 
@@ -887,7 +897,7 @@ struct mpid_context {
         check_lists(all_atoms);
         recip_cycle_induced(P, false, all_atoms);
         timed(P_INDUCED, [&] {
-            launch_zero_acc(stream, field_acc.p, field_acc.n);
+            launch_zero_acc(stream, field_acc.p, field_acc.n, iteration_skip());
             if (all_atoms)
                 launch_nl_induced_field<R>(stream, n, nullptr, nlist(), pair_atoms(P), P.mu.p, pair_params<R>(),
                                            field_acc.p);
@@ -956,8 +966,11 @@ struct mpid_context {
     // than a blocking synchronisation.
     void wait_stream() {
         HIP_CHECK(hipEventRecord(ev_sync, stream));
+        wait_event(ev_sync);
+    }
+    void wait_event(hipEvent_t e) {
         for (;;) {
-            const hipError_t q = hipEventQuery(ev_sync);
+            const hipError_t q = hipEventQuery(e);
             if (q == hipSuccess) return;
             if (q != hipErrorNotReady) HIP_CHECK(q);
         }
@@ -972,6 +985,12 @@ struct mpid_context {
     // Conjugate gradients on (I - S T S) sol = rhs.  With zero_start the
     // iteration starts from sol = 0 (r = rhs), else from the current sol.
     // Stops when the RMS-Debye norm of S r falls below stop_eps; returns it.
+    //
+    // The device makes the stopping test (check c after c iterations, see
+    // launch_cg_begin) and the host reads it from pinned memory.  Replayed
+    // from the graph, the next iteration is issued before the host reads the
+    // previous check, so the device never waits for the host; an iteration
+    // issued after the solve has stopped does no work.
     template <class R>
     double cg(Prec<R>& P, double* sol, const double* rhs, bool zero_start, double stop_eps, int& iters, int max_iter) {
         const int n3 = 3 * n;
@@ -980,13 +999,13 @@ struct mpid_context {
             join();
             timed(P_SOLVER, [&] {
                 launch_cg_iteration<R>(stream, n, sqrtpol.p, field_acc.p, P.erec.p, P.mu.p, selfc(), p.p, Ap.p, sol,
-                                       r.p, partial.p, scal.p, hscal_dev);
+                                       r.p, partial.p, scal.p, hscal_dev + kCheck, stop_eps);
             });
         };
         // The iteration graph is recorded first: the host does it while the
         // device still runs the work issued before.  (Timing events cannot sit
         // in a graph: when the iteration's categories are timed, the loop runs
-        // eagerly.)
+        // eagerly and the host reads every check before the next iteration.)
         const bool eager = (prof_mask & kIterationCats) != 0;
         if (!eager && iters < max_iter) capture_cg(body);
         timed(P_SOLVER, [&] {
@@ -1003,29 +1022,40 @@ struct mpid_context {
             HIP_CHECK(hipMemcpyAsync(p.p, r.p, n3 * sizeof(double), hipMemcpyDeviceToDevice, stream));
             launch_residual_norm(stream, n, sqrtpol.p, r.p, partial.p, scal.p + 2);  // scal[2] = rr, scal[3] = ss
             HIP_CHECK(hipMemcpyAsync(scal.p, scal.p + 2, sizeof(double), hipMemcpyDeviceToDevice, stream));
+            launch_cg_begin(stream, n, scal.p, hscal_dev + kCheck, stop_eps);
             // the first direction's operator input (each iteration leaves the
             // next one's behind); issued before the norm is known
             launch_apply_S<R>(stream, n, sqrtpol.p, p.p, P.mu.p, nullptr);
         });
-        double eps = debye_eps(read_scalar(scal.p + 3));
-        check_lists_pending();
-        if (eager) {
-            while (eps >= stop_eps && iters < max_iter) {
+        HIP_CHECK(hipEventRecord(ev_check[0], stream));
+        const int iters0 = iters;
+        int issued = 0, seen = 0;  // iterations issued; the check the host reads next
+        auto issue = [&] {
+            if (iters0 + issued >= max_iter) return;
+            if (eager) {
                 body();
                 check_launch();
-                stats.matvecs++;
-                iters++;
-                eps = debye_eps(read_scalar(scal.p + 3));
+            } else {
+                HIP_CHECK(hipGraphLaunch(cg_exec, stream));
             }
-            return eps;
+            issued++;
+            HIP_CHECK(hipEventRecord(ev_check[issued & 1], stream));
+        };
+        if (!eager) issue();
+        double eps;
+        for (;;) {
+            wait_event(ev_check[seen & 1]);
+            if (seen == 0) check_lists_pending();
+            const double* c = hscal + kCheck + 2 * (seen & 1);
+            eps = debye_eps(c[0]);
+            if (c[1] != 0.0) break;  // converged: an iteration issued after this one does nothing
+            if (eager) issue();
+            if (issued == seen) break;  // the iteration limit
+            seen++;                     // iteration `seen` runs (issued, and the solve had not stopped)
+            if (!eager) issue();        // one iteration ahead of the host
         }
-        while (eps >= stop_eps && iters < max_iter) {
-            HIP_CHECK(hipGraphLaunch(cg_exec, stream));
-            wait_stream();
-            stats.matvecs++;
-            iters++;
-            eps = debye_eps(hscal[0]);
-        }
+        iters = iters0 + seen;
+        stats.matvecs += seen;
         return eps;
     }
 
@@ -1161,6 +1191,10 @@ struct mpid_context {
                 break;
             } catch (const ListsOverflow&) {
                 if (attempt > 0) throw MpidError(MPID_ERR_HIP, "neighbour list overflow after regrowth");
+                // a solver iteration may still run on the old lists: let it
+                // finish before the lists are reallocated
+                HIP_CHECK(hipStreamSynchronize(stream));
+                HIP_CHECK(hipStreamSynchronize(rstream));
             }
         }
         double e = forces_and_energy(P, d_forces_out);
@@ -1295,7 +1329,8 @@ int mpid_create(const mpid_parameters* params, const mpid_options* opts, mpid_co
         HIP_CHECK(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
         HIP_CHECK(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
         HIP_CHECK(hipEventCreateWithFlags(&ctx->ev_sync, hipEventDisableTiming));
-        HIP_CHECK(hipHostMalloc((void**)&ctx->hscal, 8 * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent));
+        HIP_CHECK(hipHostMalloc((void**)&ctx->hscal, 16 * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent));
+        for (hipEvent_t& e : ctx->ev_check) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         HIP_CHECK(hipHostGetDevicePointer((void**)&ctx->hscal_dev, ctx->hscal, 0));
 
         HIP_CHECK(hipHostMalloc((void**)&ctx->hnl, 4 * sizeof(int), 0));

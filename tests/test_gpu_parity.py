@@ -250,3 +250,57 @@ def test_graph_replays_match_eager_iterations(water4096, precision):
         assert k.stats()["iterations"] == it0, rep
         assert abs(e1 - e0) <= tol_e * abs(e0), rep
         assert rel_rms(f1, f0) <= tol_f, rep
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_iteration_limit_boundary(water216, precision):
+    """Each solver iteration is issued before the host has read the previous
+    iteration's convergence check.  With the iteration limit set to exactly
+    the number of iterations the solve takes, it converges with that count
+    and the same result; one fewer is an error."""
+    print(f"[gpu] iteration limit {precision}", flush=True)
+    f, pos, box, _ = water216
+    k = HipCalcMPIDForceKernel(precision)
+    k.initialize(len(pos), f)
+    e0, f0 = k.execute(pos, box)
+    it0 = k.stats()["iterations"]
+    assert it0 >= 2
+    tol_e, tol_f = (1e-12, 1e-10) if precision == "double" else (1e-9, 1e-6)
+    g, _, _ = make_waterbox("water216")
+    g.setMutualInducedMaxIterations(it0)
+    k1 = HipCalcMPIDForceKernel(precision)
+    k1.initialize(len(pos), g)
+    e1, f1 = k1.execute(pos, box)
+    assert k1.stats()["iterations"] == it0
+    assert abs(e1 - e0) <= tol_e * abs(e0)
+    assert rel_rms(f1, f0) <= tol_f
+    h, _, _ = make_waterbox("water216")
+    h.setMutualInducedMaxIterations(it0 - 1)
+    k2 = HipCalcMPIDForceKernel(precision)
+    k2.initialize(len(pos), h)
+    with pytest.raises(MPIDConvergenceError, match="did not converge"):
+        k2.execute(pos, box)
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_list_overflow_retry(water4096, precision):
+    """After its first evaluation a context does not wait for the
+    neighbour-list build; the overflow check is read at the solver's first
+    synchronisation, with a solver iteration already issued.  Compressing the
+    box by 0.85 per edge (1.63 times the density) overflows the lists sized
+    for the first density: the evaluation is repeated with larger lists and
+    must give what a fresh context gives for the compressed box."""
+    print(f"[gpu] list overflow retry {precision}", flush=True)
+    f, pos, box, _ = water4096
+    s = 0.85
+    k = HipCalcMPIDForceKernel(precision)
+    k.initialize(len(pos), f)
+    k.execute(pos, box)
+    e1, f1 = k.execute(pos * s, np.asarray(box) * s)
+    fresh = HipCalcMPIDForceKernel(precision)
+    fresh.initialize(len(pos), f)
+    e2, f2 = fresh.execute(pos * s, np.asarray(box) * s)
+    tol_e, tol_f = (1e-12, 1e-10) if precision == "double" else (1e-9, 1e-6)
+    assert k.stats()["iterations"] == fresh.stats()["iterations"]
+    assert abs(e1 - e2) <= tol_e * abs(e2)
+    assert rel_rms(f1, f2) <= tol_f
