@@ -251,7 +251,9 @@ def run(args):
             ("f32" if mixed else "f64"), "avg_launch_ms": e_avg, "launches_timed": live_n,
             "flops_per_launch": e_flops,
             "note": "vector ALU arithmetic, no matrix instructions; peak is MI355X's published dense FP32/FP64 "
-                    "rate; traffic in bytes per launch"}
+                    "rate; traffic in bytes per launch" +
+                    ("; the mixed path's FP64 check of the induced dipoles runs on another stream during this "
+                     "kernel and shares its CUs, and the live duration includes that" if mixed else "")}
 
     # host-buffer rate (positions and forces crossing PCIe each evaluation)
     t = time.perf_counter()

@@ -181,6 +181,11 @@ struct mpid_context {
     // stream so far; join() makes stream wait for the work issued to rstream.
     hipStream_t rstream = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // The mixed path's FP64 check of the solution (the true residual) runs on
+    // vstream beside the final pass, which needs only the solution itself.
+    hipStream_t vstream = nullptr;
+    hipEvent_t ev_v = nullptr, ev_vdone = nullptr;
+    bool final_issued = false;  // the final pass of the current solution is on its way
     // CG iterations are replayed from a graph captured once per solve (its
     // kernel arguments are those of that solve); hscal is the pinned host
     // copy of the residual norm that each replay leaves behind.
@@ -275,6 +280,9 @@ struct mpid_context {
             if (e) (void)hipEventDestroy(e);
         if (ev_join) (void)hipEventDestroy(ev_join);
         if (rstream) (void)hipStreamDestroy(rstream);
+        if (vstream) (void)hipStreamDestroy(vstream);
+        if (ev_v) (void)hipEventDestroy(ev_v);
+        if (ev_vdone) (void)hipEventDestroy(ev_vdone);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -309,8 +317,34 @@ struct mpid_context {
         HIP_CHECK(hipEventRecord(ev_join, rstream));
         HIP_CHECK(hipStreamWaitEvent(stream, ev_join, 0));
     }
+    // vstream waits for the work issued to stream so far
+    void mark_v() {
+        HIP_CHECK(hipEventRecord(ev_v, stream));
+        HIP_CHECK(hipStreamWaitEvent(vstream, ev_v, 0));
+    }
+    // Runs f, which issues work to stream and rstream, on vstream instead (in
+    // order there, after the last mark_v()).
+    template <class F> void on_vstream(F&& f) {
+        const hipStream_t s0 = stream, r0 = rstream;
+        stream = rstream = vstream;
+        try {
+            f();
+        } catch (...) {
+            stream = s0;
+            rstream = r0;
+            throw;
+        }
+        stream = s0;
+        rstream = r0;
+    }
+    // stream waits for the work issued to vstream
+    void join_v() {
+        HIP_CHECK(hipEventRecord(ev_vdone, vstream));
+        HIP_CHECK(hipStreamWaitEvent(stream, ev_vdone, 0));
+    }
     void collect_profile() {
         if (!prof_mask) return;
+        HIP_CHECK(hipStreamSynchronize(vstream));
         HIP_CHECK(hipStreamSynchronize(rstream));
         HIP_CHECK(hipStreamSynchronize(stream));
         for (auto& e : events) {
@@ -1091,31 +1125,45 @@ struct mpid_context {
         }
         int iters = 0;
         double eps = cg(P, x.p, b.p, false, opt.target_epsilon, iters, opt.max_iterations);
+        // Variational correction of the energy: the energy expression of the
+        // reference is linear in the induced dipoles, U(mu) = E(mu) - eps/2 y.r
+        // (r = b - A y) is stationary at the solution, so it carries no
+        // first-order error from the solver tolerance.  It is read with the
+        // energy at the end of the evaluation (forces_and_energy).
+        auto correction_dot = [&](const double* res) {
+            timed(P_SOLVER, [&] { launch_dot(stream, 3 * n, x.p, res, partial.p, scal.p + 6); });
+            HIP_CHECK(hipMemcpyAsync(hscal + 2, scal.p + 6, sizeof(double), hipMemcpyDeviceToHost, stream));
+        };
         if (opt.precision == MPID_PRECISION_MIXED) {
             // FP64 iterative refinement: the residual of the FP32 solution is
             // formed with the FP64 operator, the correction is solved with the
-            // FP32 operator and added in FP64, until the target is met.
-            eps = true_residual();
+            // FP32 operator and added in FP64, until the target is met.  The
+            // final pass needs only the solution: it is issued first, and the
+            // FP64 residual runs on vstream beside it.  Outputs are written
+            // only once the check has passed (forces_and_energy); a
+            // correction issues the final pass again.
+            mark_v();
+            issue_final_pass(P);
+            on_vstream([&] { eps = true_residual(); });
             int refine = 0;
             while (eps >= opt.target_epsilon && refine < 8 && iters < opt.max_iterations) {
+                join();    // the final pass's reciprocal chain has read mu
+                join_v();  // rt
                 refine++;
                 const int before = iters;
                 cg(P, dx.p, rt.p, true, std::max(0.3 * opt.target_epsilon, 1e-4 * eps), iters, opt.max_iterations);
                 if (iters == before) iters++;
                 timed(P_SOLVER, [&] { launch_axpy(stream, 3 * n, 1.0, dx.p, x.p); });
-                eps = true_residual();
+                mark_v();
+                issue_final_pass(P);
+                on_vstream([&] { eps = true_residual(); });
             }
             stats.refinements = refine;
+            on_vstream([&] { correction_dot(rt.p); });
+            join_v();
+        } else {
+            correction_dot(r.p);
         }
-        // Variational correction of the energy: the energy expression of the
-        // reference is linear in the induced dipoles, U(mu) = E(mu) - eps/2 y.r
-        // (r = b - A y) is stationary at the solution, so it carries no
-        // first-order error from the solver tolerance.
-        timed(P_SOLVER, [&] {
-            launch_dot(stream, 3 * n, x.p, opt.precision == MPID_PRECISION_MIXED ? rt.p : r.p, partial.p, scal.p + 6);
-        });
-        // read with the energy at the end of the evaluation (forces_and_energy)
-        HIP_CHECK(hipMemcpyAsync(hscal + 2, scal.p + 6, sizeof(double), hipMemcpyDeviceToHost, stream));
         correction_pending = true;
         stats.iterations = iters;
         stats.epsilon = eps;
@@ -1133,10 +1181,10 @@ struct mpid_context {
     // force, torque and energy accumulators (integer adds, so the order does
     // not matter).  energy_acc[0] takes the real-space energy, [1] the
     // reciprocal and self terms.
-    template <class R> double forces_and_energy(Prec<R>& P, double* d_forces_out) {
+    template <class R> void issue_final_pass(Prec<R>& P) {
+        join();  // a previous final pass's reciprocal kernels have added into the accumulators
         // induced dipoles: mu = S x (double for the per-atom terms, R for the kernels)
         timed(P_SOLVER, [&] { launch_apply_S<R>(stream, n, sqrtpol.p, x.p, P.mu.p, mud.p); });
-        mu_valid = true;
         timed(P_OTHER, [&] {
             launch_zero_acc(stream, force_acc.p, force_acc.n);
             launch_zero_acc(stream, torque_acc.p, torque_acc.n);
@@ -1163,6 +1211,13 @@ struct mpid_context {
                                               torque_acc.p, energy_acc.p);
         });
         check_launch();
+        final_issued = true;
+    }
+
+    template <class R> double forces_and_energy(Prec<R>& P, double* d_forces_out) {
+        if (!final_issued) issue_final_pass(P);
+        final_issued = false;
+        mu_valid = true;
         join();
         timed(P_OTHER, [&] {
             launch_torque_to_force(stream, n, dbox, spos.p, saxes.p, torque_acc.p, force_acc.p);
@@ -1185,6 +1240,7 @@ struct mpid_context {
     template <class R> double run(Prec<R>& P, const double* d_pos, double* d_forces_out) {
         for (int attempt = 0;; attempt++) {
             try {
+                final_issued = false;
                 prepare(d_pos);
                 fixed_field(P);
                 solve(P);
@@ -1326,6 +1382,9 @@ int mpid_create(const mpid_parameters* params, const mpid_options* opts, mpid_co
             throw MpidError(MPID_ERR_HIP, "no HIP device is available");
         HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         HIP_CHECK(hipStreamCreateWithFlags(&ctx->rstream, hipStreamNonBlocking));
+        HIP_CHECK(hipStreamCreateWithFlags(&ctx->vstream, hipStreamNonBlocking));
+        HIP_CHECK(hipEventCreateWithFlags(&ctx->ev_v, hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&ctx->ev_vdone, hipEventDisableTiming));
         HIP_CHECK(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
         HIP_CHECK(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
         HIP_CHECK(hipEventCreateWithFlags(&ctx->ev_sync, hipEventDisableTiming));

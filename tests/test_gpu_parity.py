@@ -304,3 +304,25 @@ def test_list_overflow_retry(water4096, precision):
     assert k.stats()["iterations"] == fresh.stats()["iterations"]
     assert abs(e1 - e2) <= tol_e * abs(e2)
     assert rel_rms(f1, f2) <= tol_f
+
+
+def test_mixed_refinement_tight_target(water216):
+    """A target beyond FP32 arithmetic (1e-8 Debye) makes the mixed path
+    refine: the FP64 residual of the FP32 solution is above the target, so a
+    correction is solved and the final pass, issued beside the check, is
+    issued again.  The result must then be as close to the FP64 oracle
+    (converged to 1e-10) as the target allows, and the first final pass must
+    leave no trace in the outputs (forces accumulate into a buffer of ones)."""
+    print("[gpu] mixed refinement water216", flush=True)
+    f, pos, box, ref = water216
+    g, _, _ = make_waterbox("water216", target_eps=1e-8)
+    k = HipCalcMPIDForceKernel("mixed")
+    k.initialize(len(pos), g)
+    base = np.ones((len(pos), 3))
+    e, F = k.execute(pos, box, forces=base.copy())
+    st = k.stats()
+    assert st["converged"] and st["refinements"] >= 1, st
+    assert st["epsilon"] < 1e-8
+    assert abs(e - ref["energy"]) <= 1e-7 * abs(ref["energy"])
+    assert rel_rms(F - 1.0, ref["forces"]) <= 1e-6
+    assert rel_rms(k.getInducedDipoles(pos, box), ref["induced"]) <= 1e-6
