@@ -237,5 +237,6 @@ class MPIDForce:
             scale14=self._scale14,
             max_iter=self._max_iter,
             target_eps=self._target_eps,
+            extrapolation_coefficients=np.array(self._extrapolation, dtype=np.float64),
         )
         return out

@@ -1,4 +1,4 @@
-// FP64 CPU oracle for the MPIDForce PME + mutual-polarization path.
+// FP64 CPU oracle for the MPIDForce PME path (Mutual, Direct and Extrapolated polarization).
 //
 // TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and the
 // cpu_baseline leg of bench.py as the checker.  The product library never
@@ -365,6 +365,12 @@ struct Oracle {
     std::vector<double> mfrac;     // [20n] fractional multipoles
     std::vector<double> phi, phidp;
     std::vector<V3> E0, mu;
+    // Extrapolated polarization: the perturbation-order dipoles mu_(m), the
+    // field of mu_(m) (with the self term) and its gradient (xx yy zz xy xz yz,
+    // without it) for m = 0 .. N-2, and the partial sums of the coefficients.
+    std::vector<std::vector<V3>> pt_mu, pt_E;
+    std::vector<std::vector<double>> pt_G;
+    std::vector<double> cpart;
     // real-space pair list
     struct Pair { int i, j; double mscale, pscale; };
     std::vector<Pair> pairs;
@@ -391,6 +397,9 @@ struct Oracle {
             if (t != 5 && t != 4 && (S.atom_x[i] < 0 || S.atom_x[i] >= n)) throw std::runtime_error("MPIDForce: invalid x axis particle");
             if ((t == 2 || t == 3) && (S.atom_y[i] < 0 || S.atom_y[i] >= n)) throw std::runtime_error("MPIDForce: invalid y axis particle");
         }
+        if (S.polarization < 0 || S.polarization > 2) throw std::runtime_error("MPIDForce: unknown polarization type");
+        if (S.polarization == 2 && (S.n_coeff < 1 || !S.coeffs))
+            throw std::runtime_error("MPIDForce: extrapolated polarization needs at least one coefficient");
     }
 
     // ---- setPeriodicBoxSize, MPIDReferenceForce.cpp:2615-2633
@@ -849,6 +858,54 @@ struct Oracle {
         E[P.j] += dr * (m[P.i].dot(dr) * f2) + m[P.i] * f1;
     }
 
+    // Field gradient of the induced dipoles, the extra terms of
+    // calculateDirectInducedDipolePairIxns (PME) for Extrapolated
+    // polarization, :4225-4281: at i from mu_j with dr = r_j - r_i added, at
+    // j from mu_i subtracted (components xx yy zz xy xz yz).
+    void induced_grad_pair(const Pair& P, const std::vector<V3>& m, std::vector<double>& G) const {
+        const Site &I = site[P.i], &J = site[P.j];
+        V3 dr = J.pos - I.pos;
+        min_image(dr);
+        double r2 = dr.dot(dr), r = std::sqrt(r2);
+        double ar = S.alpha * r;
+        double bn0 = std::erfc(ar) / r;
+        double al2 = 2.0 * S.alpha * S.alpha;
+        double aln = 1.0 / (kSqrtPi * S.alpha);
+        double ex = std::exp(-ar * ar);
+        aln *= al2; double bn1 = (bn0 + aln * ex) / r2;
+        aln *= al2; double bn2 = (3.0 * bn1 + aln * ex) / r2;
+        aln *= al2; double bn3 = (5.0 * bn2 + aln * ex) / r2;
+        double s5 = 1.0, s7 = 1.0;
+        double damp = I.damp * J.damp;
+        if (damp != 0.0) {
+            double a = (P.pscale == 0.0) ? I.thole + J.thole : S.default_thole;
+            double u = a * r / damp;
+            if (u < 50.0) {
+                double e = std::exp(-u), u2 = u * u, u3 = u2 * u, u4 = u3 * u;
+                s5 = 1.0 - e * (1.0 + u + 0.5 * u2 + u3 / 6.0);
+                s7 = 1.0 - e * (1.0 + u + 0.5 * u2 + u3 / 6.0 + u4 / 30.0);
+            }
+        }
+        double r5 = r2 * r2 * r, r7 = r5 * r2;
+        double f2 = bn2 - 3.0 * (1.0 - s5) / r5, f3 = bn3 - 15.0 * (1.0 - s7) / r7;
+        // the gradient tensor of the field of dipole u at separation dr
+        auto grad = [&](const V3& u, double g[6]) {
+            const int ab[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
+            double ud = u.dot(dr);
+            for (int k = 0; k < 6; k++) {
+                int a = ab[k][0], b = ab[k][1];
+                g[k] = ud * dr[a] * dr[b] * f3 - (u[a] * dr[b] + u[b] * dr[a] + (a == b ? ud : 0.0)) * f2;
+            }
+        };
+        double gi[6], gj[6];
+        grad(m[P.j], gi);
+        grad(m[P.i], gj);
+        for (int k = 0; k < 6; k++) {
+            G[6 * (size_t)P.i + k] += gi[k];
+            G[6 * (size_t)P.j + k] -= gj[k];
+        }
+    }
+
     // ---- fixed field: calculateFixedMultipoleField (PME), :2922-2949
     void fixed_field() {
         fractional_multipoles();
@@ -877,6 +934,83 @@ struct Oracle {
         double term = (4.0 / 3.0) * (S.alpha * S.alpha * S.alpha) / kSqrtPi;
         for (int i = 0; i < n; i++) E[i] += m[i] * term;
         if (phd) *phd = ph;
+    }
+
+    // Field (real + reciprocal + self) and field gradient (real + reciprocal,
+    // no self term) of the dipoles m: calculateInducedDipoleFields (PME)
+    // with the gradient terms of Extrapolated polarization, :4073-4141.  The
+    // reciprocal gradient is minus the Cartesian second derivatives of the
+    // dipoles' reciprocal potential, A phi_frac A^T (:4094-4127).
+    void induced_field_grad(const std::vector<V3>& m, std::vector<V3>& E, std::vector<double>& G) {
+        std::vector<double> ph;
+        induced_field(m, E, &ph);
+        G.assign(6 * (size_t)n, 0.0);
+        for (const Pair& p : pairs) induced_grad_pair(p, m, G);
+        const int ab[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
+        for (int i = 0; i < n; i++)
+            for (int c = 0; c < 6; c++) {
+                const int a = ab[c][0], b = ab[c][1];
+                double v = 0;
+                for (int k = 0; k < 3; k++)
+                    for (int l = 0; l < 3; l++) {
+                        int e[3] = {0, 0, 0};
+                        e[k]++;
+                        e[l]++;
+                        v += A[a][k] * A[b][l] * ph[35 * (size_t)i + phi_index(e[0], e[1], e[2])];
+                    }
+                G[6 * (size_t)i + c] -= v;
+            }
+    }
+
+    // convergeInduceDipolesByExtrapolation, :1125-1180, with the partial sums
+    // of setExtrapolationCoefficients, :173-183: mu_(0) = alpha E0,
+    // mu_(m+1) = alpha E(mu_(m)), mu = sum_m c'_m mu_(m), c'_m = sum_{j>=m} c_j.
+    void extrapolate() {
+        const int N = S.n_coeff;
+        cpart.assign(N, 0.0);
+        for (int i = 0; i < N; i++)
+            for (int j = i; j < N; j++) cpart[i] += S.coeffs[j];
+        pt_mu.assign(1, mu);
+        pt_E.clear();
+        pt_G.clear();
+        for (int order = 1; order < N; order++) {
+            std::vector<V3> E;
+            std::vector<double> G;
+            induced_field_grad(pt_mu.back(), E, G);
+            std::vector<V3> next(n);
+            for (int i = 0; i < n; i++) next[i] = apply_pol(i, E[i]);
+            pt_E.push_back(E);
+            pt_G.push_back(G);
+            pt_mu.push_back(next);
+        }
+        mu.assign(n, V3());
+        for (int o = 0; o < N; o++)
+            for (int i = 0; i < n; i++) mu[i] += pt_mu[o][i] * cpart[o];
+        converged = true;
+        iterations = 0;
+        last_eps = 0;
+    }
+
+    // The dipole-response forces and torques of Extrapolated polarization,
+    // :4955-4985: sum over l, m (l + m <= N - 2) of c'_(l+m+1) times mu_(l)
+    // contracted with the field gradient of mu_(m) (force) or crossed with
+    // its field (torque, anisotropic sites only).  Terms with
+    // |c'| < 1e-6 are skipped as in the reference.
+    void extrapolation_response(std::vector<V3>& F, std::vector<V3>& T) const {
+        const int N = (int)cpart.size();
+        for (int i = 0; i < n; i++)
+            for (int l = 0; l < N - 1; l++)
+                for (int m = 0; m < N - 1 - l; m++) {
+                    const double p = cpart[l + m + 1];
+                    if (std::fabs(p) < 1e-6) continue;
+                    const V3& u = pt_mu[l][i];
+                    const double* g = &pt_G[m][6 * (size_t)i];
+                    const double pf = p * kElectric;
+                    F[i][0] += pf * (u[0] * g[0] + u[1] * g[3] + u[2] * g[4]);
+                    F[i][1] += pf * (u[0] * g[3] + u[1] * g[1] + u[2] * g[5]);
+                    F[i][2] += pf * (u[0] * g[4] + u[1] * g[5] + u[2] * g[2]);
+                    if (site[i].aniso) T[i] += u.cross(pt_E[m][i]) * pf;
+                }
     }
 
     V3 apply_pol(int i, const V3& e) const {
@@ -951,6 +1085,7 @@ struct Oracle {
         for (int i = 0; i < n; i++) pE[i] = apply_pol(i, E0[i]);
         mu = pE;
         if (S.polarization == 1) { converged = true; iterations = 0; return; }
+        if (S.polarization == 2) { extrapolate(); return; }
         std::vector<std::vector<V3>> prevD, prevE;
         const int maxPrev = 20;
         for (int it = 0;; it++) {
@@ -1539,6 +1674,7 @@ int oracle_evaluate(const oracle_system* sys, const double* pos, oracle_out* out
         double eind = o.recip_induced(F, T);
         double efix = o.recip_fixed(F, T);
         double eself = o.self_energy();
+        if (sys->polarization == 2) o.extrapolation_response(F, T);
         if (out) {
             std::vector<V3> T0 = T;
             for (int i = 0; i < n; i++) o.torque_to_force(i, T[i], F);

@@ -37,7 +37,7 @@ typedef struct {
     const double* polarity;     /* [3n]   */
     const int* cov_offsets;     /* [4n+1] */
     const int* cov_atoms;
-    int polarization;           /* 0 = Mutual, 1 = Direct */
+    int polarization;           /* 0 = Mutual, 1 = Direct, 2 = Extrapolated */
     double cutoff;
     double alpha;
     int nx, ny, nz;
@@ -47,6 +47,8 @@ typedef struct {
     double target_eps;
     double box[9];              /* three box vectors, row-major */
     int flags;                  /* 1: restore the omitted induced-induced z torque */
+    int n_coeff;                /* Extrapolated: number of coefficients (>= 1) */
+    const double* coeffs;       /* Extrapolated: MPIDForce::getExtrapolationCoefficients */
 } oracle_system;
 
 /* Every pointer may be NULL.  Internal orderings: quadrupole QXX QXY QXZ QYY

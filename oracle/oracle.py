@@ -37,6 +37,7 @@ class _System(ctypes.Structure):
         ("max_iter", ctypes.c_int), ("target_eps", ctypes.c_double),
         ("box", ctypes.c_double * 9),
         ("flags", ctypes.c_int),
+        ("n_coeff", ctypes.c_int), ("coeffs", _dp),
     ]
 
 
@@ -103,9 +104,14 @@ def _system(flat, box, polarization=None, flags=0):
     s.cov_offsets = _ptr(arr("cov_offsets", np.int32), "i")
     s.cov_atoms = _ptr(arr("cov_atoms", np.int32), "i")
     pol = flat["polarization"] if polarization is None else polarization
-    if pol not in (0, 1):
-        raise OracleError("oracle supports Mutual (0) and Direct (1) polarization only")
+    if pol not in (0, 1, 2):
+        raise OracleError("unknown polarization type %r" % (pol,))
     s.polarization = int(pol)
+    coeffs = np.ascontiguousarray(flat.get("extrapolation_coefficients", [-0.154, 0.017, 0.658, 0.474]),
+                                  dtype=np.float64)
+    keep["coeffs"] = coeffs
+    s.n_coeff = int(coeffs.size)
+    s.coeffs = _ptr(coeffs, "d")
     s.cutoff = float(flat["cutoff"])
     s.alpha = float(flat["alpha"])
     s.nx, s.ny, s.nz = (int(g) for g in flat["grid"])

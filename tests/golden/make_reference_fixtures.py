@@ -133,6 +133,17 @@ def main():
                               [0.9812786184, -1.522494372, -2.19205581], [2.482142255, -3.688423769, -6.596946686],
                               [12.53264302, -46.6801623, 204.8169521], [13.32737364, -12.47126727, -64.76450554],
                               [-26.51129625, 3.16340949, -46.15822949], [-2.283744668, 62.41956649, -85.09314563]]))
+    # Extrapolated (OPT) polarization with the default coefficients
+    # (-0.154, 0.017, 0.658, 0.474; MPIDForce.cpp:46-49),
+    # TestReferenceMPIDForce.cpp:1192-1245
+    cases.append(dict(name="methanol_pme_extrapolated", polarization=2, target_eps=1e-5, **mpme,
+                      energy=100.0480906, source="TestReferenceMPIDForce.cpp:1223-1235",
+                      forces=[[0.438635498, 0.9521799034, 0.2673223945], [2.777143984, 1.837985034, -1.32278651],
+                              [-171.6327822, -55.43082349, 12.31056078], [54.15935455, 41.79153141, -18.66398402],
+                              [67.20558687, 12.0987565, 23.9882874], [46.52473894, -2.46854161, -16.59424569],
+                              [0.9814929295, -1.523627818, -2.192349756], [2.481269398, -3.686996821, -6.596448702],
+                              [12.53309724, -46.68066444, 204.8169435], [13.32746646, -12.47126075, -64.76446756],
+                              [-26.51126553, 3.163426109, -46.15818436], [-2.283699129, 62.41955233, -85.09312721]]))
     # water PME (TestReferenceMPIDForce.cpp:1350-1400, 1449-1501)
     wpme = dict(system="water", method=1, alpha=3.0, grid=64, cutoff=0.6, box=2.0, default_thole=3.0)
     cases.append(dict(name="water_pme_direct", polarization=1, target_eps=1e-5, **wpme,
@@ -145,18 +156,24 @@ def main():
                       forces=[[-140.0801113, -184.8502938, 30.90206227], [37.10990648, -5.575145037, 7.692659824],
                               [41.55181662, 119.50422, 34.31895915], [-117.0338412, -101.5516429, -33.10846732],
                               [127.7947382, 167.7834207, -16.87461724], [50.65754476, 4.689434353, -22.93075376]]))
+    # TestReferenceMPIDForce.cpp:1503-1550
+    cases.append(dict(name="water_pme_extrapolated", polarization=2, target_eps=1e-5, **wpme,
+                      energy=-2.527846018, source="TestReferenceMPIDForce.cpp:1535-1541",
+                      forces=[[-140.156749, -184.9802098, 30.95576142], [37.14821556, -5.561204369, 7.691990976],
+                              [41.56730743, 119.5695275, 34.32030003], [-117.1007878, -101.6308286, -33.11577527],
+                              [127.9092349, 167.929753, -16.90691988], [50.63283256, 4.672955505, -22.94551458]]))
     # 1-4 scaling with PME (TestReferenceMPIDForce.cpp:1648-1700): closed form, tolerance 1e-3
     for s14, e in ((1.0, -1389.35), (0.5, -1389.35 / 2), (0.0, 0.0)):
         cases.append(dict(name=f"charge_square_pme_s14_{s14}", system="square", method=1, alpha=0.001, grid=64,
                           cutoff=0.4, box=4.0, default_thole=5.0, polarization=1, target_eps=1e-5,
                           scale14=s14, energy=e, energy_tol=1e-3, source="TestReferenceMPIDForce.cpp:1648-1700"))
-    # The NoCutoff and Extrapolated goldens (energies only) are recorded for later
-    # rows of the build; they are not exercised by the PME path.
+    # The NoCutoff goldens (energies only) are recorded for a later row of the
+    # build; they are not exercised by the PME path.
     other = dict(
         methanol_nocut_direct=100.1426571, methanol_nocut_mutual=100.1424251,
-        methanol_pme_extrapolated=100.0480906, methanol_nocut_extrapolated=100.1424271,
+        methanol_nocut_extrapolated=100.1424271,
         water_nocut_direct=-1.949902453, water_nocut_mutual=-1.952917117,
-        water_pme_extrapolated=-2.527846018, water_nocut_extrapolated=-1.94668563)
+        water_nocut_extrapolated=-1.94668563)
     out = dict(systems=dict(water=water, methanol=methanol, square=square), cases=cases,
                other_goldens=other, tolerance=1e-4)
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_tests.json")

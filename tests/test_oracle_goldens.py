@@ -25,10 +25,13 @@ def test_oracle_reproduces_reference_goldens(case, reference_tests):
         assert abs(res["energy"] - case["energy"]) / abs(case["energy"]) < 1e-6
 
 
-def test_oracle_finite_differences(reference_tests):
-    """check_finite_differences of the reference tests (TestReferenceMPIDForce.cpp:913-936):
-    a step along the force changes the energy by |F| * step."""
-    case = next(c for c in CASES if c["name"] == "water_pme_mutual")
+@pytest.mark.parametrize("name", ["water_pme_mutual", "water_pme_extrapolated", "methanol_pme_extrapolated"])
+def test_oracle_finite_differences(reference_tests, name):
+    """check_finite_differences of the reference tests (TestReferenceMPIDForce.cpp:913-936,
+    called by the Mutual and Extrapolated tests, e.g. :1244 and :1551): a step along the force
+    changes the energy by |F| * step.  For Extrapolated polarization this holds only with
+    the dipole-response forces (MPIDReferenceForce.cpp:4955-4985)."""
+    case = next(c for c in CASES if c["name"] == name)
     f, pos, box = force_from_case(case, reference_tests["systems"])
     flat = f.flatten()
     res = oracle.evaluate(flat, pos, box)
