@@ -213,6 +213,28 @@ def run(args):
             "reference": "FP64 GPU path converged to 1e-10 Debye",
         }
 
+    # Extrapolated (OPT) polarization on the same box: both precisions timed
+    # like the headline path, and the mixed path's accuracy against FP64
+    if not args.skip_fp64:
+        fx, _, _ = make_waterbox(args.workload, polarization=2)
+        ext, outs = {}, {}
+        for prec in ("mixed", "double"):
+            kx = HipCalcMPIDForceKernel(prec)
+            kx.initialize(n, fx)
+            msx, _ = timed_rate(kx, max(1, args.steps // 2), 1)
+            outs[prec] = kx.execute(pos, box)
+            ext[prec + "_ms_per_step"] = 1000 * msx
+            del kx
+        (e_x, f_x), (e_xd, f_xd) = outs["mixed"], outs["double"]
+        ext["speedup_mixed_vs_fp64"] = ext["double_ms_per_step"] / ext["mixed_ms_per_step"]
+        ext["accuracy_mixed_vs_fp64"] = {
+            "energy_rel": abs(e_x - e_xd) / abs(e_xd),
+            "force_rel_rms": float(np.sqrt(((f_x - f_xd) ** 2).sum() / (f_xd ** 2).sum())),
+        }
+        print(f"[bench] {args.workload} extrapolated: mixed {ext['mixed_ms_per_step']:.2f} ms/step, "
+              f"double {ext['double_ms_per_step']:.2f} ms/step", file=sys.stderr, flush=True)
+        result["extrapolated"] = ext
+
     # roofline of every kernel family; the dominant one is reported
     flat = force.flatten()
     pc = count_pairs(pos, box, cfg["cutoff"], flat)

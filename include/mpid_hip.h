@@ -49,7 +49,7 @@ enum { MPID_FLAG_CONSERVATIVE_TORQUES = 1 };
 /* Global settings of the force (MPIDForce.h:118-470 setters). */
 typedef struct {
     int nonbonded_method;   /* MPID_NONBONDED_PME */
-    int polarization;       /* MPID_POLARIZATION_MUTUAL or _DIRECT */
+    int polarization;       /* MPID_POLARIZATION_MUTUAL, _DIRECT or _EXTRAPOLATED */
     double cutoff;          /* nm */
     double ewald_alpha;     /* nm^-1, must be > 0 */
     int grid[3];            /* PME grid, sizes of the form 2^a 3^b 5^c */
@@ -59,7 +59,15 @@ typedef struct {
     double target_epsilon;  /* MPIDForce::getMutualInducedTargetEpsilon (RMS Debye) */
     int precision;          /* MPID_PRECISION_* */
     int flags;              /* MPID_FLAG_* bits, 0 for the reference behaviour */
+    /* Extrapolated polarization: MPIDForce::setExtrapolationCoefficients
+     * (MPIDForce.h:346; copied to the reference force at
+     * MPIDReferenceKernels.cpp:149-150, 213-215).  n_extrapolation = 0 takes
+     * MPIDForce's defaults (-0.154, 0.017, 0.658, 0.474; MPIDForce.cpp:46-49);
+     * at most MPID_MAX_EXTRAPOLATION coefficients.  The array is copied. */
+    int n_extrapolation;
+    const double* extrapolation_coefficients;
 } mpid_options;
+enum { MPID_MAX_EXTRAPOLATION = 16 };
 
 /* Per-particle parameters (MPIDForce::addMultipole, setCovalentMap).
  * Covalent lists in CSR form: the atoms in list t (0 = Covalent12, 1 = 13,

@@ -43,6 +43,8 @@ class Options(ctypes.Structure):
         ("target_epsilon", ctypes.c_double),
         ("precision", ctypes.c_int),
         ("flags", ctypes.c_int),
+        ("n_extrapolation", ctypes.c_int),
+        ("extrapolation_coefficients", _dp),
     ]
 
 
@@ -159,4 +161,10 @@ def make_options(flat, precision, flags=0):
     o.target_epsilon = float(flat["target_eps"])
     o.precision = int(precision)
     o.flags = int(flags)
+    coeffs = flat.get("extrapolation_coefficients")
+    if coeffs is not None and len(coeffs) > 0:
+        c = np.ascontiguousarray(coeffs, dtype=np.float64)
+        o._keep = c  # the library copies the array during the call that takes the options
+        o.n_extrapolation = int(c.size)
+        o.extrapolation_coefficients = c.ctypes.data_as(_dp)
     return o

@@ -90,6 +90,27 @@ void launch_matvec_finish(hipStream_t s, int n, const double* sqrtpol, const acc
                           const R* mu, double selfc, const double* p, double* Ap);
 // deterministic FP64 reductions
 void launch_dot(hipStream_t s, int m, const double* a, const double* b, double* partial, double* result);
+// Extrapolated (OPT) polarization: the partial sums c'_o = sum_{j>=o} c_j of
+// the coefficients (at most kMaxExtrapolation), passed by value.
+constexpr int kMaxExtrapolation = 16;
+struct ExtrapCoefficients {
+    int n;
+    double c[kMaxExtrapolation];
+};
+// One order: E_(m), G_(m) (field with the self term, gradient without it, of
+// mu_(m) at every atom, from the real-space accumulators, the reciprocal
+// potential derivatives phidp and the self term) and mu_(m+1) = alpha_lab E.
+template <class R>
+void launch_extrap_order(hipStream_t s, int n, const acc_t* field, const acc_t* gacc, const R* phidp, double3 gscale,
+                         double selfc, const double* labpol, const double* mu, double* E, double* G, double* mu_next,
+                         R* mu_next_r);
+// mu = sum_o c'_o mu_(o) (mus: [order][3n])
+template <class R>
+void launch_extrap_combine(hipStream_t s, int n, const ExtrapCoefficients& cp, const double* mus, double* mud,
+                           R* mu_r);
+// the dipole-response forces and torques into the fixed-point accumulators
+void launch_extrap_response(hipStream_t s, int n, const ExtrapCoefficients& cp, const double* mus, const double* Es,
+                            const double* Gs, const int* aniso, acc_t* force, acc_t* torque);
 // The solver's device-side state in scal: [0] rr, [1] pAp, [2] rr_new,
 // [3] |S r|^2, [7] beta, [kCgCount] iterations done, [kCgStop] 1 once the
 // stopping test has passed.  Kernels of an iteration given the stop flag
@@ -247,14 +268,15 @@ void launch_pol_fill(hipStream_t s, int n, const double4* pos, const int* start,
 template <class R>
 void launch_nl_fixed_field(hipStream_t s, int natoms, const int* atoms, NbrList l, PairAtoms<R> a, PairParams<R> p,
                            acc_t* field);
+// grad != nullptr: also the field gradient (xx yy zz xy xz yz per atom)
 template <class R>
 void launch_nl_induced_field(hipStream_t s, int natoms, const int* atoms, NbrList l, PairAtoms<R> a, const R* mu,
-                             PairParams<R> p, acc_t* field);
+                             PairParams<R> p, acc_t* field, acc_t* grad = nullptr);
 // the scaled covalent pairs, which the list and tile kernels leave out
 template <class R> void launch_cov_fixed_field(hipStream_t s, int n, PairAtoms<R> a, PairParams<R> p, acc_t* field);
 template <class R>
 void launch_cov_induced_field(hipStream_t s, int n, PairAtoms<R> a, const R* mu, PairParams<R> p, int pol_only,
-                              acc_t* field);
+                              acc_t* field, acc_t* grad = nullptr);
 template <class R>
 void launch_cov_electrostatics(hipStream_t s, int n, PairAtoms<R> a, const R* mu, PairParams<R> p, acc_t* force,
                                acc_t* torque, acc_t* energy);

@@ -486,6 +486,97 @@ __global__ void k_cg_direction_apply(int n, const double* __restrict__ S, const 
     mu[3 * (size_t)s + 2] = (R)v.z;
 }
 
+// ---- Extrapolated (OPT) polarization --------------------------------------
+// convergeInduceDipolesByExtrapolation, MPIDReferenceForce.cpp:1125-1180.
+// One order: the field of mu_(m) at every atom, E = real space (acc) +
+// reciprocal (-A phi_1) + selfc mu_(m), and its gradient G = real space
+// (gacc) - A phi_2 A (A = diag(gscale), rectangular boxes; :4094-4127), kept
+// for the response forces; then mu_(m+1) = alpha_lab E, in FP64 and in the
+// precision of the next order's kernels.
+template <class R>
+__global__ void k_extrap_order(int n, const acc_t* __restrict__ field, const acc_t* __restrict__ gacc,
+                               const R* __restrict__ phidp, double3 gs, double selfc,
+                               const double* __restrict__ labpol, const double* __restrict__ mu, double* E,
+                               double* G, double* mu_next, R* mu_next_r) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const R* ph = phidp + 35 * (size_t)s;
+    const size_t i3 = 3 * (size_t)s, i6 = 6 * (size_t)s;
+    D3 e;
+    e.x = acc_get(field[i3]) - gs.x * (double)ph[1] + selfc * mu[i3];
+    e.y = acc_get(field[i3 + 1]) - gs.y * (double)ph[2] + selfc * mu[i3 + 1];
+    e.z = acc_get(field[i3 + 2]) - gs.z * (double)ph[3] + selfc * mu[i3 + 2];
+    E[i3] = e.x;
+    E[i3 + 1] = e.y;
+    E[i3 + 2] = e.z;
+    G[i6 + 0] = acc_get(gacc[i6 + 0]) - gs.x * gs.x * (double)ph[4];
+    G[i6 + 1] = acc_get(gacc[i6 + 1]) - gs.y * gs.y * (double)ph[5];
+    G[i6 + 2] = acc_get(gacc[i6 + 2]) - gs.z * gs.z * (double)ph[6];
+    G[i6 + 3] = acc_get(gacc[i6 + 3]) - gs.x * gs.y * (double)ph[7];
+    G[i6 + 4] = acc_get(gacc[i6 + 4]) - gs.x * gs.z * (double)ph[8];
+    G[i6 + 5] = acc_get(gacc[i6 + 5]) - gs.y * gs.z * (double)ph[9];
+    const D3 m = sym_mul(labpol + i6, e);
+    mu_next[i3] = m.x;
+    mu_next[i3 + 1] = m.y;
+    mu_next[i3 + 2] = m.z;
+    mu_next_r[i3] = (R)m.x;
+    mu_next_r[i3 + 1] = (R)m.y;
+    mu_next_r[i3 + 2] = (R)m.z;
+}
+
+// mu = sum_o c'_o mu_(o), c'_o = sum_{j >= o} c_j (setExtrapolationCoefficients,
+// MPIDReferenceForce.cpp:173-183; the sum at :1170-1177).
+template <class R>
+__global__ void k_extrap_combine(int n, ExtrapCoefficients cp, const double* __restrict__ mus, double* mud, R* mu_r) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t n3 = 3 * (size_t)n;
+    if (id >= (int)n3) return;
+    double v = 0.0;
+    for (int o = 0; o < cp.n; o++) v += cp.c[o] * mus[o * n3 + id];
+    mud[id] = v;
+    mu_r[id] = (R)v;
+}
+
+// The dipole-response forces and torques, MPIDReferenceForce.cpp:4955-4985:
+// for l + m <= N - 2, c'_(l+m+1) times mu_(l) contracted with the gradient
+// of the field of mu_(m) (force) or crossed with that field (torque on
+// anisotropic sites).  Terms with |c'| < 1e-6 are skipped as there.
+__global__ void k_extrap_response(int n, ExtrapCoefficients cp, const double* __restrict__ mus,
+                                  const double* __restrict__ Es, const double* __restrict__ Gs,
+                                  const int* __restrict__ aniso, acc_t* force, acc_t* torque) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const size_t n3 = 3 * (size_t)n, n6 = 6 * (size_t)n;
+    const bool an = aniso[s] != 0;
+    D3 f = mk(0.0, 0.0, 0.0), t = mk(0.0, 0.0, 0.0);
+    for (int l = 0; l < cp.n - 1; l++) {
+        const double* u = mus + l * n3 + 3 * (size_t)s;
+        for (int m = 0; m < cp.n - 1 - l; m++) {
+            const double p = cp.c[l + m + 1];
+            if (fabs(p) < 1e-6) continue;
+            const double pf = p * kElectric;
+            const double* g = Gs + m * n6 + 6 * (size_t)s;
+            f.x += pf * (u[0] * g[0] + u[1] * g[3] + u[2] * g[4]);
+            f.y += pf * (u[0] * g[3] + u[1] * g[1] + u[2] * g[5]);
+            f.z += pf * (u[0] * g[4] + u[1] * g[5] + u[2] * g[2]);
+            if (an) {
+                const double* e = Es + m * n3 + 3 * (size_t)s;
+                t.x += pf * (u[1] * e[2] - u[2] * e[1]);
+                t.y += pf * (u[2] * e[0] - u[0] * e[2]);
+                t.z += pf * (u[0] * e[1] - u[1] * e[0]);
+            }
+        }
+    }
+    acc_add(&force[3 * (size_t)s + 0], f.x);
+    acc_add(&force[3 * (size_t)s + 1], f.y);
+    acc_add(&force[3 * (size_t)s + 2], f.z);
+    if (an) {
+        acc_add(&torque[3 * (size_t)s + 0], t.x);
+        acc_add(&torque[3 * (size_t)s + 1], t.y);
+        acc_add(&torque[3 * (size_t)s + 2], t.z);
+    }
+}
+
 __global__ void k_axpy(int m, double a, const double* __restrict__ x, double* y) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) y[i] += a * x[i];
 }
@@ -935,6 +1026,23 @@ void launch_cg_iteration(hipStream_t s, int n, const double* S, const acc_t* acc
     hipLaunchKernelGGL(k_cg_reduce_beta, dim3(1), dim3(kRedThreads), 0, s, partial, scal, hcheck, n, stop_eps);
     hipLaunchKernelGGL(k_cg_direction_apply<R>, blocks_for(n, 256), dim3(256), 0, s, n, S, scal, r, p, mu);
 }
+template <class R>
+void launch_extrap_order(hipStream_t s, int n, const acc_t* field, const acc_t* gacc, const R* phidp, double3 gs,
+                         double selfc, const double* labpol, const double* mu, double* E, double* G, double* mu_next,
+                         R* mu_next_r) {
+    hipLaunchKernelGGL(k_extrap_order<R>, blocks_for(n, 256), dim3(256), 0, s, n, field, gacc, phidp, gs, selfc,
+                       labpol, mu, E, G, mu_next, mu_next_r);
+}
+template <class R>
+void launch_extrap_combine(hipStream_t s, int n, const ExtrapCoefficients& cp, const double* mus, double* mud,
+                           R* mu_r) {
+    hipLaunchKernelGGL(k_extrap_combine<R>, blocks_for(3 * n, 256), dim3(256), 0, s, n, cp, mus, mud, mu_r);
+}
+void launch_extrap_response(hipStream_t s, int n, const ExtrapCoefficients& cp, const double* mus, const double* Es,
+                            const double* Gs, const int* aniso, acc_t* force, acc_t* torque) {
+    hipLaunchKernelGGL(k_extrap_response, blocks_for(n, 256), dim3(256), 0, s, n, cp, mus, Es, Gs, aniso, force,
+                       torque);
+}
 void launch_cg_begin(hipStream_t s, int n, double* scal, double* hcheck, double stop_eps) {
     hipLaunchKernelGGL(k_cg_begin, dim3(1), dim3(64), 0, s, scal, hcheck, n, stop_eps);
 }
@@ -965,6 +1073,9 @@ void launch_recip_atoms(hipStream_t s, int n, int mutual, const double* phi, con
     template void launch_apply_S<R>(hipStream_t, int, const double*, const double*, R*, double*);                   \
     template void launch_matvec_finish<R>(hipStream_t, int, const double*, const acc_t*, const R*, const R*, double, \
                                           const double*, double*);                                                  \
+    template void launch_extrap_order<R>(hipStream_t, int, const acc_t*, const acc_t*, const R*, double3, double,   \
+                                         const double*, const double*, double*, double*, double*, R*);              \
+    template void launch_extrap_combine<R>(hipStream_t, int, const ExtrapCoefficients&, const double*, double*, R*); \
     template void launch_cg_iteration<R>(hipStream_t, int, const double*, const acc_t*, const R*, R*, double,       \
                                          double*, double*, double*, double*, double*, double*, double*, double);    \
     template void launch_recip_atoms<R>(hipStream_t, int, int, const double*, const R*, const double*, const double*,\

@@ -350,10 +350,12 @@ __global__ __launch_bounds__(256) void k_nl_fixed_field(NbrList L, const int* __
     }
 }
 
-template <class R>
+// GRAD: also the field gradient (xx yy zz xy xz yz) into grad[6 i ..]
+// (Extrapolated polarization).
+template <class R, bool GRAD>
 __global__ __launch_bounds__(256) void k_nl_induced_field(NbrList L, const int* __restrict__ atoms, int natoms,
                                                           PairAtoms<R> A, const R* __restrict__ mu, PairParams<R> P,
-                                                          acc_t* __restrict__ field) {
+                                                          acc_t* __restrict__ field, acc_t* __restrict__ grad) {
     if (skipped(P.skip)) return;
     const unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
     const int t = (int)(g / kNlSplit), sub = (int)(g % kNlSplit);
@@ -369,6 +371,7 @@ __global__ __launch_bounds__(256) void k_nl_induced_field(NbrList L, const int* 
     const size_t stride = (size_t)(atoms ? natoms : n) * kNlSplit;
     int jn = sub < cnt ? lp[0] : 0;
     R ex = 0, ey = 0, ez = 0;
+    R gs[6] = {0, 0, 0, 0, 0, 0};
     for (int k = sub; k < cnt; k += kNlSplit) {
         const int j = jn;
         lp += stride;
@@ -377,8 +380,14 @@ __global__ __launch_bounds__(256) void k_nl_induced_field(NbrList L, const int* 
         const V3<R> dr = min_image(mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z), P.box);
         const V3<R> uj = mk(mu[3 * (size_t)j], mu[3 * (size_t)j + 1], mu[3 * (size_t)j + 2]);
         // listed pairs are not covalent partners: polarization scale 1, default Thole width
-        const V3<R> fi =
-            induced_field_from<R>(uj, dr, induced_field_radial<R>(dot(dr, dr), pi.w, pj.w, P.default_thole, P));
+        V3<R> fi;
+        if constexpr (GRAD) {
+            const InducedRadialGrad<R> f = induced_grad_radial<R>(dot(dr, dr), pi.w, pj.w, P.default_thole, P);
+            fi = induced_field_from<R>(uj, dr, InducedRadial<R>{f.f1, f.f2});
+            induced_grad_add<R>(uj, dr, f.f2, f.f3, gs);
+        } else {
+            fi = induced_field_from<R>(uj, dr, induced_field_radial<R>(dot(dr, dr), pi.w, pj.w, P.default_thole, P));
+        }
         ex += fi.x;
         ey += fi.y;
         ez += fi.z;
@@ -388,11 +397,19 @@ __global__ __launch_bounds__(256) void k_nl_induced_field(NbrList L, const int* 
         ex += __shfl_xor(ex, off);
         ey += __shfl_xor(ey, off);
         ez += __shfl_xor(ez, off);
+        if constexpr (GRAD) {
+#pragma unroll
+            for (int c = 0; c < 6; c++) gs[c] += __shfl_xor(gs[c], off);
+        }
     }
     if (iv && sub == 0) {
         acc_add(&field[3 * (size_t)i + 0], ex);
         acc_add(&field[3 * (size_t)i + 1], ey);
         acc_add(&field[3 * (size_t)i + 2], ez);
+        if constexpr (GRAD) {
+#pragma unroll
+            for (int c = 0; c < 6; c++) acc_add(&grad[6 * (size_t)i + c], gs[c]);
+        }
     }
 }
 
@@ -435,18 +452,23 @@ void launch_nl_fixed_field(hipStream_t s, int natoms, const int* atoms, NbrList 
 }
 template <class R>
 void launch_nl_induced_field(hipStream_t s, int natoms, const int* atoms, NbrList l, PairAtoms<R> a, const R* mu,
-                             PairParams<R> p, acc_t* field) {
+                             PairParams<R> p, acc_t* field, acc_t* grad) {
     if (natoms <= 0) return;
     const size_t threads = (size_t)natoms * kNlSplit;
-    hipLaunchKernelGGL(k_nl_induced_field<R>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, l, atoms,
-                       natoms, a, mu, p, field);
+    const dim3 grid((unsigned)((threads + 255) / 256));
+    if (grad)
+        hipLaunchKernelGGL((k_nl_induced_field<R, true>), grid, dim3(256), 0, s, l, atoms, natoms, a, mu, p, field,
+                           grad);
+    else
+        hipLaunchKernelGGL((k_nl_induced_field<R, false>), grid, dim3(256), 0, s, l, atoms, natoms, a, mu, p, field,
+                           nullptr);
 }
 
 #define MPID_INSTANTIATE(R)                                                                                         \
     template void launch_nl_fixed_field<R>(hipStream_t, int, const int*, NbrList, PairAtoms<R>, PairParams<R>,       \
                                            acc_t*);                                                                \
     template void launch_nl_induced_field<R>(hipStream_t, int, const int*, NbrList, PairAtoms<R>, const R*,          \
-                                             PairParams<R>, acc_t*);
+                                             PairParams<R>, acc_t*, acc_t*);
 MPID_INSTANTIATE(float)
 MPID_INSTANTIATE(double)
 #undef MPID_INSTANTIATE

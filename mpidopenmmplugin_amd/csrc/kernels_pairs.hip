@@ -369,10 +369,13 @@ __global__ __launch_bounds__(256) void k_cov_fixed_field(PairAtoms<R> A, PairPar
     acc_add(&field[3 * (size_t)i + 2], ez);
 }
 
-template <class R>
-// pol_only: only pairs of two polarisable sites (the solver's operator)
+// pol_only: only pairs of two polarisable sites (the solver's operator).
+// GRAD: also the field gradient (xx yy zz xy xz yz) into grad[6 i ..]
+// (Extrapolated polarization).
+template <class R, bool GRAD>
 __global__ __launch_bounds__(256) void k_cov_induced_field(PairAtoms<R> A, const R* __restrict__ mu, PairParams<R> P,
-                                                           int pol_only, acc_t* __restrict__ field) {
+                                                           int pol_only, acc_t* __restrict__ field,
+                                                           acc_t* __restrict__ grad) {
     if (skipped(P.skip)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.n) return;
@@ -383,6 +386,7 @@ __global__ __launch_bounds__(256) void k_cov_induced_field(PairAtoms<R> A, const
     const R2<R> tdi = A.td[i];
     if (pol_only && tdi.y == R(0)) return;
     R ex = 0, ey = 0, ez = 0;
+    R gi[6] = {0, 0, 0, 0, 0, 0};
     for (int k = 0; k < np; k++) {
         const int e = A.partners[(size_t)i * kMaxPartners + k];
         const int j = e >> 2;
@@ -395,8 +399,21 @@ __global__ __launch_bounds__(256) void k_cov_induced_field(PairAtoms<R> A, const
         V3<R> fi, fj;
         const R2<R> tdj = A.td[j];
         const R a = partner_scale<R>(e, P.scale14) == R(0) ? tdi.x + tdj.x : P.default_thole;
-        induced_field_pair<R>(dr, r2, ui, mk(mu[3 * (size_t)j], mu[3 * (size_t)j + 1], mu[3 * (size_t)j + 2]), tdi.y,
-                              tdj.y, a, P, fi, fj);
+        const V3<R> uj = mk(mu[3 * (size_t)j], mu[3 * (size_t)j + 1], mu[3 * (size_t)j + 2]);
+        if constexpr (GRAD) {
+            const InducedRadialGrad<R> f = induced_grad_radial<R>(r2, tdi.y, tdj.y, a, P);
+            const InducedRadial<R> f12{f.f1, f.f2};
+            fi = induced_field_from<R>(uj, dr, f12);
+            fj = induced_field_from<R>(ui, dr, f12);
+            // at i from mu_j (separation dr), at j from mu_i (separation -dr)
+            induced_grad_add<R>(uj, dr, f.f2, f.f3, gi);
+            R gj[6] = {0, 0, 0, 0, 0, 0};
+            induced_grad_add<R>(ui, dr * R(-1), f.f2, f.f3, gj);
+#pragma unroll
+            for (int c = 0; c < 6; c++) acc_add(&grad[6 * (size_t)j + c], gj[c]);
+        } else {
+            induced_field_pair<R>(dr, r2, ui, uj, tdi.y, tdj.y, a, P, fi, fj);
+        }
         ex += fi.x;
         ey += fi.y;
         ez += fi.z;
@@ -407,6 +424,10 @@ __global__ __launch_bounds__(256) void k_cov_induced_field(PairAtoms<R> A, const
     acc_add(&field[3 * (size_t)i + 0], ex);
     acc_add(&field[3 * (size_t)i + 1], ey);
     acc_add(&field[3 * (size_t)i + 2], ez);
+    if constexpr (GRAD) {
+#pragma unroll
+        for (int c = 0; c < 6; c++) acc_add(&grad[6 * (size_t)i + c], gi[c]);
+    }
 }
 
 template <class R>
@@ -494,8 +515,13 @@ template <class R> void launch_cov_fixed_field(hipStream_t s, int n, PairAtoms<R
 }
 template <class R>
 void launch_cov_induced_field(hipStream_t s, int n, PairAtoms<R> a, const R* mu, PairParams<R> p, int pol_only,
-                              acc_t* field) {
-    hipLaunchKernelGGL(k_cov_induced_field<R>, dim3((n + 255) / 256), dim3(256), 0, s, a, mu, p, pol_only, field);
+                              acc_t* field, acc_t* grad) {
+    if (grad)
+        hipLaunchKernelGGL((k_cov_induced_field<R, true>), dim3((n + 255) / 256), dim3(256), 0, s, a, mu, p, pol_only,
+                           field, grad);
+    else
+        hipLaunchKernelGGL((k_cov_induced_field<R, false>), dim3((n + 255) / 256), dim3(256), 0, s, a, mu, p,
+                           pol_only, field, nullptr);
 }
 template <class R>
 void launch_cov_electrostatics(hipStream_t s, int n, PairAtoms<R> a, const R* mu, PairParams<R> p, acc_t* force,
@@ -513,7 +539,7 @@ void launch_cov_electrostatics(hipStream_t s, int n, PairAtoms<R> a, const R* mu
                                                  PairParams<double>, acc_t*, acc_t*, acc_t*);                       \
     template void launch_cov_fixed_field<R>(hipStream_t, int, PairAtoms<R>, PairParams<R>, acc_t*);                \
     template void launch_cov_induced_field<R>(hipStream_t, int, PairAtoms<R>, const R*, PairParams<R>, int,        \
-                                              acc_t*);                                                            \
+                                              acc_t*, acc_t*);                                                    \
     template void launch_cov_electrostatics<R>(hipStream_t, int, PairAtoms<R>, const R*, PairParams<R>, acc_t*,    \
                                                acc_t*, acc_t*);
 MPID_INSTANTIATE(float)

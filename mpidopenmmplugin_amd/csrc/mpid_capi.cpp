@@ -242,6 +242,13 @@ struct mpid_context {
     int* hnl = nullptr;  // pinned: [0] largest list length, [1] mirror flag
     // accumulators
     DBuf<acc_t> field_acc, force_acc, torque_acc, energy_acc;
+    // Extrapolated polarization: coefficient partial sums, the order dipoles
+    // mu_(m) [N][3n], the fields E_(m) and gradients G_(m) of mu_(0..N-2)
+    // ([N-1][3n], [N-1][6n]) and the real-space gradient accumulator
+    ExtrapCoefficients ext_cp{};
+    DBuf<double> ptmu, ptE, ptG;
+    DBuf<acc_t> grad_acc;
+    bool mu_set = false;  // the induced dipoles are in place (Extrapolated); the final pass does not form S x
     // solver
     DBuf<double> E0, b, x, r, p, Ap, dx, rt, mud, partial, scal, tmp3n, fout;
     // precisions
@@ -463,8 +470,27 @@ struct mpid_context {
         opt = *o;
         if (opt.nonbonded_method != MPID_NONBONDED_PME)
             throw MpidError(MPID_ERR_UNSUPPORTED, "only the PME nonbonded method is implemented on this platform");
-        if (opt.polarization != MPID_POLARIZATION_MUTUAL && opt.polarization != MPID_POLARIZATION_DIRECT)
-            throw MpidError(MPID_ERR_UNSUPPORTED, "only Mutual and Direct polarization are implemented on this platform");
+        if (opt.polarization != MPID_POLARIZATION_MUTUAL && opt.polarization != MPID_POLARIZATION_DIRECT &&
+            opt.polarization != MPID_POLARIZATION_EXTRAPOLATED)
+            throw MpidError(MPID_ERR_INVALID, "unknown polarization type");
+        {
+            // coefficients copied here (the caller's array need not outlive the call)
+            static const double kDefault[4] = {-0.154, 0.017, 0.658, 0.474};  // MPIDForce.cpp:46-49
+            const int nc = opt.n_extrapolation > 0 ? opt.n_extrapolation : 4;
+            const double* c = opt.n_extrapolation > 0 ? opt.extrapolation_coefficients : kDefault;
+            if (opt.n_extrapolation < 0 || (opt.n_extrapolation > 0 && !c))
+                throw MpidError(MPID_ERR_INVALID, "invalid extrapolation coefficients");
+            if (nc > kMaxExtrapolation)
+                throw MpidError(MPID_ERR_UNSUPPORTED, "more than " + std::to_string(kMaxExtrapolation) +
+                                                          " extrapolation coefficients");
+            ExtrapCoefficients cp{};
+            cp.n = nc;
+            for (int i = 0; i < nc; i++)
+                for (int j = i; j < nc; j++) cp.c[i] += c[j];
+            ext_cp = cp;
+            opt.n_extrapolation = nc;
+            opt.extrapolation_coefficients = nullptr;
+        }
         if (!(opt.ewald_alpha > 0.0))
             throw MpidError(MPID_ERR_INVALID, "the Ewald alpha must be given explicitly (setPMEParameters)");
         if (!(opt.cutoff > 0.0)) throw MpidError(MPID_ERR_INVALID, "invalid cutoff");
@@ -1093,6 +1119,48 @@ struct mpid_context {
         return eps;
     }
 
+    // Extrapolated (OPT) polarization, convergeInduceDipolesByExtrapolation
+    // (MPIDReferenceForce.cpp:1125-1180): mu_(0) = alpha E0 = S b, then for
+    // m = 0 .. N-2 the field and field gradient of mu_(m) (the operator's
+    // kernels with the gradient terms, the reciprocal potential derivatives
+    // at every atom, the self term) and mu_(m+1) = alpha E_(m); finally
+    // mu = sum_m c'_m mu_(m).  E_(m) and G_(m) stay for the response forces
+    // of the final pass.
+    template <class R> void extrapolate(Prec<R>& P) {
+        check_lists(false);
+        const int N = ext_cp.n;
+        const size_t n3 = 3 * (size_t)n, n6 = 6 * (size_t)n;
+        ptmu.resize((size_t)N * n3);
+        ptE.resize((size_t)std::max(1, N - 1) * n3);
+        ptG.resize((size_t)std::max(1, N - 1) * n6);
+        grad_acc.resize(n6);
+        timed(P_SOLVER, [&] { launch_apply_S<R>(stream, n, sqrtpol.p, b.p, P.mu.p, ptmu.p); });
+        for (int m = 0; m + 1 < N; m++) {
+            recip_cycle_induced(P, true);  // phidp of mu_(m) at every atom, on rstream
+            timed(P_INDUCED, [&] {
+                launch_zero_acc(stream, field_acc.p, field_acc.n);
+                launch_zero_acc(stream, grad_acc.p, grad_acc.n);
+                launch_nl_induced_field<R>(stream, npol, pol_idx.p, nlist_pol(), pair_atoms(P), P.mu.p,
+                                           pair_params<R>(), field_acc.p, grad_acc.p);
+            });
+            timed(P_COV, [&] {
+                launch_cov_induced_field<R>(stream, n, pair_atoms(P), P.mu.p, pair_params<R>(), 1, field_acc.p,
+                                            grad_acc.p);
+            });
+            join();
+            timed(P_SOLVER, [&] {
+                launch_extrap_order<R>(stream, n, field_acc.p, grad_acc.p, P.phidp.p, gscale3(), selfc(), labpol.p,
+                                       ptmu.p + m * n3, ptE.p + m * n3, ptG.p + m * n6, ptmu.p + (m + 1) * n3,
+                                       P.mu.p);
+            });
+            check_launch();
+            stats.matvecs++;
+        }
+        timed(P_SOLVER, [&] { launch_extrap_combine<R>(stream, n, ext_cp, ptmu.p, mud.p, P.mu.p); });
+        check_launch();
+        mu_set = true;
+    }
+
     // residual with the FP64 operator: rt = b - A x; returns its RMS-Debye norm
     double true_residual() {
         const int n3 = 3 * n;
@@ -1114,7 +1182,8 @@ struct mpid_context {
         check_launch();
         energy_correction = 0.0;
         correction_pending = false;
-        if (opt.polarization == MPID_POLARIZATION_DIRECT) {
+        if (opt.polarization == MPID_POLARIZATION_DIRECT || opt.polarization == MPID_POLARIZATION_EXTRAPOLATED) {
+            if (opt.polarization == MPID_POLARIZATION_EXTRAPOLATED) extrapolate(P);
             if (lists_pending) {
                 wait_stream();
                 check_lists_pending();
@@ -1183,13 +1252,18 @@ struct mpid_context {
     // reciprocal and self terms.
     template <class R> void issue_final_pass(Prec<R>& P) {
         join();  // a previous final pass's reciprocal kernels have added into the accumulators
-        // induced dipoles: mu = S x (double for the per-atom terms, R for the kernels)
-        timed(P_SOLVER, [&] { launch_apply_S<R>(stream, n, sqrtpol.p, x.p, P.mu.p, mud.p); });
+        // induced dipoles: mu = S x (double for the per-atom terms, R for the
+        // kernels), unless they are in place already (Extrapolated)
+        if (!mu_set) timed(P_SOLVER, [&] { launch_apply_S<R>(stream, n, sqrtpol.p, x.p, P.mu.p, mud.p); });
         timed(P_OTHER, [&] {
             launch_zero_acc(stream, force_acc.p, force_acc.n);
             launch_zero_acc(stream, torque_acc.p, torque_acc.n);
             launch_zero_acc(stream, energy_acc.p, 2);
         });
+        if (opt.polarization == MPID_POLARIZATION_EXTRAPOLATED)
+            timed(P_SOLVER, [&] {
+                launch_extrap_response(stream, n, ext_cp, ptmu.p, ptE.p, ptG.p, aniso.p, force_acc.p, torque_acc.p);
+            });
         // Order of issue: the reciprocal chain first (short kernels with large
         // LDS blocks, which would wait for whole CUs behind the
         // electrostatics), then the electrostatics, then the per-atom
@@ -1241,6 +1315,7 @@ struct mpid_context {
         for (int attempt = 0;; attempt++) {
             try {
                 final_issued = false;
+                mu_set = false;
                 prepare(d_pos);
                 fixed_field(P);
                 solve(P);

@@ -160,6 +160,56 @@ MPID_HD InducedRadial<R> induced_field_radial(R r2, R dampi, R dampj, R a, const
 template <class R> MPID_HD V3<R> induced_field_from(V3<R> u, V3<R> dr, const InducedRadial<R>& f) {
     return dr * (dot(u, dr) * f.f2) + u * f.f1;
 }
+
+// The field factors with the third one of the field gradient (Extrapolated
+// polarization): f3 = bn3 - 15 (1 - lambda7) / r^7 with the Thole factor
+// lambda7 of MPIDReferenceForce.cpp:4214 (the damped pair terms of
+// calculateDirectInducedDipolePairIxns, :4161-4228).
+template <class R> struct InducedRadialGrad {
+    R f1, f2, f3;
+};
+template <class R>
+MPID_HD InducedRadialGrad<R> induced_grad_radial(R r2, R dampi, R dampj, R a, const PairParams<R>& P) {
+    const R rinv = rsqrt_r(r2);
+    const R r = r2 * rinv;
+    const R rinv2 = rinv * rinv;
+    const R ar = P.alpha * r;
+    const R ex2 = exp_r(-ar * ar);
+    const R bn0 = erfc_r(ar, ex2) * rinv;
+    const R bn1 = (bn0 + P.aln[1] * ex2) * rinv2;
+    const R bn2 = (R(3) * bn1 + P.aln[2] * ex2) * rinv2;
+    const R bn3 = (R(5) * bn2 + P.aln[3] * ex2) * rinv2;
+    R m3 = R(0), m5 = R(0), m7 = R(0);
+    const R damp = dampi * dampj;
+    if (damp != R(0)) {
+        const R u = div_r(a * r, damp);
+        if (u < R(50)) {
+            const R e = exp_r(-u);
+            const R u2 = u * u;
+            m3 = e * (R(1) + u + R(0.5) * u2);
+            m5 = m3 + e * u2 * u * R(1.0 / 6.0);
+            m7 = m5 + e * u2 * u2 * R(1.0 / 30.0);
+        }
+    }
+    const R rinv3 = rinv * rinv2, rinv5 = rinv3 * rinv2;
+    InducedRadialGrad<R> f;
+    f.f1 = m3 * rinv3 - bn1;
+    f.f2 = bn2 - R(3) * m5 * rinv5;
+    f.f3 = bn3 - R(15) * m7 * rinv5 * rinv2;
+    return f;
+}
+// Adds the gradient of the field of dipole u at separation dr = (source -
+// field point) to g (xx yy zz xy xz yz), MPIDReferenceForce.cpp:4236-4248.
+template <class R> MPID_HD void induced_grad_add(V3<R> u, V3<R> dr, R f2, R f3, R* g) {
+    const R ud = dot(u, dr);
+    const R t = ud * f3;
+    g[0] += t * dr.x * dr.x - (R(2) * u.x * dr.x + ud) * f2;
+    g[1] += t * dr.y * dr.y - (R(2) * u.y * dr.y + ud) * f2;
+    g[2] += t * dr.z * dr.z - (R(2) * u.z * dr.z + ud) * f2;
+    g[3] += t * dr.x * dr.y - (u.x * dr.y + u.y * dr.x) * f2;
+    g[4] += t * dr.x * dr.z - (u.x * dr.z + u.z * dr.x) * f2;
+    g[5] += t * dr.y * dr.z - (u.y * dr.z + u.z * dr.y) * f2;
+}
 template <class R>
 MPID_HD void induced_field_pair(V3<R> dr, R r2, V3<R> ui, V3<R> uj, R dampi, R dampj, R a, const PairParams<R>& P,
                                 V3<R>& fi, V3<R>& fj) {
