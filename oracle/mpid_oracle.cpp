@@ -378,11 +378,18 @@ struct Oracle {
     double last_eps = 0;
     bool converged = false;
 
-    explicit Oracle(const oracle_system& s) : S(s), n(s.n) {}
+    // NoCutoff (MPIDReferenceForce, the non-PME class): every pair, no
+    // periodic images, no reciprocal or self terms.  Its pair formulas are
+    // the PME real-space ones at alpha = 0 (erfc = 1, no Gaussian terms);
+    // the reference's NoCutoff goldens pin that identity.
+    bool periodic = true;
+
+    explicit Oracle(const oracle_system& s) : S(s), n(s.n), periodic(s.method == 1) {}
 
     // ---- validation: MPIDForceImpl.cpp:51-150 and MPIDReferenceKernels.cpp:193-197
     void validate() {
-        for (int k = 0; k < 3; k++)
+        if (S.method != 0 && S.method != 1) throw std::runtime_error("MPIDForce: unknown nonbonded method");
+        for (int k = 0; k < 3 && periodic; k++)
             if (S.cutoff > 0.5 * S.box[4 * k]) throw std::runtime_error("MPIDForce: The cutoff distance cannot be greater than half the periodic box size.");
         for (int i = 0; i < n; i++) {
             const double* q = S.quadrupoles + 6 * i;
@@ -405,6 +412,14 @@ struct Oracle {
     // ---- setPeriodicBoxSize, MPIDReferenceForce.cpp:2615-2633
     void set_box() {
         for (int k = 0; k < 3; k++) box[k] = V3(S.box[3 * k], S.box[3 * k + 1], S.box[3 * k + 2]);
+        if (!periodic) {  // no images, no grid: the reciprocal transforms vanish
+            for (int k = 0; k < 3; k++) {
+                recip[k] = V3();
+                gn[k] = 0;
+                for (int j = 0; j < 3; j++) A[k][j] = 0.0;
+            }
+            return;
+        }
         double det = box[0][0] * box[1][1] * box[2][2];
         double sc = 1.0 / det;
         recip[0] = V3(box[1][1] * box[2][2], 0, 0) * sc;
@@ -416,6 +431,7 @@ struct Oracle {
     }
     // getPeriodicDelta, MPIDReferenceForce.cpp:2671-2676
     void min_image(V3& d) const {
+        if (!periodic) return;
         d -= box[2] * std::floor(d[2] * recip[2][2] + 0.5);
         d -= box[1] * std::floor(d[1] * recip[1][1] + 0.5);
         d -= box[0] * std::floor(d[0] * recip[0][0] + 0.5);
@@ -535,6 +551,16 @@ struct Oracle {
             scale_of(i, j, p.mscale, p.pscale);
             pairs.push_back(p);
         };
+        if (!periodic) {
+            for (int i = 0; i < n; i++)
+                for (int j = i + 1; j < n; j++) {
+                    Pair p;
+                    p.i = i; p.j = j;
+                    scale_of(i, j, p.mscale, p.pscale);
+                    pairs.push_back(p);
+                }
+            return;
+        }
         if (n <= 4096 || !rect) {
             for (int i = 0; i < n; i++)
                 for (int j = i + 1; j < n; j++) consider(i, j);
@@ -775,7 +801,7 @@ struct Oracle {
         double ar = S.alpha * r;
         double bn0 = std::erfc(ar) / r;
         double al2 = 2.0 * S.alpha * S.alpha;
-        double aln = 1.0 / (kSqrtPi * S.alpha);
+        double aln = S.alpha > 0.0 ? 1.0 / (kSqrtPi * S.alpha) : 0.0;
         double ex = std::exp(-ar * ar);
         aln *= al2; double bn1 = (bn0 + aln * ex) / r2;
         aln *= al2; double bn2 = (3.0 * bn1 + aln * ex) / r2;
@@ -836,7 +862,7 @@ struct Oracle {
         double ar = S.alpha * r;
         double bn0 = std::erfc(ar) / r;
         double al2 = 2.0 * S.alpha * S.alpha;
-        double aln = 1.0 / (kSqrtPi * S.alpha);
+        double aln = S.alpha > 0.0 ? 1.0 / (kSqrtPi * S.alpha) : 0.0;
         double ex = std::exp(-ar * ar);
         aln *= al2; double bn1 = (bn0 + aln * ex) / r2;
         aln *= al2; double bn2 = (3.0 * bn1 + aln * ex) / r2;
@@ -870,7 +896,7 @@ struct Oracle {
         double ar = S.alpha * r;
         double bn0 = std::erfc(ar) / r;
         double al2 = 2.0 * S.alpha * S.alpha;
-        double aln = 1.0 / (kSqrtPi * S.alpha);
+        double aln = S.alpha > 0.0 ? 1.0 / (kSqrtPi * S.alpha) : 0.0;
         double ex = std::exp(-ar * ar);
         aln *= al2; double bn1 = (bn0 + aln * ex) / r2;
         aln *= al2; double bn2 = (3.0 * bn1 + aln * ex) / r2;
@@ -908,8 +934,13 @@ struct Oracle {
 
     // ---- fixed field: calculateFixedMultipoleField (PME), :2922-2949
     void fixed_field() {
-        fractional_multipoles();
-        recip_cycle(mfrac, 20, phi);
+        if (periodic) {
+            fractional_multipoles();
+            recip_cycle(mfrac, 20, phi);
+        } else {
+            mfrac.assign(20 * (size_t)n, 0.0);
+            phi.assign(35 * (size_t)n, 0.0);
+        }
         E0.assign(n, V3());
         for (int i = 0; i < n; i++) E0[i] = field_from(phi, i);
         double term = (4.0 / 3.0) * (S.alpha * S.alpha * S.alpha) / kSqrtPi;
@@ -929,8 +960,12 @@ struct Oracle {
                 m20[20 * (size_t)i + 1 + j] = v;
             }
         std::vector<double> ph;
-        recip_cycle(m20, 4, ph);
-        for (int i = 0; i < n; i++) E[i] += field_from(ph, i);
+        if (periodic) {
+            recip_cycle(m20, 4, ph);
+            for (int i = 0; i < n; i++) E[i] += field_from(ph, i);
+        } else {
+            ph.assign(35 * (size_t)n, 0.0);
+        }
         double term = (4.0 / 3.0) * (S.alpha * S.alpha * S.alpha) / kSqrtPi;
         for (int i = 0; i < n; i++) E[i] += m[i] * term;
         if (phd) *phd = ph;
@@ -1622,13 +1657,14 @@ int oracle_fft3d(double* data, int nx, int ny, int nz, int sign) {
 static void setup(Oracle& o, const double* pos) {
     o.validate();
     o.set_box();
-    for (int k = 0; k < 3; k++) {
-        o.moduli[k].resize(o.gn[k]);
-        bspline_moduli(o.gn[k], o.moduli[k].data());
-    }
+    if (o.periodic)
+        for (int k = 0; k < 3; k++) {
+            o.moduli[k].resize(o.gn[k]);
+            bspline_moduli(o.gn[k], o.moduli[k].data());
+        }
     o.load_and_rotate(pos);
     o.build_pairs();
-    o.compute_splines();
+    if (o.periodic) o.compute_splines();
 }
 
 int oracle_induced_field(const oracle_system* sys, const double* pos, const double* mu, double* field_out) {

@@ -49,6 +49,7 @@ typedef struct {
     int flags;                  /* 1: restore the omitted induced-induced z torque */
     int n_coeff;                /* Extrapolated: number of coefficients (>= 1) */
     const double* coeffs;       /* Extrapolated: MPIDForce::getExtrapolationCoefficients */
+    int method;                 /* 0 = NoCutoff (all pairs, no periodicity, no Ewald), 1 = PME */
 } oracle_system;
 
 /* Every pointer may be NULL.  Internal orderings: quadrupole QXX QXY QXZ QYY

@@ -38,6 +38,7 @@ class _System(ctypes.Structure):
         ("box", ctypes.c_double * 9),
         ("flags", ctypes.c_int),
         ("n_coeff", ctypes.c_int), ("coeffs", _dp),
+        ("method", ctypes.c_int),
     ]
 
 
@@ -125,6 +126,7 @@ def _system(flat, box, polarization=None, flags=0):
     for k in range(9):
         s.box[k] = float(b[k])
     s.flags = int(flags)
+    s.method = int(flat.get("nonbonded_method", 1))
     return s, keep
 
 

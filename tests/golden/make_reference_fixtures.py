@@ -162,20 +162,56 @@ def main():
                       forces=[[-140.156749, -184.9802098, 30.95576142], [37.14821556, -5.561204369, 7.691990976],
                               [41.56730743, 119.5695275, 34.32030003], [-117.1007878, -101.6308286, -33.11577527],
                               [127.9092349, 167.929753, -16.90691988], [50.63283256, 4.672955505, -22.94551458]]))
+    # NoCutoff (TestReferenceMPIDForce.cpp:1029-1080, 1139-1190, 1249-1299,
+    # 1301-1347, 1401-1447, 1553-1601): every pair, no periodic images; the box
+    # is given to the context but unused; default Thole width 3.0
+    mnc = dict(system="methanol", method=0, alpha=0.0, grid=0, cutoff=1.0, box=2.461817, default_thole=3.0)
+    wnc = dict(system="water", method=0, alpha=0.0, grid=0, cutoff=1.0, box=2.0, default_thole=3.0)
+    cases.append(dict(name="methanol_nocut_direct", polarization=1, target_eps=1e-5, **mnc,
+                      energy=100.1426571, source="TestReferenceMPIDForce.cpp:1055-1068",
+                      forces=[[0.6174848862, 1.244067416, 0.348981665], [3.137044182, 3.021402996, -1.106973251],
+                              [-171.9428202, -56.29908471, 12.15602242], [54.08049294, 41.61038079, -18.72605327],
+                              [67.10967846, 11.90263439, 23.93203305], [46.43698088, -2.657546533, -16.65203133],
+                              [1.1456374, -1.273419433, -2.195068492], [3.279289766, -2.747123834, -6.42588385],
+                              [11.97737104, -47.37054551, 204.7348033], [13.21305881, -12.64877074, -64.79230192],
+                              [-26.64547941, 2.986613971, -46.17381233], [-2.408738783, 62.23139119, -85.099716]]))
+    cases.append(dict(name="methanol_nocut_mutual", polarization=0, target_eps=1e-9, **mnc,
+                      energy=100.1424251, source="TestReferenceMPIDForce.cpp:1166-1179",
+                      forces=[[0.6146835636, 1.24282615, 0.3501711642], [3.13797926, 3.015595591, -1.097188673],
+                              [-171.9420969, -56.29281618, 12.14742654], [54.08018498, 41.61211644, -18.72680649],
+                              [67.11010165, 11.90386444, 23.93032756], [46.43703861, -2.656708806, -16.65281219],
+                              [1.148286492, -1.284971833, -2.197342953], [3.272556789, -2.742411944, -6.424646877],
+                              [11.97974305, -47.36853963, 204.7356438], [13.21447088, -12.64805524, -64.79191248],
+                              [-26.64493967, 2.987395526, -46.17328869], [-2.40800871, 62.23170548, -85.09957069]]))
+    cases.append(dict(name="methanol_nocut_extrapolated", polarization=2, target_eps=1e-5, **mnc,
+                      energy=100.1424271, source="TestReferenceMPIDForce.cpp:1275-1288",
+                      forces=[[0.6144183532, 1.242877198, 0.3502970714], [3.13789228, 3.015096318, -1.095457146],
+                              [-171.941967, -56.29241757, 12.14584015], [54.08021516, 41.61225418, -18.72696219],
+                              [67.11022091, 11.90392367, 23.93009922], [46.43708221, -2.656656187, -16.65293107],
+                              [1.148617423, -1.286157531, -2.197592114], [3.271428661, -2.741098117, -6.424340583],
+                              [11.98035816, -47.36903306, 204.7356612], [13.21458611, -12.64799137, -64.79184765],
+                              [-26.64490265, 2.987465959, -46.17322487], [-2.40794957, 62.2317365, -85.09954205]]))
+    cases.append(dict(name="water_nocut_direct", polarization=1, target_eps=1e-5, **wnc,
+                      energy=-1.949902453, source="TestReferenceMPIDForce.cpp:1329-1336",
+                      forces=[[-138.7310812, -182.9709838, 35.70961618], [37.1153441, -5.548490702, 5.04277195],
+                              [41.13860764, 118.8270727, 31.47279046], [-116.4297925, -100.864177, -27.61517965],
+                              [126.6370205, 165.8966158, -19.33373258], [50.26990146, 4.659962993, -25.27626636]]))
+    cases.append(dict(name="water_nocut_mutual", polarization=0, target_eps=1e-8, **wnc,
+                      energy=-1.952917117, source="TestReferenceMPIDForce.cpp:1430-1437",
+                      forces=[[-139.7835608, -184.4337529, 35.62953533], [37.434981, -5.522902943, 5.11681405],
+                              [41.23101208, 119.3674074, 31.61700973], [-116.9476192, -101.4714619, -27.86430037],
+                              [127.7709383, 167.4188741, -19.23010497], [50.29424862, 4.641836191, -25.26895377]]))
+    cases.append(dict(name="water_nocut_extrapolated", polarization=2, target_eps=1e-5, **wnc,
+                      energy=-1.94668563, source="TestReferenceMPIDForce.cpp:1583-1590",
+                      forces=[[-139.8529084, -184.5568497, 35.69566243], [37.47391144, -5.507167568, 5.113662284],
+                              [41.24807762, 119.4349691, 31.61668322], [-117.0172298, -101.5532625, -27.86681578],
+                              [127.8817331, 167.5584812, -19.27180201], [50.26641605, 4.623829403, -25.28739015]]))
     # 1-4 scaling with PME (TestReferenceMPIDForce.cpp:1648-1700): closed form, tolerance 1e-3
     for s14, e in ((1.0, -1389.35), (0.5, -1389.35 / 2), (0.0, 0.0)):
         cases.append(dict(name=f"charge_square_pme_s14_{s14}", system="square", method=1, alpha=0.001, grid=64,
                           cutoff=0.4, box=4.0, default_thole=5.0, polarization=1, target_eps=1e-5,
                           scale14=s14, energy=e, energy_tol=1e-3, source="TestReferenceMPIDForce.cpp:1648-1700"))
-    # The NoCutoff goldens (energies only) are recorded for a later row of the
-    # build; they are not exercised by the PME path.
-    other = dict(
-        methanol_nocut_direct=100.1426571, methanol_nocut_mutual=100.1424251,
-        methanol_nocut_extrapolated=100.1424271,
-        water_nocut_direct=-1.949902453, water_nocut_mutual=-1.952917117,
-        water_nocut_extrapolated=-1.94668563)
-    out = dict(systems=dict(water=water, methanol=methanol, square=square), cases=cases,
-               other_goldens=other, tolerance=1e-4)
+    out = dict(systems=dict(water=water, methanol=methanol, square=square), cases=cases, tolerance=1e-4)
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_tests.json")
     with open(path, "w") as f:
         json.dump(out, f, indent=1)
