@@ -30,9 +30,11 @@ enum {
     MPID_ERR_INVALID = 1,        /* invalid parameter (OpenMMException in the reference) */
     MPID_ERR_NOT_CONVERGED = 2,  /* mutual induced dipoles did not converge */
     MPID_ERR_HIP = 3,            /* HIP runtime failure */
-    MPID_ERR_UNSUPPORTED = 4     /* feature outside this build (NoCutoff, triclinic box, ...) */
+    MPID_ERR_UNSUPPORTED = 4     /* feature outside this build (triclinic box, NoCutoff beyond 16,384 particles, ...) */
 };
 
+/* NOCUTOFF: every pair, no periodic images, no reciprocal space; the cutoff,
+ * Ewald and grid settings and the box are ignored; at most 16,384 particles. */
 enum { MPID_NONBONDED_NOCUTOFF = 0, MPID_NONBONDED_PME = 1 };
 enum { MPID_POLARIZATION_MUTUAL = 0, MPID_POLARIZATION_DIRECT = 1, MPID_POLARIZATION_EXTRAPOLATED = 2 };
 /* MIXED: FP32 pair and reciprocal arithmetic, 64-bit fixed-point accumulation,
@@ -48,7 +50,7 @@ enum { MPID_FLAG_CONSERVATIVE_TORQUES = 1 };
 
 /* Global settings of the force (MPIDForce.h:118-470 setters). */
 typedef struct {
-    int nonbonded_method;   /* MPID_NONBONDED_PME */
+    int nonbonded_method;   /* MPID_NONBONDED_PME or _NOCUTOFF */
     int polarization;       /* MPID_POLARIZATION_MUTUAL, _DIRECT or _EXTRAPOLATED */
     double cutoff;          /* nm */
     double ewald_alpha;     /* nm^-1, must be > 0 */

@@ -372,6 +372,9 @@ struct mpid_context {
         if (!first && P->n != n)
             throw MpidError(MPID_ERR_INVALID, "updateParametersInContext: The number of multipoles has changed");
         n = P->n;
+        if (!pme() && n > kNoCutoffMaxAtoms)
+            throw MpidError(MPID_ERR_UNSUPPORTED, "NoCutoff is limited to " + std::to_string(kNoCutoffMaxAtoms) +
+                                                      " particles on this platform (every pair is listed)");
         // validation, MPIDForceImpl.cpp:51-150
         for (int i = 0; i < n; i++) {
             const double* q = P->quadrupoles + 6 * i;
@@ -465,11 +468,26 @@ struct mpid_context {
         mu_valid = false;
     }
 
+    // NoCutoff (the reference's non-PME MPIDReferenceForce): every pair, no
+    // periodic images, no reciprocal or self terms.  It runs the PME path's
+    // real-space kernels with alpha = 0 (their formulas then reduce to the
+    // bare damped and scaled interactions), a cutoff that every pair passes,
+    // a box of zero edges (no wrapping, no images) and no reciprocal cycles.
+    // The lists then hold every atom, so systems are limited in size.
+    static constexpr double kNoCutoffDistance = 1.0e6;  // nm
+    static constexpr int kNoCutoffMaxAtoms = 16384;
+    bool pme() const { return opt.nonbonded_method == MPID_NONBONDED_PME; }
+
     void set_options(const mpid_options* o) {
         if (!o) throw MpidError(MPID_ERR_INVALID, "missing options");
         opt = *o;
-        if (opt.nonbonded_method != MPID_NONBONDED_PME)
-            throw MpidError(MPID_ERR_UNSUPPORTED, "only the PME nonbonded method is implemented on this platform");
+        if (opt.nonbonded_method != MPID_NONBONDED_PME && opt.nonbonded_method != MPID_NONBONDED_NOCUTOFF)
+            throw MpidError(MPID_ERR_INVALID, "unknown nonbonded method");
+        if (!pme()) {
+            opt.ewald_alpha = 0.0;
+            opt.cutoff = kNoCutoffDistance;
+            opt.grid[0] = opt.grid[1] = opt.grid[2] = 0;
+        }
         if (opt.polarization != MPID_POLARIZATION_MUTUAL && opt.polarization != MPID_POLARIZATION_DIRECT &&
             opt.polarization != MPID_POLARIZATION_EXTRAPOLATED)
             throw MpidError(MPID_ERR_INVALID, "unknown polarization type");
@@ -491,14 +509,14 @@ struct mpid_context {
             opt.n_extrapolation = nc;
             opt.extrapolation_coefficients = nullptr;
         }
-        if (!(opt.ewald_alpha > 0.0))
+        if (pme() && !(opt.ewald_alpha > 0.0))
             throw MpidError(MPID_ERR_INVALID, "the Ewald alpha must be given explicitly (setPMEParameters)");
         if (!(opt.cutoff > 0.0)) throw MpidError(MPID_ERR_INVALID, "invalid cutoff");
         if (opt.precision != MPID_PRECISION_MIXED && opt.precision != MPID_PRECISION_DOUBLE)
             throw MpidError(MPID_ERR_INVALID, "invalid precision");
         if (opt.max_iterations < 0) throw MpidError(MPID_ERR_INVALID, "invalid iteration limit");
         if (opt.flags & ~MPID_FLAG_CONSERVATIVE_TORQUES) throw MpidError(MPID_ERR_INVALID, "unknown option flags");
-        for (int k = 0; k < 3; k++) {
+        for (int k = 0; k < 3 && pme(); k++) {
             int rr[16], c;
             FftAxis ax;
             if (opt.grid[k] < kOrder || opt.grid[k] > 1024 || !factor_radices(opt.grid[k], rr, c) ||
@@ -521,6 +539,14 @@ struct mpid_context {
         HIP_CHECK(hipMemsetAsync(P.erec.p, 0, P.erec.n * sizeof(R), stream));
         P.phi.resize(35 * (size_t)n);
         P.phidp.resize(35 * (size_t)n);
+        if (!pme()) {
+            // no reciprocal space: the potentials and fields it would give stay zero
+            HIP_CHECK(hipMemsetAsync(P.phi.p, 0, P.phi.n * sizeof(R), stream));
+            HIP_CHECK(hipMemsetAsync(P.phidp.p, 0, P.phidp.n * sizeof(R), stream));
+            HIP_CHECK(hipStreamSynchronize(stream));
+            P.ready = true;
+            return;
+        }
         P.grid.resize(G);
         P.spec.resize((size_t)opt.grid[0] * opt.grid[1] * (opt.grid[2] / 2 + 1));
         std::vector<R> mods;
@@ -604,6 +630,8 @@ struct mpid_context {
     // Largest Morton level m with n_k = 2^m T_k, T_k >= 6, and a window that fits in LDS.
     void choose_tiles() {
         tiled_spread = false;
+        owner_spread = false;
+        if (!pme()) return;
         for (int m = 10; m >= 1; m--) {
             const int a = 1 << m;
             bool ok = true;
@@ -639,6 +667,12 @@ struct mpid_context {
     double box_vec[9];
 
     void set_box(const double* boxv) {
+        if (!pme()) {
+            // no periodicity: zero edges make every wrap and image shift vanish
+            for (int k = 0; k < 9; k++) box_vec[k] = boxv ? boxv[k] : 0.0;
+            for (int k = 0; k < 3; k++) dbox.L[k] = dbox.inv[k] = 0.0;
+            return;
+        }
         if (!boxv) throw MpidError(MPID_ERR_INVALID, "PME requires periodic box vectors");
         for (int k = 0; k < 9; k++) box_vec[k] = boxv[k];
         if (boxv[1] != 0 || boxv[2] != 0 || boxv[3] != 0 || boxv[5] != 0 || boxv[6] != 0 || boxv[7] != 0)
@@ -681,7 +715,8 @@ struct mpid_context {
         P.n = n;
         P.rc2 = (R)(opt.cutoff * opt.cutoff);
         P.alpha = (R)opt.ewald_alpha;
-        double a = opt.ewald_alpha, aln = 1.0 / (kSqrtPi * a);
+        // (2 alpha^2)^k / (alpha sqrt(pi)); all zero at alpha = 0 (NoCutoff)
+        double a = opt.ewald_alpha, aln = a > 0.0 ? 1.0 / (kSqrtPi * a) : 0.0;
         P.aln[0] = (R)0;
         for (int k = 1; k <= 4; k++) {
             aln *= 2.0 * a * a;
@@ -774,9 +809,13 @@ struct mpid_context {
     // list that overflows is rebuilt with a larger one.
     void build_neighbour_lists(int nb) {
         if (nl_cap == 0) {
-            const double vol = dbox.L[0] * dbox.L[1] * dbox.L[2];
-            const double mean = n / vol * (4.0 / 3.0) * M_PI * opt.cutoff * opt.cutoff * opt.cutoff;
-            nl_cap = std::min<int>((int)(1.3 * mean) + 64, n);
+            if (!pme()) {
+                nl_cap = n;  // every atom is a neighbour
+            } else {
+                const double vol = dbox.L[0] * dbox.L[1] * dbox.L[2];
+                const double mean = n / vol * (4.0 / 3.0) * M_PI * opt.cutoff * opt.cutoff * opt.cutoff;
+                nl_cap = std::min<int>((int)(1.3 * mean) + 64, n);
+            }
         }
         timed(P_TILES, [&] {
             launch_block_rows(stream, nb, bmin.p, bmax.p, rbox<float>(), (float)opt.cutoff, brow_start.p, nullptr,
@@ -852,6 +891,7 @@ struct mpid_context {
     // Reciprocal cycles: issued to rstream after a fork(); the caller joins
     // before it reads their output (phi, phidp, erec).
     template <class R> void recip_cycle_fixed(Prec<R>& P) {
+        if (!pme()) return;  // phi stays zero
         PmeGeom<R> g = geom<R>();
         fork();
         const size_t G = P.grid.n;
@@ -881,6 +921,7 @@ struct mpid_context {
     // gathered.  full: the 35 potential derivatives at every atom (forces).
     // forked: the caller has already made rstream wait for its work
     template <class R> void recip_cycle_induced(Prec<R>& P, bool full, bool all_atoms = false, bool forked = false) {
+        if (!pme()) return;  // erec and phidp stay zero
         PmeGeom<R> g = geom<R>();
         if (!forked) fork();
         const size_t G = P.grid.n;

@@ -30,6 +30,8 @@ _PRECISIONS = {"mixed": _capi.MPID_PRECISION_MIXED, "double": _capi.MPID_PRECISI
 
 
 def _box9(box):
+    if box is None:  # NoCutoff takes no box; PME rejects a zero box
+        return np.zeros(9)
     b = np.asarray(box, dtype=np.float64)
     if b.size == 3:
         b = np.diag(b.reshape(3))

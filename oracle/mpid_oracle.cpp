@@ -352,7 +352,7 @@ void comp_exp(int k, int e[3]) {
 }
 
 struct Oracle {
-    const oracle_system& S;
+    oracle_system S;  // a copy: NoCutoff sets its Ewald alpha to zero
     int n;
     std::vector<Site> site;
     V3 box[3], recip[3];
@@ -384,7 +384,9 @@ struct Oracle {
     // the reference's NoCutoff goldens pin that identity.
     bool periodic = true;
 
-    explicit Oracle(const oracle_system& s) : S(s), n(s.n), periodic(s.method == 1) {}
+    explicit Oracle(const oracle_system& s) : S(s), n(s.n), periodic(s.method == 1) {
+        if (!periodic) S.alpha = 0.0;  // the non-PME reference force has no Ewald screening
+    }
 
     // ---- validation: MPIDForceImpl.cpp:51-150 and MPIDReferenceKernels.cpp:193-197
     void validate() {

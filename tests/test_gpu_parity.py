@@ -344,3 +344,25 @@ def test_extrapolated_water216x(precision):
     assert abs(st.getPotentialEnergy() - ref["energy"]) <= e_tol * abs(ref["energy"])
     assert rel_rms(st.getForces(), ref["forces"]) <= f_tol
     assert rel_rms(ctx.getInducedDipoles(), ref["induced"]) <= f_tol
+
+
+@pytest.mark.parametrize("polarization", [MPIDForce.Mutual, MPIDForce.Extrapolated], ids=["mutual", "extrapolated"])
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_nocutoff_water216x(precision, polarization):
+    """NoCutoff on 648 atoms with octopoles and anisotropic polarisability:
+    every pair interacts, with no periodic images, no reciprocal and no self
+    terms.  The FP64 oracle is the reference here (its NoCutoff mode is pinned
+    by the reference's six NoCutoff goldens).  The box passed in is ignored."""
+    print(f"[gpu] nocutoff water216x {precision} pol={polarization}", flush=True)
+    f, pos, box = make_waterbox("water216x", polarization=polarization, target_eps=1e-10)
+    f.setNonbondedMethod(MPIDForce.NoCutoff)
+    ref = oracle.evaluate(f.flatten(), pos, box, details=True)
+    g, _, _ = make_waterbox("water216x", polarization=polarization)
+    g.setNonbondedMethod(MPIDForce.NoCutoff)
+    k = HipCalcMPIDForceKernel(precision)
+    k.initialize(len(pos), g if precision == "mixed" else f)
+    e, F = k.execute(pos, None)
+    e_tol, f_tol = (1e-9, 1e-8) if precision == "double" else (1e-6, 1e-5)
+    assert abs(e - ref["energy"]) <= e_tol * abs(ref["energy"])
+    assert rel_rms(F, ref["forces"]) <= f_tol
+    assert rel_rms(k.getInducedDipoles(pos, None), ref["induced"]) <= f_tol

@@ -101,3 +101,15 @@ def test_reference_anisotropic_mutual_torque_is_not_conservative():
     assert err_ref > 1e-2            # reference behaviour reproduced (non-conservative)
     assert err_fix < 2e-4            # exact gradient once the term is restored
     assert err_fix < 1e-6 * scale * 1e3
+
+
+def test_oracle_nocutoff_ignores_ewald_parameters(reference_tests):
+    """The non-PME reference force has no Ewald screening: a NoCutoff system
+    that still carries PME parameters (alpha, grid) gives the same result."""
+    case = next(c for c in CASES if c["name"] == "water_nocut_mutual")
+    f, pos, box = force_from_case(case, reference_tests["systems"])
+    a = oracle.evaluate(f.flatten(), pos, box)
+    f.setPMEParameters(3.0, 64, 64, 64)
+    b = oracle.evaluate(f.flatten(), pos, box)
+    assert abs(a["energy"] - b["energy"]) <= 1e-12 * abs(a["energy"])
+    assert np.abs(a["forces"] - b["forces"]).max() <= 1e-12 * np.abs(a["forces"]).max()
