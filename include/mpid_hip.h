@@ -125,8 +125,9 @@ int mpid_create(const mpid_parameters* params, const mpid_options* opts, mpid_co
  * particle count must not change. */
 int mpid_update_parameters(mpid_context* ctx, const mpid_parameters* params);
 
-/* CalcMPIDForceKernel::execute (mpidKernels.h:77): host positions; forces are
- * ADDED into forces_inout ([n][3], may be NULL); energy (kJ/mol) written to
+/* CalcMPIDForceKernel::execute (mpidKernels.h:77): host positions; box: the
+ * three box vectors, row-major (may be NULL for NoCutoff); forces are ADDED
+ * into forces_inout ([n][3], may be NULL); energy (kJ/mol) written to
  * *energy_out (may be NULL). */
 int mpid_execute(mpid_context* ctx, const double* positions, const double* box, int include_forces,
                  int include_energy, double* forces_inout, double* energy_out);

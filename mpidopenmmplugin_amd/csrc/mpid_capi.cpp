@@ -1387,8 +1387,10 @@ struct mpid_context {
     }
     bool same_as_last(const double* pos, const double* boxv) const {
         if (!have_last || !mu_valid) return false;
+        double b9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // a null box (NoCutoff) reads as zeros
+        if (boxv) std::memcpy(b9, boxv, sizeof(b9));
         return std::memcmp(last_pos.data(), pos, 3 * (size_t)n * sizeof(double)) == 0 &&
-               std::memcmp(last_box.data(), boxv, 9 * sizeof(double)) == 0;
+               std::memcmp(last_box.data(), b9, sizeof(b9)) == 0;
     }
 
     double execute_host(const double* pos, const double* boxv, double* forces_inout) {
