@@ -141,6 +141,21 @@ int mpid_execute_device(mpid_context* ctx, const double* d_positions, const doub
  * getTotalDipoles (mpidKernels.h:79-81): [n][3] e nm at the given positions
  * (re-evaluated when they differ from the last execute). */
 int mpid_get_induced_dipoles(mpid_context* ctx, const double* positions, const double* box, double* out);
+/* CalcMPIDForceKernel::getElectrostaticPotential (mpidKernels.h:83-84): at
+ * npoints points ([npoints][3], nm) the potential (kJ/mol/e) of every
+ * particle's charge, dipole plus induced dipole and quadrupole (octopoles
+ * are not included, as MPIDReferenceForce.cpp:2464-2489), the separation
+ * taken to its nearest image under PME, after an evaluation at positions. */
+int mpid_get_electrostatic_potential(mpid_context* ctx, const double* positions, const double* box, int npoints,
+                                     const double* points, double* out);
+/* CalcMPIDForceKernel::getSystemMultipoleMoments (mpidKernels.h:86): 13
+ * values about the centre of mass of masses ([n], the System's particle
+ * masses, MPIDReferenceKernels.cpp:325-337): net charge (e), dipole (Debye,
+ * with the induced dipoles) and the traceless quadrupole (Buckingham, 3x3
+ * row-major) as MPIDReferenceForce.cpp:2352-2462, after an evaluation at
+ * positions. */
+int mpid_get_system_multipole_moments(mpid_context* ctx, const double* positions, const double* box,
+                                      const double* masses, double* out13);
 int mpid_get_lab_frame_permanent_dipoles(mpid_context* ctx, const double* positions, const double* box, double* out);
 int mpid_get_total_dipoles(mpid_context* ctx, const double* positions, const double* box, double* out);
 

@@ -81,6 +81,8 @@ SIGNATURES = {
     "mpid_execute": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp]),
     "mpid_execute_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _dp, ctypes.c_void_p, _dp]),
     "mpid_get_induced_dipoles": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
+    "mpid_get_electrostatic_potential": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, ctypes.c_int, _dp, _dp]),
+    "mpid_get_system_multipole_moments": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp]),
     "mpid_get_lab_frame_permanent_dipoles": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
     "mpid_get_total_dipoles": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
     "mpid_get_pme_parameters": (ctypes.c_int, [ctypes.c_void_p, _dp, _ip, _ip, _ip]),

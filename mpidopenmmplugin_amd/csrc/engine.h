@@ -90,6 +90,15 @@ void launch_matvec_finish(hipStream_t s, int n, const double* sqrtpol, const acc
                           const R* mu, double selfc, const double* p, double* Ap);
 // deterministic FP64 reductions
 void launch_dot(hipStream_t s, int m, const double* a, const double* b, double* partial, double* result);
+// Getters after an evaluation (sorted-order state).  Potential at npts points
+// of the charges, dipoles plus induced dipoles and quadrupoles (FP64).
+void launch_potential_points(hipStream_t s, int n, int npts, const double4* pos, const double* lab, const double* mu,
+                             Box<double> box, const double* points, double* out);
+// System multipole sums (one block): out[0] total mass, [1..3] mass-weighted
+// position, [4] charge, [5..7] dipole, [8..13] traced second moment about the
+// centre of mass (xx xy xz yy yz zz), [14..19] atomic quadrupole sum.
+void launch_multipole_moments(hipStream_t s, int n, const int* perm, const double* pos, const double* masses,
+                              const double* lab, const double* mu, double* out);
 // Extrapolated (OPT) polarization: the partial sums c'_o = sum_{j>=o} c_j of
 // the coefficients (at most kMaxExtrapolation), passed by value.
 constexpr int kMaxExtrapolation = 16;

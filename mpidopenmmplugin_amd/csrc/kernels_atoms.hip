@@ -577,6 +577,117 @@ __global__ void k_extrap_response(int n, ExtrapCoefficients cp, const double* __
     }
 }
 
+// ---- getters after an evaluation -----------------------------------------
+// getElectrostaticPotential, MPIDReferenceForce.cpp:2464-2531: at each point,
+// the Coulomb constant times the bare potential of every particle's charge,
+// dipole plus induced dipole and quadrupole (no octopoles), with the
+// separation particle minus point taken to its nearest image.  One thread per
+// point; the particles pass through LDS in groups of kPotThreads.
+constexpr int kPotThreads = 256;
+__global__ __launch_bounds__(kPotThreads) void k_potential_points(int n, int npts, const double4* __restrict__ pos,
+                                                                  const double* __restrict__ lab,
+                                                                  const double* __restrict__ mu, Box<double> box,
+                                                                  const double* __restrict__ points, double* out) {
+    __shared__ double sh[13][kPotThreads];  // x y z, q, d + mu (3), Q xx xy xz yy yz zz
+    const int g = blockIdx.x * kPotThreads + threadIdx.x;
+    const bool gv = g < npts;
+    const V3<double> pt = gv ? mk(points[3 * (size_t)g], points[3 * (size_t)g + 1], points[3 * (size_t)g + 2])
+                             : mk(0.0, 0.0, 0.0);
+    double phi = 0.0;
+    for (int base = 0; base < n; base += kPotThreads) {
+        const int a = base + threadIdx.x;
+        if (a < n) {
+            const double4 p = pos[a];
+            const double* m = lab + (size_t)a * kMP;
+            sh[0][threadIdx.x] = p.x;
+            sh[1][threadIdx.x] = p.y;
+            sh[2][threadIdx.x] = p.z;
+            sh[3][threadIdx.x] = m[0];
+            for (int k = 0; k < 3; k++) sh[4 + k][threadIdx.x] = m[1 + k] + mu[3 * (size_t)a + k];
+            for (int k = 0; k < 6; k++) sh[7 + k][threadIdx.x] = m[4 + k];
+        }
+        __syncthreads();
+        const int cnt = min(kPotThreads, n - base);
+        for (int j = 0; j < cnt; j++) {
+            const V3<double> d =
+                min_image(mk(sh[0][j] - pt.x, sh[1][j] - pt.y, sh[2][j] - pt.z), box);
+            const double r2 = dot(d, d);
+            const double rr1 = 1.0 / sqrt(r2), rr3 = rr1 * rr1 * rr1, rr5 = 3.0 * rr3 * rr1 * rr1;
+            const double qx = sh[7][j] * d.x + sh[8][j] * d.y + sh[9][j] * d.z;
+            const double qy = sh[8][j] * d.x + sh[10][j] * d.y + sh[11][j] * d.z;
+            const double qz = sh[9][j] * d.x + sh[11][j] * d.y + sh[12][j] * d.z;
+            phi += sh[3][j] * rr1 - (sh[4][j] * d.x + sh[5][j] * d.y + sh[6][j] * d.z) * rr3 +
+                   (d.x * qx + d.y * qy + d.z * qz) * rr5;
+        }
+        __syncthreads();
+    }
+    if (gv) out[g] = kElectric * phi;
+}
+
+// getSystemMultipoleMoments, MPIDReferenceForce.cpp:2352-2462, one block in
+// a fixed order: the centre of mass of the given masses, then about it the
+// net charge, the dipole (with the induced dipoles) and the traced second
+// moment (xx xy xz yy yz zz), and the sum of the atomic quadrupoles.  Raw
+// positions and masses are in the caller's order (perm: sorted -> caller).
+// out: [0] mass, [1..3] mass-weighted position, [4] charge, [5..7] dipole,
+// [8..13] second moment, [14..19] quadrupole sum.
+constexpr int kMomThreads = 1024;
+__device__ __forceinline__ double block_sum_1024(double v, double* sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = kMomThreads / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+__global__ __launch_bounds__(kMomThreads) void k_multipole_moments(int n, const int* __restrict__ perm,
+                                                                   const double* __restrict__ pos,
+                                                                   const double* __restrict__ masses,
+                                                                   const double* __restrict__ lab,
+                                                                   const double* __restrict__ mu, double* out) {
+    __shared__ double sh[kMomThreads];
+    double m = 0, mx = 0, my = 0, mz = 0;
+    for (int s = threadIdx.x; s < n; s += kMomThreads) {
+        const int a = perm[s];
+        const double w = masses[a];
+        m += w;
+        mx += w * pos[3 * (size_t)a];
+        my += w * pos[3 * (size_t)a + 1];
+        mz += w * pos[3 * (size_t)a + 2];
+    }
+    m = block_sum_1024(m, sh);
+    mx = block_sum_1024(mx, sh);
+    my = block_sum_1024(my, sh);
+    mz = block_sum_1024(mz, sh);
+    const double c[3] = {m > 0.0 ? mx / m : 0.0, m > 0.0 ? my / m : 0.0, m > 0.0 ? mz / m : 0.0};
+    double acc[16];
+    for (int k = 0; k < 16; k++) acc[k] = 0.0;
+    for (int s = threadIdx.x; s < n; s += kMomThreads) {
+        const int a = perm[s];
+        const double* l = lab + (size_t)s * kMP;
+        const double x[3] = {pos[3 * (size_t)a] - c[0], pos[3 * (size_t)a + 1] - c[1], pos[3 * (size_t)a + 2] - c[2]};
+        const double d[3] = {l[1] + mu[3 * (size_t)s], l[2] + mu[3 * (size_t)s + 1], l[3] + mu[3 * (size_t)s + 2]};
+        const double q = l[0];
+        acc[0] += q;
+        for (int k = 0; k < 3; k++) acc[1 + k] += x[k] * q + d[k];
+        const int ia[6] = {0, 0, 0, 1, 1, 2}, ic[6] = {0, 1, 2, 1, 2, 2};
+        for (int k = 0; k < 6; k++)
+            acc[4 + k] += x[ia[k]] * x[ic[k]] * q + x[ia[k]] * d[ic[k]] + x[ic[k]] * d[ia[k]];
+        for (int k = 0; k < 6; k++) acc[10 + k] += l[4 + k];
+    }
+    for (int k = 0; k < 16; k++) acc[k] = block_sum_1024(acc[k], sh);
+    if (threadIdx.x == 0) {
+        out[0] = m;
+        out[1] = mx;
+        out[2] = my;
+        out[3] = mz;
+        for (int k = 0; k < 16; k++) out[4 + k] = acc[k];
+    }
+}
+
 __global__ void k_axpy(int m, double a, const double* __restrict__ x, double* y) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) y[i] += a * x[i];
 }
@@ -1042,6 +1153,16 @@ void launch_extrap_response(hipStream_t s, int n, const ExtrapCoefficients& cp, 
                             const double* Gs, const int* aniso, acc_t* force, acc_t* torque) {
     hipLaunchKernelGGL(k_extrap_response, blocks_for(n, 256), dim3(256), 0, s, n, cp, mus, Es, Gs, aniso, force,
                        torque);
+}
+void launch_potential_points(hipStream_t s, int n, int npts, const double4* pos, const double* lab, const double* mu,
+                             Box<double> box, const double* points, double* out) {
+    if (npts <= 0) return;
+    hipLaunchKernelGGL(k_potential_points, dim3((npts + kPotThreads - 1) / kPotThreads), dim3(kPotThreads), 0, s, n,
+                       npts, pos, lab, mu, box, points, out);
+}
+void launch_multipole_moments(hipStream_t s, int n, const int* perm, const double* pos, const double* masses,
+                              const double* lab, const double* mu, double* out) {
+    hipLaunchKernelGGL(k_multipole_moments, dim3(1), dim3(kMomThreads), 0, s, n, perm, pos, masses, lab, mu, out);
 }
 void launch_cg_begin(hipStream_t s, int n, double* scal, double* hcheck, double stop_eps) {
     hipLaunchKernelGGL(k_cg_begin, dim3(1), dim3(64), 0, s, scal, hcheck, n, stop_eps);

@@ -1428,6 +1428,41 @@ struct mpid_context {
             for (int k = 0; k < 3; k++) out[3 * (size_t)pm[s] + k] = l[(size_t)s * kMP + 1 + k];
     }
 
+    // getElectrostaticPotential after an evaluation at pos (ensure())
+    void potential_at_points(int npts, const double* points, double* out) {
+        DBuf<double> dpts, dout;
+        dpts.upload(points, 3 * (size_t)npts, stream);
+        dout.resize(std::max(1, npts));
+        launch_potential_points(stream, n, npts, spos.p, lab.p, mud.p, dbox, dpts.p, dout.p);
+        check_launch();
+        HIP_CHECK(hipMemcpyAsync(out, dout.p, (size_t)npts * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+    }
+    // getSystemMultipoleMoments after an evaluation at the positions in pos_in
+    // (MPIDReferenceForce.cpp:2415-2461 for the final traceless form and units)
+    void multipole_moments(const double* masses, double* out) {
+        DBuf<double> dm, dsum;
+        dm.upload(masses, n, stream);
+        dsum.resize(20);
+        launch_multipole_moments(stream, n, perm.p, pos_in.p, dm.p, lab.p, mud.p, dsum.p);
+        check_launch();
+        double v[20];
+        HIP_CHECK(hipMemcpyAsync(v, dsum.p, sizeof(v), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        const double* t = v + 8;   // xx xy xz yy yz zz
+        const double* qa = v + 14; // atomic quadrupoles, same order
+        const double tr = (t[0] + t[3] + t[5]) / 3.0;
+        const double debye = 4.80321;  // the reference's value, :2447
+        out[0] = v[4];
+        for (int k = 0; k < 3; k++) out[1 + k] = v[5 + k] * 10.0 * debye;
+        const int at[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+        for (int a = 0; a < 3; a++)
+            for (int c = 0; c < 3; c++) {
+                const int k = at[a][c];
+                out[4 + 3 * a + c] = (0.5 * (t[k] - (a == c ? tr : 0.0)) + qa[k]) * 100.0 * 3.0 * debye;
+            }
+    }
+
     void compute_fixed_field(const double* pos, const double* boxv, double* out) {
         set_box(boxv);
         pos_in.upload(pos, 3 * (size_t)n, stream);
@@ -1547,6 +1582,24 @@ int mpid_execute_device(mpid_context* ctx, const double* d_positions, const doub
         double e = ctx->execute_device(d_positions, box, d_forces_inout);
         ctx->have_last = false;
         if (energy_out) *energy_out = e;
+    });
+}
+
+int mpid_get_electrostatic_potential(mpid_context* ctx, const double* positions, const double* box, int npoints,
+                                     const double* points, double* out) {
+    if (!ctx || !positions || npoints < 0 || (npoints > 0 && (!points || !out))) return MPID_ERR_INVALID;
+    return guarded(ctx, [&] {
+        ctx->ensure(positions, box);
+        if (npoints > 0) ctx->potential_at_points(npoints, points, out);
+    });
+}
+
+int mpid_get_system_multipole_moments(mpid_context* ctx, const double* positions, const double* box,
+                                      const double* masses, double* out) {
+    if (!ctx || !positions || !masses || !out) return MPID_ERR_INVALID;
+    return guarded(ctx, [&] {
+        ctx->ensure(positions, box);
+        ctx->multipole_moments(masses, out);
     });
 }
 

@@ -122,6 +122,32 @@ class HipCalcMPIDForceKernel:
     def getTotalDipoles(self, positions, box):
         return self._getter(self._lib.mpid_get_total_dipoles, positions, box)
 
+    def getElectrostaticPotential(self, points, positions, box):
+        """CalcMPIDForceKernel::getElectrostaticPotential (mpidKernels.h:83-84):
+        kJ/mol/e at points [m,3] (nm) from charges, dipoles + induced dipoles and
+        quadrupoles, after an evaluation at ``positions``."""
+        pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1)
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1)
+        out = np.zeros(pts.size // 3)
+        b = _box9(box)
+        self._check(self._lib.mpid_get_electrostatic_potential(self._ctx, _capi.dptr(pos), _capi.dptr(b),
+                                                                int(out.size), _capi.dptr(pts), _capi.dptr(out)))
+        return out
+
+    def getSystemMultipoleMoments(self, positions, box, masses):
+        """CalcMPIDForceKernel::getSystemMultipoleMoments (mpidKernels.h:86): 13
+        values (charge e, dipole Debye, traceless quadrupole Buckingham, 3x3)
+        about the centre of mass of ``masses``."""
+        pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1)
+        m = np.ascontiguousarray(masses, dtype=np.float64).reshape(-1)
+        if m.size != self._n:
+            raise MPIDForceError("one mass per particle is required")
+        out = np.zeros(13)
+        b = _box9(box)
+        self._check(self._lib.mpid_get_system_multipole_moments(self._ctx, _capi.dptr(pos), _capi.dptr(b),
+                                                                 _capi.dptr(m), _capi.dptr(out)))
+        return out
+
     def copyParametersToContext(self, force: MPIDForce):
         flat = force.flatten()
         params, keep = _capi.make_parameters(flat)
@@ -221,6 +247,15 @@ class MPIDContext:
 
     def getTotalDipoles(self):
         return self.kernel.getTotalDipoles(self.positions, self.box)
+
+    def getElectrostaticPotential(self, points):
+        """MPIDForce::getElectrostaticPotential (MPIDForce.h:404)."""
+        return self.kernel.getElectrostaticPotential(points, self.positions, self.box)
+
+    def getSystemMultipoleMoments(self, masses):
+        """MPIDForce::getSystemMultipoleMoments (MPIDForce.h:423); the masses are
+        the System's particle masses."""
+        return self.kernel.getSystemMultipoleMoments(self.positions, self.box, masses)
 
     def updateParametersInContext(self):
         self.kernel.copyParametersToContext(self.force)

@@ -1684,6 +1684,80 @@ int oracle_induced_field(const oracle_system* sys, const double* pos, const doub
     }
 }
 
+// calculateElectrostaticPotentialForParticleGridPoint and
+// calculateElectrostaticPotential, MPIDReferenceForce.cpp:2464-2531.
+int oracle_potential(const oracle_system* sys, const double* pos, int npts, const double* points, double* out) {
+    try {
+        Oracle o(*sys);
+        setup(o, pos);
+        o.fixed_field();
+        o.solve();
+        if (!o.converged) return 2;
+        for (int g = 0; g < npts; g++) {
+            const V3 pt(points[3 * g], points[3 * g + 1], points[3 * g + 2]);
+            double phi = 0.0;
+            for (int i = 0; i < o.n; i++) {
+                const Site& s = o.site[i];
+                V3 d = s.pos - pt;  // particle minus point, as the reference
+                o.min_image(d);
+                const double r2 = d.dot(d), r = std::sqrt(r2);
+                const double rr1 = 1.0 / r, rr3 = rr1 * rr1 * rr1, rr5 = 3.0 * rr3 * rr1 * rr1;
+                double qrr = 0.0;
+                for (int a = 0; a < 3; a++)
+                    for (int c = 0; c < 3; c++) qrr += d[a] * s.Q[idx2(a, c)] * d[c];
+                phi += s.q * rr1 - (s.d + o.mu[i]).dot(d) * rr3 + qrr * rr5;
+            }
+            out[g] = kElectric * phi;
+        }
+        return 0;
+    } catch (const std::exception&) {
+        return 1;
+    }
+}
+
+// calculateMPIDSystemMultipoleMoments, MPIDReferenceForce.cpp:2352-2462.
+int oracle_multipole_moments(const oracle_system* sys, const double* pos, const double* masses, double* out) {
+    try {
+        Oracle o(*sys);
+        setup(o, pos);
+        o.fixed_field();
+        o.solve();
+        if (!o.converged) return 2;
+        double mt = 0.0;
+        V3 com;
+        for (int i = 0; i < o.n; i++) {
+            mt += masses[i];
+            com += o.site[i].pos * masses[i];
+        }
+        if (mt > 0.0) com = com * (1.0 / mt);
+        double q = 0.0, t[6] = {0, 0, 0, 0, 0, 0};  // traced second moment xx xy xz yy yz zz
+        V3 dpl;
+        for (int i = 0; i < o.n; i++) {
+            const Site& s = o.site[i];
+            const V3 x = s.pos - com, d = s.d + o.mu[i];
+            q += s.q;
+            dpl += x * s.q + d;
+            for (int a = 0; a < 3; a++)
+                for (int c = a; c < 3; c++) t[idx2(a, c)] += x[a] * x[c] * s.q + x[a] * d[c] + x[c] * d[a];
+        }
+        const double tr = (t[idx2(0, 0)] + t[idx2(1, 1)] + t[idx2(2, 2)]) / 3.0;
+        double Q[3][3];
+        for (int a = 0; a < 3; a++)
+            for (int c = 0; c < 3; c++) {
+                Q[a][c] = 0.5 * (t[idx2(a, c)] - (a == c ? tr : 0.0));
+                for (int i = 0; i < o.n; i++) Q[a][c] += o.site[i].Q[idx2(a, c)];
+            }
+        const double debye = 4.80321;  // the reference's value (:2447)
+        out[0] = q;
+        for (int a = 0; a < 3; a++) out[1 + a] = dpl[a] * 10.0 * debye;
+        for (int a = 0; a < 3; a++)
+            for (int c = 0; c < 3; c++) out[4 + 3 * a + c] = Q[a][c] * 100.0 * 3.0 * debye;
+        return 0;
+    } catch (const std::exception&) {
+        return 1;
+    }
+}
+
 int oracle_evaluate(const oracle_system* sys, const double* pos, oracle_out* out) {
     try {
         Oracle o(*sys);

@@ -84,6 +84,15 @@ int oracle_fft3d(double* data_interleaved, int nx, int ny, int nz, int sign);
 /* Induced field T*mu (real + reciprocal + self) for given lab dipoles, no
  * solver: the matrix-vector product of the mutual solver. */
 int oracle_induced_field(const oracle_system* sys, const double* pos, const double* mu, double* field_out);
+/* getElectrostaticPotential after the solve (MPIDReferenceForce.cpp:2464-2531):
+ * at each of the npts points, the Coulomb constant times the bare potential
+ * of every particle's charge, dipole plus induced dipole and quadrupole
+ * (octopoles are not included), nearest image under PME. */
+int oracle_potential(const oracle_system* sys, const double* pos, int npts, const double* points, double* out);
+/* getSystemMultipoleMoments after the solve (MPIDReferenceForce.cpp:2352-2462):
+ * net charge, dipole (Debye) and traceless quadrupole (Buckingham), 13
+ * values, about the centre of mass of the given masses. */
+int oracle_multipole_moments(const oracle_system* sys, const double* pos, const double* masses, double* out13);
 
 #ifdef __cplusplus
 }

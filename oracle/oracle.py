@@ -71,6 +71,10 @@ def lib():
         L.oracle_evaluate.restype = ctypes.c_int
         L.oracle_induced_field.argtypes = [ctypes.POINTER(_System), _dp, _dp, _dp]
         L.oracle_induced_field.restype = ctypes.c_int
+        L.oracle_potential.argtypes = [ctypes.POINTER(_System), _dp, ctypes.c_int, _dp, _dp]
+        L.oracle_potential.restype = ctypes.c_int
+        L.oracle_multipole_moments.argtypes = [ctypes.POINTER(_System), _dp, _dp, _dp]
+        L.oracle_multipole_moments.restype = ctypes.c_int
         L.oracle_bspline.argtypes = [ctypes.c_double, _dp]
         L.oracle_bspline_moduli.argtypes = [ctypes.c_int, _dp]
         L.oracle_fft3d.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
@@ -120,7 +124,7 @@ def _system(flat, box, polarization=None, flags=0):
     s.scale14 = float(flat["scale14"])
     s.max_iter = int(flat["max_iter"])
     s.target_eps = float(flat["target_eps"])
-    b = np.asarray(box, dtype=np.float64).reshape(-1)
+    b = np.zeros(9) if box is None else np.asarray(box, dtype=np.float64).reshape(-1)  # NoCutoff: no box
     if b.size == 3:
         b = np.diag(b).reshape(-1)
     for k in range(9):
@@ -186,3 +190,27 @@ def fft3d(a, sign=-1):
     buf = np.ascontiguousarray(np.stack([a.real, a.imag], axis=-1), dtype=np.float64)
     lib().oracle_fft3d(_ptr(buf, "d"), a.shape[0], a.shape[1], a.shape[2], int(sign))
     return buf[..., 0] + 1j * buf[..., 1]
+
+
+def electrostatic_potential(flat, positions, box, points):
+    """getElectrostaticPotential (kJ/mol/e) at points [m,3] after the solve."""
+    s, keep = _system(flat, box)
+    pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1)
+    pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1)
+    out = np.zeros(pts.size // 3)
+    rc = lib().oracle_potential(ctypes.byref(s), _ptr(pos, "d"), int(out.size), _ptr(pts, "d"), _ptr(out, "d"))
+    if rc != 0:
+        raise OracleError("oracle_potential failed (%d)" % rc)
+    return out
+
+
+def system_multipole_moments(flat, positions, box, masses):
+    """getSystemMultipoleMoments: 13 values (charge, Debye, Buckingham)."""
+    s, keep = _system(flat, box)
+    pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1)
+    m = np.ascontiguousarray(masses, dtype=np.float64).reshape(-1)
+    out = np.zeros(13)
+    rc = lib().oracle_multipole_moments(ctypes.byref(s), _ptr(pos, "d"), _ptr(m, "d"), _ptr(out, "d"))
+    if rc != 0:
+        raise OracleError("oracle_multipole_moments failed (%d)" % rc)
+    return out

@@ -366,3 +366,30 @@ def test_nocutoff_water216x(precision, polarization):
     assert abs(e - ref["energy"]) <= e_tol * abs(ref["energy"])
     assert rel_rms(F, ref["forces"]) <= f_tol
     assert rel_rms(k.getInducedDipoles(pos, None), ref["induced"]) <= f_tol
+
+
+@pytest.mark.parametrize("method", ["pme", "nocutoff"])
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_potential_and_moments_water216x(precision, method):
+    """getElectrostaticPotential at 64 points and getSystemMultipoleMoments
+    after an evaluation, against the oracle (whose forms are pinned in
+    tests/test_oracle_getters.py)."""
+    print(f"[gpu] potential and moments water216x {precision} {method}", flush=True)
+    tight, pos, box = make_waterbox("water216x", target_eps=1e-10)
+    f, _, _ = make_waterbox("water216x")
+    if method == "nocutoff":
+        tight.setNonbondedMethod(MPIDForce.NoCutoff)
+        f.setNonbondedMethod(MPIDForce.NoCutoff)
+    rng = np.random.default_rng(11)
+    pts = rng.uniform(0.0, float(np.asarray(box).reshape(-1)[0]), (64, 3))
+    masses = np.tile([15.999, 1.008, 1.008], len(pos) // 3)
+    ref_phi = oracle.electrostatic_potential(tight.flatten(), pos, box, pts)
+    ref_mom = oracle.system_multipole_moments(tight.flatten(), pos, box, masses)
+    k = HipCalcMPIDForceKernel(precision)
+    k.initialize(len(pos), tight if precision == "double" else f)
+    phi = k.getElectrostaticPotential(pts, pos, box)
+    mom = k.getSystemMultipoleMoments(pos, box, masses)
+    tol = 1e-9 if precision == "double" else 1e-5
+    assert rel_rms(phi, ref_phi) <= tol
+    assert abs(mom[0] - ref_mom[0]) <= 1e-9
+    assert rel_rms(mom[1:], ref_mom[1:]) <= tol
