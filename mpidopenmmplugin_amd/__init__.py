@@ -1,2 +1,3 @@
 """MI355X-native MPIDForce (multipoles + induced dipoles, PME) evaluation."""
 from .mpidforce import MPIDForce, MPIDForceError  # noqa: F401
+from .serialization import XmlSerializer  # noqa: F401
