@@ -212,9 +212,13 @@ __global__ __launch_bounds__(64) void k_nl_masks(int n, const int* __restrict__ 
 
 // Phase 2, one workgroup of kFillWaves wavefronts per block I: the row of I
 // is cut into kFillWaves contiguous chunks in ascending J.  Each wavefront
-// counts its chunk's entries per atom, a prefix over the chunks gives each
-// its offsets, and lane i then appends the atoms of each mask in ascending
-// order.  all != 0: the full list of every atom, stored by atom index.
+// counts its chunk's entries per atom, and a prefix over the chunks gives
+// each its offsets.  The entries are then written atom by atom: for each
+// row entry, the wavefront takes in turn each stored atom whose mask is not
+// empty, and lane jj writes neighbour jj at the atom's offset plus the
+// number of mask bits below jj (all lanes work; a lane-per-atom walk over
+// the mask bits would idle the lanes of non-stored atoms and of short
+// masks).  all != 0: the full list of every atom, stored by atom index.
 // all == 0: for the polarisable atoms only, the full list and the list of
 // polarisable neighbours (polmask of the neighbour's block), stored by the
 // atom's position p in the sorted index of polarisable atoms
@@ -259,20 +263,35 @@ __global__ __launch_bounds__(kFillWaves * kTile) void k_nl_fill(
         tot += scnt[v][lane];
         ptot += spcnt[v][lane];
     }
-    if (store) {
-        for (int r = a; r < b; r++) {
-            unsigned long long m = nmask[(size_t)r * kTile + lane];
-            const int J = rows[r];
-            const unsigned long long polJ = all ? 0ull : polmask[J];
-            while (m) {
-                const int jj = __ffsll((long long)m) - 1;
-                m &= m - 1ull;
-                if (cnt < cap) nbr[nl_index(li, cnt, nrows)] = J * kTile + jj;
-                cnt++;
-                if ((polJ >> jj) & 1ull) {
-                    if (pcnt < cap) pnbr[nl_index(li, pcnt, nrows)] = J * kTile + jj;
-                    pcnt++;
+    const unsigned long long below = (1ull << lane) - 1ull;  // mask bits of the lanes below this one
+    for (int r = a; r < b; r++) {
+        const unsigned long long mrow = iv ? nmask[(size_t)r * kTile + lane] : 0ull;
+        const int J = rows[r];
+        const unsigned long long polJ = all ? 0ull : polmask[J];
+        unsigned long long act = __ballot(store && mrow != 0ull);
+        while (act) {
+            const int t = __ffsll((long long)act) - 1;  // uniform: the atom of lane t
+            act &= act - 1ull;
+            const unsigned lo = __builtin_amdgcn_readlane((unsigned)mrow, t);
+            const unsigned hi = __builtin_amdgcn_readlane((unsigned)(mrow >> 32), t);
+            const unsigned long long m = ((unsigned long long)hi << 32) | lo;
+            const int base = __builtin_amdgcn_readlane(cnt, t);
+            const int lt = __builtin_amdgcn_readlane(li, t);
+            if ((m >> lane) & 1ull) {
+                const int k = base + __popcll(m & below);
+                if (k < cap) nbr[nl_index(lt, k, nrows)] = J * kTile + lane;
+            }
+            const unsigned long long mp = m & polJ;
+            if (!all) {
+                const int pbase = __builtin_amdgcn_readlane(pcnt, t);
+                if ((mp >> lane) & 1ull) {
+                    const int k = pbase + __popcll(mp & below);
+                    if (k < cap) pnbr[nl_index(lt, k, nrows)] = J * kTile + lane;
                 }
+            }
+            if (lane == t) {
+                cnt += __popcll(m);
+                pcnt += __popcll(mp);
             }
         }
     }
