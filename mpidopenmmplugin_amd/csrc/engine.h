@@ -255,8 +255,10 @@ void launch_electrostatics_pairs(hipStream_t s, int nrows, const int* rows, cons
                                  const double* lab64, PairParams<double> p64, acc_t* force, acc_t* torque,
                                  acc_t* energy);
 // ---- neighbour lists (kernels_nlist.hip) ---------------------------------------
+// Block pairs within the cutoff: counts (rows == nullptr) or the rows at
+// start[I] (at most cap entries are stored; the count may exceed it)
 void launch_block_rows(hipStream_t s, int nb, const float4* bmin, const float4* bmax, Box<float> box, float cutoff,
-                       const int* start, int* counts, int* rows, int* owner);
+                       const int* start, int* counts, int* rows, int* owner, int cap);
 void launch_exclusive_scan(hipStream_t s, int m, const int* counts, int* start);
 // pos.w of the FP64 positions is the damping factor: nonzero iff the site is polarisable
 // Masks, then lists: all = true gives the full list of every atom stored by
@@ -265,8 +267,10 @@ void launch_exclusive_scan(hipStream_t s, int m, const int* counts, int* start);
 // the exclusive scan of polcnt, is written on the way).  nmask: 64 words per
 // row entry (nrows entries); polmask: one word per block; scalars[0]:
 // largest stored list length, scalars[1]: nonzero if a block pair lacked its
-// mirror entry.
-void launch_nl_build(hipStream_t s, int n, int nrows, bool all, const int* rstart, const int* rows,
+// mirror entry.  nrows: row entries launched (the count, or the capacity
+// when the count is not known on the host: entries past rstart[nb] return);
+// rcap: the capacity of rows / nmask (entries beyond it are ignored).
+void launch_nl_build(hipStream_t s, int n, int nrows, int rcap, bool all, const int* rstart, const int* rows,
                      const int* owner, const float4* bmin, const float4* bmax, const double4* pos,
                      const int* partners, const int* npart, Box<double> box, double rc2, int cap,
                      unsigned long long* nmask, unsigned long long* polmask, int* polcnt, int* polstart, int* nbr,

@@ -61,7 +61,7 @@ __device__ __forceinline__ bool blocks_within(float4 a0, float4 a1, float4 b0, f
 __global__ __launch_bounds__(64) void k_block_rows(int nb, const float4* __restrict__ bmin,
                                                    const float4* __restrict__ bmax, Box<float> box, float cutoff,
                                                    const int* __restrict__ start, int* __restrict__ counts,
-                                                   int* __restrict__ rows, int* __restrict__ owner) {
+                                                   int* __restrict__ rows, int* __restrict__ owner, int cap) {
     const int I = blockIdx.x, lane = threadIdx.x;
     const float4 a0 = bmin[I], a1 = bmax[I];
     const int base = rows ? start[I] : 0;
@@ -72,8 +72,10 @@ __global__ __launch_bounds__(64) void k_block_rows(int nb, const float4* __restr
         const unsigned long long m = __ballot(hit);
         if (hit && rows) {
             const int at = base + c + __popcll(m & ((1ull << lane) - 1ull));
-            rows[at] = J;
-            owner[at] = I;
+            if (at < cap) {  // entries beyond the capacity: the host grows it and builds again
+                rows[at] = J;
+                owner[at] = I;
+            }
         }
         c += __popcll(m);
     }
@@ -125,10 +127,11 @@ __global__ __launch_bounds__(64) void k_nl_masks(int n, const int* __restrict__ 
                                                  unsigned long long* __restrict__ nmask,
                                                  unsigned long long* __restrict__ tmask,
                                                  unsigned long long* __restrict__ tmaskcc,
-                                                 unsigned long long* __restrict__ polmask, int* __restrict__ polcnt,
-                                                 int* __restrict__ flag) {
+                                                 int* __restrict__ flag, int cap) {
     __shared__ double4 sp[kTile];
     const int r = blockIdx.x, lane = threadIdx.x;
+    const int nbk = (n + kTile - 1) / kTile;
+    if (r >= min(rstart[nbk], cap)) return;  // uniform: the grid may cover the capacity, not the count
     const int I = owner[r], J = rows[r];
     if (J < I) return;  // uniform: written by the mirrored entry
     const int i = I * kTile + lane, j = J * kTile + lane;
@@ -137,14 +140,6 @@ __global__ __launch_bounds__(64) void k_nl_masks(int n, const int* __restrict__ 
     const double4 pi = pos[ic], pj = pos[jc];
     const bool ibare = iv && chg[ic] != 0;
     const unsigned long long jbare = __ballot(jv && chg[jc] != 0);
-    if (J == I) {
-        // the polarisable atoms of the block (damping factor nonzero)
-        const unsigned long long pm = __ballot(jv && pj.w != 0.0);
-        if (lane == 0) {
-            polmask[I] = pm;
-            polcnt[I] = __popcll(pm);
-        }
-    }
     sp[lane] = pj;
     // candidates: atoms of J whose distance to block I's box is within the
     // cutoff (minimum image; never rejects a pair within the cutoff)
@@ -199,7 +194,7 @@ __global__ __launch_bounds__(64) void k_nl_masks(int n, const int* __restrict__ 
             if (lane == k) t = col;
         }
         // the mirrored entry: I in the ascending row of J
-        int lo = rstart[J], hi = rstart[J + 1] - 1;
+        int lo = min(rstart[J], cap), hi = min(rstart[J + 1], cap) - 1;
         while (lo < hi) {
             const int mid = (lo + hi) >> 1;
             if (rows[mid] < I) lo = mid + 1;
@@ -231,7 +226,7 @@ __global__ __launch_bounds__(kFillWaves * kTile) void k_nl_fill(
     int n, int all, const int* __restrict__ rstart, const int* __restrict__ rows,
     const unsigned long long* __restrict__ polmask, const int* __restrict__ polstart,
     const unsigned long long* __restrict__ nmask, int cap, int* __restrict__ nbr, int* __restrict__ count,
-    int* __restrict__ maxcount, int* __restrict__ pnbr, int* __restrict__ pcount) {
+    int* __restrict__ maxcount, int* __restrict__ pnbr, int* __restrict__ pcount, int rcap) {
     __shared__ int scnt[kFillWaves][kTile], spcnt[kFillWaves][kTile];
     const int I = blockIdx.x, lane = threadIdx.x & (kTile - 1), w = threadIdx.x / kTile;
     const int nb = (n + kTile - 1) / kTile;
@@ -242,7 +237,8 @@ __global__ __launch_bounds__(kFillWaves * kTile) void k_nl_fill(
     const bool store = iv && (all || ipol);
     const int li = all ? i : polstart[I] + __popcll(pmI & ((1ull << lane) - 1ull));
     const int nrows = all ? n : polstart[nb];
-    const int r0 = rstart[I], r1 = rstart[I + 1];
+    // (rows beyond the capacity rcap were not stored; the host grows it and builds again)
+    const int r0 = min(rstart[I], rcap), r1 = min(rstart[I + 1], rcap);
     const int per = (r1 - r0 + kFillWaves - 1) / kFillWaves;
     const int a = min(r0 + w * per, r1), b = min(a + per, r1);
     int c = 0, pc = 0;
@@ -299,6 +295,19 @@ __global__ __launch_bounds__(kFillWaves * kTile) void k_nl_fill(
         count[i] = min(tot, cap);
         if (!all) pcount[i] = min(ptot, cap);
         if (store) atomicMax(maxcount, tot);
+    }
+}
+
+// The polarisable atoms of each block (damping factor nonzero): a mask and
+// its count, one wavefront per block.
+__global__ __launch_bounds__(64) void k_pol_mask(int n, const double4* __restrict__ pos,
+                                                 unsigned long long* __restrict__ polmask, int* __restrict__ polcnt) {
+    const int I = blockIdx.x, lane = threadIdx.x;
+    const int i = I * kTile + lane;
+    const unsigned long long pm = __ballot(i < n && pos[i < n ? i : n - 1].w != 0.0);
+    if (lane == 0) {
+        polmask[I] = pm;
+        polcnt[I] = __popcll(pm);
     }
 }
 
@@ -436,26 +445,27 @@ __global__ __launch_bounds__(256) void k_nl_induced_field(NbrList L, const int* 
 // launchers
 // ---------------------------------------------------------------------------
 void launch_block_rows(hipStream_t s, int nb, const float4* bmin, const float4* bmax, Box<float> box, float cutoff,
-                       const int* start, int* counts, int* rows, int* owner) {
+                       const int* start, int* counts, int* rows, int* owner, int cap) {
     hipLaunchKernelGGL(k_block_rows, dim3(nb), dim3(kTile), 0, s, nb, bmin, bmax, box, cutoff, start, counts, rows,
-                       owner);
+                       owner, cap);
 }
 void launch_exclusive_scan(hipStream_t s, int m, const int* counts, int* start) {
     hipLaunchKernelGGL(k_exclusive_scan, dim3(1), dim3(1024), 0, s, m, counts, start);
 }
-void launch_nl_build(hipStream_t s, int n, int nrows, bool all, const int* rstart, const int* rows,
+void launch_nl_build(hipStream_t s, int n, int nrows, int rcap, bool all, const int* rstart, const int* rows,
                      const int* owner, const float4* bmin, const float4* bmax, const double4* pos,
                      const int* partners, const int* npart, Box<double> box, double rc2, int cap,
                      unsigned long long* nmask, unsigned long long* polmask, int* polcnt, int* polstart, int* nbr,
                      int* count, int* scalars, unsigned long long* tmask, int* pnbr, int* pcount, const int* chg,
                      unsigned long long* tmaskcc) {
     const int nb = (n + kTile - 1) / kTile;
+    hipLaunchKernelGGL(k_pol_mask, dim3(nb), dim3(kTile), 0, s, n, pos, polmask, polcnt);
     if (nrows > 0)
         hipLaunchKernelGGL(k_nl_masks, dim3(nrows), dim3(kTile), 0, s, n, rstart, rows, owner, bmin, bmax, pos,
-                           partners, npart, box, rc2, chg, nmask, tmask, tmaskcc, polmask, polcnt, scalars + 1);
+                           partners, npart, box, rc2, chg, nmask, tmask, tmaskcc, scalars + 1, rcap);
     launch_exclusive_scan(s, nb, polcnt, polstart);
     hipLaunchKernelGGL(k_nl_fill, dim3(nb), dim3(kFillWaves * kTile), 0, s, n, all ? 1 : 0, rstart, rows, polmask,
-                       polstart, nmask, cap, nbr, count, scalars, pnbr, pcount);
+                       polstart, nmask, cap, nbr, count, scalars, pnbr, pcount, rcap);
 }
 void launch_pol_fill(hipStream_t s, int n, const double4* pos, const int* start, int* idx) {
     const int nb = (n + kTile - 1) / kTile;

@@ -230,6 +230,8 @@ struct mpid_context {
     int npol = 0;
     DBuf<unsigned long long> tmask, tmaskcc, nmask, polmask;
     int rows_total = 0;
+    bool rows_pending = false;  // the block-row count of this build is still on its way to hnl[2]
+    int rows_grow_to = 0;       // a row count that overflowed the capacity (grown before the retry)
     bool all_lists = false;  // lists of the last build: every atom (true) or the polarisable ones
     int nl_cap = 0;
     int nl_max = 0;
@@ -783,24 +785,36 @@ struct mpid_context {
         timed(P_TILES, [&] {
             launch_block_bounds<double>(stream, n, pd.pos.p, bmin.p, bmax.p);
             launch_block_rows(stream, nb, bmin.p, bmax.p, rbox<float>(), (float)opt.cutoff, nullptr, brow_count.p,
-                              nullptr, nullptr);
+                              nullptr, nullptr, 0);
             launch_exclusive_scan(stream, nb, brow_count.p, brow_start.p);
         });
         check_launch();
-        HIP_CHECK(hipMemcpyAsync(hscal + 3, brow_start.p + nb, sizeof(int), hipMemcpyDeviceToHost, stream));
-        if (fixed_recip) recip_cycle_fixed(pd);
-        wait_stream();
-        std::memcpy(&rows_total, hscal + 3, sizeof(int));
-        if ((size_t)rows_total > brows.n) {
-            const size_t cap = rows_total + rows_total / 4 + 64;
-            brows.resize(cap);
-            brow_owner.resize(cap);
-            tmask.resize(cap * kTile);
-            tmaskcc.resize(cap * kTile);
-            nmask.resize(cap * kTile);
+        // After a verified build for the current parameters the count is
+        // not waited for: the row kernels run over the capacity and stop at
+        // the count on the device, which is checked with the lists at the
+        // solver's first synchronisation (check_lists_pending).
+        rows_pending = !all && npol_cached >= 0 && brows.n > 0;
+        if (rows_pending) {
+            HIP_CHECK(hipMemcpyAsync(hnl + 2, brow_start.p + nb, sizeof(int), hipMemcpyDeviceToHost, stream));
+            if (fixed_recip) recip_cycle_fixed(pd);
+        } else {
+            HIP_CHECK(hipMemcpyAsync(hscal + 3, brow_start.p + nb, sizeof(int), hipMemcpyDeviceToHost, stream));
+            if (fixed_recip) recip_cycle_fixed(pd);
+            wait_stream();
+            std::memcpy(&rows_total, hscal + 3, sizeof(int));
+            grow_rows(rows_total);
+            stats.ntiles = (rows_total + nb) / 2;  // block pairs with J >= I
         }
-        stats.ntiles = (rows_total + nb) / 2;  // block pairs with J >= I
         build_neighbour_lists(nb);
+    }
+    void grow_rows(int count) {
+        if ((size_t)count <= brows.n) return;
+        const size_t cap = count + count / 4 + 64;
+        brows.resize(cap);
+        brow_owner.resize(cap);
+        tmask.resize(cap * kTile);
+        tmaskcc.resize(cap * kTile);
+        nmask.resize(cap * kTile);
     }
 
     // Per-atom lists: of the polarisable atoms (by position in pol_idx) for an
@@ -819,7 +833,7 @@ struct mpid_context {
         }
         timed(P_TILES, [&] {
             launch_block_rows(stream, nb, bmin.p, bmax.p, rbox<float>(), (float)opt.cutoff, brow_start.p, nullptr,
-                              brows.p, brow_owner.p);
+                              brows.p, brow_owner.p, (int)brows.n);
         });
         for (int attempt = 0; attempt < 2; attempt++) {
             nl_cap = (nl_cap + kNlSplit - 1) / kNlSplit * kNlSplit;
@@ -827,7 +841,8 @@ struct mpid_context {
             pnbr.resize((size_t)n * nl_cap);
             timed(P_TILES, [&] {
                 HIP_CHECK(hipMemsetAsync(nl_scalars.p, 0, 2 * sizeof(int), stream));
-                launch_nl_build(stream, n, rows_total, all_lists, brow_start.p, brows.p, brow_owner.p, bmin.p, bmax.p,
+                launch_nl_build(stream, n, rows_pending ? (int)brows.n : rows_total, (int)brows.n, all_lists,
+                                brow_start.p, brows.p, brow_owner.p, bmin.p, bmax.p,
                                 pd.pos.p, spart.p, snpart.p, dbox, opt.cutoff * opt.cutoff, nl_cap, nmask.p,
                                 polmask.p, polcnt.p, polstart.p, nbr.p, ncount.p, nl_scalars.p, tmask.p, pnbr.p,
                                 pcount.p, chg.p, tmaskcc.p);
@@ -867,6 +882,17 @@ struct mpid_context {
     void check_lists_pending() {
         if (!lists_pending) return;
         lists_pending = false;
+        if (rows_pending) {
+            rows_pending = false;
+            const int count = hnl[2];
+            if ((size_t)count > brows.n) {  // rows beyond the capacity were dropped: grow and evaluate again
+                rows_grow_to = count;
+                npol_cached = -1;
+                throw ListsOverflow{};
+            }
+            rows_total = count;
+            stats.ntiles = (rows_total + (n + kTile - 1) / kTile) / 2;
+        }
         if (hnl[1] != 0) throw MpidError(MPID_ERR_HIP, "neighbour masks: a block pair lacks its mirror entry");
         nl_max = hnl[0];
         if (nl_max > nl_cap) {
@@ -1367,6 +1393,7 @@ struct mpid_context {
                 // finish before the lists are reallocated
                 HIP_CHECK(hipStreamSynchronize(stream));
                 HIP_CHECK(hipStreamSynchronize(rstream));
+                grow_rows(rows_grow_to);
             }
         }
         double e = forces_and_energy(P, d_forces_out);
