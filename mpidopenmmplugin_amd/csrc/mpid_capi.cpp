@@ -806,6 +806,7 @@ struct mpid_context {
             stats.ntiles = (rows_total + nb) / 2;  // block pairs with J >= I
         }
         build_neighbour_lists(nb);
+        recip_cycle_fixed_rest(pd);  // if the list build has not issued it
     }
     void grow_rows(int count) {
         if ((size_t)count <= brows.n) return;
@@ -855,6 +856,7 @@ struct mpid_context {
                 }
             });
             check_launch();
+            recip_cycle_fixed_rest(pd);
             if (!all_lists && npol_cached >= 0) {
                 HIP_CHECK(hipMemcpyAsync(hnl, nl_scalars.p, 2 * sizeof(int), hipMemcpyDeviceToHost, stream));
                 npol = npol_cached;
@@ -916,6 +918,10 @@ struct mpid_context {
 
     // Reciprocal cycles: issued to rstream after a fork(); the caller joins
     // before it reads their output (phi, phidp, erec).
+    // Issued in two parts: the spreading first, the transforms and the gather
+    // once the neighbour-list kernels are issued (the host's launches would
+    // otherwise hold those back while the spreading runs).
+    bool fixed_recip_rest = false;
     template <class R> void recip_cycle_fixed(Prec<R>& P) {
         if (!pme()) return;  // phi stays zero
         PmeGeom<R> g = geom<R>();
@@ -936,6 +942,13 @@ struct mpid_context {
                 launch_spread_fixed<R>(rstream, n, P.pos.p, P.lab.p, g, P.grid.p);
             }
         });
+        check_launch();
+        fixed_recip_rest = true;
+    }
+    template <class R> void recip_cycle_fixed_rest(Prec<R>& P) {
+        if (!fixed_recip_rest) return;
+        fixed_recip_rest = false;
+        PmeGeom<R> g = geom<R>();
         timed_on(rstream, P_FFT, [&] { launch_recip_convolution<R>(rstream, P.plan, P.grid.p, P.spec.p, conv_params(P)); });
         timed_on(rstream, P_GATHER, [&] { launch_gather_phi<R>(rstream, n, P.pos.p, g, P.grid.p, P.phi.p); });
         check_launch();
