@@ -99,6 +99,10 @@ void launch_potential_points(hipStream_t s, int n, int npts, const double4* pos,
 // centre of mass (xx xy xz yy yz zz), [14..19] atomic quadrupole sum.
 void launch_multipole_moments(hipStream_t s, int n, const int* perm, const double* pos, const double* masses,
                               const double* lab, const double* mu, double* out);
+// Energies accumulate in fixed point over kEnergySlots words per term (a
+// workgroup adds to slot blockIdx % kEnergySlots; the host sums the slots):
+// atomics from every workgroup on one word serialise across the chip.
+constexpr int kEnergySlots = 256;
 // Extrapolated (OPT) polarization: the partial sums c'_o = sum_{j>=o} c_j of
 // the coefficients (at most kMaxExtrapolation), passed by value.
 constexpr int kMaxExtrapolation = 16;

@@ -904,7 +904,7 @@ __global__ __launch_bounds__(128) void k_recip_atoms(int n, int mutual, const do
         if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
         __syncthreads();
     }
-    if (threadIdx.x == 0) acc_add(energy, sh[0]);
+    if (threadIdx.x == 0) acc_add(energy + (blockIdx.x & (kEnergySlots - 1)), sh[0]);
 }
 
 // ---------------------------------------------------------------------------

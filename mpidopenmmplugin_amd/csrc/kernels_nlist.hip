@@ -291,10 +291,17 @@ __global__ __launch_bounds__(kFillWaves * kTile) void k_nl_fill(
             }
         }
     }
-    if (w == 0 && iv) {
-        count[i] = min(tot, cap);
-        if (!all) pcount[i] = min(ptot, cap);
-        if (store) atomicMax(maxcount, tot);
+    if (w == 0) {
+        if (iv) {
+            count[i] = min(tot, cap);
+            if (!all) pcount[i] = min(ptot, cap);
+        }
+        // the block's longest list, then one atomic per block (atomics from
+        // every lane on one word serialise across the chip)
+        int lmax = store ? tot : 0;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) lmax = max(lmax, __shfl_xor(lmax, off));
+        if (lane == 0 && lmax > 0) atomicMax(maxcount, lmax);
     }
 }
 

@@ -297,7 +297,7 @@ __device__ __forceinline__ void electrostatics_tile(const int* __restrict__ rows
     if (tid == 0) {
         double e = 0.0;
         for (int w = 0; w < kElecThreads / kTile; w++) e += sen[w];
-        acc_add(energy, e);
+        acc_add(energy + (blockIdx.x & (kEnergySlots - 1)), e);
     }
 }
 
@@ -485,7 +485,7 @@ __global__ __launch_bounds__(128) void k_cov_electrostatics(PairAtoms<R> A, cons
     // every lane reaches the reduction
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) en += __shfl_xor(en, off);
-    if ((threadIdx.x & 63) == 0 && en != 0.0) acc_add(energy, en);
+    if ((threadIdx.x & 63) == 0 && en != 0.0) acc_add(energy + (blockIdx.x & (kEnergySlots - 1)), en);
 }
 
 // ---------------------------------------------------------------------------

@@ -610,7 +610,7 @@ struct mpid_context {
         field_acc.resize(n3);
         force_acc.resize(n3);
         torque_acc.resize(n3);
-        energy_acc.resize(2);
+        energy_acc.resize(2 * kEnergySlots);
         E0.resize(n3);
         b.resize(n3);
         x.resize(n3);
@@ -1328,7 +1328,7 @@ struct mpid_context {
     // reciprocal per-atom terms and the covalent pairs run on rstream beside
     // the real-space electrostatics; all of them add into the fixed-point
     // force, torque and energy accumulators (integer adds, so the order does
-    // not matter).  energy_acc[0] takes the real-space energy, [1] the
+    // not matter).  energy_acc[0, kEnergySlots) takes the real-space energy, the next kEnergySlots the
     // reciprocal and self terms.
     template <class R> void issue_final_pass(Prec<R>& P) {
         join();  // a previous final pass's reciprocal kernels have added into the accumulators
@@ -1338,7 +1338,7 @@ struct mpid_context {
         timed(P_OTHER, [&] {
             launch_zero_acc(stream, force_acc.p, force_acc.n);
             launch_zero_acc(stream, torque_acc.p, torque_acc.n);
-            launch_zero_acc(stream, energy_acc.p, 2);
+            launch_zero_acc(stream, energy_acc.p, 2 * kEnergySlots);
         });
         if (opt.polarization == MPID_POLARIZATION_EXTRAPOLATED)
             timed(P_SOLVER, [&] {
@@ -1358,7 +1358,7 @@ struct mpid_context {
         timed_on(rstream, P_RECIP, [&] {
             launch_recip_atoms<R>(rstream, n, opt.polarization == MPID_POLARIZATION_MUTUAL ? 1 : 0, pd.phi.p,
                                   P.phidp.p, lab.p, mud.p, aniso.p, gscale3(), opt.ewald_alpha, force_acc.p,
-                                  torque_acc.p, energy_acc.p + 1);
+                                  torque_acc.p, energy_acc.p + kEnergySlots);
         });
         timed_on(rstream, P_COV, [&] {
             launch_cov_electrostatics<double>(rstream, n, pair_atoms(pd), mud.p, pair_params<double>(), force_acc.p,
@@ -1379,8 +1379,13 @@ struct mpid_context {
         });
         check_launch();
         acc_t e[2] = {0, 0};
-        HIP_CHECK(hipMemcpyAsync(e, energy_acc.p, 2 * sizeof(acc_t), hipMemcpyDeviceToHost, stream));
+        acc_t slots[2 * kEnergySlots];
+        HIP_CHECK(hipMemcpyAsync(slots, energy_acc.p, sizeof(slots), hipMemcpyDeviceToHost, stream));
         HIP_CHECK(hipStreamSynchronize(stream));
+        for (int k = 0; k < kEnergySlots; k++) {  // integer sums: any order gives the same bits
+            e[0] += slots[k];
+            e[1] += slots[kEnergySlots + k];
+        }
         if (correction_pending) {
             energy_correction = -0.5 * kElectric * hscal[2];
             correction_pending = false;
