@@ -80,38 +80,35 @@ def random_traceless_octopole(rng, scale):
     return [o[c] for c in combos]
 
 
-def make_waterbox(name="water216", seed=None, polarization=MPIDForce.Mutual, target_eps=1e-6):
-    """(MPIDForce, positions [n,3] nm, box lengths [3] nm) for a named workload."""
-    cfg = CONFIGS[name]
-    rng = np.random.default_rng(SEED_BASE + cfg["index"] if seed is None else seed)
-    k = cfg["k"]
-    L = cfg["box"]
-    spacing = L / k
-    # per-type O parameters
+def _o_parameters(rng, octopole, anisotropic):
+    """Per-type O multipoles and polarisability, drawn once per seed."""
     o_dip = list(rng.normal(0.0, 0.005, size=3))
     o_quad = random_traceless_quadrupole(rng, 1e-4)
-    o_oct = random_traceless_octopole(rng, 1e-5) if cfg["octopole"] else [0.0] * 10
-    if cfg["anisotropic"]:
-        o_alpha = list(rng.uniform(0.0008, 0.0012, size=3))
-    else:
-        o_alpha = [0.001] * 3
-    zero3, zero6, zero10 = [0.0] * 3, [0.0] * 6, [0.0] * 10
+    o_oct = random_traceless_octopole(rng, 1e-5) if octopole else [0.0] * 10
+    o_alpha = list(rng.uniform(0.0008, 0.0012, size=3)) if anisotropic else [0.001] * 3
+    return o_dip, o_quad, o_oct, o_alpha
+
+
+def _waters(rng, sites, spacing):
+    """Positions of waters whose oxygens sit at the given lattice sites
+    (jittered by U(-0.02, 0.02) nm), each randomly oriented."""
     half = math.radians(104.52) / 2.0
     roh = 0.09572
     body = np.array([[0.0, 0.0, 0.0],
                      [roh * math.sin(half), 0.0, roh * math.cos(half)],
                      [-roh * math.sin(half), 0.0, roh * math.cos(half)]])
-    nw = k ** 3
-    pos = np.zeros((3 * nw, 3))
+    pos = np.zeros((3 * len(sites), 3))
+    for w, site in enumerate(sites):
+        o = (np.asarray(site, dtype=np.float64) + 0.5) * spacing + rng.uniform(-0.02, 0.02, size=3)
+        R = _random_rotation(rng)
+        pos[3 * w:3 * w + 3] = o + body @ R.T
+    return pos
+
+
+def _water_force(nw, params):
+    o_dip, o_quad, o_oct, o_alpha = params
+    zero3, zero6, zero10 = [0.0] * 3, [0.0] * 6, [0.0] * 10
     force = MPIDForce()
-    w = 0
-    for ix in range(k):
-        for iy in range(k):
-            for iz in range(k):
-                o = (np.array([ix, iy, iz]) + 0.5) * spacing + rng.uniform(-0.02, 0.02, size=3)
-                R = _random_rotation(rng)
-                pos[3 * w:3 * w + 3] = o + body @ R.T
-                w += 1
     for m in range(nw):
         a = 3 * m
         force.addMultipole(-0.8, o_dip, o_quad, o_oct, MPIDForce.Bisector, a + 1, a + 2, -1, 1.3, o_alpha)
@@ -122,11 +119,100 @@ def make_waterbox(name="water216", seed=None, polarization=MPIDForce.Mutual, tar
         force.setCovalentMap(a + 1, MPIDForce.Covalent13, [a + 2])
         force.setCovalentMap(a + 2, MPIDForce.Covalent12, [a])
         force.setCovalentMap(a + 2, MPIDForce.Covalent13, [a + 1])
+    force.setDefaultTholeWidth(5.0)
+    force.setMutualInducedMaxIterations(60)
+    return force
+
+
+def make_waterbox(name="water216", seed=None, polarization=MPIDForce.Mutual, target_eps=1e-6):
+    """(MPIDForce, positions [n,3] nm, box lengths [3] nm) for a named workload."""
+    cfg = CONFIGS[name]
+    rng = np.random.default_rng(SEED_BASE + cfg["index"] if seed is None else seed)
+    k = cfg["k"]
+    L = cfg["box"]
+    params = _o_parameters(rng, cfg["octopole"], cfg["anisotropic"])
+    sites = [(ix, iy, iz) for ix in range(k) for iy in range(k) for iz in range(k)]
+    pos = _waters(rng, sites, L / k)
+    force = _water_force(k ** 3, params)
     force.setNonbondedMethod(MPIDForce.PME)
     force.setCutoffDistance(cfg["cutoff"])
     force.setPMEParameters(ewald_alpha(cfg["cutoff"]), cfg["grid"], cfg["grid"], cfg["grid"])
-    force.setDefaultTholeWidth(5.0)
     force.setPolarizationType(polarization)
     force.setMutualInducedTargetEpsilon(target_eps)
-    force.setMutualInducedMaxIterations(60)
     return force, pos, np.array([L, L, L])
+
+
+WATERS_PER_NM3 = 33.428  # 1.0 g/cm^3
+
+
+def smooth_235(m):
+    """The smallest integer >= m whose prime factors are 2, 3 and 5 only."""
+    m = max(int(m), 1)
+    while True:
+        v = m
+        for p in (2, 3, 5):
+            while v % p == 0:
+                v //= p
+        if v == 1:
+            return m
+        m += 1
+
+
+def lattice_sites(nwaters):
+    """Sites for nwaters oxygens in a unit cube: of the simple, body-centred
+    and face-centred cubic lattices with the fewest cells that hold nwaters
+    sites, the one with the widest nearest-neighbour spacing (k**3, 2 k**3
+    and 4 k**3 sites; spacing 1/k, sqrt(3)/2k and 1/(sqrt(2) k)).  Returns
+    (sites [m,3] in lattice units, spacing of the unit cube in lattice
+    units)."""
+    best = None
+    for basis, nn in (([(0, 0, 0)], 1.0),
+                      ([(0, 0, 0), (0.5, 0.5, 0.5)], math.sqrt(3.0) / 2.0),
+                      ([(0, 0, 0), (0.5, 0.5, 0), (0.5, 0, 0.5), (0, 0.5, 0.5)], 1.0 / math.sqrt(2.0))):
+        k = 1
+        while len(basis) * k ** 3 < nwaters:
+            k += 1
+        if best is None or nn / k > best[0]:
+            sites = [(ix + b[0], iy + b[1], iz + b[2]) for ix in range(k) for iy in range(k) for iz in range(k)
+                     for b in basis]
+            best = (nn / k, sites, k)
+    _, sites, k = best
+    return np.array(sites, dtype=np.float64), k
+
+
+def make_small_waterbox(nwaters, seed, params=None, polarization=MPIDForce.Mutual, target_eps=1e-6):
+    """One system of the batched workload: nwaters waters at 1.0 g/cm^3 in a
+    cubic periodic box of edge (n / 33.428)^(1/3) nm; PME grid spacing about
+    0.08 nm (rounded up to a 2,3,5-smooth size, at least 6); cutoff
+    min(0.8, L/2 - 1e-3) so that it never exceeds half the box; Ewald alpha
+    sqrt(-ln 1e-3) / cutoff.  The oxygens take a seeded choice of nwaters
+    sites of the widest-spaced cubic lattice that holds them (lattice_sites).
+    params: the per-type O parameters (the water4096 set: octopoles,
+    anisotropic alpha), shared by the systems of a batch."""
+    rng = np.random.default_rng(seed)
+    if params is None:
+        params = _o_parameters(rng, True, True)
+    L = (nwaters / WATERS_PER_NM3) ** (1.0 / 3.0)
+    sites, k = lattice_sites(nwaters)
+    chosen = np.sort(rng.choice(len(sites), size=nwaters, replace=False))
+    pos = _waters(rng, sites[chosen], L / k)
+    force = _water_force(nwaters, params)
+    cutoff = min(0.8, 0.5 * L - 1e-3)
+    grid = max(6, smooth_235(math.ceil(L / 0.08)))
+    force.setNonbondedMethod(MPIDForce.PME)
+    force.setCutoffDistance(cutoff)
+    force.setPMEParameters(ewald_alpha(cutoff), grid, grid, grid)
+    force.setPolarizationType(polarization)
+    force.setMutualInducedTargetEpsilon(target_eps)
+    return force, pos, np.array([L, L, L])
+
+
+def make_batch(count=512, seed=SEED_BASE + 3, lo=32, hi=1024, polarization=MPIDForce.Mutual, target_eps=1e-6):
+    """The batched workload: count systems of U(lo, hi) waters (seeded), with
+    one per-type parameter set for the batch.  A list of (force, positions,
+    box)."""
+    rng = np.random.default_rng(seed)
+    params = _o_parameters(rng, True, True)
+    sizes = rng.integers(lo, hi + 1, size=count)
+    return [make_small_waterbox(int(m), seed * 1000 + i, params, polarization, target_eps)
+            for i, m in enumerate(sizes)]
