@@ -47,7 +47,7 @@ class BatchEvaluator:
             if not isinstance(force, MPIDForce):
                 raise TypeError("each system must be (MPIDForce, positions, box)")
             k = HipCalcMPIDForceKernel(precision, conservative_torques)
-            k.initialize(force.getNumMultipoles(), force)
+            k.initialize(force.getNumMultipoles(), force, default_box=box)
             self.kernels.append(k)
             self.positions.append(np.ascontiguousarray(pos, dtype=np.float64))
             self.boxes.append(box)

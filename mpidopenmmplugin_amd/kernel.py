@@ -15,7 +15,7 @@ import ctypes
 import numpy as np
 
 from . import _capi
-from .mpidforce import MPIDForce, MPIDForceError
+from .mpidforce import MPIDForce, MPIDForceError, automatic_pme_parameters
 
 
 class MPIDConvergenceError(MPIDForceError):
@@ -69,10 +69,20 @@ class HipCalcMPIDForceKernel:
         raise RuntimeError(f"libmpid_hip error {rc}: {msg}")
 
     # -- CalcMPIDForceKernel ------------------------------------------------------
-    def initialize(self, system_num_particles, force: MPIDForce):
+    def initialize(self, system_num_particles, force: MPIDForce, default_box=None):
+        """default_box: the System's default periodic box (edges or vectors,
+        nm); needed only by PME with alpha or the grid left at 0, which
+        takes both from the Ewald error tolerance
+        (MPIDCudaKernels.cpp:441-448)."""
         if force.getNumMultipoles() != system_num_particles:
             raise MPIDForceError("MPIDForce must have exactly as many particles as the System it belongs to.")
         flat = force.flatten()
+        if flat["nonbonded_method"] == MPIDForce.PME and (flat["alpha"] == 0.0 or flat["grid"][0] == 0):
+            if default_box is None:
+                raise MPIDForceError("PME with alpha or grid 0 takes them from the Ewald error tolerance and "
+                                     "needs the default box")
+            a, nx, ny, nz = automatic_pme_parameters(flat["cutoff"], force.getEwaldErrorTolerance(), default_box)
+            flat["alpha"], flat["grid"] = a, (nx, ny, nz)
         params, keep = _capi.make_parameters(flat)
         opts = _capi.make_options(flat, _PRECISIONS[self.precision], self.flags)
         ctx = ctypes.c_void_p()
@@ -226,7 +236,7 @@ class MPIDContext:
         self.force = force
         self.box = _box9(box)
         self.kernel = HipCalcMPIDForceKernel(precision, conservative_torques)
-        self.kernel.initialize(force.getNumMultipoles(), force)
+        self.kernel.initialize(force.getNumMultipoles(), force, default_box=box)
         self.positions = None
 
     def setPositions(self, positions):
@@ -259,3 +269,8 @@ class MPIDContext:
 
     def updateParametersInContext(self):
         self.kernel.copyParametersToContext(self.force)
+
+    def getPMEParametersInContext(self):
+        """MPIDForce::getPMEParametersInContext (MPIDForce.cpp:126-128):
+        (alpha, nx, ny, nz) in use, also when taken from the tolerance."""
+        return self.kernel.getPMEParameters()

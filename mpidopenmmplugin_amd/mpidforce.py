@@ -14,7 +14,41 @@ This file holds parameters only; evaluation happens in :mod:`.kernel` on the GPU
 """
 from __future__ import annotations
 
+import math
+
 import numpy as np
+
+
+def legal_fft_size(n):
+    """The smallest size >= n (and >= 6) whose prime factors are 2, 3 and 5:
+    the sizes the gfx950 reciprocal-space FFT transforms."""
+    m = max(int(n), 6)
+    while True:
+        v = m
+        for p in (2, 3, 5):
+            while v % p == 0:
+                v //= p
+        if v == 1:
+            return m
+        m += 1
+
+
+def automatic_pme_parameters(cutoff, ewald_tolerance, box):
+    """Ewald alpha and grid from the error tolerance, as the reference's
+    kernels obtain them when alpha or the grid is 0
+    (MPIDReferenceKernels.cpp:161-170, MPIDCudaKernels.cpp:441-448): OpenMM's
+    NonbondedForceImpl::calcPMEParameters (OpenMM 7/8, not part of the
+    reference) sets alpha = sqrt(-ln(2 tol)) / r_c and
+    n_k = max(6, ceil(2 alpha L_k / (3 tol^(1/5)))) over the System's default
+    box edges L_k.  The CUDA platform then rounds each n_k up to a size its
+    FFT handles; here that is the next 2,3,5-smooth size (legal_fft_size).
+    box: the three edge lengths (nm) or a 3x3 matrix of box vectors."""
+    b = np.asarray(box, dtype=np.float64)
+    edges = np.diag(b.reshape(3, 3)) if b.size == 9 else b.reshape(3)
+    tol = float(ewald_tolerance)
+    alpha = math.sqrt(-math.log(2.0 * tol)) / float(cutoff)
+    grid = [legal_fft_size(max(6, math.ceil(2.0 * alpha * L / (3.0 * tol ** 0.2)))) for L in edges]
+    return alpha, grid[0], grid[1], grid[2]
 
 
 class MPIDForceError(RuntimeError):
