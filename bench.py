@@ -321,8 +321,9 @@ def run(args):
 
 def batch_leg(args):
     """The batched workload: independent periodic water boxes of U(32, 1024)
-    waters, one context each, evaluated together from a pool of host
-    threads; also run one system after another (one thread) for comparison."""
+    waters, one single-stream context each, evaluated together from a pool
+    of host threads; also run one system after another (one thread) for
+    comparison."""
     from mpidopenmmplugin_amd.batch import BatchEvaluator
     from mpidopenmmplugin_amd.waterbox import make_batch
 
@@ -397,7 +398,7 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--batch-systems", type=int, default=512,
                     help="systems in the batched-workload leg (0 skips it)")
-    ap.add_argument("--batch-threads", type=int, default=8)
+    ap.add_argument("--batch-threads", type=int, default=16)
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")

@@ -45,8 +45,12 @@ enum { MPID_PRECISION_MIXED = 0, MPID_PRECISION_DOUBLE = 1 };
  * CONSERVATIVE_TORQUES: add the half of the induced-induced torque about the
  * pair axis that the reference omits (MPIDReferenceForce.cpp:4854-4855).  Off
  * by default, which reproduces the reference; with it the forces on
- * anisotropically polarizable sites are exact gradients of the energy. */
-enum { MPID_FLAG_CONSERVATIVE_TORQUES = 1 };
+ * anisotropically polarizable sites are exact gradients of the energy.
+ * SINGLE_STREAM: issue all of the context's work on one HIP stream instead of
+ * three (real space, reciprocal space, the mixed path's FP64 check).  For many
+ * small contexts driven at once, where cross-stream waits would hold up the
+ * hardware queues the contexts share; results are the same. */
+enum { MPID_FLAG_CONSERVATIVE_TORQUES = 1, MPID_FLAG_SINGLE_STREAM = 2 };
 
 /* Global settings of the force (MPIDForce.h:118-470 setters). */
 typedef struct {

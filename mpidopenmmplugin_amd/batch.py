@@ -12,7 +12,10 @@ kernel-launch latency, so one system after another leaves the chip idle.
 streams, lists, grids and solver graph) per system and drives them from a
 pool of host threads: the C ABI releases the GIL for the whole evaluation,
 so the launches of several systems are in flight at once and their kernels
-share the chip.  Systems are handed out largest first (longest-processing-
+share the chip.  Each context issues its work on a single HIP stream: the
+contexts share the process's few hardware queues, and a context's
+cross-stream waits would hold up the other contexts' kernels queued behind
+them.  Systems are handed out largest first (longest-processing-
 time order) so that the batch does not end on one big system.  Every result
 is exactly the one the system's own context gives (same kernels, same
 fixed-point accumulation), whatever the thread count or order.
@@ -32,10 +35,11 @@ class BatchEvaluator:
 
     systems: a list of (MPIDForce, positions [n,3] nm, box) as the named
     workload generators return them.  threads: host threads issuing work
-    (each blocks while its system's solver runs).
+    (each blocks while its system's solver runs).  single_stream: each
+    context issues its work on one stream (see MPID_FLAG_SINGLE_STREAM).
     """
 
-    def __init__(self, systems, precision="mixed", threads=8, conservative_torques=False):
+    def __init__(self, systems, precision="mixed", threads=16, conservative_torques=False, single_stream=True):
         if threads < 1:
             raise ValueError("threads must be at least 1")
         self.precision = precision
@@ -46,7 +50,7 @@ class BatchEvaluator:
         for force, pos, box in systems:
             if not isinstance(force, MPIDForce):
                 raise TypeError("each system must be (MPIDForce, positions, box)")
-            k = HipCalcMPIDForceKernel(precision, conservative_torques)
+            k = HipCalcMPIDForceKernel(precision, conservative_torques, single_stream)
             k.initialize(force.getNumMultipoles(), force, default_box=box)
             self.kernels.append(k)
             self.positions.append(np.ascontiguousarray(pos, dtype=np.float64))

@@ -288,8 +288,8 @@ struct mpid_context {
         for (hipEvent_t e : ev_check)
             if (e) (void)hipEventDestroy(e);
         if (ev_join) (void)hipEventDestroy(ev_join);
-        if (rstream) (void)hipStreamDestroy(rstream);
-        if (vstream) (void)hipStreamDestroy(vstream);
+        if (rstream && rstream != stream) (void)hipStreamDestroy(rstream);
+        if (vstream && vstream != stream) (void)hipStreamDestroy(vstream);
         if (ev_v) (void)hipEventDestroy(ev_v);
         if (ev_vdone) (void)hipEventDestroy(ev_vdone);
         if (stream) (void)hipStreamDestroy(stream);
@@ -319,15 +319,18 @@ struct mpid_context {
     }
     template <class F> void timed(int cat, F&& f) { timed_on(stream, cat, std::forward<F>(f)); }
     void fork() {
+        if (rstream == stream) return;
         HIP_CHECK(hipEventRecord(ev_fork, stream));
         HIP_CHECK(hipStreamWaitEvent(rstream, ev_fork, 0));
     }
     void join() {
+        if (rstream == stream) return;
         HIP_CHECK(hipEventRecord(ev_join, rstream));
         HIP_CHECK(hipStreamWaitEvent(stream, ev_join, 0));
     }
     // vstream waits for the work issued to stream so far
     void mark_v() {
+        if (vstream == stream) return;
         HIP_CHECK(hipEventRecord(ev_v, stream));
         HIP_CHECK(hipStreamWaitEvent(vstream, ev_v, 0));
     }
@@ -348,6 +351,7 @@ struct mpid_context {
     }
     // stream waits for the work issued to vstream
     void join_v() {
+        if (vstream == stream) return;
         HIP_CHECK(hipEventRecord(ev_vdone, vstream));
         HIP_CHECK(hipStreamWaitEvent(stream, ev_vdone, 0));
     }
@@ -517,7 +521,8 @@ struct mpid_context {
         if (opt.precision != MPID_PRECISION_MIXED && opt.precision != MPID_PRECISION_DOUBLE)
             throw MpidError(MPID_ERR_INVALID, "invalid precision");
         if (opt.max_iterations < 0) throw MpidError(MPID_ERR_INVALID, "invalid iteration limit");
-        if (opt.flags & ~MPID_FLAG_CONSERVATIVE_TORQUES) throw MpidError(MPID_ERR_INVALID, "unknown option flags");
+        if (opt.flags & ~(MPID_FLAG_CONSERVATIVE_TORQUES | MPID_FLAG_SINGLE_STREAM))
+            throw MpidError(MPID_ERR_INVALID, "unknown option flags");
         for (int k = 0; k < 3 && pme(); k++) {
             int rr[16], c;
             FftAxis ax;
@@ -1579,8 +1584,12 @@ int mpid_create(const mpid_parameters* params, const mpid_options* opts, mpid_co
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
             throw MpidError(MPID_ERR_HIP, "no HIP device is available");
         HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-        HIP_CHECK(hipStreamCreateWithFlags(&ctx->rstream, hipStreamNonBlocking));
-        HIP_CHECK(hipStreamCreateWithFlags(&ctx->vstream, hipStreamNonBlocking));
+        if (opts && (opts->flags & MPID_FLAG_SINGLE_STREAM)) {
+            ctx->rstream = ctx->vstream = ctx->stream;
+        } else {
+            HIP_CHECK(hipStreamCreateWithFlags(&ctx->rstream, hipStreamNonBlocking));
+            HIP_CHECK(hipStreamCreateWithFlags(&ctx->vstream, hipStreamNonBlocking));
+        }
         HIP_CHECK(hipEventCreateWithFlags(&ctx->ev_v, hipEventDisableTiming));
         HIP_CHECK(hipEventCreateWithFlags(&ctx->ev_vdone, hipEventDisableTiming));
         HIP_CHECK(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));

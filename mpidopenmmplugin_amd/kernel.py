@@ -43,14 +43,18 @@ class HipCalcMPIDForceKernel:
 
     Name = "CalcMPIDForce"
 
-    def __init__(self, precision="mixed", conservative_torques=False):
+    def __init__(self, precision="mixed", conservative_torques=False, single_stream=False):
         """``conservative_torques`` restores the half of the induced-induced torque
         about the pair axis that the reference omits
-        (MPIDReferenceForce.cpp:4854-4855); the default reproduces the reference."""
+        (MPIDReferenceForce.cpp:4854-4855); the default reproduces the reference.
+        ``single_stream`` issues all work on one HIP stream (for many small
+        contexts evaluated at once)."""
         if precision not in _PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(_PRECISIONS)}")
         self.precision = precision
         self.flags = _capi.MPID_FLAG_CONSERVATIVE_TORQUES if conservative_torques else 0
+        if single_stream:
+            self.flags |= _capi.MPID_FLAG_SINGLE_STREAM
         self._lib = _capi.load()
         self._ctx = ctypes.c_void_p()
         self._n = 0

@@ -306,17 +306,20 @@ def test_list_overflow_retry(water4096, precision):
     assert rel_rms(f1, f2) <= tol_f
 
 
-def test_mixed_refinement_tight_target(water216):
+@pytest.mark.parametrize("single_stream", [False, True], ids=["three_streams", "single_stream"])
+def test_mixed_refinement_tight_target(water216, single_stream):
     """A target beyond FP32 arithmetic (1e-8 Debye) makes the mixed path
     refine: the FP64 residual of the FP32 solution is above the target, so a
     correction is solved and the final pass, issued beside the check, is
     issued again.  The result must then be as close to the FP64 oracle
     (converged to 1e-10) as the target allows, and the first final pass must
-    leave no trace in the outputs (forces accumulate into a buffer of ones)."""
-    print("[gpu] mixed refinement water216", flush=True)
+    leave no trace in the outputs (forces accumulate into a buffer of ones).
+    With single_stream the check runs after the final pass on the same
+    stream instead of beside it."""
+    print(f"[gpu] mixed refinement water216 single_stream={single_stream}", flush=True)
     f, pos, box, ref = water216
     g, _, _ = make_waterbox("water216", target_eps=1e-8)
-    k = HipCalcMPIDForceKernel("mixed")
+    k = HipCalcMPIDForceKernel("mixed", single_stream=single_stream)
     k.initialize(len(pos), g)
     base = np.ones((len(pos), 3))
     e, F = k.execute(pos, box, forces=base.copy())
