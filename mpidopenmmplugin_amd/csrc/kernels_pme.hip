@@ -80,10 +80,32 @@ __device__ __forceinline__ void grid_coord(R x, R L, R inv, int n, int& ig, R& w
 // ---------------------------------------------------------------------------
 // spreading
 // ---------------------------------------------------------------------------
+// Untiled spreading, for grids the tiled kernels do not take (sizes with no
+// power-of-two tiling, or windows too large for LDS).  One wavefront per
+// atom, its 64 lanes over the 6 x 6 x 6 stencil points (216 = 3 x 64 + 24):
+// the adds of one wave go to distinct grid points, and every atom is a wave
+// of its own, so that a small system still spreads over many CUs.  Every
+// lane evaluates the atom's splines and selects its (ix, iy, iz) entries with
+// unrolled compares (no dynamically indexed register arrays).
+template <class R, int ND>
+__device__ __forceinline__ void spline_at(const R (&t)[kOrder][ND + 1], int k, R (&out)[ND + 1]) {
+#pragma unroll
+    for (int d = 0; d <= ND; d++) out[d] = t[0][d];
+#pragma unroll
+    for (int m = 1; m < kOrder; m++)
+        if (k == m) {
+#pragma unroll
+            for (int d = 0; d <= ND; d++) out[d] = t[m][d];
+        }
+}
+
+constexpr int kSpreadWaves = 4;  // atoms (wavefronts) per 256-thread block
+
 template <class R>
 __global__ __launch_bounds__(256) void k_spread_fixed(int n, const R4<R>* __restrict__ pos, const R* __restrict__ lab,
                                                        PmeGeom<R> g, R* __restrict__ grid) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.x * kSpreadWaves + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
     if (i >= n) return;
     const R4<R> p = pos[i];
     int ig[3];
@@ -106,36 +128,39 @@ __global__ __launch_bounds__(256) void k_spread_fixed(int n, const R4<R>* __rest
     const R oxyy = R(3) * gx * gy * gy * m[13], oxyz = R(6) * gx * gy * gz * m[14], oxzz = R(3) * gx * gz * gz * m[15];
     const R oyyy = gy * gy * gy * m[16], oyyz = R(3) * gy * gy * gz * m[17], oyzz = R(3) * gy * gz * gz * m[18];
     const R ozzz = gz * gz * gz * m[19];
-    const int ny = g.n[1], nz = g.n[2];
-    for (int iy = 0; iy < kOrder; iy++) {
-        int y = ig[1] + iy;
+    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    for (int s = lane; s < kOrder * kOrder * kOrder; s += 64) {
+        const int ix = s / (kOrder * kOrder), rem = s - ix * kOrder * kOrder;
+        const int iy = rem / kOrder, iz = rem - iy * kOrder;
+        R a[4], u[4], v[4];
+        spline_at<R, 3>(tx, ix, a);
+        spline_at<R, 3>(ty, iy, u);
+        spline_at<R, 3>(tz, iz, v);
+        const R t0 = q * u[0] * v[0] + dy * u[1] * v[0] + dz * u[0] * v[1] + qyy * u[2] * v[0] + qzz * u[0] * v[2] +
+                     qyz * u[1] * v[1] + oyyy * u[3] * v[0] + oyyz * u[2] * v[1] + oyzz * u[1] * v[2] +
+                     ozzz * u[0] * v[3];
+        const R t1 = dx * u[0] * v[0] + qxy * u[1] * v[0] + qxz * u[0] * v[1] + oxyy * u[2] * v[0] +
+                     oxyz * u[1] * v[1] + oxzz * u[0] * v[2];
+        const R t2 = qxx * u[0] * v[0] + oxxy * u[1] * v[0] + oxxz * u[0] * v[1];
+        const R t3 = oxxx * u[0] * v[0];
+        int x = ig[0] + ix, y = ig[1] + iy, z = ig[2] + iz;
+        if (x >= nx) x -= nx;
         if (y >= ny) y -= ny;
-        const R u0 = ty[iy][0], u1 = ty[iy][1], u2 = ty[iy][2], u3 = ty[iy][3];
-        for (int iz = 0; iz < kOrder; iz++) {
-            int z = ig[2] + iz;
-            if (z >= nz) z -= nz;
-            const R v0 = tz[iz][0], v1 = tz[iz][1], v2 = tz[iz][2], v3 = tz[iz][3];
-            const R t0 = q * u0 * v0 + dy * u1 * v0 + dz * u0 * v1 + qyy * u2 * v0 + qzz * u0 * v2 + qyz * u1 * v1 +
-                         oyyy * u3 * v0 + oyyz * u2 * v1 + oyzz * u1 * v2 + ozzz * u0 * v3;
-            const R t1 = dx * u0 * v0 + qxy * u1 * v0 + qxz * u0 * v1 + oxyy * u2 * v0 + oxyz * u1 * v1 + oxzz * u0 * v2;
-            const R t2 = qxx * u0 * v0 + oxxy * u1 * v0 + oxxz * u0 * v1;
-            const R t3 = oxxx * u0 * v0;
-#pragma unroll
-            for (int ix = 0; ix < kOrder; ix++) {
-                int x = ig[0] + ix;
-                if (x >= g.n[0]) x -= g.n[0];
-                const R v = t0 * tx[ix][0] + t1 * tx[ix][1] + t2 * tx[ix][2] + t3 * tx[ix][3];
-                atomicAdd(&grid[((size_t)x * ny + y) * nz + z], v);
-            }
-        }
+        if (z >= nz) z -= nz;
+        atomicAdd(&grid[((size_t)x * ny + y) * nz + z], t0 * a[0] + t1 * a[1] + t2 * a[2] + t3 * a[3]);
     }
 }
 
 template <class R>
 __global__ __launch_bounds__(256) void k_spread_induced(int n, const R4<R>* __restrict__ pos, const R* __restrict__ mu,
                                                          PmeGeom<R> g, R* __restrict__ grid) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.x * kSpreadWaves + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
     if (i >= n) return;
+    const R ux = g.gscale[0] * mu[3 * (size_t)i], uy = g.gscale[1] * mu[3 * (size_t)i + 1],
+            uz = g.gscale[2] * mu[3 * (size_t)i + 2];
+    // a site without a dipole adds nothing (the whole wave leaves together)
+    if (ux == R(0) && uy == R(0) && uz == R(0)) return;
     const R4<R> p = pos[i];
     int ig[3];
     R w[3];
@@ -146,24 +171,21 @@ __global__ __launch_bounds__(256) void k_spread_induced(int n, const R4<R>* __re
     bspline6<R, 1>(w[0], tx);
     bspline6<R, 1>(w[1], ty);
     bspline6<R, 1>(w[2], tz);
-    const R ux = g.gscale[0] * mu[3 * (size_t)i], uy = g.gscale[1] * mu[3 * (size_t)i + 1],
-            uz = g.gscale[2] * mu[3 * (size_t)i + 2];
-    const int ny = g.n[1], nz = g.n[2];
-    for (int iy = 0; iy < kOrder; iy++) {
-        int y = ig[1] + iy;
+    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    for (int s = lane; s < kOrder * kOrder * kOrder; s += 64) {
+        const int ix = s / (kOrder * kOrder), rem = s - ix * kOrder * kOrder;
+        const int iy = rem / kOrder, iz = rem - iy * kOrder;
+        R a[2], u[2], v[2];
+        spline_at<R, 1>(tx, ix, a);
+        spline_at<R, 1>(ty, iy, u);
+        spline_at<R, 1>(tz, iz, v);
+        const R t0 = uy * u[1] * v[0] + uz * u[0] * v[1];
+        const R t1 = ux * u[0] * v[0];
+        int x = ig[0] + ix, y = ig[1] + iy, z = ig[2] + iz;
+        if (x >= nx) x -= nx;
         if (y >= ny) y -= ny;
-        for (int iz = 0; iz < kOrder; iz++) {
-            int z = ig[2] + iz;
-            if (z >= nz) z -= nz;
-            const R t0 = uy * ty[iy][1] * tz[iz][0] + uz * ty[iy][0] * tz[iz][1];
-            const R t1 = ux * ty[iy][0] * tz[iz][0];
-#pragma unroll
-            for (int ix = 0; ix < kOrder; ix++) {
-                int x = ig[0] + ix;
-                if (x >= g.n[0]) x -= g.n[0];
-                atomicAdd(&grid[((size_t)x * ny + y) * nz + z], t0 * tx[ix][0] + t1 * tx[ix][1]);
-            }
-        }
+        if (z >= nz) z -= nz;
+        atomicAdd(&grid[((size_t)x * ny + y) * nz + z], t0 * a[0] + t1 * a[1]);
     }
 }
 
@@ -728,11 +750,15 @@ __global__ __launch_bounds__(256) void k_gather_field(int natoms, const int* __r
 // ---------------------------------------------------------------------------
 template <class R>
 void launch_spread_fixed(hipStream_t s, int n, const R4<R>* pos, const R* lab, PmeGeom<R> g, R* grid) {
-    hipLaunchKernelGGL(k_spread_fixed<R>, dim3((n + 255) / 256), dim3(256), 0, s, n, pos, lab, g, grid);
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_spread_fixed<R>, dim3((n + kSpreadWaves - 1) / kSpreadWaves), dim3(64 * kSpreadWaves), 0, s,
+                       n, pos, lab, g, grid);
 }
 template <class R>
 void launch_spread_induced(hipStream_t s, int n, const R4<R>* pos, const R* mu, PmeGeom<R> g, R* grid) {
-    hipLaunchKernelGGL(k_spread_induced<R>, dim3((n + 255) / 256), dim3(256), 0, s, n, pos, mu, g, grid);
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_spread_induced<R>, dim3((n + kSpreadWaves - 1) / kSpreadWaves), dim3(64 * kSpreadWaves), 0,
+                       s, n, pos, mu, g, grid);
 }
 void launch_tile_ranges(hipStream_t s, int n, const unsigned* keys_sorted, const int* idx, const int* count,
                         int shift, int* tstart, int* tend) {
