@@ -60,3 +60,57 @@ def assert_equal_vec(expected, found, tol):
         norm = np.sqrt(e @ e)
         diff = np.sqrt((e - f) @ (e - f))
         assert diff / max(norm, 1.0) <= tol, f"vector mismatch {e} vs {f}: {diff / max(norm, 1.0):.3e}"
+
+
+def frames_cluster(seed=5, periodic=False, polarization=MPIDForce.Mutual, target_eps=1e-6, threefold=True):
+    """Eight four-atom molecules whose sites use every local-frame type:
+    the centre ThreeFold (z, x, y = its three neighbours), neighbour 1
+    ZBisect (z = centre, bisector of neighbours 2 and 3), neighbour 2
+    ZThenX (z = centre, x = neighbour 1), neighbour 3 Bisector (centre,
+    neighbour 1) in even molecules and ZOnly / NoAxisType in odd ones.
+    Random dipoles, traceless quadrupoles and octopoles; isotropic
+    polarizabilities.  periodic: PME in a 1.2 nm box (cutoff 0.55 nm,
+    16^3 grid); else NoCutoff.  threefold=False gives the centres ZThenX
+    frames (z, x = neighbours 1, 2) instead.  Returns (force, positions
+    [32,3] nm, box)."""
+    from mpidopenmmplugin_amd.waterbox import (_random_rotation, ewald_alpha, random_traceless_octopole,
+                                               random_traceless_quadrupole)
+    rng = np.random.default_rng(seed)
+    f = MPIDForce()
+    pos = []
+    dirs = np.array([[0.0, 0.0, 1.0], [0.943, 0.0, -0.333], [-0.471, 0.816, -0.333]])
+    for m in range(8):
+        c = 4 * m
+        origin = (np.array([m & 1, (m >> 1) & 1, (m >> 2) & 1]) + 0.5) * 0.6 + rng.uniform(-0.03, 0.03, 3)
+        R = _random_rotation(rng)
+        bonds = 0.1 + rng.uniform(-0.01, 0.01, 3)
+        nb = [origin + bonds[k] * (dirs[k] @ R.T) for k in range(3)]
+        pos.extend([origin] + nb)
+        centre = (MPIDForce.ThreeFold, c + 1, c + 2, c + 3) if threefold else (MPIDForce.ZThenX, c + 1, c + 2, -1)
+        frames = [centre, (MPIDForce.ZBisect, c, c + 2, c + 3),
+                  (MPIDForce.ZThenX, c, c + 1, -1),
+                  (MPIDForce.Bisector, c, c + 1, -1) if m % 2 == 0 else
+                  ((MPIDForce.ZOnly, c, -1, -1) if m % 4 == 1 else (MPIDForce.NoAxisType, -1, -1, -1))]
+        charges = [-0.6, 0.2, 0.2, 0.2]
+        alphas = [0.001, 0.0005, 0.0005, 0.0003]
+        for k, (axis, z, x, y) in enumerate(frames):
+            f.addMultipole(charges[k], list(rng.normal(0.0, 0.005, 3)), random_traceless_quadrupole(rng, 1e-4),
+                           random_traceless_octopole(rng, 1e-5), axis, z, x, y, 1.3, [alphas[k]] * 3)
+        for k in range(4):
+            others = [c + j for j in range(4) if j != k]
+            if k == 0:
+                f.setCovalentMap(c, MPIDForce.Covalent12, others)
+            else:
+                f.setCovalentMap(c + k, MPIDForce.Covalent12, [c])
+                f.setCovalentMap(c + k, MPIDForce.Covalent13, [o for o in others if o != c])
+    f.setPolarizationType(polarization)
+    f.setMutualInducedTargetEpsilon(target_eps)
+    f.setMutualInducedMaxIterations(200)
+    pos = np.array(pos)
+    if periodic:
+        f.setNonbondedMethod(MPIDForce.PME)
+        f.setCutoffDistance(0.55)
+        f.setPMEParameters(ewald_alpha(0.55), 16, 16, 16)
+        return f, pos, np.array([1.2, 1.2, 1.2])
+    f.setNonbondedMethod(MPIDForce.NoCutoff)
+    return f, pos, None
