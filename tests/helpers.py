@@ -66,8 +66,10 @@ def frames_cluster(seed=5, periodic=False, polarization=MPIDForce.Mutual, target
     """Eight four-atom molecules whose sites use every local-frame type:
     the centre ThreeFold (z, x, y = its three neighbours), neighbour 1
     ZBisect (z = centre, bisector of neighbours 2 and 3), neighbour 2
-    ZThenX (z = centre, x = neighbour 1), neighbour 3 Bisector (centre,
-    neighbour 1) in even molecules and ZOnly / NoAxisType in odd ones.
+    ZThenX (z = centre, x = neighbour 1; in even molecules with neighbour 3
+    as the y atom that sets its chirality, and molecules 2 and 6 are mirror
+    images), neighbour 3 Bisector (centre, neighbour 1) in even molecules and
+    ZOnly / NoAxisType in odd ones.
     Random dipoles, traceless quadrupoles and octopoles; isotropic
     polarizabilities.  periodic: PME in a 1.2 nm box (cutoff 0.55 nm,
     16^3 grid); else NoCutoff.  threefold=False gives the centres ZThenX
@@ -84,11 +86,12 @@ def frames_cluster(seed=5, periodic=False, polarization=MPIDForce.Mutual, target
         origin = (np.array([m & 1, (m >> 1) & 1, (m >> 2) & 1]) + 0.5) * 0.6 + rng.uniform(-0.03, 0.03, 3)
         R = _random_rotation(rng)
         bonds = 0.1 + rng.uniform(-0.01, 0.01, 3)
-        nb = [origin + bonds[k] * (dirs[k] @ R.T) for k in range(3)]
+        d = dirs * np.array([-1.0, 1.0, 1.0]) if m % 4 == 2 else dirs  # mirror image: the other handedness
+        nb = [origin + bonds[k] * (d[k] @ R.T) for k in range(3)]
         pos.extend([origin] + nb)
         centre = (MPIDForce.ThreeFold, c + 1, c + 2, c + 3) if threefold else (MPIDForce.ZThenX, c + 1, c + 2, -1)
         frames = [centre, (MPIDForce.ZBisect, c, c + 2, c + 3),
-                  (MPIDForce.ZThenX, c, c + 1, -1),
+                  (MPIDForce.ZThenX, c, c + 1, c + 3 if m % 2 == 0 else -1),
                   (MPIDForce.Bisector, c, c + 1, -1) if m % 2 == 0 else
                   ((MPIDForce.ZOnly, c, -1, -1) if m % 4 == 1 else (MPIDForce.NoAxisType, -1, -1, -1))]
         charges = [-0.6, 0.2, 0.2, 0.2]

@@ -37,15 +37,23 @@ def test_lab_frame_dipoles_of_every_frame(periodic):
     f, pos, box = frames_cluster(periodic=periodic)
     res = oracle.evaluate(f.flatten(), pos, box, details=True)
     lab = res["lab_dipoles"]
-    seen = set()
+    seen, chirality = set(), set()
     for i in range(f.getNumMultipoles()):
         q, d, quad, octo, axis, z, x, y, *_ = f.getMultipoleParameters(i)
         if axis in (MPIDForce.ZBisect, MPIDForce.ThreeFold, MPIDForce.ZThenX, MPIDForce.Bisector):
             ex, ey, ez = frame(axis, pos, i, z, x, y)
-            want = d[0] * ex + d[1] * ey + d[2] * ez
+            sy = 1.0
+            if axis == MPIDForce.ZThenX and y >= 0:
+                # a ZThenX site whose y atom lies on the other side flips its
+                # y component (checkChiralCenterAtParticle, :357-380)
+                volume = np.cross(pos[z] - pos[y], pos[x] - pos[y]) @ (pos[i] - pos[y])
+                sy = -1.0 if volume < 0.0 else 1.0
+                chirality.add(sy)
+            want = d[0] * ex + sy * d[1] * ey + d[2] * ez
             assert np.allclose(lab[i], want, rtol=0, atol=1e-14), (i, axis)
             seen.add(axis)
     assert {MPIDForce.ZBisect, MPIDForce.ThreeFold} <= seen
+    assert chirality == {1.0, -1.0}
 
 
 def _fd_worst(f, pos, box, atoms):
@@ -66,11 +74,12 @@ def _fd_worst(f, pos, box, atoms):
 
 def test_forces_are_energy_gradients():
     """Central differences (h = 1e-5 nm) on atoms of every frame type but
-    ThreeFold (ZBisect, ZThenX, Bisector, ZOnly, NoAxisType, and the frame
-    atoms that take their torques), mutual polarization converged to 1e-10
-    Debye, no cutoff."""
+    ThreeFold (ZBisect, ZThenX with and without a chirality y atom, in both
+    handednesses, Bisector, ZOnly, NoAxisType, and the frame atoms that take
+    their torques), mutual polarization converged to 1e-10 Debye, no
+    cutoff."""
     f, pos, box = frames_cluster(target_eps=1e-10, threefold=False)
-    assert _fd_worst(f, pos, box, (0, 1, 2, 3, 5, 7, 13, 15)) <= 1e-4
+    assert _fd_worst(f, pos, box, (0, 1, 2, 3, 5, 7, 10, 11, 13, 15)) <= 1e-4
 
 
 def test_reference_threefold_torque_is_not_a_gradient():
