@@ -140,7 +140,7 @@ class PDBFile:
                     last_key = key
                 element = line[76:78].strip() if len(line) >= 78 else ""
                 if not element:
-                    element = "".join(c for c in name if c.isalpha())[:1]
+                    element = "".join(c for c in name if c.isalpha())[:1] or "X"
                 element = element[0].upper() + element[1:].lower()
                 a = self.topology.addAtom(name, element, res)
                 serial = line[6:11].strip()
@@ -496,11 +496,20 @@ class ForceField:
         self._types = {}  # name -> (class, element, mass)
         self._templates = {}
         self._generators = []
-        for f in files:
-            self.loadFile(f)
+        self.loadFile(*files)
 
-    def loadFile(self, f):
-        root = ET.parse(f).getroot() if not isinstance(f, ET.Element) else f
+    def loadFile(self, *files):
+        """Reads the atom types and residue templates of every file before
+        any <MPIDForce> section, as openmm.app.ForceField does, so that a
+        force section may name types defined in another file."""
+        roots = [f if isinstance(f, ET.Element) else ET.parse(f).getroot() for f in files]
+        for root in roots:
+            self._load_types(root)
+        for root in roots:
+            for el in root.findall("MPIDForce"):
+                MPIDGenerator.parseElement(el, self)
+
+    def _load_types(self, root):
         for t in root.findall("AtomTypes/Type"):
             self._types[t.get("name")] = (t.get("class"), t.get("element"), float(t.get("mass", "0")))
         for r in root.findall("Residues/Residue"):
@@ -515,8 +524,6 @@ class ForceField:
                 else:
                     tpl.bonds.append((names[b.get("atomName1")], names[b.get("atomName2")]))
             self._templates[tpl.name] = tpl
-        for el in root.findall("MPIDForce"):
-            MPIDGenerator.parseElement(el, self)
 
     def _findAtomTypes(self, attrib):
         """The types an entry names: its ``type``, or every type of its

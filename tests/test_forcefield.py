@@ -171,6 +171,17 @@ def test_multiple_mpid_tags():
         ForceField(types_only, d)
 
 
+def test_force_section_before_its_types():
+    import xml.etree.ElementTree as ET
+    forces = ET.fromstring('<ForceField><MPIDForce><Multipole type="A" c0="-1"/></MPIDForce></ForceField>')
+    types = ET.fromstring('<ForceField><AtomTypes><Type name="A" class="A" element="Na" mass="23"/></AtomTypes>'
+                          '<Residues><Residue name="ION"><Atom name="NA" type="A"/></Residue></Residues></ForceField>')
+    top = Topology()
+    top.addAtom("NA", "Na", top.addResidue("ION"))
+    f = ForceField(forces, types).createSystem(top, NoCutoff, 1.0).getForce(0)
+    assert f.getMultipoleParameters(0)[0] == -1.0 and f.getMultipoleParameters(0)[4] == MPIDForce.NoAxisType
+
+
 def test_conflicting_file_and_argument_warns():
     with pytest.warns(UserWarning, match="coulomb14scale"):
         f = ForceField(ff("ethane_water.xml")).createSystem(PDBFile(ff("ethane_cluster.pdb")).topology, NoCutoff, 1.0,
