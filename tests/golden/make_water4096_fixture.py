@@ -1,8 +1,8 @@
 """Writes tests/golden/water4096_oracle.npz: the FP64 oracle's energy, forces,
 permanent field and induced dipoles for the 12,288-atom water box (octopoles,
 anisotropic polarisability, 64^3 grid), mutual polarisation converged to
-1e-10 Debye.  The GPU box has no reference sources and the oracle needs about a
-minute here, so the result is committed as a fixture.
+1e-10 Debye.  The oracle needs about a minute for it, longer than a GPU test
+should spend, so the result is committed as a fixture.
 
 Run:  python tests/golden/make_water4096_fixture.py
 """
