@@ -327,6 +327,94 @@ __global__ __launch_bounds__(fft_threads<R>()) void k_fft_x_convolve(R2<R>* __re
 }
 
 // ---------------------------------------------------------------------------
+// Small planes (two n0 x n1 buffers and both twiddle tables within 64 KB of
+// LDS): passes 2-4 fused, one workgroup per kz plane.  The plane is loaded as
+// n0 lines along y, transformed, transposed to n1 lines along x, transformed,
+// multiplied by the influence function (as k_fft_x_convolve), transformed
+// back along x, transposed back, transformed back along y and stored: one
+// dispatch instead of three for the grids of small systems.
+template <class R> __host__ __device__ constexpr size_t plane_small_lds(int n0, int n1) {
+    return (2 * (size_t)n0 * n1 + n0 + n1) * sizeof(Cx<R>);
+}
+constexpr size_t kPlaneSmallLds = 65536;
+
+template <class R>
+__global__ __launch_bounds__(fft_threads<R>()) void k_fft_plane_small(R2<R>* __restrict__ S, FftAxis ax0, FftAxis ax1,
+                                                                 const R2<R>* __restrict__ tw0,
+                                                                 const R2<R>* __restrict__ tw1, int n2,
+                                                                 ConvParams<R> cp) {
+    if (skipped(cp.skip)) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int n0 = ax0.n, n1 = ax1.n, np = n0 * n1;
+    Cx<R>* A = reinterpret_cast<Cx<R>*>(smem_raw);
+    Cx<R>* B = A + np;
+    Cx<R>* W0 = B + np;
+    Cx<R>* W1 = W0 + n0;
+    const int kz = blockIdx.x;
+    R2<R>* base = S + (size_t)kz * np;
+    for (int idx = threadIdx.x; idx < np; idx += blockDim.x) {
+        const R2<R> v = base[idx];
+        A[idx] = Cx<R>{v.x, v.y};
+    }
+    stage_twiddles(W0, tw0, n0);
+    stage_twiddles(W1, tw1, n1);
+    __syncthreads();
+    // y forward: n0 lines [x][y]
+    Cx<R>* res = fft_in_lds<R>(A, B, ax1, n0, W1, R(-1));
+    Cx<R>* other = (res == A) ? B : A;
+    // transpose [x][y] -> [y][x]
+    for (int idx = threadIdx.x; idx < np; idx += blockDim.x) {
+        const int x = idx / n1, y = idx - x * n1;
+        other[y * n0 + x] = res[idx];
+    }
+    __syncthreads();
+    // x forward: n1 lines [y][kx]
+    res = fft_in_lds<R>(other, res, ax0, n1, W0, R(-1));
+    // influence function exp(-pi^2 m^2 / alpha^2) / (pi V m^2 |b(m)|^2), m = 0 removed
+    const int mz = kz < (n2 + 1) / 2 ? kz : kz - n2;
+    const R hz = R(mz) * cp.inv[2];
+    const R bz = cp.moduli[n0 + n1 + kz];
+    for (int idx = threadIdx.x; idx < np; idx += blockDim.x) {
+        const int y = idx / n0, kx = idx - y * n0;
+        R et = R(0);
+        if ((kx | y | kz) != 0) {
+            const int mx = kx < (n0 + 1) / 2 ? kx : kx - n0;
+            const int my = y < (n1 + 1) / 2 ? y : y - n1;
+            const R hx = R(mx) * cp.inv[0], hy = R(my) * cp.inv[1];
+            const R m2 = hx * hx + hy * hy + hz * hz;
+            const R den = m2 * cp.moduli[kx] * cp.moduli[n0 + y] * bz;
+            et = cp.sc * exp_r(-cp.expf * m2) / den;
+        }
+        res[idx].x *= et;
+        res[idx].y *= et;
+    }
+    __syncthreads();
+    other = (res == A) ? B : A;
+    // x backward
+    res = fft_in_lds<R>(res, other, ax0, n1, W0, R(1));
+    other = (res == A) ? B : A;
+    // transpose [y][x] -> [x][y]
+    for (int idx = threadIdx.x; idx < np; idx += blockDim.x) {
+        const int y = idx / n0, x = idx - y * n0;
+        other[x * n1 + y] = res[idx];
+    }
+    __syncthreads();
+    // y backward
+    res = fft_in_lds<R>(other, res, ax1, n0, W1, R(1));
+    for (int idx = threadIdx.x; idx < np; idx += blockDim.x) base[idx] = R2<R>{res[idx].x, res[idx].y};
+}
+
+template <class R> static bool plane_small_pass(hipStream_t s, const FftPlan<R>& p, R2<R>* S, int n2h, int n2,
+                                                ConvParams<R> cp) {
+    const int n0 = p.ax[0].n, n1 = p.ax[1].n;
+    const size_t lds = plane_small_lds<R>(n0, n1);
+    if (lds > kPlaneSmallLds) return false;
+    hipLaunchKernelGGL(k_fft_plane_small<R>, dim3(n2h), dim3(fft_threads<R>()), lds, s, S, p.ax[0], p.ax[1], p.tw[0],
+                       p.tw[1], n2, cp);
+    return true;
+}
+
+// ---------------------------------------------------------------------------
 // FP32, n0 = n1 = 128: passes 2-4 fused, one workgroup per kz plane.
 // The 128 x 128 complex plane S[kz][.][.] (128 KB) is held in LDS, and the
 // y transform, the x transform, the influence function and both inverse
@@ -719,7 +807,7 @@ void launch_recip_convolution(hipStream_t s, const FftPlan<R>& p, R* Q, R2<R>* S
     if (!zline_pass<R>(s, p, Q, S, -1, cp.skip))
         hipLaunchKernelGGL(k_fft_z<R>, dim3((nlines + Tz - 1) / Tz), dim3(fft_threads<R>()), fft_lds<R>(Tz, n2), s, Q,
                            S, p.ax[2], Tz, log2i(Tz), p.tw[2], nlines, -1, cp.skip);
-    if (!plane_pass<R>(s, p, S, n2h, n2, cp)) {
+    if (!plane_pass<R>(s, p, S, n2h, n2, cp) && !plane_small_pass<R>(s, p, S, n2h, n2, cp)) {
         // 2. y forward
         hipLaunchKernelGGL(k_fft_y<R>, dim3(n2h * nxb), dim3(fft_threads<R>()), fft_lds<R>(Ty, n1), s, S, p.ax[1], Ty,
                            p.tw[1], n0, nxb, -1, cp.skip);
