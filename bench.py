@@ -249,7 +249,7 @@ def run(args):
         if flops is not None:
             achieved = flops / (avg_ms * 1e-3) / 1e12
             peak = MI355X_PEAK["fp32_TFLOPs"] if mixed else MI355X_PEAK["fp64_TFLOPs"]
-            rooflines[cat] = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+            rooflines[cat] = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                               "frac": achieved / peak, "traffic": None, "avg_launch_ms": avg_ms,
                               "flops_per_launch": flops,
                               "note": "vector ALU arithmetic (no matrix instructions); peak is MI355X's published "
@@ -268,7 +268,7 @@ def run(args):
     e_avg = live_ms / max(1, live_n)
     e_peak = MI355X_PEAK["fp32_TFLOPs"] if mixed else MI355X_PEAK["fp64_TFLOPs"]
     e_ach = e_flops / (e_avg * 1e-3) / 1e12
-    roof = {"bound": "mfma", "achieved": e_ach, "peak": e_peak, "unit": "TFLOP/s", "frac": e_ach / e_peak,
+    roof = {"bound": "valu", "achieved": e_ach, "peak": e_peak, "unit": "TFLOP/s", "frac": e_ach / e_peak,
             "traffic": profiled_traffic(args.workload, args.precision), "kernel": "k_electrostatics_" +
             ("f32" if mixed else "f64"), "avg_launch_ms": e_avg, "launches_timed": live_n,
             "flops_per_launch": e_flops,

@@ -17,8 +17,10 @@ contexts share the process's few hardware queues, and a context's
 cross-stream waits would hold up the other contexts' kernels queued behind
 them.  Systems are handed out largest first (longest-processing-
 time order) so that the batch does not end on one big system.  Every result
-is exactly the one the system's own context gives (same kernels, same
-fixed-point accumulation), whatever the thread count or order.
+agrees with the one the system's own context gives to the rounding of the
+grid atomics (same kernels, same fixed-point accumulation; the untiled PME
+spread adds into the grid with floating-point atomics whose order varies),
+whatever the thread count or order.
 """
 from __future__ import annotations
 
