@@ -13,7 +13,6 @@ constexpr double kDebye = 48.033324;
 constexpr double kSqrtPi = 1.77245385091;
 constexpr int kOrder = 6;            // B-spline order, hard-wired in the reference
 constexpr int kTile = 64;            // atoms per block = one wavefront
-constexpr int kMaxPartners = 16;     // covalent partners with a non-unit scale per atom
 constexpr double kFixed = 4294967296.0;  // 2^32: fixed-point scale of every accumulator
 
 typedef unsigned long long acc_t;

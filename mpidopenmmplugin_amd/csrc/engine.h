@@ -36,8 +36,9 @@ template <class R> struct PairAtoms {
     const R* lab;          // [n*20] lab-frame multipoles
     const R2<R>* td;       // thole, damping factor
     const int* aniso;      // [n]
-    const int* partners;   // [n*kMaxPartners] (sorted index << 2 | class)
+    const int* partners;   // [n*pstride] (sorted index << 2 | class)
     const int* npart;      // [n]
+    int pstride;           // partner slots per atom: the largest partner count of the force
 };
 
 constexpr int kNlSplit = 4;  // threads per atom in the list-driven kernels
@@ -55,8 +56,8 @@ void launch_morton_keys(hipStream_t s, int n, const double* pos, Box<double> box
 void launch_invert_perm(hipStream_t s, int n, const int* perm, int* inv);
 void launch_gather_sorted(hipStream_t s, int n, const int* perm, const int* inv, const double* pos, Box<double> box,
                           const double* mol, const int4* axes, const double* thole, const double* pol,
-                          const int* partners, const int* npart, double4* spos, double* smol, int4* saxes,
-                          double* sthole, double* spol, int* spartners, int* snpart);
+                          const int* partners, const int* npart, int pstride, double4* spos, double* smol,
+                          int4* saxes, double* sthole, double* spol, int* spartners, int* snpart);
 // chg[s] = 1: the site carries a charge only (no higher moment, no polarisability)
 void launch_rotate(hipStream_t s, int n, Box<double> box, const double4* pos, const int4* axes, const double* mol,
                    const double* pol, double* lab, double* labpol, double* sqrtpol, double* damp, int* aniso,
@@ -276,7 +277,7 @@ void launch_exclusive_scan(hipStream_t s, int m, const int* counts, int* start);
 // rcap: the capacity of rows / nmask (entries beyond it are ignored).
 void launch_nl_build(hipStream_t s, int n, int nrows, int rcap, bool all, const int* rstart, const int* rows,
                      const int* owner, const float4* bmin, const float4* bmax, const double4* pos,
-                     const int* partners, const int* npart, Box<double> box, double rc2, int cap,
+                     const int* partners, const int* npart, int pstride, Box<double> box, double rc2, int cap,
                      unsigned long long* nmask, unsigned long long* polmask, int* polcnt, int* polstart, int* nbr,
                      int* count, int* scalars, unsigned long long* tmask, int* pnbr, int* pcount, const int* chg,
                      unsigned long long* tmaskcc);

@@ -63,8 +63,8 @@ __global__ void k_gather_sorted(int n, const int* __restrict__ perm, const int* 
                                 const double* __restrict__ pos, Box<double> box, const double* __restrict__ mol,
                                 const int4* __restrict__ axes, const double* __restrict__ thole,
                                 const double* __restrict__ pol, const int* __restrict__ partners,
-                                const int* __restrict__ npart, double4* spos, double* smol, int4* saxes,
-                                double* sthole, double* spol, int* spartners, int* snpart) {
+                                const int* __restrict__ npart, int pstride, double4* spos, double* smol,
+                                int4* saxes, double* sthole, double* spol, int* spartners, int* snpart) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const int a = perm[s];
@@ -84,8 +84,8 @@ __global__ void k_gather_sorted(int n, const int* __restrict__ perm, const int* 
     const int np = npart[a];
     snpart[s] = np;
     for (int k = 0; k < np; k++) {
-        int e = partners[(size_t)a * kMaxPartners + k];
-        spartners[(size_t)s * kMaxPartners + k] = (inv[e >> 2] << 2) | (e & 3);
+        int e = partners[(size_t)a * pstride + k];
+        spartners[(size_t)s * pstride + k] = (inv[e >> 2] << 2) | (e & 3);
     }
 }
 
@@ -1060,10 +1060,10 @@ void launch_invert_perm(hipStream_t s, int n, const int* perm, int* inv) {
 }
 void launch_gather_sorted(hipStream_t s, int n, const int* perm, const int* inv, const double* pos, Box<double> box,
                           const double* mol, const int4* axes, const double* thole, const double* pol,
-                          const int* partners, const int* npart, double4* spos, double* smol, int4* saxes,
-                          double* sthole, double* spol, int* spartners, int* snpart) {
+                          const int* partners, const int* npart, int pstride, double4* spos, double* smol,
+                          int4* saxes, double* sthole, double* spol, int* spartners, int* snpart) {
     hipLaunchKernelGGL(k_gather_sorted, blocks_for(n, 128), dim3(128), 0, s, n, perm, inv, pos, box, mol, axes, thole,
-                       pol, partners, npart, spos, smol, saxes, sthole, spol, spartners, snpart);
+                       pol, partners, npart, pstride, spos, smol, saxes, sthole, spol, spartners, snpart);
 }
 void launch_rotate(hipStream_t s, int n, Box<double> box, const double4* pos, const int4* axes, const double* mol,
                    const double* pol, double* lab, double* labpol, double* sqrtpol, double* damp, int* aniso,

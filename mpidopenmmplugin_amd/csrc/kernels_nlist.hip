@@ -123,7 +123,8 @@ __global__ __launch_bounds__(64) void k_nl_masks(int n, const int* __restrict__ 
                                                  const int* __restrict__ owner, const float4* __restrict__ bmin,
                                                  const float4* __restrict__ bmax, const double4* __restrict__ pos,
                                                  const int* __restrict__ partners, const int* __restrict__ npart,
-                                                 Box<double> box, double rc2, const int* __restrict__ chg,
+                                                 int pstride, Box<double> box, double rc2,
+                                                 const int* __restrict__ chg,
                                                  unsigned long long* __restrict__ nmask,
                                                  unsigned long long* __restrict__ tmask,
                                                  unsigned long long* __restrict__ tmaskcc,
@@ -164,7 +165,7 @@ __global__ __launch_bounds__(64) void k_nl_masks(int n, const int* __restrict__ 
     unsigned long long cmask = __ballot(cand);
     unsigned long long skip = J == I ? 1ull << lane : 0ull;
     const int np = iv ? npart[ic] : 0;
-    const int* parts = partners + (size_t)ic * kMaxPartners;
+    const int* parts = partners + (size_t)ic * pstride;
     for (int k = 0; k < np; k++) {
         const int p = parts[k] >> 2;
         if ((p >> 6) == J) skip |= 1ull << (p & 63);
@@ -461,7 +462,7 @@ void launch_exclusive_scan(hipStream_t s, int m, const int* counts, int* start) 
 }
 void launch_nl_build(hipStream_t s, int n, int nrows, int rcap, bool all, const int* rstart, const int* rows,
                      const int* owner, const float4* bmin, const float4* bmax, const double4* pos,
-                     const int* partners, const int* npart, Box<double> box, double rc2, int cap,
+                     const int* partners, const int* npart, int pstride, Box<double> box, double rc2, int cap,
                      unsigned long long* nmask, unsigned long long* polmask, int* polcnt, int* polstart, int* nbr,
                      int* count, int* scalars, unsigned long long* tmask, int* pnbr, int* pcount, const int* chg,
                      unsigned long long* tmaskcc) {
@@ -469,7 +470,7 @@ void launch_nl_build(hipStream_t s, int n, int nrows, int rcap, bool all, const 
     hipLaunchKernelGGL(k_pol_mask, dim3(nb), dim3(kTile), 0, s, n, pos, polmask, polcnt);
     if (nrows > 0)
         hipLaunchKernelGGL(k_nl_masks, dim3(nrows), dim3(kTile), 0, s, n, rstart, rows, owner, bmin, bmax, pos,
-                           partners, npart, box, rc2, chg, nmask, tmask, tmaskcc, scalars + 1, rcap);
+                           partners, npart, pstride, box, rc2, chg, nmask, tmask, tmaskcc, scalars + 1, rcap);
     launch_exclusive_scan(s, nb, polcnt, polstart);
     hipLaunchKernelGGL(k_nl_fill, dim3(nb), dim3(kFillWaves * kTile), 0, s, n, all ? 1 : 0, rstart, rows, polmask,
                        polstart, nmask, cap, nbr, count, scalars, pnbr, pcount, rcap);

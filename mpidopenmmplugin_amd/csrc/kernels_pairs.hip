@@ -347,7 +347,7 @@ __global__ __launch_bounds__(256) void k_cov_fixed_field(PairAtoms<R> A, PairPar
     const R2<R> tdi = A.td[i];
     R ex = 0, ey = 0, ez = 0;
     for (int k = 0; k < np; k++) {
-        const int e = A.partners[(size_t)i * kMaxPartners + k];
+        const int e = A.partners[(size_t)i * A.pstride + k];
         const int j = e >> 2;
         if (j <= i) continue;
         const R4<R> pj = A.pos[j];
@@ -388,7 +388,7 @@ __global__ __launch_bounds__(256) void k_cov_induced_field(PairAtoms<R> A, const
     R ex = 0, ey = 0, ez = 0;
     R gi[6] = {0, 0, 0, 0, 0, 0};
     for (int k = 0; k < np; k++) {
-        const int e = A.partners[(size_t)i * kMaxPartners + k];
+        const int e = A.partners[(size_t)i * A.pstride + k];
         const int j = e >> 2;
         if (j <= i) continue;
         if (pol_only && A.td[j].y == R(0)) continue;
@@ -446,7 +446,7 @@ __global__ __launch_bounds__(128) void k_cov_electrostatics(PairAtoms<R> A, cons
         R fx = 0, fy = 0, fz = 0, tx = 0, ty = 0, tz = 0;
         bool any = false;
         for (int k = 0; k < np; k++) {
-            const int e = A.partners[(size_t)i * kMaxPartners + k];
+            const int e = A.partners[(size_t)i * A.pstride + k];
             const int j = e >> 2;
             if (j <= i) continue;
             const R4<R> pj = A.pos[j];

@@ -203,6 +203,7 @@ struct mpid_context {
     DBuf<double> mol, thole, pol;
     DBuf<int4> axes;
     DBuf<int> partners, npart;
+    int pstride = 1;  // partner slots per atom: the largest scaled-partner count
     std::vector<double> h_charges;
 
     // sorted per-evaluation state
@@ -431,6 +432,10 @@ struct mpid_context {
         // lists of i, later list types overwrite earlier ones; only classes
         // with a scale other than 1 are stored, for both atoms of the pair
         // (setupScaleMaps / getMultipoleScaleFactor, MPIDReferenceForce.cpp:190-241).
+        // A 1-4 pair at scale14 = 1 is an ordinary pair (its Thole width is
+        // the default one, as for every pair with a nonzero polarization
+        // scale), so it is not stored either.  There is no limit on the
+        // number of partners: the slots per atom follow the largest count.
         std::vector<std::vector<std::pair<int, int>>> plist(n);
         for (int a = 0; a < n; a++) {
             std::vector<std::pair<int, int>> mine;  // (partner, class) with partner > a
@@ -448,20 +453,21 @@ struct mpid_context {
             for (auto& e : mine) {
                 int cls;
                 if (e.second <= 1) cls = kScaleZero;            // 1-2, 1-3: scale 0
-                else if (e.second == 2) cls = kScale14;         // 1-4: scale14
-                else continue;                                  // 1-5: scale 1
+                else if (e.second == 2 && opt.scale14 != 1.0) cls = kScale14;  // 1-4: scale14
+                else continue;                                  // 1-5 (and 1-4 at scale14 = 1): scale 1
                 plist[a].push_back({e.first, cls});
                 plist[e.first].push_back({a, cls});
             }
         }
-        std::vector<int> part((size_t)n * kMaxPartners, 0), np(n);
+        int stride = 1;
+        for (int a = 0; a < n; a++) stride = std::max(stride, (int)plist[a].size());
+        std::vector<int> part((size_t)n * stride, 0), np(n);
         for (int a = 0; a < n; a++) {
-            if ((int)plist[a].size() > kMaxPartners)
-                throw MpidError(MPID_ERR_UNSUPPORTED, "MPIDForce: more than " + std::to_string(kMaxPartners) +
-                                                          " scaled covalent partners on particle " + std::to_string(a));
             np[a] = (int)plist[a].size();
-            for (int k = 0; k < np[a]; k++) part[(size_t)a * kMaxPartners + k] = (plist[a][k].first << 2) | plist[a][k].second;
+            for (int k = 0; k < np[a]; k++) part[(size_t)a * stride + k] = (plist[a][k].first << 2) | plist[a][k].second;
         }
+        pstride = stride;
+        spart.resize((size_t)n * pstride);
         std::vector<double> th(P->tholes, P->tholes + n), po(P->polarity, P->polarity + 3 * n);
         mol.upload(m.data(), m.size(), stream);
         axes.upload(ax.data(), ax.size(), stream);
@@ -592,7 +598,6 @@ struct mpid_context {
         saxes.resize(n);
         sthole.resize(n);
         spol.resize(n3);
-        spart.resize((size_t)n * kMaxPartners);
         snpart.resize(n);
         lab.resize((size_t)n * kMP);
         labpol.resize(6 * (size_t)n);
@@ -744,6 +749,7 @@ struct mpid_context {
         A.aniso = aniso.p;
         A.partners = spart.p;
         A.npart = snpart.p;
+        A.pstride = pstride;
         return A;
     }
     double selfc() const { return (4.0 / 3.0) * opt.ewald_alpha * opt.ewald_alpha * opt.ewald_alpha / kSqrtPi; }
@@ -765,7 +771,7 @@ struct mpid_context {
             if (rc != 0) throw HipError("rocprim radix sort failed");
             launch_invert_perm(stream, n, perm.p, inv.p);
             launch_gather_sorted(stream, n, perm.p, inv.p, d_pos, dbox, mol.p, axes.p, thole.p, pol.p, partners.p,
-                                 npart.p, spos.p, smol.p, saxes.p, sthole.p, spol.p, spart.p, snpart.p);
+                                 npart.p, pstride, spos.p, smol.p, saxes.p, sthole.p, spol.p, spart.p, snpart.p);
         });
         if (tiled_spread) {
             timed(P_SORT, [&] {
@@ -849,7 +855,7 @@ struct mpid_context {
                 HIP_CHECK(hipMemsetAsync(nl_scalars.p, 0, 2 * sizeof(int), stream));
                 launch_nl_build(stream, n, rows_pending ? (int)brows.n : rows_total, (int)brows.n, all_lists,
                                 brow_start.p, brows.p, brow_owner.p, bmin.p, bmax.p,
-                                pd.pos.p, spart.p, snpart.p, dbox, opt.cutoff * opt.cutoff, nl_cap, nmask.p,
+                                pd.pos.p, spart.p, snpart.p, pstride, dbox, opt.cutoff * opt.cutoff, nl_cap, nmask.p,
                                 polmask.p, polcnt.p, polstart.p, nbr.p, ncount.p, nl_scalars.p, tmask.p, pnbr.p,
                                 pcount.p, chg.p, tmaskcc.p);
                 launch_pol_fill(stream, n, pd.pos.p, polstart.p, pol_idx.p);
@@ -1424,7 +1430,16 @@ struct mpid_context {
         return e;
     }
 
+    // Every path that rewrites the device state forgets the last evaluation
+    // first: a getter must not serve a cached result that a failed
+    // evaluation has partly overwritten (positions, permutation, dipoles).
+    void invalidate_last() {
+        have_last = false;
+        mu_valid = false;
+    }
+
     double execute_device(const double* d_pos, const double* boxv, double* d_forces) {
+        invalidate_last();
         set_box(boxv);
         double e = opt.precision == MPID_PRECISION_MIXED ? run(pf, d_pos, d_forces) : run(pd, d_pos, d_forces);
         return e;
@@ -1514,6 +1529,7 @@ struct mpid_context {
     }
 
     void compute_fixed_field(const double* pos, const double* boxv, double* out) {
+        invalidate_last();
         set_box(boxv);
         pos_in.upload(pos, 3 * (size_t)n, stream);
         prepare(pos_in.p, true);
@@ -1523,6 +1539,7 @@ struct mpid_context {
         collect_profile();
     }
     void compute_induced_field(const double* pos, const double* boxv, const double* mu, double* out) {
+        invalidate_last();
         set_box(boxv);
         pos_in.upload(pos, 3 * (size_t)n, stream);
         prepare(pos_in.p, true, false);

@@ -117,3 +117,77 @@ def frames_cluster(seed=5, periodic=False, polarization=MPIDForce.Mutual, target
         return f, pos, np.array([1.2, 1.2, 1.2])
     f.setNonbondedMethod(MPIDForce.NoCutoff)
     return f, pos, None
+
+
+def make_branched_cluster(seed=5, scale14=0.5, layers=(6, 3, 2), spacing=0.3):
+    """A tree-shaped molecule whose centre has more scaled covalent partners
+    than any fixed per-atom slot count would hold: the centre has layers[0]
+    bonded neighbours, each of those layers[1] children and each child
+    layers[2] grandchildren (6 + 18 + 36 = 60 partners at 1-2, 1-3 and 1-4
+    with the defaults).  The atoms sit on a jittered cubic lattice (the
+    covalent maps are just lists, so bond lengths do not matter here).
+    Returns (MPIDForce, positions, centre-partner count)."""
+    from mpidopenmmplugin_amd.waterbox import random_traceless_octopole, random_traceless_quadrupole
+    rng = np.random.default_rng(seed)
+    parent = [-1]
+    frontier = [0]
+    for width in layers:
+        nxt = []
+        for p in frontier:
+            for _ in range(width):
+                parent.append(p)
+                nxt.append(len(parent) - 1)
+        frontier = nxt
+    n = len(parent)
+    adj = [[] for _ in range(n)]
+    for a, p in enumerate(parent):
+        if p >= 0:
+            adj[a].append(p)
+            adj[p].append(a)
+    k = int(np.ceil(n ** (1.0 / 3.0)))
+    sites = np.array([(x, y, z) for x in range(k) for y in range(k) for z in range(k)], dtype=np.float64)
+    order = rng.permutation(len(sites))[:n]
+    pos = sites[order] * spacing + rng.uniform(-0.02, 0.02, size=(n, 3)) + 0.5
+    f = MPIDForce()
+    for a in range(n):
+        nb = adj[a]
+        dip = list(rng.normal(0.0, 0.005, size=3))
+        quad = random_traceless_quadrupole(rng, 1e-4)
+        octo = random_traceless_octopole(rng, 1e-5)
+        alpha = list(rng.uniform(0.0008, 0.0012, size=3)) if a % 3 else [0.001] * 3
+        if len(nb) >= 2:
+            f.addMultipole(rng.normal(0.0, 0.3), dip, quad, octo, MPIDForce.ZThenX, nb[0], nb[1], -1, 1.3, alpha)
+        else:
+            sib = [b for b in adj[nb[0]] if b != a][0]
+            f.addMultipole(rng.normal(0.0, 0.3), dip, quad, octo, MPIDForce.Bisector, nb[0], sib, -1, 1.3, alpha)
+    # 1-2, 1-3, 1-4 sets by breadth-first distance
+    for a in range(n):
+        dist = {a: 0}
+        q = [a]
+        while q:
+            b = q.pop(0)
+            if dist[b] == 3:
+                continue
+            for c in adj[b]:
+                if c not in dist:
+                    dist[c] = dist[b] + 1
+                    q.append(c)
+        for d, kind in ((1, MPIDForce.Covalent12), (2, MPIDForce.Covalent13), (3, MPIDForce.Covalent14)):
+            lst = sorted(b for b, v in dist.items() if v == d)
+            if lst:
+                f.setCovalentMap(a, kind, lst)
+    f.set14ScaleFactor(scale14)
+    npartners = sum(1 for v in _bfs_counts(adj, 0) if 1 <= v <= 3)
+    return f, pos, npartners
+
+
+def _bfs_counts(adj, a):
+    dist = {a: 0}
+    q = [a]
+    while q:
+        b = q.pop(0)
+        for c in adj[b]:
+            if c not in dist:
+                dist[c] = dist[b] + 1
+                q.append(c)
+    return list(dist.values())

@@ -148,6 +148,42 @@ def test_nonconvergence_is_an_error(water216):
         ctx.getState()
 
 
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_getters_after_failed_evaluation(water216, precision):
+    """A failed evaluation must not leave a getter serving the last good
+    one's cache: after execute(A) succeeds and execute(B) fails to converge
+    (it has rewritten the sorted positions, permutation and dipoles), the
+    induced dipoles at A equal those of a fresh context."""
+    print(f"[gpu] getters after a failed evaluation {precision}", flush=True)
+    f, pos, box, _ = water216
+    rng = np.random.default_rng(11)
+    # two configurations whose solves take different iteration counts
+    configs = [pos] + [pos + rng.normal(scale=s, size=pos.shape) for s in (0.004, 0.008, 0.012, 0.016, 0.02)]
+    probe = MPIDContext(f, box, precision=precision)
+    its = []
+    for c in configs:
+        probe.setPositions(c)
+        probe.getState()
+        its.append(probe.kernel.stats()["iterations"])
+    a, b = int(np.argmin(its)), int(np.argmax(its))
+    assert its[b] > its[a], its
+    fresh = MPIDContext(f, box, precision=precision)
+    fresh.setPositions(configs[a])
+    fresh.getState()
+    want = fresh.getInducedDipoles()
+    g, _, _ = make_waterbox("water216")
+    g.setMutualInducedMaxIterations(its[a])
+    ctx = MPIDContext(g, box, precision=precision)
+    ctx.setPositions(configs[a])
+    ctx.getState()
+    ctx.setPositions(configs[b])
+    with pytest.raises(MPIDConvergenceError):
+        ctx.getState()
+    ctx.setPositions(configs[a])
+    got = ctx.getInducedDipoles()
+    assert rel_rms(got, want) <= (1e-12 if precision == "double" else 1e-6)
+
+
 def test_invalid_parameters_raise():
     f, pos, box = make_waterbox("water216")
     f.setCutoffDistance(1.0)  # more than half of the 1.862 nm box
