@@ -1656,6 +1656,24 @@ int oracle_fft3d(double* data, int nx, int ny, int nz, int sign) {
     return 0;
 }
 
+// mapTorqueToForce (:2112-2131) alone: the forces that the frame chain rule
+// gives for the given per-site torques at the given positions (overwritten).
+int oracle_torque_to_force(const oracle_system* sys, const double* pos, const double* torques, double* forces) {
+    try {
+        Oracle o(*sys);
+        const int n = sys->n;
+        o.site.resize(n);
+        for (int i = 0; i < n; i++) o.site[i].pos = V3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+        std::vector<V3> F(n);
+        for (int i = 0; i < n; i++) o.torque_to_force(i, V3(torques[3 * i], torques[3 * i + 1], torques[3 * i + 2]), F);
+        for (int i = 0; i < n; i++)
+            for (int k = 0; k < 3; k++) forces[3 * i + k] = F[i][k];
+        return 0;
+    } catch (const std::exception&) {
+        return 1;
+    }
+}
+
 static void setup(Oracle& o, const double* pos) {
     o.validate();
     o.set_box();

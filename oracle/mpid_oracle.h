@@ -84,6 +84,10 @@ int oracle_fft3d(double* data_interleaved, int nx, int ny, int nz, int sign);
 /* Induced field T*mu (real + reciprocal + self) for given lab dipoles, no
  * solver: the matrix-vector product of the mutual solver. */
 int oracle_induced_field(const oracle_system* sys, const double* pos, const double* mu, double* field_out);
+/* mapTorqueToForce (MPIDReferenceForce.cpp:1895-2131) alone: forces [3n]
+ * (overwritten) from per-site torques [3n] at positions [3n]; only n, the
+ * axis types and the frame atoms of sys are read. */
+int oracle_torque_to_force(const oracle_system* sys, const double* pos, const double* torques, double* forces);
 /* getElectrostaticPotential after the solve (MPIDReferenceForce.cpp:2464-2531):
  * at each of the npts points, the Coulomb constant times the bare potential
  * of every particle's charge, dipole plus induced dipole and quadrupole

@@ -78,6 +78,8 @@ def lib():
         L.oracle_bspline.argtypes = [ctypes.c_double, _dp]
         L.oracle_bspline_moduli.argtypes = [ctypes.c_int, _dp]
         L.oracle_fft3d.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.oracle_torque_to_force.argtypes = [ctypes.POINTER(_System), _dp, _dp, _dp]
+        L.oracle_torque_to_force.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -214,3 +216,17 @@ def system_multipole_moments(flat, positions, box, masses):
     if rc != 0:
         raise OracleError("oracle_multipole_moments failed (%d)" % rc)
     return out
+
+
+def torque_to_force(flat, positions, torques):
+    """mapTorqueToForce alone: the forces [n,3] that the frame chain rule
+    gives for per-site torques [n,3] at the given positions."""
+    n = int(flat["n"])
+    s, keep = _system(flat, None)
+    pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1)
+    t = np.ascontiguousarray(torques, dtype=np.float64).reshape(-1)
+    out = np.zeros(3 * n)
+    rc = lib().oracle_torque_to_force(ctypes.byref(s), _ptr(pos, "d"), _ptr(t, "d"), _ptr(out, "d"))
+    if rc != 0:
+        raise OracleError("oracle_torque_to_force failed (%d)" % rc)
+    return out.reshape(n, 3)

@@ -206,6 +206,11 @@ def main():
                       forces=[[-139.8529084, -184.5568497, 35.69566243], [37.47391144, -5.507167568, 5.113662284],
                               [41.24807762, 119.4349691, 31.61668322], [-117.0172298, -101.5532625, -27.86681578],
                               [127.8817331, 167.5584812, -19.27180201], [50.26641605, 4.623829403, -25.28739015]]))
+    # 1-4 scaling without cutoff (TestReferenceMPIDForce.cpp:1603-1646): closed form, tolerance 1e-5
+    for s14, e in ((1.0, -1389.35), (0.5, -1389.35 / 2), (0.0, 0.0)):
+        cases.append(dict(name=f"charge_square_nocut_s14_{s14}", system="square", method=0, alpha=0.0, grid=0,
+                          cutoff=1.0, box=2.0, default_thole=5.0, polarization=1, target_eps=1e-5,
+                          scale14=s14, energy=e, energy_tol=1e-5, source="TestReferenceMPIDForce.cpp:1603-1646"))
     # 1-4 scaling with PME (TestReferenceMPIDForce.cpp:1648-1700): closed form, tolerance 1e-3
     for s14, e in ((1.0, -1389.35), (0.5, -1389.35 / 2), (0.0, 0.0)):
         cases.append(dict(name=f"charge_square_pme_s14_{s14}", system="square", method=1, alpha=0.001, grid=64,

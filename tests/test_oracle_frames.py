@@ -91,3 +91,85 @@ def test_reference_threefold_torque_is_not_a_gradient():
     ThreeFold sites)."""
     f, pos, box = frames_cluster(target_eps=1e-10, polarization=MPIDForce.Direct)
     assert _fd_worst(f, pos, box, (0,)) > 1e-2
+
+
+# ---------------------------------------------------------------------------
+# The ThreeFold torque mapping (mapTorqueToForceForParticle,
+# MPIDReferenceForce.cpp:2068-2097) is an approximate projection whose forces
+# are not the energy gradient, so finite differences cannot pin it.  These
+# checks pin it by properties that the formula has and that a dropped term, a
+# wrong sign, a wrong bond length or a wrong prefactor would break.
+# ---------------------------------------------------------------------------
+def _threefold(pos_i, legs):
+    """A ThreeFold site 0 whose z, x and y atoms sit at pos_i + legs[k]."""
+    f = MPIDForce()
+    z3, z6, z10 = [0.0] * 3, [0.0] * 6, [0.0] * 10
+    f.addMultipole(0.0, z3, z6, z10, MPIDForce.ThreeFold, 1, 2, 3, 0.0, z3)
+    for _ in range(3):
+        f.addMultipole(0.0, z3, z6, z10, MPIDForce.NoAxisType, -1, -1, -1, 0.0, z3)
+    pos = np.vstack([pos_i] + [pos_i + np.asarray(l, dtype=np.float64) for l in legs])
+    return f.flatten(), pos
+
+
+def _map(pos_i, legs, tau):
+    flat, pos = _threefold(pos_i, legs)
+    t = np.zeros((4, 3))
+    t[0] = tau
+    return oracle.torque_to_force(flat, pos, t), pos
+
+
+def test_threefold_forces_are_perpendicular_and_balanced():
+    """Each frame atom's force is perpendicular to its bond vector, and the
+    four forces sum to zero, for general (non-symmetric) geometries."""
+    rng = np.random.default_rng(17)
+    for _ in range(20):
+        pos_i = rng.normal(size=3)
+        legs = rng.normal(size=(3, 3)) * rng.uniform(0.5, 2.0, size=(3, 1))
+        F, pos = _map(pos_i, legs, rng.normal(size=3))
+        for k in range(3):
+            assert abs(F[1 + k] @ legs[k]) <= 1e-12 * np.linalg.norm(F[1 + k]) * np.linalg.norm(legs[k]) + 1e-15
+        assert np.abs(F.sum(axis=0)).max() <= 1e-12 * np.abs(F).max()
+
+
+def test_threefold_orthogonal_tripod_closed_form():
+    """Bonds along x, y and z with lengths a, b, c: the formula reduces by
+    hand (every sine is 1, the cross products are unit axes) to
+    F_z-atom = -(y (tx - tz) + z (ty - tx)) / 3a,
+    F_x-atom = -(x (tz - ty) + z (ty - tx)) / 3b,
+    F_y-atom = -(x (tz - ty) + y (tx - tz)) / 3c,
+    and the centre takes minus their sum."""
+    a, b, c = 0.7, 1.3, 2.1
+    tau = np.array([0.37, -1.1, 0.52])
+    tx, ty, tz = tau
+    F, _ = _map(np.zeros(3), [[a, 0, 0], [0, b, 0], [0, 0, c]], tau)
+    want_u = -np.array([0.0, tx - tz, ty - tx]) / (3 * a)
+    want_v = -np.array([tz - ty, 0.0, ty - tx]) / (3 * b)
+    want_w = -np.array([tz - ty, tx - tz, 0.0]) / (3 * c)
+    assert np.allclose(F[1], want_u, rtol=0, atol=1e-14)
+    assert np.allclose(F[2], want_v, rtol=0, atol=1e-14)
+    assert np.allclose(F[3], want_w, rtol=0, atol=1e-14)
+    assert np.allclose(F[0], -(want_u + want_v + want_w), rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("tilt", [-0.6, -0.33, 0.0, 0.4])
+def test_threefold_symmetric_tripod_torque(tilt):
+    """On a symmetric tripod (three equal legs at 120 degrees about an axis)
+    the mapped forces exert exactly the part of the torque perpendicular to
+    the three-fold axis, and a torque along that axis maps to no force at
+    all: rotating the site about its own symmetry axis is what the
+    projection leaves out."""
+    axis = np.array([0.2, -0.5, 0.84])
+    axis /= np.linalg.norm(axis)
+    e1 = np.cross(axis, [1.0, 0.0, 0.0])
+    e1 /= np.linalg.norm(e1)
+    e2 = np.cross(axis, e1)
+    legs = [np.cos(p) * e1 + np.sin(p) * e2 - tilt * axis for p in (0.3, 0.3 + 2 * np.pi / 3, 0.3 + 4 * np.pi / 3)]
+    pos_i = np.array([0.4, 1.0, -0.2])
+    F, _ = _map(pos_i, legs, axis * 1.7)
+    assert np.abs(F).max() <= 1e-14
+    rng = np.random.default_rng(23)
+    for _ in range(5):
+        tau = rng.normal(size=3)
+        F, pos = _map(pos_i, legs, tau)
+        net = sum(np.cross(pos[1 + k] - pos_i, F[1 + k]) for k in range(3))
+        assert np.allclose(net, tau - (tau @ axis) * axis, rtol=0, atol=1e-13)
