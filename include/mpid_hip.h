@@ -152,6 +152,19 @@ int mpid_get_induced_dipoles(mpid_context* ctx, const double* positions, const d
  * taken to its nearest image under PME, after an evaluation at positions. */
 int mpid_get_electrostatic_potential(mpid_context* ctx, const double* positions, const double* box, int npoints,
                                      const double* points, double* out);
+/* Allocation-free form of mpid_get_electrostatic_potential.  It has no
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  It returns the potential of the context's last completed
+ * evaluation (mpid_execute, mpid_execute_device or a getter), with host arrays
+ * in and out, and calls no hipMalloc, hipFree or allocating routine.  Points
+ * and values are staged in one fixed device array per GPU (2 MiB, held until
+ * the process ends), so a call takes at most MPID_NOALLOC_MAX_POINTS points:
+ * more fail with MPID_ERR_INVALID, as does a context without a completed
+ * evaluation.  Every context on one GPU shares that array; calls within a
+ * process are serialised on it, and the array keeps the last call's data. */
+enum { MPID_NOALLOC_MAX_POINTS = 65536 };
+int mpid_get_electrostatic_potential_noalloc(mpid_context* ctx, int npoints, const double* points, double* out);
 /* CalcMPIDForceKernel::getSystemMultipoleMoments (mpidKernels.h:86): 13
  * values about the centre of mass of masses ([n], the System's particle
  * masses, MPIDReferenceKernels.cpp:325-337): net charge (e), dipole (Debye,

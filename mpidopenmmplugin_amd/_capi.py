@@ -19,6 +19,8 @@ MPID_ERR_NOT_CONVERGED = 2
 MPID_ERR_HIP = 3
 MPID_ERR_UNSUPPORTED = 4
 
+MPID_NOALLOC_MAX_POINTS = 65536
+
 MPID_PRECISION_MIXED = 0
 MPID_PRECISION_DOUBLE = 1
 
@@ -83,6 +85,7 @@ SIGNATURES = {
     "mpid_execute_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _dp, ctypes.c_void_p, _dp]),
     "mpid_get_induced_dipoles": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
     "mpid_get_electrostatic_potential": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, ctypes.c_int, _dp, _dp]),
+    "mpid_get_electrostatic_potential_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, _dp]),
     "mpid_get_system_multipole_moments": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp]),
     "mpid_get_lab_frame_permanent_dipoles": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
     "mpid_get_total_dipoles": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),

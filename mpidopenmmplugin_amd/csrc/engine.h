@@ -95,6 +95,13 @@ void launch_dot(hipStream_t s, int m, const double* a, const double* b, double* 
 // of the charges, dipoles plus induced dipoles and quadrupoles (FP64).
 void launch_potential_points(hipStream_t s, int n, int npts, const double4* pos, const double* lab, const double* mu,
                              Box<double> box, const double* points, double* out);
+// The allocation-free form of the potential getter stages its points and
+// values in a file-scope device array of kNoAllocScratchDoubles doubles
+// (kernels_atoms.hip): at most kNoAllocMaxPoints points per call.  Its
+// address on the current device (looked up once per device).
+constexpr int kNoAllocMaxPoints = 65536;
+constexpr int kNoAllocScratchDoubles = 4 * kNoAllocMaxPoints;
+hipError_t noalloc_scratch_address(double** out);
 // System multipole sums (one block): out[0] total mass, [1..3] mass-weighted
 // position, [4] charge, [5..7] dipole, [8..13] traced second moment about the
 // centre of mass (xx xy xz yy yz zz), [14..19] atomic quadrupole sum.

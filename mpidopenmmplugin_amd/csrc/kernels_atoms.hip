@@ -14,6 +14,8 @@
 #include "engine.h"
 
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <rocprim/rocprim.hpp>
 
 namespace mpid {
@@ -1159,6 +1161,34 @@ void launch_potential_points(hipStream_t s, int n, int npts, const double4* pos,
     if (npts <= 0) return;
     hipLaunchKernelGGL(k_potential_points, dim3((npts + kPotThreads - 1) / kPotThreads), dim3(kPotThreads), 0, s, n,
                        npts, pos, lab, mu, box, points, out);
+}
+// Staging memory of the allocation-free potential getter: the points
+// (3 doubles each) at [0, 3 kNoAllocMaxPoints), their potentials after them.
+// Every GPU the library runs on holds its own copy until the process ends.
+// Host code never passes the name itself to a kernel (on the host it is a
+// stand-in, not a device address): it uses the address that
+// noalloc_scratch_address() looked up for the current device.
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+hipError_t noalloc_scratch_address(double** out) {
+    static std::mutex lock;
+    static std::map<int, double*> addr;  // device -> address of that device's copy
+    *out = nullptr;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> g(lock);
+    auto it = addr.find(dev);
+    if (it != addr.end()) {
+        *out = it->second;
+        return hipSuccess;
+    }
+    double* p = nullptr;
+    e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[dev] = p;
+    *out = p;
+    return hipSuccess;
 }
 void launch_multipole_moments(hipStream_t s, int n, const int* perm, const double* pos, const double* masses,
                               const double* lab, const double* mu, double* out) {

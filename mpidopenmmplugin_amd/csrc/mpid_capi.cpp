@@ -24,6 +24,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -198,6 +199,8 @@ struct mpid_context {
     hipEvent_t ev_sync = nullptr;
     std::string error;
     mpid_stats stats{};
+    // this device's copy of the allocation-free getter's staging array
+    double* noalloc_scratch = nullptr;
 
     // static parameters (original order)
     DBuf<double> mol, thole, pol;
@@ -1528,6 +1531,32 @@ struct mpid_context {
             }
     }
 
+    static_assert(kNoAllocMaxPoints == MPID_NOALLOC_MAX_POINTS, "staging limit of the header");
+    // getElectrostaticPotential of the last completed evaluation without any
+    // allocation: points and values pass through the per-device staging array
+    // (noalloc_scratch), under one lock, since every context on a device shares
+    // that array.
+    void potential_at_points_noalloc(int npts, const double* points, double* out) {
+        if (!mu_valid)
+            throw MpidError(MPID_ERR_INVALID, "getElectrostaticPotential (allocation-free): the context holds no "
+                                              "completed evaluation");
+        if (npts > kNoAllocMaxPoints)
+            throw MpidError(MPID_ERR_INVALID, "getElectrostaticPotential (allocation-free): " + std::to_string(npts) +
+                                                  " points exceed the limit of " +
+                                                  std::to_string(kNoAllocMaxPoints) + " per call");
+        if (!noalloc_scratch) throw MpidError(MPID_ERR_HIP, "the staging array's device address is unknown");
+        if (npts == 0) return;
+        static std::mutex lock;
+        std::lock_guard<std::mutex> g(lock);
+        double* dpts = noalloc_scratch;
+        double* dval = noalloc_scratch + 3 * (size_t)kNoAllocMaxPoints;
+        HIP_CHECK(hipMemcpyAsync(dpts, points, 3 * (size_t)npts * sizeof(double), hipMemcpyHostToDevice, stream));
+        launch_potential_points(stream, n, npts, spos.p, lab.p, mud.p, dbox, dpts, dval);
+        check_launch();
+        HIP_CHECK(hipMemcpyAsync(out, dval, (size_t)npts * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+    }
+
     void compute_fixed_field(const double* pos, const double* boxv, double* out) {
         invalidate_last();
         set_box(boxv);
@@ -1620,6 +1649,9 @@ int mpid_create(const mpid_parameters* params, const mpid_options* opts, mpid_co
         ctx->set_options(opts);
         ctx->set_parameters(params, true);
         ctx->alloc_state();
+        // the allocation-free getter's staging array: its address on this
+        // device, looked up once per device with the device current
+        HIP_CHECK(noalloc_scratch_address(&ctx->noalloc_scratch));
     });
     if (rc != MPID_OK) {
         g_create_error = ctx->error;
@@ -1663,6 +1695,11 @@ int mpid_get_electrostatic_potential(mpid_context* ctx, const double* positions,
         ctx->ensure(positions, box);
         if (npoints > 0) ctx->potential_at_points(npoints, points, out);
     });
+}
+
+int mpid_get_electrostatic_potential_noalloc(mpid_context* ctx, int npoints, const double* points, double* out) {
+    if (!ctx || npoints < 0 || (npoints > 0 && (!points || !out))) return MPID_ERR_INVALID;
+    return guarded(ctx, [&] { ctx->potential_at_points_noalloc(npoints, points, out); });
 }
 
 int mpid_get_system_multipole_moments(mpid_context* ctx, const double* positions, const double* box,
