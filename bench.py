@@ -32,10 +32,11 @@ sys.path.insert(0, ROOT)
 # factors of a pair, and the field one site's moments produce at the other;
 # elec_pair is the quasi-internal-frame interaction without its charge-charge
 # channel, which charge_pair evaluates (in FP64) for every pair; elec_pair_bare
-# is elec_pair for a pair whose second site is a bare charge.
+# is elec_pair for a pair whose second site is a bare charge, and
+# elec_pair_bare_cart its Cartesian form, which the tile kernel runs.
 PAIR_FLOPS = {"fixed_field_radial": (73, 47), "fixed_field_from": (105, 105), "induced_field_radial": (36, 23),
               "induced_field_from": (15, 15), "elec_pair": (2106, 2104), "elec_pair_bare": (570, 568),
-              "charge_pair": (19, 19)}
+              "elec_pair_bare_cart": (194, 194), "charge_pair": (19, 19)}
 
 
 def pair_kernel_flops(cat, pc):
@@ -50,7 +51,7 @@ def pair_kernel_flops(cat, pc):
     if cat == "electrostatic_pairs":  # the charge-charge channel of every pair, the rest where moments exist
         general = pc["all"] - pc["cc"] - pc["bare1"]
         return (pc["all"] * F["charge_pair"][0] + pc["pol"] * F["elec_pair"][0] +
-                (general - pc["pol"]) * F["elec_pair"][1] + pc["bare1"] * F["elec_pair_bare"][1])
+                (general - pc["pol"]) * F["elec_pair"][1] + pc["bare1"] * F["elec_pair_bare_cart"][1])
     return None
 
 
@@ -107,25 +108,56 @@ def grid_bytes(cat, cfg, n, launches, mixed):
     return None
 
 
-def cpu_baseline(seconds_hint=20.0):
-    """FP64 oracle (single thread) on the 12,288-atom box; scaled to the
-    98,304-atom box by the atom and grid-point ratio (8x)."""
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(full=False):
+    """The FP64 oracle on one host thread: a cell-list restatement of the
+    reference's MPIDReferencePmeForce (same formulas and DIIS solver; pairs
+    from cell lists instead of the reference's all-pairs loop), timed on the
+    216-water and 4096-water boxes, median of 3 evaluations each.  value is
+    the 98,304-atom box's rate scaled from the 4096-water box by the atom and
+    grid-point ratio (8x), or, with full, one evaluation of the 98,304-atom
+    box itself (about two minutes)."""
     from mpidopenmmplugin_amd.waterbox import make_waterbox
     from oracle import oracle
-    f, pos, box = make_waterbox("water4096")
-    t = time.perf_counter()
-    r = oracle.evaluate(f.flatten(), pos, box)
-    dt = time.perf_counter() - t
-    return {
-        "value": (1.0 / dt) / 8.0,
-        "unit": "evals/s",
-        "cores": 1,
-        "kind": "port",
-        "sample": (f"one full evaluation of the 12,288-atom water box (64^3 grid, mutual to 1e-6 Debye, "
-                   f"{r['iterations']} DIIS iterations) by the FP64 oracle on 1 host thread took {dt:.1f} s; "
-                   "value = 1/(8 x that), the 98,304-atom box having 8x the atoms and grid points"),
-        "sample_seconds": dt,
-    }
+
+    def median_time(name, reps):
+        f, pos, box = make_waterbox(name)
+        flat = f.flatten()
+        times, iters = [], None
+        for _ in range(reps):
+            t = time.perf_counter()
+            r = oracle.evaluate(flat, pos, box)
+            times.append(time.perf_counter() - t)
+            iters = r["iterations"]
+        return float(np.median(times)), times, iters
+
+    c0, c0_all, _ = median_time("water216", 3)
+    c1, c1_all, it1 = median_time("water4096", 3)
+    out = {"cores": 1, "kind": "port", "unit": "evals/s", "cpu_model": _cpu_model(),
+           "host_cpus": len(os.sched_getaffinity(0)), "water216_s": c0_all, "water4096_s": c1_all}
+    if full:
+        c2, c2_all, it2 = median_time("water32768", 1)
+        out.update(value=1.0 / c2, water32768_s=c2_all, sample_seconds=c2,
+                   sample=(f"the FP64 oracle (a cell-list restatement of the reference's FP64 path with its DIIS "
+                           f"solver, not its all-pairs loop) on 1 host thread: one evaluation of the 98,304-atom "
+                           f"box took {c2:.1f} s ({it2} DIIS iterations); medians of 3 on the 648- and 12,288-atom "
+                           f"boxes {c0:.2f} s and {c1:.1f} s"))
+    else:
+        out.update(value=(1.0 / c1) / 8.0, sample_seconds=float(sum(c0_all) + sum(c1_all)),
+                   sample=(f"the FP64 oracle (a cell-list restatement of the reference's FP64 path with its DIIS "
+                           f"solver, not its all-pairs loop) on 1 host thread: median of 3 evaluations of the "
+                           f"12,288-atom box {c1:.1f} s ({it1} DIIS iterations; 648-atom box {c0:.2f} s); value = "
+                           f"1/(8 x that), the 98,304-atom box having 8x the atoms and grid points"))
+    return out
 
 
 def run(args):
@@ -193,8 +225,9 @@ def run(args):
     if not args.skip_fp64:
         kd = make("double")
         ms64, e64 = timed_rate(kd, max(1, args.steps // 2), 1)
+        kt64 = kernel_times(kd, 2)
         fp64 = {"ms_per_step": 1000 * ms64, "evals_per_s": 1.0 / ms64, "stats": kd.stats(),
-                "kernels": {k: round(v["ms_per_step"], 4) for k, v in kernel_times(kd, 2).items()}}
+                "kernels": {k: round(v["ms_per_step"], 4) for k, v in kt64.items()}, "kernel_times": kt64}
         print(f"[bench] {args.workload} double: {1000 * ms64:.2f} ms/step", file=sys.stderr, flush=True)
         del kd
         # accuracy of the timed path against the FP64 path converged to 1e-10 Debye
@@ -277,6 +310,44 @@ def run(args):
                     ("; the mixed path's FP64 check of the induced dipoles runs on another stream during this "
                      "kernel and shares its CUs, and the live duration includes that" if mixed else "")}
 
+    # the hand-written FFT convolution against rocFFT (through torch.fft,
+    # which calls hipFFT/rocFFT on ROCm) on the same grid: R2C, a real
+    # influence-function multiply, C2R, in the working precision
+    fft_check = None
+    if "fft" in kernels and kernels["fft"]["launches_per_step"] > 0:
+        g = cfg["grid"]
+
+        def rocfft_ms(rdt):
+            q = torch.randn((g, g, g), dtype=rdt, device=dev)
+            infl = torch.rand((g, g, g // 2 + 1), dtype=rdt, device=dev)
+            for _ in range(3):
+                torch.fft.irfftn(torch.fft.rfftn(q) * infl, s=(g, g, g))
+            torch.cuda.synchronize()
+            ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            reps = 20
+            ev0.record()
+            for _ in range(reps):
+                torch.fft.irfftn(torch.fft.rfftn(q) * infl, s=(g, g, g))
+            ev1.record()
+            torch.cuda.synchronize()
+            return ev0.elapsed_time(ev1) / reps
+
+        fft_check = {"grid": [g, g, g], "rocfft_fp32_ms_per_convolution": rocfft_ms(torch.float32),
+                     "rocfft_fp64_ms_per_convolution": rocfft_ms(torch.float64),
+                     "note": "rocFFT through torch.fft (R2C, elementwise multiply by a real array, C2R; torch's "
+                             "own kernel for the multiply), HIP events on torch's stream; the hand-written passes "
+                             "apply the influence function inside the transform.  The hand-written times are "
+                             "per-launch averages of the fft family: FP32 in the mixed solver iterations (the "
+                             "mixed path's one FP64 permanent-multipole convolution included), FP64 in the "
+                             "FP64 path"}
+        kf = kernels["fft"]
+        if mixed:
+            fft_check["hand_written_mixed_ms_per_convolution"] = kf["ms_per_step"] / kf["launches_per_step"]
+        if fp64 and "fft" in fp64["kernel_times"] and fp64["kernel_times"]["fft"]["launches_per_step"] > 0:
+            kf = fp64["kernel_times"]["fft"]
+            fft_check["hand_written_fp64_ms_per_convolution"] = kf["ms_per_step"] / kf["launches_per_step"]
+        print(f"[bench] fft check {g}^3: {fft_check}", file=sys.stderr, flush=True)
+
     # host-buffer rate (positions and forces crossing PCIe each evaluation)
     t = time.perf_counter()
     for _ in range(3):
@@ -304,18 +375,22 @@ def run(args):
         "kernels": kernels,
         "roofline": roof,
         "rooflines": rooflines,
+        "fft_check": fft_check,
         "host_buffer_ms_per_step": pcie_ms,
         "setup_s": setup_s,
     }
     if fp64:
+        fp64.pop("kernel_times", None)
         out["fp64_path"] = {"evals_per_s": fp64["evals_per_s"], "ms_per_step": fp64["ms_per_step"],
                             "kernels_ms_per_step": fp64["kernels"]}
         out["speedup_vs_fp64"] = fp64["ms_per_step"] / (1000 * ms_step)
     out.update(result)
+    if not args.skip_gradients:
+        out["gradients"] = adjoint_leg(args)
     if args.batch_systems > 0:
         out["batch"] = batch_leg(args)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline"] = cpu_baseline(full=args.cpu_full)
     return out
 
 
@@ -360,6 +435,85 @@ def batch_leg(args):
             "generate_s": gen_s, "contexts_s": init_s}
 
 
+def adjoint_leg(args):
+    """Energy gradients of the 12,288-atom box (water4096: octopoles,
+    anisotropic polarisability, 64^3 grid, mutual to 1e-6 Debye) with respect
+    to the positions and the per-particle parameters, through
+    mpid_energy_gradients (forward solve, adjoint solve, gradient kernels),
+    timed against a plain evaluation on the same host buffers; the same
+    through the torch.autograd.Function with the per-type parameterisation
+    (float32 leaves); and the per-type gradients against the committed
+    central differences of the FP64 oracle (tests/golden).  Also the
+    gradient/forward ratio on the 98,304-atom box."""
+    import torch
+    from mpidopenmmplugin_amd.autograd import MPIDEnergy, expand_water_parameters, water_type_parameters
+    from mpidopenmmplugin_amd.kernel import HipCalcMPIDForceKernel
+    from mpidopenmmplugin_amd.waterbox import make_waterbox
+
+    def ratio(workload, prec, reps):
+        force, pos, box = make_waterbox(workload)
+        k = HipCalcMPIDForceKernel(prec)
+        k.initialize(force.getNumMultipoles(), force)
+        for _ in range(2):
+            k.execute(pos, box)
+            g = k.energyGradients(pos, box)
+        t = time.perf_counter()
+        for _ in range(reps):
+            k.execute(pos, box)
+        fwd = (time.perf_counter() - t) / reps
+        t = time.perf_counter()
+        for _ in range(reps):
+            g = k.energyGradients(pos, box)
+        grd = (time.perf_counter() - t) / reps
+        return {"forward_ms": 1000 * fwd, "gradients_ms": 1000 * grd, "gradients_over_forward": grd / fwd,
+                "adjoint_iterations": g["adjoint_iterations"]}
+
+    out = {"workload": "water4096: 12,288 atoms, 64^3 grid, mutual to 1e-6 Debye; gradients of the energy with "
+                       "respect to positions and every per-particle parameter (host buffers both ways)"}
+    for prec in ("mixed", "double"):
+        out[prec] = ratio("water4096", prec, args.steps)
+        print(f"[bench] gradients water4096 {prec}: {out[prec]}", file=sys.stderr, flush=True)
+    out["water32768_mixed"] = ratio("water32768", "mixed", max(2, args.steps // 2))
+    print(f"[bench] gradients water32768 mixed: {out['water32768_mixed']}", file=sys.stderr, flush=True)
+
+    # the autograd path with per-type parameters
+    force, pos, box = make_waterbox("water4096")
+    nw = force.getNumMultipoles() // 3
+    model = MPIDEnergy(force, box, "mixed")
+    p = water_type_parameters(force, dtype=torch.float32)
+    x = torch.tensor(pos, dtype=torch.float32, requires_grad=True)
+
+    def step():
+        for v in list(p.values()) + [x]:
+            v.grad = None
+        e = model(x, *expand_water_parameters(p, nw), p["default_thole"])
+        e.backward()
+        return e
+
+    step()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    out["autograd_ms"] = 1000 * (time.perf_counter() - t) / args.steps
+    got = np.concatenate([np.atleast_1d(p[k].grad.double().numpy()) for k in
+                          ("q_O", "q_H", "d_O", "Q_O", "O_O", "alpha_O", "thole_O", "thole_H", "default_thole")])
+    fx = os.path.join(ROOT, "tests", "golden", "water4096_gradients_fd.npz")
+    if os.path.exists(fx):
+        ref = np.load(fx)
+        fd = ref["gradient"]
+        big = np.abs(fd) > 1e-9 * np.abs(fd).max()
+        rel = np.abs(got - fd)[big] / np.abs(fd)[big]
+        out["per_type_vs_fd"] = {"max_rel_err": float(rel.max()), "median_rel_err": float(np.median(rel)),
+                                 "params_compared": int(big.sum()), "params": int(fd.size),
+                                 "reference": "central differences of the FP64 oracle's energy re-converged to "
+                                              "1e-10 Debye (steps 1e-2 relative for the charges and multipoles, "
+                                              "in which the energy is quadratic; four-point, 1e-4, for the "
+                                              "polarisabilities and Thole widths); float32 parameter leaves"}
+    print(f"[bench] gradients autograd: {out['autograd_ms']:.2f} ms, {out.get('per_type_vs_fd')}", file=sys.stderr,
+          flush=True)
+    return out
+
+
 def profiled_traffic(workload, precision):
     """HBM bytes per launch of the electrostatics kernel from the committed
     rocprofv3 summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
@@ -396,6 +550,9 @@ def main():
     ap.add_argument("--workload", default="water32768", choices=["water216", "water4096", "water32768"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--skip-gradients", action="store_true")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="time the CPU baseline on the 98,304-atom box itself (about two minutes)")
     ap.add_argument("--batch-systems", type=int, default=512,
                     help="systems in the batched-workload leg (0 skips it)")
     ap.add_argument("--batch-threads", type=int, default=16)

@@ -13,7 +13,10 @@
  *   quadrupole  XX XY YY XZ YZ ZZ
  *   octopole    XXX XXY XYY YYY XXZ XYZ YYZ XZZ YZZ ZZZ
  * Units: nm, e, kJ/mol.  Positions are [n][3] doubles; box is three box
- * vectors, row-major [3][3] (only rectangular boxes are supported).
+ * vectors a, b, c, row-major [3][3], in OpenMM's reduced form (a along x, b
+ * in the xy plane, |bx|, |cx| <= ax/2, |cy| <= by/2; rectangular or
+ * triclinic, MPIDReferenceForce.cpp:2615-2676): other boxes are
+ * MPID_ERR_INVALID.
  */
 #ifndef MPID_HIP_H
 #define MPID_HIP_H
@@ -30,7 +33,7 @@ enum {
     MPID_ERR_INVALID = 1,        /* invalid parameter (OpenMMException in the reference) */
     MPID_ERR_NOT_CONVERGED = 2,  /* mutual induced dipoles did not converge */
     MPID_ERR_HIP = 3,            /* HIP runtime failure */
-    MPID_ERR_UNSUPPORTED = 4     /* feature outside this build (triclinic box, NoCutoff beyond 16,384 particles, ...) */
+    MPID_ERR_UNSUPPORTED = 4     /* feature outside this build (NoCutoff beyond 16,384 particles, ...) */
 };
 
 /* NOCUTOFF: every pair, no periodic images, no reciprocal space; the cutoff,
@@ -43,9 +46,11 @@ enum { MPID_POLARIZATION_MUTUAL = 0, MPID_POLARIZATION_DIRECT = 1, MPID_POLARIZA
 enum { MPID_PRECISION_MIXED = 0, MPID_PRECISION_DOUBLE = 1 };
 /* Option flags.
  * CONSERVATIVE_TORQUES: add the half of the induced-induced torque about the
- * pair axis that the reference omits (MPIDReferenceForce.cpp:4854-4855).  Off
- * by default, which reproduces the reference; with it the forces on
- * anisotropically polarizable sites are exact gradients of the energy.
+ * pair axis that the reference omits (MPIDReferenceForce.cpp:4854-4855), and
+ * map the torques of ThreeFold sites onto their frame atoms by the exact
+ * chain rule of the frame instead of the reference's approximate projection
+ * (:2068-2097).  Off by default, which reproduces the reference; with it the
+ * forces are exact gradients of the energy for every site.
  * SINGLE_STREAM: issue all of the context's work on one HIP stream instead of
  * three (real space, reciprocal space, the mixed path's FP64 check).  For many
  * small contexts driven at once, where cross-stream waits would hold up the
@@ -185,6 +190,41 @@ int mpid_get_pme_parameters(const mpid_context* ctx, double* alpha, int* nx, int
 int mpid_compute_fixed_field(mpid_context* ctx, const double* positions, const double* box, double* field_out);
 int mpid_compute_induced_field(mpid_context* ctx, const double* positions, const double* box, const double* mu,
                                double* field_out);
+
+/* Gradients of the energy with respect to the positions and the force's
+ * per-particle parameters.  No counterpart in the reference, which returns
+ * forces only; the interface it extends is CalcMPIDForceKernel::execute
+ * (mpidKernels.h:77) and the parameters of MPIDForce::addMultipole
+ * (MPIDForce.h:236-281).
+ *
+ * One evaluation at positions (mutual or direct polarization; Extrapolated
+ * returns MPID_ERR_UNSUPPORTED), with the conservative torques whatever the
+ * context's flags, followed by an adjoint solve with the solver's symmetric
+ * operator, whose right-hand side is the derivative of the energy in the
+ * induced dipoles (the residual of the solution).  The gradients are those of
+ * the energy at the refined dipoles, which *energy_out receives.  Each output
+ * pointer may be NULL.  Gradients are with respect to the numbers in the
+ * layouts of mpid_parameters, each listed component separately (a
+ * quadrupole's XY once, not XY and YX).
+ *
+ * Sites with neither higher moments nor polarizability ("bare" charges) get
+ * the charge gradient only, and zeros for their other parameters, unless
+ * flags has MPID_GRADIENT_ALL_SITES, which evaluates every pair with the
+ * general formula and the total field at every site (slower). */
+typedef struct {
+    double* positions;      /* [n][3] dE/dr (kJ/mol/nm): minus the forces */
+    double* charges;        /* [n] kJ/mol/e */
+    double* dipoles;        /* [3n] molecular frame */
+    double* quadrupoles;    /* [6n] molecular frame, XX XY YY XZ YZ ZZ */
+    double* octopoles;      /* [10n] molecular frame, XXX XXY XYY YYY XXZ XYZ YYZ XZZ YZZ ZZZ */
+    double* polarity;       /* [3n] molecular-frame xx yy zz polarizabilities (kJ/mol/nm^3) */
+    double* tholes;         /* [n] */
+    double* default_thole;  /* [1] MPIDForce::getDefaultTholeWidth */
+    int adjoint_iterations; /* out: iterations of the adjoint solve */
+} mpid_gradients;
+enum { MPID_GRADIENT_ALL_SITES = 1 };
+int mpid_energy_gradients(mpid_context* ctx, const double* positions, const double* box, int flags,
+                          double* energy_out, mpid_gradients* grads);
 
 int mpid_get_stats(const mpid_context* ctx, mpid_stats* stats);
 /* Profiling (no counterpart in the reference): mpid_set_profiling times every

@@ -76,6 +76,16 @@ class Stats(ctypes.Structure):
     ]
 
 
+class Gradients(ctypes.Structure):
+    _fields_ = [
+        ("positions", _dp), ("charges", _dp), ("dipoles", _dp), ("quadrupoles", _dp), ("octopoles", _dp),
+        ("polarity", _dp), ("tholes", _dp), ("default_thole", _dp),
+        ("adjoint_iterations", ctypes.c_int),
+    ]
+
+
+MPID_GRADIENT_ALL_SITES = 1
+
 # name -> (restype, argtypes); the full exported surface of include/mpid_hip.h
 SIGNATURES = {
     "mpid_create": (ctypes.c_int, [ctypes.POINTER(Parameters), ctypes.POINTER(Options),
@@ -92,6 +102,8 @@ SIGNATURES = {
     "mpid_get_pme_parameters": (ctypes.c_int, [ctypes.c_void_p, _dp, _ip, _ip, _ip]),
     "mpid_compute_fixed_field": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
     "mpid_compute_induced_field": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp]),
+    "mpid_energy_gradients": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, ctypes.c_int, _dp,
+                                             ctypes.POINTER(Gradients)]),
     "mpid_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Stats)]),
     "mpid_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "mpid_set_profile_mask": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint]),

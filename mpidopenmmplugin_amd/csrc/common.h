@@ -77,17 +77,39 @@ __device__ __forceinline__ float erfc_r(float x, float e) {
 }
 MPID_HD double erfc_r(double x, double) { return erfc(x); }
 
-// Rectangular periodic box as seen by the kernels.
+// Periodic box as seen by the kernels, in OpenMM's reduced triclinic form:
+// box vectors a = (L0, 0, 0), b = (bx, L1, 0), c = (cx, cy, L2) with
+// |bx|, |cx| <= L0/2 and |cy| <= L1/2 (bx = cx = cy = 0: rectangular).
 template <class R> struct Box {
     R L[3];
-    R inv[3];
+    R inv[3];  // 1 / L_k
+    R bx, cx, cy;
 };
 
+// Nearest image of a separation: the c, b and a shifts in turn, as
+// getPeriodicDelta does (MPIDReferenceForce.cpp:2671-2676).  For a
+// rectangular box the off-diagonal terms vanish and each axis wraps alone.
 template <class R> MPID_HD V3<R> min_image(V3<R> d, const Box<R>& b) {
+    const R nc = floor(d.z * b.inv[2] + R(0.5));
+    d.z -= nc * b.L[2];
+    d.y -= nc * b.cy;
+    d.x -= nc * b.cx;
+    const R nb = floor(d.y * b.inv[1] + R(0.5));
+    d.y -= nb * b.L[1];
+    d.x -= nb * b.bx;
     d.x -= b.L[0] * floor(d.x * b.inv[0] + R(0.5));
-    d.y -= b.L[1] * floor(d.y * b.inv[1] + R(0.5));
-    d.z -= b.L[2] * floor(d.z * b.inv[2] + R(0.5));
     return d;
+}
+
+// Fractional coordinates (along a, b, c) of a position: r = s0 a + s1 b + s2 c.
+template <class R> MPID_HD V3<R> to_frac(V3<R> r, const Box<R>& b) {
+    const R s2 = r.z * b.inv[2];
+    const R s1 = (r.y - s2 * b.cy) * b.inv[1];
+    const R s0 = (r.x - s1 * b.bx - s2 * b.cx) * b.inv[0];
+    return V3<R>{s0, s1, s2};
+}
+template <class R> MPID_HD V3<R> from_frac(V3<R> s, const Box<R>& b) {
+    return V3<R>{s.x * b.L[0] + s.y * b.bx + s.z * b.cx, s.y * b.L[1] + s.z * b.cy, s.z * b.L[2]};
 }
 
 // Per-atom multipole record, 20 reals: q, dipole xyz, quadrupole

@@ -16,6 +16,86 @@ template <> struct R2T<float> { typedef float2 type; };
 template <> struct R2T<double> { typedef double2 type; };
 template <class R> using R2 = typename R2T<R>::type;
 
+// Grid coordinates u_k = n_k s_k of a position (s its fractional coordinates
+// along the box vectors): A_jk = du_k/dr_j = n_k (box^-1)_jk, lower
+// triangular (A01 = A02 = A12 = 0); a rectangular box has the diagonal only
+// (tri = 0).  Cartesian and fractional multipoles and potential derivatives
+// convert through it (transformMultipolesToFractionalCoordinates and
+// transformPotentialToCartesianCoordinates, MPIDReferenceForce.cpp:3077-3267).
+template <class R> struct FracT {
+    R a00, a10, a11, a20, a21, a22;
+    int tri;
+};
+// v_k = sum_j d_j A_jk: the fractional components of a dipole (or of any
+// contravariant vector)
+template <class R> MPID_HD V3<R> frac_vector(V3<R> d, const FracT<R>& A) {
+    return V3<R>{d.x * A.a00 + d.y * A.a10 + d.z * A.a20, d.y * A.a11 + d.z * A.a21, d.z * A.a22};
+}
+// c_j = sum_k A_jk g_k: Cartesian derivatives from grid-coordinate ones
+template <class R> MPID_HD V3<R> cart_vector(V3<R> g, const FracT<R>& A) {
+    return V3<R>{A.a00 * g.x, A.a10 * g.x + A.a11 * g.y, A.a20 * g.x + A.a21 * g.y + A.a22 * g.z};
+}
+// Fractional moments of a Cartesian record (q, dipole, quadrupole XX XY XZ YY
+// YZ ZZ, octopole XXX XXY XXZ XYY XYZ XZZ YYY YYZ YZZ ZZZ): each tensor
+// contracted with A on every index, times the number of index orderings its
+// stored component stands for (1 2 2 1 2 1; 1 3 3 3 6 3 1 3 3 1), so that
+// sum_k mf_k phi_k over the grid-coordinate derivatives is the energy.
+template <class R> MPID_HD void frac_moments(const R* m, const FracT<R>& A, R* mf) {
+    mf[0] = m[0];
+    if (!A.tri) {
+        const R gx = A.a00, gy = A.a11, gz = A.a22;
+        mf[1] = gx * m[1]; mf[2] = gy * m[2]; mf[3] = gz * m[3];
+        mf[4] = gx * gx * m[4]; mf[5] = R(2) * gx * gy * m[5]; mf[6] = R(2) * gx * gz * m[6];
+        mf[7] = gy * gy * m[7]; mf[8] = R(2) * gy * gz * m[8]; mf[9] = gz * gz * m[9];
+        mf[10] = gx * gx * gx * m[10]; mf[11] = R(3) * gx * gx * gy * m[11];
+        mf[12] = R(3) * gx * gx * gz * m[12]; mf[13] = R(3) * gx * gy * gy * m[13];
+        mf[14] = R(6) * gx * gy * gz * m[14]; mf[15] = R(3) * gx * gz * gz * m[15];
+        mf[16] = gy * gy * gy * m[16]; mf[17] = R(3) * gy * gy * gz * m[17];
+        mf[18] = R(3) * gy * gz * gz * m[18]; mf[19] = gz * gz * gz * m[19];
+        return;
+    }
+    const R a[3][3] = {{A.a00, R(0), R(0)}, {A.a10, A.a11, R(0)}, {A.a20, A.a21, A.a22}};
+    const V3<R> d = frac_vector(mk(m[1], m[2], m[3]), A);
+    mf[1] = d.x; mf[2] = d.y; mf[3] = d.z;
+    // full tensors from the stored components
+    const int q2[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+    const int q3[3][3][3] = {{{0, 1, 2}, {1, 3, 4}, {2, 4, 5}}, {{1, 3, 4}, {3, 6, 7}, {4, 7, 8}},
+                             {{2, 4, 5}, {4, 7, 8}, {5, 8, 9}}};
+    R t2[3][3];  // Q A (second index)
+    for (int i = 0; i < 3; i++)
+        for (int l = 0; l < 3; l++) {
+            R v = R(0);
+            for (int j = 0; j < 3; j++) v += m[4 + q2[i][j]] * a[j][l];
+            t2[i][l] = v;
+        }
+    const int kk[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+    const R mult2[6] = {R(1), R(2), R(2), R(1), R(2), R(1)};
+    for (int c = 0; c < 6; c++) {
+        const int k = kk[c][0], l = kk[c][1];
+        R v = R(0);
+        for (int i = 0; i < 3; i++) v += a[i][k] * t2[i][l];
+        mf[4 + c] = mult2[c] * v;
+    }
+    R t3[3][3][3];  // O contracted with A on its last index
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            for (int n = 0; n < 3; n++) {
+                R v = R(0);
+                for (int p = 0; p < 3; p++) v += m[10 + q3[i][j][p]] * a[p][n];
+                t3[i][j][n] = v;
+            }
+    const int kkk[10][3] = {{0, 0, 0}, {0, 0, 1}, {0, 0, 2}, {0, 1, 1}, {0, 1, 2},
+                            {0, 2, 2}, {1, 1, 1}, {1, 1, 2}, {1, 2, 2}, {2, 2, 2}};
+    const R mult3[10] = {R(1), R(3), R(3), R(3), R(6), R(3), R(1), R(3), R(3), R(1)};
+    for (int c = 0; c < 10; c++) {
+        const int k = kkk[c][0], l = kkk[c][1], n = kkk[c][2];
+        R v = R(0);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) v += a[i][k] * a[j][l] * t3[i][j][n];
+        mf[10 + c] = mult3[c] * v;
+    }
+}
+
 // Parameters of the real-space pair kernels.
 template <class R> struct PairParams {
     Box<R> box;
@@ -65,8 +145,10 @@ void launch_rotate(hipStream_t s, int n, Box<double> box, const double4* pos, co
 template <class R>
 void launch_convert(hipStream_t s, int n, const double4* pos, const double* lab, const double* thole,
                     const double* damp, R4<R>* rpos, R* rlab, R2<R>* rtd);
+// exact_threefold: ThreeFold frames by the exact chain rule (conservative
+// torques) instead of the reference's approximate projection
 void launch_torque_to_force(hipStream_t s, int n, Box<double> box, const double4* pos, const int4* axes,
-                            const acc_t* torque, acc_t* force);
+                            const acc_t* torque, acc_t* force, int exact_threefold);
 void launch_output_forces(hipStream_t s, int n, const int* perm, const acc_t* force, double* out, int accumulate);
 void launch_unpermute3(hipStream_t s, int n, const int* perm, const double* in, double* out);
 void launch_zero_acc(hipStream_t s, acc_t* p, size_t count, const double* skip = nullptr);
@@ -80,7 +162,7 @@ template <class R> void launch_fill(hipStream_t s, R* p, size_t count, R v);
 // fixed field: E0 = recip (from phi) + self + real-space accumulator
 template <class R>
 void launch_fixed_field_finish(hipStream_t s, int n, const R* phi, const double* lab, const acc_t* acc,
-                               double3 gscale, double selfc, double* E0);
+                               FracT<double> A, double selfc, double* E0);
 // mu0 = alpha_lab E0 ; y0 = S E0 ; b = S E0
 void launch_solver_init(hipStream_t s, int n, const double* sqrtpol, const double* E0, double* b, double* y);
 // mu = S y (double and precision copies)
@@ -122,7 +204,7 @@ struct ExtrapCoefficients {
 // mu_(m) at every atom, from the real-space accumulators, the reciprocal
 // potential derivatives phidp and the self term) and mu_(m+1) = alpha_lab E.
 template <class R>
-void launch_extrap_order(hipStream_t s, int n, const acc_t* field, const acc_t* gacc, const R* phidp, double3 gscale,
+void launch_extrap_order(hipStream_t s, int n, const acc_t* field, const acc_t* gacc, const R* phidp, FracT<double> A,
                          double selfc, const double* labpol, const double* mu, double* E, double* G, double* mu_next,
                          R* mu_next_r);
 // mu = sum_o c'_o mu_(o) (mus: [order][3n])
@@ -159,13 +241,14 @@ void launch_sub(hipStream_t s, int m, const double* a, const double* b, double* 
 // reciprocal-space energy, forces and torques, self terms (per atom)
 template <class R>
 void launch_recip_atoms(hipStream_t s, int n, int mutual, const double* phi, const R* phidp, const double* lab,
-                        const double* mu, const int* aniso, double3 gscale, double alpha, acc_t* force,
+                        const double* mu, const int* aniso, FracT<double> A, double alpha, acc_t* force,
                         acc_t* torque, acc_t* energy);
 
 // ---- PME (kernels_pme.hip) ----------------------------------------------------
 template <class R> struct PmeGeom {
     int n[3];
-    R gscale[3];      // n_k / L_k
+    R gscale[3];      // n_k / L_k (the diagonal of A)
+    FracT<R> A;
     Box<R> box;
     const double* skip;  // the solver's stop flag in the spreading and gathering of a solver iteration, else null
 };
@@ -235,12 +318,18 @@ template <class R> struct FftPlan {
     const R2<R>* tw[3];  // per-axis twiddles exp(-2 pi i k / n), k < n
 };
 template <class R> struct ConvParams {
-    R expf;          // pi^2 / alpha^2
-    R sc;            // 1 / (pi V)
-    R inv[3];        // 1 / box edge
-    const R* moduli; // |b(k)|^2, [n0 | n1 | n2]
+    const R* infl;       // the influence function on the half spectrum, [kz][ky][kx] (launch_influence_table)
     const double* skip;  // the solver's stop flag in the FFT passes of a solver iteration, else null
 };
+// Inputs of the influence-function table, FP64
+struct InflParams {
+    int n[3];
+    double expf;                          // pi^2 / alpha^2
+    double sc;                            // 1 / (pi V)
+    double r00, r10, r11, r20, r21, r22;  // box^-1 (lower triangular): m = box^-1 k
+};
+// out[(kz * n1 + ky) * n0 + kx], kz < n2 / 2 + 1; moduli64: |b(k)|^2, [n0 | n1 | n2]
+template <class R> void launch_influence_table(hipStream_t s, InflParams ip, const double* moduli64, R* out);
 // Fold for tilings with W = T + 8 <= 2T in every dimension: one workgroup per
 // Morton cell sums, for each point of its T^3 core, its own window and those
 // of its lower neighbours (launch_fold handles any tiling)
@@ -254,23 +343,25 @@ void launch_recip_convolution(hipStream_t s, const FftPlan<R>& p, R* Q, R2<R>* S
 // ---- real space (kernels_pairs.hip) -------------------------------------------
 template <class R>
 void launch_block_bounds(hipStream_t s, int n, const R4<R>* pos, float4* bmin, float4* bmax);
-// one wavefront per block-row entry r; entries with J < I return at once
+// One workgroup per tile act[0 .. *nact) (row entries r of block pairs
+// J >= I); the launch covers maxtiles >= *nact.
 // tmask: pairs with a multipole or a polarisable site; tmaskcc: pairs of two
 // bare charges; chg[i] != 0: atom i is a bare charge (pairs of tmask with one
 // such site take the reduced formula).  The charge-charge channel of all
 // pairs is evaluated in FP64 from pos64 (x, y, z) and the FP64 lab-frame
 // charges lab64[20 i].
 template <class R>
-void launch_electrostatics_pairs(hipStream_t s, int nrows, const int* rows, const int* owner,
-                                 const unsigned long long* tmask, const unsigned long long* tmaskcc, const int* chg,
-                                 PairAtoms<R> a, const R* mu, PairParams<R> p, const double4* pos64,
-                                 const double* lab64, PairParams<double> p64, acc_t* force, acc_t* torque,
-                                 acc_t* energy);
+void launch_electrostatics_pairs(hipStream_t s, int maxtiles, const int4* act, const int* nact, const int* rows,
+                                 const int* owner, const unsigned long long* tmask,
+                                 const unsigned long long* tmaskcc, const int* chg, PairAtoms<R> a, const R* mu,
+                                 PairParams<R> p, const double4* pos64, const double* lab64, PairParams<double> p64,
+                                 acc_t* force, acc_t* torque, acc_t* energy);
 // ---- neighbour lists (kernels_nlist.hip) ---------------------------------------
-// Block pairs within the cutoff: counts (rows == nullptr) or the rows at
-// start[I] (at most cap entries are stored; the count may exceed it)
+// Block pairs within the cutoff: counts and upper (those with J >= I) when
+// rows == nullptr, or the rows at start[I] (at most cap entries are stored;
+// the count may exceed it)
 void launch_block_rows(hipStream_t s, int nb, const float4* bmin, const float4* bmax, Box<float> box, float cutoff,
-                       const int* start, int* counts, int* rows, int* owner, int cap);
+                       const int* start, int* counts, int* upper, int* rows, int* owner, int cap);
 void launch_exclusive_scan(hipStream_t s, int m, const int* counts, int* start);
 // pos.w of the FP64 positions is the damping factor: nonzero iff the site is polarisable
 // Masks, then lists: all = true gives the full list of every atom stored by
@@ -279,7 +370,9 @@ void launch_exclusive_scan(hipStream_t s, int m, const int* counts, int* start);
 // the exclusive scan of polcnt, is written on the way).  nmask: 64 words per
 // row entry (nrows entries); polmask: one word per block; scalars[0]:
 // largest stored list length, scalars[1]: nonzero if a block pair lacked its
-// mirror entry.  nrows: row entries launched (the count, or the capacity
+// mirror entry.  act[ustart[I] + k] = {r, I, J} for the k-th row entry r of
+// row I with J >= I (ustart: the exclusive scan of launch_block_rows'
+// upper, whose last element is the number of such entries).  nrows: row entries launched (the count, or the capacity
 // when the count is not known on the host: entries past rstart[nb] return);
 // rcap: the capacity of rows / nmask (entries beyond it are ignored).
 void launch_nl_build(hipStream_t s, int n, int nrows, int rcap, bool all, const int* rstart, const int* rows,
@@ -287,7 +380,7 @@ void launch_nl_build(hipStream_t s, int n, int nrows, int rcap, bool all, const 
                      const int* partners, const int* npart, int pstride, Box<double> box, double rc2, int cap,
                      unsigned long long* nmask, unsigned long long* polmask, int* polcnt, int* polstart, int* nbr,
                      int* count, int* scalars, unsigned long long* tmask, int* pnbr, int* pcount, const int* chg,
-                     unsigned long long* tmaskcc);
+                     unsigned long long* tmaskcc, int4* act, const int* ustart);
 void launch_pol_fill(hipStream_t s, int n, const double4* pos, const int* start, int* idx);
 // fields at the atoms atoms[0..natoms) (all atoms when atoms == nullptr, natoms = n)
 template <class R>
@@ -302,8 +395,55 @@ template <class R> void launch_cov_fixed_field(hipStream_t s, int n, PairAtoms<R
 template <class R>
 void launch_cov_induced_field(hipStream_t s, int n, PairAtoms<R> a, const R* mu, PairParams<R> p, int pol_only,
                               acc_t* field, acc_t* grad = nullptr);
+// pot != nullptr: also each pair's potential vectors dE/dm (20 per atom)
 template <class R>
 void launch_cov_electrostatics(hipStream_t s, int n, PairAtoms<R> a, const R* mu, PairParams<R> p, acc_t* force,
-                               acc_t* torque, acc_t* energy);
+                               acc_t* torque, acc_t* energy, acc_t* pot = nullptr);
+
+// ---- energy gradients ------------------------------------------------------------
+// The final pass of the gradients: launch_electrostatics_pairs plus the
+// potential vectors dE/dm of every pair (20 stored lab components per atom,
+// induced dipoles fixed) into pot.  all_general: every pair takes the general
+// formula, so that bare-charge sites get all 20 components (else the charge
+// component only).
+template <class R>
+void launch_electrostatics_pot(hipStream_t s, int maxtiles, const int4* act, const int* nact, const int* rows,
+                               const int* owner, const unsigned long long* tmask, const unsigned long long* tmaskcc,
+                               const int* chg, PairAtoms<R> a, const R* mu, PairParams<R> p, const double4* pos64,
+                               const double* lab64, PairParams<double> p64, acc_t* force, acc_t* torque,
+                               acc_t* energy, acc_t* pot, int all_general);
+// Thole damping derivatives h = u dE/du (thole_u_derivative) of the pairs of
+// polarisable sites.  hd[i]: sum of h over the pairs of i (dE/ddamp_i =
+// -hd_i / damp_i); hdef[i]: the same over the pairs at the default Thole
+// width, so that dE/d(default width) = sum_i hdef_i / (2 width); ht[i]: sum
+// of h / a over the pairs whose width a is thole_i + thole_j (dE/dthole_i).
+// List kernel: the polarisable atoms atoms[0..natoms) and their polarisable
+// neighbours (each pair from both sides); covalent kernel: the scaled
+// partner pairs, each once.
+template <class R>
+void launch_nl_thole(hipStream_t s, int natoms, const int* atoms, NbrList l, PairAtoms<R> a, const R* mu,
+                     PairParams<R> p, acc_t* hd, acc_t* hdef);
+template <class R>
+void launch_cov_thole(hipStream_t s, int n, PairAtoms<R> a, const R* mu, PairParams<R> p, acc_t* hd, acc_t* hdef,
+                      acc_t* ht);
+// Reciprocal and self parts of the potential vectors (kernels_atoms.hip);
+// pme = 0 leaves out the reciprocal part (NoCutoff, where alpha = 0 removes
+// the self part too).
+template <class R>
+void launch_recip_potential(hipStream_t s, int n, int pme, const double* phi, const R* phidp, const double* lab,
+                            const double* mu, FracT<double> A, double alpha, acc_t* pot);
+// Total field E0 + T mu (mutual != 0) or E0: the real-space part of T mu in
+// acc, its reciprocal part from phidp, the self term selfc mu.
+template <class R>
+void launch_total_field(hipStream_t s, int n, int mutual, const double* E0, const acc_t* acc, const R* phidp,
+                        FracT<double> A, const double* mu, double selfc, double* out);
+// The per-atom parameter gradients (caller's order): gmol [20 n] (molecular
+// record, internal component order), gpol [3 n], gthole [n]; *gdef += the
+// default Thole width's.
+void launch_param_gradients(hipStream_t s, int n, int all_sites, const int* perm, Box<double> box, const double4* pos,
+                            const int4* axes, const double* mol, const double* pol, const double* damp,
+                            const int* chg, const acc_t* pot, const double* Etot, const acc_t* hd,
+                            const acc_t* hdef, const acc_t* ht, double default_thole, double* gmol, double* gpol,
+                            double* gthole, acc_t* gdef);
 
 }  // namespace mpid

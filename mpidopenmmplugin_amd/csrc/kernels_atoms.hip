@@ -45,9 +45,11 @@ __global__ void k_morton_keys(int n, const double* __restrict__ pos, Box<double>
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     unsigned c[3];
+    const D3 fr = to_frac(mk(pos[3 * (size_t)i], pos[3 * (size_t)i + 1], pos[3 * (size_t)i + 2]), box);
+    const double fs[3] = {fr.x, fr.y, fr.z};
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        double f = pos[3 * (size_t)i + k] * box.inv[k];
+        double f = fs[k];
         f -= floor(f);
         int q = (int)(f * 1024.0);
         c[k] = (unsigned)min(max(q, 0), 1023);
@@ -70,13 +72,17 @@ __global__ void k_gather_sorted(int n, const int* __restrict__ perm, const int* 
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const int a = perm[s];
-    double x[3];
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        double v = pos[3 * (size_t)a + k];
-        v -= box.L[k] * floor(v * box.inv[k]);
-        x[k] = v;
+    // wrapped into the cell (fractional coordinates in [0, 1)); without
+    // periodicity (zero box) the positions stay as they are
+    D3 w = mk(pos[3 * (size_t)a], pos[3 * (size_t)a + 1], pos[3 * (size_t)a + 2]);
+    if (box.L[0] > 0.0) {
+        D3 f = to_frac(w, box);
+        f.x -= floor(f.x);
+        f.y -= floor(f.y);
+        f.z -= floor(f.z);
+        w = from_frac(f, box);
     }
+    const double x[3] = {w.x, w.y, w.z};
     spos[s] = make_double4(x[0], x[1], x[2], mol[(size_t)a * kMP]);
     for (int k = 0; k < kMP; k++) smol[(size_t)s * kMP + k] = mol[(size_t)a * kMP + k];
     int4 ax = axes[a];
@@ -246,14 +252,15 @@ __device__ __forceinline__ D3 sym_mul(const double* S, D3 v) {
 
 template <class R>
 __global__ void k_fixed_field_finish(int n, const R* __restrict__ phi, const double* __restrict__ lab,
-                                     const acc_t* __restrict__ acc, double3 g, double selfc, double* E0) {
+                                     const acc_t* __restrict__ acc, FracT<double> A, double selfc, double* E0) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const R* ph = phi + (size_t)s * 35;
     const double* m = lab + (size_t)s * kMP;
-    E0[3 * (size_t)s + 0] = -g.x * (double)ph[1] + selfc * m[1] + acc_get(acc[3 * (size_t)s + 0]);
-    E0[3 * (size_t)s + 1] = -g.y * (double)ph[2] + selfc * m[2] + acc_get(acc[3 * (size_t)s + 1]);
-    E0[3 * (size_t)s + 2] = -g.z * (double)ph[3] + selfc * m[3] + acc_get(acc[3 * (size_t)s + 2]);
+    const D3 c1 = cart_vector(mk((double)ph[1], (double)ph[2], (double)ph[3]), A);
+    E0[3 * (size_t)s + 0] = -c1.x + selfc * m[1] + acc_get(acc[3 * (size_t)s + 0]);
+    E0[3 * (size_t)s + 1] = -c1.y + selfc * m[2] + acc_get(acc[3 * (size_t)s + 1]);
+    E0[3 * (size_t)s + 2] = -c1.z + selfc * m[3] + acc_get(acc[3 * (size_t)s + 2]);
 }
 
 __global__ void k_solver_init(int n, const double* __restrict__ S, const double* __restrict__ E0, double* b,
@@ -488,35 +495,41 @@ __global__ void k_cg_direction_apply(int n, const double* __restrict__ S, const 
     mu[3 * (size_t)s + 2] = (R)v.z;
 }
 
+template <class R>
+__device__ void cart_potential(const R* ph, const FracT<double>& A, double* c1, double* c2, double* c3);
+
 // ---- Extrapolated (OPT) polarization --------------------------------------
 // convergeInduceDipolesByExtrapolation, MPIDReferenceForce.cpp:1125-1180.
 // One order: the field of mu_(m) at every atom, E = real space (acc) +
 // reciprocal (-A phi_1) + selfc mu_(m), and its gradient G = real space
-// (gacc) - A phi_2 A (A = diag(gscale), rectangular boxes; :4094-4127), kept
+// (gacc) - A phi_2 A^T (A the grid-coordinate transform; :4094-4127), kept
 // for the response forces; then mu_(m+1) = alpha_lab E, in FP64 and in the
 // precision of the next order's kernels.
 template <class R>
 __global__ void k_extrap_order(int n, const acc_t* __restrict__ field, const acc_t* __restrict__ gacc,
-                               const R* __restrict__ phidp, double3 gs, double selfc,
+                               const R* __restrict__ phidp, FracT<double> A, double selfc,
                                const double* __restrict__ labpol, const double* __restrict__ mu, double* E,
                                double* G, double* mu_next, R* mu_next_r) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const R* ph = phidp + 35 * (size_t)s;
     const size_t i3 = 3 * (size_t)s, i6 = 6 * (size_t)s;
+    double c1[3], c2[6], c3[10];
+    cart_potential<R>(ph, A, c1, c2, c3);
     D3 e;
-    e.x = acc_get(field[i3]) - gs.x * (double)ph[1] + selfc * mu[i3];
-    e.y = acc_get(field[i3 + 1]) - gs.y * (double)ph[2] + selfc * mu[i3 + 1];
-    e.z = acc_get(field[i3 + 2]) - gs.z * (double)ph[3] + selfc * mu[i3 + 2];
+    e.x = acc_get(field[i3]) - c1[0] + selfc * mu[i3];
+    e.y = acc_get(field[i3 + 1]) - c1[1] + selfc * mu[i3 + 1];
+    e.z = acc_get(field[i3 + 2]) - c1[2] + selfc * mu[i3 + 2];
     E[i3] = e.x;
     E[i3 + 1] = e.y;
     E[i3 + 2] = e.z;
-    G[i6 + 0] = acc_get(gacc[i6 + 0]) - gs.x * gs.x * (double)ph[4];
-    G[i6 + 1] = acc_get(gacc[i6 + 1]) - gs.y * gs.y * (double)ph[5];
-    G[i6 + 2] = acc_get(gacc[i6 + 2]) - gs.z * gs.z * (double)ph[6];
-    G[i6 + 3] = acc_get(gacc[i6 + 3]) - gs.x * gs.y * (double)ph[7];
-    G[i6 + 4] = acc_get(gacc[i6 + 4]) - gs.x * gs.z * (double)ph[8];
-    G[i6 + 5] = acc_get(gacc[i6 + 5]) - gs.y * gs.z * (double)ph[9];
+    // gradient order xx yy zz xy xz yz; c2 is XX XY XZ YY YZ ZZ
+    G[i6 + 0] = acc_get(gacc[i6 + 0]) - c2[0];
+    G[i6 + 1] = acc_get(gacc[i6 + 1]) - c2[3];
+    G[i6 + 2] = acc_get(gacc[i6 + 2]) - c2[5];
+    G[i6 + 3] = acc_get(gacc[i6 + 3]) - c2[1];
+    G[i6 + 4] = acc_get(gacc[i6 + 4]) - c2[2];
+    G[i6 + 5] = acc_get(gacc[i6 + 5]) - c2[4];
     const D3 m = sym_mul(labpol + i6, e);
     mu_next[i3] = m.x;
     mu_next[i3 + 1] = m.y;
@@ -770,30 +783,75 @@ __device__ D3 multipole_torque(const double* d, const double* Q, const double* O
     return mk(t[0], t[1], t[2]);
 }
 
-// Cartesian derivative tensors of the potential from fractional ones
-// (rectangular box: each derivative index scales by n_k/L_k).
+// Cartesian derivative tensors of the potential from grid-coordinate ones:
+// c1_j = sum_k A_jk phi_k, c2_ij = sum_kl A_ik A_jl phi_kl, c3_ijn =
+// sum_klm A_ik A_jl A_nm phi_klm (transformPotentialToCartesianCoordinates,
+// MPIDReferenceForce.cpp:3171-3267); c2 in the order XX XY XZ YY YZ ZZ, c3 in
+// the octopole order.  A rectangular box has A diagonal.
 template <class R>
-__device__ void cart_potential(const R* ph, const double g[3], double* c1, double* c2, double* c3) {
-    for (int a = 0; a < 3; a++) {
-        int e[3] = {0, 0, 0};
-        e[a]++;
-        c1[a] = g[a] * (double)ph[phi_idx(e[0], e[1], e[2])];
-    }
-    for (int a = 0; a < 3; a++)
-        for (int b = a; b < 3; b++) {
+__device__ void cart_potential(const R* ph, const FracT<double>& A, double* c1, double* c2, double* c3) {
+    const double a[3][3] = {{A.a00, 0.0, 0.0}, {A.a10, A.a11, 0.0}, {A.a20, A.a21, A.a22}};
+    if (!A.tri) {
+        const double g[3] = {A.a00, A.a11, A.a22};
+        for (int i = 0; i < 3; i++) {
             int e[3] = {0, 0, 0};
-            e[a]++;
-            e[b]++;
-            c2[i2(a, b)] = g[a] * g[b] * (double)ph[phi_idx(e[0], e[1], e[2])];
+            e[i]++;
+            c1[i] = g[i] * (double)ph[phi_idx(e[0], e[1], e[2])];
         }
-    for (int a = 0; a < 3; a++)
-        for (int b = a; b < 3; b++)
-            for (int c = b; c < 3; c++) {
+        for (int i = 0; i < 3; i++)
+            for (int j = i; j < 3; j++) {
                 int e[3] = {0, 0, 0};
-                e[a]++;
-                e[b]++;
-                e[c]++;
-                c3[i3(a, b, c)] = g[a] * g[b] * g[c] * (double)ph[phi_idx(e[0], e[1], e[2])];
+                e[i]++;
+                e[j]++;
+                c2[i2(i, j)] = g[i] * g[j] * (double)ph[phi_idx(e[0], e[1], e[2])];
+            }
+        for (int i = 0; i < 3; i++)
+            for (int j = i; j < 3; j++)
+                for (int k = j; k < 3; k++) {
+                    int e[3] = {0, 0, 0};
+                    e[i]++;
+                    e[j]++;
+                    e[k]++;
+                    c3[i3(i, j, k)] = g[i] * g[j] * g[k] * (double)ph[phi_idx(e[0], e[1], e[2])];
+                }
+        return;
+    }
+    double p1[3], p2[3][3], p3[3][3][3];
+    for (int k = 0; k < 3; k++) {
+        int e[3] = {0, 0, 0};
+        e[k]++;
+        p1[k] = (double)ph[phi_idx(e[0], e[1], e[2])];
+        for (int l = 0; l < 3; l++) {
+            int f[3] = {e[0], e[1], e[2]};
+            f[l]++;
+            p2[k][l] = (double)ph[phi_idx(f[0], f[1], f[2])];
+            for (int m = 0; m < 3; m++) {
+                int h[3] = {f[0], f[1], f[2]};
+                h[m]++;
+                p3[k][l][m] = (double)ph[phi_idx(h[0], h[1], h[2])];
+            }
+        }
+    }
+    for (int i = 0; i < 3; i++) {
+        double v = 0;
+        for (int k = 0; k < 3; k++) v += a[i][k] * p1[k];
+        c1[i] = v;
+    }
+    for (int i = 0; i < 3; i++)
+        for (int j = i; j < 3; j++) {
+            double v = 0;
+            for (int k = 0; k < 3; k++)
+                for (int l = 0; l < 3; l++) v += a[i][k] * a[j][l] * p2[k][l];
+            c2[i2(i, j)] = v;
+        }
+    for (int i = 0; i < 3; i++)
+        for (int j = i; j < 3; j++)
+            for (int n = j; n < 3; n++) {
+                double v = 0;
+                for (int k = 0; k < 3; k++)
+                    for (int l = 0; l < 3; l++)
+                        for (int m = 0; m < 3; m++) v += a[i][k] * a[j][l] * a[n][m] * p3[k][l][m];
+                c3[i3(i, j, n)] = v;
             }
 }
 
@@ -801,13 +859,12 @@ template <class R>
 __global__ __launch_bounds__(128) void k_recip_atoms(int n, int mutual, const double* __restrict__ phi,
                                                       const R* __restrict__ phidp, const double* __restrict__ lab,
                                                       const double* __restrict__ mu, const int* __restrict__ aniso,
-                                                      double3 gs, double alpha, acc_t* force, acc_t* torque,
+                                                      FracT<double> A, double alpha, acc_t* force, acc_t* torque,
                                                       acc_t* energy) {
     __shared__ double sh[128];
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     double en = 0;
     if (s < n) {
-        const double g[3] = {gs.x, gs.y, gs.z};
         const double* m = lab + (size_t)s * kMP;
         const double* ph = phi + (size_t)s * 35;
         const R* pd = phidp + (size_t)s * 35;
@@ -815,14 +872,7 @@ __global__ __launch_bounds__(128) void k_recip_atoms(int n, int mutual, const do
         const bool an = aniso[s] != 0;
         // fractional multipoles
         double mf[20];
-        for (int k = 0; k < 20; k++) {
-            int e[3];
-            comp_exp(k, e);
-            double sc = comp_mult(k);
-            for (int a = 0; a < 3; a++)
-                for (int p = 0; p < e[a]; p++) sc *= g[a];
-            mf[k] = sc * m[k];
-        }
+        frac_moments<double>(m, A, mf);
         // fixed: energy 1/2 eps M.phi, force -eps grad, torque
         double ef = 0, f[3] = {0, 0, 0}, fi[3] = {0, 0, 0};
         for (int k = 0; k < 20; k++) {
@@ -838,7 +888,8 @@ __global__ __launch_bounds__(128) void k_recip_atoms(int n, int mutual, const do
             }
         }
         // induced: energy 1/4 eps 2 u'.grad phi, force
-        const double uf[3] = {g[0] * u[0], g[1] * u[1], g[2] * u[2]};
+        const D3 ufv = frac_vector(mk(u[0], u[1], u[2]), A);
+        const double uf[3] = {ufv.x, ufv.y, ufv.z};
         double ei = 0;
         for (int k = 0; k < 3; k++) {
             int e[3] = {0, 0, 0};
@@ -853,8 +904,10 @@ __global__ __launch_bounds__(128) void k_recip_atoms(int n, int mutual, const do
                 if (mutual) fi[a] += 2.0 * uf[k] * (double)pd[id];
             }
         }
-        double F[3];
-        for (int a = 0; a < 3; a++) F[a] = -g[a] * (kElectric * f[a] + 0.5 * kElectric * fi[a]);
+        const D3 Fc = cart_vector(mk(kElectric * f[0] + 0.5 * kElectric * fi[0], kElectric * f[1] + 0.5 * kElectric * fi[1],
+                                     kElectric * f[2] + 0.5 * kElectric * fi[2]),
+                                  A);
+        const double F[3] = {-Fc.x, -Fc.y, -Fc.z};
         // torques
         double c1[3], c2[6], c3[10];
         double dd[3] = {m[1], m[2], m[3]};
@@ -863,7 +916,7 @@ __global__ __launch_bounds__(128) void k_recip_atoms(int n, int mutual, const do
             dd[1] += u[1];
             dd[2] += u[2];
         }
-        cart_potential<double>(ph, g, c1, c2, c3);
+        cart_potential<double>(ph, A, c1, c2, c3);
         D3 T = multipole_torque(dd, m + 4, m + 10, c1, c2, c3) * kElectric;
         double di[3] = {m[1], m[2], m[3]};
         if (mutual && an) {
@@ -871,7 +924,7 @@ __global__ __launch_bounds__(128) void k_recip_atoms(int n, int mutual, const do
             di[1] += u[1];
             di[2] += u[2];
         }
-        cart_potential<R>(pd, g, c1, c2, c3);
+        cart_potential<R>(pd, A, c1, c2, c3);
         T = T + multipole_torque(di, m + 4, m + 10, c1, c2, c3) * kElectric;
         // self energy and torque
         const double sqp = 1.7320508075688772;
@@ -910,11 +963,208 @@ __global__ __launch_bounds__(128) void k_recip_atoms(int n, int mutual, const do
 }
 
 // ---------------------------------------------------------------------------
+// energy gradients (per atom)
+// ---------------------------------------------------------------------------
+// Reciprocal and self parts of the potential vector dE/dm of each atom (the
+// 20 stored lab components, induced dipoles fixed), added to pot.  The
+// permanent reciprocal energy is 1/2 eps sum mf.phi over the fractional
+// moments mf_k = c_k m_k, and the induced dipoles' reciprocal interaction
+// with the moments eps sum mf.phidp (phidp: the potential of the induced
+// dipoles), so dE/dm_k = eps c_k (phi + phidp) at the index of component k.
+// Self terms: the derivatives of calculatePmeSelfEnergy's expression
+// (MPIDReferenceForce.cpp:4283-4318, as in k_recip_atoms) with the dipole-
+// induced dipole product counted twice, as -mu.E_self carries it at fixed
+// induced dipoles.
+template <class R>
+__global__ __launch_bounds__(128) void k_recip_potential(int n, int pme, const double* __restrict__ phi,
+                                                          const R* __restrict__ phidp, const double* __restrict__ lab,
+                                                          const double* __restrict__ mu, FracT<double> A,
+                                                          double alpha, acc_t* pot) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const double* m = lab + (size_t)s * kMP;
+    double v[kMP];
+    for (int k = 0; k < kMP; k++) v[k] = 0.0;
+    if (pme) {
+        // the Cartesian derivatives of phi + phidp, times the number of index
+        // orderings each stored component stands for
+        const double* ph = phi + (size_t)s * 35;
+        const R* pd = phidp + (size_t)s * 35;
+        double c1[3], c2[6], c3[10], d1[3], d2[6], d3[10];
+        cart_potential<double>(ph, A, c1, c2, c3);
+        cart_potential<R>(pd, A, d1, d2, d3);
+        v[0] = kElectric * (ph[0] + (double)pd[0]);
+        for (int k = 0; k < 3; k++) v[1 + k] = kElectric * (c1[k] + d1[k]);
+        for (int k = 0; k < 6; k++) v[4 + k] = kElectric * comp_mult(4 + k) * (c2[k] + d2[k]);
+        for (int k = 0; k < 10; k++) v[10 + k] = kElectric * comp_mult(10 + k) * (c3[k] + d3[k]);
+    }
+    if (alpha > 0.0) {
+        const double c = -alpha * kElectric / kSqrtPi, a2 = alpha * alpha;
+        v[0] += 2.0 * c * m[0];
+        for (int a = 0; a < 3; a++) v[1 + a] += c * (4.0 / 3.0) * a2 * (m[1 + a] + mu[3 * (size_t)s + a]);
+        // the spherical quadrupole and octopole norms of the self energy
+        const double s3 = 1.7320508075688772;
+        const double sq[5] = {3.0 * m[9], 2.0 * s3 * m[6], 2.0 * s3 * m[8], s3 * (m[4] - m[7]), 2.0 * s3 * m[5]};
+        const double r32 = 1.2247448713915890, r35 = 0.7745966692414834, r110 = 0.31622776601683794;
+        const double so[7] = {15.0 * m[19], 15.0 * r32 * m[15], 15.0 * r32 * m[18], 15.0 * r35 * (m[12] - m[17]),
+                              30.0 * r35 * m[14], 15.0 * r110 * (m[10] - 3.0 * m[13]),
+                              15.0 * r110 * (3.0 * m[11] - m[16])};
+        const double cq = c * (4.0 / 15.0) * a2 * a2 * 2.0, co = c * (8.0 / 105.0) * a2 * a2 * a2 * 2.0;
+        v[9] += cq * 3.0 * sq[0];
+        v[6] += cq * 2.0 * s3 * sq[1];
+        v[8] += cq * 2.0 * s3 * sq[2];
+        v[4] += cq * s3 * sq[3];
+        v[7] -= cq * s3 * sq[3];
+        v[5] += cq * 2.0 * s3 * sq[4];
+        v[19] += co * 15.0 * so[0];
+        v[15] += co * 15.0 * r32 * so[1];
+        v[18] += co * 15.0 * r32 * so[2];
+        v[12] += co * 15.0 * r35 * so[3];
+        v[17] -= co * 15.0 * r35 * so[3];
+        v[14] += co * 30.0 * r35 * so[4];
+        v[10] += co * 15.0 * r110 * so[5];
+        v[13] -= co * 45.0 * r110 * so[5];
+        v[11] += co * 45.0 * r110 * so[6];
+        v[16] -= co * 15.0 * r110 * so[6];
+    }
+    for (int k = 0; k < kMP; k++)
+        if (v[k] != 0.0) acc_add(&pot[kMP * (size_t)s + k], v[k]);
+}
+
+// The total field at each atom, E0 + T mu (mutual polarization: the real-space
+// pairs in acc, the reciprocal field -grad phidp and the self term), or E0
+// alone (direct polarization, where mu = alpha E0).
+template <class R>
+__global__ void k_total_field(int n, int mutual, const double* __restrict__ E0, const acc_t* __restrict__ acc,
+                              const R* __restrict__ phidp, FracT<double> A, const double* __restrict__ mu,
+                              double selfc, double* out) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const R* pd = phidp + (size_t)s * 35;
+    const D3 gp = cart_vector(mk((double)pd[1], (double)pd[2], (double)pd[3]), A);
+    const double grad[3] = {gp.x, gp.y, gp.z};
+    for (int k = 0; k < 3; k++) {
+        const size_t id = 3 * (size_t)s + k;
+        double e = E0[id];
+        if (mutual) e += acc_get(acc[id]) - grad[k] + selfc * mu[id];
+        out[id] = e;
+    }
+}
+
+// Per-atom gradients of the energy with respect to the force's parameters,
+// written in the caller's atom order (perm: sorted -> caller):
+//  gmol[20]  dE/dm of the molecular-frame record (internal order), the lab
+//            potential vector pot taken back through the rotation of k_rotate
+//            and the chirality flip (a transpose: lab = L m);
+//  gpol[3]   dE/dalpha_k of the molecular-frame polarisabilities: at fixed
+//            induced dipoles the energy holds 1/2 mu alpha_lab^-1 mu, whose
+//            derivative is -1/2 eps E E^T (E the total field, alpha_lab E =
+//            mu), projected on molecular axis k; plus the damping factor
+//            (mean alpha)^(1/6) through the Thole terms (-hd / damp);
+//  gthole    dE/dthole from ht.
+// Sites without moments and polarisability (bare) get the charge component
+// only unless all_sites: the other pair kernels leave their components out.
+// hdef is reduced into *gdef: dE/d(default Thole width).
+__global__ __launch_bounds__(128) void k_param_gradients(int n, int all_sites, const int* __restrict__ perm,
+                                                          Box<double> box, const double4* __restrict__ pos,
+                                                          const int4* __restrict__ axes, const double* __restrict__ mol,
+                                                          const double* __restrict__ pol, const double* __restrict__ damp,
+                                                          const int* __restrict__ chg, const acc_t* __restrict__ pot,
+                                                          const double* __restrict__ Etot, const acc_t* __restrict__ hd,
+                                                          const acc_t* __restrict__ hdef, const acc_t* __restrict__ ht,
+                                                          double default_thole, double* gmol, double* gpol,
+                                                          double* gthole, acc_t* gdef) {
+    __shared__ double sh[128];
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    double hsum = 0.0;
+    if (s < n) {
+        const int a = perm[s];
+        const int4 ax = axes[s];
+        const bool bare = chg[s] != 0 && !all_sites;
+        double v[kMP];
+        for (int k = 0; k < kMP; k++) v[k] = (bare && k > 0) ? 0.0 : acc_get(pot[kMP * (size_t)s + k]);
+        D3 e[3];
+        frame_axes(s, pos, ax, box, e);
+        double E[3][3];
+        for (int k = 0; k < 3; k++) {
+            E[k][0] = e[k].x;
+            E[k][1] = e[k].y;
+            E[k][2] = e[k].z;
+        }
+        // transpose of the rotation in k_rotate
+        double g[kMP];
+        g[0] = v[0];
+        for (int k = 0; k < 3; k++) {
+            double t = 0;
+            for (int b = 0; b < 3; b++) t += E[k][b] * v[1 + b];
+            g[1 + k] = t;
+        }
+        for (int c = 4; c < kMP; c++) g[c] = 0.0;
+        for (int k = 0; k < 3; k++)
+            for (int l = 0; l < 3; l++) {
+                double t = 0;
+                for (int p = 0; p < 3; p++)
+                    for (int q = p; q < 3; q++) t += E[k][p] * E[l][q] * v[4 + i2(p, q)];
+                g[4 + i2(k, l)] += t;
+            }
+        for (int k = 0; k < 3; k++)
+            for (int l = 0; l < 3; l++)
+                for (int w = 0; w < 3; w++) {
+                    double t = 0;
+                    for (int p = 0; p < 3; p++)
+                        for (int q = p; q < 3; q++)
+                            for (int r = q; r < 3; r++) t += E[k][p] * E[l][q] * E[w][r] * v[10 + i3(p, q, r)];
+                    g[10 + i3(k, l, w)] += t;
+                }
+        // chirality (ZThenX with a y atom): k_rotate flips these components
+        if (ax.w == 0 && ax.z >= 0) {
+            const D3 py = d3(pos[ax.z]);
+            D3 ad = min_image(d3(pos[s]) - py, box), bd = min_image(d3(pos[ax.x]) - py, box),
+               cd = min_image(d3(pos[ax.y]) - py, box);
+            if (dot(cross(bd, cd), ad) < 0.0) {
+                g[2] = -g[2];
+                g[5] = -g[5];
+                g[8] = -g[8];
+            }
+        }
+        for (int k = 0; k < kMP; k++) gmol[kMP * (size_t)a + k] = g[k];
+        // polarisabilities
+        const double p0 = pol[3 * (size_t)s], p1 = pol[3 * (size_t)s + 1], p2 = pol[3 * (size_t)s + 2];
+        const bool polar = p0 != 0.0 || p1 != 0.0 || p2 != 0.0;
+        const D3 Et = mk(Etot[3 * (size_t)s], Etot[3 * (size_t)s + 1], Etot[3 * (size_t)s + 2]);
+        const double pbar = (p0 + p1 + p2) / 3.0;
+        const double dd = damp[s];
+        // dE/ddamp = -hd / damp; ddamp/dalpha_k = damp / (18 mean alpha)
+        const double ddamp = (dd > 0.0 && pbar > 0.0) ? -acc_get(hd[s]) / (18.0 * pbar) : 0.0;
+        for (int k = 0; k < 3; k++) {
+            const double ek = dot(e[k], Et);
+            gpol[3 * (size_t)a + k] = (polar || all_sites) ? -0.5 * kElectric * ek * ek + ddamp : 0.0;
+        }
+        gthole[a] = acc_get(ht[s]);
+        hsum = acc_get(hdef[s]);
+    }
+    sh[threadIdx.x] = hsum;
+    __syncthreads();
+    for (int off = 64; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && sh[0] != 0.0) acc_add(gdef, sh[0] / (2.0 * default_thole));
+}
+
+// ---------------------------------------------------------------------------
 // torque -> forces on the frame atoms
 // ---------------------------------------------------------------------------
+// exact_threefold: ThreeFold frames by the exact chain rule of the frame
+// (z = s/|s|, s = u + v + w the sum of the unit vectors to the z, x and y
+// atoms; x = t/|t|, t = v - (v.z) z).  Moving an atom turns the frame by
+// dtheta = z x dz + (y.dx) z, so with tau = -dE/dtheta the force on atom a
+// is (dz/dr_a)^T (tau x z) + (tau.z) (dx/dr_a)^T y.  Without it, the
+// reference's approximate projection (MPIDReferenceForce.cpp:2068-2097).
 __global__ __launch_bounds__(128) void k_torque_to_force(int n, Box<double> box, const double4* __restrict__ pos,
                                                           const int4* __restrict__ axes,
-                                                          const acc_t* __restrict__ torque, acc_t* force) {
+                                                          const acc_t* __restrict__ torque, acc_t* force,
+                                                          int exact_threefold) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const int4 ax = axes[s];
@@ -924,6 +1174,32 @@ __global__ __launch_bounds__(128) void k_torque_to_force(int n, Box<double> box,
                      acc_get(torque[3 * (size_t)s + 2]));
     const D3 pI = d3(pos[s]);
     const int iz = ax.x, ix = ax.y, iy = ax.z;
+    if (t == 3 && exact_threefold && iy >= 0) {
+        D3 u = min_image(d3(pos[iz]) - pI, box), v = min_image(d3(pos[ix]) - pI, box),
+           w = min_image(d3(pos[iy]) - pI, box);
+        const double nu = dnorm(u), nv = dnorm(v), nw = dnorm(w);
+        D3 z = u + v + w;
+        const double ns = dnorm(z);
+        D3 x = v - z * dot(v, z);
+        const double nt = dnorm(x);
+        const D3 y = cross(z, x);
+        const D3 c = y * (dot(tq, z) / nt);          // through x: on t, hence on v and on z
+        const D3 bz = cross(tq, z) - c * dot(v, z);  // everything that moves with z
+        const D3 gs = (bz - z * dot(z, bz)) * (1.0 / ns);
+        const D3 gx = gs + c;
+        const D3 FZ = (gs - u * dot(u, gs)) * (1.0 / nu);
+        const D3 FX = (gx - v * dot(v, gx)) * (1.0 / nv);
+        const D3 FY = (gs - w * dot(w, gs)) * (1.0 / nw);
+        const D3 FI = (FZ + FX + FY) * -1.0;
+        const int at[4] = {iz, ix, iy, s};
+        const D3 fs[4] = {FZ, FX, FY, FI};
+        for (int q = 0; q < 4; q++) {
+            acc_add(&force[3 * (size_t)at[q] + 0], fs[q].x);
+            acc_add(&force[3 * (size_t)at[q] + 1], fs[q].y);
+            acc_add(&force[3 * (size_t)at[q] + 2], fs[q].z);
+        }
+        return;
+    }
     D3 U = min_image(d3(pos[iz]) - pI, box);
     const double nU = dnorm(U);
     D3 V;
@@ -1078,9 +1354,30 @@ void launch_convert(hipStream_t s, int n, const double4* pos, const double* lab,
                     const double* damp, R4<R>* rpos, R* rlab, R2<R>* rtd) {
     hipLaunchKernelGGL(k_convert<R>, blocks_for(n, 256), dim3(256), 0, s, n, pos, lab, thole, damp, rpos, rlab, rtd);
 }
+template <class R>
+void launch_recip_potential(hipStream_t s, int n, int pme, const double* phi, const R* phidp, const double* lab,
+                            const double* mu, FracT<double> A, double alpha, acc_t* pot) {
+    hipLaunchKernelGGL(k_recip_potential<R>, blocks_for(n, 128), dim3(128), 0, s, n, pme, phi, phidp, lab, mu, A,
+                       alpha, pot);
+}
+template <class R>
+void launch_total_field(hipStream_t s, int n, int mutual, const double* E0, const acc_t* acc, const R* phidp,
+                        FracT<double> A, const double* mu, double selfc, double* out) {
+    hipLaunchKernelGGL(k_total_field<R>, blocks_for(n, 256), dim3(256), 0, s, n, mutual, E0, acc, phidp, A, mu, selfc,
+                       out);
+}
+void launch_param_gradients(hipStream_t s, int n, int all_sites, const int* perm, Box<double> box, const double4* pos,
+                            const int4* axes, const double* mol, const double* pol, const double* damp,
+                            const int* chg, const acc_t* pot, const double* Etot, const acc_t* hd,
+                            const acc_t* hdef, const acc_t* ht, double default_thole, double* gmol, double* gpol,
+                            double* gthole, acc_t* gdef) {
+    hipLaunchKernelGGL(k_param_gradients, blocks_for(n, 128), dim3(128), 0, s, n, all_sites, perm, box, pos, axes, mol,
+                       pol, damp, chg, pot, Etot, hd, hdef, ht, default_thole, gmol, gpol, gthole, gdef);
+}
 void launch_torque_to_force(hipStream_t s, int n, Box<double> box, const double4* pos, const int4* axes,
-                            const acc_t* torque, acc_t* force) {
-    hipLaunchKernelGGL(k_torque_to_force, blocks_for(n, 128), dim3(128), 0, s, n, box, pos, axes, torque, force);
+                            const acc_t* torque, acc_t* force, int exact_threefold) {
+    hipLaunchKernelGGL(k_torque_to_force, blocks_for(n, 128), dim3(128), 0, s, n, box, pos, axes, torque, force,
+                       exact_threefold);
 }
 void launch_output_forces(hipStream_t s, int n, const int* perm, const acc_t* force, double* out, int accumulate) {
     hipLaunchKernelGGL(k_output_forces, blocks_for(n, 256), dim3(256), 0, s, n, perm, force, out, accumulate);
@@ -1099,9 +1396,9 @@ template <class R> void launch_fill(hipStream_t s, R* p, size_t count, R v) {
     hipLaunchKernelGGL(k_fill<R>, dim3(b), dim3(256), 0, s, p, count, v);
 }
 template <class R>
-void launch_fixed_field_finish(hipStream_t s, int n, const R* phi, const double* lab, const acc_t* acc, double3 g,
-                               double selfc, double* E0) {
-    hipLaunchKernelGGL(k_fixed_field_finish<R>, blocks_for(n, 256), dim3(256), 0, s, n, phi, lab, acc, g, selfc, E0);
+void launch_fixed_field_finish(hipStream_t s, int n, const R* phi, const double* lab, const acc_t* acc,
+                               FracT<double> A, double selfc, double* E0) {
+    hipLaunchKernelGGL(k_fixed_field_finish<R>, blocks_for(n, 256), dim3(256), 0, s, n, phi, lab, acc, A, selfc, E0);
 }
 void launch_solver_init(hipStream_t s, int n, const double* S, const double* E0, double* b, double* y) {
     hipLaunchKernelGGL(k_solver_init, blocks_for(n, 256), dim3(256), 0, s, n, S, E0, b, y);
@@ -1140,10 +1437,10 @@ void launch_cg_iteration(hipStream_t s, int n, const double* S, const acc_t* acc
     hipLaunchKernelGGL(k_cg_direction_apply<R>, blocks_for(n, 256), dim3(256), 0, s, n, S, scal, r, p, mu);
 }
 template <class R>
-void launch_extrap_order(hipStream_t s, int n, const acc_t* field, const acc_t* gacc, const R* phidp, double3 gs,
-                         double selfc, const double* labpol, const double* mu, double* E, double* G, double* mu_next,
-                         R* mu_next_r) {
-    hipLaunchKernelGGL(k_extrap_order<R>, blocks_for(n, 256), dim3(256), 0, s, n, field, gacc, phidp, gs, selfc,
+void launch_extrap_order(hipStream_t s, int n, const acc_t* field, const acc_t* gacc, const R* phidp,
+                         FracT<double> A, double selfc, const double* labpol, const double* mu, double* E, double* G,
+                         double* mu_next, R* mu_next_r) {
+    hipLaunchKernelGGL(k_extrap_order<R>, blocks_for(n, 256), dim3(256), 0, s, n, field, gacc, phidp, A, selfc,
                        labpol, mu, E, G, mu_next, mu_next_r);
 }
 template <class R>
@@ -1162,6 +1459,7 @@ void launch_potential_points(hipStream_t s, int n, int npts, const double4* pos,
     hipLaunchKernelGGL(k_potential_points, dim3((npts + kPotThreads - 1) / kPotThreads), dim3(kPotThreads), 0, s, n,
                        npts, pos, lab, mu, box, points, out);
 }
+
 // Staging memory of the allocation-free potential getter: the points
 // (3 doubles each) at [0, 3 kNoAllocMaxPoints), their potentials after them.
 // Every GPU the library runs on holds its own copy until the process ends.
@@ -1209,29 +1507,33 @@ void launch_sub(hipStream_t s, int m, const double* a, const double* b_, double*
 }
 template <class R>
 void launch_recip_atoms(hipStream_t s, int n, int mutual, const double* phi, const R* phidp, const double* lab,
-                        const double* mu, const int* aniso, double3 gscale, double alpha, acc_t* force, acc_t* torque,
-                        acc_t* energy) {
+                        const double* mu, const int* aniso, FracT<double> A, double alpha, acc_t* force,
+                        acc_t* torque, acc_t* energy) {
     hipLaunchKernelGGL(k_recip_atoms<R>, blocks_for(n, 128), dim3(128), 0, s, n, mutual, phi, phidp, lab, mu, aniso,
-                       gscale, alpha, force, torque, energy);
+                       A, alpha, force, torque, energy);
 }
 
 #define MPID_INSTANTIATE(R)                                                                                         \
     template void launch_convert<R>(hipStream_t, int, const double4*, const double*, const double*, const double*,  \
                                     R4<R>*, R*, R2<R>*);                                                            \
     template void launch_fill<R>(hipStream_t, R*, size_t, R);                                                        \
-    template void launch_fixed_field_finish<R>(hipStream_t, int, const R*, const double*, const acc_t*, double3,    \
+    template void launch_fixed_field_finish<R>(hipStream_t, int, const R*, const double*, const acc_t*, FracT<double>, \
                                                double, double*);                                                    \
     template void launch_apply_S<R>(hipStream_t, int, const double*, const double*, R*, double*);                   \
     template void launch_matvec_finish<R>(hipStream_t, int, const double*, const acc_t*, const R*, const R*, double, \
                                           const double*, double*);                                                  \
-    template void launch_extrap_order<R>(hipStream_t, int, const acc_t*, const acc_t*, const R*, double3, double,   \
+    template void launch_extrap_order<R>(hipStream_t, int, const acc_t*, const acc_t*, const R*, FracT<double>, double,   \
                                          const double*, const double*, double*, double*, double*, R*);              \
     template void launch_extrap_combine<R>(hipStream_t, int, const ExtrapCoefficients&, const double*, double*, R*); \
     template void launch_cg_iteration<R>(hipStream_t, int, const double*, const acc_t*, const R*, R*, double,       \
                                          double*, double*, double*, double*, double*, double*, double*, double);    \
     template void launch_recip_atoms<R>(hipStream_t, int, int, const double*, const R*, const double*, const double*,\
-                                        const int*, double3, double, acc_t*, acc_t*, acc_t*);                      \
-    template void launch_field_sum<R>(hipStream_t, int, const acc_t*, const R*, const R*, double, double*);
+                                        const int*, FracT<double>, double, acc_t*, acc_t*, acc_t*);                      \
+    template void launch_field_sum<R>(hipStream_t, int, const acc_t*, const R*, const R*, double, double*);                    \
+    template void launch_recip_potential<R>(hipStream_t, int, int, const double*, const R*, const double*,            \
+                                            const double*, FracT<double>, double, acc_t*);                                 \
+    template void launch_total_field<R>(hipStream_t, int, int, const double*, const acc_t*, const R*, FracT<double>,    \
+                                        const double*, double, double*);
 MPID_INSTANTIATE(float)
 MPID_INSTANTIATE(double)
 #undef MPID_INSTANTIATE

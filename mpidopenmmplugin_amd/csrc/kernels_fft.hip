@@ -21,6 +21,8 @@
 // Every pass loads a batch of T lines into LDS, runs Stockham stages of radix
 // 4, 2, 3 and 5 (twiddles staged in LDS) and writes the lines back.  Index
 // arithmetic uses precomputed multiply-shift division by the stage constants.
+#include <algorithm>
+
 #include "engine.h"
 
 namespace mpid {
@@ -294,22 +296,12 @@ __global__ __launch_bounds__(fft_threads<R>()) void k_fft_x_convolve(R2<R>* __re
     }
     __syncthreads();
     Cx<R>* res = fft_in_lds<R>(A, B, ax, T, W, R(-1));
-    // influence function exp(-pi^2 m^2 / alpha^2) / (pi V m^2 |b(m)|^2), m = 0 removed
-    const int mz = kz < (n2 + 1) / 2 ? kz : kz - n2;
-    const R hz = R(mz) * cp.inv[2];
-    const R bz = cp.moduli[n + n1 + kz];
+    // the influence function, from its table ([kz][ky][kx])
+    const R* gz = cp.infl + (size_t)kz * n1 * n;
     for (int idx = threadIdx.x; idx < T * n; idx += blockDim.x) {
         const int kx = idx >> Tshift, t = idx & (T - 1);
         const int y = y0 + t;
-        R et = R(0);
-        if (t < count && (kx | y | kz) != 0) {
-            const int mx = kx < (n + 1) / 2 ? kx : kx - n;
-            const int my = y < (n1 + 1) / 2 ? y : y - n1;
-            const R hx = R(mx) * cp.inv[0], hy = R(my) * cp.inv[1];
-            const R m2 = hx * hx + hy * hy + hz * hz;
-            const R den = m2 * cp.moduli[kx] * cp.moduli[n + y] * bz;
-            et = cp.sc * exp_r(-cp.expf * m2) / den;
-        }
+        const R et = t < count ? gz[(size_t)y * n + kx] : R(0);
         Cx<R>& v = res[t * n + kx];
         v.x *= et;
         v.y *= et;
@@ -370,21 +362,10 @@ __global__ __launch_bounds__(fft_threads<R>()) void k_fft_plane_small(R2<R>* __r
     __syncthreads();
     // x forward: n1 lines [y][kx]
     res = fft_in_lds<R>(other, res, ax0, n1, W0, R(-1));
-    // influence function exp(-pi^2 m^2 / alpha^2) / (pi V m^2 |b(m)|^2), m = 0 removed
-    const int mz = kz < (n2 + 1) / 2 ? kz : kz - n2;
-    const R hz = R(mz) * cp.inv[2];
-    const R bz = cp.moduli[n0 + n1 + kz];
+    // the influence function, from its table ([kz][ky][kx])
+    const R* gz = cp.infl + (size_t)kz * np;
     for (int idx = threadIdx.x; idx < np; idx += blockDim.x) {
-        const int y = idx / n0, kx = idx - y * n0;
-        R et = R(0);
-        if ((kx | y | kz) != 0) {
-            const int mx = kx < (n0 + 1) / 2 ? kx : kx - n0;
-            const int my = y < (n1 + 1) / 2 ? y : y - n1;
-            const R hx = R(mx) * cp.inv[0], hy = R(my) * cp.inv[1];
-            const R m2 = hx * hx + hy * hy + hz * hz;
-            const R den = m2 * cp.moduli[kx] * cp.moduli[n0 + y] * bz;
-            et = cp.sc * exp_r(-cp.expf * m2) / den;
-        }
+        const R et = gz[idx];
         res[idx].x *= et;
         res[idx].y *= et;
     }
@@ -555,23 +536,14 @@ __global__ __launch_bounds__(plane::kThreads) void k_fft_plane128_convolve(R2<fl
     __syncthreads();
     step8<false, true>(P, W, -1.f);
     __syncthreads();
-    // influence function at (kx, ky, kz), slots in digit-reversed order
+    // the influence function at (kx, ky, kz) from its table ([kz][ky][kx]),
+    // slots in digit-reversed order
     {
-        const int mz = kz < (n2 + 1) / 2 ? kz : kz - n2;
-        const float hz = float(mz) * cp.inv[2];
-        const float bz = cp.moduli[2 * N + kz];
+        const float* gz = cp.infl + (size_t)kz * N * N;
         for (int idx = threadIdx.x; idx < N * N; idx += kThreads) {
             const int sx = idx >> 7, sy = idx & 127;
             const int kx = (sx >> 3) + A * (sx & 7), ky = (sy >> 3) + A * (sy & 7);
-            float et = 0.f;
-            if ((kx | ky | kz) != 0) {
-                const int mx = kx < (N + 1) / 2 ? kx : kx - N;
-                const int my = ky < (N + 1) / 2 ? ky : ky - N;
-                const float hx = float(mx) * cp.inv[0], hy = float(my) * cp.inv[1];
-                const float m2 = hx * hx + hy * hy + hz * hz;
-                const float den = m2 * cp.moduli[kx] * cp.moduli[N + ky] * bz;
-                et = cp.sc * exp_r(-cp.expf * m2) / den;
-            }
+            const float et = gz[ky * N + kx];
             C& v = P[sx * kPitch + slot(sy)];
             v.x *= et;
             v.y *= et;
@@ -794,6 +766,49 @@ bool plane_pass<float>(hipStream_t s, const FftPlan<float>& p, R2<float>* S, int
     hipLaunchKernelGGL(k_fft_plane128_convolve, dim3(n2h), dim3(plane::kThreads), 0, s, S, p.tw[0], n2, cp);
     return true;
 }
+
+// The influence function of performMPIDReciprocalConvolution
+// (MPIDReferenceForce.cpp:3329-3366), exp(-pi^2 m^2 / alpha^2) /
+// (pi V m^2 |b(k)|^2) with m = box^-1 k, k folded to (-n/2, n/2], at every
+// point of the half spectrum, in FP64.  The reference multiplies the full
+// complex spectrum by it and keeps the real part of the inverse transform,
+// which is the same as multiplying by its symmetric part
+// G_s(k) = (G(k) + G(-k)) / 2.  G(-k) = G(k) except where k sits on a
+// Nyquist plane of a triclinic cell (the fold keeps +n/2, so m(-k) != -m(k))
+// and on the planes k = (n - 1) / 2 of an odd grid (the moduli's optimal-zeta
+// factor folds there asymmetrically, :2780-2806); the half-spectrum
+// transform needs the symmetric form, so the table holds G_s.
+__device__ double influence_at(int kx, int ky, int kz, const InflParams& ip, const double* __restrict__ mod) {
+    if ((kx | ky | kz) == 0) return 0.0;
+    const int mx = kx < (ip.n[0] + 1) / 2 ? kx : kx - ip.n[0];
+    const int my = ky < (ip.n[1] + 1) / 2 ? ky : ky - ip.n[1];
+    const int mz = kz < (ip.n[2] + 1) / 2 ? kz : kz - ip.n[2];
+    const double hx = mx * ip.r00, hy = mx * ip.r10 + my * ip.r11, hz = mx * ip.r20 + my * ip.r21 + mz * ip.r22;
+    const double m2 = hx * hx + hy * hy + hz * hz;
+    const double den = m2 * mod[kx] * mod[ip.n[0] + ky] * mod[ip.n[0] + ip.n[1] + kz];
+    return ip.sc * exp(-ip.expf * m2) / den;
+}
+
+template <class R>
+__global__ void k_influence_table(InflParams ip, const double* __restrict__ mod, R* __restrict__ out) {
+    const int n0 = ip.n[0], n1 = ip.n[1], n2 = ip.n[2], n2h = n2 / 2 + 1;
+    const size_t total = (size_t)n2h * n1 * n0;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (size_t)gridDim.x * blockDim.x) {
+        const int kx = (int)(idx % n0), ky = (int)((idx / n0) % n1), kz = (int)(idx / ((size_t)n0 * n1));
+        const double g = influence_at(kx, ky, kz, ip, mod);
+        const double gm = influence_at((n0 - kx) % n0, (n1 - ky) % n1, (n2 - kz) % n2, ip, mod);
+        out[idx] = (R)(0.5 * (g + gm));
+    }
+}
+
+template <class R> void launch_influence_table(hipStream_t s, InflParams ip, const double* moduli64, R* out) {
+    const size_t total = (size_t)(ip.n[2] / 2 + 1) * ip.n[1] * ip.n[0];
+    const int blocks = (int)std::min<size_t>((total + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_influence_table<R>, dim3(blocks), dim3(256), 0, s, ip, moduli64, out);
+}
+template void launch_influence_table<float>(hipStream_t, InflParams, const double*, float*);
+template void launch_influence_table<double>(hipStream_t, InflParams, const double*, double*);
 
 template <class R>
 void launch_recip_convolution(hipStream_t s, const FftPlan<R>& p, R* Q, R2<R>* S, ConvParams<R> cp) {

@@ -35,37 +35,78 @@ __device__ __forceinline__ size_t nl_index(int l, int k, int rows) {
     return ((size_t)(k / kNlSplit) * rows + l) * kNlSplit + (k % kNlSplit);
 }
 
-// Bounding boxes of blocks I and J within the cutoff (minimum image between
-// the box centres, with a little slack for the float rounding of the bounds).
-__device__ __forceinline__ bool blocks_within(float4 a0, float4 a1, float4 b0, float4 b1, const Box<float>& box,
-                                              float cutoff) {
-    const float ca[3] = {0.5f * (a0.x + a1.x), 0.5f * (a0.y + a1.y), 0.5f * (a0.z + a1.z)};
-    const float ha[3] = {0.5f * (a1.x - a0.x), 0.5f * (a1.y - a0.y), 0.5f * (a1.z - a0.z)};
-    const float cb[3] = {0.5f * (b0.x + b1.x), 0.5f * (b0.y + b1.y), 0.5f * (b0.z + b1.z)};
-    const float hb[3] = {0.5f * (b1.x - b0.x), 0.5f * (b1.y - b0.y), 0.5f * (b1.z - b0.z)};
-    float d2 = 0.f;
+// Whether an axis-aligned box of half-widths h, centred at separation d
+// from a point, comes within rc of that point under some periodic image:
+// the point-to-box gap of every image is at most the distance between any
+// point of the box and the given point, so a pair within rc passes.  A
+// rectangular cell wraps each axis alone.  A triclinic cell (reduced form)
+// tries every lattice shift k c + j b + i a that brings the z, then the y,
+// then the x separation within rc + h (one shift more on each side for the
+// float rounding of the bounds); such a shift exists for any pair within rc,
+// whichever image the pair's own minimum image picks.
+__device__ __forceinline__ bool gap_within(float dx, float dy, float dz, float hx, float hy, float hz,
+                                           const Box<float>& box, float rc) {
+    const float rc2 = rc * rc;
+    if (box.bx == 0.f && box.cx == 0.f && box.cy == 0.f) {
+        const float d[3] = {dx, dy, dz}, h[3] = {hx, hy, hz};
+        float d2 = 0.f;
 #pragma unroll
-    for (int k = 0; k < 3; k++) {
-        float d = cb[k] - ca[k];
-        d -= box.L[k] * floorf(d * box.inv[k] + 0.5f);
-        const float g = fmaxf(0.f, fabsf(d) - ha[k] - hb[k]);
-        d2 += g * g;
+        for (int k = 0; k < 3; k++) {
+            const float dk = d[k] - box.L[k] * floorf(d[k] * box.inv[k] + 0.5f);
+            const float g = fmaxf(0.f, fabsf(dk) - h[k]);
+            d2 += g * g;
+        }
+        return d2 <= rc2;
     }
-    const float rc = cutoff * 1.0001f + 1e-5f;
-    return d2 <= rc * rc;
+    const V3<float> d = min_image(mk(dx, dy, dz), box);
+    const int k0 = (int)floorf((-rc - hz - d.z) * box.inv[2]) - 1, k1 = (int)ceilf((rc + hz - d.z) * box.inv[2]) + 1;
+    for (int k = k0; k <= k1; k++) {
+        const float z = d.z + (float)k * box.L[2];
+        const float gz = fmaxf(0.f, fabsf(z) - hz);
+        if (gz * gz > rc2) continue;
+        const float y0 = d.y + (float)k * box.cy, x0 = d.x + (float)k * box.cx;
+        const int j0 = (int)floorf((-rc - hy - y0) * box.inv[1]) - 1, j1 = (int)ceilf((rc + hy - y0) * box.inv[1]) + 1;
+        for (int j = j0; j <= j1; j++) {
+            const float y = y0 + (float)j * box.L[1];
+            const float gy = fmaxf(0.f, fabsf(y) - hy);
+            if (gz * gz + gy * gy > rc2) continue;
+            const float x1 = x0 + (float)j * box.bx;
+            const int i0 = (int)floorf((-rc - hx - x1) * box.inv[0]) - 1,
+                      i1 = (int)ceilf((rc + hx - x1) * box.inv[0]) + 1;
+            for (int i = i0; i <= i1; i++) {
+                const float gx = fmaxf(0.f, fabsf(x1 + (float)i * box.L[0]) - hx);
+                if (gz * gz + gy * gy + gx * gx <= rc2) return true;
+            }
+        }
+    }
+    return false;
 }
 
-// One wavefront per block I.  Count mode (rows == nullptr) writes counts[I];
-// fill mode writes the row at rows[start[I] ...] in ascending J, and I into
-// owner[] at the same positions.
+// Bounding boxes of blocks I and J within the cutoff (the gap test above on
+// the separation of the box centres with the summed half-widths, with a
+// little slack for the float rounding of the bounds).
+__device__ __forceinline__ bool blocks_within(float4 a0, float4 a1, float4 b0, float4 b1, const Box<float>& box,
+                                              float cutoff) {
+    const float rc = cutoff * 1.0001f + 1e-5f;
+    return gap_within(0.5f * (b0.x + b1.x) - 0.5f * (a0.x + a1.x), 0.5f * (b0.y + b1.y) - 0.5f * (a0.y + a1.y),
+                      0.5f * (b0.z + b1.z) - 0.5f * (a0.z + a1.z), 0.5f * (a1.x - a0.x) + 0.5f * (b1.x - b0.x),
+                      0.5f * (a1.y - a0.y) + 0.5f * (b1.y - b0.y), 0.5f * (a1.z - a0.z) + 0.5f * (b1.z - b0.z), box,
+                      rc);
+}
+
+// One wavefront per block I.  Count mode (rows == nullptr) writes counts[I]
+// and upper[I], the number of those blocks J with J >= I; fill mode writes
+// the row at rows[start[I] ...] in ascending J, and I into owner[] at the
+// same positions.
 __global__ __launch_bounds__(64) void k_block_rows(int nb, const float4* __restrict__ bmin,
                                                    const float4* __restrict__ bmax, Box<float> box, float cutoff,
                                                    const int* __restrict__ start, int* __restrict__ counts,
-                                                   int* __restrict__ rows, int* __restrict__ owner, int cap) {
+                                                   int* __restrict__ upper, int* __restrict__ rows,
+                                                   int* __restrict__ owner, int cap) {
     const int I = blockIdx.x, lane = threadIdx.x;
     const float4 a0 = bmin[I], a1 = bmax[I];
     const int base = rows ? start[I] : 0;
-    int c = 0;
+    int c = 0, cu = 0;
     for (int J0 = 0; J0 < nb; J0 += kTile) {
         const int J = J0 + lane;
         const bool hit = J < nb && blocks_within(a0, a1, bmin[J], bmax[J], box, cutoff);
@@ -78,8 +119,12 @@ __global__ __launch_bounds__(64) void k_block_rows(int nb, const float4* __restr
             }
         }
         c += __popcll(m);
+        cu += __popcll(__ballot(hit && J >= I));
     }
-    if (lane == 0 && !rows) counts[I] = c;
+    if (lane == 0 && !rows) {
+        counts[I] = c;
+        upper[I] = cu;
+    }
 }
 
 // start[k] = counts[0] + ... + counts[k-1] for k = 0..m (one workgroup).
@@ -128,7 +173,8 @@ __global__ __launch_bounds__(64) void k_nl_masks(int n, const int* __restrict__ 
                                                  unsigned long long* __restrict__ nmask,
                                                  unsigned long long* __restrict__ tmask,
                                                  unsigned long long* __restrict__ tmaskcc,
-                                                 int* __restrict__ flag, int cap) {
+                                                 int* __restrict__ flag, int cap, int4* __restrict__ act,
+                                                 const int* __restrict__ ustart) {
     __shared__ double4 sp[kTile];
     const int r = blockIdx.x, lane = threadIdx.x;
     const int nbk = (n + kTile - 1) / kTile;
@@ -150,17 +196,16 @@ __global__ __launch_bounds__(64) void k_nl_masks(int n, const int* __restrict__ 
         const float cI[3] = {0.5f * (b0.x + b1.x), 0.5f * (b0.y + b1.y), 0.5f * (b0.z + b1.z)};
         const float hI[3] = {0.5f * (b1.x - b0.x), 0.5f * (b1.y - b0.y), 0.5f * (b1.z - b0.z)};
         const float rcf = sqrtf((float)rc2) * 1.0001f + 1e-5f;
-        const float L[3] = {(float)box.L[0], (float)box.L[1], (float)box.L[2]};
-        const float inv[3] = {(float)box.inv[0], (float)box.inv[1], (float)box.inv[2]};
-        const float d[3] = {(float)pj.x - cI[0], (float)pj.y - cI[1], (float)pj.z - cI[2]};
-        float d2 = 0.f;
-#pragma unroll
+        Box<float> fb;
         for (int k = 0; k < 3; k++) {
-            const float dk = d[k] - L[k] * floorf(d[k] * inv[k] + 0.5f);
-            const float g = fmaxf(0.f, fabsf(dk) - hI[k]);
-            d2 += g * g;
+            fb.L[k] = (float)box.L[k];
+            fb.inv[k] = (float)box.inv[k];
         }
-        cand = jv && d2 <= rcf * rcf;
+        fb.bx = (float)box.bx;
+        fb.cx = (float)box.cx;
+        fb.cy = (float)box.cy;
+        cand = jv && gap_within((float)pj.x - cI[0], (float)pj.y - cI[1], (float)pj.z - cI[2], hI[0], hI[1], hI[2],
+                                fb, rcf);
     }
     unsigned long long cmask = __ballot(cand);
     unsigned long long skip = J == I ? 1ull << lane : 0ull;
@@ -188,6 +233,10 @@ __global__ __launch_bounds__(64) void k_nl_masks(int n, const int* __restrict__ 
     const unsigned long long cc = ibare ? (m & jbare) : 0ull;
     tmask[(size_t)r * kTile + lane] = (m & ~cc) & upper;
     tmaskcc[(size_t)r * kTile + lane] = cc & upper;
+    // the tile's entry in the electrostatics kernels' work list: the row
+    // entries J >= I of row I are its last upper[I] entries, placed at
+    // ustart[I] (the exclusive scan of upper) in the same order
+    if (lane == 0) act[ustart[I + 1] - (rstart[I + 1] - r)] = make_int4(r, I, J, 0);
     if (J > I) {
         unsigned long long t = 0;
         for (int k = 0; k < kTile; k++) {
@@ -449,13 +498,56 @@ __global__ __launch_bounds__(256) void k_nl_induced_field(NbrList L, const int* 
     }
 }
 
+// Thole damping derivatives of the energy gradients: h = u dE/du of each
+// listed pair of polarisable sites (thole_u_derivative; listed pairs are not
+// covalent partners: polarization scale 1, default Thole width), summed over
+// the atom's polarisable neighbours into hd[i] and hdef[i].  Each pair is
+// visited from both of its atoms.
+template <class R>
+__global__ __launch_bounds__(256) void k_nl_thole(NbrList L, const int* __restrict__ atoms, int natoms, PairAtoms<R> A,
+                                                  const R* __restrict__ mu, PairParams<R> P, acc_t* __restrict__ hd,
+                                                  acc_t* __restrict__ hdef) {
+    const unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = (int)(g / kNlSplit), sub = (int)(g % kNlSplit);
+    const bool iv = t < natoms;
+    const int i = atoms[iv ? t : natoms - 1];
+    const R4<R> pi = A.pos[i];
+    R mi[kMP];
+#pragma unroll
+    for (int k = 0; k < kMP; k++) mi[k] = A.lab[(size_t)i * kMP + k];
+    const V3<R> ui = mk(mu[3 * (size_t)i], mu[3 * (size_t)i + 1], mu[3 * (size_t)i + 2]);
+    const int cnt = iv ? L.count[i] : 0;
+    const int* lp = L.nbr + (size_t)(iv ? t : natoms - 1) * kNlSplit + sub;
+    const size_t stride = (size_t)natoms * kNlSplit;
+    int jn = sub < cnt ? lp[0] : 0;
+    R h = 0;
+    for (int k = sub; k < cnt; k += kNlSplit) {
+        const int j = jn;
+        lp += stride;
+        if (k + kNlSplit < cnt) jn = *lp;
+        const R4<R> pj = A.pos[j];
+        const V3<R> dr = min_image(mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z), P.box);
+        R mj[kMP];
+#pragma unroll
+        for (int c = 0; c < kMP; c++) mj[c] = A.lab[(size_t)j * kMP + c];
+        const V3<R> uj = mk(mu[3 * (size_t)j], mu[3 * (size_t)j + 1], mu[3 * (size_t)j + 2]);
+        h += thole_u_derivative<R>(dr, dot(dr, dr), mi, mj, ui, uj, pi.w, pj.w, P.default_thole, R(1), P.mutual != 0);
+    }
+#pragma unroll
+    for (int off = 1; off < kNlSplit; off <<= 1) h += __shfl_xor(h, off);
+    if (iv && sub == 0 && h != R(0)) {
+        acc_add(&hd[i], h);
+        acc_add(&hdef[i], h);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 void launch_block_rows(hipStream_t s, int nb, const float4* bmin, const float4* bmax, Box<float> box, float cutoff,
-                       const int* start, int* counts, int* rows, int* owner, int cap) {
-    hipLaunchKernelGGL(k_block_rows, dim3(nb), dim3(kTile), 0, s, nb, bmin, bmax, box, cutoff, start, counts, rows,
-                       owner, cap);
+                       const int* start, int* counts, int* upper, int* rows, int* owner, int cap) {
+    hipLaunchKernelGGL(k_block_rows, dim3(nb), dim3(kTile), 0, s, nb, bmin, bmax, box, cutoff, start, counts, upper,
+                       rows, owner, cap);
 }
 void launch_exclusive_scan(hipStream_t s, int m, const int* counts, int* start) {
     hipLaunchKernelGGL(k_exclusive_scan, dim3(1), dim3(1024), 0, s, m, counts, start);
@@ -465,12 +557,13 @@ void launch_nl_build(hipStream_t s, int n, int nrows, int rcap, bool all, const 
                      const int* partners, const int* npart, int pstride, Box<double> box, double rc2, int cap,
                      unsigned long long* nmask, unsigned long long* polmask, int* polcnt, int* polstart, int* nbr,
                      int* count, int* scalars, unsigned long long* tmask, int* pnbr, int* pcount, const int* chg,
-                     unsigned long long* tmaskcc) {
+                     unsigned long long* tmaskcc, int4* act, const int* ustart) {
     const int nb = (n + kTile - 1) / kTile;
     hipLaunchKernelGGL(k_pol_mask, dim3(nb), dim3(kTile), 0, s, n, pos, polmask, polcnt);
     if (nrows > 0)
         hipLaunchKernelGGL(k_nl_masks, dim3(nrows), dim3(kTile), 0, s, n, rstart, rows, owner, bmin, bmax, pos,
-                           partners, npart, pstride, box, rc2, chg, nmask, tmask, tmaskcc, scalars + 1, rcap);
+                           partners, npart, pstride, box, rc2, chg, nmask, tmask, tmaskcc, scalars + 1, rcap, act,
+                           ustart);
     launch_exclusive_scan(s, nb, polcnt, polstart);
     hipLaunchKernelGGL(k_nl_fill, dim3(nb), dim3(kFillWaves * kTile), 0, s, n, all ? 1 : 0, rstart, rows, polmask,
                        polstart, nmask, cap, nbr, count, scalars, pnbr, pcount, rcap);
@@ -501,7 +594,18 @@ void launch_nl_induced_field(hipStream_t s, int natoms, const int* atoms, NbrLis
                            nullptr);
 }
 
+template <class R>
+void launch_nl_thole(hipStream_t s, int natoms, const int* atoms, NbrList l, PairAtoms<R> a, const R* mu,
+                     PairParams<R> p, acc_t* hd, acc_t* hdef) {
+    if (natoms <= 0) return;
+    const size_t threads = (size_t)natoms * kNlSplit;
+    hipLaunchKernelGGL(k_nl_thole<R>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, l, atoms, natoms, a,
+                       mu, p, hd, hdef);
+}
+
 #define MPID_INSTANTIATE(R)                                                                                         \
+    template void launch_nl_thole<R>(hipStream_t, int, const int*, NbrList, PairAtoms<R>, const R*, PairParams<R>,  \
+                                     acc_t*, acc_t*);                                                              \
     template void launch_nl_fixed_field<R>(hipStream_t, int, const int*, NbrList, PairAtoms<R>, PairParams<R>,       \
                                            acc_t*);                                                                \
     template void launch_nl_induced_field<R>(hipStream_t, int, const int*, NbrList, PairAtoms<R>, const R*,          \

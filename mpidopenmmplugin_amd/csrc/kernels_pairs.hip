@@ -56,8 +56,13 @@ __global__ __launch_bounds__(64) void k_block_bounds(int n, const R4<R>* __restr
 
 constexpr int kElecThreads = 256;  // four wavefronts share one tile
 
-template <class R>
-__device__ __forceinline__ void electrostatics_tile(const int* __restrict__ rows,
+// kPot (the energy gradients): also the potential vectors dE/dm of every
+// pair (20 stored lab components per atom, induced dipoles fixed) into pot;
+// all_general: every pair takes the general formula, so that sites that are
+// bare charges get all 20 components (else they get the charge one only).
+template <class R, bool kPot>
+__device__ __forceinline__ void electrostatics_tile(const int4* __restrict__ act, const int* __restrict__ nact,
+                                                                  const int* __restrict__ rows,
                                                                   const int* __restrict__ owner,
                                                                   const unsigned long long* __restrict__ tmask,
                                                                   const unsigned long long* __restrict__ tmaskcc,
@@ -67,7 +72,8 @@ __device__ __forceinline__ void electrostatics_tile(const int* __restrict__ rows
                                                                   const double* __restrict__ lab64,
                                                                   PairParams<double> P64, acc_t* __restrict__ force,
                                                                   acc_t* __restrict__ torque,
-                                                                  acc_t* __restrict__ energy) {
+                                                                  acc_t* __restrict__ energy, acc_t* __restrict__ pot,
+                                                                  int all_general) {
     // Multipole rows of 20 reals, read as 16-byte vectors: with a 20-word
     // stride, 16 lanes reading 16 different rows cover each bank once.
     constexpr int kS = kMP;
@@ -78,16 +84,22 @@ __device__ __forceinline__ void electrostatics_tile(const int* __restrict__ rows
     // components: force x, y, z, torque x, y, z.  FP64 in both precisions
     // (LDS atomic adds of doubles keep up on gfx950, those of floats stall).
     __shared__ double acc[2][6][kTile];
+    // potential vectors (kPot): [side][component][atom]
+    __shared__ double pacc[kPot ? 2 : 1][kPot ? kMP : 1][kTile];
     __shared__ R4<R> spos[2][kTile];
     __shared__ __align__(16) R smp[2][kTile * kS];
     __shared__ R smu[2][kTile * 3];
     __shared__ int san[2][kTile];
     __shared__ double4 spos64[2][kTile];  // x, y, z, charge in FP64 for the charge-charge channel
     __shared__ double sen[kElecThreads / kTile];
-    const int r = blockIdx.x, tid = threadIdx.x, lane = tid & (kTile - 1), wave = tid / kTile;
-    const int I = owner[r], J = rows[r];
-    if (J < I) return;  // each block pair once (uniform over the workgroup)
+    const int tid = threadIdx.x, lane = tid & (kTile - 1), wave = tid / kTile;
     const int n = P.n;
+    // One workgroup per entry {r, I, J} of the list of block pairs (J >= I);
+    // the launch covers an upper bound of its length (the entries past it
+    // lie inside the buffer and are not used).
+    const int4 tile = act[blockIdx.x];
+    if ((int)blockIdx.x >= *nact) return;
+    const int r = tile.x, I = tile.y, J = tile.z;
     // A workgroup's wavefronts sit on fixed SIMDs of a CU, and a tile's work
     // is uneven: one wavefront takes the general pairs (about four times the
     // cost of a pair with a bare charge).  The roles therefore rotate over
@@ -95,15 +107,59 @@ __device__ __forceinline__ void electrostatics_tile(const int* __restrict__ rows
     // index), so that the workgroups sharing a CU load its SIMDs alike.
     const int role = (wave - (int)(((unsigned)r * 0x9E3779B1u) >> 30)) & 3;
 
+    // Every load of the tile is issued here, before any of it is used, so
+    // that their latencies overlap: the masks and bare-charge flags for the
+    // ranking; roles 0 and 3 the per-atom fields of sides 0 and 1; all
+    // threads their share of the two 64 x 20 multipole blocks (contiguous in
+    // HBM and in LDS, copied as 16-byte vectors; atoms past the last read as
+    // zeros).
+    bool bi = chg[min(I * kTile + lane, n - 1)] != 0;
+    const bool bjl = chg[min(J * kTile + lane, n - 1)] != 0;
+    unsigned long long mc = tmaskcc[(size_t)r * kTile + lane];
+    unsigned long long mg = tmask[(size_t)r * kTile + lane];
+    const bool stager = role == 0 || role == 3;
+    const int sside = role == 0 ? 0 : 1;
+    const int sa = min((sside == 0 ? I : J) * kTile + lane, n - 1);
+    R4<R> st_pos{};
+    R st_mu[3] = {};
+    int st_an = 0;
+    double4 st_p64{};
+    double st_q = 0.0;
+    if (stager) {
+        st_pos = A.pos[sa];
+        st_mu[0] = mu[3 * (size_t)sa + 0];
+        st_mu[1] = mu[3 * (size_t)sa + 1];
+        st_mu[2] = mu[3 * (size_t)sa + 2];
+        st_an = A.aniso[sa];
+        st_p64 = pos64[sa];
+        st_q = lab64[(size_t)sa * kMP];
+    }
+    typedef R vec __attribute__((ext_vector_type(kV)));
+    constexpr int kVecs = kTile * kMP / kV;  // per side
+    constexpr int kVecPer = (2 * kVecs + kElecThreads - 1) / kElecThreads;
+    vec st_mp[kVecPer];
+#pragma unroll
+    for (int u = 0; u < kVecPer; u++) {
+        const int e = tid + u * kElecThreads;
+        const int side = e >= kVecs, v = e - side * kVecs;
+        const int a0 = (side == 0 ? I : J) * kTile;
+        const int lim = (min(a0 + kTile, n) - a0) * (kMP / kV);
+        const vec* src = reinterpret_cast<const vec*>(A.lab + (size_t)a0 * kMP);
+        st_mp[u] = (e < 2 * kVecs && v < lim) ? src[v] : vec(R(0));
+    }
+
     // Every wavefront ranks the tile's pairs (prefix sums of the lanes'
     // popcounts): pairs of two bare charges at [0, tc), general pairs at
     // [tc, tc + t2), pairs with one bare charge after.  Bit 15 of a one-bare
     // entry says that i (not j) is the bare site.  Roles 1, 2 and 3 write out
     // one class each.
-    const bool bi = chg[min(I * kTile + lane, n - 1)] != 0;
-    const unsigned long long bj = __ballot(chg[min(J * kTile + lane, n - 1)] != 0);
-    const unsigned long long mc = tmaskcc[(size_t)r * kTile + lane];
-    const unsigned long long mg = tmask[(size_t)r * kTile + lane];
+    unsigned long long bj = __ballot(bjl);
+    if (kPot && all_general) {
+        mg |= mc;
+        mc = 0ull;
+        bi = false;
+        bj = 0ull;
+    }
     const unsigned long long m1 = bi ? mg : (mg & bj);
     const unsigned long long m2 = bi ? 0ull : (mg & ~bj);
     const int cc = __popcll(mc), c1 = __popcll(m1), c2 = __popcll(m2);
@@ -133,33 +189,27 @@ __device__ __forceinline__ void electrostatics_tile(const int* __restrict__ rows
         for (unsigned long long b = m2; b; b &= b - 1ull)
             splist[o++] = (unsigned short)((lane << 8) | (__ffsll((long long)b) - 1));
     }
-    // stage both blocks in LDS: roles 0 and 3 the per-atom fields of sides
-    // 0 and 1, all threads the multipole rows
-    if (role == 0 || role == 3) {
-        const int side = role == 0 ? 0 : 1;
-        const int a = min((side == 0 ? I : J) * kTile + lane, n - 1);
-        spos[side][lane] = A.pos[a];
-        smu[side][3 * lane + 0] = mu[3 * (size_t)a + 0];
-        smu[side][3 * lane + 1] = mu[3 * (size_t)a + 1];
-        smu[side][3 * lane + 2] = mu[3 * (size_t)a + 2];
-        san[side][lane] = A.aniso[a];
-        const double4 p64 = pos64[a];
-        spos64[side][lane] = make_double4(p64.x, p64.y, p64.z, lab64[(size_t)a * kMP]);
+    // the staged loads into LDS, and the accumulators cleared
+    if (stager) {
+        spos[sside][lane] = st_pos;
+        smu[sside][3 * lane + 0] = st_mu[0];
+        smu[sside][3 * lane + 1] = st_mu[1];
+        smu[sside][3 * lane + 2] = st_mu[2];
+        san[sside][lane] = st_an;
+        spos64[sside][lane] = make_double4(st_p64.x, st_p64.y, st_p64.z, st_q);
 #pragma unroll
-        for (int k = 0; k < 6; k++) acc[side][k][lane] = 0.0;
+        for (int k = 0; k < 6; k++) acc[sside][k][lane] = 0.0;
+        if constexpr (kPot) {
+#pragma unroll
+            for (int k = 0; k < kMP; k++) pacc[sside][k][lane] = 0.0;
+        }
     }
-    {
-        // the 64 x 20 multipole block is contiguous in HBM and in LDS: copy it
-        // as 16-byte vectors (atoms past the last one read as zeros)
-        typedef R vec __attribute__((ext_vector_type(kV)));
-        constexpr int kVecs = kTile * kMP / kV;  // per side
-        for (int e = tid; e < 2 * kVecs; e += kElecThreads) {
+#pragma unroll
+    for (int u = 0; u < kVecPer; u++) {
+        const int e = tid + u * kElecThreads;
+        if (e < 2 * kVecs) {
             const int side = e >= kVecs, v = e - side * kVecs;
-            const int a0 = (side == 0 ? I : J) * kTile;
-            const int lim = (min(a0 + kTile, n) - a0) * (kMP / kV);
-            const vec* src = reinterpret_cast<const vec*>(A.lab + (size_t)a0 * kMP);
-            vec* dst = reinterpret_cast<vec*>(&smp[side][0]);
-            dst[v] = v < lim ? src[v] : vec(R(0));
+            reinterpret_cast<vec*>(&smp[side][0])[v] = st_mp[u];
         }
     }
     __syncthreads();
@@ -182,7 +232,14 @@ __device__ __forceinline__ void electrostatics_tile(const int* __restrict__ rows
         const V3<double> dr = min_image(mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z), P64.box);
         V3<double> f;
         double e;
-        charge_pair<double>(dr, dot(dr, dr), pi.w, pj.w, P64, f, e);
+        if constexpr (kPot) {
+            double phi_i = 0.0, phi_j = 0.0;
+            charge_pair<double, true>(dr, dot(dr, dr), pi.w, pj.w, P64, f, e, &phi_i, &phi_j);
+            atomicAdd(&pacc[0][0][il], phi_i);
+            atomicAdd(&pacc[1][0][jj], phi_j);
+        } else {
+            charge_pair<double>(dr, dot(dr, dr), pi.w, pj.w, P64, f, e);
+        }
         en += e;
         atomicAdd(&acc[0][0][il], -f.x);
         atomicAdd(&acc[0][1][il], -f.y);
@@ -193,7 +250,8 @@ __device__ __forceinline__ void electrostatics_tile(const int* __restrict__ rows
     }
     // Pairs with one bare charge b: the FP64 charge-charge channel, and b's
     // charge against a's moments and induced dipole in the working
-    // precision.  Chunks start at role 1.
+    // precision (the Cartesian form, elec_pair_bare_cart).  Chunks start at
+    // role 1.
     for (int q = ((role - 1) & 3) * kTile + perm; q < t1; q += kElecThreads) {
         const int ent = splist[tc + t2 + q], il = (ent >> 8) & 0x3f, jj = ent & 0xff;
         const bool ibare = (ent & 0x8000) != 0;
@@ -213,13 +271,24 @@ __device__ __forceinline__ void electrostatics_tile(const int* __restrict__ rows
             }
         }
         const V3<R> ua = mk(smu[sa][3 * ia], smu[sa][3 * ia + 1], smu[sa][3 * ia + 2]);
-        const ElecBareOut<R> o = elec_pair_bare<R>(dr, dot(dr, dr), ma, smp[sb][ib * kS], ua, R2<R>{R(0), pa.w},
-                                                   R2<R>{R(0), pb.w}, san[sa][ia] != 0, R(1), R(1), P);
+        R pot_a[kPot ? kMP : 1] = {};
+        R pot_b0 = R(0);
+        const ElecBareOut<R> o = elec_pair_bare_cart<R, kPot>(dr, dot(dr, dr), ma, smp[sb][ib * kS], ua,
+                                                              san[sa][ia] != 0, P, pot_a, &pot_b0);
         const double4 qa = spos64[sa][ia], qb = spos64[sb][ib];
         const V3<double> dr64 = min_image(mk(qb.x - qa.x, qb.y - qa.y, qb.z - qa.z), P64.box);
         V3<double> f;
         double e;
-        charge_pair<double>(dr64, dot(dr64, dr64), qa.w, qb.w, P64, f, e);
+        if constexpr (kPot) {
+            double phi_a = 0.0, phi_b = 0.0;
+            charge_pair<double, true>(dr64, dot(dr64, dr64), qa.w, qb.w, P64, f, e, &phi_a, &phi_b);
+            atomicAdd(&pacc[sa][0][ia], phi_a);
+            atomicAdd(&pacc[sb][0][ib], phi_b + (double)pot_b0);
+#pragma unroll
+            for (int k = 1; k < kMP; k++) atomicAdd(&pacc[sa][k][ia], (double)pot_a[k]);
+        } else {
+            charge_pair<double>(dr64, dot(dr64, dr64), qa.w, qb.w, P64, f, e);
+        }
         en += e + (double)o.energy;
         f.x += (double)o.f.x;
         f.y += (double)o.f.y;
@@ -257,8 +326,17 @@ __device__ __forceinline__ void electrostatics_tile(const int* __restrict__ rows
         }
         const V3<R> ui = mk(smu[0][3 * il], smu[0][3 * il + 1], smu[0][3 * il + 2]);
         const V3<R> uj = mk(smu[1][3 * jj], smu[1][3 * jj + 1], smu[1][3 * jj + 2]);
-        const ElecOut<R> o = elec_pair<R>(dr, dot(dr, dr), mi, mj, ui, uj, R2<R>{R(0), pi.w}, R2<R>{R(0), pj.w},
-                                          san[0][il] != 0, san[1][jj] != 0, R(1), R(1), P, false);
+        R pot_i[kPot ? kMP : 1] = {}, pot_j[kPot ? kMP : 1] = {};
+        const ElecOut<R> o = elec_pair<R, kPot>(dr, dot(dr, dr), mi, mj, ui, uj, R2<R>{R(0), pi.w},
+                                                R2<R>{R(0), pj.w}, san[0][il] != 0, san[1][jj] != 0, R(1), R(1), P,
+                                                false, pot_i, pot_j);
+        if constexpr (kPot) {
+#pragma unroll
+            for (int k = 0; k < kMP; k++) {
+                atomicAdd(&pacc[0][k][il], (double)pot_i[k]);
+                atomicAdd(&pacc[1][k][jj], (double)pot_j[k]);
+            }
+        }
         en += (double)o.energy;
         const V3<R> f = o.f;
         atomicAdd(&acc[0][0][il], -(double)f.x);
@@ -274,7 +352,7 @@ __device__ __forceinline__ void electrostatics_tile(const int* __restrict__ rows
         atomicAdd(&acc[1][4][jj], (double)o.tj.y);
         atomicAdd(&acc[1][5][jj], (double)o.tj.z);
     }
-    // wavefront sums of the pair energies
+    // wavefront sums of the pair energies; every wavefront's pair adds are in LDS
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) en += __shfl_xor(en, off);
     if (lane == 0) sen[wave] = en;
@@ -292,31 +370,58 @@ __device__ __forceinline__ void electrostatics_tile(const int* __restrict__ rows
                 if (f != 0.0) acc_add(&force[3 * (size_t)a + k], f);
                 if (t != 0.0) acc_add(&torque[3 * (size_t)a + k], t);
             }
+            if constexpr (kPot) {
+#pragma unroll
+                for (int k = 0; k < kMP; k++) {
+                    const double v = pacc[side][k][lane];
+                    if (v != 0.0) acc_add(&pot[kMP * (size_t)a + k], v);
+                }
+            }
         }
     }
     if (tid == 0) {
         double e = 0.0;
-        for (int w = 0; w < kElecThreads / kTile; w++) e += sen[w];
-        acc_add(energy + (blockIdx.x & (kEnergySlots - 1)), e);
+        for (int k = 0; k < kElecThreads / kTile; k++) e += sen[k];
+        if (e != 0.0) acc_add(energy + (blockIdx.x & (kEnergySlots - 1)), e);
     }
 }
 
 // FP32: at most 128 VGPRs, so that four wavefronts fit per SIMD
 __global__ __launch_bounds__(kElecThreads) __attribute__((amdgpu_waves_per_eu(4)))
-void k_electrostatics_f32(const int* __restrict__ rows, const int* __restrict__ owner,
+void k_electrostatics_f32(const int4* __restrict__ act, const int* __restrict__ nact, const int* __restrict__ rows,
+                          const int* __restrict__ owner,
                           const unsigned long long* __restrict__ tmask, const unsigned long long* __restrict__ tmaskcc,
                           const int* __restrict__ chg, PairAtoms<float> A, const float* __restrict__ mu, PairParams<float> P,
                           const double4* __restrict__ pos64, const double* __restrict__ lab64, PairParams<double> P64,
                           acc_t* __restrict__ force, acc_t* __restrict__ torque, acc_t* __restrict__ energy) {
-    electrostatics_tile<float>(rows, owner, tmask, tmaskcc, chg, A, mu, P, pos64, lab64, P64, force, torque, energy);
+    electrostatics_tile<float, false>(act, nact, rows, owner, tmask, tmaskcc, chg, A, mu, P, pos64, lab64, P64, force,
+                                      torque,
+                                      energy, nullptr, 0);
 }
 __global__ __launch_bounds__(kElecThreads)
-void k_electrostatics_f64(const int* __restrict__ rows, const int* __restrict__ owner,
+void k_electrostatics_f64(const int4* __restrict__ act, const int* __restrict__ nact, const int* __restrict__ rows,
+                          const int* __restrict__ owner,
                           const unsigned long long* __restrict__ tmask, const unsigned long long* __restrict__ tmaskcc,
                           const int* __restrict__ chg, PairAtoms<double> A, const double* __restrict__ mu, PairParams<double> P,
                           const double4* __restrict__ pos64, const double* __restrict__ lab64, PairParams<double> P64,
                           acc_t* __restrict__ force, acc_t* __restrict__ torque, acc_t* __restrict__ energy) {
-    electrostatics_tile<double>(rows, owner, tmask, tmaskcc, chg, A, mu, P, pos64, lab64, P64, force, torque, energy);
+    electrostatics_tile<double, false>(act, nact, rows, owner, tmask, tmaskcc, chg, A, mu, P, pos64, lab64, P64, force,
+                                       torque,
+                                       energy, nullptr, 0);
+}
+// The final pass of the energy gradients: the same, with the potential vectors.
+template <class R>
+__global__ __launch_bounds__(kElecThreads)
+void k_electrostatics_pot(const int4* __restrict__ act, const int* __restrict__ nact, const int* __restrict__ rows,
+                          const int* __restrict__ owner,
+                          const unsigned long long* __restrict__ tmask, const unsigned long long* __restrict__ tmaskcc,
+                          const int* __restrict__ chg, PairAtoms<R> A, const R* __restrict__ mu, PairParams<R> P,
+                          const double4* __restrict__ pos64, const double* __restrict__ lab64, PairParams<double> P64,
+                          acc_t* __restrict__ force, acc_t* __restrict__ torque, acc_t* __restrict__ energy,
+                          acc_t* __restrict__ pot, int all_general) {
+    electrostatics_tile<R, true>(act, nact, rows, owner, tmask, tmaskcc, chg, A, mu, P, pos64, lab64, P64, force, torque,
+                                 energy,
+                                 pot, all_general);
 }
 template <class R> struct ElecKernel;
 template <> struct ElecKernel<float> {
@@ -430,10 +535,11 @@ __global__ __launch_bounds__(256) void k_cov_induced_field(PairAtoms<R> A, const
     }
 }
 
-template <class R>
+// kPot: also the pairs' potential vectors (dE/dm, 20 per atom) into pot.
+template <class R, bool kPot>
 __global__ __launch_bounds__(128) void k_cov_electrostatics(PairAtoms<R> A, const R* __restrict__ mu, PairParams<R> P,
                                                             acc_t* __restrict__ force, acc_t* __restrict__ torque,
-                                                            acc_t* __restrict__ energy) {
+                                                            acc_t* __restrict__ energy, acc_t* __restrict__ pot) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double en = 0.0;
     if (i < P.n && A.npart[i] > 0) {
@@ -444,6 +550,7 @@ __global__ __launch_bounds__(128) void k_cov_electrostatics(PairAtoms<R> A, cons
         const R2<R> tdi = A.td[i];
         const bool ani = A.aniso[i] != 0;
         R fx = 0, fy = 0, fz = 0, tx = 0, ty = 0, tz = 0;
+        R pi_[kPot ? kMP : 1] = {};
         bool any = false;
         for (int k = 0; k < np; k++) {
             const int e = A.partners[(size_t)i * A.pstride + k];
@@ -454,10 +561,15 @@ __global__ __launch_bounds__(128) void k_cov_electrostatics(PairAtoms<R> A, cons
             const R r2 = dot(dr, dr);
             if (r2 > P.rc2) continue;
             const R sc = partner_scale<R>(e, P.scale14);
+            R pj_[kPot ? kMP : 1] = {};
             const ElecOut<R> o =
-                elec_pair<R>(dr, r2, mi, A.lab + (size_t)j * kMP, ui,
-                             mk(mu[3 * (size_t)j], mu[3 * (size_t)j + 1], mu[3 * (size_t)j + 2]), tdi, A.td[j], ani,
-                             A.aniso[j] != 0, sc, sc, P);
+                elec_pair<R, kPot>(dr, r2, mi, A.lab + (size_t)j * kMP, ui,
+                                   mk(mu[3 * (size_t)j], mu[3 * (size_t)j + 1], mu[3 * (size_t)j + 2]), tdi, A.td[j],
+                                   ani, A.aniso[j] != 0, sc, sc, P, true, pi_, pj_);
+            if constexpr (kPot) {
+#pragma unroll
+                for (int c = 0; c < kMP; c++) acc_add(&pot[kMP * (size_t)j + c], pj_[c]);
+            }
             any = true;
             en += (double)o.energy;
             fx -= o.f.x;
@@ -480,6 +592,10 @@ __global__ __launch_bounds__(128) void k_cov_electrostatics(PairAtoms<R> A, cons
             acc_add(&torque[3 * (size_t)i + 0], tx);
             acc_add(&torque[3 * (size_t)i + 1], ty);
             acc_add(&torque[3 * (size_t)i + 2], tz);
+            if constexpr (kPot) {
+#pragma unroll
+                for (int c = 0; c < kMP; c++) acc_add(&pot[kMP * (size_t)i + c], pi_[c]);
+            }
         }
     }
     // every lane reaches the reduction
@@ -499,15 +615,14 @@ void launch_block_bounds(hipStream_t s, int n, const R4<R>* pos, float4* bmin, f
 
 
 template <class R>
-void launch_electrostatics_pairs(hipStream_t s, int nrows, const int* rows, const int* owner,
-                                 const unsigned long long* tmask, const unsigned long long* tmaskcc, const int* chg,
-                                 PairAtoms<R> a, const R* mu, PairParams<R> p, const double4* pos64,
-                                 const double* lab64, PairParams<double> p64, acc_t* force, acc_t* torque,
-                                 acc_t* energy) {
-    if (nrows <= 0) return;
-    hipLaunchKernelGGL(ElecKernel<R>::fn, dim3(nrows), dim3(kElecThreads), 0, s, rows, owner, tmask, tmaskcc, chg, a,
-                       mu, p,
-                       pos64, lab64, p64, force, torque, energy);
+void launch_electrostatics_pairs(hipStream_t s, int maxtiles, const int4* act, const int* nact, const int* rows,
+                                 const int* owner, const unsigned long long* tmask,
+                                 const unsigned long long* tmaskcc, const int* chg, PairAtoms<R> a, const R* mu,
+                                 PairParams<R> p, const double4* pos64, const double* lab64, PairParams<double> p64,
+                                 acc_t* force, acc_t* torque, acc_t* energy) {
+    if (maxtiles <= 0) return;
+    hipLaunchKernelGGL(ElecKernel<R>::fn, dim3(maxtiles), dim3(kElecThreads), 0, s, act, nact, rows, owner, tmask, tmaskcc,
+                       chg, a, mu, p, pos64, lab64, p64, force, torque, energy);
 }
 
 template <class R> void launch_cov_fixed_field(hipStream_t s, int n, PairAtoms<R> a, PairParams<R> p, acc_t* field) {
@@ -525,23 +640,99 @@ void launch_cov_induced_field(hipStream_t s, int n, PairAtoms<R> a, const R* mu,
 }
 template <class R>
 void launch_cov_electrostatics(hipStream_t s, int n, PairAtoms<R> a, const R* mu, PairParams<R> p, acc_t* force,
-                               acc_t* torque, acc_t* energy) {
-    hipLaunchKernelGGL(k_cov_electrostatics<R>, dim3((n + 127) / 128), dim3(128), 0, s, a, mu, p, force, torque,
-                       energy);
+                               acc_t* torque, acc_t* energy, acc_t* pot) {
+    if (pot)
+        hipLaunchKernelGGL((k_cov_electrostatics<R, true>), dim3((n + 127) / 128), dim3(128), 0, s, a, mu, p, force,
+                           torque, energy, pot);
+    else
+        hipLaunchKernelGGL((k_cov_electrostatics<R, false>), dim3((n + 127) / 128), dim3(128), 0, s, a, mu, p, force,
+                           torque, energy, nullptr);
+}
+
+// ---------------------------------------------------------------------------
+// Thole damping of the covalent pairs (energy gradients): h = u dE/du of
+// each pair of polarisable partners, once per pair (thread = atom i,
+// partners j > i).  hd takes h at both atoms (the damping factors); a pair
+// at polarization scale 0 has the width thole_i + thole_j, so ht takes h/a
+// at both atoms; any other pair has the default width, and hdef takes 2h
+// (hdef sums h over both atoms of every pair, like the neighbour-list kernel).
+// ---------------------------------------------------------------------------
+template <class R>
+__global__ __launch_bounds__(256) void k_cov_thole(PairAtoms<R> A, const R* __restrict__ mu, PairParams<R> P,
+                                                   acc_t* __restrict__ hd, acc_t* __restrict__ hdef,
+                                                   acc_t* __restrict__ ht) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const int np = A.npart[i];
+    const R2<R> tdi = A.td[i];
+    if (np == 0 || tdi.y == R(0)) return;
+    const R4<R> pi = A.pos[i];
+    const R* mi = A.lab + (size_t)i * kMP;
+    const V3<R> ui = mk(mu[3 * (size_t)i], mu[3 * (size_t)i + 1], mu[3 * (size_t)i + 2]);
+    R hi = 0, hdi = 0, hti = 0;
+    for (int k = 0; k < np; k++) {
+        const int e = A.partners[(size_t)i * A.pstride + k];
+        const int j = e >> 2;
+        if (j <= i) continue;
+        const R2<R> tdj = A.td[j];
+        if (tdj.y == R(0)) continue;
+        const R4<R> pj = A.pos[j];
+        const V3<R> dr = min_image(mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z), P.box);
+        const R r2 = dot(dr, dr);
+        if (r2 > P.rc2) continue;
+        const R ps = partner_scale<R>(e, P.scale14);
+        const R a = ps == R(0) ? tdi.x + tdj.x : P.default_thole;
+        const V3<R> uj = mk(mu[3 * (size_t)j], mu[3 * (size_t)j + 1], mu[3 * (size_t)j + 2]);
+        const R h = thole_u_derivative<R>(dr, r2, mi, A.lab + (size_t)j * kMP, ui, uj, tdi.y, tdj.y, a, ps,
+                                          P.mutual != 0);
+        if (h == R(0)) continue;
+        hi += h;
+        acc_add(&hd[j], h);
+        if (ps == R(0)) {
+            const R t = h / a;
+            hti += t;
+            acc_add(&ht[j], t);
+        } else {
+            hdi += R(2) * h;
+        }
+    }
+    if (hi != R(0)) acc_add(&hd[i], hi);
+    if (hti != R(0)) acc_add(&ht[i], hti);
+    if (hdi != R(0)) acc_add(&hdef[i], hdi);
+}
+template <class R>
+void launch_cov_thole(hipStream_t s, int n, PairAtoms<R> a, const R* mu, PairParams<R> p, acc_t* hd, acc_t* hdef,
+                      acc_t* ht) {
+    hipLaunchKernelGGL(k_cov_thole<R>, dim3((n + 255) / 256), dim3(256), 0, s, a, mu, p, hd, hdef, ht);
+}
+template <class R>
+void launch_electrostatics_pot(hipStream_t s, int maxtiles, const int4* act, const int* nact, const int* rows,
+                               const int* owner, const unsigned long long* tmask, const unsigned long long* tmaskcc,
+                               const int* chg, PairAtoms<R> a, const R* mu, PairParams<R> p, const double4* pos64,
+                               const double* lab64, PairParams<double> p64, acc_t* force, acc_t* torque,
+                               acc_t* energy, acc_t* pot, int all_general) {
+    if (maxtiles <= 0) return;
+    hipLaunchKernelGGL(k_electrostatics_pot<R>, dim3(maxtiles), dim3(kElecThreads), 0, s, act, nact, rows, owner, tmask,
+                       tmaskcc, chg, a, mu, p, pos64, lab64, p64, force, torque, energy, pot, all_general);
 }
 
 #define MPID_INSTANTIATE(R)                                                                                        \
     template void launch_block_bounds<R>(hipStream_t, int, const R4<R>*, float4*, float4*);                         \
-    template void launch_electrostatics_pairs<R>(hipStream_t, int, const int*, const int*,                         \
+    template void launch_electrostatics_pairs<R>(hipStream_t, int, const int4*, const int*, const int*, const int*,\
                                                  const unsigned long long*, const unsigned long long*, const int*,   \
-                                                 PairAtoms<R>,                                                      \
-                                                 const R*, PairParams<R>, const double4*, const double*,            \
-                                                 PairParams<double>, acc_t*, acc_t*, acc_t*);                       \
+                                                 PairAtoms<R>, const R*, PairParams<R>, const double4*,             \
+                                                 const double*, PairParams<double>, acc_t*, acc_t*, acc_t*);        \
     template void launch_cov_fixed_field<R>(hipStream_t, int, PairAtoms<R>, PairParams<R>, acc_t*);                \
     template void launch_cov_induced_field<R>(hipStream_t, int, PairAtoms<R>, const R*, PairParams<R>, int,        \
                                               acc_t*, acc_t*);                                                    \
     template void launch_cov_electrostatics<R>(hipStream_t, int, PairAtoms<R>, const R*, PairParams<R>, acc_t*,    \
-                                               acc_t*, acc_t*);
+                                               acc_t*, acc_t*, acc_t*);                                             \
+    template void launch_cov_thole<R>(hipStream_t, int, PairAtoms<R>, const R*, PairParams<R>, acc_t*, acc_t*,      \
+                                      acc_t*);                                                                      \
+    template void launch_electrostatics_pot<R>(hipStream_t, int, const int4*, const int*, const int*, const int*,   \
+                                               const unsigned long long*, const unsigned long long*, const int*,     \
+                                               PairAtoms<R>, const R*, PairParams<R>, const double4*, const double*, \
+                                               PairParams<double>, acc_t*, acc_t*, acc_t*, acc_t*, int);
 MPID_INSTANTIATE(float)
 MPID_INSTANTIATE(double)
 #undef MPID_INSTANTIATE

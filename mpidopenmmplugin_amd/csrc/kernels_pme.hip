@@ -62,15 +62,15 @@ __device__ __forceinline__ unsigned spread_bits_pme(unsigned v) {
     return v;
 }
 
-// Grid origin and spline offset of one atom along axis k, as the reference
-// computes them (position wrapped to [-L/2, L/2), fractional coordinate
-// shifted by one half box).
+// Grid origin and spline offset of one atom along an axis of n points from
+// its fractional coordinate f along that box vector, as the reference
+// computes them (f wrapped to [-1/2, 1/2), then shifted by one half box,
+// MPIDReferenceForce.cpp:3049-3075).
 template <class R>
-__device__ __forceinline__ void grid_coord(R x, R L, R inv, int n, int& ig, R& w) {
-    x -= L * floor(x * inv + R(0.5));
-    R f = x * inv;
-    R fr = R(n) * (f - R((int)(f + R(0.5))) + R(0.5));
-    int ifr = (int)floor(fr);
+__device__ __forceinline__ void grid_coord(R f, int n, int& ig, R& w) {
+    f -= floor(f + R(0.5));
+    const R fr = R(n) * (f + R(0.5));
+    const int ifr = (int)floor(fr);
     w = fr - R(ifr);
     ig = ifr - kOrder + 1;
     ig %= n;
@@ -110,24 +110,24 @@ __global__ __launch_bounds__(256) void k_spread_fixed(int n, const R4<R>* __rest
     const R4<R> p = pos[i];
     int ig[3];
     R w[3];
-    grid_coord(p.x, g.box.L[0], g.box.inv[0], g.n[0], ig[0], w[0]);
-    grid_coord(p.y, g.box.L[1], g.box.inv[1], g.n[1], ig[1], w[1]);
-    grid_coord(p.z, g.box.L[2], g.box.inv[2], g.n[2], ig[2], w[2]);
+    {
+        const V3<R> f = to_frac(mk(p.x, p.y, p.z), g.box);
+        grid_coord(f.x, g.n[0], ig[0], w[0]);
+        grid_coord(f.y, g.n[1], ig[1], w[1]);
+        grid_coord(f.z, g.n[2], ig[2], w[2]);
+    }
     R tx[kOrder][4], ty[kOrder][4], tz[kOrder][4];
     bspline6<R, 3>(w[0], tx);
     bspline6<R, 3>(w[1], ty);
     bspline6<R, 3>(w[2], tz);
-    const R* m = lab + (size_t)i * kMP;
-    const R gx = g.gscale[0], gy = g.gscale[1], gz = g.gscale[2];
-    // fractional multipoles (rectangular box: each Cartesian index scales by n_k/L_k)
-    const R q = m[0];
-    const R dx = gx * m[1], dy = gy * m[2], dz = gz * m[3];
-    const R qxx = gx * gx * m[4], qxy = R(2) * gx * gy * m[5], qxz = R(2) * gx * gz * m[6];
-    const R qyy = gy * gy * m[7], qyz = R(2) * gy * gz * m[8], qzz = gz * gz * m[9];
-    const R oxxx = gx * gx * gx * m[10], oxxy = R(3) * gx * gx * gy * m[11], oxxz = R(3) * gx * gx * gz * m[12];
-    const R oxyy = R(3) * gx * gy * gy * m[13], oxyz = R(6) * gx * gy * gz * m[14], oxzz = R(3) * gx * gz * gz * m[15];
-    const R oyyy = gy * gy * gy * m[16], oyyz = R(3) * gy * gy * gz * m[17], oyzz = R(3) * gy * gz * gz * m[18];
-    const R ozzz = gz * gz * gz * m[19];
+    // fractional multipoles
+    R mf[kMP];
+    frac_moments(lab + (size_t)i * kMP, g.A, mf);
+    const R q = mf[0];
+    const R dx = mf[1], dy = mf[2], dz = mf[3];
+    const R qxx = mf[4], qxy = mf[5], qxz = mf[6], qyy = mf[7], qyz = mf[8], qzz = mf[9];
+    const R oxxx = mf[10], oxxy = mf[11], oxxz = mf[12], oxyy = mf[13], oxyz = mf[14], oxzz = mf[15];
+    const R oyyy = mf[16], oyyz = mf[17], oyzz = mf[18], ozzz = mf[19];
     const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
     for (int s = lane; s < kOrder * kOrder * kOrder; s += 64) {
         const int ix = s / (kOrder * kOrder), rem = s - ix * kOrder * kOrder;
@@ -157,16 +157,19 @@ __global__ __launch_bounds__(256) void k_spread_induced(int n, const R4<R>* __re
     const int i = blockIdx.x * kSpreadWaves + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (i >= n) return;
-    const R ux = g.gscale[0] * mu[3 * (size_t)i], uy = g.gscale[1] * mu[3 * (size_t)i + 1],
-            uz = g.gscale[2] * mu[3 * (size_t)i + 2];
+    const V3<R> uf = frac_vector(mk(mu[3 * (size_t)i], mu[3 * (size_t)i + 1], mu[3 * (size_t)i + 2]), g.A);
+    const R ux = uf.x, uy = uf.y, uz = uf.z;
     // a site without a dipole adds nothing (the whole wave leaves together)
     if (ux == R(0) && uy == R(0) && uz == R(0)) return;
     const R4<R> p = pos[i];
     int ig[3];
     R w[3];
-    grid_coord(p.x, g.box.L[0], g.box.inv[0], g.n[0], ig[0], w[0]);
-    grid_coord(p.y, g.box.L[1], g.box.inv[1], g.n[1], ig[1], w[1]);
-    grid_coord(p.z, g.box.L[2], g.box.inv[2], g.n[2], ig[2], w[2]);
+    {
+        const V3<R> f = to_frac(mk(p.x, p.y, p.z), g.box);
+        grid_coord(f.x, g.n[0], ig[0], w[0]);
+        grid_coord(f.y, g.n[1], ig[1], w[1]);
+        grid_coord(f.z, g.n[2], ig[2], w[2]);
+    }
     R tx[kOrder][2], ty[kOrder][2], tz[kOrder][2];
     bspline6<R, 1>(w[0], tx);
     bspline6<R, 1>(w[1], ty);
@@ -256,12 +259,13 @@ __global__ __launch_bounds__(256) void k_spread_tiled(const R4<R>* __restrict__ 
         if ((int)threadIdx.x < cnt) {
             const int a = atoms ? atoms[a0 + threadIdx.x] : a0 + threadIdx.x;
             const R4<R> p = pos[a];
-            const R px[3] = {p.x, p.y, p.z};
+            const V3<R> fp = to_frac(mk(p.x, p.y, p.z), g.box);
+            const R px[3] = {fp.x, fp.y, fp.z};
 #pragma unroll
             for (int k = 0; k < 3; k++) {
                 int ig;
                 R w;
-                grid_coord(px[k], g.box.L[k], g.box.inv[k], g.n[k], ig, w);
+                grid_coord(px[k], g.n[k], ig, w);
                 R th[kOrder][ND + 1];
                 bspline6<R, ND>(w, th);
                 for (int m = 0; m < kOrder; m++)
@@ -269,22 +273,14 @@ __global__ __launch_bounds__(256) void k_spread_tiled(const R4<R>* __restrict__ 
                 sig[threadIdx.x * 3 + k] = (ig - w0[k] + g.n[k]) % g.n[k];
             }
             R* mf = smf + threadIdx.x * NC;
-            const R gx = g.gscale[0], gy = g.gscale[1], gz = g.gscale[2];
             if (MODE == 0) {
-                const R* m = src + (size_t)a * kMP;
-                mf[0] = m[0];
-                mf[1] = gx * m[1]; mf[2] = gy * m[2]; mf[3] = gz * m[3];
-                mf[4] = gx * gx * m[4]; mf[5] = R(2) * gx * gy * m[5]; mf[6] = R(2) * gx * gz * m[6];
-                mf[7] = gy * gy * m[7]; mf[8] = R(2) * gy * gz * m[8]; mf[9] = gz * gz * m[9];
-                mf[10] = gx * gx * gx * m[10]; mf[11] = R(3) * gx * gx * gy * m[11];
-                mf[12] = R(3) * gx * gx * gz * m[12]; mf[13] = R(3) * gx * gy * gy * m[13];
-                mf[14] = R(6) * gx * gy * gz * m[14]; mf[15] = R(3) * gx * gz * gz * m[15];
-                mf[16] = gy * gy * gy * m[16]; mf[17] = R(3) * gy * gy * gz * m[17];
-                mf[18] = R(3) * gy * gz * gz * m[18]; mf[19] = gz * gz * gz * m[19];
+                frac_moments(src + (size_t)a * kMP, g.A, mf);
             } else {
-                mf[0] = gx * src[3 * (size_t)a];
-                mf[1] = gy * src[3 * (size_t)a + 1];
-                mf[2] = gz * src[3 * (size_t)a + 2];
+                const V3<R> uf =
+                    frac_vector(mk(src[3 * (size_t)a], src[3 * (size_t)a + 1], src[3 * (size_t)a + 2]), g.A);
+                mf[0] = uf.x;
+                mf[1] = uf.y;
+                mf[2] = uf.z;
             }
         }
         __syncthreads();
@@ -408,13 +404,14 @@ __global__ __launch_bounds__(256) void k_spread_owner(const R4<R>* __restrict__ 
             const int s = sbeg[d] + (q - soff[d]);
             const int a = atoms ? atoms[s] : s;
             const R4<R> p = pos[a];
-            const R px[3] = {p.x, p.y, p.z};
+            const V3<R> fp = to_frac(mk(p.x, p.y, p.z), g.box);
+            const R px[3] = {fp.x, fp.y, fp.z};
             int ig[3], rel[3];
             R w[3];
             bool meets = true;
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                grid_coord(px[k], g.box.L[k], g.box.inv[k], g.n[k], ig[k], w[k]);
+                grid_coord(px[k], g.n[k], ig[k], w[k]);
                 int r = ig[k] - w0[k];
                 if (r < -(g.n[k] >> 1)) r += g.n[k];
                 if (r >= g.n[k] - (g.n[k] >> 1)) r -= g.n[k];
@@ -432,22 +429,14 @@ __global__ __launch_bounds__(256) void k_spread_owner(const R4<R>* __restrict__ 
                     srel[slot * 3 + k] = rel[k];
                 }
                 R* mf = smf + slot * NC;
-                const R gx = g.gscale[0], gy = g.gscale[1], gz = g.gscale[2];
-                if (MODE == 0) {
-                    const R* m = src + (size_t)a * kMP;
-                    mf[0] = m[0];
-                    mf[1] = gx * m[1]; mf[2] = gy * m[2]; mf[3] = gz * m[3];
-                    mf[4] = gx * gx * m[4]; mf[5] = R(2) * gx * gy * m[5]; mf[6] = R(2) * gx * gz * m[6];
-                    mf[7] = gy * gy * m[7]; mf[8] = R(2) * gy * gz * m[8]; mf[9] = gz * gz * m[9];
-                    mf[10] = gx * gx * gx * m[10]; mf[11] = R(3) * gx * gx * gy * m[11];
-                    mf[12] = R(3) * gx * gx * gz * m[12]; mf[13] = R(3) * gx * gy * gy * m[13];
-                    mf[14] = R(6) * gx * gy * gz * m[14]; mf[15] = R(3) * gx * gz * gz * m[15];
-                    mf[16] = gy * gy * gy * m[16]; mf[17] = R(3) * gy * gy * gz * m[17];
-                    mf[18] = R(3) * gy * gz * gz * m[18]; mf[19] = gz * gz * gz * m[19];
+                    if (MODE == 0) {
+                    frac_moments(src + (size_t)a * kMP, g.A, mf);
                 } else {
-                    mf[0] = gx * src[3 * (size_t)a];
-                    mf[1] = gy * src[3 * (size_t)a + 1];
-                    mf[2] = gz * src[3 * (size_t)a + 2];
+                    const V3<R> uf =
+                        frac_vector(mk(src[3 * (size_t)a], src[3 * (size_t)a + 1], src[3 * (size_t)a + 2]), g.A);
+                    mf[0] = uf.x;
+                    mf[1] = uf.y;
+                    mf[2] = uf.z;
                 }
             }
         }
@@ -609,9 +598,12 @@ __global__ __launch_bounds__(256) void k_gather_phi(int n, const R4<R>* __restri
     const R4<R> p = pos[i];
     int ig[3];
     R w[3];
-    grid_coord(p.x, g.box.L[0], g.box.inv[0], g.n[0], ig[0], w[0]);
-    grid_coord(p.y, g.box.L[1], g.box.inv[1], g.n[1], ig[1], w[1]);
-    grid_coord(p.z, g.box.L[2], g.box.inv[2], g.n[2], ig[2], w[2]);
+    {
+        const V3<R> f = to_frac(mk(p.x, p.y, p.z), g.box);
+        grid_coord(f.x, g.n[0], ig[0], w[0]);
+        grid_coord(f.y, g.n[1], ig[1], w[1]);
+        grid_coord(f.z, g.n[2], ig[2], w[2]);
+    }
     R tx[kOrder][5], ty[kOrder][5], tz[kOrder][5];
     bspline6<R, 4>(w[0], tx);
     bspline6<R, 4>(w[1], ty);
@@ -691,9 +683,12 @@ __global__ __launch_bounds__(256) void k_gather_field(int natoms, const int* __r
     const R4<R> p = pos[i];
     int ig[3];
     R w[3];
-    grid_coord(p.x, g.box.L[0], g.box.inv[0], g.n[0], ig[0], w[0]);
-    grid_coord(p.y, g.box.L[1], g.box.inv[1], g.n[1], ig[1], w[1]);
-    grid_coord(p.z, g.box.L[2], g.box.inv[2], g.n[2], ig[2], w[2]);
+    {
+        const V3<R> f = to_frac(mk(p.x, p.y, p.z), g.box);
+        grid_coord(f.x, g.n[0], ig[0], w[0]);
+        grid_coord(f.y, g.n[1], ig[1], w[1]);
+        grid_coord(f.z, g.n[2], ig[2], w[2]);
+    }
     R tx[kOrder][2], ty[kOrder][2], tz[kOrder][2];
     bspline6<R, 1>(w[0], tx);
     bspline6<R, 1>(w[1], ty);
@@ -739,9 +734,10 @@ __global__ __launch_bounds__(256) void k_gather_field(int natoms, const int* __r
         s001 += __shfl_xor(s001, off);
     }
     if (tv && iz == 0) {
-        field[3 * (size_t)i + 0] = -g.gscale[0] * s100;
-        field[3 * (size_t)i + 1] = -g.gscale[1] * s010;
-        field[3 * (size_t)i + 2] = -g.gscale[2] * s001;
+        const V3<R> e = cart_vector(mk(s100, s010, s001), g.A);
+        field[3 * (size_t)i + 0] = -e.x;
+        field[3 * (size_t)i + 1] = -e.y;
+        field[3 * (size_t)i + 2] = -e.z;
     }
 }
 

@@ -163,7 +163,10 @@ template <class R> struct Prec {
     DBuf<R> phi, phidp;
     DBuf<R> grid;       // real charge / potential grid [n0][n1][n2]
     DBuf<R2<R>> spec;   // half spectrum [n0][n1][n2/2+1]
-    DBuf<R> moduli;
+    DBuf<double> moduli;  // |b(k)|^2 in FP64, [n0 | n1 | n2]
+    DBuf<R> infl;         // the influence function on the half spectrum (launch_influence_table)
+    double infl_key[10] = {};  // box vectors and alpha it was computed for
+    bool infl_valid = false;
     DBuf<R2<R>> tw[3];
     DBuf<R> scratch;  // spreading windows of the Morton tiles
     FftPlan<R> plan;
@@ -187,6 +190,7 @@ struct mpid_context {
     hipStream_t vstream = nullptr;
     hipEvent_t ev_v = nullptr, ev_vdone = nullptr;
     bool final_issued = false;  // the final pass of the current solution is on its way
+    bool defer_final = false;   // the solve leaves the final pass to its caller (energy gradients)
     // CG iterations are replayed from a graph captured once per solve (its
     // kernel arguments are those of that solve); hscal is the pinned host
     // copy of the residual norm that each replay leaves behind.
@@ -229,6 +233,8 @@ struct mpid_context {
     // of capacity nl_cap, and the tile masks of the row entries with J >= I
     DBuf<float4> bmin, bmax;
     DBuf<int> brow_count, brow_start, brows, brow_owner, nbr, ncount, nl_scalars;
+    DBuf<int4> tile_act;  // the electrostatics kernels' tiles {r, I, J} (row entries with J >= I)
+    DBuf<int> brow_upper, brow_ustart;  // per block row: entries with J >= I, and their exclusive scan
     // polarisable neighbours only (pnbr/pcount), and the sorted indices of the polarisable atoms
     DBuf<int> pnbr, pcount, polcnt, polstart, pol_idx;
     int npol = 0;
@@ -565,10 +571,10 @@ struct mpid_context {
         }
         P.grid.resize(G);
         P.spec.resize((size_t)opt.grid[0] * opt.grid[1] * (opt.grid[2] / 2 + 1));
-        std::vector<R> mods;
+        std::vector<double> mods;
         for (int k = 0; k < 3; k++) {
             std::vector<double> m = bspline_moduli(opt.grid[k]);
-            for (double v : m) mods.push_back((R)v);
+            for (double v : m) mods.push_back(v);
             const int nk = opt.grid[k];
             std::vector<R2<R>> tw(nk);
             for (int j = 0; j < nk; j++) {
@@ -581,6 +587,8 @@ struct mpid_context {
             if (!make_fft_axis(nk, P.plan.ax[k])) throw MpidError(MPID_ERR_UNSUPPORTED, "PME grid size not supported");
         }
         P.moduli.upload(mods.data(), mods.size(), stream);
+        P.infl.resize(P.spec.n);
+        P.infl_valid = false;
         if (tiled_spread && !owner_spread)
             P.scratch.resize((size_t)tgeom.ntiles * tgeom.W[0] * tgeom.W[1] * tgeom.W[2]);
         HIP_CHECK(hipStreamSynchronize(stream));
@@ -613,6 +621,8 @@ struct mpid_context {
         bmax.resize(nb);
         brow_count.resize(nb);
         brow_start.resize(nb + 1);
+        brow_upper.resize(nb);
+        brow_ustart.resize(nb + 1);
         ncount.resize(n);
         pcount.resize(n);
         polcnt.resize(nb);
@@ -678,20 +688,34 @@ struct mpid_context {
     }
 
     // ---- evaluation ------------------------------------------------------------
-    Box<double> dbox;
+    Box<double> dbox{};
     double box_vec[9];
+    // box^-1 (the reciprocal box vectors as rows), lower triangular:
+    // MPIDReferenceForce.cpp:2627-2632
+    double rinv[3][3] = {};
 
     void set_box(const double* boxv) {
         if (!pme()) {
             // no periodicity: zero edges make every wrap and image shift vanish
             for (int k = 0; k < 9; k++) box_vec[k] = boxv ? boxv[k] : 0.0;
-            for (int k = 0; k < 3; k++) dbox.L[k] = dbox.inv[k] = 0.0;
+            dbox = Box<double>{};
+            for (auto& row : rinv)
+                for (double& v : row) v = 0.0;
             return;
         }
         if (!boxv) throw MpidError(MPID_ERR_INVALID, "PME requires periodic box vectors");
+        // Box vectors in OpenMM's reduced form: a = (ax, 0, 0), b = (bx, by, 0),
+        // c = (cx, cy, cz) with |bx|, |cx| <= ax / 2 and |cy| <= by / 2
+        // (the form the reference's getPeriodicDelta and reciprocal box assume)
+        if (boxv[1] != 0 || boxv[2] != 0 || boxv[5] != 0)
+            throw MpidError(MPID_ERR_INVALID,
+                            "box vectors must be in reduced form: a along x, b in the xy plane");
+        const double tol = 1.0 + 1e-9;
+        if (std::fabs(boxv[3]) > 0.5 * boxv[0] * tol || std::fabs(boxv[6]) > 0.5 * boxv[0] * tol ||
+            std::fabs(boxv[7]) > 0.5 * boxv[4] * tol)
+            throw MpidError(MPID_ERR_INVALID,
+                            "box vectors must be in reduced form: |bx|, |cx| <= ax / 2 and |cy| <= by / 2");
         for (int k = 0; k < 9; k++) box_vec[k] = boxv[k];
-        if (boxv[1] != 0 || boxv[2] != 0 || boxv[3] != 0 || boxv[5] != 0 || boxv[6] != 0 || boxv[7] != 0)
-            throw MpidError(MPID_ERR_UNSUPPORTED, "only rectangular periodic boxes are supported on this platform");
         for (int k = 0; k < 3; k++) {
             double L = boxv[4 * k];
             if (!(L > 0.0)) throw MpidError(MPID_ERR_INVALID, "Box size of zero is invalid.");
@@ -702,14 +726,44 @@ struct mpid_context {
             dbox.L[k] = L;
             dbox.inv[k] = 1.0 / L;
         }
+        dbox.bx = boxv[3];
+        dbox.cx = boxv[6];
+        dbox.cy = boxv[7];
+        const double ax = boxv[0], by = boxv[4], cz = boxv[8];
+        const double sc = 1.0 / (ax * by * cz);
+        for (auto& row : rinv)
+            for (double& v : row) v = 0.0;
+        rinv[0][0] = by * cz * sc;
+        rinv[1][0] = -dbox.bx * cz * sc;
+        rinv[1][1] = ax * cz * sc;
+        rinv[2][0] = (dbox.bx * dbox.cy - by * dbox.cx) * sc;
+        rinv[2][1] = -ax * dbox.cy * sc;
+        rinv[2][2] = ax * by * sc;
     }
+    bool triclinic() const { return dbox.bx != 0.0 || dbox.cx != 0.0 || dbox.cy != 0.0; }
     template <class R> Box<R> rbox() const {
         Box<R> bx;
         for (int k = 0; k < 3; k++) {
             bx.L[k] = (R)dbox.L[k];
             bx.inv[k] = (R)dbox.inv[k];
         }
+        bx.bx = (R)dbox.bx;
+        bx.cx = (R)dbox.cx;
+        bx.cy = (R)dbox.cy;
         return bx;
+    }
+    // Grid-coordinate transform A_jk = n_k (box^-1)_jk
+    // (MPIDReferenceForce.cpp:3083 and 3177)
+    template <class R> FracT<R> frac_transform() const {
+        FracT<R> A;
+        A.a00 = (R)(opt.grid[0] * rinv[0][0]);
+        A.a10 = (R)(opt.grid[0] * rinv[1][0]);
+        A.a11 = (R)(opt.grid[1] * rinv[1][1]);
+        A.a20 = (R)(opt.grid[0] * rinv[2][0]);
+        A.a21 = (R)(opt.grid[1] * rinv[2][1]);
+        A.a22 = (R)(opt.grid[2] * rinv[2][2]);
+        A.tri = triclinic() ? 1 : 0;
+        return A;
     }
     template <class R> PmeGeom<R> geom() const {
         PmeGeom<R> g;
@@ -717,6 +771,7 @@ struct mpid_context {
             g.n[k] = opt.grid[k];
             g.gscale[k] = (R)(opt.grid[k] * dbox.inv[k]);
         }
+        g.A = frac_transform<R>();
         g.box = rbox<R>();
         g.skip = iteration_skip();
         return g;
@@ -756,9 +811,6 @@ struct mpid_context {
         return A;
     }
     double selfc() const { return (4.0 / 3.0) * opt.ewald_alpha * opt.ewald_alpha * opt.ewald_alpha / kSqrtPi; }
-    double3 gscale3() const {
-        return make_double3(opt.grid[0] * dbox.inv[0], opt.grid[1] * dbox.inv[1], opt.grid[2] * dbox.inv[2]);
-    }
 
     // positions (device, original order) -> sorted lab-frame state
     // all: build the lists of every atom (the inspection entry points, whose
@@ -768,6 +820,8 @@ struct mpid_context {
     // beside the neighbour-list build); fixed_field() joins it.
     void prepare(const double* d_pos, bool all = false, bool fixed_recip = true) {
         all_lists = all;
+        update_influence(pd);
+        if (opt.precision == MPID_PRECISION_MIXED) update_influence(pf);
         timed(P_SORT, [&] {
             launch_morton_keys(stream, n, d_pos, dbox, keys.p, vals.p);
             int rc = launch_sort_pairs(stream, sort_tmp.p, sort_tmp.n, keys.p, keys_sorted.p, vals.p, perm.p, n);
@@ -799,8 +853,9 @@ struct mpid_context {
         timed(P_TILES, [&] {
             launch_block_bounds<double>(stream, n, pd.pos.p, bmin.p, bmax.p);
             launch_block_rows(stream, nb, bmin.p, bmax.p, rbox<float>(), (float)opt.cutoff, nullptr, brow_count.p,
-                              nullptr, nullptr, 0);
+                              brow_upper.p, nullptr, nullptr, 0);
             launch_exclusive_scan(stream, nb, brow_count.p, brow_start.p);
+            launch_exclusive_scan(stream, nb, brow_upper.p, brow_ustart.p);
         });
         check_launch();
         // After a verified build for the current parameters the count is
@@ -822,11 +877,19 @@ struct mpid_context {
         build_neighbour_lists(nb);
         recip_cycle_fixed_rest(pd);  // if the list build has not issued it
     }
+    // an upper bound of the electrostatics kernels' tile count: the row
+    // entries with J >= I, of the verified count or of the capacity while
+    // the count is pending
+    int tile_bound() const {
+        const int nb = (n + kTile - 1) / kTile;
+        return ((rows_pending ? (int)brows.n : rows_total) + nb) / 2 + 1;
+    }
     void grow_rows(int count) {
         if ((size_t)count <= brows.n) return;
         const size_t cap = count + count / 4 + 64;
         brows.resize(cap);
         brow_owner.resize(cap);
+        tile_act.resize(cap);
         tmask.resize(cap * kTile);
         tmaskcc.resize(cap * kTile);
         nmask.resize(cap * kTile);
@@ -848,7 +911,7 @@ struct mpid_context {
         }
         timed(P_TILES, [&] {
             launch_block_rows(stream, nb, bmin.p, bmax.p, rbox<float>(), (float)opt.cutoff, brow_start.p, nullptr,
-                              brows.p, brow_owner.p, (int)brows.n);
+                              nullptr, brows.p, brow_owner.p, (int)brows.n);
         });
         for (int attempt = 0; attempt < 2; attempt++) {
             nl_cap = (nl_cap + kNlSplit - 1) / kNlSplit * kNlSplit;
@@ -860,7 +923,7 @@ struct mpid_context {
                                 brow_start.p, brows.p, brow_owner.p, bmin.p, bmax.p,
                                 pd.pos.p, spart.p, snpart.p, pstride, dbox, opt.cutoff * opt.cutoff, nl_cap, nmask.p,
                                 polmask.p, polcnt.p, polstart.p, nbr.p, ncount.p, nl_scalars.p, tmask.p, pnbr.p,
-                                pcount.p, chg.p, tmaskcc.p);
+                                pcount.p, chg.p, tmaskcc.p, tile_act.p, brow_ustart.p);
                 launch_pol_fill(stream, n, pd.pos.p, polstart.p, pol_idx.p);
                 if (tiled_spread) {
                     HIP_CHECK(hipMemsetAsync(tpstart.p, 0, tpstart.n * sizeof(int), stream));
@@ -922,12 +985,32 @@ struct mpid_context {
 
     template <class R> ConvParams<R> conv_params(const Prec<R>& P) const {
         ConvParams<R> c;
-        c.expf = (R)(M_PI * M_PI / (opt.ewald_alpha * opt.ewald_alpha));
-        c.sc = (R)(1.0 / M_PI * dbox.inv[0] * dbox.inv[1] * dbox.inv[2]);
-        for (int k = 0; k < 3; k++) c.inv[k] = (R)dbox.inv[k];
-        c.moduli = P.moduli.p;
+        c.infl = P.infl.p;
         c.skip = iteration_skip();
         return c;
+    }
+    // The influence-function table of a precision, recomputed (on stream)
+    // when the box or alpha differs from the ones it holds.
+    template <class R> void update_influence(Prec<R>& P) {
+        if (!pme() || !P.ready) return;
+        double key[10];
+        for (int k = 0; k < 9; k++) key[k] = box_vec[k];
+        key[9] = opt.ewald_alpha;
+        if (P.infl_valid && std::memcmp(key, P.infl_key, sizeof(key)) == 0) return;
+        InflParams ip;
+        for (int k = 0; k < 3; k++) ip.n[k] = opt.grid[k];
+        ip.expf = M_PI * M_PI / (opt.ewald_alpha * opt.ewald_alpha);
+        ip.sc = 1.0 / (M_PI * dbox.L[0] * dbox.L[1] * dbox.L[2]);
+        ip.r00 = rinv[0][0];
+        ip.r10 = rinv[1][0];
+        ip.r11 = rinv[1][1];
+        ip.r20 = rinv[2][0];
+        ip.r21 = rinv[2][1];
+        ip.r22 = rinv[2][2];
+        timed(P_FFT, [&] { launch_influence_table<R>(stream, ip, P.moduli.p, P.infl.p); });
+        check_launch();
+        std::memcpy(P.infl_key, key, sizeof(key));
+        P.infl_valid = true;
     }
 
     // Reciprocal cycles: issued to rstream after a fork(); the caller joins
@@ -1036,7 +1119,7 @@ struct mpid_context {
         });
         join();
         timed(P_OTHER, [&] {
-            launch_fixed_field_finish<double>(stream, n, pd.phi.p, lab.p, field_acc.p, gscale3(), selfc(), E0.p);
+            launch_fixed_field_finish<double>(stream, n, pd.phi.p, lab.p, field_acc.p, frac_transform<double>(), selfc(), E0.p);
         });
         check_launch();
     }
@@ -1243,7 +1326,7 @@ struct mpid_context {
             });
             join();
             timed(P_SOLVER, [&] {
-                launch_extrap_order<R>(stream, n, field_acc.p, grad_acc.p, P.phidp.p, gscale3(), selfc(), labpol.p,
+                launch_extrap_order<R>(stream, n, field_acc.p, grad_acc.p, P.phidp.p, frac_transform<double>(), selfc(), labpol.p,
                                        ptmu.p + m * n3, ptE.p + m * n3, ptG.p + m * n6, ptmu.p + (m + 1) * n3,
                                        P.mu.p);
             });
@@ -1306,7 +1389,7 @@ struct mpid_context {
             // only once the check has passed (forces_and_energy); a
             // correction issues the final pass again.
             mark_v();
-            issue_final_pass(P);
+            if (!defer_final) issue_final_pass(P);
             on_vstream([&] { eps = true_residual(); });
             int refine = 0;
             while (eps >= opt.target_epsilon && refine < 8 && iters < opt.max_iterations) {
@@ -1318,7 +1401,7 @@ struct mpid_context {
                 if (iters == before) iters++;
                 timed(P_SOLVER, [&] { launch_axpy(stream, 3 * n, 1.0, dx.p, x.p); });
                 mark_v();
-                issue_final_pass(P);
+                if (!defer_final) issue_final_pass(P);
                 on_vstream([&] { eps = true_residual(); });
             }
             stats.refinements = refine;
@@ -1365,13 +1448,15 @@ struct mpid_context {
         recip_cycle_induced(P, true);  // phidp
         // this category's events bracket the one kernel (its live time in the benchmark)
         timed(P_ELEC, [&] {
-            launch_electrostatics_pairs<R>(stream, rows_total, brows.p, brow_owner.p, tmask.p, tmaskcc.p, chg.p,
+            launch_electrostatics_pairs<R>(stream, tile_bound(), tile_act.p, brow_ustart.p + (n + kTile - 1) / kTile,
+                                           brows.p, brow_owner.p,
+                                           tmask.p, tmaskcc.p, chg.p,
                                            pair_atoms(P), P.mu.p, pair_params<R>(), pd.pos.p, pd.lab.p,
                                            pair_params<double>(), force_acc.p, torque_acc.p, energy_acc.p);
         });
         timed_on(rstream, P_RECIP, [&] {
             launch_recip_atoms<R>(rstream, n, opt.polarization == MPID_POLARIZATION_MUTUAL ? 1 : 0, pd.phi.p,
-                                  P.phidp.p, lab.p, mud.p, aniso.p, gscale3(), opt.ewald_alpha, force_acc.p,
+                                  P.phidp.p, lab.p, mud.p, aniso.p, frac_transform<double>(), opt.ewald_alpha, force_acc.p,
                                   torque_acc.p, energy_acc.p + kEnergySlots);
         });
         timed_on(rstream, P_COV, [&] {
@@ -1388,7 +1473,8 @@ struct mpid_context {
         mu_valid = true;
         join();
         timed(P_OTHER, [&] {
-            launch_torque_to_force(stream, n, dbox, spos.p, saxes.p, torque_acc.p, force_acc.p);
+            launch_torque_to_force(stream, n, dbox, spos.p, saxes.p, torque_acc.p, force_acc.p,
+                                   (opt.flags & MPID_FLAG_CONSERVATIVE_TORQUES) ? 1 : 0);
             if (d_forces_out) launch_output_forces(stream, n, perm.p, force_acc.p, d_forces_out, 1);
         });
         check_launch();
@@ -1410,7 +1496,9 @@ struct mpid_context {
         return acc_get(e[0]) + acc_get(e[1]) + energy_correction;
     }
 
-    template <class R> double run(Prec<R>& P, const double* d_pos, double* d_forces_out) {
+    // Lists, permanent field and induced dipoles at d_pos (a list overflow
+    // grows the lists and starts over).
+    template <class R> void converge(Prec<R>& P, const double* d_pos) {
         for (int attempt = 0;; attempt++) {
             try {
                 final_issued = false;
@@ -1428,9 +1516,187 @@ struct mpid_context {
                 grow_rows(rows_grow_to);
             }
         }
+    }
+    template <class R> double run(Prec<R>& P, const double* d_pos, double* d_forces_out) {
+        converge(P, d_pos);
         double e = forces_and_energy(P, d_forces_out);
         collect_profile();
         return e;
+    }
+
+    // ---- energy gradients -------------------------------------------------------
+    // The energy of mutual polarization is the variational form
+    // U(mu) = E_perm - eps mu.E0 + eps/2 mu.(alpha^-1 - T) mu (what the
+    // evaluation returns, stationary in mu), so at the solution its total
+    // derivative in any parameter is the partial derivative at fixed mu.  The
+    // adjoint solve A lambda = dU/dmu = -r (A = I - S T S, the symmetric
+    // operator of the solver, r its residual at the forward solution) moves mu
+    // onto the solution to second order in the solver's tolerance; the
+    // gradients are then formed at mu + lambda:
+    //  positions   minus the forces of the final pass, with the conservative
+    //              torques (the exact derivative for anisotropic sites);
+    //  moments     the potential vectors dU/dm (pair kernels, reciprocal and
+    //              self terms), taken back to the molecular frame;
+    //  alpha       -1/2 eps E E^T projected on the molecular axes (E the total
+    //              field), plus the damping factors through the Thole terms;
+    //  Thole       the derivatives of the pair terms in their damping.
+    // Direct polarization (mu = alpha E0, no solve) is U with A = alpha^-1.
+    DBuf<acc_t> pot_acc, hd_acc, hdef_acc, ht_acc, gdef_acc;
+    DBuf<double> Etot, gmol, gpol, gthole, gforce;
+    int adjoint_iterations = 0;
+    void alloc_gradients() {
+        const size_t n3 = 3 * (size_t)n;
+        pot_acc.resize((size_t)kMP * n);
+        hd_acc.resize(n);
+        hdef_acc.resize(n);
+        ht_acc.resize(n);
+        gdef_acc.resize(1);
+        Etot.resize(n3);
+        gmol.resize((size_t)kMP * n);
+        gpol.resize(n3);
+        gthole.resize(n);
+        gforce.resize(n3);
+    }
+    struct GradientsOut {
+        std::vector<double> force, mol, pol, thole;
+        double default_thole = 0.0;
+    };
+    template <class R> double gradients(Prec<R>& P, const double* d_pos, bool all_sites, GradientsOut& out) {
+        if (opt.polarization == MPID_POLARIZATION_EXTRAPOLATED)
+            throw MpidError(MPID_ERR_UNSUPPORTED, "energy gradients: Extrapolated polarization is not supported "
+                                                  "(its energy is not stationary in the induced dipoles)");
+        alloc_gradients();
+        const int n3 = 3 * n;
+        const bool mutual = opt.polarization == MPID_POLARIZATION_MUTUAL;
+        // conservative torques for this evaluation; the flag and the deferred
+        // final pass are restored however the evaluation ends
+        struct Restore {
+            mpid_context* c;
+            int flags;
+            ~Restore() {
+                c->opt.flags = flags;
+                c->defer_final = false;
+            }
+        } restore{this, opt.flags};
+        opt.flags |= MPID_FLAG_CONSERVATIVE_TORQUES;
+        defer_final = true;
+        converge(P, d_pos);
+        defer_final = false;
+        correction_pending = false;  // the energy is taken at the refined dipoles below
+        adjoint_iterations = 0;
+        if (mutual) {
+            // right-hand side: the FP64 residual of the solution (mixed: formed
+            // by the check of the solution; FP64: the solver's residual)
+            if (opt.precision != MPID_PRECISION_MIXED)
+                HIP_CHECK(hipMemcpyAsync(rt.p, r.p, n3 * sizeof(double), hipMemcpyDeviceToDevice, stream));
+            const double stop = 0.01 * stats.epsilon;
+            if (stop > 0.0) {
+                int it = 0;
+                cg(P, dx.p, rt.p, true, stop, it, 20);
+                adjoint_iterations = it;
+                timed(P_SOLVER, [&] { launch_axpy(stream, n3, 1.0, dx.p, x.p); });
+            }
+        }
+        // final pass at the refined dipoles, with the potential vectors
+        timed(P_SOLVER, [&] { launch_apply_S<R>(stream, n, sqrtpol.p, x.p, P.mu.p, mud.p); });
+        timed(P_OTHER, [&] {
+            launch_zero_acc(stream, force_acc.p, force_acc.n);
+            launch_zero_acc(stream, torque_acc.p, torque_acc.n);
+            launch_zero_acc(stream, energy_acc.p, 2 * kEnergySlots);
+            launch_zero_acc(stream, pot_acc.p, pot_acc.n);
+            launch_zero_acc(stream, hd_acc.p, hd_acc.n);
+            launch_zero_acc(stream, hdef_acc.p, hdef_acc.n);
+            launch_zero_acc(stream, ht_acc.p, ht_acc.n);
+            launch_zero_acc(stream, gdef_acc.p, gdef_acc.n);
+        });
+        recip_cycle_induced(P, true);  // phidp at every atom
+        timed(P_ELEC, [&] {
+            launch_electrostatics_pot<R>(stream, tile_bound(), tile_act.p, brow_ustart.p + (n + kTile - 1) / kTile,
+                                           brows.p, brow_owner.p,
+                                         tmask.p, tmaskcc.p, chg.p,
+                                         pair_atoms(P), P.mu.p, pair_params<R>(), pd.pos.p, pd.lab.p,
+                                         pair_params<double>(), force_acc.p, torque_acc.p, energy_acc.p, pot_acc.p,
+                                         all_sites ? 1 : 0);
+        });
+        timed_on(rstream, P_RECIP, [&] {
+            launch_recip_atoms<R>(rstream, n, mutual ? 1 : 0, pd.phi.p, P.phidp.p, lab.p, mud.p, aniso.p, frac_transform<double>(),
+                                  opt.ewald_alpha, force_acc.p, torque_acc.p, energy_acc.p + kEnergySlots);
+            launch_recip_potential<R>(rstream, n, pme() ? 1 : 0, pd.phi.p, P.phidp.p, lab.p, mud.p, frac_transform<double>(),
+                                      opt.ewald_alpha, pot_acc.p);
+        });
+        timed_on(rstream, P_COV, [&] {
+            launch_cov_electrostatics<double>(rstream, n, pair_atoms(pd), mud.p, pair_params<double>(), force_acc.p,
+                                              torque_acc.p, energy_acc.p, pot_acc.p);
+        });
+        // Thole damping of the pairs of polarisable sites
+        timed(P_INDUCED, [&] {
+            launch_nl_thole<R>(stream, npol, pol_idx.p, nlist_pol(), pair_atoms(P), P.mu.p, pair_params<R>(),
+                               hd_acc.p, hdef_acc.p);
+        });
+        timed(P_COV, [&] {
+            launch_cov_thole<double>(stream, n, pair_atoms(pd), mud.p, pair_params<double>(), hd_acc.p, hdef_acc.p,
+                                     ht_acc.p);
+        });
+        // the total field: at the polarisable sites from their lists, or at
+        // every site from lists of all atoms (all_sites), E0 included
+        if (all_sites) {
+            join();  // the reciprocal kernels above read the lab-frame state that prepare() rewrites
+            prepare(d_pos, true, false);
+            fixed_field(P, true);
+        }
+        if (mutual) {
+            timed(P_INDUCED, [&] {
+                launch_zero_acc(stream, field_acc.p, field_acc.n);
+                if (all_sites)
+                    launch_nl_induced_field<R>(stream, n, nullptr, nlist(), pair_atoms(P), P.mu.p, pair_params<R>(),
+                                               field_acc.p);
+                else
+                    launch_nl_induced_field<R>(stream, npol, pol_idx.p, nlist_pol(), pair_atoms(P), P.mu.p,
+                                               pair_params<R>(), field_acc.p);
+            });
+            timed(P_COV, [&] {
+                launch_cov_induced_field<R>(stream, n, pair_atoms(P), P.mu.p, pair_params<R>(), all_sites ? 0 : 1,
+                                            field_acc.p);
+            });
+        }
+        join();
+        timed(P_OTHER, [&] {
+            launch_total_field<R>(stream, n, mutual ? 1 : 0, E0.p, field_acc.p, P.phidp.p, frac_transform<double>(), mud.p, selfc(),
+                                  Etot.p);
+            launch_torque_to_force(stream, n, dbox, spos.p, saxes.p, torque_acc.p, force_acc.p,
+                                   (opt.flags & MPID_FLAG_CONSERVATIVE_TORQUES) ? 1 : 0);
+            launch_output_forces(stream, n, perm.p, force_acc.p, gforce.p, 0);
+            launch_param_gradients(stream, n, all_sites ? 1 : 0, perm.p, dbox, spos.p, saxes.p, smol.p, spol.p,
+                                   damp.p, chg.p, pot_acc.p, Etot.p, hd_acc.p, hdef_acc.p, ht_acc.p,
+                                   opt.default_thole, gmol.p, gpol.p, gthole.p, gdef_acc.p);
+        });
+        check_launch();
+        out.force.resize(n3);
+        out.mol.resize((size_t)kMP * n);
+        out.pol.resize(n3);
+        out.thole.resize(n);
+        acc_t slots[2 * kEnergySlots];
+        acc_t gdef = 0;
+        HIP_CHECK(hipMemcpyAsync(slots, energy_acc.p, sizeof(slots), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipMemcpyAsync(&gdef, gdef_acc.p, sizeof(acc_t), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipMemcpyAsync(out.force.data(), gforce.p, n3 * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipMemcpyAsync(out.mol.data(), gmol.p, out.mol.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                 stream));
+        HIP_CHECK(hipMemcpyAsync(out.pol.data(), gpol.p, n3 * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipMemcpyAsync(out.thole.data(), gthole.p, n * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        acc_t e[2] = {0, 0};
+        for (int k = 0; k < kEnergySlots; k++) {
+            e[0] += slots[k];
+            e[1] += slots[kEnergySlots + k];
+        }
+        out.default_thole = acc_get(gdef);
+        stats.energy_terms[0] = acc_get(e[0]);
+        stats.energy_terms[1] = acc_get(e[1]);
+        stats.energy_terms[2] = 0.0;
+        mu_valid = true;
+        collect_profile();
+        return acc_get(e[0]) + acc_get(e[1]);
     }
 
     // Every path that rewrites the device state forgets the last evaluation
@@ -1555,6 +1821,16 @@ struct mpid_context {
         check_launch();
         HIP_CHECK(hipMemcpyAsync(out, dval, (size_t)npts * sizeof(double), hipMemcpyDeviceToHost, stream));
         HIP_CHECK(hipStreamSynchronize(stream));
+    }
+
+    double energy_gradients(const double* pos, const double* boxv, bool all_sites, GradientsOut& out) {
+        invalidate_last();
+        set_box(boxv);
+        pos_in.upload(pos, 3 * (size_t)n, stream);
+        const double e = opt.precision == MPID_PRECISION_MIXED ? gradients(pf, pos_in.p, all_sites, out)
+                                                               : gradients(pd, pos_in.p, all_sites, out);
+        remember(pos);
+        return e;
     }
 
     void compute_fixed_field(const double* pos, const double* boxv, double* out) {
@@ -1700,6 +1976,41 @@ int mpid_get_electrostatic_potential(mpid_context* ctx, const double* positions,
 int mpid_get_electrostatic_potential_noalloc(mpid_context* ctx, int npoints, const double* points, double* out) {
     if (!ctx || npoints < 0 || (npoints > 0 && (!points || !out))) return MPID_ERR_INVALID;
     return guarded(ctx, [&] { ctx->potential_at_points_noalloc(npoints, points, out); });
+}
+
+int mpid_energy_gradients(mpid_context* ctx, const double* positions, const double* box, int flags,
+                          double* energy_out, mpid_gradients* grads) {
+    if (!ctx || !positions || !grads) return MPID_ERR_INVALID;
+    return guarded(ctx, [&] {
+        mpid_context::GradientsOut g;
+        const double e = ctx->energy_gradients(positions, box, (flags & MPID_GRADIENT_ALL_SITES) != 0, g);
+        const int n = ctx->n;
+        if (energy_out) *energy_out = e;
+        grads->adjoint_iterations = ctx->adjoint_iterations;
+        for (int i = 0; i < n; i++) {
+            const double* m = &g.mol[(size_t)kMP * i];
+            if (grads->positions)
+                for (int k = 0; k < 3; k++) grads->positions[3 * (size_t)i + k] = -g.force[3 * (size_t)i + k];
+            if (grads->charges) grads->charges[i] = m[0];
+            if (grads->dipoles)
+                for (int k = 0; k < 3; k++) grads->dipoles[3 * (size_t)i + k] = m[1 + k];
+            // internal XX XY XZ YY YZ ZZ -> MPIDForce XX XY YY XZ YZ ZZ (the
+            // inverse of the ordering map in set_parameters)
+            if (grads->quadrupoles) {
+                double* q = grads->quadrupoles + 6 * (size_t)i;
+                q[0] = m[4]; q[1] = m[5]; q[3] = m[6]; q[2] = m[7]; q[4] = m[8]; q[5] = m[9];
+            }
+            if (grads->octopoles) {
+                double* o = grads->octopoles + 10 * (size_t)i;
+                o[0] = m[10]; o[1] = m[11]; o[4] = m[12]; o[2] = m[13]; o[5] = m[14];
+                o[7] = m[15]; o[3] = m[16]; o[6] = m[17]; o[8] = m[18]; o[9] = m[19];
+            }
+            if (grads->polarity)
+                for (int k = 0; k < 3; k++) grads->polarity[3 * (size_t)i + k] = g.pol[3 * (size_t)i + k];
+            if (grads->tholes) grads->tholes[i] = g.thole[i];
+        }
+        if (grads->default_thole) *grads->default_thole = g.default_thole;
+    });
 }
 
 int mpid_get_system_multipole_moments(mpid_context* ctx, const double* positions, const double* box,

@@ -99,7 +99,7 @@ int main() {
     const PairParams<Cnt> P = params(true);
     // one-sided pieces (radial factors, field from one site) and the
     // two-sided electrostatics; radial factors with and without damping
-    long frad[2], ffrom = 0, irad[2], ifrom = 0, elec[2], bare[2];
+    long frad[2], ffrom = 0, irad[2], ifrom = 0, elec[2], bare[2], cart[2];
     for (int d = 0; d < 2; d++) {
         const double damp = d == 0 ? 0.316 : 0.0;  // d = 0: both sites polarisable
         R2<Cnt> ta{Cnt(1.3), Cnt(damp)}, tb{Cnt(1.3), Cnt(0.316)};
@@ -125,6 +125,11 @@ int main() {
         ElecBareOut<Cnt> ob = elec_pair_bare<Cnt>(dr, r2, ma, mb[0], ua, ta, tb, true, Cnt(1.0), Cnt(1.0), P);
         (void)ob;
         bare[d] = Ops::n;
+        // its Cartesian form at scale factors 1, which the tile kernel runs
+        Ops::n = 0;
+        ElecBareOut<Cnt> oc = elec_pair_bare_cart<Cnt>(dr, r2, ma, mb[0], ua, true, P);
+        (void)oc;
+        cart[d] = Ops::n;
     }
     V3<Cnt> fc;
     Cnt ec;
@@ -137,6 +142,7 @@ int main() {
     std::printf("induced_field_from %ld %ld\n", ifrom, ifrom);
     std::printf("elec_pair %ld %ld\n", elec[0], elec[1]);
     std::printf("elec_pair_bare %ld %ld\n", bare[0], bare[1]);
+    std::printf("elec_pair_bare_cart %ld %ld\n", cart[0], cart[1]);
     std::printf("charge_pair %ld %ld\n", cc, cc);
     return 0;
 }

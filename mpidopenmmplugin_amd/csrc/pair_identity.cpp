@@ -12,6 +12,10 @@
 // Prints, per output, the largest difference over all pairs relative to the
 // largest magnitude of that output (energy_rel, force_rel, torque_rel), the
 // largest torque given to a bare charge, the magnitudes, and max_rel_diff.
+// Then elec_pair_bare_cart (the tile kernel's Cartesian form of the
+// one-bare-charge pair at scale factors 1) against elec_pair_bare, for the
+// pairs whose bare site has no damping factor (no polarisability), and its
+// potential vectors against elec_pair_bare<R, true>: cart_*_rel.
 // Build and run:  make -C mpidopenmmplugin_amd/csrc pair_identity
 #include <algorithm>
 #include <cmath>
@@ -58,6 +62,7 @@ int main() {
     std::mt19937_64 rng(20261202);
     std::uniform_real_distribution<double> U(-1.0, 1.0);
     MaxRel de, df, dt, dtb;
+    MaxRel ce, cf, ct, cp;  // elec_pair_bare_cart against elec_pair_bare (scale factors 1)
     for (int trial = 0; trial < 4000; trial++) {
         const PairParams<double> P = params(trial & 1);
         V3<double> dr;
@@ -95,6 +100,22 @@ int main() {
         df.add(-s.f.x, b.f.x); df.add(-s.f.y, b.f.y); df.add(-s.f.z, b.f.z);
         dt.add(g.ti.x, b.ta.x); dt.add(g.ti.y, b.ta.y); dt.add(g.ti.z, b.ta.z);
         dt.add(s.tj.x, b.ta.x); dt.add(s.tj.y, b.ta.y); dt.add(s.tj.z, b.ta.z);
+        // the Cartesian form of the one-bare-charge pair, with its potential
+        // vectors, where b is what a bare site is: without polarisability,
+        // so without a damping factor
+        if (dmpb == 0.0) {
+            const ElecBareOut<double> q =
+                elec_pair_bare<double>(dr, r2, ma, mb[0], mua, ta, tb, anisoA, 1.0, 1.0, P);
+            const ElecBareOut<double> c = elec_pair_bare_cart<double>(dr, r2, ma, mb[0], mua, anisoA, P);
+            ce.add(q.energy, c.energy);
+            cf.add(q.f.x, c.f.x); cf.add(q.f.y, c.f.y); cf.add(q.f.z, c.f.z);
+            ct.add(q.ta.x, c.ta.x); ct.add(q.ta.y, c.ta.y); ct.add(q.ta.z, c.ta.z);
+            double pq[kMP] = {}, pc[kMP] = {}, bq = 0.0, bc = 0.0;
+            elec_pair_bare<double, true>(dr, r2, ma, mb[0], mua, ta, tb, anisoA, 1.0, 1.0, P, pq, &bq);
+            elec_pair_bare_cart<double, true>(dr, r2, ma, mb[0], mua, anisoA, P, pc, &bc);
+            for (int k = 0; k < kMP; k++) cp.add(pq[k], pc[k]);
+            cp.add(bq, bc);
+        }
         // the bare charge takes no torque in either orientation
         dtb.add(g.tj.x, 0.0); dtb.add(g.tj.y, 0.0); dtb.add(g.tj.z, 0.0);
         dtb.add(s.ti.x, 0.0); dtb.add(s.ti.y, 0.0); dtb.add(s.ti.z, 0.0);
@@ -104,5 +125,7 @@ int main() {
                 dt.rel(), dtb.diff);
     std::printf("energy_mag %.3e\nforce_mag %.3e\ntorque_mag %.3e\n", de.mag, df.mag, dt.mag);
     std::printf("max_rel_diff %.3e\n", worst);
+    std::printf("cart_energy_rel %.3e\ncart_force_rel %.3e\ncart_torque_rel %.3e\ncart_pot_rel %.3e\n", ce.rel(),
+                cf.rel(), ct.rel(), cp.rel());
     return 0;
 }

@@ -219,6 +219,47 @@ MPID_HD void induced_field_pair(V3<R> dr, R r2, V3<R> ui, V3<R> uj, R dampi, R d
 }
 
 // ---------------------------------------------------------------------------
+// Energy gradient with respect to the Thole damping of a pair
+// ---------------------------------------------------------------------------
+// h = u dE/du for a pair of polarisable sites (kJ/mol), u = a r / (damp_i
+// damp_j) the argument of the Thole damping and E the energy at fixed induced
+// dipoles ui, uj.  The damping enters the permanent field that each site's
+// induced dipole feels from the other (E contains -mu.E_perm, scale ds) and,
+// with mutual polarization, the dipole-dipole coupling (E contains
+// -1/2 mu.T mu, unscaled).  With lambda_k = 1 - exp(-u) P_k(u) as in
+// fixed_field_radial: dlambda3/du = e u^2/2, dlambda5/du = e u^3/6,
+// dlambda7/du = e (u^3 + u^4)/30, dlambda9/du = e (3u^3 + 3u^4 + u^5)/210.
+// The chain rule then gives dE/da = h/a and dE/ddamp_i = -h/damp_i.
+template <class R>
+MPID_HD R thole_u_derivative(V3<R> dr, R r2, const R* mi, const R* mj, V3<R> ui, V3<R> uj, R dampi, R dampj,
+                             R a, R ds, bool mutual) {
+    const R damp = dampi * dampj;
+    if (damp == R(0)) return R(0);
+    const R rinv = rsqrt_r(r2);
+    const R r = r2 * rinv;
+    const R u = div_r(a * r, damp);
+    if (!(u < R(50))) return R(0);
+    const R e = exp_r(-u), u2 = u * u, u3 = u2 * u, u4 = u3 * u, u5 = u4 * u;
+    const R dl3 = e * R(0.5) * u2, dl5 = e * u3 * R(1.0 / 6.0), dl7 = e * (u3 + u4) * R(1.0 / 30.0),
+            dl9 = e * (R(3) * u3 + R(3) * u4 + u5) * R(1.0 / 210.0);
+    const R rinv2 = rinv * rinv, rinv3 = rinv * rinv2, rinv5 = rinv3 * rinv2, rinv7 = rinv5 * rinv2,
+            rinv9 = rinv7 * rinv2;
+    R dE = R(0);
+    if (ds != R(0)) {
+        // d g_k/du of the permanent field's radial factors
+        const FixedRadial<R> g{ds * dl3 * rinv3, R(3) * ds * dl5 * rinv5, R(15) * ds * dl7 * rinv7,
+                               R(105) * ds * dl9 * rinv9};
+        dE = dot(ui, fixed_field_from<R>(mj, dr, g)) + dot(uj, fixed_field_from<R>(mi, dr * R(-1), g));
+    }
+    if (mutual) {
+        // d f1/du = -dlambda3/du / r^3, d f2/du = 3 dlambda5/du / r^5
+        const InducedRadial<R> f{-dl3 * rinv3, R(3) * dl5 * rinv5};
+        dE += dot(ui, induced_field_from<R>(uj, dr, f));
+    }
+    return -R(kElectric) * u * dE;
+}
+
+// ---------------------------------------------------------------------------
 // Final real-space energy, force and torque in the quasi-internal (QI) frame.
 // ---------------------------------------------------------------------------
 // Spherical moments in a frame (ex, ey, ez): Cartesian tensors are contracted
@@ -260,6 +301,72 @@ MPID_HD void qi_moments(const R* m, V3<R> ex, V3<R> ey, V3<R> ez, R* q) {
     q[15] = R(15) * r110 * (R(3) * oxxy - oyyy);
 }
 
+// The transpose of qi_moments: given V_k = dE/dq_k for the 16 spherical QI
+// moments, adds dE/dm_c for the 20 stored lab components m_c of the record
+// (q, dipole, quadrupole XX XY XZ YY YZ ZZ, octopole XXX XXY XXZ XYY XYZ XZZ
+// YYY YYZ YZZ ZZZ).  A stored off-diagonal component stands for every index
+// ordering it represents, so its derivative sums the orderings.
+template <class R> MPID_HD void sym2_add(R c, V3<R> a, V3<R> b, R* g) {
+    // c (a_k b_l + a_l b_k)/2 summed over the orderings of each stored pair
+    g[0] += c * a.x * b.x;
+    g[1] += c * (a.x * b.y + a.y * b.x);
+    g[2] += c * (a.x * b.z + a.z * b.x);
+    g[3] += c * a.y * b.y;
+    g[4] += c * (a.y * b.z + a.z * b.y);
+    g[5] += c * a.z * b.z;
+}
+template <class R> MPID_HD void sym3_add(R c, V3<R> a, V3<R> b, V3<R> d, R* g) {
+    // c a_k b_l d_m summed over the distinct orderings (k, l, m) of each stored triple
+    const R ax = a.x, ay = a.y, az = a.z, bx = b.x, by = b.y, bz = b.z, dx = d.x, dy = d.y, dz = d.z;
+    g[0] += c * ax * bx * dx;                                                       // xxx
+    g[1] += c * (ax * bx * dy + ax * by * dx + ay * bx * dx);                       // xxy
+    g[2] += c * (ax * bx * dz + ax * bz * dx + az * bx * dx);                       // xxz
+    g[3] += c * (ax * by * dy + ay * bx * dy + ay * by * dx);                       // xyy
+    g[4] += c * (ax * by * dz + ax * bz * dy + ay * bx * dz + ay * bz * dx + az * bx * dy + az * by * dx);  // xyz
+    g[5] += c * (ax * bz * dz + az * bx * dz + az * bz * dx);                       // xzz
+    g[6] += c * ay * by * dy;                                                       // yyy
+    g[7] += c * (ay * by * dz + ay * bz * dy + az * by * dy);                       // yyz
+    g[8] += c * (ay * bz * dz + az * by * dz + az * bz * dy);                       // yzz
+    g[9] += c * az * bz * dz;                                                       // zzz
+}
+template <class R>
+MPID_HD void qi_potential_to_lab(const R* V, V3<R> ex, V3<R> ey, V3<R> ez, R* g) {
+    const R s3 = R(1.7320508075688772);
+    g[0] += V[0];
+    const V3<R> d = ez * V[1] + ex * V[2] + ey * V[3];
+    g[1] += d.x;
+    g[2] += d.y;
+    g[3] += d.z;
+    R* Q = g + 4;
+    sym2_add(R(3) * V[4], ez, ez, Q);
+    sym2_add(R(2) * s3 * V[5], ex, ez, Q);
+    sym2_add(R(2) * s3 * V[6], ey, ez, Q);
+    sym2_add(s3 * V[7], ex, ex, Q);
+    sym2_add(-s3 * V[7], ey, ey, Q);
+    sym2_add(R(2) * s3 * V[8], ey, ex, Q);
+    const R r32 = R(1.2247448713915890), r35 = R(0.7745966692414834), r110 = R(0.31622776601683794);
+    R* O = g + 10;
+    sym3_add(R(15) * V[9], ez, ez, ez, O);
+    sym3_add(R(15) * r32 * V[10], ex, ez, ez, O);
+    sym3_add(R(15) * r32 * V[11], ey, ez, ez, O);
+    sym3_add(R(15) * r35 * V[12], ez, ex, ex, O);
+    sym3_add(-R(15) * r35 * V[12], ez, ey, ey, O);
+    sym3_add(R(30) * r35 * V[13], ez, ex, ey, O);
+    sym3_add(R(15) * r110 * V[14], ex, ex, ex, O);
+    sym3_add(-R(45) * r110 * V[14], ex, ey, ey, O);
+    sym3_add(R(45) * r110 * V[15], ex, ex, ey, O);
+    sym3_add(-R(15) * r110 * V[15], ey, ey, ey, O);
+}
+// The same for a potential with only its m = 0 components (V1, V4, V9)
+// nonzero besides the charge (a site facing a bare charge).
+template <class R> MPID_HD void qi_potential_to_lab_m0(R V1, R V4, R V9, V3<R> ez, R* g) {
+    g[1] += V1 * ez.x;
+    g[2] += V1 * ez.y;
+    g[3] += V1 * ez.z;
+    sym2_add(R(3) * V4, ez, ez, g + 4);
+    sym3_add(R(15) * V9, ez, ez, ez, g + 10);
+}
+
 // sum_i (dq_i/dtheta_k) V_i for rotations about the QI x, y and z axes.  The
 // generator tables were derived symbolically from the Cartesian definition of
 // the moments; they coincide with the hand-written tables at
@@ -284,14 +391,21 @@ template <class R> MPID_HD void qi_torque_sums(const R* q, const R* V, R& tx, R&
 // and the force on b along r.  For two bare charges (no higher moment, no
 // polarisability) it is the whole of the quasi-internal-frame interaction
 // below.
-template <class R>
-MPID_HD void charge_pair(V3<R> dr, R r2, R qa, R qb, const PairParams<R>& P, V3<R>& f, R& energy) {
+// kPot: also adds the potentials (energy per unit charge) at a and b to *pa, *pb.
+template <class R, bool kPot = false>
+MPID_HD void charge_pair(V3<R> dr, R r2, R qa, R qb, const PairParams<R>& P, V3<R>& f, R& energy, R* pa = nullptr,
+                         R* pb = nullptr) {
     const R rinv = rsqrt_r(r2);
     const R r = r2 * rinv;
     const R ar = P.alpha * r;
     const R e = exp_r(-ar * ar);
     const R ec = erfc_r(ar, e);
     const R c = R(kElectric) * qa * qb * rinv;
+    if constexpr (kPot) {
+        const R g = R(kElectric) * rinv * ec;
+        *pa += g * qb;
+        *pb += g * qa;
+    }
     energy = c * ec;
     f = dr * (c * (ec + ar * R(2.0 / kSqrtPi) * e) * rinv * rinv);
 }
@@ -320,10 +434,14 @@ template <class R> MPID_HD void qi_frame(V3<R> dr, R rinv, V3<R>& ex, V3<R>& ey,
     ey = cross(ez, ex);
 }
 
-template <class R>
+// kPot: also adds the pair's potential vectors dE/dm to pot_a and pot_b (20
+// stored lab components each, the induced dipoles held fixed); without
+// with_cc they leave out the charge-charge channel, as the energy does.
+template <class R, bool kPot = false>
 MPID_HD ElecOut<R> elec_pair(V3<R> dr, R r2, const R* ma, const R* mb, V3<R> mua, V3<R> mub,
                                                 R2<R> tda, R2<R> tdb, bool anisoA, bool anisoB, R ms, R ps,
-                                                const PairParams<R>& P, bool with_cc = true) {
+                                                const PairParams<R>& P, bool with_cc = true, R* pot_a = nullptr,
+                                                R* pot_b = nullptr) {
     const R rinv = rsqrt_r(r2);
     const R r = r2 * rinv;
     V3<R> ex, ey, ez;
@@ -547,6 +665,12 @@ MPID_HD ElecOut<R> elec_pair(V3<R> dr, R r2, const R* ma, const R* mb, V3<R> mua
         Va[c] = eP * qb[c]; Vb[c] = eP * qa[c]; VaR[c] = dP * qb[c]; VbR[c] = dP * qa[c];
     }
 
+    // Va, Vb are dE/dqa, dE/dqb at fixed induced dipoles (the induced terms
+    // enter with ua = mua/2 and twice the energy coefficient)
+    if constexpr (kPot) {
+        qi_potential_to_lab(Va, ex, ey, ez, pot_a);
+        qi_potential_to_lab(Vb, ex, ey, ez, pot_b);
+    }
     ElecOut<R> o;
     R energy = R(0), fIZ = R(0), fJZ = R(0);
 #pragma unroll
@@ -618,9 +742,12 @@ template <class R> struct ElecBareOut {
 // torque and all induced-induced terms; what remains is b's charge against
 // a's dipole, quadrupole, octopole (the m = 0 components along r) and a's
 // induced dipole.  Same formulas, same order of operations, for those terms.
-template <class R>
+// kPot: also adds a's potential vector (without its charge-charge channel)
+// to pot_a and b's potential (charge component only) to *pot_b0.
+template <class R, bool kPot = false>
 MPID_HD ElecBareOut<R> elec_pair_bare(V3<R> dr, R r2, const R* ma, R qb, V3<R> mua, R2<R> tda, R2<R> tdb,
-                                       bool anisoA, R ms, R ps, const PairParams<R>& P) {
+                                       bool anisoA, R ms, R ps, const PairParams<R>& P, R* pot_a = nullptr,
+                                       R* pot_b0 = nullptr) {
     const R rinv = rsqrt_r(r2);
     const R r = r2 * rinv;
     V3<R> ex, ey, ez;
@@ -676,6 +803,10 @@ MPID_HD ElecBareOut<R> elec_pair_bare(V3<R> dr, R r2, const R* ma, R qb, V3<R> m
     Vb0 -= eP * qa[9];
     VbR0 -= dP * qa[9];
 
+    if constexpr (kPot) {
+        qi_potential_to_lab_m0(Va1, Va4, Va9, ez, pot_a);
+        *pot_b0 += Vb0;
+    }
     ElecBareOut<R> o;
     o.energy = R(0.5) * (qa[1] * Va1 + qa[4] * Va4 + qa[9] * Va9 + qb * Vb0);
     const R fIZ = qa[1] * VaR1 + qa[4] * VaR4 + qa[9] * VaR9, fJZ = qb * VbR0;
@@ -690,6 +821,61 @@ MPID_HD ElecBareOut<R> elec_pair_bare(V3<R> dr, R r2, const R* ma, R qb, V3<R> m
     if (anisoA) { tix -= iEIX; tiy -= iEIY; }
     o.f = ex * qfx + ey * qfy + ez * qfz;
     o.ta = ex * tix + ey * tiy;
+    return o;
+}
+
+// The pair of elec_pair_bare at scale factors 1 (every pair of the tile
+// kernel), in Cartesian form.  b's charge sits in the potential of a's
+// moments, phi(dr) = B1 (d.dr) + B2 (dr.Q.dr) + B3 (O:dr dr dr) with
+// dr = r_b - r_a and B_k the Ewald-screened radial functions (B_k' =
+// -r B_(k+1)); the charge-charge term is left to the caller, as there.  b is
+// not polarisable, so no Thole damping enters.  a's induced dipole adds to
+// d: half of it in the energy (the reference's polarization energy), all of
+// it in the force, and in the torque on a only when a is anisotropic (the
+// reference's convention).  Energy q_b phi, force on b -q_b grad phi, torque
+// on a -q_b (B1 d x dr + 2 B2 (Q.dr) x dr + 3 B3 (O:dr dr) x dr).  Same
+// quantities as elec_pair_bare, in about a third of its arithmetic.
+// kPot: adds dE/dm of a's stored components (the charge one excepted) to
+// pot_a and b's potential to *pot_b0, as elec_pair_bare<R, true>.
+template <class R, bool kPot = false>
+MPID_HD ElecBareOut<R> elec_pair_bare_cart(V3<R> dr, R r2, const R* ma, R qb, V3<R> mua, bool anisoA,
+                                           const PairParams<R>& P, R* pot_a = nullptr, R* pot_b0 = nullptr) {
+    const R rinv = rsqrt_r(r2);
+    const R r = r2 * rinv;
+    const R rinv2 = rinv * rinv;
+    const R ar = P.alpha * r;
+    const R ex2 = exp_r(-ar * ar);
+    const R B0 = erfc_r(ar, ex2) * rinv;
+    const R B1 = (B0 + P.aln[1] * ex2) * rinv2;
+    const R B2 = (R(3) * B1 + P.aln[2] * ex2) * rinv2;
+    const R B3 = (R(5) * B2 + P.aln[3] * ex2) * rinv2;
+    const R B4 = (R(7) * B3 + P.aln[4] * ex2) * rinv2;
+    const V3<R> d = mk(ma[1], ma[2], ma[3]);
+    const V3<R> qr = quad_dot(ma + 4, dr);  // Q.dr
+    R M[6];
+    oct_dot(ma + 10, dr, M);
+    const V3<R> orr = quad_dot(M, dr);  // O:dr dr
+    const R dd = dot(d, dr), du = dot(mua, dr), qrr = dot(qr, dr), orrr = dot(orr, dr);
+    const R ce = R(kElectric) * qb;
+    ElecBareOut<R> o;
+    o.energy = ce * (B1 * (dd + R(0.5) * du) + B2 * qrr + B3 * orrr);
+    // grad[B1 d.r] = B1 d - B2 r (d.r); grad[B2 r.Q.r] = 2 B2 Q.r - B3 r (r.Q.r);
+    // grad[B3 O:rrr] = 3 B3 O:rr - B4 r (O:rrr)
+    const V3<R> dfv = d + mua;
+    const R df = dd + du;
+    const V3<R> g = dfv * B1 + qr * (R(2) * B2) + orr * (R(3) * B3) - dr * (B2 * df + B3 * qrr + B4 * orrr);
+    o.f = g * (-ce);
+    const V3<R> td = anisoA ? dfv : d;
+    o.ta = (cross(td, dr) * B1 + cross(qr, dr) * (R(2) * B2) + cross(orr, dr) * (R(3) * B3)) * (-ce);
+    if constexpr (kPot) {
+        const R c1 = ce * B1, c2 = ce * B2, c3 = ce * B3;
+        pot_a[1] += c1 * dr.x;
+        pot_a[2] += c1 * dr.y;
+        pot_a[3] += c1 * dr.z;
+        sym2_add(c2, dr, dr, pot_a + 4);
+        sym3_add(c3, dr, dr, dr, pot_a + 10);
+        *pot_b0 += R(kElectric) * (B1 * df + B2 * qrr + B3 * orrr);
+    }
     return o;
 }
 

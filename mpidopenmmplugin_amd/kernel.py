@@ -167,6 +167,41 @@ class HipCalcMPIDForceKernel:
         params, keep = _capi.make_parameters(flat)
         self._check(self._lib.mpid_update_parameters(self._ctx, ctypes.byref(params)))
 
+    def updateParameterArrays(self, flat):
+        """copyParametersToContext from flat arrays (the layout of
+        ``MPIDForce.flatten()``): charges, dipoles, quadrupoles, octopoles,
+        polarity and tholes may change, the frames and covalent maps not."""
+        params, keep = _capi.make_parameters(flat)
+        self._check(self._lib.mpid_update_parameters(self._ctx, ctypes.byref(params)))
+
+    def energyGradients(self, positions, box, all_sites=False):
+        """Energy (kJ/mol) and its gradients with respect to the positions and
+        the per-particle parameters (mpid_energy_gradients): a dict of numpy
+        arrays in the layouts of ``MPIDForce.flatten()`` -- positions [n,3],
+        charges [n], dipoles [n,3], quadrupoles [n,6], octopoles [n,10],
+        polarity [n,3], tholes [n] -- plus default_thole (float), energy and
+        adjoint_iterations.  ``all_sites``: also the gradients of the higher
+        moments and polarisabilities of bare-charge sites."""
+        n = self._n
+        pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1)
+        out = dict(positions=np.zeros((n, 3)), charges=np.zeros(n), dipoles=np.zeros((n, 3)),
+                   quadrupoles=np.zeros((n, 6)), octopoles=np.zeros((n, 10)), polarity=np.zeros((n, 3)),
+                   tholes=np.zeros(n))
+        dth = np.zeros(1)
+        g = _capi.Gradients()
+        for name, arr in out.items():
+            setattr(g, name, _capi.dptr(arr))
+        g.default_thole = _capi.dptr(dth)
+        e = ctypes.c_double(0.0)
+        b = _box9(box)
+        flags = _capi.MPID_GRADIENT_ALL_SITES if all_sites else 0
+        self._check(self._lib.mpid_energy_gradients(self._ctx, _capi.dptr(pos), _capi.dptr(b), flags,
+                                                    ctypes.byref(e), ctypes.byref(g)))
+        out["default_thole"] = float(dth[0])
+        out["energy"] = e.value
+        out["adjoint_iterations"] = int(g.adjoint_iterations)
+        return out
+
     def getPMEParameters(self):
         a = ctypes.c_double()
         nx, ny, nz = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

@@ -1567,10 +1567,43 @@ struct Oracle {
         }
     }
 
+    // ThreeFold frame by the exact chain rule (flag 1; the reference's own
+    // mapping at :2068-2097 is an approximate projection).  The frame is
+    // z = s/|s|, s = u + v + w (unit vectors to the z, x and y atoms), and
+    // x = t/|t|, t = v - (v.z) z.  Moving an atom turns the frame by
+    // dtheta = z x dz + (y.dx) z, so with tau = -dE/dtheta the force on atom a
+    // is F_a = (dz/dr_a)^T (tau x z) + (tau.z) (dx/dr_a)^T y.
+    void threefold_exact(int i, const V3& tq, std::vector<V3>& F) const {
+        const V3 pI = site[i].pos;
+        const int iz = S.atom_z[i], ix = S.atom_x[i], iy = S.atom_y[i];
+        V3 u = site[iz].pos - pI, v = site[ix].pos - pI, w = site[iy].pos - pI;
+        const double nu = normalize(u), nv = normalize(v), nw = normalize(w);
+        V3 z = u + v + w;
+        const double ns = normalize(z);
+        V3 x = v - z * v.dot(z);
+        const double nt = normalize(x);
+        const V3 y = z.cross(x);
+        const V3 c = y * (tq.dot(z) / nt);             // through x: on t, hence on v and on z
+        const V3 bz = tq.cross(z) - c * v.dot(z);       // everything that moves with z
+        const V3 gs = (bz - z * z.dot(bz)) * (1.0 / ns);  // on s = u + v + w
+        auto perp = [](const V3& e, const V3& g) { return g - e * e.dot(g); };
+        const V3 FZ = perp(u, gs) * (1.0 / nu);
+        const V3 FX = perp(v, gs + c) * (1.0 / nv);
+        const V3 FY = perp(w, gs) * (1.0 / nw);
+        F[iz] += FZ;
+        F[ix] += FX;
+        F[iy] += FY;
+        F[i] -= FZ + FX + FY;
+    }
+
     // mapTorqueToForceForParticle, :1895-2110
     void torque_to_force(int i, const V3& tq, std::vector<V3>& F) const {
         int t = S.axis_types[i];
         if (t == 5 || S.atom_z[i] < 0) return;
+        if (t == 3 && (S.flags & 1) && S.atom_y[i] >= 0) {
+            threefold_exact(i, tq, F);
+            return;
+        }
         const V3 pI = site[i].pos;
         int iz = S.atom_z[i], ix = S.atom_x[i], iy = S.atom_y[i];
         double nU, nV, nW;
@@ -1813,6 +1846,32 @@ int oracle_evaluate(const oracle_system* sys, const double* pos, oracle_out* out
             out->energy_terms[2] = eind;
             out->energy_terms[3] = eself;
             out->energy = ereal + efix + eind + eself;
+            out->energy_variational = out->energy;
+            if (sys->polarization == 0) {
+                // residual r = E0 + T mu - alpha^-1 mu of the solution (mu = 0 where alpha = 0)
+                double mr = 0.0;
+                for (int i = 0; i < n; i++) {
+                    const double* a = o.site[i].labpol;
+                    const double A[3][3] = {{a[0], a[1], a[2]}, {a[1], a[3], a[4]}, {a[2], a[4], a[5]}};
+                    const double det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) -
+                                       A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0]) +
+                                       A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+                    if (det == 0.0) continue;
+                    // alpha^-1 mu by Cramer's rule
+                    const V3& m = o.mu[i];
+                    double x[3];
+                    for (int c = 0; c < 3; c++) {
+                        double B[3][3];
+                        for (int r = 0; r < 3; r++)
+                            for (int k = 0; k < 3; k++) B[r][k] = (k == c) ? m[r] : A[r][k];
+                        x[c] = (B[0][0] * (B[1][1] * B[2][2] - B[1][2] * B[2][1]) -
+                                B[0][1] * (B[1][0] * B[2][2] - B[1][2] * B[2][0]) +
+                                B[0][2] * (B[1][0] * B[2][1] - B[1][1] * B[2][0])) / det;
+                    }
+                    for (int k = 0; k < 3; k++) mr += m[k] * (o.E0[i][k] + Ei[i][k] - x[k]);
+                }
+                out->energy_variational -= 0.5 * kElectric * mr;
+            }
             for (int i = 0; i < n; i++) {
                 const Site& s = o.site[i];
                 for (int k = 0; k < 3; k++) {

@@ -68,6 +68,11 @@ typedef struct {
     double* forces;          /* [3n]  overwritten (not accumulated) */
     double energy_terms[4];  /* real space, recip fixed, recip induced, self */
     double energy;
+    /* Mutual polarization: the energy plus -1/2 eps mu.(E0 + T mu - alpha^-1 mu),
+     * which vanishes at the exact solution and makes the value stationary in
+     * the dipoles (its error is second order in the solver's); otherwise the
+     * energy itself.  For finite differences in the parameters. */
+    double energy_variational;
     int iterations;
     double eps;
     int converged;

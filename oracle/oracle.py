@@ -47,7 +47,7 @@ class _Out(ctypes.Structure):
         ("lab_dipoles", _dp), ("lab_quads", _dp), ("lab_octs", _dp), ("lab_pol", _dp),
         ("fixed_field", _dp), ("induced", _dp), ("phi", _dp), ("phidp", _dp),
         ("torques", _dp), ("forces", _dp),
-        ("energy_terms", ctypes.c_double * 4), ("energy", ctypes.c_double),
+        ("energy_terms", ctypes.c_double * 4), ("energy", ctypes.c_double), ("energy_variational", ctypes.c_double),
         ("iterations", ctypes.c_int), ("eps", ctypes.c_double), ("converged", ctypes.c_int),
         ("error", ctypes.c_char * 256),
     ]
@@ -155,7 +155,7 @@ def evaluate(flat, positions, box, polarization=None, details=False, conservativ
     rc = lib().oracle_evaluate(ctypes.byref(s), _ptr(pos, "d"), ctypes.byref(out))
     if rc != 0:
         raise OracleError(out.error.decode())
-    res = dict(energy=out.energy, forces=bufs["forces"].reshape(n, 3),
+    res = dict(energy=out.energy, energy_variational=out.energy_variational, forces=bufs["forces"].reshape(n, 3),
                energy_terms=list(out.energy_terms), iterations=out.iterations, eps=out.eps)
     if details:
         for name, k in sizes.items():
@@ -218,11 +218,12 @@ def system_multipole_moments(flat, positions, box, masses):
     return out
 
 
-def torque_to_force(flat, positions, torques):
+def torque_to_force(flat, positions, torques, conservative_torques=False):
     """mapTorqueToForce alone: the forces [n,3] that the frame chain rule
-    gives for per-site torques [n,3] at the given positions."""
+    gives for per-site torques [n,3] at the given positions
+    (``conservative_torques``: ThreeFold frames by the exact chain rule)."""
     n = int(flat["n"])
-    s, keep = _system(flat, None)
+    s, keep = _system(flat, None, flags=1 if conservative_torques else 0)
     pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1)
     t = np.ascontiguousarray(torques, dtype=np.float64).reshape(-1)
     out = np.zeros(3 * n)

@@ -191,3 +191,66 @@ def _bfs_counts(adj, a):
                 dist[c] = dist[b] + 1
                 q.append(c)
     return list(dist.values())
+
+
+def replicate_force(force, copies, alpha, grid):
+    """An MPIDForce of `copies` copies of `force` (atoms of copy c offset by
+    c * n in every index), PME with the given alpha and grid, the cutoff,
+    Thole width, solver and polarization settings of `force`."""
+    n = force.getNumMultipoles()
+    out = MPIDForce()
+    for c in range(copies):
+        o = c * n
+        for i in range(n):
+            q, d, quad, octo, axis, z, x, y, thole, pol = force.getMultipoleParameters(i)
+
+            def sh(a):
+                return a + o if a >= 0 else -1
+
+            out.addMultipole(q, d, quad, octo, axis, sh(z), sh(x), sh(y), thole, pol)
+    for c in range(copies):
+        o = c * n
+        for i in range(n):
+            for t in range(MPIDForce.CovalentEnd):
+                m = force.getCovalentMap(i, t)
+                if len(m):
+                    out.setCovalentMap(i + o, t, [a + o for a in m])
+    out.setNonbondedMethod(MPIDForce.PME)
+    out.setCutoffDistance(force.getCutoffDistance())
+    out.setPMEParameters(alpha, *grid)
+    out.setDefaultTholeWidth(force.getDefaultTholeWidth())
+    out.setMutualInducedMaxIterations(force.getMutualInducedMaxIterations())
+    out.setMutualInducedTargetEpsilon(force.getMutualInducedTargetEpsilon())
+    out.setPolarizationType(force.getPolarizationType())
+    return out
+
+
+# A reduced triclinic cell (a along x, b in the xy plane, |bx|, |cx| <= ax/2,
+# |cy| <= by/2) whose lattice is a sublattice of the cubic lattice of edge L:
+# vectors (2L, 0, 0), (L, 2L, 0), (L, L, L), volume 4 L^3, every off-diagonal
+# at its largest allowed value.  A cubic cell's contents placed at the four
+# coset shifts 0, (L, 0, 0), (0, L, 0) and (L, L, 0) fill it with the same
+# infinite periodic system.
+TRICLINIC_SUPERCELL = np.array([[2, 0, 0], [1, 2, 0], [1, 1, 1]], dtype=np.float64)
+TRICLINIC_SHIFTS = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0]], dtype=np.float64)
+
+
+def triclinic_supercell(force, pos, L, alpha, grid):
+    """(force, positions, box vectors) of the four-copy triclinic supercell
+    of a cubic system of edge L; grid: the PME grid of the cubic cell
+    (the supercell takes 2g, 2g, g, the same points per unit length)."""
+    f4 = replicate_force(force, 4, alpha, (2 * grid, 2 * grid, grid))
+    p4 = np.concatenate([pos + s * L for s in TRICLINIC_SHIFTS])
+    return f4, p4, TRICLINIC_SUPERCELL * L
+
+
+def sheared_waters(pos, L, vecs):
+    """Water positions of a cubic cell of edge L moved rigidly, molecule by
+    molecule (atoms 3m, 3m+1, 3m+2), so that each oxygen sits at the same
+    fractional coordinates of the cell with box vectors vecs (rows)."""
+    out = np.array(pos, dtype=np.float64)
+    for m in range(len(pos) // 3):
+        s = pos[3 * m] / L
+        new_o = s @ vecs
+        out[3 * m:3 * m + 3] += new_o - pos[3 * m]
+    return out

@@ -46,3 +46,22 @@ def test_every_frame_type(precision, periodic):
     assert rel_rms(ctx.getInducedDipoles(), ref["induced"]) <= mu_tol
     assert np.allclose(ctx.getLabFramePermanentDipoles(), ref["lab_dipoles"], rtol=0, atol=1e-12 if
                        precision == "double" else 1e-9)
+
+
+@pytest.mark.parametrize("periodic", [False, True], ids=["nocutoff", "pme"])
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_conservative_torques(precision, periodic):
+    """With MPID_FLAG_CONSERVATIVE_TORQUES, ThreeFold torques go through the
+    exact chain rule of the frame: the forces against the oracle's
+    conservative mapping (itself pinned by central differences of the
+    energy, tests/test_oracle_frames.py)."""
+    print(f"[gpu] frames cluster conservative {precision} periodic={periodic}", flush=True)
+    tight, pos, box = frames_cluster(periodic=periodic, target_eps=1e-10)
+    ref = oracle.evaluate(tight.flatten(), pos, box, conservative_torques=True)
+    f = tight if precision == "double" else frames_cluster(periodic=periodic, target_eps=1e-8)[0]
+    ctx = MPIDContext(f, box, precision=precision, conservative_torques=True)
+    ctx.setPositions(pos)
+    st = ctx.getState()
+    e_tol, f_tol = (1e-8, 1e-8) if precision == "double" else (1e-6, 1e-5)
+    assert abs(st.getPotentialEnergy() - ref["energy"]) <= e_tol * abs(ref["energy"])
+    assert rel_rms(st.getForces(), ref["forces"]) <= f_tol

@@ -45,3 +45,7 @@ def test_counts_follow_the_thole_branch(counts):
     assert 0 < d - u <= 4
     # a pair of bare charges costs a small fraction of the general interaction
     assert counts["charge_pair"][0] * 50 < counts["elec_pair"][1]
+
+
+def test_cartesian_bare_formula_is_cheaper(counts):
+    assert 0 < counts["elec_pair_bare_cart"][0] * 2 < counts["elec_pair_bare"][0]

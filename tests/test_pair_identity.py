@@ -30,3 +30,6 @@ def test_bare_charge_formula_matches_the_general_one():
     assert v["force_rel"] < 1e-13
     assert v["torque_rel"] < 1e-13
     assert v["bare_torque_abs"] == 0.0
+    # the Cartesian form of the tile kernel, a different order of operations
+    for k in ("cart_energy_rel", "cart_force_rel", "cart_torque_rel", "cart_pot_rel"):
+        assert v[k] < 1e-13, k
