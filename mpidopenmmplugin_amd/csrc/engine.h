@@ -371,8 +371,9 @@ void launch_exclusive_scan(hipStream_t s, int m, const int* counts, int* start);
 // row entry (nrows entries); polmask: one word per block; scalars[0]:
 // largest stored list length, scalars[1]: nonzero if a block pair lacked its
 // mirror entry.  act[ustart[I] + k] = {r, I, J} for the k-th row entry r of
-// row I with J >= I (ustart: the exclusive scan of launch_block_rows'
-// upper, whose last element is the number of such entries).  nrows: row entries launched (the count, or the capacity
+// row I with J >= I, r = -1 if the block pair holds no pair (ustart: the
+// exclusive scan of launch_block_rows' upper, whose last element is the
+// number of such entries).  nrows: row entries launched (the count, or the capacity
 // when the count is not known on the host: entries past rstart[nb] return);
 // rcap: the capacity of rows / nmask (entries beyond it are ignored).
 void launch_nl_build(hipStream_t s, int n, int nrows, int rcap, bool all, const int* rstart, const int* rows,

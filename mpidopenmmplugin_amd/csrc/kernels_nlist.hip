@@ -235,8 +235,10 @@ __global__ __launch_bounds__(64) void k_nl_masks(int n, const int* __restrict__ 
     tmaskcc[(size_t)r * kTile + lane] = cc & upper;
     // the tile's entry in the electrostatics kernels' work list: the row
     // entries J >= I of row I are its last upper[I] entries, placed at
-    // ustart[I] (the exclusive scan of upper) in the same order
-    if (lane == 0) act[ustart[I + 1] - (rstart[I + 1] - r)] = make_int4(r, I, J, 0);
+    // ustart[I] (the exclusive scan of upper) in the same order; a tile
+    // without pairs is marked r = -1
+    const bool has_pairs = __ballot((m & upper) != 0ull) != 0ull;
+    if (lane == 0) act[ustart[I + 1] - (rstart[I + 1] - r)] = make_int4(has_pairs ? r : -1, I, J, 0);
     if (J > I) {
         unsigned long long t = 0;
         for (int k = 0; k < kTile; k++) {

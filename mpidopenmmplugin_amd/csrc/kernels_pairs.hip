@@ -96,9 +96,10 @@ __device__ __forceinline__ void electrostatics_tile(const int4* __restrict__ act
     const int n = P.n;
     // One workgroup per entry {r, I, J} of the list of block pairs (J >= I);
     // the launch covers an upper bound of its length (the entries past it
-    // lie inside the buffer and are not used).
+    // lie inside the buffer and are not used), and r = -1 marks a block
+    // pair without pairs.
     const int4 tile = act[blockIdx.x];
-    if ((int)blockIdx.x >= *nact) return;
+    if ((int)blockIdx.x >= *nact || tile.x < 0) return;
     const int r = tile.x, I = tile.y, J = tile.z;
     // A workgroup's wavefronts sit on fixed SIMDs of a CU, and a tile's work
     // is uneven: one wavefront takes the general pairs (about four times the

@@ -289,6 +289,12 @@ struct mpid_context {
     int prof_count[MPID_PROF_COUNT] = {};
 
     ~mpid_context() {
+        // Work still in flight must end before its streams, events, pinned
+        // words and buffers go: a call that ended in an error can leave the
+        // solver's issued-ahead iteration (which writes hscal through its
+        // mapping) or the check stream running.
+        for (hipStream_t s : {stream, rstream, vstream})
+            if (s) (void)hipStreamSynchronize(s);
         for (auto e : pool) (void)hipEventDestroy(e);
         if (cg_exec) (void)hipGraphExecDestroy(cg_exec);
         if (hscal) (void)hipHostFree(hscal);
