@@ -273,6 +273,7 @@ def run(args):
     pc = count_pairs(pos, box, cfg["cutoff"], flat)
     mixed = args.precision == "mixed"
     rooflines = {}
+    traffic = profiled_traffic(args.workload, args.precision)
     for cat, kv in kernels.items():
         ms, nl = kv["ms_per_step"], kv["launches_per_step"]
         if ms <= 0 or nl <= 0:
@@ -283,7 +284,8 @@ def run(args):
             achieved = flops / (avg_ms * 1e-3) / 1e12
             peak = MI355X_PEAK["fp32_TFLOPs"] if mixed else MI355X_PEAK["fp64_TFLOPs"]
             rooflines[cat] = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                              "frac": achieved / peak, "traffic": None, "avg_launch_ms": avg_ms,
+                              "frac": achieved / peak, "traffic": traffic[cat] / nl if cat in traffic else None,
+                              "avg_launch_ms": avg_ms,
                               "flops_per_launch": flops,
                               "note": "vector ALU arithmetic (no matrix instructions); peak is MI355X's published "
                                       "dense FP32/FP64 rate"}
@@ -293,7 +295,8 @@ def run(args):
                 continue
             achieved = b / (ms * 1e-3) / 1e9
             rooflines[cat] = {"bound": "hbm", "achieved": achieved, "peak": MI355X_PEAK["hbm_GBps"], "unit": "GB/s",
-                              "frac": achieved / MI355X_PEAK["hbm_GBps"], "traffic": None, "avg_launch_ms": avg_ms,
+                              "frac": achieved / MI355X_PEAK["hbm_GBps"],
+                              "traffic": traffic[cat] / nl if cat in traffic else None, "avg_launch_ms": avg_ms,
                               "bytes_per_step": b}
     # the dominant single kernel: the electrostatics, timed live in the timed
     # region; HBM traffic per launch from the committed rocprofv3 counters
@@ -302,7 +305,7 @@ def run(args):
     e_peak = MI355X_PEAK["fp32_TFLOPs"] if mixed else MI355X_PEAK["fp64_TFLOPs"]
     e_ach = e_flops / (e_avg * 1e-3) / 1e12
     roof = {"bound": "valu", "achieved": e_ach, "peak": e_peak, "unit": "TFLOP/s", "frac": e_ach / e_peak,
-            "traffic": profiled_traffic(args.workload, args.precision), "kernel": "k_electrostatics_" +
+            "traffic": traffic.get("electrostatic_pairs"), "kernel": "k_electrostatics_" +
             ("f32" if mixed else "f64"), "avg_launch_ms": e_avg, "launches_timed": live_n,
             "flops_per_launch": e_flops,
             "note": "vector ALU arithmetic, no matrix instructions; peak is MI355X's published dense FP32/FP64 "
@@ -515,17 +518,20 @@ def adjoint_leg(args):
 
 
 def profiled_traffic(workload, precision):
-    """HBM bytes per launch of the electrostatics kernel from the committed
-    rocprofv3 summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
-    calibration recorded there), or None."""
-    path = os.path.join(ROOT, "profiles", f"traffic_electrostatics_{workload}_{precision}.md")
+    """HBM bytes per evaluation of each kernel family from the committed
+    rocprofv3 summary (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the
+    copy-kernel calibration recorded there, times the dispatches per
+    evaluation): {family: bytes}, empty if there is none."""
+    path = os.path.join(ROOT, "profiles", f"counters_{workload}_{precision}.md")
+    out = {}
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes_per_launch:"):
-                return float(line.split(":", 1)[1])
+            if line.startswith("traffic_bytes_per_evaluation["):
+                fam = line[line.index("[") + 1:line.index("]")]
+                out[fam] = float(line.split(":", 1)[1])
     except OSError:
         pass
-    return None
+    return out
 
 
 def calibrate_copy(mib):

@@ -444,7 +444,7 @@ __global__ __launch_bounds__(256) void k_nl_induced_field(NbrList L, const int* 
                                                           PairAtoms<R> A, const R* __restrict__ mu, PairParams<R> P,
                                                           acc_t* __restrict__ field, acc_t* __restrict__ grad) {
     if (skipped(P.skip)) return;
-    const unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned g = (unsigned)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int t = (int)(g / kNlSplit), sub = (int)(g % kNlSplit);
     const int n = P.n;
     const bool iv = t < natoms;

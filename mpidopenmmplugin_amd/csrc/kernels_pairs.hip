@@ -98,8 +98,9 @@ __device__ __forceinline__ void electrostatics_tile(const int4* __restrict__ act
     // the launch covers an upper bound of its length (the entries past it
     // lie inside the buffer and are not used), and r = -1 marks a block
     // pair without pairs.
-    const int4 tile = act[blockIdx.x];
-    if ((int)blockIdx.x >= *nact || tile.x < 0) return;
+    const int w = blockIdx.x;
+    const int4 tile = act[w];
+    if (w >= *nact || tile.x < 0) return;
     const int r = tile.x, I = tile.y, J = tile.z;
     // A workgroup's wavefronts sit on fixed SIMDs of a CU, and a tile's work
     // is uneven: one wavefront takes the general pairs (about four times the
@@ -110,10 +111,11 @@ __device__ __forceinline__ void electrostatics_tile(const int4* __restrict__ act
 
     // Every load of the tile is issued here, before any of it is used, so
     // that their latencies overlap: the masks and bare-charge flags for the
-    // ranking; roles 0 and 3 the per-atom fields of sides 0 and 1; all
-    // threads their share of the two 64 x 20 multipole blocks (contiguous in
-    // HBM and in LDS, copied as 16-byte vectors; atoms past the last read as
-    // zeros).
+    // ranking; roles 0 and 3 the per-atom fields of sides 0 and 1; and the
+    // multipole rows, two threads per atom (thread t: atom t mod 64, side
+    // (t / 64) mod 2, half t / 128 of the row's 16-byte vectors).  A bare
+    // charge's row is its charge and zeros (its other components are zero),
+    // so only its first vector is read; atoms past the last read as zeros.
     bool bi = chg[min(I * kTile + lane, n - 1)] != 0;
     const bool bjl = chg[min(J * kTile + lane, n - 1)] != 0;
     unsigned long long mc = tmaskcc[(size_t)r * kTile + lane];
@@ -136,17 +138,20 @@ __device__ __forceinline__ void electrostatics_tile(const int4* __restrict__ act
         st_q = lab64[(size_t)sa * kMP];
     }
     typedef R vec __attribute__((ext_vector_type(kV)));
-    constexpr int kVecs = kTile * kMP / kV;  // per side
-    constexpr int kVecPer = (2 * kVecs + kElecThreads - 1) / kElecThreads;
-    vec st_mp[kVecPer];
+    static_assert(kElecThreads == 4 * kTile, "two sides x two halves of the multipole rows");
+    constexpr int kVA = kMP / kV;            // vectors per row
+    constexpr int kHalf = (kVA + 1) / 2;     // vectors per thread
+    const int cside = (tid >> 6) & 1, cpart = tid >> 7;
+    const int catom = (cside == 0 ? I : J) * kTile + lane;
+    const bool cbare = cside == 0 ? bi : bjl;
+    vec st_mp[kHalf];
+    {
+        const vec* row = reinterpret_cast<const vec*>(A.lab + (size_t)min(catom, n - 1) * kMP);
 #pragma unroll
-    for (int u = 0; u < kVecPer; u++) {
-        const int e = tid + u * kElecThreads;
-        const int side = e >= kVecs, v = e - side * kVecs;
-        const int a0 = (side == 0 ? I : J) * kTile;
-        const int lim = (min(a0 + kTile, n) - a0) * (kMP / kV);
-        const vec* src = reinterpret_cast<const vec*>(A.lab + (size_t)a0 * kMP);
-        st_mp[u] = (e < 2 * kVecs && v < lim) ? src[v] : vec(R(0));
+        for (int u = 0; u < kHalf; u++) {
+            const int v = cpart * kHalf + u;
+            st_mp[u] = (catom < n && v < kVA && (v == 0 || !cbare)) ? row[v] : vec(R(0));
+        }
     }
 
     // Every wavefront ranks the tile's pairs (prefix sums of the lanes'
@@ -206,12 +211,9 @@ __device__ __forceinline__ void electrostatics_tile(const int4* __restrict__ act
         }
     }
 #pragma unroll
-    for (int u = 0; u < kVecPer; u++) {
-        const int e = tid + u * kElecThreads;
-        if (e < 2 * kVecs) {
-            const int side = e >= kVecs, v = e - side * kVecs;
-            reinterpret_cast<vec*>(&smp[side][0])[v] = st_mp[u];
-        }
+    for (int u = 0; u < kHalf; u++) {
+        const int v = cpart * kHalf + u;
+        if (v < kVA) reinterpret_cast<vec*>(&smp[cside][0])[lane * kVA + v] = st_mp[u];
     }
     __syncthreads();
     // Within a wavefront's chunk of 64 entries, lane l takes entry

@@ -591,7 +591,7 @@ __global__ __launch_bounds__(256) void k_fold_tiles(PmeGeom<R> g, TileGeom tg, F
 template <class R>
 __global__ __launch_bounds__(256) void k_gather_phi(int n, const R4<R>* __restrict__ pos, PmeGeom<R> g,
                                                      const R* __restrict__ grid, R* __restrict__ phi) {
-    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gt = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int t = gt >> 3, iz = gt & 7;
     const bool tv = t < n;
     const int i = tv ? t : n - 1;
@@ -615,26 +615,25 @@ __global__ __launch_bounds__(256) void k_gather_phi(int n, const R4<R>* __restri
     if (tv && iz < kOrder) {
         int z = ig[2] + iz;
         if (z >= nz) z -= nz;
-        R q[kOrder][kOrder];
-#pragma unroll
-        for (int iy = 0; iy < kOrder; iy++) {
-            int y = ig[1] + iy;
-            if (y >= ny) y -= ny;
-#pragma unroll
-            for (int ix = 0; ix < kOrder; ix++) {
-                int x = ig[0] + ix;
-                if (x >= g.n[0]) x -= g.n[0];
-                q[iy][ix] = grid[((size_t)x * ny + y) * nz + z];
-            }
-        }
+        // each row of six grid values is used as it arrives (holding all 36
+        // put the FP64 kernel's arrays in scratch memory)
         R b00 = 0, b10 = 0, b01 = 0, b20 = 0, b11 = 0, b02 = 0, b30 = 0, b21 = 0, b12 = 0, b03 = 0, b40 = 0, b04 = 0,
           b31 = 0, b13 = 0, b22 = 0;
 #pragma unroll
         for (int iy = 0; iy < kOrder; iy++) {
+            int y = ig[1] + iy;
+            if (y >= ny) y -= ny;
+            R row[kOrder];
+#pragma unroll
+            for (int ix = 0; ix < kOrder; ix++) {
+                int x = ig[0] + ix;
+                if (x >= g.n[0]) x -= g.n[0];
+                row[ix] = grid[((size_t)x * ny + y) * nz + z];
+            }
             R a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
 #pragma unroll
             for (int ix = 0; ix < kOrder; ix++) {
-                const R v = q[iy][ix];
+                const R v = row[ix];
                 a0 += v * tx[ix][0];
                 a1 += v * tx[ix][1];
                 a2 += v * tx[ix][2];
@@ -646,7 +645,18 @@ __global__ __launch_bounds__(256) void k_gather_phi(int n, const R4<R>* __restri
             b02 += a0 * u2; b30 += a3 * u0; b21 += a2 * u1; b12 += a1 * u2; b03 += a0 * u3;
             b40 += a4 * u0; b04 += a0 * u4; b31 += a3 * u1; b13 += a1 * u3; b22 += a2 * u2;
         }
-        const R v0 = tz[iz][0], v1 = tz[iz][1], v2 = tz[iz][2], v3 = tz[iz][3], v4 = tz[iz][4];
+        // this lane's z weights (a select over the rows: a runtime index
+        // into tz would put the table in scratch memory)
+        R v0 = tz[0][0], v1 = tz[0][1], v2 = tz[0][2], v3 = tz[0][3], v4 = tz[0][4];
+#pragma unroll
+        for (int m = 1; m < kOrder; m++)
+            if (iz == m) {
+                v0 = tz[m][0];
+                v1 = tz[m][1];
+                v2 = tz[m][2];
+                v3 = tz[m][3];
+                v4 = tz[m][4];
+            }
         // order of the reference's 35 derivatives (tuv000 ... tuv112)
         sv[0] = b00 * v0; sv[1] = b10 * v0; sv[2] = b01 * v0; sv[3] = b00 * v1;
         sv[4] = b20 * v0; sv[5] = b02 * v0; sv[6] = b00 * v2; sv[7] = b11 * v0; sv[8] = b10 * v1; sv[9] = b01 * v1;
@@ -662,10 +672,18 @@ __global__ __launch_bounds__(256) void k_gather_phi(int n, const R4<R>* __restri
         sv[k] += __shfl_xor(sv[k], 2);
         sv[k] += __shfl_xor(sv[k], 4);
     }
-    if (tv && iz == 0) {
+    // every lane of the group holds the sums: the eight write the atom's 35
+    // values together, eight consecutive words per store
+    if (tv) {
         R* o = phi + (size_t)i * 35;
 #pragma unroll
-        for (int k = 0; k < 35; k++) o[k] = sv[k];
+        for (int r = 0; r < 5; r++) {
+            R v = sv[8 * r];
+#pragma unroll
+            for (int j = 1; j < 8; j++)
+                if (8 * r + j < 35 && iz == j) v = sv[8 * r + j];
+            if (iz + 8 * r < 35) o[iz + 8 * r] = v;
+        }
     }
 }
 
@@ -675,7 +693,7 @@ __global__ __launch_bounds__(256) void k_gather_field(int natoms, const int* __r
                                                        const R4<R>* __restrict__ pos, PmeGeom<R> g,
                                                        const R* __restrict__ grid, R* __restrict__ field) {
     if (skipped(g.skip)) return;
-    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gt = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int t = gt >> 3, iz = gt & 7;
     const bool tv = t < natoms;
     const int tc = tv ? t : natoms - 1;
