@@ -345,9 +345,12 @@ __global__ __launch_bounds__(256) void k_spread_tiled(const R4<R>* __restrict__ 
 // ---------------------------------------------------------------------------
 // candidate atoms examined per pass (the slots of their spline weights in LDS)
 template <class R, int MODE> constexpr int owner_cand() { return MODE == 0 ? 64 : 128; }
+// stencil rows (of 6) along an axis inside a core of T points, for a stencil
+// starting at r relative to the core (r in [-5, T - 1] for a stencil that meets it)
+__device__ __forceinline__ int owner_rows(int r, int T) { return min(kOrder, T - r) - max(0, -r); }
 
 template <class R, int MODE>
-__global__ __launch_bounds__(256) void k_spread_owner(const R4<R>* __restrict__ pos, const R* __restrict__ src,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_spread_owner(const R4<R>* __restrict__ pos, const R* __restrict__ src,
                                                       PmeGeom<R> g, TileGeom tg, const int* __restrict__ atoms,
                                                       const int* __restrict__ tstart, const int* __restrict__ tend,
                                                       R* __restrict__ grid) {
@@ -362,7 +365,9 @@ __global__ __launch_bounds__(256) void k_spread_owner(const R4<R>* __restrict__ 
     R* sth = reinterpret_cast<R*>(core + TV);                       // [kOwnerCand][3][6][ND+1]
     R* smf = sth + kOwnerCand * 3 * kOrder * (ND + 1);              // [kOwnerCand][NC]
     int* srel = reinterpret_cast<int*>(smf + kOwnerCand * NC);      // [kOwnerCand][3] stencil start - core origin
-    __shared__ int sbeg[8], send_[8], soff[9], snv;
+    __shared__ int sbeg[8], send_[8], soff[9], snv, snrows;
+    __shared__ int sfirst[kOwnerCand];
+    static_assert(kOwnerCand % kTile == 0 && kOwnerCand <= 256, "slots: whole wavefronts, at most the block");
     const int t = blockIdx.x;
     const int nt = tg.tiles_per_axis;
     const int c[3] = {(int)compact_bits((unsigned)t >> 2), (int)compact_bits((unsigned)t >> 1),
@@ -442,12 +447,50 @@ __global__ __launch_bounds__(256) void k_spread_owner(const R4<R>* __restrict__ 
         }
         __syncthreads();
         const int nv = snv;
+        // The listed atoms' stencil rows (iy, iz) inside the core, numbered
+        // atom by atom (sfirst[la]: the first row of atom la), so that every
+        // thread's item below is a row with points to add: about 14 of an
+        // atom's 36 rows meet the core (T = 8).  One wavefront counts them.
+        if (threadIdx.x < kTile) {
+            int cnt[kOwnerCand / kTile], inc[kOwnerCand / kTile];
+#pragma unroll
+            for (int h = 0; h < kOwnerCand / kTile; h++) {
+                const int la = h * kTile + (int)threadIdx.x;
+                cnt[h] = la < nv ? owner_rows(srel[la * 3 + 1], T1) * owner_rows(srel[la * 3 + 2], T2) : 0;
+                inc[h] = cnt[h];
+            }
+#pragma unroll
+            for (int off = 1; off < kTile; off <<= 1) {
+#pragma unroll
+                for (int h = 0; h < kOwnerCand / kTile; h++) {
+                    const int v = __shfl_up(inc[h], off);
+                    if ((int)threadIdx.x >= off) inc[h] += v;
+                }
+            }
+            int base = 0;
+#pragma unroll
+            for (int h = 0; h < kOwnerCand / kTile; h++) {
+                const int la = h * kTile + (int)threadIdx.x;
+                if (la < nv) sfirst[la] = base + inc[h] - cnt[h];
+                base += __shfl(inc[h], kTile - 1);
+            }
+            if (threadIdx.x == 0) snrows = base;
+        }
+        __syncthreads();
+        const int nrows = snrows;
         // one item = (atom, iy, iz): the points of the stencil row along x inside the core
-        for (int it = threadIdx.x; it < nv * kOrder * kOrder; it += blockDim.x) {
-            const int la = it / (kOrder * kOrder);
-            const int iy = (it / kOrder) % kOrder, iz = it % kOrder;
-            const int ly = srel[la * 3 + 1] + iy, lz = srel[la * 3 + 2] + iz;
-            if (ly < 0 || ly >= T1 || lz < 0 || lz >= T2) continue;
+        for (int it = threadIdx.x; it < nrows; it += blockDim.x) {
+            // the atom: the last la with sfirst[la] <= it (every atom has a row)
+            int la = 0;
+#pragma unroll
+            for (int step = kOwnerCand / 2; step >= 1; step >>= 1)
+                if (la + step < nv && sfirst[la + step] <= it) la += step;
+            const int ry = srel[la * 3 + 1], rz = srel[la * 3 + 2];
+            const int y0 = max(0, -ry), z0 = max(0, -rz);
+            const int nz = min(kOrder, T2 - rz) - z0;
+            const int loc = it - sfirst[la], dy = loc / nz;
+            const int iy = y0 + dy, iz = z0 + (loc - dy * nz);
+            const int ly = ry + iy, lz = rz + iz;
             const R* tx = sth + ((la * 3 + 0) * kOrder) * (ND + 1);
             const R* ty = sth + ((la * 3 + 1) * kOrder + iy) * (ND + 1);
             const R* tz = sth + ((la * 3 + 2) * kOrder + iz) * (ND + 1);
