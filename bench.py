@@ -129,6 +129,9 @@ def cpu_baseline(full=False):
     from mpidopenmmplugin_amd.waterbox import make_waterbox
     from oracle import oracle
 
+    # the oracle prints a line per solver iteration (a large box takes minutes)
+    os.environ["MPID_ORACLE_PROGRESS"] = "1"
+
     def median_time(name, reps):
         f, pos, box = make_waterbox(name)
         flat = f.flatten()
@@ -138,6 +141,7 @@ def cpu_baseline(full=False):
             r = oracle.evaluate(flat, pos, box)
             times.append(time.perf_counter() - t)
             iters = r["iterations"]
+            print(f"[bench] cpu baseline {name}: {times[-1]:.2f} s", flush=True)
         return float(np.median(times)), times, iters
 
     c0, c0_all, _ = median_time("water216", 3)

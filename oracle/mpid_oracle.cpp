@@ -30,6 +30,13 @@
 #include <stdexcept>
 #include <string>
 #include <vector>
+#include <cstdlib>
+
+// One line on stderr per solver iteration when MPID_ORACLE_PROGRESS is set.
+static bool progress_lines() {
+    static const bool on = std::getenv("MPID_ORACLE_PROGRESS") != nullptr;
+    return on;
+}
 
 namespace {
 
@@ -1139,6 +1146,11 @@ struct Oracle {
             prevE.push_back(er);
             double eps = kDebye * std::sqrt(e2 / n);
             last_eps = eps;
+            // progress of long solves (the benchmark's timing of large boxes)
+            if (progress_lines()) {
+                std::fprintf(stderr, "[oracle] DIIS iteration %d: eps %.3e Debye\n", it, eps);
+                std::fflush(stderr);
+            }
             if (eps < S.target_eps) converged = true;
             if (eps < S.target_eps || it == S.max_iter) { iterations = it; return; }
             if ((int)prevE.size() > maxPrev) {
