@@ -436,6 +436,9 @@ def batch_leg(args):
     return {"workload": "independent water boxes, U(32, 1024) waters each (seeded), 1.0 g/cm^3, "
                         "PME grid ~0.08 nm (2,3,5-smooth), cutoff min(0.8, L/2 - 1e-3), mutual to 1e-6 Debye",
             "systems": args.batch_systems, "atoms": atoms, "pme_grids": grids, "precision": args.precision,
+            "spread_tilings": ("autotuned (mpidopenmmplugin_amd/spread_tilings.json)"
+                               if os.path.exists(os.path.join(ROOT, "mpidopenmmplugin_amd", "spread_tilings.json"))
+                               else "the library's built-in choice"),
             "host_threads": args.batch_threads, "ms_per_batch": 1000 * dt, "systems_per_s": args.batch_systems / dt,
             "atoms_per_s": atoms / dt, "ms_per_batch_sequential": 1000 * dt_seq,
             "speedup_vs_sequential": dt_seq / dt, "energy_sum_kJ_mol": e_sum, "reps": reps,

@@ -77,6 +77,15 @@ typedef struct {
      * at most MPID_MAX_EXTRAPOLATION coefficients.  The array is copied. */
     int n_extrapolation;
     const double* extrapolation_coefficients;
+    /* Tiling of the PME spreading (no counterpart in the reference, whose
+     * spreading is one loop over the atoms): one of the pairs that
+     * mpid_spread_tilings lists for the grid, as an autotuner picks them, or
+     * spread_level = 0 for the built-in choice.  spread_level -1: no tiles
+     * (atomic adds into the grid); m >= 1: each grid axis cut into 2^m tiles,
+     * with spread_mode 1 (each workgroup owns a tile and writes it once) or
+     * 2 (overlapping windows summed into the grid afterwards). */
+    int spread_level;
+    int spread_mode;
 } mpid_options;
 enum { MPID_MAX_EXTRAPOLATION = 16 };
 
@@ -180,6 +189,13 @@ int mpid_get_system_multipole_moments(mpid_context* ctx, const double* positions
                                       const double* masses, double* out13);
 int mpid_get_lab_frame_permanent_dipoles(mpid_context* ctx, const double* positions, const double* box, double* out);
 int mpid_get_total_dipoles(mpid_context* ctx, const double* positions, const double* box, double* out);
+
+/* The spreading tilings available for a PME grid of grid[0..2] points, as
+ * (spread_level, spread_mode) pairs for mpid_options, the built-in choice
+ * first.  No counterpart in the reference: it is for an autotuner, which
+ * times each.  Writes at most max pairs and returns how many there are, or
+ * -MPID_ERR_INVALID (negative) for a grid the PME path does not take. */
+int mpid_spread_tilings(const int* grid, int max, int* levels, int* modes);
 
 /* CalcMPIDForceKernel::getPMEParameters (mpidKernels.h:103). */
 int mpid_get_pme_parameters(const mpid_context* ctx, double* alpha, int* nx, int* ny, int* nz);

@@ -47,6 +47,8 @@ class Options(ctypes.Structure):
         ("flags", ctypes.c_int),
         ("n_extrapolation", ctypes.c_int),
         ("extrapolation_coefficients", _dp),
+        ("spread_level", ctypes.c_int),
+        ("spread_mode", ctypes.c_int),
     ]
 
 
@@ -100,6 +102,7 @@ SIGNATURES = {
     "mpid_get_lab_frame_permanent_dipoles": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
     "mpid_get_total_dipoles": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
     "mpid_get_pme_parameters": (ctypes.c_int, [ctypes.c_void_p, _dp, _ip, _ip, _ip]),
+    "mpid_spread_tilings": (ctypes.c_int, [_ip, ctypes.c_int, _ip, _ip]),
     "mpid_compute_fixed_field": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
     "mpid_compute_induced_field": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp]),
     "mpid_energy_gradients": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, ctypes.c_int, _dp,
@@ -165,7 +168,8 @@ def make_parameters(flat):
     return p, keep
 
 
-def make_options(flat, precision, flags=0):
+def make_options(flat, precision, flags=0, spread_tiling=(0, 0)):
+    """spread_tiling: (spread_level, spread_mode), (0, 0) for the library's choice."""
     o = Options()
     o.nonbonded_method = int(flat["nonbonded_method"])
     o.polarization = int(flat["polarization"])
@@ -185,4 +189,19 @@ def make_options(flat, precision, flags=0):
         o._keep = c  # the library copies the array during the call that takes the options
         o.n_extrapolation = int(c.size)
         o.extrapolation_coefficients = c.ctypes.data_as(_dp)
+    o.spread_level = int(spread_tiling[0])
+    o.spread_mode = int(spread_tiling[1])
     return o
+
+
+def spread_tilings(grid):
+    """The (spread_level, spread_mode) pairs the library offers for a PME grid,
+    its built-in choice first (mpid_spread_tilings; no GPU needed)."""
+    lib = load()
+    g = (ctypes.c_int * 3)(*[int(x) for x in grid])
+    lv = (ctypes.c_int * 32)()
+    md = (ctypes.c_int * 32)()
+    count = lib.mpid_spread_tilings(g, 32, lv, md)
+    if count < 0:
+        raise ValueError(f"no spreading tilings for PME grid {tuple(grid)} (error {count})")
+    return [(int(lv[i]), int(md[i])) for i in range(min(count, 32))]

@@ -175,6 +175,48 @@ template <class R> struct Prec {
 
 }  // namespace
 
+// ---- tilings of the PME spreading ---------------------------------------------
+// (level, mode) as mpid_options takes them: level -1 without tiles; level m
+// >= 1 cuts each axis of the grid into 2^m tiles of T_k = n_k / 2^m >= 6
+// points, spread by owned cores (T_k >= 8 and the core's LDS within 64 KiB)
+// or by windows of T_k + 8 points folded into the grid (at most 3 windows
+// per point and axis, and the window's LDS within 64 KiB).
+enum { kSpreadUntiled = 0, kSpreadOwner = 1, kSpreadWindows = 2 };
+struct SpreadTiling {
+    int level, mode;
+};
+static bool tile_geometry(const int grid[3], int m, TileGeom& tg) {
+    const int a = 1 << m;
+    tg = TileGeom{};
+    tg.m = m;
+    tg.tiles_per_axis = a;
+    tg.ntiles = a * a * a;
+    for (int k = 0; k < 3; k++) {
+        if (grid[k] % a != 0 || grid[k] / a < 6) return false;
+        tg.T[k] = grid[k] / a;
+        tg.W[k] = std::min(tg.T[k] + 8, grid[k]);
+    }
+    return true;
+}
+static bool windows_fit(const TileGeom& tg) {
+    for (int k = 0; k < 3; k++)
+        if ((tg.W[k] - 1) / tg.T[k] + 1 > 3) return false;  // the fold sums at most 3 windows per axis
+    return spread_tiled_lds_bytes(tg, 1) <= 65536;
+}
+// Every available tiling; first the built-in choice: the largest level at
+// which one of the two tiled forms fits, by owned cores where they fit.
+static std::vector<SpreadTiling> spread_tilings(const int grid[3]) {
+    std::vector<SpreadTiling> out;
+    for (int m = 10; m >= 1; m--) {
+        TileGeom tg;
+        if (!tile_geometry(grid, m, tg)) continue;
+        if (spread_by_owner(tg)) out.push_back(SpreadTiling{m, kSpreadOwner});
+        if (windows_fit(tg)) out.push_back(SpreadTiling{m, kSpreadWindows});
+    }
+    out.push_back(SpreadTiling{-1, kSpreadUntiled});
+    return out;
+}
+
 struct mpid_context {
     int n = 0;
     mpid_options opt{};
@@ -658,33 +700,29 @@ struct mpid_context {
         init_prec(pd);  // FP64 path (also the refinement residual of the mixed path)
     }
 
-    // Largest Morton level m with n_k = 2^m T_k, T_k >= 6, and a window that fits in LDS.
+    // The spreading's tiling: opt.spread_level / spread_mode when given (one
+    // of spread_tilings(opt.grid)), else the first of that list.
     void choose_tiles() {
         tiled_spread = false;
         owner_spread = false;
         if (!pme()) return;
-        for (int m = 10; m >= 1; m--) {
-            const int a = 1 << m;
-            bool ok = true;
-            for (int k = 0; k < 3; k++)
-                if (opt.grid[k] % a != 0 || opt.grid[k] / a < 6) ok = false;
-            if (!ok) continue;
-            TileGeom tg{};
-            tg.m = m;
-            tg.tiles_per_axis = a;
-            tg.ntiles = a * a * a;
-            for (int k = 0; k < 3; k++) {
-                tg.T[k] = opt.grid[k] / a;
-                tg.W[k] = std::min(tg.T[k] + 8, opt.grid[k]);
-                if ((tg.W[k] - 1) / tg.T[k] + 1 > 3) ok = false;  // the fold sums at most 3 windows per dimension
-            }
-            if (!ok) continue;
-            if (spread_tiled_lds_bytes(tg, 1) > 65536) continue;
-            tgeom = tg;
-            tiled_spread = true;
-            break;
+        const std::vector<SpreadTiling> avail = spread_tilings(opt.grid);
+        SpreadTiling pick = avail.front();
+        if (opt.spread_level != 0) {
+            bool found = false;
+            for (const SpreadTiling& t : avail)
+                if (t.level == opt.spread_level && t.mode == opt.spread_mode) found = true;
+            if (!found)
+                throw MpidError(MPID_ERR_INVALID, "spread tiling (level " + std::to_string(opt.spread_level) +
+                                                      ", mode " + std::to_string(opt.spread_mode) +
+                                                      ") is not available for this PME grid");
+            pick = SpreadTiling{opt.spread_level, opt.spread_mode};
         }
-        owner_spread = tiled_spread && spread_by_owner(tgeom);
+        if (pick.level > 0) {
+            tile_geometry(opt.grid, pick.level, tgeom);
+            tiled_spread = true;
+            owner_spread = pick.mode == kSpreadOwner;
+        }
         if (tiled_spread) {
             tstart.resize(tgeom.ntiles);
             tend.resize(tgeom.ntiles);
@@ -2053,6 +2091,20 @@ int mpid_get_total_dipoles(mpid_context* ctx, const double* positions, const dou
         ctx->lab_dipoles(out);
         for (size_t k = 0; k < ind.size(); k++) out[k] += ind[k];
     });
+}
+
+int mpid_spread_tilings(const int* grid, int max, int* levels, int* modes) {
+    if (!grid || max < 0 || (max > 0 && (!levels || !modes))) return -MPID_ERR_INVALID;
+    for (int k = 0; k < 3; k++) {
+        int rr[16], c;
+        if (grid[k] < kOrder || grid[k] > 1024 || !factor_radices(grid[k], rr, c)) return -MPID_ERR_INVALID;
+    }
+    const std::vector<SpreadTiling> t = spread_tilings(grid);
+    for (int i = 0; i < (int)t.size() && i < max; i++) {
+        levels[i] = t[i].level;
+        modes[i] = t[i].mode;
+    }
+    return (int)t.size();
 }
 
 int mpid_get_pme_parameters(const mpid_context* ctx, double* alpha, int* nx, int* ny, int* nz) {

@@ -14,7 +14,7 @@ import ctypes
 
 import numpy as np
 
-from . import _capi
+from . import _capi, autotune
 from .mpidforce import MPIDForce, MPIDForceError, automatic_pme_parameters
 
 
@@ -43,15 +43,19 @@ class HipCalcMPIDForceKernel:
 
     Name = "CalcMPIDForce"
 
-    def __init__(self, precision="mixed", conservative_torques=False, single_stream=False):
+    def __init__(self, precision="mixed", conservative_torques=False, single_stream=False, spread_tiling=None):
         """``conservative_torques`` restores the half of the induced-induced torque
         about the pair axis that the reference omits
         (MPIDReferenceForce.cpp:4854-4855); the default reproduces the reference.
         ``single_stream`` issues all work on one HIP stream (for many small
-        contexts evaluated at once)."""
+        contexts evaluated at once).  ``spread_tiling``: None takes the PME
+        spreading's tiling from the autotuner's table (the library's own
+        choice for grids not in it), "library" the library's choice, or a
+        (spread_level, spread_mode) pair of ``_capi.spread_tilings(grid)``."""
         if precision not in _PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(_PRECISIONS)}")
         self.precision = precision
+        self.spread_tiling = spread_tiling
         self.flags = _capi.MPID_FLAG_CONSERVATIVE_TORQUES if conservative_torques else 0
         if single_stream:
             self.flags |= _capi.MPID_FLAG_SINGLE_STREAM
@@ -88,7 +92,13 @@ class HipCalcMPIDForceKernel:
             a, nx, ny, nz = automatic_pme_parameters(flat["cutoff"], force.getEwaldErrorTolerance(), default_box)
             flat["alpha"], flat["grid"] = a, (nx, ny, nz)
         params, keep = _capi.make_parameters(flat)
-        opts = _capi.make_options(flat, _PRECISIONS[self.precision], self.flags)
+        tiling = (0, 0)
+        if flat["nonbonded_method"] == MPIDForce.PME:
+            if self.spread_tiling is None:
+                tiling = autotune.lookup(flat["grid"]) or (0, 0)
+            elif self.spread_tiling != "library":
+                tiling = tuple(int(x) for x in self.spread_tiling)
+        opts = _capi.make_options(flat, _PRECISIONS[self.precision], self.flags, tiling)
         ctx = ctypes.c_void_p()
         rc = self._lib.mpid_create(ctypes.byref(params), ctypes.byref(opts), ctypes.byref(ctx))
         if rc != _capi.MPID_OK:
