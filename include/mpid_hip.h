@@ -197,6 +197,11 @@ int mpid_get_total_dipoles(mpid_context* ctx, const double* positions, const dou
  * -MPID_ERR_INVALID (negative) for a grid the PME path does not take. */
 int mpid_spread_tilings(const int* grid, int max, int* levels, int* modes);
 
+/* MPIDForce::getPmeBSplineOrder (MPIDForce.h:196): the B-spline order this
+ * library was built for, 6 (as the reference) or 5 (libmpid_hip_order5.so,
+ * built with -DMPID_PME_ORDER=5). */
+int mpid_get_pme_bspline_order(void);
+
 /* CalcMPIDForceKernel::getPMEParameters (mpidKernels.h:103). */
 int mpid_get_pme_parameters(const mpid_context* ctx, double* alpha, int* nx, int* ny, int* nz);
 

@@ -11,7 +11,10 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libmpid_hip.so")
+# MPID_PME_ORDER=5 in the environment selects the library built for B-spline
+# order 5 (a compile-time option; the reference and the default build use 6)
+PME_ORDER = 5 if os.environ.get("MPID_PME_ORDER", "6") == "5" else 6
+LIB_PATH = os.path.join(_HERE, "libmpid_hip_order5.so" if PME_ORDER == 5 else "libmpid_hip.so")
 
 MPID_OK = 0
 MPID_ERR_INVALID = 1
@@ -103,6 +106,7 @@ SIGNATURES = {
     "mpid_get_total_dipoles": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
     "mpid_get_pme_parameters": (ctypes.c_int, [ctypes.c_void_p, _dp, _ip, _ip, _ip]),
     "mpid_spread_tilings": (ctypes.c_int, [_ip, ctypes.c_int, _ip, _ip]),
+    "mpid_get_pme_bspline_order": (ctypes.c_int, []),
     "mpid_compute_fixed_field": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
     "mpid_compute_induced_field": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp]),
     "mpid_energy_gradients": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, ctypes.c_int, _dp,

@@ -11,7 +11,16 @@ namespace mpid {
 constexpr double kElectric = 138.935455846;
 constexpr double kDebye = 48.033324;
 constexpr double kSqrtPi = 1.77245385091;
-constexpr int kOrder = 6;            // B-spline order, hard-wired in the reference
+// B-spline order of the PME spreading and gathering: 6, as the reference
+// hard-wires (MPIDReferenceForce.cpp:2539; MPIDForce::getPmeBSplineOrder), or 5
+// in a build with -DMPID_PME_ORDER=5 (libmpid_hip_order5.so).  The tile
+// geometry of the spreading (window origins 6 points before a tile, windows
+// of T + 8 points, owned cores of T >= 8) covers both orders.
+#ifndef MPID_PME_ORDER
+#define MPID_PME_ORDER 6
+#endif
+constexpr int kOrder = MPID_PME_ORDER;
+static_assert(kOrder == 5 || kOrder == 6, "B-spline order 5 or 6");
 constexpr int kTile = 64;            // atoms per block = one wavefront
 constexpr double kFixed = 4294967296.0;  // 2^32: fixed-point scale of every accumulator
 

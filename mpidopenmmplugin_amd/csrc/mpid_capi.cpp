@@ -2093,6 +2093,8 @@ int mpid_get_total_dipoles(mpid_context* ctx, const double* positions, const dou
     });
 }
 
+int mpid_get_pme_bspline_order(void) { return kOrder; }
+
 int mpid_spread_tilings(const int* grid, int max, int* levels, int* modes) {
     if (!grid || max < 0 || (max > 0 && (!levels || !modes))) return -MPID_ERR_INVALID;
     for (int k = 0; k < 3; k++) {

@@ -47,7 +47,13 @@ const double kElectric = 138.935455846;
 const double kDebye = 48.033324;
 // MPIDReferenceForce.cpp:2545 keeps a truncated sqrt(pi); kept for parity.
 const double kSqrtPi = 1.77245385091;
-const int kOrder = 6;  // MPIDReferenceForce.cpp:2543, B-spline order hard-wired to 6
+// MPIDReferenceForce.cpp:2543, B-spline order hard-wired to 6; 5 in a build
+// with -DMPID_PME_ORDER=5 (libmpid_oracle_order5.so), the same formulas
+#ifndef MPID_PME_ORDER
+#define MPID_PME_ORDER 6
+#endif
+const int kOrder = MPID_PME_ORDER;
+static_assert(kOrder == 5 || kOrder == 6, "B-spline order 5 or 6");
 
 struct V3 {
     double x[3];
@@ -1688,7 +1694,7 @@ struct Oracle {
 
 extern "C" {
 
-void oracle_bspline(double w, double theta[6][5]) { bspline(w, theta); }
+void oracle_bspline(double w, double* theta) { bspline(w, reinterpret_cast<double(*)[5]>(theta)); }
 
 void oracle_bspline_moduli(int n, double* moduli) { bspline_moduli(n, moduli); }
 

@@ -83,7 +83,9 @@ typedef struct {
 int oracle_evaluate(const oracle_system* sys, const double* pos, oracle_out* out);
 
 /* Building blocks, exposed for unit tests. */
-void oracle_bspline(double w, double theta[6][5]);
+/* theta[order][5]: the weights of the order (6, or 5 in the order-5 build)
+ * grid points and their first four derivatives */
+void oracle_bspline(double w, double* theta);
 void oracle_bspline_moduli(int n, double* moduli);
 int oracle_fft3d(double* data_interleaved, int nx, int ny, int nz, int sign);
 /* Induced field T*mu (real + reciprocal + self) for given lab dipoles, no

@@ -16,7 +16,10 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "libmpid_oracle.so")
+# MPID_PME_ORDER=5 in the environment selects the restatement built for
+# B-spline order 5 (the reference's order is 6)
+PME_ORDER = 5 if os.environ.get("MPID_PME_ORDER", "6") == "5" else 6
+_LIB_PATH = os.path.join(_HERE, "libmpid_oracle_order5.so" if PME_ORDER == 5 else "libmpid_oracle.so")
 _lib = None
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -176,7 +179,7 @@ def induced_field(flat, positions, box, mu):
 
 
 def bspline(w):
-    th = np.zeros((6, 5), dtype=np.float64)
+    th = np.zeros((PME_ORDER, 5), dtype=np.float64)
     lib().oracle_bspline(float(w), _ptr(th, "d"))
     return th
 
