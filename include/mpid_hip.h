@@ -54,8 +54,15 @@ enum { MPID_PRECISION_MIXED = 0, MPID_PRECISION_DOUBLE = 1 };
  * SINGLE_STREAM: issue all of the context's work on one HIP stream instead of
  * three (real space, reciprocal space, the mixed path's FP64 check).  For many
  * small contexts driven at once, where cross-stream waits would hold up the
- * hardware queues the contexts share; results are the same. */
-enum { MPID_FLAG_CONSERVATIVE_TORQUES = 1, MPID_FLAG_SINGLE_STREAM = 2 };
+ * hardware queues the contexts share; results are the same.
+ * DIIS: solve mutual polarization with the reference's DIIS iteration
+ * (convergeInduceDipolesByDIIS, MPIDReferenceForce.cpp:1182-1291: history of
+ * 20, coefficients from the pseudo-inverse of the bordered error matrix,
+ * the same Debye-RMS stopping test) instead of preconditioned conjugate
+ * gradients, so that the iterates and the iteration count follow the
+ * reference's; one host synchronisation per iteration.  Double precision
+ * only (MPID_ERR_UNSUPPORTED otherwise). */
+enum { MPID_FLAG_CONSERVATIVE_TORQUES = 1, MPID_FLAG_SINGLE_STREAM = 2, MPID_FLAG_DIIS = 4 };
 
 /* Global settings of the force (MPIDForce.h:118-470 setters). */
 typedef struct {

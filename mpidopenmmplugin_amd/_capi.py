@@ -57,6 +57,7 @@ class Options(ctypes.Structure):
 
 MPID_FLAG_CONSERVATIVE_TORQUES = 1
 MPID_FLAG_SINGLE_STREAM = 2
+MPID_FLAG_DIIS = 4
 
 
 class Parameters(ctypes.Structure):

@@ -584,8 +584,10 @@ struct mpid_context {
         if (opt.precision != MPID_PRECISION_MIXED && opt.precision != MPID_PRECISION_DOUBLE)
             throw MpidError(MPID_ERR_INVALID, "invalid precision");
         if (opt.max_iterations < 0) throw MpidError(MPID_ERR_INVALID, "invalid iteration limit");
-        if (opt.flags & ~(MPID_FLAG_CONSERVATIVE_TORQUES | MPID_FLAG_SINGLE_STREAM))
+        if (opt.flags & ~(MPID_FLAG_CONSERVATIVE_TORQUES | MPID_FLAG_SINGLE_STREAM | MPID_FLAG_DIIS))
             throw MpidError(MPID_ERR_INVALID, "unknown option flags");
+        if ((opt.flags & MPID_FLAG_DIIS) && opt.precision != MPID_PRECISION_DOUBLE)
+            throw MpidError(MPID_ERR_UNSUPPORTED, "the DIIS solver runs in double precision only");
         for (int k = 0; k < 3 && pme(); k++) {
             int rr[16], c;
             FftAxis ax;
@@ -1382,6 +1384,126 @@ struct mpid_context {
         mu_set = true;
     }
 
+    // DIIS, the reference's solver for mutual polarization
+    // (convergeInduceDipolesByDIIS, MPIDReferenceForce.cpp:1182-1291), on the
+    // same operator, for parity of the iterates with the reference rather
+    // than for speed (MPID_FLAG_DIIS; one host synchronisation per
+    // iteration).  In the variables x (mu = S x) the fixed point
+    // mu = alpha (E0 + T mu) reads x = b + (I - A) x: the new dipoles are
+    // x + r and their error is S r (r = b - A x), the reference's nd - mu,
+    // whose RMS in Debye is the stopping test.  The next x combines the last
+    // (at most 20) new dipoles with the coefficients of the least-squares
+    // combination of their errors (diis_coefficients).  Stops at iteration
+    // it with the current x when the error is below the target or it reaches
+    // the iteration limit, as the reference; r is then that x's residual.
+    static constexpr int kDiisHistory = 20;
+    DBuf<double> diis_d, diis_e, diis_dots;
+    template <class R> double diis(Prec<R>& P, int& iters) {
+        const int n3 = 3 * n;
+        const int nslot = kDiisHistory + 1;  // a new entry is added before the oldest leaves
+        diis_d.resize((size_t)nslot * n3);
+        diis_e.resize((size_t)nslot * n3);
+        diis_dots.resize(nslot);
+        std::vector<int> hist;  // slots, oldest first
+        std::vector<double> B((size_t)nslot * nslot, 0.0), dots(nslot);
+        for (int it = 0;; it++) {
+            matvec(P, x.p, Ap.p);
+            int slot = 0;
+            while (std::find(hist.begin(), hist.end(), slot) != hist.end()) slot++;
+            hist.push_back(slot);
+            double* d = diis_d.p + (size_t)slot * n3;
+            double* e = diis_e.p + (size_t)slot * n3;
+            timed(P_SOLVER, [&] {
+                launch_sub(stream, n3, b.p, Ap.p, r.p);
+                HIP_CHECK(hipMemcpyAsync(d, x.p, n3 * sizeof(double), hipMemcpyDeviceToDevice, stream));
+                launch_axpy(stream, n3, 1.0, r.p, d);
+                launch_apply_S<double>(stream, n, sqrtpol.p, r.p, e, nullptr);
+                for (int s2 : hist)
+                    launch_dot(stream, n3, e, diis_e.p + (size_t)s2 * n3, partial.p, diis_dots.p + s2);
+            });
+            check_launch();
+            HIP_CHECK(hipMemcpyAsync(dots.data(), diis_dots.p, nslot * sizeof(double), hipMemcpyDeviceToHost, stream));
+            HIP_CHECK(hipStreamSynchronize(stream));
+            check_lists_pending();
+            const double eps = debye_eps(dots[slot]);
+            if (eps < opt.target_epsilon || it == opt.max_iterations) {
+                iters = it;
+                return eps;
+            }
+            for (int s2 : hist) B[(size_t)slot * nslot + s2] = B[(size_t)s2 * nslot + slot] = dots[s2];
+            if ((int)hist.size() > kDiisHistory) hist.erase(hist.begin());
+            const std::vector<double> c = diis_coefficients(B, nslot, hist);
+            timed(P_SOLVER, [&] {
+                launch_fill<double>(stream, x.p, n3, 0.0);
+                for (size_t j = 0; j < hist.size(); j++)
+                    launch_axpy(stream, n3, c[j], diis_d.p + (size_t)hist[j] * n3, x.p);
+            });
+            check_launch();
+        }
+    }
+    // Coefficients c (sum 1) of the entries hist of the error dot products B
+    // (nslot x nslot, by slot) that minimise |sum c_j e_j|: the first column,
+    // negated, of the pseudo-inverse of the bordered matrix
+    // [[0, -1...], [-1, B]], as the reference takes it from JAMA's SVD
+    // (MPIDReferenceForce.cpp:1254-1291); here from a cyclic Jacobi
+    // eigen-decomposition of the symmetric matrix, with the SVD's rank
+    // tolerance.
+    static std::vector<double> diis_coefficients(const std::vector<double>& B, int nslot, const std::vector<int>& hist) {
+        const int m = (int)hist.size();
+        std::vector<double> c(m, 0.0);
+        if (m == 1) {
+            c[0] = 1.0;
+            return c;
+        }
+        const int R = m + 1;
+        std::vector<double> a((size_t)R * R, 0.0), V((size_t)R * R, 0.0);
+        for (int i = 0; i < m; i++) a[(size_t)(i + 1) * R] = a[i + 1] = -1.0;
+        for (int i = 0; i < m; i++)
+            for (int j = 0; j < m; j++) a[(size_t)(i + 1) * R + j + 1] = B[(size_t)hist[i] * nslot + hist[j]];
+        for (int i = 0; i < R; i++) V[(size_t)i * R + i] = 1.0;
+        for (int sweep = 0; sweep < 100; sweep++) {
+            double off = 0.0;
+            for (int p = 0; p < R; p++)
+                for (int q = p + 1; q < R; q++) off += a[(size_t)p * R + q] * a[(size_t)p * R + q];
+            if (off < 1e-300) break;
+            for (int p = 0; p < R; p++)
+                for (int q = p + 1; q < R; q++) {
+                    const double apq = a[(size_t)p * R + q];
+                    if (std::fabs(apq) < 1e-300) continue;
+                    const double th = (a[(size_t)q * R + q] - a[(size_t)p * R + p]) / (2.0 * apq);
+                    const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+                    const double cs = 1.0 / std::sqrt(t * t + 1.0), sn = t * cs;
+                    for (int k = 0; k < R; k++) {
+                        const double akp = a[(size_t)k * R + p], akq = a[(size_t)k * R + q];
+                        a[(size_t)k * R + p] = cs * akp - sn * akq;
+                        a[(size_t)k * R + q] = sn * akp + cs * akq;
+                    }
+                    for (int k = 0; k < R; k++) {
+                        const double apk = a[(size_t)p * R + k], aqk = a[(size_t)q * R + k];
+                        a[(size_t)p * R + k] = cs * apk - sn * aqk;
+                        a[(size_t)q * R + k] = sn * apk + cs * aqk;
+                    }
+                    for (int k = 0; k < R; k++) {
+                        const double vkp = V[(size_t)k * R + p], vkq = V[(size_t)k * R + q];
+                        V[(size_t)k * R + p] = cs * vkp - sn * vkq;
+                        V[(size_t)k * R + q] = sn * vkp + cs * vkq;
+                    }
+                }
+        }
+        double smax = 0.0;
+        for (int i = 0; i < R; i++) smax = std::max(smax, std::fabs(a[(size_t)i * R + i]));
+        const double tol = R * smax * 2.220446049250313e-16;
+        for (int i = 1; i < R; i++) {
+            double d = 0.0;
+            for (int j = 0; j < R; j++) {
+                const double w = a[(size_t)j * R + j];
+                if (std::fabs(w) > tol) d -= V[j] * V[(size_t)i * R + j] / w;
+            }
+            c[i - 1] = d;
+        }
+        return c;
+    }
+
     // residual with the FP64 operator: rt = b - A x; returns its RMS-Debye norm
     double true_residual() {
         const int n3 = 3 * n;
@@ -1414,7 +1536,8 @@ struct mpid_context {
             return;
         }
         int iters = 0;
-        double eps = cg(P, x.p, b.p, false, opt.target_epsilon, iters, opt.max_iterations);
+        double eps = (opt.flags & MPID_FLAG_DIIS) ? diis(P, iters)
+                                                 : cg(P, x.p, b.p, false, opt.target_epsilon, iters, opt.max_iterations);
         // Variational correction of the energy: the energy expression of the
         // reference is linear in the induced dipoles, U(mu) = E(mu) - eps/2 y.r
         // (r = b - A y) is stationary at the solution, so it carries no

@@ -43,7 +43,8 @@ class HipCalcMPIDForceKernel:
 
     Name = "CalcMPIDForce"
 
-    def __init__(self, precision="mixed", conservative_torques=False, single_stream=False, spread_tiling=None):
+    def __init__(self, precision="mixed", conservative_torques=False, single_stream=False, spread_tiling=None,
+                 solver="cg"):
         """``conservative_torques`` restores the half of the induced-induced torque
         about the pair axis that the reference omits
         (MPIDReferenceForce.cpp:4854-4855); the default reproduces the reference.
@@ -51,7 +52,10 @@ class HipCalcMPIDForceKernel:
         contexts evaluated at once).  ``spread_tiling``: None takes the PME
         spreading's tiling from the autotuner's table (the library's own
         choice for grids not in it), "library" the library's choice, or a
-        (spread_level, spread_mode) pair of ``_capi.spread_tilings(grid)``."""
+        (spread_level, spread_mode) pair of ``_capi.spread_tilings(grid)``.
+        ``solver``: "cg" (preconditioned conjugate gradients) or "diis" (the
+        reference's DIIS iteration, for parity of the iterates; double
+        precision only)."""
         if precision not in _PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(_PRECISIONS)}")
         self.precision = precision
@@ -59,6 +63,10 @@ class HipCalcMPIDForceKernel:
         self.flags = _capi.MPID_FLAG_CONSERVATIVE_TORQUES if conservative_torques else 0
         if single_stream:
             self.flags |= _capi.MPID_FLAG_SINGLE_STREAM
+        if solver not in ("cg", "diis"):
+            raise ValueError("solver must be 'cg' or 'diis'")
+        if solver == "diis":
+            self.flags |= _capi.MPID_FLAG_DIIS
         self._lib = _capi.load()
         self._ctx = ctypes.c_void_p()
         self._n = 0
