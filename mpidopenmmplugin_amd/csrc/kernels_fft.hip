@@ -417,46 +417,49 @@ constexpr int kPitch = (N / B) * (B + 1) + 8;  // 152
 __device__ __forceinline__ int slot(int y) { return (y >> 3) * (B + 1) + (y & 7); }
 
 typedef Cx<float> C;
-__device__ __forceinline__ C tw_s(const C* tw, int j, float s) { return C{tw[j].x, -s * tw[j].y}; }
+// the register steps below serve both precisions (the z passes also run in FP64)
+template <class R> __device__ __forceinline__ Cx<R> tw_s(const Cx<R>* tw, int j, R s) {
+    return Cx<R>{tw[j].x, -s * tw[j].y};
+}
 
 // exp(s 2 pi i j / 16), j < 16
-__device__ __forceinline__ C w16(int j, float s) {
-    constexpr float c[16] = {1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508979f,
-                             0.0f, -0.38268343236508979f, -0.70710678118654752f, -0.92387953251128674f,
-                             -1.0f, -0.92387953251128674f, -0.70710678118654752f, -0.38268343236508979f,
-                             0.0f, 0.38268343236508979f, 0.70710678118654752f, 0.92387953251128674f};
-    constexpr float sn[16] = {0.0f, 0.38268343236508979f, 0.70710678118654752f, 0.92387953251128674f,
-                              1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508979f,
-                              0.0f, -0.38268343236508979f, -0.70710678118654752f, -0.92387953251128674f,
-                              -1.0f, -0.92387953251128674f, -0.70710678118654752f, -0.38268343236508979f};
-    return C{c[j], s * sn[j]};
+template <class R> __device__ __forceinline__ Cx<R> w16(int j, R s) {
+    constexpr double c[16] = {1.0, 0.92387953251128674, 0.70710678118654752, 0.38268343236508979,
+                              0.0, -0.38268343236508979, -0.70710678118654752, -0.92387953251128674,
+                              -1.0, -0.92387953251128674, -0.70710678118654752, -0.38268343236508979,
+                              0.0, 0.38268343236508979, 0.70710678118654752, 0.92387953251128674};
+    constexpr double sn[16] = {0.0, 0.38268343236508979, 0.70710678118654752, 0.92387953251128674,
+                               1.0, 0.92387953251128674, 0.70710678118654752, 0.38268343236508979,
+                               0.0, -0.38268343236508979, -0.70710678118654752, -0.92387953251128674,
+                               -1.0, -0.92387953251128674, -0.70710678118654752, -0.38268343236508979};
+    return Cx<R>{R(c[j]), s * R(sn[j])};
 }
 // natural-order DFTs: X[k] = sum_n v[n] exp(s 2 pi i n k / M)
-__device__ __forceinline__ void dft8(C* v, float s) {
-    C e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
-    dft_small<float, 4>(e, s);
-    dft_small<float, 4>(o, s);
+template <class R> __device__ __forceinline__ void dft8(Cx<R>* v, R s) {
+    Cx<R> e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+    dft_small<R, 4>(e, s);
+    dft_small<R, 4>(o, s);
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        const C t = k == 0 ? o[0] : cmul(o[k], w16(2 * k, s));
+        const Cx<R> t = k == 0 ? o[0] : cmul(o[k], w16<R>(2 * k, s));
         v[k] = cadd(e[k], t);
         v[k + 4] = csub(e[k], t);
     }
 }
-__device__ __forceinline__ void dft16(C* v, float s) {
-    C d[4][4];
+template <class R> __device__ __forceinline__ void dft16(Cx<R>* v, R s) {
+    Cx<R> d[4][4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
 #pragma unroll
         for (int m = 0; m < 4; m++) d[q][m] = v[4 * m + q];
-        dft_small<float, 4>(d[q], s);
+        dft_small<R, 4>(d[q], s);
     }
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        C y[4];
+        Cx<R> y[4];
 #pragma unroll
-        for (int q = 0; q < 4; q++) y[q] = (q * k == 0) ? d[q][k] : cmul(d[q][k], w16(q * k, s));
-        dft_small<float, 4>(y, s);
+        for (int q = 0; q < 4; q++) y[q] = (q * k == 0) ? d[q][k] : cmul(d[q][k], w16<R>(q * k, s));
+        dft_small<R, 4>(y, s);
 #pragma unroll
         for (int l = 0; l < 4; l++) v[k + 4 * l] = y[l];
     }
@@ -578,48 +581,53 @@ __global__ __launch_bounds__(plane::kThreads) void k_fft_plane128_convolve(R2<fl
 // banks).
 // ---------------------------------------------------------------------------
 namespace zline {
-constexpr int N = 128, A = 16, B = 8, T = 64;
-constexpr int kThreads = 512;
+constexpr int N = 128, A = 16, B = 8;
+// lines per workgroup: 64 in FP32, 32 in FP64 (74 KB of LDS either way)
+template <class R> constexpr int lines() { return sizeof(R) == 4 ? 64 : 32; }
+template <class R> constexpr int threads() { return lines<R>() * B; }
 constexpr int kPitch = (N / B) * (B + 1) + 1;  // 145
 __device__ __forceinline__ int at(int t, int z) { return t * kPitch + (z >> 3) * (B + 1) + (z & 7); }
 }  // namespace zline
 
-__global__ __launch_bounds__(zline::kThreads) void k_fft_z128_r2c(const float* __restrict__ Q, R2<float>* __restrict__ S,
-                                                                  const R2<float>* __restrict__ tw, int nlines,
-                                                                  const double* skip) {
+template <class R>
+__global__ __launch_bounds__(zline::threads<R>()) void k_fft_z128_r2c(const R* __restrict__ Q, R2<R>* __restrict__ S,
+                                                                     const R2<R>* __restrict__ tw, int nlines,
+                                                                     const double* skip) {
     if (skipped(skip)) return;
     using namespace zline;
-    typedef Cx<float> C;
+    typedef Cx<R> C;
+    constexpr int T = lines<R>(), kThreads = threads<R>();
+    constexpr int kV = 16 / (int)sizeof(R);  // reals per 16-byte load
+    typedef R vec __attribute__((ext_vector_type(kV)));
     __shared__ C L[T * kPitch];
     __shared__ C W[N];
     const int l0 = blockIdx.x * T;
     if (threadIdx.x < N) W[threadIdx.x] = C{tw[threadIdx.x].x, tw[threadIdx.x].y};
     {
-        // T lines of N reals: T * N / 4 float4 loads, 4 per thread
-        const float4* src = reinterpret_cast<const float4*>(Q + (size_t)l0 * N);
-        float4 v[T * N / 4 / kThreads];
+        // T lines of N reals in 16-byte loads
+        constexpr int kLoads = T * N / kV / kThreads;
+        const vec* src = reinterpret_cast<const vec*>(Q + (size_t)l0 * N);
+        vec v[kLoads];
 #pragma unroll
-        for (int u = 0; u < T * N / 4 / kThreads; u++) v[u] = src[threadIdx.x + u * kThreads];
+        for (int u = 0; u < kLoads; u++) v[u] = src[threadIdx.x + u * kThreads];
 #pragma unroll
-        for (int u = 0; u < T * N / 4 / kThreads; u++) {
-            const int f = threadIdx.x + u * kThreads, t = f >> 5, z = (f & 31) * 4;
-            L[at(t, z)] = C{v[u].x, 0.f};
-            L[at(t, z + 1)] = C{v[u].y, 0.f};
-            L[at(t, z + 2)] = C{v[u].z, 0.f};
-            L[at(t, z + 3)] = C{v[u].w, 0.f};
+        for (int u = 0; u < kLoads; u++) {
+            const int f = threadIdx.x + u * kThreads, t = f / (N / kV), z = (f % (N / kV)) * kV;
+#pragma unroll
+            for (int c = 0; c < kV; c++) L[at(t, z + c)] = C{v[u][c], R(0)};
         }
     }
     __syncthreads();
     {
-        const int t = threadIdx.x & (T - 1), b = threadIdx.x >> 6;
+        const int t = threadIdx.x % T, b = threadIdx.x / T;
         C v[A];
 #pragma unroll
         for (int a = 0; a < A; a++) v[a] = L[at(t, a * B + b)];
-        plane::dft16(v, -1.f);
+        plane::dft16<R>(v, R(-1));
 #pragma unroll
         for (int k1 = 0; k1 < A; k1++) {
             C r = v[k1];
-            if (b * k1 != 0) r = cmul(r, plane::tw_s(W, b * k1, -1.f));
+            if (b * k1 != 0) r = cmul(r, plane::tw_s<R>(W, b * k1, R(-1)));
             L[at(t, k1 * B + b)] = r;
         }
     }
@@ -627,25 +635,29 @@ __global__ __launch_bounds__(zline::kThreads) void k_fft_z128_r2c(const float* _
 #pragma unroll
     for (int h = 0; h < 2; h++) {
         const int task = threadIdx.x + h * kThreads;
-        const int t = task & (T - 1), k1 = task >> 6;
+        const int t = task % T, k1 = task / T;
         C v[B];
 #pragma unroll
         for (int j = 0; j < B; j++) v[j] = L[at(t, k1 * B + j)];
-        plane::dft8(v, -1.f);
+        plane::dft8<R>(v, R(-1));
 #pragma unroll
         for (int k2 = 0; k2 < B; k2++) {
             const int kz = k1 + A * k2;
-            if (kz <= N / 2) S[(size_t)kz * nlines + l0 + t] = R2<float>{v[k2].x, v[k2].y};
+            if (kz <= N / 2) S[(size_t)kz * nlines + l0 + t] = R2<R>{v[k2].x, v[k2].y};
         }
     }
 }
 
-__global__ __launch_bounds__(zline::kThreads) void k_fft_z128_c2r(float* __restrict__ Q, const R2<float>* __restrict__ S,
-                                                                  const R2<float>* __restrict__ tw, int nlines,
-                                                                  const double* skip) {
+template <class R>
+__global__ __launch_bounds__(zline::threads<R>()) void k_fft_z128_c2r(R* __restrict__ Q, const R2<R>* __restrict__ S,
+                                                                     const R2<R>* __restrict__ tw, int nlines,
+                                                                     const double* skip) {
     if (skipped(skip)) return;
     using namespace zline;
-    typedef Cx<float> C;
+    typedef Cx<R> C;
+    constexpr int T = lines<R>(), kThreads = threads<R>();
+    constexpr int kV = 16 / (int)sizeof(R);
+    typedef R vec __attribute__((ext_vector_type(kV)));
     __shared__ C L[T * kPitch];
     __shared__ C W[N];
     const int l0 = blockIdx.x * T;
@@ -654,40 +666,44 @@ __global__ __launch_bounds__(zline::kThreads) void k_fft_z128_c2r(float* __restr
 #pragma unroll
     for (int h = 0; h < 2; h++) {
         const int task = threadIdx.x + h * kThreads;
-        const int t = task & (T - 1), k1 = task >> 6;
+        const int t = task % T, k1 = task / T;
         C v[B];
 #pragma unroll
         for (int k2 = 0; k2 < B; k2++) {
             const int kz = k1 + A * k2;
             const bool cj = kz > N / 2;
-            const R2<float> s = S[(size_t)(cj ? N - kz : kz) * nlines + l0 + t];
-            v[k2] = C{s.x, cj ? -s.y : s.y};
+            const R2<R> sv = S[(size_t)(cj ? N - kz : kz) * nlines + l0 + t];
+            v[k2] = C{sv.x, cj ? -sv.y : sv.y};
         }
-        plane::dft8(v, 1.f);
+        plane::dft8<R>(v, R(1));
 #pragma unroll
         for (int j = 0; j < B; j++) {
             C r = v[j];
-            if (j * k1 != 0) r = cmul(r, plane::tw_s(W, j * k1, 1.f));
+            if (j * k1 != 0) r = cmul(r, plane::tw_s<R>(W, j * k1, R(1)));
             L[at(t, k1 * B + j)] = r;
         }
     }
     __syncthreads();
     {
-        const int t = threadIdx.x & (T - 1), b = threadIdx.x >> 6;
+        const int t = threadIdx.x % T, b = threadIdx.x / T;
         C v[A];
 #pragma unroll
         for (int k1 = 0; k1 < A; k1++) v[k1] = L[at(t, k1 * B + b)];
-        plane::dft16(v, 1.f);
+        plane::dft16<R>(v, R(1));
 #pragma unroll
         for (int a = 0; a < A; a++) L[at(t, a * B + b)] = v[a];
     }
     __syncthreads();
     {
-        float4* dst = reinterpret_cast<float4*>(Q + (size_t)l0 * N);
+        constexpr int kStores = T * N / kV / kThreads;
+        vec* dst = reinterpret_cast<vec*>(Q + (size_t)l0 * N);
 #pragma unroll
-        for (int u = 0; u < T * N / 4 / kThreads; u++) {
-            const int f = threadIdx.x + u * kThreads, t = f >> 5, z = (f & 31) * 4;
-            dst[f] = make_float4(L[at(t, z)].x, L[at(t, z + 1)].x, L[at(t, z + 2)].x, L[at(t, z + 3)].x);
+        for (int u = 0; u < kStores; u++) {
+            const int f = threadIdx.x + u * kThreads, t = f / (N / kV), z = (f % (N / kV)) * kV;
+            vec o;
+#pragma unroll
+            for (int c = 0; c < kV; c++) o[c] = L[at(t, z + c)].x;
+            dst[f] = o;
         }
     }
 }
@@ -738,20 +754,18 @@ static int log2i(int v) {
 }
 template <class R> static size_t fft_lds(int T, int n) { return (2 * (size_t)T * n + n) * sizeof(R2<R>); }
 
-// the z passes with the register-step kernels where they apply (FP32,
-// n2 = 128, whole batches of 64 lines)
-template <class R> static bool zline_pass(hipStream_t, const FftPlan<R>&, R*, R2<R>*, int, const double*) {
-    return false;
-}
-template <>
-bool zline_pass<float>(hipStream_t s, const FftPlan<float>& p, float* Q, R2<float>* S, int dir, const double* skip) {
+// the z passes with the register-step kernels where they apply (n2 = 128,
+// whole batches of 64 lines in FP32, 32 in FP64)
+template <class R>
+static bool zline_pass(hipStream_t s, const FftPlan<R>& p, R* Q, R2<R>* S, int dir, const double* skip) {
+    constexpr int T = zline::lines<R>();
     const int nlines = p.ax[0].n * p.ax[1].n;
-    if (p.ax[2].n != zline::N || nlines % zline::T != 0) return false;
+    if (p.ax[2].n != zline::N || nlines % T != 0) return false;
     if (dir < 0)
-        hipLaunchKernelGGL(k_fft_z128_r2c, dim3(nlines / zline::T), dim3(zline::kThreads), 0, s, Q, S, p.tw[2], nlines,
+        hipLaunchKernelGGL(k_fft_z128_r2c<R>, dim3(nlines / T), dim3(zline::threads<R>()), 0, s, Q, S, p.tw[2], nlines,
                            skip);
     else
-        hipLaunchKernelGGL(k_fft_z128_c2r, dim3(nlines / zline::T), dim3(zline::kThreads), 0, s, Q, S, p.tw[2], nlines,
+        hipLaunchKernelGGL(k_fft_z128_c2r<R>, dim3(nlines / T), dim3(zline::threads<R>()), 0, s, Q, S, p.tw[2], nlines,
                            skip);
     return true;
 }

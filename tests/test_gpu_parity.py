@@ -259,8 +259,8 @@ def test_water32768(water32768, precision):
     (converged to 1e-10 Debye).  The FP64 path, converged to 1e-10 Debye
     too, agrees to 1e-8; the mixed path at the workload's 1e-6 Debye meets
     energy 1e-6 and force and induced-dipole RMS 1e-5.  This is the only
-    size that runs the 128^3 FFT passes (the fused plane pass and the
-    128-point z transforms)."""
+    size that runs the 128^3 FFT passes (the fused FP32 plane pass and the
+    128-point z transforms in both precisions)."""
     print(f"[gpu] water32768 {precision}", flush=True)
     f, pos, box, ref = water32768
     if precision == "double":
