@@ -180,7 +180,7 @@ def run(args):
     setup_s = time.perf_counter() - t0
 
     def make(precision, eps=None):
-        k = HipCalcMPIDForceKernel(precision)
+        k = HipCalcMPIDForceKernel(precision, single_stream=args.single_stream)
         f = force
         if eps is not None:
             f, _, _ = make_waterbox(args.workload, target_eps=eps)
@@ -376,7 +376,8 @@ def run(args):
         "data": "synthetic (seeded water box generator)",
         "config": {"workload": args.workload, "atoms": n, "box_nm": float(box[0]), "pme_grid": cfg["grid"],
                    "cutoff_nm": cfg["cutoff"], "bspline_order": 6, "polarization": "mutual",
-                   "target_epsilon_debye": 1e-6, "precision": args.precision, "pairs": pc},
+                   "target_epsilon_debye": 1e-6, "precision": args.precision, "pairs": pc,
+                   "streams": 1 if args.single_stream else 3},
         "energy_kJ_mol": energy,
         "solver": st,
         "kernels": kernels,
@@ -570,6 +571,8 @@ def main():
                     help="systems in the batched-workload leg (0 skips it)")
     ap.add_argument("--batch-threads", type=int, default=16)
     ap.add_argument("--batch-reps", type=int, default=3)
+    ap.add_argument("--single-stream", action="store_true",
+                    help="issue every kernel of the evaluation on one stream (serialised kernel times)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
