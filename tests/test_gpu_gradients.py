@@ -18,6 +18,8 @@ class: DOUBLE (converged to 1e-10 Debye) 1e-6; MIXED (1e-6 Debye, FP32 pair
 and reciprocal arithmetic) 1e-4, with the per-type sums of the 12,288-atom
 box at 1e-5 of each gradient (the workload's bar).
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -95,6 +97,35 @@ def fd_default_thole(flat, pos, box, h=2e-2, target_eps=1e-10):
     return _diff(shifted, h, True)
 
 
+# the frames cluster's sites (ThreeFold, ZBisect, ZThenX+y, Bisector, ZOnly,
+# ZThenX (mirror), NoAxisType) and the classes compared at them; their
+# differences are the committed fixture frames_gradients_fd.npz
+# (tests/golden/make_frames_fd_fixture.py: about 400 oracle evaluations)
+FRAME_SITES = [0, 1, 2, 3, 7, 10, 15]
+FRAME_CLASSES = ["positions", "charges", "dipoles", "quadrupoles", "octopoles", "polarity", "tholes"]
+FRAME_INPUTS = ["charges", "dipoles", "quadrupoles", "octopoles", "polarity", "tholes", "axis_types", "atom_z",
+                "atom_x", "atom_y", "cov_offsets", "cov_atoms", "alpha", "cutoff", "grid", "default_thole", "scale14",
+                "polarization", "nonbonded_method"]
+
+
+def frame_box(box):
+    return np.zeros(9) if box is None else np.asarray(box, dtype=np.float64).reshape(-1)
+FRAMES_FD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "frames_gradients_fd.npz")
+
+
+def frames_fd(case, flat, pos, box):
+    """The oracle's differences for the frames cluster from the fixture,
+    after checking that they were taken at these positions, box and
+    parameters."""
+    with np.load(FRAMES_FD) as z:
+        np.testing.assert_array_equal(z[f"{case}_in_positions"], np.asarray(pos, dtype=np.float64))
+        np.testing.assert_array_equal(z[f"{case}_in_box"], frame_box(box))
+        for name in FRAME_INPUTS:
+            np.testing.assert_array_equal(z[f"{case}_in_{name}"], np.asarray(flat[name], dtype=np.float64),
+                                          err_msg=name)
+        return {cls: z[f"{case}_{cls}"] for cls in FRAME_CLASSES}
+
+
 _FD_CACHE = {}
 
 
@@ -169,11 +200,9 @@ def test_every_frame_type(precision, periodic):
     flat = force.flatten()
     if precision == "mixed":
         force.setMutualInducedTargetEpsilon(1e-8)
-    sites = [0, 1, 2, 3, 7, 10, 15]  # ThreeFold, ZBisect, ZThenX+y, Bisector, ZOnly, ZThenX (mirror), NoAxisType
-    fd = cached(("frames", periodic), lambda: fd_gradients(flat, pos, box, sites, [
-        "positions", "charges", "dipoles", "quadrupoles", "octopoles", "polarity", "tholes"]))
+    fd = frames_fd("pme" if periodic else "nocutoff", flat, pos, box)
     got = _kernel(force, box, precision).energyGradients(pos, box)
-    compare(got, fd, sites, 1e-6 if precision == "double" else 1e-4, precision)
+    compare(got, fd, FRAME_SITES, 1e-6 if precision == "double" else 1e-4, precision)
 
 
 @pytest.mark.parametrize("precision", ["double", "mixed"])
